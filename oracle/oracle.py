@@ -1,0 +1,155 @@
+"""TEST INFRASTRUCTURE ONLY -- Python access to the CPU oracle (oracle/libtc_oracle.so).
+
+The oracle restates the TreeCorr 5.1.3 C++ engine (tree build, 2-point
+traversal, 3-point multipole) on the CPU.  tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg are the only users; the product package never
+imports this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+NDATA, KDATA, GDATA = 0, 1, 4
+FLAT, THREED, SPHERE = 1, 2, 3
+MIDDLE, MEDIAN, MEAN, RANDOM = 0, 1, 2, 3
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, 'libtc_oracle.so')
+        if not os.path.exists(path):
+            raise RuntimeError('oracle library missing: build it with make -f oracle/Makefile')
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        L.tco_field_new.restype = vp
+        L.tco_field_new.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 7 + [
+            ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int]
+        L.tco_field_free.argtypes = [vp]
+        L.tco_field_nnodes.argtypes = [vp]
+        L.tco_field_nnodes.restype = ctypes.c_long
+        L.tco_field_ntop.argtypes = [vp]
+        L.tco_field_ntop.restype = ctypes.c_long
+        L.tco_field_sizesq.argtypes = [vp]
+        L.tco_field_sizesq.restype = ctypes.c_double
+        L.tco_field_center.argtypes = [vp, _dp]
+        L.tco_field_dump.argtypes = [vp] + [ctypes.c_void_p] * 9
+        L.tco_corr2.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [
+            ctypes.c_double] * 5 + [ctypes.c_int] * 3 + [_dp] * 8 + [
+            ctypes.POINTER(ctypes.c_longlong)]
+        L.tco_corr2_brute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long] + [_dp] * 6 + [
+            ctypes.c_long] + [_dp] * 6 + [ctypes.c_int] + [ctypes.c_double] * 3 + [
+            ctypes.c_int, ctypes.c_int] + [_dp] * 8
+        if hasattr(L, 'tco_corr3_multipole'):
+            L.tco_corr3_multipole.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int] + [ctypes.c_double] * 5 + [
+                ctypes.c_int] * 3 + [_dp] * 17 + [ctypes.POINTER(ctypes.c_longlong)]
+            L.tco_corr3_brute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long] + [
+                _dp] * 6 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3 + [
+                ctypes.c_int] + [_dp] * 17
+        _LIB = L
+    return _LIB
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=float)
+
+
+class Field:
+    """A tree built by the oracle (the reference's Field + BuildCells)."""
+
+    def __init__(self, d, coords, x, y, z=None, k=None, g1=None, g2=None, w=None, wpos=None,
+                 min_size=0., max_size=np.inf, split_method=MEAN, brute=False, min_top=3,
+                 max_top=10):
+        x, y, z = _c(x), _c(y), _c(z)
+        n = len(x)
+        w = np.ones(n) if w is None else _c(w)
+        wpos = _c(wpos)
+        d1 = _c(k) if d == KDATA else _c(g1)
+        d2 = _c(g2)
+        self._keep = (x, y, z, d1, d2, w, wpos)
+        self.d, self.coords, self.n = d, coords, n
+        self.h = lib().tco_field_new(d, coords, _ptr(x), _ptr(y), _ptr(z), _ptr(d1), _ptr(d2),
+                                     _ptr(w), _ptr(wpos), n, float(min_size), float(max_size),
+                                     int(split_method), int(bool(brute)), int(min_top),
+                                     int(max_top))
+
+    def __del__(self):
+        if getattr(self, 'h', None):
+            lib().tco_field_free(self.h)
+            self.h = None
+
+    @property
+    def ntop(self):
+        return lib().tco_field_ntop(self.h)
+
+    @property
+    def sizesq(self):
+        return lib().tco_field_sizesq(self.h)
+
+    def dump(self):
+        L = lib()
+        nn = L.tco_field_nnodes(self.h)
+        nt = L.tco_field_ntop(self.h)
+        out = dict(pos=np.zeros((nn, 3)), size=np.zeros(nn, np.float32),
+                   w=np.zeros(nn, np.float32), wk=np.zeros(nn, np.float32),
+                   g=np.zeros((nn, 2), np.float32), n=np.zeros(nn, np.int64),
+                   left=np.zeros(nn, np.int32), right=np.zeros(nn, np.int32),
+                   top=np.zeros(nt, np.int32))
+        L.tco_field_dump(self.h, *[out[k].ctypes.data for k in
+                                   ('pos', 'size', 'w', 'wk', 'g', 'n', 'left', 'right', 'top')])
+        return out
+
+
+def corr2(d, coords, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, exact=True,
+          nthreads=0):
+    """Run the oracle's 2-point traversal; returns dict of arrays and counters."""
+    out = {k: np.zeros(nbins) for k in
+           ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')}
+    cnt = (ctypes.c_longlong * 4)()
+    rc = lib().tco_corr2(f1.h, f2.h if f2 is not None else None, d, coords, nbins,
+                         float(min_sep), float(max_sep), float(bin_size), float(b), float(a),
+                         int(quick), int(exact), int(nthreads),
+                         *[_ptr(out[k]) for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr',
+                                                  'meanlogr', 'weight', 'npairs')], cnt)
+    if rc != 0:
+        raise RuntimeError('oracle corr2 failed')
+    out['counters'] = dict(visits=cnt[0], accepted=cnt[1], leafpairs=cnt[2], refined=cnt[3])
+    return out
+
+
+def brute2(d, coords, cat1, cat2, nbins, min_sep, max_sep, bin_size, nthreads=0):
+    """O(N^2) brute-force pair binning.  cat = dict(x,y,z,k|g1,g2,w)."""
+    def cols(c):
+        n = len(c['x'])
+        w = np.ones(n) if c.get('w') is None else _c(c['w'])
+        a = _c(c.get('k')) if d == KDATA else _c(c.get('g1'))
+        return (n, _c(c['x']), _c(c['y']), _c(c.get('z')), a, _c(c.get('g2')), w)
+    A = cols(cat1)
+    if cat2 is None:
+        B = (0, None, None, None, None, None, None)
+    else:
+        B = cols(cat2)
+    out = {k: np.zeros(nbins) for k in
+           ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')}
+    rc = lib().tco_corr2_brute(d, coords, A[0], *[_ptr(v) for v in A[1:]], B[0],
+                               *[_ptr(v) for v in B[1:]], nbins, float(min_sep), float(max_sep),
+                               float(bin_size), 1, int(nthreads),
+                               *[_ptr(out[k]) for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr',
+                                                        'meanlogr', 'weight', 'npairs')])
+    if rc != 0:
+        raise RuntimeError('oracle brute failed')
+    return out
