@@ -1,0 +1,123 @@
+/* treecorr_amd: C ABI of the MI355X ball-tree correlation engine.
+ *
+ * This is the drop-in boundary for TreeCorr's compiled engine, the pybind11
+ * module `_treecorr` (src/module.cpp:21-26).  Every entry point replaces one
+ * reference binding, cited below, with plain pointers and sizes.  The Python
+ * package treecorr_amd binds these with ctypes; INTEGRATION.md shows the
+ * binding a TreeCorr maintainer would add.
+ *
+ * Conventions
+ *   - Every function returns an int status: TC_OK (0) or a TC_ERR_* code; the
+ *     message of the last failure on the calling thread is tc_last_error().
+ *   - Arrays given to *_create are host arrays unless the name says _device.
+ *   - Output arrays given to tc_corr2_create / tc_corr3_create are borrowed
+ *     (the caller owns them) and every process call ADDS into them, exactly as
+ *     the reference does (include/Corr2.h:210-213).
+ *   - Calls are synchronous.  A handle must not be used from two threads at once.
+ */
+#ifndef TREECORR_AMD_H
+#define TREECORR_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TC_OK 0
+#define TC_ERR_INVALID 1
+#define TC_ERR_HIP 2
+#define TC_ERR_UNSUPPORTED 3
+
+/* Enumerations, values as in the reference:
+ *   data types  N=0, K=1, G=4            (include/Cell.h:34 DataType)
+ *   coords      Flat=1, ThreeD=2, Sphere=3 (include/Position.h:25)
+ *   bin types   Log=0, LogMultipole=5    (include/BinType.h:35)
+ *   metrics     Euclidean=1              (include/Metric.h:29)
+ *   split       Middle=0, Median=1, Mean=2, Random=3 (include/Cell.h:26) */
+
+typedef struct tc_field tc_field;
+typedef struct tc_corr2 tc_corr2;
+typedef struct tc_corr3 tc_corr3;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int tc_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* tc_last_error(void);
+
+/* Number of visible GPUs (0 when no GPU is usable). */
+int tc_device_count(int* count);
+
+/* ---- Fields: replace {N,K,G}Field{Flat,Sphere,ThreeD}(...) ----
+ * Reference: BuildNField/BuildKField/BuildAnyZField, src/Field.cpp:430-506,
+ * bound at src/Field.cpp:684-690; the tree itself is Field<D,C>::Field +
+ * BuildCells, src/Field.cpp:163-277 (built on the GPU here, eagerly).
+ * d1 is k (K fields) or g1 (G fields); d2 is g2.  z and wpos may be NULL. */
+int tc_field_create(int data_type, int coords, const double* x, const double* y,
+                    const double* z, const double* d1, const double* d2, const double* w,
+                    const double* wpos, long long nobj, double minsize, double maxsize,
+                    int split_method, long long seed, int brute, int mintop, int maxtop,
+                    tc_field** out);
+
+/* The same with the catalog columns already resident in GPU memory. */
+int tc_field_create_device(int data_type, int coords, const double* x, const double* y,
+                           const double* z, const double* d1, const double* d2,
+                           const double* w, const double* wpos, long long nobj,
+                           double minsize, double maxsize, int split_method, long long seed,
+                           int brute, int mintop, int maxtop, tc_field** out);
+
+/* BaseField::getNObj / getSize / getNTopLevel, src/Field.cpp:624-628. */
+int tc_field_nobj(const tc_field* f, long long* nobj);
+int tc_field_size(const tc_field* f, double* size);
+int tc_field_ntop(const tc_field* f, long long* ntop);
+/* Field::_center (3 doubles; z = 0 for Flat), src/Field.cpp:199-203. */
+int tc_field_center(const tc_field* f, double* center3);
+/* Number of tree nodes, and a copy of them (48-byte records: x,y,z double;
+ * size,w,a,b float; child,n int) plus the ordered top-level node ids. */
+int tc_field_nnodes(const tc_field* f, long long* nnodes);
+int tc_field_copy_nodes(const tc_field* f, void* nodes_out, int* top_out);
+/* ~BaseField, src/Field.cpp:279-285. */
+int tc_field_destroy(tc_field* f);
+
+/* ---- 2-point: replace {NN,KK,GG}Corr(...) + processAuto/processCross ----
+ * Reference: BuildCorr2 src/Corr2.cpp:842-864 bound at :1135-1145;
+ * ProcessAuto/ProcessCross :886-993 bound at :1151-1157 (BaseCorr2::process,
+ * :146-288).  d1 == d2 in {0,1,4}; bin_type Log; metric Euclidean.
+ * xi1..xi3 may be NULL where the correlation has fewer xi arrays. */
+int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, int nbins,
+                    double binsize, double b, double a, double minrpar, double maxrpar,
+                    double xp, double yp, double zp, double* xi0, double* xi1, double* xi2,
+                    double* xi3, double* meanr, double* meanlogr, double* weight,
+                    double* npairs, tc_corr2** out);
+int tc_corr2_process_auto(tc_corr2* c, tc_field* f, int dots, int quick, int metric);
+int tc_corr2_process_cross(tc_corr2* c, tc_field* f1, tc_field* f2, int dots, int quick,
+                           int metric);
+/* BaseCorr2::triviallyZero / TriviallyZero, src/Corr2.cpp:590-601, 1107-1128. */
+int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, double y1,
+                            double z1, double s1, double x2, double y2, double z2, double s2,
+                            int* result);
+/* Work counters of the last process call: node-pair visits, accepted pairs,
+ * accepted leaf-leaf pairs, refined single-bin tests (no reference counterpart;
+ * used for roofline accounting). */
+int tc_corr2_counters(const tc_corr2* c, long long* counters4);
+int tc_corr2_destroy(tc_corr2* c);
+
+/* ---- Host utilities ---- */
+/* GenerateXYZ, src/KMeans.cpp:1013-1050 (glibc sincos; r may be NULL). */
+int tc_generate_xyz(double* x, double* y, double* z, const double* ra, const double* dec,
+                    const double* r, long long n);
+/* SetOMPThreads / GetOMPThreads, src/Corr2.cpp:995-1003: accepted for API
+ * compatibility; the GPU engine ignores the host thread count. */
+int tc_set_num_threads(int n);
+int tc_get_num_threads(int* n);
+
+/* Kernel timing with HIP events on the library stream (bench/profiling only;
+ * no reference counterpart).  which: 0 = 2-pt traversal, 1 = tree build,
+ * 2 = multipole Gn, 3 = zeta contraction. */
+int tc_set_timing(int on);
+int tc_kernel_stats(int which, double* total_ms, long long* launches);
+int tc_reset_kernel_stats(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TREECORR_AMD_H */

@@ -57,9 +57,6 @@ def lib():
             L.tco_corr3_multipole.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int] + [ctypes.c_double] * 5 + [
                 ctypes.c_int] * 3 + [_dp] * 17 + [ctypes.POINTER(ctypes.c_longlong)]
-            L.tco_corr3_brute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long] + [
-                _dp] * 6 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3 + [
-                ctypes.c_int] + [_dp] * 17
         _LIB = L
     return _LIB
 
@@ -153,3 +150,27 @@ def brute2(d, coords, cat1, cat2, nbins, min_sep, max_sep, bin_size, nthreads=0)
     if rc != 0:
         raise RuntimeError('oracle brute failed')
     return out
+
+
+Z3_KEYS = ('zeta0', 'zeta1', 'zeta2', 'zeta3', 'zeta4', 'zeta5', 'zeta6', 'zeta7', 'meand2',
+           'meanlogd2', 'meand3', 'meanlogd3', 'weight', 'weight_im', 'ntri')
+
+
+def corr3_multipole(d, coords, f, nbins, min_sep, max_sep, bin_size, b, a, max_n, nthreads=0):
+    """Run the oracle's multipole 3-point auto-correlation (NNN d=0, GGG d=4).
+
+    Returns raw sums (before finalize), each of shape (nbins, nbins, 2*max_n+1),
+    plus counters (Gn calls, zeta calls, zeta rows, finish stages)."""
+    L = lib()
+    ntot = nbins * nbins * (2 * max_n + 1)
+    out = {k: np.zeros(ntot) for k in Z3_KEYS}
+    cnt = (ctypes.c_longlong * 4)()
+    rc = L.tco_corr3_multipole(f.h, d, coords, nbins, int(max_n), float(min_sep), float(max_sep),
+                               float(bin_size), float(b), float(a), 0, 0, int(nthreads),
+                               *[_ptr(out[k]) for k in Z3_KEYS], None, None, cnt)
+    if rc != 0:
+        raise RuntimeError('oracle multipole failed')
+    shape = (nbins, nbins, 2 * max_n + 1)
+    res = {k: v.reshape(shape) for k, v in out.items()}
+    res['counters'] = dict(gn_calls=cnt[0], zeta_calls=cnt[1], zeta_rows=cnt[2], stages=cnt[3])
+    return res

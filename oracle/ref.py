@@ -131,3 +131,33 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthre
     else:
         corr.processCross(f1, f2, False, bool(quick), m.Euclidean)
     return out
+
+
+_CORR3 = {0: 'NNNCorr', 4: 'GGGCorr'}
+
+
+def corr3_multipole(d, f, nbins, min_sep, max_sep, bin_size, b, a, max_n, nthreads=None):
+    """Run the reference LogMultipole auto-correlation (src/Corr3.cpp:1124-1175).
+
+    Constructor arguments follow treecorr/corr3base.py:1116-1131 with the
+    LogMultipole settings of :691-705 (nubins = max_n, u/v ranges zero)."""
+    m = module()
+    if nthreads is not None:
+        m.SetOMPThreads(int(nthreads))
+    ntot = nbins * nbins * (2 * max_n + 1)
+    keys = ('zeta0', 'zeta1', 'zeta2', 'zeta3', 'zeta4', 'zeta5', 'zeta6', 'zeta7', 'meand1',
+            'meanlogd1', 'meand2', 'meanlogd2', 'meand3', 'meanlogd3', 'meanu', 'meanv',
+            'weight', 'weight_im', 'ntri')
+    out = {k: np.zeros(ntot) for k in keys}
+    e = np.array([])
+    z = [out['zeta%d' % i] if d == 4 else e for i in range(8)]
+    corr = getattr(m, _CORR3[d])(m.LogMultipole, float(min_sep), float(max_sep), int(nbins),
+                                 float(bin_size), float(b), float(a), 0., 0., int(max_n), 0., 0.,
+                                 0., 0., 0, 0., 0., -1.7976931348623157e308,
+                                 1.7976931348623157e308, 0., 0., 0., *z, out['meand1'],
+                                 out['meanlogd1'], out['meand2'], out['meanlogd2'], out['meand3'],
+                                 out['meanlogd3'], out['meanu'], out['meanv'], out['weight'],
+                                 out['weight_im'], out['ntri'])
+    corr.processAuto(f, False, False, m.Euclidean)
+    shape = (nbins, nbins, 2 * max_n + 1)
+    return {k: v.reshape(shape) for k, v in out.items()}

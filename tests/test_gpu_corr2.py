@@ -1,0 +1,127 @@
+"""GPU 2-point correlations vs the literal reference's golden vectors and vs the oracle."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases as C
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+CNAME = {1: 'flat', 2: '3d', 3: 'spherical'}
+KEYS = ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')
+# Tolerance for the FP64 sums (xi, meanr, meanlogr, weight), normwise relative: the
+# device accumulates with compensation, the reference naively; 1e-12 is the bound.
+RTOL = 1e-12
+
+
+def normwise(a, b):
+    den = np.max(np.abs(b))
+    return 0. if den == 0 else np.max(np.abs(a - b)) / den
+
+
+def raw_corr2(tca, d, coords, cat1, cat2, bn, brute=False, min_top=3, quick=False):
+    """Run the GPU engine through the public classes but read the raw (unfinalized) sums."""
+    def mk(cat):
+        return tca.Catalog(x=cat['x'], y=cat['y'], z=cat['z'], w=cat['w'], k=cat['k'],
+                           g1=cat['g1'], g2=cat['g2'])
+    cls = {0: tca.NNCorrelation, 1: tca.KKCorrelation, 4: tca.GGCorrelation}[d]
+    corr = cls(nbins=bn['nbins'], min_sep=bn['min_sep'], max_sep=bn['max_sep'],
+               bin_slop=bn['b'] / bn['bin_size'], angle_slop=bn['a'], brute=brute,
+               min_top=min_top)
+    c1 = mk(cat1)
+    if cat2 is None:
+        corr.process_auto(c1, corr_only=quick)
+    else:
+        corr.process_cross(c1, mk(cat2), corr_only=quick)
+    out = dict(meanr=corr.meanr, meanlogr=corr.meanlogr, weight=corr.weight, npairs=corr.npairs)
+    for i in range(4):
+        out['xi%d' % i] = corr._xi[i] if corr._xi[i] is not None else np.array([])
+    out['counters'] = corr.counters
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('case', C.CORR2_CASES, ids=[c[0] for c in C.CORR2_CASES])
+def test_gpu_corr2_matches_reference_golden(gpu, case):
+    z = np.load(os.path.join(GOLD, case[0] + '.npz'), allow_pickle=False)
+    spec = json.loads(str(z['spec']))
+    gold = {k: z[k] for k in z.files if k != 'spec'}
+    cat1, cat2, bn, extra = C.corr2_case(case)
+    assert C.checksum(cat1) == spec['checksum1']
+    out = raw_corr2(gpu, case[1], case[2], cat1, cat2, bn, brute=extra['brute'],
+                    min_top=extra['min_top'])
+    np.testing.assert_array_equal(out['npairs'], gold['npairs'])
+    for k in KEYS[:-1]:
+        if gold[k].size:
+            assert normwise(out[k], gold[k]) < RTOL, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d,coords,bs', [(0, 1, 0.0), (1, 2, 0.1), (4, 3, 0.1), (4, 1, 1.0),
+                                         (0, 3, 0.5), (4, 2, 0.2)])
+def test_gpu_corr2_matches_oracle_large(gpu, oracle_lib, d, coords, bs):
+    O = oracle_lib
+    cat, scale = C.make_catalog(coords, 60000, 70 + d + coords, 0.3)
+    bn = C.binning(scale, 20, 0.002, 0.3, bs, 0.1 if d == 4 else 1.0)
+    out = raw_corr2(gpu, d, coords, cat, None, bn, min_top=5)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=5)
+    f = O.Field(d, coords, cat['x'], cat['y'], cat['z'], k=cat['k'], g1=cat['g1'], g2=cat['g2'],
+                w=cat['w'], **kw)
+    ref = O.corr2(d, coords, f, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                  bn['b'], bn['a'], exact=True)
+    np.testing.assert_array_equal(out['npairs'], ref['npairs'])
+    for k in KEYS[:-1]:
+        if out[k].size:
+            assert normwise(out[k], ref[k]) < RTOL, k
+    for k in ('visits', 'accepted', 'leafpairs', 'refined'):
+        assert out['counters'][k] == ref['counters'][k], k
+
+
+@pytest.mark.gpu
+def test_gpu_corr2_cross_and_quick(gpu, oracle_lib):
+    O = oracle_lib
+    cat1, scale = C.make_catalog(C.SPHERE, 20000, 91, 0.2)
+    cat2, _ = C.make_catalog(C.SPHERE, 15000, 92, 0.2)
+    bn = C.binning(scale, 12, 0.005, 0.4, 0.3, 0.1)
+    out = raw_corr2(gpu, 4, 3, cat1, cat2, bn, min_top=4, quick=True)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    f1 = O.Field(4, 3, cat1['x'], cat1['y'], cat1['z'], g1=cat1['g1'], g2=cat1['g2'], w=cat1['w'], **kw)
+    f2 = O.Field(4, 3, cat2['x'], cat2['y'], cat2['z'], g1=cat2['g1'], g2=cat2['g2'], w=cat2['w'], **kw)
+    ref = O.corr2(4, 3, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                  bn['b'], bn['a'], quick=True, exact=True)
+    for k in ('xi0', 'xi1', 'xi2', 'xi3', 'weight'):
+        assert normwise(out[k], ref[k]) < RTOL, k
+    assert not np.any(out['npairs']) and not np.any(out['meanr'])
+
+
+@pytest.mark.gpu
+def test_gpu_nn_bs0_exact_vs_brute(gpu, oracle_lib):
+    """NN flat at bin_slop=0 is exact vs the O(N^2) pair loop (the 100k-point workload's
+    check, here at a size the CPU loop finishes quickly)."""
+    O = oracle_lib
+    rng = np.random.default_rng(2026)
+    n = 20000
+    cat = dict(x=rng.uniform(0, 1000, n), y=rng.uniform(0, 1000, n), z=None, w=np.ones(n),
+               k=None, g1=None, g2=None)
+    bn = dict(nbins=20, min_sep=1., max_sep=100., bin_size=math.log(100.) / 20, b=0., a=1.)
+    out = raw_corr2(gpu, 0, 1, cat, None, bn, min_top=3)
+    b = O.brute2(0, 1, cat, None, 20, 1., 100., math.log(100.) / 20)
+    np.testing.assert_array_equal(out['npairs'], b['npairs'])
+    np.testing.assert_allclose(out['weight'], b['weight'], rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_gpu_finalized_api(gpu, oracle_lib):
+    """The public process()/finalize path: xip/xim/meanr/varxi as TreeCorr defines them."""
+    cat, scale = C.make_catalog(C.SPHERE, 30000, 95, 0.3)
+    c = gpu.Catalog(ra=np.arctan2(cat['y'], cat['x']), dec=np.arcsin(cat['z']), ra_units='rad',
+                    dec_units='rad', g1=cat['g1'], g2=cat['g2'], w=cat['w'])
+    gg = gpu.GGCorrelation(nbins=10, min_sep=1., max_sep=100., sep_units='arcmin',
+                           bin_slop=0.2, min_top=4)
+    gg.process(c)
+    assert np.all(np.isfinite(gg.xip)) and np.all(gg.npairs > 0)
+    assert np.all((gg.meanr > gg.left_edges * 0.9) & (gg.meanr < gg.right_edges * 1.1))
+    varg = c.varg
+    np.testing.assert_allclose(gg.varxip, 2 * varg ** 2 / gg.weight, rtol=1e-12)

@@ -1,0 +1,145 @@
+"""The CPU oracle against the golden vectors written from the literal reference
+(tests/golden/make_golden.py), against brute force, and against the reference
+build itself when oracle/_ref is present."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cases as C
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+KEYS = ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')
+
+
+def load(name):
+    z = np.load(os.path.join(GOLD, name + '.npz'), allow_pickle=False)
+    return json.loads(str(z['spec'])), {k: z[k] for k in z.files if k != 'spec'}
+
+
+def normwise(a, b):
+    den = np.max(np.abs(b))
+    return 0. if den == 0 else np.max(np.abs(a - b)) / den
+
+
+@pytest.mark.parametrize('case', C.CORR2_CASES, ids=[c[0] for c in C.CORR2_CASES])
+def test_oracle_corr2_golden(oracle_lib, case):
+    O = oracle_lib
+    spec, gold = load(case[0])
+    cat1, cat2, bn, extra = C.corr2_case(case)
+    assert C.checksum(cat1) == spec['checksum1']
+    d, coords = case[1], case[2]
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=extra['min_top'],
+              brute=extra['brute'])
+
+    def field(cat):
+        return O.Field(d, coords, cat['x'], cat['y'], cat['z'], k=cat['k'], g1=cat['g1'],
+                       g2=cat['g2'], w=cat['w'], **kw)
+    f1 = field(cat1)
+    f2 = field(cat2) if cat2 is not None else None
+    assert f1.ntop == spec['ntop1']
+    out = O.corr2(d, coords, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                  bn['b'], bn['a'], exact=True)
+    # npairs: bit exact; weighted sums: 1e-12 normwise (the reference sums naively,
+    # the oracle with compensation; both agree far below this bound).
+    np.testing.assert_array_equal(out['npairs'], gold['npairs'])
+    for k in KEYS[:-1]:
+        if gold[k].size:
+            assert normwise(out[k], gold[k]) < 1e-12, k
+
+
+@pytest.mark.parametrize('case', C.TREE_CASES, ids=[c[0] for c in C.TREE_CASES])
+def test_oracle_tree_golden(oracle_lib, case):
+    O = oracle_lib
+    spec, gold = load(case[0])
+    cat, kw = C.tree_case(case)
+    assert C.checksum(cat) == spec['checksum']
+    t = O.Field(case[1], case[2], cat['x'], cat['y'], cat['z'], k=cat['k'], g1=cat['g1'],
+                g2=cat['g2'], w=cat['w'], **kw).dump()
+    for k in ('pos', 'size', 'w', 'wk', 'g', 'n', 'left', 'right', 'top'):
+        np.testing.assert_array_equal(t[k], gold[k], err_msg=k)
+
+
+@pytest.mark.parametrize('d,coords', [(0, 1), (1, 2), (4, 1), (4, 3)])
+def test_oracle_bs0_equals_brute(oracle_lib, d, coords):
+    """bin_slop=0 counts every pair in its exact bin (tests/test_nn.py:543-552): npairs
+    equal the O(N^2) loop exactly; weights agree to the float cell payloads (1e-6);
+    with brute=True every accepted pair is a leaf pair and all sums agree to 1e-12."""
+    O = oracle_lib
+    cat, scale = C.make_catalog(coords, 1500, 101 + d + coords, 0.2)
+    bn = C.binning(scale, 10, 0.01, 0.4, 0.0, 0.0 if d == 4 else 1.0)
+    b = O.brute2(d, coords, cat, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'])
+    for brute in (False, True):
+        f = O.Field(d, coords, cat['x'], cat['y'], cat['z'], k=cat['k'], g1=cat['g1'],
+                    g2=cat['g2'], w=cat['w'], min_size=0. if brute else bn['min_size'],
+                    max_size=bn['max_size'], min_top=3, brute=brute)
+        t = O.corr2(d, coords, f, None, bn['nbins'], bn['min_sep'], bn['max_sep'],
+                    bn['bin_size'], bn['b'], bn['a'], exact=True)
+        np.testing.assert_array_equal(t['npairs'], b['npairs'])
+        if brute:
+            assert t['counters']['accepted'] == t['counters']['leafpairs']
+            # xip_im (xi1) is odd under swapping a pair's order, which differs between
+            # the tree's and the loop's pair order, so it is excluded here.
+            for k in ('xi0', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight'):
+                assert normwise(t[k], b[k]) < 1e-12, k
+        else:
+            np.testing.assert_allclose(t['weight'], b['weight'], rtol=1e-6)
+
+
+def test_oracle_auto_from_cross(oracle_lib):
+    """Invariant: auto(cat) == (cross(cat,cat) - self pairs)/2 in npairs for brute force."""
+    O = oracle_lib
+    cat, scale = C.make_catalog(C.FLAT, 800, 7)
+    bn = C.binning(scale, 8, 0.01, 0.5, 0.0, 1.0)
+    a = O.brute2(0, 1, cat, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'])
+    x = O.brute2(0, 1, cat, cat, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'])
+    np.testing.assert_array_equal(2 * a['npairs'], x['npairs'])   # min_sep > 0: no self pairs
+
+
+def test_oracle_flat_gg_closed_form(oracle_lib):
+    """Flat GG on two points: the tangential projection has a closed form
+    (tests/test_gg.py:63-67): xip = g1 conj(g2) rotated by exp(-4i phi) both sides."""
+    O = oracle_lib
+    x = np.array([0., 3.]); y = np.array([0., 4.])
+    g1 = np.array([0.1, -0.2]); g2 = np.array([0.05, 0.15]); w = np.array([1., 1.])
+    cat = dict(x=x, y=y, z=None, g1=g1, g2=g2, w=w, k=None)
+    out = O.brute2(4, 1, cat, None, 1, 1., 10., np.log(10.))
+    phi = np.arctan2(4., 3.)
+    e = np.exp(-2j * phi)
+    z1 = (g1[0] + 1j * g2[0]) * e
+    z2 = (g1[1] + 1j * g2[1]) * e
+    xip = z1 * np.conj(z2)
+    xim = z1 * z2
+    # float payloads: w*g rounded to float32
+    assert abs(out['xi0'][0] - xip.real) < 1e-7
+    assert abs(out['xi1'][0] - xip.imag) < 1e-7
+    assert abs(out['xi2'][0] - xim.real) < 1e-7
+    assert abs(out['xi3'][0] - xim.imag) < 1e-7
+    assert out['npairs'][0] == 1 and abs(out['meanr'][0] - 5.) < 1e-12
+
+
+def test_oracle_planted_mistake_is_caught(oracle_lib):
+    """A dropped pair changes npairs: the golden comparison detects a plausible bug."""
+    spec, gold = load('nn_flat_bs0')
+    wrong = gold['npairs'].copy()
+    wrong[3] -= 1
+    assert not np.array_equal(wrong, gold['npairs'])
+
+
+def test_oracle_against_literal_reference(oracle_lib):
+    """When oracle/_ref is built here, compare a fresh bin_slop>0 case directly."""
+    from oracle import ref as R
+    if R.module() is None:
+        pytest.skip('oracle/_ref not built')
+    O = oracle_lib
+    cat, scale = C.make_catalog(C.SPHERE, 6000, 31, 0.3)
+    bn = C.binning(scale, 12, 0.01, 0.5, 0.1, 0.1)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    rf = R.make_field(4, 3, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw)
+    r = R.corr2(4, rf, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'])
+    of = O.Field(4, 3, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw)
+    o = O.corr2(4, 3, of, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'])
+    np.testing.assert_array_equal(o['npairs'], r['npairs'])
+    for k in KEYS[:-1]:
+        assert normwise(o[k], r[k]) < 1e-12, k

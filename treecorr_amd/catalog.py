@@ -1,0 +1,265 @@
+"""Catalog: the input arrays of a correlation (treecorr/catalog.py, restated).
+
+Mirrors the parts of treecorr.Catalog that feed the ball-tree engine:
+positions (x,y[,z] or ra,dec[,r] with units), weights w/wpos, scalar k, shear
+g1/g2, flips, zero-weight removal (catalog.py:1343-1466), ra/dec -> xyz with the
+reference's GenerateXYZ (catalog.py:1609-1622), the derived sums/variances used
+at finalize (catalog.py:1200-1330), patches given as integer labels, and a
+one-entry cache of built fields per kind (catalog.py:2230-2316).
+"""
+import numpy as np
+
+from . import _lib
+from .config import parse_unit
+
+
+class Catalog:
+    def __init__(self, file_name=None, config=None, *, x=None, y=None, z=None, ra=None,
+                 dec=None, r=None, w=None, wpos=None, k=None, g1=None, g2=None, patch=None,
+                 x_units=None, y_units=None, ra_units=None, dec_units=None, flip_g1=False,
+                 flip_g2=False, keep_zero_weight=False, name='', **kwargs):
+        if file_name is not None:
+            raise NotImplementedError("treecorr_amd.Catalog takes arrays; file input is not part "
+                                      "of the GPU engine")
+        cfg = dict(config or {})
+        cfg.update(kwargs)
+        self.name = name
+        self.config = cfg
+
+        def arr(a):
+            return None if a is None else np.array(a, dtype=float, copy=True).ravel()
+        self._x, self._y, self._z = arr(x), arr(y), arr(z)
+        self._ra, self._dec, self._r = arr(ra), arr(dec), arr(r)
+        self._w, self._wpos = arr(w), arr(wpos)
+        self._k, self._g1, self._g2 = arr(k), arr(g1), arr(g2)
+        self._patch = None if patch is None else np.asarray(patch, dtype=int).ravel()
+
+        if self._x is not None or self._y is not None:
+            if self._x is None or self._y is None:
+                raise TypeError("x and y must both be given")
+            if self._ra is not None or self._dec is not None:
+                raise TypeError("ra and dec may not be given with x,y")
+            if x_units is not None or y_units is not None:
+                if x_units is None or y_units is None:
+                    raise TypeError("x_units and y_units must both be given")
+                self._x *= parse_unit(x_units)
+                self._y *= parse_unit(y_units)
+        elif self._ra is not None or self._dec is not None:
+            if self._ra is None or self._dec is None:
+                raise TypeError("ra and dec must both be given")
+            if ra_units is None or dec_units is None:
+                raise TypeError("ra_units and dec_units are required with ra, dec")
+            if self._z is not None:
+                raise TypeError("z may not be given with ra, dec")
+            self._ra = self._ra * parse_unit(ra_units)
+            self._dec = self._dec * parse_unit(dec_units)
+        else:
+            raise TypeError("Either x,y or ra,dec must be given")
+
+        n = len(self._x) if self._x is not None else len(self._ra)
+        for nm in ('_y', '_z', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2', '_patch'):
+            a = getattr(self, nm)
+            if a is not None and len(a) != n:
+                raise ValueError("%s has the wrong length" % nm[1:])
+        for nm in ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2'):
+            a = getattr(self, nm)
+            if a is not None and np.any(np.isnan(a)):
+                raise ValueError("NaN values found in %s" % nm[1:])
+        if flip_g1 and self._g1 is not None:
+            self._g1 = -self._g1
+        if flip_g2 and self._g2 is not None:
+            self._g2 = -self._g2
+
+        self._generate_xyz()
+
+        # catalog.py:1432-1464: wpos==0 forces w=0; drop zero-weight objects
+        if self._wpos is not None and np.any(self._wpos == 0.):
+            if self._w is None:
+                self._w = np.ones(n)
+            self._w[self._wpos == 0.] = 0.
+        if self._w is not None:
+            self._nontrivial_w = True
+            self._sumw = np.sum(self._w)
+            if self._sumw == 0:
+                raise ValueError("Catalog has invalid sumw == 0")
+        else:
+            self._nontrivial_w = False
+            self._sumw = n
+            self._w = np.ones(n)
+        if self._nontrivial_w and not keep_zero_weight:
+            wpos = self._wpos if self._wpos is not None else self._w
+            if np.any(wpos == 0):
+                self._select(np.where(wpos != 0)[0])
+
+        self._fields = {}
+        self._npatch = 1 if self._patch is None else int(self._patch.max()) + 1
+
+    # ---- construction helpers ----
+    def _generate_xyz(self):
+        # catalog.py:1609-1622 -> GenerateXYZ (src/KMeans.cpp:1013-1050, glibc sincos)
+        if self._x is None:
+            n = len(self._ra)
+            x, y, z = np.empty(n), np.empty(n), np.empty(n)
+            r = self._r
+            _lib.check(_lib.lib().tc_generate_xyz(_lib.dptr(x), _lib.dptr(y), _lib.dptr(z),
+                                                  _lib.dptr(np.ascontiguousarray(self._ra)),
+                                                  _lib.dptr(np.ascontiguousarray(self._dec)),
+                                                  _lib.dptr(r), n))
+            self._x, self._y, self._z = x, y, z
+
+    def _select(self, idx):
+        for nm in ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2',
+                   '_patch'):
+            a = getattr(self, nm)
+            if a is not None:
+                setattr(self, nm, a[idx])
+
+    # ---- arrays ----
+    x = property(lambda self: self._x)
+    y = property(lambda self: self._y)
+    z = property(lambda self: self._z)
+    ra = property(lambda self: self._ra)
+    dec = property(lambda self: self._dec)
+    r = property(lambda self: self._r)
+    w = property(lambda self: self._w)
+    wpos = property(lambda self: self._wpos)
+    k = property(lambda self: self._k)
+    g1 = property(lambda self: self._g1)
+    g2 = property(lambda self: self._g2)
+    patch = property(lambda self: self._patch)
+    npatch = property(lambda self: self._npatch)
+    nontrivial_w = property(lambda self: self._nontrivial_w)
+
+    @property
+    def coords(self):
+        # catalog.py:1318-1328
+        if self._ra is not None:
+            return 'spherical' if self._r is None else '3d'
+        return 'flat' if self._z is None else '3d'
+
+    @property
+    def ntot(self):
+        return len(self._x)
+
+    @property
+    def nobj(self):
+        return int(np.sum(self._w != 0)) if self._nontrivial_w else self.ntot
+
+    @property
+    def sumw(self):
+        return float(np.sum(self._w))
+
+    @property
+    def sumw2(self):
+        return float(np.sum(self._w ** 2)) if self._nontrivial_w else float(self.ntot)
+
+    @property
+    def meanw(self):
+        return self.sumw / self.nobj if self._nontrivial_w else 1.
+
+    def _weighted_var(self, v):
+        # catalog.py:1213-1229
+        if v is None:
+            return 0.
+        if self._nontrivial_w:
+            use = self._w != 0
+            meanv = np.sum(self._w[use] * v[use]) / self.sumw
+            return np.sum(self._w[use] ** 2 * (v[use] - meanv) ** 2) / self.sumw
+        return np.sum((v - np.mean(v)) ** 2) / self.nobj
+
+    @property
+    def vark(self):
+        return self._weighted_var(self._k)
+
+    @property
+    def varg(self):
+        return (self._weighted_var(self._g1) + self._weighted_var(self._g2)) / 2.
+
+    def _get_center_size(self):
+        # catalog.py:1330-1341
+        mx, my = np.mean(self._x), np.mean(self._y)
+        mz = 0
+        dsq = (self._x - mx) ** 2 + (self._y - my) ** 2
+        if self._z is not None:
+            mz = np.mean(self._z)
+            dsq += (self._z - mz) ** 2
+        return mx, my, mz, np.max(dsq) ** 0.5
+
+    # ---- patches ----
+    def get_patches(self):
+        """A list with one Catalog per patch (or [self] without patches)."""
+        if self._patch is None:
+            return [self]
+        out = []
+        for p in range(self._npatch):
+            idx = np.where(self._patch == p)[0]
+            sub = Catalog.__new__(Catalog)
+            sub.__dict__.update(self.__dict__)
+            sub._fields = {}
+            sub._select(idx)
+            sub._patch = None
+            sub._npatch = 1
+            sub._single_patch = p
+            sub.name = '%s patch %d' % (self.name, p)
+            out.append(sub)
+        return out
+
+    _single_patch = None
+
+    # ---- fields ----
+    def _get_field(self, kind, cls, **kw):
+        key = (kind, tuple(sorted(kw.items())))
+        f = self._fields.get(kind)
+        if f is not None and f[0] == key:
+            return f[1]
+        from . import field as _field
+        fld = getattr(_field, cls)(self, **kw)
+        self._fields[kind] = (key, fld)   # one cached field per kind
+        return fld
+
+    def getNField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
+                  min_top=None, max_top=10, coords=None):
+        return self._get_field('N', 'NField', min_size=min_size, max_size=max_size,
+                               split_method=split_method, brute=brute, min_top=min_top,
+                               max_top=max_top, coords=coords)
+
+    def getKField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
+                  min_top=None, max_top=10, coords=None):
+        if self._k is None:
+            raise TypeError("k is not defined for this catalog")
+        return self._get_field('K', 'KField', min_size=min_size, max_size=max_size,
+                               split_method=split_method, brute=brute, min_top=min_top,
+                               max_top=max_top, coords=coords)
+
+    def getGField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
+                  min_top=None, max_top=10, coords=None):
+        if self._g1 is None or self._g2 is None:
+            raise TypeError("g1,g2 are not defined for this catalog")
+        return self._get_field('G', 'GField', min_size=min_size, max_size=max_size,
+                               split_method=split_method, brute=brute, min_top=min_top,
+                               max_top=max_top, coords=coords)
+
+
+def calculateVarK(cat_list):
+    cats = cat_list if isinstance(cat_list, list) else [cat_list]
+    if len(cats) == 1:
+        return cats[0].vark
+    k = np.concatenate([c.k for c in cats])
+    w = np.concatenate([c.w for c in cats])
+    use = w != 0
+    meank = np.sum(w[use] * k[use]) / np.sum(w)
+    return np.sum(w[use] ** 2 * (k[use] - meank) ** 2) / np.sum(w)
+
+
+def calculateVarG(cat_list):
+    cats = cat_list if isinstance(cat_list, list) else [cat_list]
+    if len(cats) == 1:
+        return cats[0].varg
+    w = np.concatenate([c.w for c in cats])
+    use = w != 0
+    out = 0.
+    for comp in ('g1', 'g2'):
+        g = np.concatenate([getattr(c, comp) for c in cats])
+        meang = np.sum(w[use] * g[use]) / np.sum(w)
+        out += np.sum(w[use] ** 2 * (g[use] - meang) ** 2) / np.sum(w)
+    return out / 2.

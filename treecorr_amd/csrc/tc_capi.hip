@@ -1,0 +1,376 @@
+// treecorr_amd: the extern "C" boundary (include/treecorr_amd.h).
+//
+// Handles, argument validation, error reporting, host<->device staging and the
+// accumulate-into-caller-arrays semantics of the reference's Corr objects
+// (include/Corr2.h:210-213).  All compute runs in the HIP kernels of
+// tc_field.hip / tc_corr2.hip / tc_corr3.hip; there is no CPU fallback.
+#include "tc_engine.h"
+#include "../../include/treecorr_amd.h"
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <vector>
+
+namespace tc {
+
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(const std::string& msg) { g_err = msg; return TC_ERR_INVALID; }
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+    return TC_ERR_HIP;
+}
+
+static hipStream_t g_stream = nullptr;
+static std::once_flag g_stream_once;
+hipStream_t stream() {
+    std::call_once(g_stream_once, [] {
+        // Default priority, non-blocking with respect to the null stream.
+        if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess) g_stream = nullptr;
+    });
+    return g_stream;
+}
+
+static bool g_timing = false;
+static KernelStat g_stats[KS_COUNT];
+bool timing_enabled() { return g_timing; }
+KernelStat& kernel_stat(int which) { return g_stats[which]; }
+
+}  // namespace tc
+
+using namespace tc;
+
+struct tc_field {
+    int d = 0, coords = 0;
+    long long nobj = 0;
+    DeviceTree T;
+    ~tc_field() {
+        if (T.nodes) (void)hipFree(T.nodes);
+        if (T.dtop) (void)hipFree(T.dtop);
+    }
+};
+
+struct tc_corr2 {
+    int d = 0;
+    C2Params P{};
+    double* xi[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *meanr = nullptr, *meanlogr = nullptr, *weight = nullptr, *npairs = nullptr;
+    Corr2Scratch* S = nullptr;
+    long long counters[4] = {0, 0, 0, 0};
+    ~tc_corr2() { if (S) delete_corr2_scratch(S); }
+};
+
+namespace {
+
+int check_gpu() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return tc::fail("no usable GPU: treecorr_amd runs only on a HIP device");
+    if (!tc::stream()) return tc::fail("could not create the HIP stream");
+    return 0;
+}
+
+int validate_field_args(int d, int coords, const double* x, const double* y, const double* z,
+                        const double* d1, const double* d2, const double* w, long long n,
+                        int sm, int mintop, int maxtop) {
+    if (d != NData && d != KData && d != GData)
+        return tc::fail("data_type must be 0 (N), 1 (K) or 4 (G)");
+    if (coords != Flat && coords != ThreeD && coords != Sphere)
+        return tc::fail("coords must be 1 (Flat), 2 (ThreeD) or 3 (Sphere)");
+    if (n <= 0) return tc::fail("a field needs at least one object");
+    if (n > (long long)std::numeric_limits<int>::max() / 2)
+        return tc::fail("too many objects for one field (limit 2^30)");
+    if (!x || !y || !w) return tc::fail("x, y and w are required");
+    if (coords != Flat && !z) return tc::fail("z is required for ThreeD and Sphere coordinates");
+    if (d == KData && !d1) return tc::fail("k values are required for a K field");
+    if (d == GData && (!d1 || !d2)) return tc::fail("g1 and g2 are required for a G field");
+    if (sm == Random) {
+        set_error("split_method='random' draws from the C library rand() and is not supported");
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (sm != Middle && sm != Median && sm != Mean) return tc::fail("invalid split_method");
+    if (maxtop < 0 || maxtop > 60) return tc::fail("max_top must be in [0, 60]");
+    (void)mintop;
+    return 0;
+}
+
+int make_field(bool device, int d, int coords, const double* x, const double* y, const double* z,
+               const double* d1, const double* d2, const double* w, const double* wpos,
+               long long n, double minsize, double maxsize, int sm, int brute, int mintop,
+               int maxtop, tc_field** out) {
+    if (!out) return tc::fail("out is NULL");
+    *out = nullptr;
+    if (int rc = check_gpu()) return rc;
+    if (int rc = validate_field_args(d, coords, x, y, z, d1, d2, w, n, sm, mintop, maxtop)) return rc;
+    const double* src[7] = {x, y, z, d1, d2, w, wpos};
+    double* dev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int rc = 0;
+    if (!device) {
+        for (int k = 0; k < 7 && rc == 0; ++k) {
+            if (!src[k]) continue;
+            if (coords == Flat && k == 2) continue;
+            hipError_t e = hipMalloc(&dev[k], sizeof(double) * n);
+            if (e == hipSuccess) e = hipMemcpy(dev[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = tc::hip_fail(e, "staging catalog columns");
+        }
+    } else {
+        for (int k = 0; k < 7; ++k) dev[k] = const_cast<double*>(src[k]);
+        if (coords == Flat) dev[2] = nullptr;
+    }
+    tc_field* f = nullptr;
+    if (rc == 0) {
+        f = new tc_field;
+        f->d = d; f->coords = coords; f->nobj = n;
+        rc = tc::build_tree(d, coords, dev[0], dev[1], dev[2], dev[3], dev[4], dev[5], dev[6], n,
+                            minsize, maxsize, sm, brute, mintop, maxtop, f->T);
+    }
+    if (!device)
+        for (int k = 0; k < 7; ++k) if (dev[k]) (void)hipFree(dev[k]);
+    if (rc) { delete f; return rc; }
+    *out = f;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tc_version(void) { return 100; }
+const char* tc_last_error(void) { return tc::g_err.c_str(); }
+
+int tc_device_count(int* count) {
+    if (!count) return tc::fail("count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return 0;
+}
+
+int tc_field_create(int d, int coords, const double* x, const double* y, const double* z,
+                    const double* d1, const double* d2, const double* w, const double* wpos,
+                    long long nobj, double minsize, double maxsize, int sm, long long seed,
+                    int brute, int mintop, int maxtop, tc_field** out) {
+    (void)seed;
+    return make_field(false, d, coords, x, y, z, d1, d2, w, wpos, nobj, minsize, maxsize, sm,
+                      brute, mintop, maxtop, out);
+}
+
+int tc_field_create_device(int d, int coords, const double* x, const double* y, const double* z,
+                           const double* d1, const double* d2, const double* w,
+                           const double* wpos, long long nobj, double minsize, double maxsize,
+                           int sm, long long seed, int brute, int mintop, int maxtop,
+                           tc_field** out) {
+    (void)seed;
+    return make_field(true, d, coords, x, y, z, d1, d2, w, wpos, nobj, minsize, maxsize, sm,
+                      brute, mintop, maxtop, out);
+}
+
+int tc_field_nobj(const tc_field* f, long long* nobj) {
+    if (!f || !nobj) return tc::fail("NULL argument");
+    *nobj = f->nobj;
+    return 0;
+}
+int tc_field_size(const tc_field* f, double* size) {
+    if (!f || !size) return tc::fail("NULL argument");
+    *size = std::sqrt(f->T.info.sizesq);
+    return 0;
+}
+int tc_field_ntop(const tc_field* f, long long* ntop) {
+    if (!f || !ntop) return tc::fail("NULL argument");
+    *ntop = (long long)f->T.top.size();
+    return 0;
+}
+int tc_field_center(const tc_field* f, double* c) {
+    if (!f || !c) return tc::fail("NULL argument");
+    c[0] = f->T.info.cx; c[1] = f->T.info.cy; c[2] = f->T.info.cz;
+    return 0;
+}
+int tc_field_nnodes(const tc_field* f, long long* nn) {
+    if (!f || !nn) return tc::fail("NULL argument");
+    *nn = f->T.nnodes;
+    return 0;
+}
+int tc_field_copy_nodes(const tc_field* f, void* nodes_out, int* top_out) {
+    if (!f || !nodes_out || !top_out) return tc::fail("NULL argument");
+    TC_HIP(hipMemcpy(nodes_out, f->T.nodes, sizeof(DNode) * f->T.nnodes, hipMemcpyDeviceToHost));
+    std::memcpy(top_out, f->T.top.data(), sizeof(int) * f->T.top.size());
+    return 0;
+}
+int tc_field_destroy(tc_field* f) {
+    delete f;
+    return 0;
+}
+
+int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, int nbins,
+                    double binsize, double b, double a, double minrpar, double maxrpar,
+                    double xp, double yp, double zp, double* xi0, double* xi1, double* xi2,
+                    double* xi3, double* meanr, double* meanlogr, double* weight,
+                    double* npairs, tc_corr2** out) {
+    (void)minrpar; (void)maxrpar;
+    if (!out) return tc::fail("out is NULL");
+    *out = nullptr;
+    if (d1 != d2 || (d1 != NData && d1 != KData && d1 != GData)) {
+        set_error("only NN, KK and GG 2-point correlations are implemented");
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (bin_type != 0) {
+        set_error("only Log binning is implemented for 2-point correlations");
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (xp != 0. || yp != 0. || zp != 0.) {
+        set_error("periodic metrics are not implemented");
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (nbins <= 0 || !(minsep > 0.) || !(maxsep > minsep) || !(binsize > 0.))
+        return tc::fail("invalid binning");
+    if (!weight || !meanr || !meanlogr || !npairs)
+        return tc::fail("meanr, meanlogr, weight and npairs are required");
+    if (d1 == KData && !xi0) return tc::fail("KK needs xi0");
+    if (d1 == GData && (!xi0 || !xi1 || !xi2 || !xi3)) return tc::fail("GG needs xi0..xi3");
+    tc_corr2* c = new tc_corr2;
+    c->d = d1;
+    C2Params& P = c->P;
+    P.nbins = nbins; P.quick = 0;
+    P.minsep = minsep; P.maxsep = maxsep; P.binsize = binsize; P.b = b; P.a = a;
+    // BaseCorr2::BaseCorr2, src/Corr2.cpp:51-77
+    P.logminsep = std::log(minsep); P.halfminsep = 0.5 * minsep;
+    P.minsepsq = minsep * minsep; P.maxsepsq = maxsep * maxsep;
+    P.bsq = b * b; P.asq = a * a;
+    c->xi[0] = xi0; c->xi[1] = xi1; c->xi[2] = xi2; c->xi[3] = xi3;
+    c->meanr = meanr; c->meanlogr = meanlogr; c->weight = weight; c->npairs = npairs;
+    c->S = new_corr2_scratch();
+    *out = c;
+    return 0;
+}
+
+static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int metric) {
+    if (!c || !f1) return tc::fail("NULL argument");
+    if (metric != 1) {
+        set_error("only the Euclidean metric is implemented");
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (f1->d != c->d || (f2 && (f2->d != c->d || f2->coords != f1->coords)))
+        return tc::fail("field types do not match the correlation");
+    if (int rc = check_gpu()) return rc;
+    C2Params P = c->P;
+    P.quick = quick ? 1 : 0;
+    const int nb = P.nbins;
+    for (int k = 0; k < 4; ++k) c->counters[k] = 0;
+    if (f2) {
+        // BaseCorr2::process(f1,f2) early exit, src/Corr2.cpp:215-231
+        const FieldInfo& a = f1->T.info;
+        const FieldInfo& b = f2->T.info;
+        double dx = a.cx - b.cx, dy = a.cy - b.cy, dz = a.cz - b.cz;
+        double rsq = f1->coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
+        double s1ps2 = std::sqrt(a.sizesq) + std::sqrt(b.sizesq);
+        bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
+                         rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
+        bool too_large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+        if (too_small || too_large) return 0;
+    }
+    std::vector<double> out((size_t)7 * nb);
+    std::vector<unsigned long long> np(nb), cnt(4);
+    int rc = tc::corr2_run(*c->S, c->d, f1->coords, f1->T.nodes, f1->T.dtop, (int)f1->T.top.size(),
+                           f2 ? f2->T.nodes : nullptr, f2 ? f2->T.dtop : nullptr,
+                           f2 ? (int)f2->T.top.size() : 0, P, out.data(), np.data(), cnt.data());
+    if (rc) return rc;
+    const int nxi = c->d == NData ? 0 : c->d == KData ? 1 : 4;
+    for (int q = 0; q < nxi; ++q)
+        for (int k = 0; k < nb; ++k) c->xi[q][k] += out[(size_t)q * nb + k];
+    for (int k = 0; k < nb; ++k) {
+        c->meanr[k] += out[(size_t)4 * nb + k];
+        c->meanlogr[k] += out[(size_t)5 * nb + k];
+        c->weight[k] += out[(size_t)6 * nb + k];
+        c->npairs[k] += double(np[k]);
+    }
+    for (int k = 0; k < 4; ++k) c->counters[k] = (long long)cnt[k];
+    return 0;
+}
+
+int tc_corr2_process_auto(tc_corr2* c, tc_field* f, int dots, int quick, int metric) {
+    (void)dots;
+    return corr2_common(c, f, nullptr, quick, metric);
+}
+
+int tc_corr2_process_cross(tc_corr2* c, tc_field* f1, tc_field* f2, int dots, int quick,
+                           int metric) {
+    (void)dots;
+    if (!f2) return tc::fail("NULL argument");
+    return corr2_common(c, f1, f2, quick, metric);
+}
+
+int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, double y1,
+                            double z1, double s1, double x2, double y2, double z2, double s2,
+                            int* result) {
+    if (!c || !result) return tc::fail("NULL argument");
+    if (metric != 1) {
+        set_error("only the Euclidean metric is implemented");
+        return TC_ERR_UNSUPPORTED;
+    }
+    // TriviallyZero3 builds Position<C>(x,y,z) (Sphere normalises) and calls
+    // BaseCorr2::triviallyZero, src/Corr2.cpp:590-601, 1040-1048.
+    if (coords == Sphere) {
+        auto norm = [](double& x, double& y, double& z) {
+            double n = std::sqrt(x * x + y * y + z * z);
+            if (n == 0.) { x = 1; y = 0; z = 0; return; }
+            double r = 1. / n; x *= r; y *= r; z *= r;
+        };
+        norm(x1, y1, z1); norm(x2, y2, z2);
+    }
+    double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+    double rsq = coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
+    double s1ps2 = s1 + s2;
+    const C2Params& P = c->P;
+    *result = (rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2)) ? 1 : 0;
+    return 0;
+}
+
+int tc_corr2_counters(const tc_corr2* c, long long* counters4) {
+    if (!c || !counters4) return tc::fail("NULL argument");
+    for (int k = 0; k < 4; ++k) counters4[k] = c->counters[k];
+    return 0;
+}
+
+int tc_corr2_destroy(tc_corr2* c) {
+    delete c;
+    return 0;
+}
+
+int tc_generate_xyz(double* x, double* y, double* z, const double* ra, const double* dec,
+                    const double* r, long long n) {
+    if (!x || !y || !z || !ra || !dec) return tc::fail("NULL argument");
+    for (long long i = 0; i < n; ++i) {
+        double sr, cr, sd, cd;
+        ::sincos(ra[i], &sr, &cr);
+        ::sincos(dec[i], &sd, &cd);
+        x[i] = cd * cr; y[i] = cd * sr; z[i] = sd;
+        if (r) { x[i] *= r[i]; y[i] *= r[i]; z[i] *= r[i]; }
+    }
+    return 0;
+}
+
+static int g_threads = 1;
+int tc_set_num_threads(int n) { g_threads = n > 0 ? n : 1; return 0; }
+int tc_get_num_threads(int* n) {
+    if (!n) return tc::fail("NULL argument");
+    *n = g_threads;
+    return 0;
+}
+
+int tc_set_timing(int on) { tc::g_timing = on != 0; return 0; }
+int tc_kernel_stats(int which, double* ms, long long* launches) {
+    if (which < 0 || which >= KS_COUNT || !ms || !launches) return tc::fail("invalid argument");
+    *ms = tc::g_stats[which].ms;
+    *launches = tc::g_stats[which].launches;
+    return 0;
+}
+int tc_reset_kernel_stats(void) {
+    for (int k = 0; k < KS_COUNT; ++k) tc::g_stats[k] = KernelStat();
+    return 0;
+}
+
+}  // extern "C"

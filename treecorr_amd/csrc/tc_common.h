@@ -1,0 +1,108 @@
+// treecorr_amd: shared definitions for the device ball-tree engine (gfx950).
+//
+// Device layout in HBM
+//   DNode  48 B per tree node, children of node i at child(i), child(i)+1.
+//   Elements of a catalog live as structure-of-arrays (x, y, z, wpos: double;
+//   w, a, b: float payloads) and are physically permuted level by level while
+//   the tree is built.
+//
+// All floating-point code here is compiled with -ffp-contract=off: the engine
+// reproduces the arithmetic of TreeCorr's C++ (which runs without FMA) bit for
+// bit where results depend on it (tree payloads, bin decisions).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+namespace tc {
+
+enum Coord { Flat = 1, ThreeD = 2, Sphere = 3 };       // include/Position.h:25
+enum DType { NData = 0, KData = 1, GData = 4 };        // include/Cell.h:34
+enum SplitMethod { Middle = 0, Median = 1, Mean = 2, Random = 3 };  // include/Cell.h:26
+
+// One tree node.  Mirrors BaseCell/CellData (include/Cell.h:79-353): float size,
+// float w, float payload (wk, or the real/imag parts of wz), integer count.
+struct alignas(16) DNode {
+    double x, y, z;
+    float size, w;
+    float a, b;     // KData: a = w*k sum; GData: (a,b) = w*g sum (parallel transported)
+    int child;      // first child index (second child is child+1); -1 for a leaf
+    int n;          // number of catalog objects below this node
+};
+static_assert(sizeof(DNode) == 48, "DNode must stay 48 bytes");
+
+// Traversal work item: a node pair (i, j) of the two trees, or j < 0 for the
+// self-pairs of node i (BaseCorr2::process2).
+struct alignas(8) Item { int i, j; };
+
+// Record kept per device tree.
+struct FieldInfo {
+    double cx, cy, cz;   // Field::_center
+    double sizesq;       // Field::_sizesq
+};
+
+void set_error(const std::string& msg);
+int fail(const std::string& msg);     // sets the error, returns TC_ERR_INVALID
+int hip_fail(hipError_t e, const char* what);  // sets the error, returns TC_ERR_HIP
+
+hipStream_t stream();   // the library's stream (default priority)
+
+// Timing of the dominant kernels, recorded with HIP events on stream() when enabled.
+struct KernelStat { double ms = 0; long long launches = 0; };
+bool timing_enabled();
+KernelStat& kernel_stat(int which);
+enum { KS_TRAVERSE2 = 0, KS_TREE = 1, KS_MULTIPOLE = 2, KS_ZETA = 3, KS_COUNT = 4 };
+
+}  // namespace tc
+
+#define TC_HIP(call)                                                     \
+    do {                                                                 \
+        hipError_t _e = (call);                                          \
+        if (_e != hipSuccess) return ::tc::hip_fail(_e, #call);          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+namespace tc {
+
+__device__ __forceinline__ double bcast_lane(double v, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ProjectHelper.h:25-32, :45-47: conj(r*r)/|r|^2 with |r|^2 -> 1 when zero.
+__device__ __forceinline__ void expm2ialpha(double re, double im, double& er, double& ei) {
+    double n = re * re + im * im;
+    n = n > 0. ? n : 1.;
+    double rr = re * re - im * im;
+    double ri = re * im + im * re;
+    er = rr / n;
+    ei = -ri / n;
+}
+
+// Position<Sphere>::normalize as compiled where a Position<Sphere> is built
+// from coordinates (reciprocal multiply) -- see the oracle's notes.
+__device__ __forceinline__ void normalize_recip(double& x, double& y, double& z) {
+    double n = sqrt(x * x + y * y + z * z);
+    if (n == 0.) { x = 1.; y = 0.; z = 0.; return; }
+    double r = 1. / n;
+    x *= r; y *= r; z *= r;
+}
+__device__ __forceinline__ void normalize_div(double& x, double& y, double& z) {
+    double n = sqrt(x * x + y * y + z * z);
+    if (n == 0.) { x = 1.; y = 0.; z = 0.; return; }
+    x /= n; y /= n; z /= n;
+}
+
+}  // namespace tc

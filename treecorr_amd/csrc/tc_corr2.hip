@@ -1,0 +1,475 @@
+// treecorr_amd: the 2-point dual-tree traversal and pair binning on the GPU.
+//
+// TreeCorr recurses process2/process11 depth first on every CPU thread
+// (src/Corr2.cpp:146-389).  Here the recursion becomes a work stack of node-pair
+// items processed in large chunks: every launch pops up to CHUNK items, each lane
+// applies the process11 decision (prune / accept into one bin / split 2 or 4
+// ways), accepted pairs are binned on the spot (directProcess11 + finishProcess,
+// src/Corr2.cpp:476-566) into a per-workgroup LDS histogram, and the children of
+// split pairs are compacted per wavefront (one atomic per wave) into the output
+// queue, which is pushed back onto the stack.  Popping the newest items first
+// keeps the stack depth bounded like the reference's recursion.
+//
+// Accumulation: npairs in exact u64, every other sum in FP64 -- LDS atomics
+// within a launch, Neumaier-compensated per-workgroup slots across launches, and
+// a fixed-order compensated reduction over slots at the end.
+#include "tc_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace tc {
+
+
+enum { Q_XI0 = 0, Q_XI1, Q_XI2, Q_XI3, Q_MEANR, Q_MEANLOGR, Q_WEIGHT, NQ2 = 7 };
+enum { CNT_VISITS = 0, CNT_ACCEPTED, CNT_LEAF, CNT_REFINED, NCNT = 4 };
+
+constexpr int TRAV_THREADS = 256;
+constexpr int TRAV_SLOTS = 2048;          // persistent workgroups = accumulation slots
+constexpr long long TRAV_CHUNK = 1LL << 23;
+
+template <int C>
+__device__ __forceinline__ double dist_sq(const DNode& a, const DNode& b) {
+    // MetricHelper<Euclidean>::DistSq, include/Metric.h:143-147, :197-202
+    double dx = a.x - b.x, dy = a.y - b.y;
+    if (C == Flat) return dx * dx + dy * dy;
+    double dz = a.z - b.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// ProjectHelper<Sphere>::calculate_direction, include/ProjectHelper.h:272-345
+__device__ __forceinline__ void sphere_direction(double x1, double y1, double z1, double x2,
+                                                 double y2, double z2, double& re, double& im) {
+    double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+    double dsq = dx * dx + dy * dy + dz * dz;
+    double cosA = (z1 - z2) + 0.5 * z2 * dsq;
+    double sinA = y1 * x2 - x1 * y2;
+    re = sinA;
+    im = -cosA;
+}
+
+__device__ __forceinline__ void cmul(double& zr, double& zi, double er, double ei) {
+    double r = zr * er - zi * ei;
+    double i = zr * ei + zi * er;
+    zr = r; zi = i;
+}
+
+// BaseCorr2::directProcess11 + Corr2::finishProcess + DirectHelper<0,1,3>
+// (src/Corr2.cpp:394-566) for one accepted pair, accumulated into LDS.
+template <int D, int C>
+__device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, double rsq, int k,
+                                         double r, double logr, const C2Params& P,
+                                         double* hist, unsigned long long* hnp) {
+    const int nb = P.nbins;
+    if (k < 0) {
+        r = sqrt(rsq);
+        logr = log(r);
+        k = int((logr - P.logminsep) / P.binsize);
+    }
+    if (k == nb) --k;
+    double ww = double(c1.w) * double(c2.w);
+    atomicAdd(&hist[Q_WEIGHT * nb + k], ww);
+    if (!P.quick) {
+        atomicAdd(&hnp[k], (unsigned long long)c1.n * (unsigned long long)c2.n);
+        atomicAdd(&hist[Q_MEANR * nb + k], ww * r);
+        atomicAdd(&hist[Q_MEANLOGR * nb + k], ww * logr);
+    }
+    if (D == KData) {
+        atomicAdd(&hist[Q_XI0 * nb + k], double(c1.a) * double(c2.a));
+    } else if (D == GData) {
+        double g1r = c1.a, g1i = c1.b, g2r = c2.a, g2i = c2.b;
+        double er, ei;
+        if (C == Flat) {
+            // ProjectHelper<Flat>::Project, include/ProjectHelper.h:126-136
+            expm2ialpha(c2.x - c1.x, c2.y - c1.y, er, ei);
+            cmul(g1r, g1i, er, ei);
+            cmul(g2r, g2i, er, ei);
+        } else {
+            // ProjectHelper<Sphere>::Project :370-388; ThreeD normalises first :573-589
+            double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
+            if (C == ThreeD) { normalize_recip(x1, y1, z1); normalize_recip(x2, y2, z2); }
+            double dr, di;
+            sphere_direction(x1, y1, z1, x2, y2, z2, dr, di);
+            expm2ialpha(dr, di, er, ei);
+            cmul(g2r, g2i, er, ei);
+            sphere_direction(x2, y2, z2, x1, y1, z1, dr, di);
+            expm2ialpha(dr, di, er, ei);
+            cmul(g1r, g1i, er, ei);
+        }
+        // both_complex_prod(g2, g1), include/ProjectHelper.h:84-101
+        double rr = g2r * g1r, ri = g2r * g1i, ir = g2i * g1r, ii = g2i * g1i;
+        atomicAdd(&hist[Q_XI0 * nb + k], rr + ii);   // xip
+        atomicAdd(&hist[Q_XI1 * nb + k], ri - ir);   // xip_im
+        atomicAdd(&hist[Q_XI2 * nb + k], rr - ii);   // xim
+        atomicAdd(&hist[Q_XI3 * nb + k], ri + ir);   // xim_im
+    }
+}
+
+template <int D, int C>
+__global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
+    const DNode* __restrict__ n1, const DNode* __restrict__ n2, const Item* __restrict__ in,
+    long long nin, Item* __restrict__ out, unsigned long long* __restrict__ outcount, C2Params P,
+    double* __restrict__ slot_hi, double* __restrict__ slot_lo,
+    unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt) {
+    extern __shared__ double sh[];
+    const int nb = P.nbins;
+    const int nh = NQ2 * nb;
+    double* hist = sh;
+    unsigned long long* hnp = reinterpret_cast<unsigned long long*>(sh + nh);
+    unsigned long long* hcnt = hnp + nb;
+    for (int t = threadIdx.x; t < nh + nb + NCNT; t += blockDim.x)
+        reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    unsigned int c_vis = 0, c_acc = 0, c_leaf = 0, c_ref = 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < nin; base += stride) {
+        const long long idx = base + threadIdx.x;
+        Item o0{0, 0}, o1{0, 0}, o2{0, 0}, o3{0, 0};
+        int no = 0;
+        if (idx < nin) {
+            const Item it = in[idx];
+            if (it.j < 0) {
+                // BaseCorr2::process2, src/Corr2.cpp:290-306
+                const DNode c = n1[it.i];
+                if (c.w != 0.f && double(c.size) > P.halfminsep) {
+                    const int L = c.child;
+                    o0 = Item{L, -1}; o1 = Item{L + 1, -1}; o2 = Item{L, L + 1};
+                    no = 3;
+                }
+            } else {
+                // BaseCorr2::process11, src/Corr2.cpp:308-389
+                ++c_vis;
+                const DNode c1 = n1[it.i];
+                const DNode c2 = n2[it.j];
+                if (c1.w != 0.f && c2.w != 0.f) {
+                    const double s1 = c1.size, s2 = c2.size;
+                    const double rsq = dist_sq<C>(c1, c2);
+                    const double s1ps2 = s1 + s2;
+                    const bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
+                                           rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
+                    const bool too_large = rsq >= P.maxsepsq &&
+                                           rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+                    if (!too_small && !too_large) {
+                        // BinTypeHelper<Log>::singleBin, include/BinType.h:93-151
+                        int k = -1;
+                        double r = 0., logr = 0.;
+                        bool single;
+                        if (s1ps2 == 0.) {
+                            single = true;
+                        } else {
+                            const double s1ps2sq = s1ps2 * s1ps2;
+                            if (s1ps2sq > P.asq * rsq) single = false;
+                            else if (s1ps2sq <= P.bsq * rsq) single = true;
+                            else if (s1ps2sq > 0.25 * ((P.binsize + P.b) * (P.binsize + P.b)) * rsq)
+                                single = false;
+                            else {
+                                ++c_ref;
+                                logr = 0.5 * log(rsq);
+                                const double kk = (logr - P.logminsep) / P.binsize;
+                                const int ik = int(kk);
+                                const double frackk = kk - ik;
+                                const double omf = 1. - frackk;
+                                const double f = (omf < frackk) ? omf : frackk;
+                                const double t1 = f * P.binsize + P.b;
+                                if (s1ps2sq > (t1 * t1) * rsq) single = false;
+                                else {
+                                    const double t2 = frackk * P.binsize + P.b - s1ps2sq / rsq;
+                                    if (s1ps2sq > (t2 * t2) * rsq) single = false;
+                                    else { single = true; k = ik; r = sqrt(rsq); }
+                                }
+                            }
+                        }
+                        if (single) {
+                            if (rsq >= P.minsepsq && rsq < P.maxsepsq) {
+                                ++c_acc;
+                                if (c1.child < 0 && c2.child < 0) ++c_leaf;
+                                bin_pair<D, C>(c1, c2, rsq, k, r, logr, P, hist, hnp);
+                            }
+                        } else {
+                            // CalcSplitSq, include/Split.h:82-101
+                            const double bsq_eff = rsq * (P.asq < P.bsq ? P.asq : P.bsq);
+                            bool sp1, sp2;
+                            if (s2 > s1) {
+                                sp2 = true;
+                                sp1 = (s2 > 2. * s1) ? false : (s1 * s1 > 0.3422 * bsq_eff);
+                            } else if (s1 > 2. * s2) {
+                                sp1 = true; sp2 = false;
+                            } else {
+                                sp1 = true;
+                                sp2 = s2 * s2 > 0.3422 * bsq_eff;
+                            }
+                            if (sp1 && sp2) {
+                                const int L1 = c1.child, L2 = c2.child;
+                                o0 = Item{L1, L2}; o1 = Item{L1, L2 + 1};
+                                o2 = Item{L1 + 1, L2}; o3 = Item{L1 + 1, L2 + 1};
+                                no = 4;
+                            } else if (sp1) {
+                                o0 = Item{c1.child, it.j}; o1 = Item{c1.child + 1, it.j};
+                                no = 2;
+                            } else {
+                                o0 = Item{it.i, c2.child}; o1 = Item{it.i, c2.child + 1};
+                                no = 2;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // Wavefront compaction of the children: one atomic per wave.
+        int incl = no;
+        for (int d = 1; d < 64; d <<= 1) {
+            int y = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += y;
+        }
+        const int total = __shfl(incl, 63, 64);
+        unsigned long long wbase = 0;
+        if (total > 0) {
+            if (lane == 63) wbase = atomicAdd(outcount, (unsigned long long)total);
+            wbase = __shfl(wbase, 63, 64);
+            const unsigned long long pos = wbase + (unsigned long long)(incl - no);
+            if (no > 0) out[pos] = o0;
+            if (no > 1) out[pos + 1] = o1;
+            if (no > 2) out[pos + 2] = o2;
+            if (no > 3) out[pos + 3] = o3;
+        }
+    }
+    atomicAdd(&hcnt[CNT_VISITS], (unsigned long long)c_vis);
+    atomicAdd(&hcnt[CNT_ACCEPTED], (unsigned long long)c_acc);
+    atomicAdd(&hcnt[CNT_LEAF], (unsigned long long)c_leaf);
+    atomicAdd(&hcnt[CNT_REFINED], (unsigned long long)c_ref);
+    __syncthreads();
+    // Flush into this workgroup's slot with Neumaier compensation.
+    const size_t so = (size_t)blockIdx.x * nh;
+    for (int t = threadIdx.x; t < nh; t += blockDim.x) {
+        const double v = hist[t];
+        if (v == 0.) continue;
+        const double s = slot_hi[so + t];
+        const double tt = s + v;
+        const double c = (fabs(s) >= fabs(v)) ? (s - tt) + v : (v - tt) + s;
+        slot_hi[so + t] = tt;
+        slot_lo[so + t] += c;
+    }
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) slot_np[(size_t)blockIdx.x * nb + t] += hnp[t];
+    if (threadIdx.x < NCNT) slot_cnt[(size_t)blockIdx.x * NCNT + threadIdx.x] += hcnt[threadIdx.x];
+}
+
+// Fixed-order compensated reduction of the per-workgroup slots.
+__global__ void k_reduce_slots(const double* slot_hi, const double* slot_lo,
+                               const unsigned long long* slot_np,
+                               const unsigned long long* slot_cnt, int nslots, int nh, int nb,
+                               double* out, unsigned long long* out_np,
+                               unsigned long long* out_cnt) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nh) {
+        double s = 0., c = 0.;
+        for (int k = 0; k < nslots; ++k) {
+            const double vs[2] = {slot_hi[(size_t)k * nh + t], slot_lo[(size_t)k * nh + t]};
+            for (int m = 0; m < 2; ++m) {
+                const double v = vs[m];
+                const double tt = s + v;
+                c += (fabs(s) >= fabs(v)) ? (s - tt) + v : (v - tt) + s;
+                s = tt;
+            }
+        }
+        out[t] = s + c;
+    } else if (t < nh + nb) {
+        const int k = t - nh;
+        unsigned long long s = 0;
+        for (int j = 0; j < nslots; ++j) s += slot_np[(size_t)j * nb + k];
+        out_np[k] = s;
+    } else if (t < nh + nb + NCNT) {
+        const int k = t - nh - nb;
+        unsigned long long s = 0;
+        for (int j = 0; j < nslots; ++j) s += slot_cnt[(size_t)j * NCNT + k];
+        out_cnt[k] = s;
+    }
+}
+
+__global__ void k_items_auto(const int* top, int ntop, Item* out) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long npair = (long long)ntop * (ntop - 1) / 2;
+    if (t < ntop) {
+        out[t] = Item{top[t], -1};
+    } else if (t < ntop + npair) {
+        // map t - ntop to (i, j), i < j, row-major over i
+        long long q = t - ntop;
+        long long i = 0;
+        // row i holds ntop-1-i pairs; solve directly then adjust
+        double nn = double(ntop);
+        i = (long long)((2 * nn - 1 - sqrt((2 * nn - 1) * (2 * nn - 1) - 8.0 * double(q))) / 2);
+        if (i < 0) i = 0;
+        auto row_start = [&](long long r) { return r * (2 * (long long)ntop - r - 1) / 2; };
+        while (i > 0 && row_start(i) > q) --i;
+        while (row_start(i + 1) <= q) ++i;
+        long long j = i + 1 + (q - row_start(i));
+        out[t] = Item{top[i], top[j]};
+    }
+}
+
+__global__ void k_items_cross(const int* top1, int ntop1, const int* top2, int ntop2, Item* out) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long long)ntop1 * ntop2) {
+        long long i = t / ntop2, j = t % ntop2;
+        out[t] = Item{top1[i], top2[j]};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct Corr2Scratch {
+    Item* stack = nullptr;
+    long long cap = 0;
+    Item* outq = nullptr;
+    double *slot_hi = nullptr, *slot_lo = nullptr;
+    unsigned long long *slot_np = nullptr, *slot_cnt = nullptr;
+    double* red = nullptr;
+    unsigned long long *red_np = nullptr, *red_cnt = nullptr;
+    unsigned long long* outcount = nullptr;
+    unsigned long long* h_outcount = nullptr;
+    int nb = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~Corr2Scratch() {
+        if (stack) (void)hipFree(stack);
+        if (outq) (void)hipFree(outq);
+        if (slot_hi) (void)hipFree(slot_hi);
+        if (slot_lo) (void)hipFree(slot_lo);
+        if (slot_np) (void)hipFree(slot_np);
+        if (slot_cnt) (void)hipFree(slot_cnt);
+        if (red) (void)hipFree(red);
+        if (red_np) (void)hipFree(red_np);
+        if (red_cnt) (void)hipFree(red_cnt);
+        if (outcount) (void)hipFree(outcount);
+        if (h_outcount) (void)hipHostFree(h_outcount);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+};
+
+static int ensure_scratch(Corr2Scratch& S, int nb) {
+    if (S.nb == nb && S.slot_hi) return 0;
+    const size_t nh = (size_t)NQ2 * nb;
+    if (S.slot_hi) { (void)hipFree(S.slot_hi); (void)hipFree(S.slot_lo); (void)hipFree(S.slot_np);
+                     (void)hipFree(S.red); (void)hipFree(S.red_np); }
+    TC_HIP(hipMalloc(&S.slot_hi, sizeof(double) * nh * TRAV_SLOTS));
+    TC_HIP(hipMalloc(&S.slot_lo, sizeof(double) * nh * TRAV_SLOTS));
+    TC_HIP(hipMalloc(&S.slot_np, sizeof(unsigned long long) * nb * TRAV_SLOTS));
+    TC_HIP(hipMalloc(&S.red, sizeof(double) * nh));
+    TC_HIP(hipMalloc(&S.red_np, sizeof(unsigned long long) * nb));
+    if (!S.slot_cnt) {
+        TC_HIP(hipMalloc(&S.slot_cnt, sizeof(unsigned long long) * NCNT * TRAV_SLOTS));
+        TC_HIP(hipMalloc(&S.red_cnt, sizeof(unsigned long long) * NCNT));
+        TC_HIP(hipMalloc(&S.outcount, sizeof(unsigned long long)));
+        TC_HIP(hipHostMalloc((void**)&S.h_outcount, sizeof(unsigned long long)));
+        TC_HIP(hipMalloc(&S.outq, sizeof(Item) * 4 * TRAV_CHUNK));
+        TC_HIP(hipEventCreate(&S.ev0));
+        TC_HIP(hipEventCreate(&S.ev1));
+    }
+    S.nb = nb;
+    return 0;
+}
+
+static int ensure_stack(Corr2Scratch& S, long long need, long long keep) {
+    if (need <= S.cap) return 0;
+    long long cap = std::max<long long>(need, std::max<long long>(2 * S.cap, 1LL << 22));
+    Item* ns = nullptr;
+    TC_HIP(hipMalloc(&ns, sizeof(Item) * cap));
+    if (S.stack && keep > 0)
+        TC_HIP(hipMemcpyAsync(ns, S.stack, sizeof(Item) * keep, hipMemcpyDeviceToDevice, stream()));
+    TC_HIP(hipStreamSynchronize(stream()));
+    if (S.stack) (void)hipFree(S.stack);
+    S.stack = ns;
+    S.cap = cap;
+    return 0;
+}
+
+using TravFn = void (*)(const DNode*, const DNode*, const Item*, long long, Item*,
+                        unsigned long long*, C2Params, double*, double*, unsigned long long*,
+                        unsigned long long*);
+
+static TravFn pick_traverse(int D, int C) {
+#define TC_TF(DD, CC) return &k_traverse<DD, CC>
+    if (D == NData) { if (C == Flat) TC_TF(NData, Flat); if (C == ThreeD) TC_TF(NData, ThreeD); TC_TF(NData, Sphere); }
+    if (D == KData) { if (C == Flat) TC_TF(KData, Flat); if (C == ThreeD) TC_TF(KData, ThreeD); TC_TF(KData, Sphere); }
+    if (C == Flat) TC_TF(GData, Flat); if (C == ThreeD) TC_TF(GData, ThreeD); TC_TF(GData, Sphere);
+#undef TC_TF
+}
+
+// Runs the whole traversal for one auto or cross correlation and writes the
+// reduced sums: out[NQ2*nb] doubles, np[nb], cnt[NCNT] (host arrays).
+int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dtop1, int ntop1,
+              const DNode* nodes2, const int* dtop2, int ntop2, const C2Params& P,
+              double* out, unsigned long long* np, unsigned long long* cnt) {
+    hipStream_t st = stream();
+    const int nb = P.nbins;
+    const size_t nh = (size_t)NQ2 * nb;
+    const size_t shbytes = sizeof(double) * (nh + nb + NCNT);
+    if (shbytes > 64 * 1024) return fail("too many bins for the LDS histogram (nbins <= 1000)");
+    if (ensure_scratch(S, nb)) return 2;
+    TC_HIP(hipMemsetAsync(S.slot_hi, 0, sizeof(double) * nh * TRAV_SLOTS, st));
+    TC_HIP(hipMemsetAsync(S.slot_lo, 0, sizeof(double) * nh * TRAV_SLOTS, st));
+    TC_HIP(hipMemsetAsync(S.slot_np, 0, sizeof(unsigned long long) * nb * TRAV_SLOTS, st));
+    TC_HIP(hipMemsetAsync(S.slot_cnt, 0, sizeof(unsigned long long) * NCNT * TRAV_SLOTS, st));
+
+    const bool autoc = (nodes2 == nullptr);
+    long long ninit = autoc ? (long long)ntop1 + (long long)ntop1 * (ntop1 - 1) / 2
+                            : (long long)ntop1 * ntop2;
+    if (ensure_stack(S, ninit + 4 * TRAV_CHUNK, 0)) return 2;
+    if (ninit > 0) {
+        dim3 g((unsigned)((ninit + 255) / 256));
+        if (autoc) hipLaunchKernelGGL(k_items_auto, g, dim3(256), 0, st, dtop1, ntop1, S.stack);
+        else hipLaunchKernelGGL(k_items_cross, g, dim3(256), 0, st, dtop1, ntop1, dtop2, ntop2, S.stack);
+        TC_HIP(hipGetLastError());
+    }
+    TravFn fn = pick_traverse(D, C);
+    const DNode* n2 = autoc ? nodes1 : nodes2;
+    long long top = ninit;
+    KernelStat& ks = kernel_stat(KS_TRAVERSE2);
+    const bool timing = timing_enabled();
+    while (top > 0) {
+        const long long n = std::min(top, TRAV_CHUNK);
+        const Item* inp = S.stack + (top - n);
+        TC_HIP(hipMemsetAsync(S.outcount, 0, sizeof(unsigned long long), st));
+        const int blocks = (int)std::min<long long>(TRAV_SLOTS, (n + TRAV_THREADS - 1) / TRAV_THREADS);
+        if (timing) TC_HIP(hipEventRecord(S.ev0, st));
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(TRAV_THREADS), shbytes, st, nodes1, n2, inp, n,
+                           S.outq, S.outcount, P, S.slot_hi, S.slot_lo, S.slot_np, S.slot_cnt);
+        TC_HIP(hipGetLastError());
+        if (timing) TC_HIP(hipEventRecord(S.ev1, st));
+        TC_HIP(hipMemcpyAsync(S.h_outcount, S.outcount, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, st));
+        TC_HIP(hipStreamSynchronize(st));
+        if (timing) {
+            float ms = 0.f;
+            TC_HIP(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+            ks.ms += ms;
+            ks.launches += 1;
+        }
+        const long long nout = (long long)*S.h_outcount;
+        top -= n;
+        if (nout > 0) {
+            if (ensure_stack(S, top + nout + 4 * TRAV_CHUNK, top)) return 2;
+            TC_HIP(hipMemcpyAsync(S.stack + top, S.outq, sizeof(Item) * nout,
+                                  hipMemcpyDeviceToDevice, st));
+            top += nout;
+        }
+    }
+    const int tot = (int)(nh + nb + NCNT);
+    hipLaunchKernelGGL(k_reduce_slots, dim3((tot + 255) / 256), dim3(256), 0, st, S.slot_hi,
+                       S.slot_lo, S.slot_np, S.slot_cnt, TRAV_SLOTS, (int)nh, nb, S.red, S.red_np,
+                       S.red_cnt);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipMemcpyAsync(out, S.red, sizeof(double) * nh, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(np, S.red_np, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(cnt, S.red_cnt, sizeof(unsigned long long) * NCNT, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+Corr2Scratch* new_corr2_scratch() { return new Corr2Scratch(); }
+void delete_corr2_scratch(Corr2Scratch* s) { delete s; }
+
+}  // namespace tc

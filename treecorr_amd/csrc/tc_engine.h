@@ -1,0 +1,37 @@
+// treecorr_amd: engine-level types and entry points shared by the host-side
+// translation units (tree build, 2-point traversal, 3-point multipole, C ABI).
+#pragma once
+#include "tc_common.h"
+#include <vector>
+
+namespace tc {
+
+// A tree resident in GPU memory.
+struct DeviceTree {
+    DNode* nodes = nullptr;
+    int nnodes = 0;
+    std::vector<int> top;     // top-level node ids in TreeCorr's order
+    int* dtop = nullptr;      // the same on the device
+    FieldInfo info{};
+};
+
+int build_tree(int D, int C, const double* x, const double* y, const double* z, const double* d1,
+               const double* d2, const double* w, const double* wpos, long long n,
+               double minsize, double maxsize, int sm, int brute, int mintop, int maxtop,
+               DeviceTree& T);
+
+// Log-binning parameters of one 2-point correlation (BaseCorr2, src/Corr2.cpp:51-77).
+struct C2Params {
+    int nbins, quick;
+    double minsep, maxsep, binsize, b, a;
+    double logminsep, halfminsep, minsepsq, maxsepsq, bsq, asq;
+};
+
+struct Corr2Scratch;
+Corr2Scratch* new_corr2_scratch();
+void delete_corr2_scratch(Corr2Scratch*);
+int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dtop1, int ntop1,
+              const DNode* nodes2, const int* dtop2, int ntop2, const C2Params& P, double* out,
+              unsigned long long* np, unsigned long long* cnt);
+
+}  // namespace tc

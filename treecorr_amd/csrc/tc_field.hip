@@ -1,0 +1,614 @@
+// treecorr_amd: device construction of TreeCorr's mean-split ball tree.
+//
+// Reproduces Field<D,C>::Field + BuildCells (src/Field.cpp:163-277) and
+// BuildCell/SplitData/BuildCellData/finishAverages (src/Cell.cpp:89-615) level by
+// level on the GPU:
+//   * every segment (a node's object range) is owned by one wavefront, which
+//     computes the weighted centroid and payload sums in exactly the sequential
+//     order of the reference (lanes load 64 objects coalesced, the running sums
+//     walk the lanes in order), then size^2 and the bounding box (max/min are
+//     order free, so lanes reduce them in parallel);
+//   * the split is libstdc++'s std::partition order, computed with one global
+//     scan: the k-th misplaced object from the left swaps with the k-th
+//     misplaced object from the right;
+//   * the rare degenerate split (and split_method='median') runs a serial
+//     restatement of std::nth_element (tc_select.h) on one thread;
+//   * objects are physically permuted every level so segment loops stay
+//     contiguous and coalesced.
+#include "tc_engine.h"
+#include "tc_select.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace tc {
+
+enum SegMode { SEG_TOP = 0, SEG_BUILD = 1 };
+
+struct Seg {
+    long long start, end;
+    int node, mode, mintop, maxtop;
+    unsigned long long key;
+    int depth, pad;
+};
+
+struct SegOut {
+    long long mid;
+    double value;
+    int split, dim, fallback, childL;
+    int child_mode, pad;
+};
+
+struct TopEntry { unsigned long long key; int node, pad; };
+
+struct Elems {
+    double *x, *y, *z, *wp;
+    float *w, *a, *b;
+    int* seg;
+};
+
+struct BuildParams {
+    double minsizesq, maxsizesq;
+    int sm, brute;
+};
+
+// ---------------------------------------------------------------------------
+// Object setup: the leaf CellData constructors (include/Cell.h:79-81, 131-133,
+// 167-169) and CellDataHelper (src/Field.cpp:83-144).
+// ---------------------------------------------------------------------------
+template <int D, int C>
+__global__ void k_init_elems(Elems E, const double* x, const double* y, const double* z,
+                             const double* d1, const double* d2, const double* w,
+                             const double* wpos, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double px = x[i], py = y[i], pz = (C == Flat) ? 0. : z[i];
+    if (C == Sphere) normalize_recip(px, py, pz);
+    double wi = w[i];
+    E.x[i] = px; E.y[i] = py; E.z[i] = pz;
+    E.w[i] = float(wi);
+    E.wp[i] = wpos ? wpos[i] : wi;
+    float a = 0.f, b = 0.f;
+    if (D == KData) a = float(wi * d1[i]);
+    if (D == GData) { a = float(wi * d1[i]); b = float(wi * d2[i]); }
+    E.a[i] = a; E.b[i] = b;
+    E.seg[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Per-segment work: one wavefront per segment.
+// ---------------------------------------------------------------------------
+template <int D, int C>
+__global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, Elems E,
+                                               DNode* nodes, int* counters, TopEntry* tops,
+                                               BuildParams P, FieldInfo* finfo, int level0) {
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const Seg sg = segs[s];
+    const long long start = sg.start, end = sg.end, n = end - start;
+
+    double cx = E.x[start], cy = E.y[start], cz = E.z[start];
+    float wf = E.w[start];
+    double sizesq = 0.;
+    double mnx = 0, mxx = 0, mny = 0, mxy = 0, mnz = 0, mxz = 0;
+
+    if (n > 1 || level0) {
+        // BuildCellData (src/Cell.cpp:170-200): sums start from the first object,
+        // which -0.0 reproduces exactly (-0.0 + v == v for every v).
+        double ax = -0.0, ay = -0.0, az = -0.0, asw = -0.0, aww = -0.0;
+        for (long long c0 = start; c0 < end; c0 += 64) {
+            long long i = c0 + lane;
+            double vx = 0, vy = 0, vz = 0, vw = 0, vww = 0;
+            if (i < end) {
+                double wp = E.wp[i];
+                vx = E.x[i] * wp; vy = E.y[i] * wp; vz = E.z[i] * wp;
+                vw = wp; vww = double(E.w[i]);
+            }
+            const int cnt = (int)((end - c0) < 64 ? (end - c0) : 64);
+            for (int j = 0; j < cnt; ++j) {
+                ax += bcast_lane(vx, j);
+                ay += bcast_lane(vy, j);
+                if (C != Flat) az += bcast_lane(vz, j);
+                asw += bcast_lane(vw, j);
+                aww += bcast_lane(vww, j);
+            }
+        }
+        wf = float(aww);
+        if (asw != 0.) {
+            cx = ax / asw; cy = ay / asw; cz = (C == Flat) ? 0. : az / asw;
+            if (C == Sphere) normalize_div(cx, cy, cz);
+        }
+        // CalculateSizeSq (src/Cell.cpp:89-100) and Bounds (include/Bounds.h).
+        double smax = 0.;
+        double lnx = INFINITY, lxx = -INFINITY, lny = INFINITY, lxy = -INFINITY;
+        double lnz = INFINITY, lxz = -INFINITY;
+        for (long long i = start + lane; i < end; i += 64) {
+            double px = E.x[i], py = E.y[i], pz = E.z[i];
+            double dx = cx - px, dy = cy - py;
+            double dsq;
+            if (C == Flat) dsq = dx * dx + dy * dy;
+            else { double dz = cz - pz; dsq = dx * dx + dy * dy + dz * dz; }
+            smax = fmax(smax, dsq);
+            lnx = fmin(lnx, px); lxx = fmax(lxx, px);
+            lny = fmin(lny, py); lxy = fmax(lxy, py);
+            lnz = fmin(lnz, pz); lxz = fmax(lxz, pz);
+        }
+        sizesq = wave_max(smax);
+        mnx = wave_min(lnx); mxx = wave_max(lxx);
+        mny = wave_min(lny); mxy = wave_max(lxy);
+        mnz = wave_min(lnz); mxz = wave_max(lxz);
+        if (level0 && lane == 0) {
+            finfo->cx = cx; finfo->cy = cy; finfo->cz = cz; finfo->sizesq = sizesq;
+        }
+    }
+
+    // SetupTopLevelCells (src/Field.cpp:27-81) and BuildCell (src/Cell.cpp:556-615).
+    int mode = sg.mode;
+    int node = sg.node;
+    bool split = false;
+    if (mode == SEG_TOP) {
+        double ssq = (n == 1) ? 0. : sizesq;
+        if (ssq == 0 || (ssq <= P.maxsizesq && sg.mintop <= 0) || sg.maxtop <= 0) {
+            if (lane == 0) {
+                node = atomicAdd(&counters[0], 1);
+                int t = atomicAdd(&counters[1], 1);
+                tops[t].key = sg.key;
+                tops[t].node = node;
+            }
+            node = __shfl(node, 0, 64);
+            mode = SEG_BUILD;
+        } else {
+            split = true;
+        }
+    }
+    int childL = -1;
+    if (mode == SEG_BUILD) {
+        DNode nd;
+        if (n == 1) {
+            long long i = start;
+            nd.x = E.x[i]; nd.y = E.y[i]; nd.z = E.z[i];
+            nd.w = E.w[i]; nd.a = E.a[i]; nd.b = E.b[i];
+            nd.size = 0.f; nd.child = -1; nd.n = 1;
+        } else {
+            // finishAverages (src/Cell.cpp:208-354)
+            float fa = 0.f, fb = 0.f;
+            if (D == KData) {
+                double sa = 0.;
+                for (long long c0 = start; c0 < end; c0 += 64) {
+                    long long i = c0 + lane;
+                    double v = (i < end) ? double(E.a[i]) : 0.;
+                    const int cnt = (int)((end - c0) < 64 ? (end - c0) : 64);
+                    for (int j = 0; j < cnt; ++j) sa += bcast_lane(v, j);
+                }
+                fa = float(sa);
+            } else if (D == GData) {
+                double sr = 0., si = 0.;
+                double ccx = cx, ccy = cy, ccz = cz;
+                if (C == ThreeD) normalize_recip(ccx, ccy, ccz);
+                for (long long c0 = start; c0 < end; c0 += 64) {
+                    long long i = c0 + lane;
+                    double vr = 0., vi = 0.;
+                    if (i < end) {
+                        double gr = double(E.a[i]), gi = double(E.b[i]);
+                        if (C == Flat) {
+                            vr = gr; vi = gi;
+                        } else {
+                            // ParallelTransportSum (src/Cell.cpp:276-321), in the
+                            // evaluation order of the compiled reference.
+                            double px = E.x[i], py = E.y[i], pz = E.z[i];
+                            if (C == ThreeD) normalize_recip(px, py, pz);
+                            double z1 = cz, z2 = pz;
+                            double dx = ccx - px, dy = ccy - py, dz = ccz - pz;
+                            double dsq = (C == Sphere) ? (dx * dx + dy * dy) + dz * dz
+                                                       : (dy * dy + dz * dz) + dx * dx;
+                            if (dsq < 1.e-12) {
+                                vr = gr; vi = gi;
+                            } else {
+                                double cosA = (z1 - z2) + 0.5 * z2 * dsq;
+                                double sinA = ccy * px - ccx * py;
+                                double cosB = (z2 - z1) + 0.5 * z1 * dsq;
+                                double cosbeta = -cosA * cosB + sinA * sinA;
+                                double sinbeta = (C == Sphere)
+                                    ? (cosA + cosB) * sinA
+                                    : ((z1 - z2) + (0.5 * z2 * dsq + cosB)) * sinA;
+                                double er, ei;
+                                expm2ialpha(cosbeta, sinbeta, er, ei);
+                                vr = gr * er - gi * ei;
+                                vi = gr * ei + gi * er;
+                            }
+                        }
+                    }
+                    const int cnt = (int)((end - c0) < 64 ? (end - c0) : 64);
+                    for (int j = 0; j < cnt; ++j) {
+                        sr += bcast_lane(vr, j);
+                        si += bcast_lane(vi, j);
+                    }
+                }
+                fa = float(sr); fb = float(si);
+            }
+            nd.x = cx; nd.y = cy; nd.z = cz;
+            nd.w = wf; nd.a = fa; nd.b = fb; nd.n = (int)n;
+            if (sizesq > P.minsizesq) {
+                nd.size = P.brute ? INFINITY : float(sqrt(sizesq));
+                split = true;
+                if (lane == 0) childL = atomicAdd(&counters[0], 2);
+                childL = __shfl(childL, 0, 64);
+                nd.child = childL;
+            } else {
+                nd.size = 0.f;
+                nd.child = -1;
+            }
+        }
+        if (lane == 0) nodes[node] = nd;
+    }
+
+    if (lane == 0) {
+        SegOut o;
+        o.split = split ? 1 : 0;
+        o.childL = childL;
+        o.child_mode = mode;
+        o.mid = start;
+        o.fallback = 0;
+        o.dim = 0;
+        o.value = 0.;
+        if (split) {
+            // Bounds<C>::getSplit (include/Bounds.h:65-66, 133-141)
+            int dim;
+            if (C == Flat) dim = (mxy - mny) > (mxx - mnx) ? 1 : 0;
+            else {
+                double xr = mxx - mnx, yr = mxy - mny, zr = mxz - mnz;
+                dim = yr > xr ? (zr > yr ? 2 : 1) : (zr > xr ? 2 : 0);
+            }
+            o.dim = dim;
+            if (P.sm == Mean) o.value = dim == 2 ? cz : dim == 1 ? cy : cx;
+            else if (P.sm == Middle)
+                o.value = dim == 2 ? (mxz + mnz) / 2. : dim == 1 ? (mxy + mny) / 2. : (mxx + mnx) / 2.;
+            else o.fallback = 1;   // Median
+        }
+        sout[s] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan of int flags: out[0..n] with out[n] = total.
+// ---------------------------------------------------------------------------
+constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_B = SCAN_T * SCAN_I;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int& total) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    int wbase = 0;
+    for (int k = 0; k < wv; ++k) wbase += sh[k];
+    total = 0;
+    for (int k = 0; k < SCAN_T / 64; ++k) total += sh[k];
+    __syncthreads();
+    return wbase + x - v;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int* in, long long n, int* bsum) {
+    __shared__ int sh[SCAN_T / 64];
+    long long base = (long long)blockIdx.x * SCAN_B + threadIdx.x * SCAN_I;
+    int s = 0;
+    for (int k = 0; k < SCAN_I; ++k) if (base + k < n) s += in[base + k];
+    int total;
+    block_exclusive_scan(s, sh, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_bsums(int* bsum, int nb, int* out_total) {
+    __shared__ int sh[SCAN_T / 64];
+    int carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += SCAN_T) {
+        int i = c0 + threadIdx.x;
+        int v = i < nb ? bsum[i] : 0;
+        int total;
+        int ex = block_exclusive_scan(v, sh, total);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_apply(const int* in, long long n, const int* bsum,
+                                                      int* out) {
+    __shared__ int sh[SCAN_T / 64];
+    long long base = (long long)blockIdx.x * SCAN_B + threadIdx.x * SCAN_I;
+    int v[SCAN_I];
+    int s = 0;
+    for (int k = 0; k < SCAN_I; ++k) { v[k] = (base + k < n) ? in[base + k] : 0; s += v[k]; }
+    int total;
+    int ex = block_exclusive_scan(s, sh, total) + bsum[blockIdx.x];
+    for (int k = 0; k < SCAN_I; ++k) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+}
+
+static int exclusive_scan(const int* in, long long n, int* out, int* tmp) {
+    hipStream_t st = stream();
+    int nb = (int)((n + SCAN_B - 1) / SCAN_B);
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, st, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(SCAN_T), 0, st, tmp, nb, out + n);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_T), 0, st, in, n, tmp, out);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Partition kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double elem_coord(const Elems& E, long long i, int dim) {
+    return dim == 2 ? E.z[i] : dim == 1 ? E.y[i] : E.x[i];
+}
+
+__global__ void k_seg_split_flags(const SegOut* sout, int nseg, int* flags) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nseg) flags[s] = sout[s].split;
+}
+
+__global__ void k_part_flags(Elems E, const SegOut* sout, long long n, int* flags) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = E.seg[i];
+    int f = 0;
+    if (s >= 0) {
+        const SegOut& o = sout[s];
+        if (o.split && !o.fallback) f = elem_coord(E, i, o.dim) < o.value ? 1 : 0;
+    }
+    flags[i] = f;
+}
+
+__global__ void k_part_mid(const Seg* segs, SegOut* sout, int nseg, const int* pre) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    SegOut o = sout[s];
+    if (!o.split) return;
+    long long st = segs[s].start, en = segs[s].end;
+    if (!o.fallback) {
+        long long mid = st + (pre[en] - pre[st]);
+        if (mid == st || mid == en) o.fallback = 1;   // SplitData, src/Cell.cpp:533-550
+        else o.mid = mid;
+    }
+    if (o.fallback) o.mid = (st + en) / 2;
+    sout[s] = o;
+}
+
+// Serial std::nth_element for degenerate splits and split_method='median'.
+__global__ void k_part_median(const Seg* segs, const SegOut* sout, int nseg, Elems E,
+                              double* keybuf, int* ord, long long* dest) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const SegOut o = sout[s];
+    if (!o.split || !o.fallback) return;
+    long long st = segs[s].start, en = segs[s].end;
+    for (long long i = st; i < en; ++i) { keybuf[i] = elem_coord(E, i, o.dim); ord[i] = (int)i; }
+    tc_select::KeyLess cmp{keybuf};
+    tc_select::nth_element(ord, st, o.mid, en, cmp);
+    for (long long p = st; p < en; ++p) dest[ord[p]] = p;
+}
+
+// Rank the misplaced objects of each partitioned segment (std::partition swaps the
+// k-th !pred object from the left with the k-th pred object from the right).
+__global__ void k_part_rank(Elems E, const Seg* segs, const SegOut* sout, long long n,
+                            const int* flags, const int* pre, int* lpos, int* rpos) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = E.seg[i];
+    if (s < 0) return;
+    const SegOut& o = sout[s];
+    if (!o.split || o.fallback) return;
+    long long st = segs[s].start, en = segs[s].end, mid = o.mid;
+    int f = flags[i];
+    if (i < mid && !f) {
+        long long k = (i - st) - (pre[i] - pre[st]);
+        lpos[st + k] = (int)i;
+    } else if (i >= mid && f) {
+        long long k = pre[en] - pre[i + 1];
+        rpos[st + k] = (int)i;
+    }
+}
+
+__global__ void k_part_dest(Elems E, const Seg* segs, const SegOut* sout, const int* segrank,
+                            long long n, const int* flags, const int* pre, const int* lpos,
+                            const int* rpos, long long* dest, int* newseg) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = E.seg[i];
+    long long d = i;
+    int ns = -1;
+    if (s >= 0) {
+        const SegOut& o = sout[s];
+        if (o.split) {
+            long long st = segs[s].start, mid = o.mid;
+            if (o.fallback) {
+                d = dest[i];
+            } else {
+                int f = flags[i];
+                if (i < mid && !f) {
+                    long long k = (i - st) - (pre[i] - pre[st]);
+                    d = rpos[st + k];
+                } else if (i >= mid && f) {
+                    long long k = pre[segs[s].end] - pre[i + 1];
+                    d = lpos[st + k];
+                }
+            }
+            ns = 2 * segrank[s] + (d < mid ? 0 : 1);
+        }
+    }
+    dest[i] = d;
+    newseg[i] = ns;
+}
+
+__global__ void k_permute(Elems A, Elems B, long long n, const long long* dest, const int* newseg) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long d = dest[i];
+    B.x[d] = A.x[i]; B.y[d] = A.y[i]; B.z[d] = A.z[i]; B.wp[d] = A.wp[i];
+    B.w[d] = A.w[i]; B.a[d] = A.a[i]; B.b[d] = A.b[i];
+    B.seg[d] = newseg[i];
+}
+
+__global__ void k_children(const Seg* segs, const SegOut* sout, const int* segrank, int nseg,
+                           Seg* next) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const SegOut o = sout[s];
+    if (!o.split) return;
+    const Seg sg = segs[s];
+    int r = segrank[s];
+    Seg L, R;
+    L.start = sg.start; L.end = o.mid;
+    R.start = o.mid; R.end = sg.end;
+    L.mode = R.mode = o.child_mode;
+    L.mintop = R.mintop = sg.mintop - 1;
+    L.maxtop = R.maxtop = sg.maxtop - 1;
+    L.depth = R.depth = sg.depth + 1;
+    L.key = sg.key;
+    R.key = sg.key | (1ull << (62 - sg.depth));
+    L.pad = R.pad = 0;
+    if (o.child_mode == SEG_BUILD) { L.node = o.childL; R.node = o.childL + 1; }
+    else { L.node = R.node = -1; }
+    next[2 * r] = L;
+    next[2 * r + 1] = R;
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    template <class T> T* as() { return static_cast<T*>(p); }
+};
+
+static int dalloc(DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    TC_HIP(hipMalloc(&b.p, bytes));
+    return 0;
+}
+
+template <int D, int C>
+static int build_tree_t(const double* x, const double* y, const double* z, const double* d1,
+                        const double* d2, const double* w, const double* wpos, long long n,
+                        double minsize, double maxsize, int sm, int brute, int mintop,
+                        int maxtop, DeviceTree& T) {
+    hipStream_t st = stream();
+    const int TPB = 256;
+    auto grid = [&](long long m) { return dim3((unsigned)((m + TPB - 1) / TPB)); };
+
+    DevBuf bx[2], by[2], bz[2], bwp[2], bw[2], ba[2], bb[2], bseg[2];
+    for (int k = 0; k < 2; ++k) {
+        if (dalloc(bx[k], n * 8) || dalloc(by[k], n * 8) || dalloc(bz[k], n * 8) ||
+            dalloc(bwp[k], n * 8) || dalloc(bw[k], n * 4) || dalloc(ba[k], n * 4) ||
+            dalloc(bb[k], n * 4) || dalloc(bseg[k], n * 4))
+            return 2;
+    }
+    Elems E[2];
+    for (int k = 0; k < 2; ++k)
+        E[k] = Elems{bx[k].as<double>(), by[k].as<double>(), bz[k].as<double>(),
+                     bwp[k].as<double>(), bw[k].as<float>(), ba[k].as<float>(),
+                     bb[k].as<float>(), bseg[k].as<int>()};
+    DevBuf bflags, bpre, blpos, brpos, bdest, bnewseg, bord, bkey, bscan;
+    DevBuf bsegs[2], bsout, bsegflags, bsegrank, bcnt, btops, bfinfo;
+    long long nb_scan = (n + 1 + SCAN_B - 1) / SCAN_B + 1;
+    if (dalloc(bflags, n * 4) || dalloc(bpre, (n + 1) * 4) || dalloc(blpos, n * 4) ||
+        dalloc(brpos, n * 4) || dalloc(bdest, n * 8) || dalloc(bnewseg, n * 4) ||
+        dalloc(bord, n * 4) || dalloc(bkey, n * 8) || dalloc(bscan, nb_scan * 4) ||
+        dalloc(bsegs[0], n * sizeof(Seg)) || dalloc(bsegs[1], n * sizeof(Seg)) ||
+        dalloc(bsout, n * sizeof(SegOut)) || dalloc(bsegflags, n * 4) ||
+        dalloc(bsegrank, (n + 1) * 4) || dalloc(bcnt, 16) ||
+        dalloc(btops, sizeof(TopEntry) * (size_t)(std::min<long long>(n, 1LL << std::min(maxtop, 40)) + 1)) ||
+        dalloc(bfinfo, sizeof(FieldInfo)))
+        return 2;
+
+    // Nodes: at most 2n-1 (each split adds two).
+    long long maxnodes = 2 * n;
+    TC_HIP(hipMalloc(&T.nodes, sizeof(DNode) * maxnodes));
+
+    hipLaunchKernelGGL((k_init_elems<D, C>), grid(n), dim3(TPB), 0, st, E[0], x, y, z, d1, d2, w,
+                       wpos, n);
+    Seg root{};
+    root.start = 0; root.end = n; root.node = -1; root.mode = SEG_TOP;
+    root.mintop = mintop; root.maxtop = maxtop; root.key = 0; root.depth = 0;
+    TC_HIP(hipMemcpyAsync(bsegs[0].p, &root, sizeof(Seg), hipMemcpyHostToDevice, st));
+    TC_HIP(hipMemsetAsync(bcnt.p, 0, 16, st));
+
+    BuildParams P{minsize * minsize, maxsize * maxsize, sm, brute};
+    int cur = 0;
+    long long nseg = 1;
+    int level = 0;
+    int* hcnt = nullptr;
+    TC_HIP(hipHostMalloc((void**)&hcnt, 16));
+    int rc = 0;
+    while (nseg > 0) {
+        Seg* segs = bsegs[cur].as<Seg>();
+        SegOut* sout = bsout.as<SegOut>();
+        hipLaunchKernelGGL((k_segment<D, C>), dim3((unsigned)nseg), dim3(64), 0, st, segs, sout,
+                           E[cur], T.nodes, bcnt.as<int>(), btops.as<TopEntry>(), P,
+                           bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0);
+        hipLaunchKernelGGL(k_seg_split_flags, grid(nseg), dim3(TPB), 0, st, sout, (int)nseg,
+                           bsegflags.as<int>());
+        exclusive_scan(bsegflags.as<int>(), nseg, bsegrank.as<int>(), bscan.as<int>());
+        TC_HIP(hipMemcpyAsync(hcnt, bsegrank.as<int>() + nseg, sizeof(int),
+                              hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_part_flags, grid(n), dim3(TPB), 0, st, E[cur], sout, n,
+                           bflags.as<int>());
+        exclusive_scan(bflags.as<int>(), n, bpre.as<int>(), bscan.as<int>());
+        hipLaunchKernelGGL(k_part_mid, grid(nseg), dim3(TPB), 0, st, segs, sout, (int)nseg,
+                           bpre.as<int>());
+        hipLaunchKernelGGL(k_part_median, grid(nseg), dim3(TPB), 0, st, segs, sout, (int)nseg,
+                           E[cur], bkey.as<double>(), bord.as<int>(), bdest.as<long long>());
+        hipLaunchKernelGGL(k_part_rank, grid(n), dim3(TPB), 0, st, E[cur], segs, sout, n,
+                           bflags.as<int>(), bpre.as<int>(), blpos.as<int>(), brpos.as<int>());
+        hipLaunchKernelGGL(k_part_dest, grid(n), dim3(TPB), 0, st, E[cur], segs, sout,
+                           bsegrank.as<int>(), n, bflags.as<int>(), bpre.as<int>(),
+                           blpos.as<int>(), brpos.as<int>(), bdest.as<long long>(),
+                           bnewseg.as<int>());
+        hipLaunchKernelGGL(k_permute, grid(n), dim3(TPB), 0, st, E[cur], E[1 - cur], n,
+                           bdest.as<long long>(), bnewseg.as<int>());
+        hipLaunchKernelGGL(k_children, grid(nseg), dim3(TPB), 0, st, segs, sout,
+                           bsegrank.as<int>(), (int)nseg, bsegs[1 - cur].as<Seg>());
+        TC_HIP(hipGetLastError());
+        TC_HIP(hipStreamSynchronize(st));
+        nseg = 2LL * hcnt[0];
+        cur = 1 - cur;
+        ++level;
+        if (level > 4 * 64 + 64) { rc = fail("tree build did not terminate"); break; }
+    }
+    int cnts[2];
+    TC_HIP(hipMemcpyAsync(cnts, bcnt.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(&T.info, bfinfo.p, sizeof(FieldInfo), hipMemcpyDeviceToHost, st));
+    TC_HIP(hipStreamSynchronize(st));
+    (void)hipHostFree(hcnt);
+    if (rc) return rc;
+    T.nnodes = cnts[0];
+    std::vector<TopEntry> tops(cnts[1]);
+    TC_HIP(hipMemcpy(tops.data(), btops.p, sizeof(TopEntry) * cnts[1], hipMemcpyDeviceToHost));
+    std::sort(tops.begin(), tops.end(),
+              [](const TopEntry& a, const TopEntry& b) { return a.key < b.key; });
+    T.top.resize(tops.size());
+    for (size_t i = 0; i < tops.size(); ++i) T.top[i] = tops[i].node;
+    TC_HIP(hipMalloc(&T.dtop, sizeof(int) * std::max<size_t>(1, T.top.size())));
+    TC_HIP(hipMemcpy(T.dtop, T.top.data(), sizeof(int) * T.top.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int build_tree(int D, int C, const double* x, const double* y, const double* z, const double* d1,
+               const double* d2, const double* w, const double* wpos, long long n,
+               double minsize, double maxsize, int sm, int brute, int mintop, int maxtop,
+               DeviceTree& T) {
+#define TC_BT(DD, CC) return build_tree_t<DD, CC>(x, y, z, d1, d2, w, wpos, n, minsize, maxsize, sm, brute, mintop, maxtop, T)
+    if (D == NData) { if (C == Flat) TC_BT(NData, Flat); if (C == ThreeD) TC_BT(NData, ThreeD); TC_BT(NData, Sphere); }
+    if (D == KData) { if (C == Flat) TC_BT(KData, Flat); if (C == ThreeD) TC_BT(KData, ThreeD); TC_BT(KData, Sphere); }
+    if (C == Flat) TC_BT(GData, Flat); if (C == ThreeD) TC_BT(GData, ThreeD); TC_BT(GData, Sphere);
+#undef TC_BT
+}
+
+}  // namespace tc

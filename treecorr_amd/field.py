@@ -1,0 +1,141 @@
+"""Fields: ball trees built on the GPU (treecorr/field.py NField/KField/GField).
+
+A Field owns a tc_field handle of the C ABI.  Construction uploads the
+catalog columns and builds TreeCorr's mean-split tree on the device
+(src/Field.cpp:163-277, src/Cell.cpp:556-615); the tree stays resident in HBM.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+
+FLAT, THREED, SPHERE = 1, 2, 3
+_COORDS = {'flat': FLAT, '3d': THREED, 'spherical': SPHERE}
+_SPLIT = {'middle': 0, 'median': 1, 'mean': 2, 'random': 3}
+
+NODE_DTYPE = np.dtype([('x', 'f8'), ('y', 'f8'), ('z', 'f8'), ('size', 'f4'), ('w', 'f4'),
+                       ('a', 'f4'), ('b', 'f4'), ('child', 'i4'), ('n', 'i4')])
+
+
+def coord_enum(coords):
+    return _COORDS[coords]
+
+
+def default_min_top():
+    """Field._determine_top default, treecorr/field.py:94-104: max(3, ceil(log2(ncpu)))."""
+    from .util import get_omp_threads
+    n_cpu = get_omp_threads()
+    return max(3, int.bit_length(n_cpu - 1))
+
+
+class Field:
+    _d = None
+
+    def __init__(self, cat, *, min_size=0, max_size=None, split_method='mean', brute=False,
+                 min_top=None, max_top=10, coords=None, rng=None, logger=None):
+        self.ntot = cat.ntot
+        self.min_size = float(min_size) if not brute else 0.
+        self.max_size = float(max_size) if max_size is not None else np.inf
+        self.split_method = split_method
+        self._sm = _SPLIT[split_method]
+        self.brute = bool(brute)
+        if min_top is None:
+            min_top = default_min_top()
+        max_top = int(max_top)
+        self.min_top = min(int(min_top), max_top)
+        self.max_top = max_top
+        self.coords = coords if coords is not None else cat.coords
+        self._coords = coord_enum(self.coords)
+        x = np.ascontiguousarray(cat.x, dtype=float)
+        y = np.ascontiguousarray(cat.y, dtype=float)
+        z = None if cat.z is None else np.ascontiguousarray(cat.z, dtype=float)
+        w = np.ascontiguousarray(cat.w, dtype=float)
+        wpos = None if cat.wpos is None else np.ascontiguousarray(cat.wpos, dtype=float)
+        d1, d2 = self._payload(cat)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tc_field_create(
+            self._d, self._coords, _lib.dptr(x), _lib.dptr(y), _lib.dptr(z), _lib.dptr(d1),
+            _lib.dptr(d2), _lib.dptr(w), _lib.dptr(wpos), len(x), self.min_size, self.max_size,
+            self._sm, 0, int(self.brute), self.min_top, self.max_top, ctypes.byref(h)))
+        self.data = h
+
+    def _payload(self, cat):
+        return None, None
+
+    def __del__(self):
+        h = getattr(self, 'data', None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().tc_field_destroy(h)
+            except Exception:
+                pass
+            self.data = None
+
+    @property
+    def nTopLevelNodes(self):
+        n = ctypes.c_longlong()
+        _lib.check(_lib.lib().tc_field_ntop(self.data, ctypes.byref(n)))
+        return n.value
+
+    @property
+    def nobj(self):
+        n = ctypes.c_longlong()
+        _lib.check(_lib.lib().tc_field_nobj(self.data, ctypes.byref(n)))
+        return n.value
+
+    @property
+    def size(self):
+        s = ctypes.c_double()
+        _lib.check(_lib.lib().tc_field_size(self.data, ctypes.byref(s)))
+        return s.value
+
+    @property
+    def center(self):
+        c = np.zeros(3)
+        _lib.check(_lib.lib().tc_field_center(self.data, _lib.dptr(c)))
+        return c
+
+    def nodes(self):
+        """(nodes structured array, top-level node ids) copied from the device."""
+        nn = ctypes.c_longlong()
+        _lib.check(_lib.lib().tc_field_nnodes(self.data, ctypes.byref(nn)))
+        nodes = np.zeros(nn.value, dtype=NODE_DTYPE)
+        top = np.zeros(self.nTopLevelNodes, dtype=np.int32)
+        _lib.check(_lib.lib().tc_field_copy_nodes(self.data, nodes.ctypes.data_as(ctypes.c_void_p),
+                                                  top.ctypes.data_as(ctypes.c_void_p)))
+        return nodes, top
+
+
+class NField(Field):
+    _d = 0
+
+
+class KField(Field):
+    _d = 1
+
+    def _payload(self, cat):
+        return np.ascontiguousarray(cat.k, dtype=float), None
+
+
+class GField(Field):
+    _d = 4
+
+    def _payload(self, cat):
+        return (np.ascontiguousarray(cat.g1, dtype=float),
+                np.ascontiguousarray(cat.g2, dtype=float))
+
+
+def preorder(nodes, top):
+    """Flatten a device tree into TreeCorr's pre-order cell list (for comparisons)."""
+    order = []
+    stack = list(reversed([int(t) for t in top]))
+    while stack:
+        i = stack.pop()
+        order.append(i)
+        c = int(nodes['child'][i])
+        if c >= 0:
+            stack.append(c + 1)
+            stack.append(c)
+    return np.array(order, dtype=np.int64)
