@@ -1,0 +1,299 @@
+"""Benchmark of the treecorr_amd ball-tree engine on one MI355X.
+
+Default workload (one "step" = one full process() of the hot path, tree build
+included, with the catalog columns already resident in HBM):
+  GG shear-shear, spherical: N = 2,000,000 galaxies in a 10 deg x 10 deg cap,
+  70% uniform + 30% from 500 Gaussian clumps (sigma = 3 arcmin), g1,g2 ~ N(0, 0.2^2)
+  clipped to |g| < 1, w ~ U[0.5, 1.5]; 25 log bins 1-200 arcmin, bin_slop = 0.1.
+Metric: pairs/s = sum(npairs) / wall time of the step.
+
+Prints ONE JSON line (rank 0) with the roofline of the dominant kernel (the
+2-point traversal) measured with HIP events inside the timed region, and a CPU
+baseline: the oracle's OpenMP restatement of the reference engine on the host
+cores, timed on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ARCMIN = math.pi / 180. / 60.
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix) dense peak
+HBM_PEAK_GBS = 8000.        # MI355X HBM3E
+
+
+# ---------------------------------------------------------------------------
+# Synthetic workloads
+# ---------------------------------------------------------------------------
+def gg_sphere_catalog(n=2_000_000, seed=2026_12_03 + 2):
+    """70% uniform + 30% clumped galaxies in a 10x10 deg cap (ra, dec in radians)."""
+    rng = np.random.default_rng(seed)
+    deg = math.pi / 180.
+    ra_lo, ra_hi = 0., 10. * deg
+    sd_lo, sd_hi = math.sin(-5. * deg), math.sin(5. * deg)
+    nu = int(round(0.7 * n))
+    ra = np.empty(n)
+    dec = np.empty(n)
+    ra[:nu] = rng.uniform(ra_lo, ra_hi, nu)
+    dec[:nu] = np.arcsin(rng.uniform(sd_lo, sd_hi, nu))
+    ncl = 500
+    cra = rng.uniform(ra_lo, ra_hi, ncl)
+    cdec = np.arcsin(rng.uniform(sd_lo, sd_hi, ncl))
+    sig = 3. * ARCMIN
+    filled = nu
+    while filled < n:
+        m = n - filled
+        k = rng.integers(0, ncl, m)
+        ddec = rng.normal(0, sig, m)
+        dra = rng.normal(0, sig, m) / np.cos(cdec[k])
+        r, d = cra[k] + dra, cdec[k] + ddec
+        ok = (r >= ra_lo) & (r < ra_hi) & (np.sin(d) >= sd_lo) & (np.sin(d) < sd_hi)
+        r, d = r[ok], d[ok]
+        ra[filled:filled + len(r)] = r
+        dec[filled:filled + len(d)] = d
+        filled += len(r)
+    g1 = rng.normal(0, 0.2, n)
+    g2 = rng.normal(0, 0.2, n)
+    bad = g1 ** 2 + g2 ** 2 >= 1
+    while np.any(bad):
+        g1[bad] = rng.normal(0, 0.2, bad.sum())
+        g2[bad] = rng.normal(0, 0.2, bad.sum())
+        bad = g1 ** 2 + g2 ** 2 >= 1
+    w = rng.uniform(0.5, 1.5, n)
+    return ra, dec, g1, g2, w
+
+
+def radec_to_xyz(ra, dec):
+    """GenerateXYZ through the library (glibc sincos, src/KMeans.cpp:1013-1050)."""
+    from treecorr_amd import _lib
+    n = len(ra)
+    x, y, z = np.empty(n), np.empty(n), np.empty(n)
+    _lib.check(_lib.lib().tc_generate_xyz(_lib.dptr(x), _lib.dptr(y), _lib.dptr(z),
+                                          _lib.dptr(np.ascontiguousarray(ra)),
+                                          _lib.dptr(np.ascontiguousarray(dec)), None, n))
+    return x, y, z
+
+
+def gg_binning():
+    nbins, min_sep, max_sep, bin_slop, angle_slop = 25, 1. * ARCMIN, 200. * ARCMIN, 0.1, 0.1
+    bin_size = math.log(max_sep / min_sep) / nbins
+    b = bin_size * bin_slop
+    b1 = min(angle_slop, b)
+    return dict(nbins=nbins, min_sep=min_sep, max_sep=max_sep, bin_size=bin_size, b=b,
+                a=angle_slop, min_size=min_sep * b1 / (2 + 3 * b1), max_size=max_sep * b1,
+                min_top=4, max_top=10)
+
+
+# ---------------------------------------------------------------------------
+# Device memory (HIP runtime through ctypes)
+# ---------------------------------------------------------------------------
+class Hip:
+    def __init__(self):
+        for p in ('/opt/rocm/lib/libamdhip64.so', 'libamdhip64.so'):
+            try:
+                self.L = ctypes.CDLL(p)
+                break
+            except OSError:
+                continue
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_int]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=float)
+        p = ctypes.c_void_p()
+        assert self.L.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+        assert self.L.hipMemcpy(p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1) == 0
+        return p
+
+    def sync(self):
+        assert self.L.hipDeviceSynchronize() == 0
+
+    def free(self, p):
+        self.L.hipFree(p)
+
+
+# ---------------------------------------------------------------------------
+# One step: device tree build + 2-point traversal
+# ---------------------------------------------------------------------------
+def gg_step(L, dev, n, bn, out):
+    from treecorr_amd import _lib
+    fh = ctypes.c_void_p()
+    _lib.check(L.tc_field_create_device(4, 3, dev['x'], dev['y'], dev['z'], dev['g1'], dev['g2'],
+                                        dev['w'], None, n, bn['min_size'], bn['max_size'], 2, 0,
+                                        0, bn['min_top'], bn['max_top'], ctypes.byref(fh)))
+    ch = ctypes.c_void_p()
+    arr = [out[k] for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')]
+    for a in arr:
+        a[:] = 0
+    _lib.check(L.tc_corr2_create(4, 4, 0, bn['min_sep'], bn['max_sep'], bn['nbins'],
+                                 bn['bin_size'], bn['b'], bn['a'], -1.7976931348623157e308,
+                                 1.7976931348623157e308, 0., 0., 0.,
+                                 *[_lib.dptr(a) for a in arr], ctypes.byref(ch)))
+    _lib.check(L.tc_corr2_process_auto(ch, fh, 0, 0, 1))
+    c = (ctypes.c_longlong * 4)()
+    _lib.check(L.tc_corr2_counters(ch, c))
+    L.tc_corr2_destroy(ch)
+    L.tc_field_destroy(fh)
+    return dict(visits=c[0], accepted=c[1], leafpairs=c[2], refined=c[3])
+
+
+# Algorithmic FP64 operations per unit of the 2-point path (GG, spherical):
+# a visited node pair ~25 (distance, range tests, split rule); an accepted GG
+# sphere pair 98 (bin index with sqrt/log, weights, two great-circle projections,
+# both_complex_prod); a refined single-bin test +25.  Bytes: 2 node records of
+# 48 B per visit.
+OPS_VISIT, OPS_ACCEPT_GG_SPHERE, OPS_REFINED, BYTES_VISIT = 25, 98, 25, 96
+
+
+def cpu_baseline(x, y, z, g1, g2, w, bn, gpu_counters, gpu_npairs, target_s=15.):
+    """The oracle's restatement of the reference engine (OpenMP, all host cores) on a
+    bounded sample: the full tree build plus every k-th top-level row of the traversal;
+    the full-job time is extrapolated by the ratio of node-pair visits."""
+    from oracle import oracle as O
+    if not os.path.exists(os.path.join(ROOT, 'oracle', 'libtc_oracle.so')):
+        return None
+    nthreads = int(os.environ.get('OMP_NUM_THREADS', '0')) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    f = O.Field(O.GDATA, O.SPHERE, x, y, z, g1=g1, g2=g2, w=w, min_size=bn['min_size'],
+                max_size=bn['max_size'], min_top=bn['min_top'], max_top=bn['max_top'])
+    t_build = time.perf_counter() - t0
+    stride = 64
+    while True:
+        t1 = time.perf_counter()
+        r = O.corr2(O.GDATA, O.SPHERE, f, None, bn['nbins'], bn['min_sep'], bn['max_sep'],
+                    bn['bin_size'], bn['b'], bn['a'], exact=False, nthreads=nthreads,
+                    row_stride=stride)
+        t_samp = time.perf_counter() - t1
+        if t_samp > 2. or stride == 1:
+            break
+        stride = max(1, stride // 4)
+    frac = r['counters']['visits'] / max(1, gpu_counters['visits'])
+    t_full = t_build + t_samp / max(frac, 1e-12)
+    return {"value": float(gpu_npairs / t_full), "unit": "pairs/s", "cores": nthreads,
+            "kind": "port",
+            "sample": (f"oracle (OpenMP restatement of TreeCorr's engine, -O2) on the same "
+                       f"2M-galaxy catalog: full tree build {t_build:.2f} s + every {stride}th "
+                       f"top-level row of the traversal ({t_samp:.2f} s, "
+                       f"{100 * frac:.2f}% of the node-pair visits); full-job time "
+                       f"extrapolated by visit count = {t_full:.1f} s")}
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
+    path = os.path.join(ROOT, 'profiles', name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith('traffic_bytes_per_launch:'):
+                return float(line.split(':', 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--n', type=int, default=2_000_000)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                    help='only run the plain copy kernel over MIB MiB (HBM counter calibration)')
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        from treecorr_amd import _lib
+        hip = Hip()
+        n = args.calibrate_copy * 1024 * 1024 // 8
+        src = hip.upload(np.ones(n))
+        dst = hip.upload(np.zeros(n))
+        _lib.check(_lib.lib().tc_copy_calibration(src, dst, n))
+        print(json.dumps({"calibration_copy_bytes_read": n * 8, "bytes_written": n * 8}))
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "treecorr_amd runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is described, not built)"}))
+        sys.exit(2)
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    hip = Hip()
+    ra, dec, g1, g2, w = gg_sphere_catalog(args.n)
+    x, y, z = radec_to_xyz(ra, dec)
+    bn = gg_binning()
+    dev = {k: hip.upload(v) for k, v in dict(x=x, y=y, z=z, g1=g1, g2=g2, w=w).items()}
+    nb = bn['nbins']
+    out = {k: np.zeros(nb) for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight',
+                                     'npairs')}
+    for i in range(args.warmup):
+        cnt = gg_step(L, dev, args.n, bn, out)
+        print(f"# warmup {i} done: npairs={out['npairs'].sum():.4g}", file=sys.stderr, flush=True)
+    L.tc_reset_kernel_stats()
+    L.tc_set_timing(1)
+    hip.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        cnt = gg_step(L, dev, args.n, bn, out)
+        print(f"# step {i} done", file=sys.stderr, flush=True)
+    hip.sync()
+    t1 = time.perf_counter()
+    L.tc_set_timing(0)
+    ms = ctypes.c_double()
+    launches = ctypes.c_longlong()
+    _lib.check(L.tc_kernel_stats(0, ctypes.byref(ms), ctypes.byref(launches)))
+    npairs = float(out['npairs'].sum())
+    ms_per_step = 1e3 * (t1 - t0) / args.steps
+    value = npairs / ((t1 - t0) / args.steps)
+    flops = (OPS_VISIT * cnt['visits'] + OPS_ACCEPT_GG_SPHERE * cnt['accepted'] +
+             OPS_REFINED * cnt['refined'])
+    kern_s = ms.value * 1e-3 / args.steps            # traversal kernel seconds per step
+    achieved = flops / kern_s / 1e12 if kern_s > 0 else 0.
+    launches_per_step = launches.value / args.steps
+    traffic = read_traffic('k_traverse_gg_sphere_2M.txt')
+    res = {
+        "metric": "pairs/s (GG xi+/xi- 2M galaxies, bin_slop=0.1)",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded; 70% uniform + 30% clumped, spherical cap)",
+        "config": {"workload": "GG shear-shear, 2,000,000 galaxies, 10x10 deg cap, 25 log bins "
+                               "1-200 arcmin, bin_slop=0.1, tree build included",
+                   "n": args.n, "nbins": nb, "bin_slop": 0.1, "min_top": bn['min_top'],
+                   "max_top": bn['max_top']},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": traffic,
+                     "kernel": "k_traverse<GData,Sphere>",
+                     "flops_per_launch": flops / max(1., launches_per_step),
+                     "avg_launch_ms": ms.value / max(1, launches.value),
+                     "launches_per_step": launches_per_step,
+                     "hbm_view_GBs": BYTES_VISIT * cnt['visits'] / kern_s / 1e9 if kern_s else 0.,
+                     "note": "FP64 peak 78.6 TF/s is the same for the vector ALUs this kernel "
+                             "uses and the matrix cores"},
+        "counters": cnt,
+        "traversal_ms_per_step": ms.value / args.steps,
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(x, y, z, g1, g2, w, bn, cnt, npairs)
+    for v in dev.values():
+        hip.free(v)
+    print(json.dumps(res))
+
+
+if __name__ == '__main__':
+    main()

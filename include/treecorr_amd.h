@@ -101,6 +101,28 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
 int tc_corr2_counters(const tc_corr2* c, long long* counters4);
 int tc_corr2_destroy(tc_corr2* c);
 
+/* ---- 3-point multipole: replace {NNN,GGG}Corr(...) + processAuto ----
+ * Reference: BuildCorr3 src/Corr3.cpp:3599-3645 bound at :4205-4226 (the
+ * LogMultipole subset: nubins = max_n, u/v bins unused); ProcessAuto
+ * :3710-3720 -> BaseCorr3::multipole :1124-1175.  d in {0 (NNN), 4 (GGG)};
+ * bin_type must be LogMultipole (5).  Every array has nbins*nbins*(2*max_n+1)
+ * entries in TreeCorr's layout ((k2*nbins + k3)*(2*max_n+1) + max_n + n);
+ * zeta0..7 are gam0r,gam0i,..,gam3i for GGG and may be NULL for NNN;
+ * meand1, meanlogd1, meanu, meanv are accepted and left untouched, as in the
+ * reference's multipole path. */
+int tc_corr3_create(int d, int bin_type, double minsep, double maxsep, int nbins,
+                    double binsize, double b, double a, int max_n, double* zeta0, double* zeta1,
+                    double* zeta2, double* zeta3, double* zeta4, double* zeta5, double* zeta6,
+                    double* zeta7, double* meand1, double* meanlogd1, double* meand2,
+                    double* meanlogd2, double* meand3, double* meanlogd3, double* meanu,
+                    double* meanv, double* weight, double* weight_im, double* ntri,
+                    tc_corr3** out);
+int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int metric);
+/* Work counters of the last process call: Gn calls, zeta calls, zeta rows,
+ * multipoleFinish stages (no reference counterpart; roofline accounting). */
+int tc_corr3_counters(const tc_corr3* c, long long* counters4);
+int tc_corr3_destroy(tc_corr3* c);
+
 /* ---- Host utilities ---- */
 /* GenerateXYZ, src/KMeans.cpp:1013-1050 (glibc sincos; r may be NULL). */
 int tc_generate_xyz(double* x, double* y, double* z, const double* ra, const double* dec,
@@ -114,6 +136,9 @@ int tc_get_num_threads(int* n);
  * no reference counterpart).  which: 0 = 2-pt traversal, 1 = tree build,
  * 2 = multipole Gn, 3 = zeta contraction. */
 int tc_set_timing(int on);
+/* Streams n doubles from src to dst (device pointers) with a plain copy kernel;
+ * used only to calibrate HBM traffic counters (FETCH_SIZE/WRITE_SIZE). */
+int tc_copy_calibration(const double* src, double* dst, long long n);
 int tc_kernel_stats(int which, double* total_ms, long long* launches);
 int tc_reset_kernel_stats(void);
 

@@ -50,6 +50,7 @@ def lib():
         L.tco_corr2.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [
             ctypes.c_double] * 5 + [ctypes.c_int] * 3 + [_dp] * 8 + [
             ctypes.POINTER(ctypes.c_longlong)]
+        L.tco_corr2_sampled.argtypes = L.tco_corr2.argtypes + [ctypes.c_int]
         L.tco_corr2_brute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long] + [_dp] * 6 + [
             ctypes.c_long] + [_dp] * 6 + [ctypes.c_int] + [ctypes.c_double] * 3 + [
             ctypes.c_int, ctypes.c_int] + [_dp] * 8
@@ -112,16 +113,19 @@ class Field:
 
 
 def corr2(d, coords, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, exact=True,
-          nthreads=0):
-    """Run the oracle's 2-point traversal; returns dict of arrays and counters."""
+          nthreads=0, row_stride=1):
+    """Run the oracle's 2-point traversal; returns dict of arrays and counters.
+
+    row_stride > 1 processes only every row_stride-th top-level row (timing samples)."""
     out = {k: np.zeros(nbins) for k in
            ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')}
     cnt = (ctypes.c_longlong * 4)()
-    rc = lib().tco_corr2(f1.h, f2.h if f2 is not None else None, d, coords, nbins,
-                         float(min_sep), float(max_sep), float(bin_size), float(b), float(a),
-                         int(quick), int(exact), int(nthreads),
-                         *[_ptr(out[k]) for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr',
-                                                  'meanlogr', 'weight', 'npairs')], cnt)
+    rc = lib().tco_corr2_sampled(f1.h, f2.h if f2 is not None else None, d, coords, nbins,
+                                 float(min_sep), float(max_sep), float(bin_size), float(b),
+                                 float(a), int(quick), int(exact), int(nthreads),
+                                 *[_ptr(out[k]) for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr',
+                                                          'meanlogr', 'weight', 'npairs')], cnt,
+                                 int(row_stride))
     if rc != 0:
         raise RuntimeError('oracle corr2 failed')
     out['counters'] = dict(visits=cnt[0], accepted=cnt[1], leafpairs=cnt[2], refined=cnt[3])

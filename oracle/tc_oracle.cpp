@@ -431,7 +431,7 @@ struct Corr2Engine {
 
 template <int D, int C>
 static void run_corr2(const Params2& P, const Tree& t1, const Tree* t2, bool quick, bool exact,
-                      int nthreads, double* out[8], long long* counters) {
+                      int nthreads, double* out[8], long long* counters, int row_stride) {
     Corr2Engine<D, C> eng(P, t1, t2 ? *t2 : t1, quick);
     const long n1 = long(t1.top.size());
     std::vector<std::unique_ptr<Acc2>> accs;
@@ -455,6 +455,7 @@ static void run_corr2(const Params2& P, const Tree& t1, const Tree* t2, bool qui
 #ifdef _OPENMP
                 tid = omp_get_thread_num();
 #endif
+                if (i % row_stride != 0) continue;
                 for (long j = 0; j < n2; ++j) eng.process11(t1, t1.top[i], *t2, t2->top[j], *accs[tid]);
             }
         }
@@ -466,6 +467,7 @@ static void run_corr2(const Params2& P, const Tree& t1, const Tree* t2, bool qui
 #ifdef _OPENMP
             tid = omp_get_thread_num();
 #endif
+            if (i % row_stride != 0) continue;
             eng.process2(t1, t1.top[i], *accs[tid]);
             for (long j = i + 1; j < n1; ++j) eng.process11(t1, t1.top[i], t1, t1.top[j], *accs[tid]);
         }
@@ -530,11 +532,13 @@ void tco_field_dump(void* h, double* pos, float* size, float* w, float* wk, floa
 // 2-point correlation over one tree (auto, f2 == NULL) or two trees (cross).
 // out arrays (nbins each, NULL allowed): xi0..xi3, meanr, meanlogr, weight, npairs.
 // Values are ADDED to the arrays.  counters[4]: node-pair visits, accepted pairs,
-// accepted leaf-leaf pairs, refined singleBin evaluations.
-int tco_corr2(void* f1, void* f2, int d, int coords, int nbins, double minsep, double maxsep,
-              double binsize, double b, double a, int quick, int exact, int nthreads,
-              double* xi0, double* xi1, double* xi2, double* xi3, double* meanr,
-              double* meanlogr, double* weight, double* npairs, long long* counters) {
+// accepted leaf-leaf pairs, refined singleBin evaluations.  row_stride > 1 keeps
+// only the top-level rows i with i % row_stride == 0 (a bounded timing sample).
+int tco_corr2_sampled(void* f1, void* f2, int d, int coords, int nbins, double minsep,
+                      double maxsep, double binsize, double b, double a, int quick, int exact,
+                      int nthreads, double* xi0, double* xi1, double* xi2, double* xi3,
+                      double* meanr, double* meanlogr, double* weight, double* npairs,
+                      long long* counters, int row_stride) {
     Params2 P;
     P.nbins = nbins; P.minsep = minsep; P.maxsep = maxsep; P.binsize = binsize; P.b = b; P.a = a;
     // BaseCorr2::BaseCorr2, src/Corr2.cpp:51-77
@@ -544,7 +548,8 @@ int tco_corr2(void* f1, void* f2, int d, int coords, int nbins, double minsep, d
     double* out[8] = {xi0, xi1, xi2, xi3, meanr, meanlogr, weight, npairs};
     Tree* t1 = static_cast<Tree*>(f1);
     Tree* t2 = static_cast<Tree*>(f2);
-#define TCO_RUN(DD, CC) run_corr2<DD, CC>(P, *t1, t2, quick != 0, exact != 0, nthreads, out, counters)
+    if (row_stride < 1) row_stride = 1;
+#define TCO_RUN(DD, CC) run_corr2<DD, CC>(P, *t1, t2, quick != 0, exact != 0, nthreads, out, counters, row_stride)
     if (d == NData) {
         if (coords == Flat) TCO_RUN(NData, Flat); else if (coords == ThreeD) TCO_RUN(NData, ThreeD); else TCO_RUN(NData, Sphere);
     } else if (d == KData) {
@@ -556,6 +561,15 @@ int tco_corr2(void* f1, void* f2, int d, int coords, int nbins, double minsep, d
     }
 #undef TCO_RUN
     return 0;
+}
+
+int tco_corr2(void* f1, void* f2, int d, int coords, int nbins, double minsep, double maxsep,
+              double binsize, double b, double a, int quick, int exact, int nthreads,
+              double* xi0, double* xi1, double* xi2, double* xi3, double* meanr,
+              double* meanlogr, double* weight, double* npairs, long long* counters) {
+    return tco_corr2_sampled(f1, f2, d, coords, nbins, minsep, maxsep, binsize, b, a, quick,
+                             exact, nthreads, xi0, xi1, xi2, xi3, meanr, meanlogr, weight,
+                             npairs, counters, 1);
 }
 
 // Brute-force O(N1*N2) pair loop over the objects of two catalogs (auto when

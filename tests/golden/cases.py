@@ -110,3 +110,28 @@ def tree_case(case):
     cat, scale = make_catalog(coords, n, seed, 0.2)
     return cat, dict(min_size=scale * fmin, max_size=scale * fmax if fmax else np.inf,
                      split_method=sm, brute=brute, min_top=min_top, max_top=10)
+
+# Multipole (LogMultipole) cases: (name, d, coords, n, seed, nbins, lo, hi, max_n, bin_slop,
+#                                  min_top, clumps)
+CORR3_CASES = [
+    ('nnn_flat_mp', N_, FLAT, 2500, 31, 8, 0.02, 0.3, 8, None, 3, 0.2),
+    ('ggg_flat_mp', G_, FLAT, 2500, 32, 8, 0.02, 0.3, 10, None, 3, 0.2),
+    ('ggg_sphere_mp', G_, SPHERE, 2500, 33, 6, 0.02, 0.3, 12, None, 3, 0.2),
+    ('ggg_3d_mp', G_, THREED, 2000, 34, 6, 0.02, 0.4, 6, 0.5, 3, 0.2),
+]
+
+
+def corr3_case(case):
+    (name, d, coords, n, seed, nbins, lo, hi, max_n, bs, min_top, clumps) = case
+    cat, scale = make_catalog(coords, n, seed, clumps)
+    min_sep, max_sep = scale * lo, scale * hi
+    bin_size = math.log(max_sep / min_sep) / nbins
+    a = math.pi / (2 * max_n + 1)
+    if bs is None:
+        b = bin_size if bin_size <= 0.1 else 0.1
+    else:
+        b = bin_size * bs
+    b1 = min(a, b)
+    bn = dict(nbins=nbins, min_sep=min_sep, max_sep=max_sep, bin_size=bin_size, b=b, a=a,
+              max_n=max_n, min_size=2 * min_sep * b1 / (2 + 3 * b1), max_size=max_sep * b1)
+    return cat, bn, dict(min_top=min_top)

@@ -41,6 +41,21 @@ def main():
                     ntop1=f1.getNTopLevel(), ntop2=f2.getNTopLevel() if f2 else None)
         np.savez_compressed(os.path.join(HERE, name + '.npz'), spec=json.dumps(spec), **out)
         print(name, 'sum npairs', out['npairs'].sum())
+    for case in C.CORR3_CASES:
+        name, d, coords = case[0], case[1], case[2]
+        cat, bn, extra = C.corr3_case(case)
+        f = R.make_field(d, coords, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
+                         w=cat['w'], min_size=bn['min_size'], max_size=bn['max_size'],
+                         min_top=extra['min_top'])
+        out = R.corr3_multipole(d, f, bn['nbins'], bn['min_sep'], bn['max_sep'],
+                                bn['bin_size'], bn['b'], bn['a'], bn['max_n'])
+        keep = ['weight', 'weight_im', 'ntri', 'meand2', 'meanlogd2', 'meand3', 'meanlogd3']
+        if d == 4:
+            keep += ['zeta%d' % i for i in range(8)]
+        spec = dict(case=list(case), checksum=C.checksum(cat), ntop=f.getNTopLevel())
+        np.savez_compressed(os.path.join(HERE, name + '.npz'), spec=json.dumps(spec),
+                            **{k: out[k] for k in keep})
+        print(name, 'sum ntri', out['ntri'][:, :, bn['max_n']].sum())
     for case in C.TREE_CASES:
         name, d, coords = case[0], case[1], case[2]
         cat, kw = C.tree_case(case)

@@ -1,0 +1,86 @@
+"""The C-ABI library builds, loads and exports every entry point include/treecorr_amd.h
+declares; host-side validation works without a GPU."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, 'include', 'treecorr_amd.h')).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(tc_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    names = declared()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # every declared entry point has a ctypes signature in the binding
+    assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
+
+
+def test_version_and_errors_without_gpu():
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    assert L.tc_version() >= 100
+    h = ctypes.c_void_p()
+    x = np.zeros(3)
+    # invalid data type is rejected before any device work
+    rc = L.tc_field_create(7, 1, _lib.dptr(x), _lib.dptr(x), None, None, None, _lib.dptr(x), None,
+                           3, 0., 1., 2, 0, 0, 3, 10, ctypes.byref(h))
+    assert rc != 0
+    assert len(L.tc_last_error()) > 0
+
+
+def test_unsupported_corr2_reports_not_implemented():
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    a = np.zeros(5)
+    rc = L.tc_corr2_create(0, 1, 0, 1., 10., 5, math.log(10) / 5, 0.1, 0.1, 0., 0., 0., 0., 0.,
+                           None, None, None, None, *[_lib.dptr(a)] * 4, ctypes.byref(h))
+    assert rc == _lib.TC_ERR_UNSUPPORTED
+
+
+def test_generate_xyz_matches_sincos():
+    """GenerateXYZ (src/KMeans.cpp:1013-1050): x = cos(dec)cos(ra), y = cos(dec)sin(ra), z = sin(dec)."""
+    from treecorr_amd import _lib
+    rng = np.random.default_rng(1)
+    ra = rng.uniform(0, 2 * np.pi, 1000)
+    dec = rng.uniform(-1.5, 1.5, 1000)
+    x, y, z = np.empty(1000), np.empty(1000), np.empty(1000)
+    _lib.check(_lib.lib().tc_generate_xyz(_lib.dptr(x), _lib.dptr(y), _lib.dptr(z), _lib.dptr(ra),
+                                          _lib.dptr(dec), None, 1000))
+    np.testing.assert_allclose(x, np.cos(dec) * np.cos(ra), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(z, np.sin(dec), rtol=0, atol=1e-15)
+
+
+def test_binning_setup_matches_treecorr_rules():
+    """corr2base.py:420-449 and _get_minmax_size :1595-1625."""
+    import treecorr_amd as tca
+    gg = tca.GGCorrelation(min_sep=1., max_sep=200., nbins=25, sep_units='arcmin', bin_slop=0.1)
+    assert gg.bin_size == pytest.approx(math.log(200.) / 25, rel=1e-15)
+    assert gg.b == pytest.approx(0.1 * gg.bin_size, rel=1e-15)
+    assert gg.angle_slop == 0.1
+    mn, mx = gg._get_minmax_size()
+    b1 = min(0.1, gg.b)
+    assert mn == pytest.approx(gg._min_sep * b1 / (2 + 3 * b1), rel=1e-15)
+    assert mx == pytest.approx(gg._max_sep * b1, rel=1e-15)
+    nn = tca.NNCorrelation(min_sep=1., max_sep=100., bin_size=0.5)
+    assert nn.nbins == int(math.ceil(math.log(100.) / 0.5))
+    assert nn.angle_slop == 1
+    # default bin_slop: min(0.1/bin_size, 1)
+    assert nn.bin_slop == pytest.approx(min(0.1 / nn.bin_size, 1.0))
+    with pytest.raises(TypeError):
+        tca.NNCorrelation(min_sep=1., bin_size=0.1)
+    with pytest.raises(ValueError):
+        tca.NNCorrelation(min_sep=10., max_sep=1., nbins=3)

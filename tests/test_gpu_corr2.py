@@ -22,24 +22,14 @@ def normwise(a, b):
 
 
 def raw_corr2(tca, d, coords, cat1, cat2, bn, brute=False, min_top=3, quick=False):
-    """Run the GPU engine through the public classes but read the raw (unfinalized) sums."""
-    def mk(cat):
-        return tca.Catalog(x=cat['x'], y=cat['y'], z=cat['z'], w=cat['w'], k=cat['k'],
-                           g1=cat['g1'], g2=cat['g2'])
-    cls = {0: tca.NNCorrelation, 1: tca.KKCorrelation, 4: tca.GGCorrelation}[d]
-    corr = cls(nbins=bn['nbins'], min_sep=bn['min_sep'], max_sep=bn['max_sep'],
-               bin_slop=bn['b'] / bn['bin_size'], angle_slop=bn['a'], brute=brute,
-               min_top=min_top)
-    c1 = mk(cat1)
-    if cat2 is None:
-        corr.process_auto(c1, corr_only=quick)
-    else:
-        corr.process_cross(c1, mk(cat2), corr_only=quick)
-    out = dict(meanr=corr.meanr, meanlogr=corr.meanlogr, weight=corr.weight, npairs=corr.npairs)
-    for i in range(4):
-        out['xi%d' % i] = corr._xi[i] if corr._xi[i] is not None else np.array([])
-    out['counters'] = corr.counters
-    return out
+    """Run the GPU engine through the C ABI and return the raw (unfinalized) sums."""
+    import capi
+    kw = dict(min_size=0. if brute else bn['min_size'], max_size=bn['max_size'], brute=brute,
+              min_top=min_top)
+    f1 = capi.Field(d, coords, cat1, **kw)
+    f2 = capi.Field(d, coords, cat2, **kw) if cat2 is not None else None
+    return capi.corr2(d, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                      bn['b'], bn['a'], quick=quick)
 
 
 @pytest.mark.gpu

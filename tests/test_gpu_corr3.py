@@ -1,0 +1,84 @@
+"""GPU multipole (NNN, GGG) vs the literal reference's golden vectors and the oracle."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cases as C
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+RTOL = 1e-12   # normwise relative tolerance for the FP64 sums; ntri is exact
+
+
+def normwise(a, b):
+    den = np.max(np.abs(b))
+    return 0. if den == 0 else np.max(np.abs(a - b)) / den
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('case', C.CORR3_CASES, ids=[c[0] for c in C.CORR3_CASES])
+def test_gpu_multipole_matches_reference_golden(gpu, case):
+    import capi
+    z = np.load(os.path.join(GOLD, case[0] + '.npz'), allow_pickle=False)
+    spec = json.loads(str(z['spec']))
+    gold = {k: z[k] for k in z.files if k != 'spec'}
+    cat, bn, extra = C.corr3_case(case)
+    assert C.checksum(cat) == spec['checksum']
+    d, coords = case[1], case[2]
+    f = capi.Field(d, coords, cat, min_size=bn['min_size'], max_size=bn['max_size'],
+                   min_top=extra['min_top'])
+    assert f.ntop == spec['ntop']
+    out = capi.corr3(d, f, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'],
+                     bn['a'], bn['max_n'])
+    np.testing.assert_array_equal(out['ntri'], gold['ntri'])
+    for k, v in gold.items():
+        if k != 'ntri':
+            assert normwise(out[k], v) < RTOL, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d,coords,n,max_n', [(4, 3, 40000, 30), (0, 1, 40000, 20),
+                                              (4, 1, 30000, 16)])
+def test_gpu_multipole_matches_oracle(gpu, oracle_lib, d, coords, n, max_n):
+    import capi
+    O = oracle_lib
+    cat, scale = C.make_catalog(coords, n, 60 + d + coords, 0.3)
+    bn = capi.multipole_binning(scale, 12, 0.01, 0.2, max_n)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    f = capi.Field(d, coords, cat, **kw)
+    out = capi.corr3(d, f, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'],
+                     bn['a'], bn['max_n'])
+    of = O.Field(d, coords, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'], w=cat['w'],
+                 **kw)
+    ref = O.corr3_multipole(d, coords, of, bn['nbins'], bn['min_sep'], bn['max_sep'],
+                            bn['bin_size'], bn['b'], bn['a'], bn['max_n'])
+    np.testing.assert_array_equal(out['ntri'], ref['ntri'])
+    keys = ['weight', 'weight_im', 'meand2', 'meanlogd2', 'meand3', 'meanlogd3']
+    if d == 4:
+        keys += ['zeta%d' % i for i in range(8)]
+    for k in keys:
+        assert normwise(out[k], ref[k]) < RTOL, k
+    for k in ('gn_calls', 'zeta_calls', 'zeta_rows', 'stages'):
+        assert out['counters'][k] == ref['counters'][k], k
+
+
+@pytest.mark.gpu
+def test_gpu_ggg_public_api(gpu):
+    """GGGCorrelation(bin_type='LogMultipole').process(cat): finalized arrays."""
+    rng = np.random.default_rng(12)
+    n = 20000
+    cat = gpu.Catalog(x=rng.uniform(0, 300, n), y=rng.uniform(0, 300, n), x_units='arcmin',
+                      y_units='arcmin', g1=rng.normal(0, 0.2, n), g2=rng.normal(0, 0.2, n),
+                      w=rng.uniform(0.5, 1.5, n))
+    ggg = gpu.GGGCorrelation(min_sep=5., max_sep=100., nbins=10, max_n=10, sep_units='arcmin',
+                             bin_type='LogMultipole', min_top=4)
+    ggg.process(cat)
+    assert ggg.gam0.shape == (10, 10, 21)
+    # ntri/meand broadcast from the n = max_n slice (corr3base.py:2386-2396)
+    assert np.all(ggg.ntri[:, :, 0] == ggg.ntri[:, :, 10])
+    good = ggg.weightr[:, :, 10] > 0
+    assert np.all(ggg.meand2[:, :, 10][good] > 4.) and np.all(ggg.meand2[:, :, 10][good] < 110.)
+    # W_{-n} = conj(W_n) symmetry of the weight
+    np.testing.assert_allclose(ggg.weightr[:, :, ::-1], ggg.weightr.transpose(1, 0, 2), rtol=1e-12,
+                               atol=1e-9 * np.max(np.abs(ggg.weightr)))

@@ -13,14 +13,19 @@ SNAME = {0: 'middle', 1: 'median', 2: 'mean'}
 
 
 def device_tree(tca, d, coords, cat, kw):
-    c = tca.Catalog(x=cat['x'], y=cat['y'], z=cat['z'], w=cat['w'], k=cat['k'], g1=cat['g1'],
-                    g2=cat['g2'])
-    cls = {0: tca.NField, 1: tca.KField, 4: tca.GField}[d]
-    f = cls(c, min_size=kw['min_size'], max_size=kw['max_size'],
-            split_method=SNAME[kw['split_method']], brute=kw['brute'], min_top=kw['min_top'],
-            max_top=kw['max_top'], coords=CNAME[coords])
-    from treecorr_amd.field import preorder
-    nodes, top = f.nodes()
+    """Build a device tree through the C ABI and return it in TreeCorr's pre-order."""
+    import ctypes
+    import capi
+    from treecorr_amd import _lib
+    from treecorr_amd.field import NODE_DTYPE, preorder
+    f = capi.Field(d, coords, cat, **kw)
+    L = _lib.lib()
+    nn = ctypes.c_longlong()
+    _lib.check(L.tc_field_nnodes(f.h, ctypes.byref(nn)))
+    nodes = np.zeros(nn.value, dtype=NODE_DTYPE)
+    top = np.zeros(f.ntop, dtype=np.int32)
+    _lib.check(L.tc_field_copy_nodes(f.h, nodes.ctypes.data_as(ctypes.c_void_p),
+                                     top.ctypes.data_as(ctypes.c_void_p)))
     order = preorder(nodes, top)
     return f, nodes[order], len(top)
 
@@ -71,10 +76,13 @@ def test_device_tree_matches_oracle_large(gpu, oracle_lib, d, coords, n):
         np.testing.assert_array_equal(nd['a'], ref['g'][:, 0])
         np.testing.assert_array_equal(nd['b'], ref['g'][:, 1])
     # Field center and size (Field::_center/_sizesq)
-    c = f.center
+    import ctypes
+    from treecorr_amd import _lib
     ofield = O.Field(d, coords, cat['x'], cat['y'], cat['z'], k=cat['k'], g1=cat['g1'],
                      g2=cat['g2'], w=cat['w'], **kw)
-    assert f.size == np.sqrt(ofield.sizesq)
+    size = ctypes.c_double()
+    _lib.check(_lib.lib().tc_field_size(f.h, ctypes.byref(size)))
+    assert size.value == np.sqrt(ofield.sizesq)
 
 
 @pytest.mark.gpu

@@ -9,9 +9,6 @@ from .field import Field, NField, KField, GField
 from .corr2 import Corr2, NNCorrelation, KKCorrelation, GGCorrelation
 from .util import set_omp_threads, get_omp_threads
 
-__version__ = '0.1.0'
+from .corr3 import Corr3, NNNCorrelation, GGGCorrelation
 
-try:
-    from .corr3 import Corr3, NNNCorrelation, GGGCorrelation
-except ImportError:   # pragma: no cover
-    pass
+__version__ = '0.1.0'

@@ -47,12 +47,12 @@ SIGNATURES = {
     'tc_set_num_threads': (_i, [_i]),
     'tc_get_num_threads': (_i, [ctypes.POINTER(_i)]),
     'tc_set_timing': (_i, [_i]),
+    'tc_copy_calibration': (_i, [_vp, _vp, _ll]),
     'tc_kernel_stats': (_i, [_i, _dp, ctypes.POINTER(_ll)]),
     'tc_reset_kernel_stats': (_i, []),
-    'tc_corr3_create': (_i, [_i, _i, _d, _d, _i, _d, _d, _d, _i] + [_dp] * 17 +
+    'tc_corr3_create': (_i, [_i, _i, _d, _d, _i, _d, _d, _d, _i] + [_dp] * 19 +
                         [ctypes.POINTER(_vp)]),
     'tc_corr3_process_auto': (_i, [_vp, _vp, _i, _i, _i]),
-    'tc_corr3_set_zeta_mode': (_i, [_vp, _i]),
     'tc_corr3_counters': (_i, [_vp, ctypes.POINTER(_ll)]),
     'tc_corr3_destroy': (_i, [_vp]),
 }

@@ -374,3 +374,120 @@ int tc_reset_kernel_stats(void) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// 3-point multipole
+// ---------------------------------------------------------------------------
+struct tc_corr3 {
+    int d = 0;
+    tc::C3Params P{};
+    double* z[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *meand2 = nullptr, *meanlogd2 = nullptr, *meand3 = nullptr, *meanlogd3 = nullptr;
+    double *weight = nullptr, *weight_im = nullptr, *ntri = nullptr;
+    long long counters[4] = {0, 0, 0, 0};
+};
+
+extern "C" {
+
+int tc_corr3_create(int d, int bin_type, double minsep, double maxsep, int nbins,
+                    double binsize, double b, double a, int max_n, double* zeta0, double* zeta1,
+                    double* zeta2, double* zeta3, double* zeta4, double* zeta5, double* zeta6,
+                    double* zeta7, double* meand1, double* meanlogd1, double* meand2,
+                    double* meanlogd2, double* meand3, double* meanlogd3, double* meanu,
+                    double* meanv, double* weight, double* weight_im, double* ntri,
+                    tc_corr3** out) {
+    (void)meand1; (void)meanlogd1; (void)meanu; (void)meanv;
+    if (!out) return tc::fail("out is NULL");
+    *out = nullptr;
+    if (d != NData && d != GData) {
+        tc::set_error("only NNN and GGG multipole correlations are implemented");
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (bin_type != 5) {
+        tc::set_error("only LogMultipole binning is implemented for 3-point correlations");
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (nbins <= 0 || max_n < 0 || !(minsep > 0.) || !(maxsep > minsep) || !(binsize > 0.))
+        return tc::fail("invalid binning");
+    if (!meand2 || !meanlogd2 || !meand3 || !meanlogd3 || !weight || !weight_im || !ntri)
+        return tc::fail("meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri are required");
+    if (d == GData && (!zeta0 || !zeta1 || !zeta2 || !zeta3 || !zeta4 || !zeta5 || !zeta6 || !zeta7))
+        return tc::fail("GGG needs zeta0..zeta7");
+    tc_corr3* c = new tc_corr3;
+    c->d = d;
+    tc::C3Params& P = c->P;
+    P.nbins = nbins; P.maxn = max_n; P.D = d; P.quick = 0;
+    P.minsep = minsep; P.maxsep = maxsep; P.binsize = binsize; P.b = b; P.a = a;
+    // BaseCorr3::BaseCorr3, src/Corr3.cpp:108-120
+    P.logminsep = std::log(minsep);
+    P.minsepsq = minsep * minsep; P.maxsepsq = maxsep * maxsep;
+    P.bsq = b * b; P.asq = a * a;
+    P.maxbsq_eff = P.maxsepsq * std::min(P.bsq, P.asq);   // getEffectiveBSq, BinType.h:50-51
+    double* zz[8] = {zeta0, zeta1, zeta2, zeta3, zeta4, zeta5, zeta6, zeta7};
+    for (int k = 0; k < 8; ++k) c->z[k] = zz[k];
+    c->meand2 = meand2; c->meanlogd2 = meanlogd2; c->meand3 = meand3; c->meanlogd3 = meanlogd3;
+    c->weight = weight; c->weight_im = weight_im; c->ntri = ntri;
+    *out = c;
+    return 0;
+}
+
+// BaseCorr3::multipole(field), src/Corr3.cpp:1124-1175 (via ProcessAuto :3654-3720).
+int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int metric) {
+    (void)dots;
+    if (!c || !f) return tc::fail("NULL argument");
+    if (metric != 1) {
+        tc::set_error("only the Euclidean metric is implemented");
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (f->d != c->d) return tc::fail("field type does not match the correlation");
+    if (int rc = check_gpu()) return rc;
+    tc::C3Params P = c->P;
+    P.quick = quick ? 1 : 0;
+    const int ntot = P.nbins * P.nbins * (2 * P.maxn + 1);
+    std::vector<double> out((size_t)15 * ntot);
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    int rc = tc::corr3_run(c->d, f->coords, f->T.nodes, f->T.nnodes, f->T.dtop,
+                           (int)f->T.top.size(), P, out.data(), cnt);
+    if (rc) return rc;
+    if (c->d == GData)
+        for (int q = 0; q < 8; ++q)
+            for (int j = 0; j < ntot; ++j) c->z[q][j] += out[(size_t)q * ntot + j];
+    double* dst[7] = {c->meand2, c->meanlogd2, c->meand3, c->meanlogd3, c->weight, c->weight_im,
+                      c->ntri};
+    for (int q = 0; q < 7; ++q)
+        for (int j = 0; j < ntot; ++j) dst[q][j] += out[(size_t)(8 + q) * ntot + j];
+    for (int k = 0; k < 4; ++k) c->counters[k] = (long long)cnt[k];
+    return 0;
+}
+
+int tc_corr3_counters(const tc_corr3* c, long long* counters4) {
+    if (!c || !counters4) return tc::fail("NULL argument");
+    for (int k = 0; k < 4; ++k) counters4[k] = c->counters[k];
+    return 0;
+}
+
+int tc_corr3_destroy(tc_corr3* c) {
+    delete c;
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// HBM counter calibration: a plain streaming copy of known size.
+// ---------------------------------------------------------------------------
+__global__ void k_copy_calibration(const double* __restrict__ src, double* __restrict__ dst,
+                                   long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int tc_copy_calibration(const double* src, double* dst, long long n) {
+    if (!src || !dst || n <= 0) return tc::fail("invalid argument");
+    if (int rc = check_gpu()) return rc;
+    hipLaunchKernelGGL(k_copy_calibration, dim3(2048), dim3(256), 0, tc::stream(), src, dst, n);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipStreamSynchronize(tc::stream()));
+    return 0;
+}

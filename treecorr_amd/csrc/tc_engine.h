@@ -34,4 +34,16 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
               const DNode* nodes2, const int* dtop2, int ntop2, const C2Params& P, double* out,
               unsigned long long* np, unsigned long long* cnt);
 
+// LogMultipole parameters of one 3-point correlation (BaseCorr3, src/Corr3.cpp:70-127).
+struct C3Params {
+    int nbins, maxn, D, quick;
+    double minsep, maxsep, binsize, b, a;
+    double logminsep, minsepsq, maxsepsq, bsq, asq, maxbsq_eff;
+};
+
+// Multipole auto-correlation of one device tree: writes 15*ntot sums (zeta0..7,
+// meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri) and 4 counters.
+int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int ntop,
+              const C3Params& P, double* out, unsigned long long* cnt);
+
 }  // namespace tc
