@@ -95,7 +95,9 @@ def test_gpu_nn_bs0_exact_vs_brute(gpu, oracle_lib):
     n = 20000
     cat = dict(x=rng.uniform(0, 1000, n), y=rng.uniform(0, 1000, n), z=None, w=np.ones(n),
                k=None, g1=None, g2=None)
-    bn = dict(nbins=20, min_sep=1., max_sep=100., bin_size=math.log(100.) / 20, b=0., a=1.)
+    # bin_slop = 0: b = 0, so min_size = max_size = 0 (corr2base.py:1595-1625)
+    bn = dict(nbins=20, min_sep=1., max_sep=100., bin_size=math.log(100.) / 20, b=0., a=1.,
+              min_size=0., max_size=0.)
     out = raw_corr2(gpu, 0, 1, cat, None, bn, min_top=3)
     b = O.brute2(0, 1, cat, None, 20, 1., 100., math.log(100.) / 20)
     np.testing.assert_array_equal(out['npairs'], b['npairs'])
