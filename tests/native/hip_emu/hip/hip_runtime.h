@@ -1,0 +1,392 @@
+// CPU emulation of the small HIP subset the engine's kernels use, for host-side
+// tests of kernel logic (built with g++ and AddressSanitizer, never with hipcc).
+//
+//  * A launch hands its blocks to EMU_WORKERS OS threads (default 1; several
+//    workers run blocks concurrently, as persistent kernels need).  Inside a
+//    block every GPU thread is a user-level fiber with its own stack; a worker
+//    runs its fibers round robin, switching only at __syncthreads() and at
+//    cross-lane operations (__shfl*, readlane), which are barriers over the
+//    block or the wavefront.  Fibers that return leave both barriers.
+//  * __shared__ variables become thread_local statics: one copy per worker,
+//    i.e. per running block.  Dynamic LDS is a buffer of exactly the launch's
+//    size.  LDS and device memory start filled with a byte pattern so that reads
+//    of unwritten memory show up; AddressSanitizer reports any access outside an
+//    allocation.
+#pragma once
+#include <math.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#if defined(__SANITIZE_ADDRESS__)
+#include <sanitizer/common_interface_defs.h>
+#define EMU_ASAN 1
+#else
+#define EMU_ASAN 0
+#endif
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static thread_local
+#define __align__(n) alignas(n)
+
+struct dim3 {
+    unsigned x, y, z;
+    constexpr dim3(unsigned x_ = 1, unsigned y_ = 1, unsigned z_ = 1) : x(x_), y(y_), z(z_) {}
+};
+
+inline thread_local dim3 threadIdx;
+inline thread_local dim3 blockIdx;
+inline dim3 blockDim;
+inline dim3 gridDim;
+
+// x86-64 System V context switch: saves the callee-saved registers on the current
+// stack, stores the stack pointer in *from, loads `to` and restores from there.
+extern "C" void emu_ctx_switch(void** from, void* to);
+asm(R"(
+.text
+.weak emu_ctx_switch
+.type emu_ctx_switch,@function
+emu_ctx_switch:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+.size emu_ctx_switch,.-emu_ctx_switch
+)");
+
+namespace emu {
+
+inline constexpr unsigned char kFill = 0x7f;
+inline size_t stack_bytes() {
+    static const size_t n = [] {
+        const char* e = std::getenv("EMU_STACK_KB");
+        return (size_t)(e ? std::max(16, std::atoi(e)) : 512) * 1024;
+    }();
+    return n;
+}
+
+struct Fiber {
+    void* sp = nullptr;
+    std::vector<unsigned char> stack;
+    dim3 tid;
+    unsigned flat = 0;
+    bool done = false;
+    bool waiting = false;
+    void* fake_stack = nullptr;
+};
+
+struct Barrier {
+    int n = 0, arrived = 0;
+    std::vector<int> waiters;
+};
+
+struct Block {
+    std::vector<Fiber> fib;
+    Barrier bar;
+    std::vector<Barrier> wave;
+    std::vector<uint64_t> xchg;
+    std::vector<unsigned char> lds;
+    std::function<void()> body;
+    void* sched_sp = nullptr;
+    int cur = -1;
+    const void* sched_bottom = nullptr;
+    size_t sched_size = 0;
+};
+
+inline thread_local Block* tl_block = nullptr;
+
+inline void switch_to_sched() {
+    Block* b = tl_block;
+    Fiber& f = b->fib[b->cur];
+#if EMU_ASAN
+    __sanitizer_start_switch_fiber(&f.fake_stack, b->sched_bottom, b->sched_size);
+#endif
+    emu_ctx_switch(&f.sp, b->sched_sp);
+#if EMU_ASAN
+    __sanitizer_finish_switch_fiber(f.fake_stack, nullptr, nullptr);
+#endif
+}
+
+inline void release(Barrier& br) {
+    Block* b = tl_block;
+    for (int i : br.waiters) b->fib[i].waiting = false;
+    br.waiters.clear();
+    br.arrived = 0;
+}
+
+inline void arrive(Barrier& br) {
+    Block* b = tl_block;
+    if (++br.arrived >= br.n) {
+        release(br);
+        return;
+    }
+    br.waiters.push_back(b->cur);
+    b->fib[b->cur].waiting = true;
+    switch_to_sched();
+}
+
+inline void leave(Barrier& br) {
+    --br.n;
+    if (br.n > 0 && br.arrived >= br.n) release(br);
+}
+
+[[noreturn]] inline void fiber_entry() {
+#if EMU_ASAN
+    __sanitizer_finish_switch_fiber(nullptr, &tl_block->sched_bottom, &tl_block->sched_size);
+#endif
+    Block* b = tl_block;
+    b->body();
+    b = tl_block;
+    Fiber& f = b->fib[b->cur];
+    leave(b->wave[f.flat / 64]);
+    leave(b->bar);
+    f.done = true;
+#if EMU_ASAN
+    __sanitizer_start_switch_fiber(nullptr, b->sched_bottom, b->sched_size);
+#endif
+    void* dummy;
+    emu_ctx_switch(&dummy, b->sched_sp);
+    __builtin_unreachable();
+}
+
+// Runs one block to completion on the calling worker thread.
+inline void run_block(Block& b, dim3 bidx, unsigned nt, size_t shmem) {
+    tl_block = &b;
+    blockIdx = bidx;
+    b.fib.resize(nt);
+    b.bar = Barrier{(int)nt, 0, {}};
+    b.wave.assign((nt + 63) / 64, Barrier{});
+    for (unsigned w = 0; w < b.wave.size(); ++w) b.wave[w].n = (int)std::min(64u, nt - 64 * w);
+    b.xchg.assign(nt, 0);
+    b.lds.assign(shmem, kFill);
+    for (unsigned t = 0; t < nt; ++t) {
+        Fiber& f = b.fib[t];
+        if (f.stack.size() != stack_bytes()) f.stack.assign(stack_bytes(), 0);
+        f.done = false;
+        f.waiting = false;
+        f.flat = t;
+        f.tid = dim3(t % blockDim.x, (t / blockDim.x) % blockDim.y, t / (blockDim.x * blockDim.y));
+        uintptr_t top = ((uintptr_t)(f.stack.data() + f.stack.size())) & ~(uintptr_t)15;
+        void** sp = (void**)top;
+        *--sp = nullptr;                       // fake return address of fiber_entry
+        *--sp = (void*)&fiber_entry;           // where emu_ctx_switch returns to
+        for (int r = 0; r < 6; ++r) *--sp = nullptr;
+        f.sp = sp;
+    }
+    unsigned left = nt;
+    while (left > 0) {
+        bool progressed = false;
+        for (unsigned t = 0; t < nt; ++t) {
+            Fiber& f = b.fib[t];
+            if (f.done || f.waiting) continue;
+            b.cur = (int)t;
+            threadIdx = f.tid;
+#if EMU_ASAN
+            void* fs = nullptr;
+            __sanitizer_start_switch_fiber(&fs, f.stack.data(), f.stack.size());
+#endif
+            emu_ctx_switch(&b.sched_sp, f.sp);
+#if EMU_ASAN
+            __sanitizer_finish_switch_fiber(fs, nullptr, nullptr);
+#endif
+            progressed = true;
+            if (f.done) --left;
+        }
+        if (!progressed && left > 0) {
+            std::fprintf(stderr, "emu: deadlock in block (%u,%u,%u): %u threads wait at a barrier "
+                                 "that the others never reach\n", bidx.x, bidx.y, bidx.z, left);
+            std::abort();
+        }
+    }
+    b.cur = -1;
+    tl_block = nullptr;
+}
+
+template <class T>
+T exchange(T v, int src) {
+    static_assert(sizeof(T) <= 8, "exchange of at most 8 bytes");
+    Block* b = tl_block;
+    const unsigned t = b->fib[b->cur].flat;
+    const unsigned w = t / 64;
+    uint64_t raw = 0;
+    std::memcpy(&raw, &v, sizeof(T));
+    b->xchg[t] = raw;
+    arrive(b->wave[w]);
+    const uint64_t got = b->xchg[w * 64 + (unsigned)src];
+    arrive(b->wave[w]);
+    T r;
+    std::memcpy(&r, &got, sizeof(T));
+    return r;
+}
+inline unsigned lane_id() { return tl_block->fib[tl_block->cur].flat % 64; }
+
+template <class F, class... A>
+void launch(F kernel, dim3 grid, dim3 block, size_t shmem, A... args) {
+    gridDim = grid;
+    blockDim = block;
+    const unsigned nt = block.x * block.y * block.z;
+    const long long nblocks = (long long)grid.x * grid.y * grid.z;
+    int workers = 1;
+    if (const char* e = std::getenv("EMU_WORKERS")) workers = std::max(1, std::atoi(e));
+    workers = (int)std::min<long long>(workers, nblocks);
+    std::atomic<long long> next{0};
+    auto work = [&] {
+        Block b;
+        b.body = [&] { kernel(args...); };
+        for (long long i; (i = next.fetch_add(1)) < nblocks;) {
+            dim3 bi((unsigned)(i % grid.x), (unsigned)((i / grid.x) % grid.y),
+                    (unsigned)(i / ((long long)grid.x * grid.y)));
+            run_block(b, bi, nt, shmem);
+        }
+    };
+    if (workers == 1) {
+        work();
+    } else {
+        std::vector<std::thread> th;
+        for (int w = 0; w < workers; ++w) th.emplace_back(work);
+        for (auto& t : th) t.join();
+    }
+}
+inline void* dyn_lds() { return tl_block->lds.data(); }
+
+}  // namespace emu
+
+inline void __syncthreads() { emu::arrive(emu::tl_block->bar); }
+inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+
+template <class T> T __shfl_xor(T v, int o, int w = 64) {
+    const int l = (int)emu::lane_id();
+    return emu::exchange(v, (l & ~(w - 1)) | ((l ^ o) & (w - 1)));
+}
+template <class T> T __shfl_up(T v, unsigned o, int w = 64) {
+    const int l = (int)emu::lane_id(), base = l & ~(w - 1);
+    return emu::exchange(v, (l - base) >= (int)o ? l - (int)o : l);
+}
+template <class T> T __shfl(T v, int s, int w = 64) {
+    const int l = (int)emu::lane_id();
+    return emu::exchange(v, (l & ~(w - 1)) + (s & (w - 1)));
+}
+inline int __builtin_amdgcn_readlane(int v, int lane) { return emu::exchange(v, lane); }
+
+inline int __double2loint(double d) { uint64_t u; std::memcpy(&u, &d, 8); return (int)(uint32_t)u; }
+inline int __double2hiint(double d) { uint64_t u; std::memcpy(&u, &d, 8); return (int)(uint32_t)(u >> 32); }
+inline double __hiloint2double(int hi, int lo) {
+    uint64_t u = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    double d; std::memcpy(&d, &u, 8); return d;
+}
+inline long long __double_as_longlong(double d) { long long u; std::memcpy(&u, &d, 8); return u; }
+inline double __longlong_as_double(long long u) { double d; std::memcpy(&d, &u, 8); return d; }
+
+template <class T, class U> T atomicAdd(T* p, U v) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }
+inline double atomicAdd(double* p, double v) {
+    uint64_t* q = (uint64_t*)p;
+    uint64_t o = __atomic_load_n(q, __ATOMIC_SEQ_CST), n;
+    do {
+        double od; std::memcpy(&od, &o, 8);
+        const double nd = od + v;
+        std::memcpy(&n, &nd, 8);
+    } while (!__atomic_compare_exchange_n(q, &o, n, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST));
+    double od; std::memcpy(&od, &o, 8);
+    return od;
+}
+inline float atomicAdd(float* p, float v) {
+    uint32_t* q = (uint32_t*)p;
+    uint32_t o = __atomic_load_n(q, __ATOMIC_SEQ_CST), n;
+    do {
+        float of; std::memcpy(&of, &o, 4);
+        const float nf = of + v;
+        std::memcpy(&n, &nf, 4);
+    } while (!__atomic_compare_exchange_n(q, &o, n, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST));
+    float of; std::memcpy(&of, &o, 4);
+    return of;
+}
+template <class T, class U> T atomicOr(T* p, U v) { return __atomic_fetch_or(p, (T)v, __ATOMIC_SEQ_CST); }
+template <class T> T atomicMax(T* p, T v) {
+    T o = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (o < v && !__atomic_compare_exchange_n(p, &o, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return o;
+}
+template <class T> T atomicMin(T* p, T v) {
+    T o = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (o > v && !__atomic_compare_exchange_n(p, &o, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return o;
+}
+template <class T> T atomicExch(T* p, T v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
+template <class T> T atomicCAS(T* p, T c, T v) {
+    __atomic_compare_exchange_n(p, &c, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+    return c;
+}
+
+// ---- runtime ----
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorLaunchFailure = 719 };
+typedef struct emu_stream* hipStream_t;
+typedef struct emu_event* hipEvent_t;
+enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2,
+                     hipMemcpyDeviceToDevice = 3, hipMemcpyDefault = 4 };
+enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize = 8 };
+enum { hipStreamNonBlocking = 1 };
+
+inline const char* hipGetErrorString(hipError_t) { return "emulated HIP error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+template <class T> hipError_t hipMalloc(T** p, size_t n) {
+    void* q = std::malloc(n ? n : 1);
+    if (!q) return hipErrorInvalidValue;
+    std::memset(q, emu::kFill, n);
+    *p = (T*)q;
+    return hipSuccess;
+}
+inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+template <class T> hipError_t hipHostMalloc(T** p, size_t n, unsigned = 0) { return hipMalloc(p, n); }
+inline hipError_t hipHostFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipMemsetAsync(void* p, int v, size_t n, hipStream_t = nullptr) { std::memset(p, v, n); return hipSuccess; }
+inline hipError_t hipMemset(void* p, int v, size_t n) { std::memset(p, v, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t = nullptr) {
+    std::memmove(d, s, n); return hipSuccess;
+}
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+    static int one_stream;   // every launch runs synchronously; the handle is only a token
+    *s = (hipStream_t)&one_stream;
+    return hipSuccess;
+}
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = nullptr; return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t = nullptr) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
+
+#define HIP_DYNAMIC_SHARED(type, var) type* var = (type*)emu::dyn_lds();
+#define hipLaunchKernelGGL(K, G, B, S, ST, ...) ((void)(ST), emu::launch(K, dim3(G), dim3(B), (size_t)(S), __VA_ARGS__))

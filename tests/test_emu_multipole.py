@@ -1,0 +1,105 @@
+"""The device multipole kernels (treecorr_amd/csrc/tc_corr3.hip) run on the CPU.
+
+tests/native/hip_emu emulates the HIP subset the kernels use (fibers for the
+threads of a workgroup, barriers, cross-lane exchange, atomics) and the kernels are
+compiled with g++ and AddressSanitizer, so every out-of-bounds access of their
+indexing is reported with its source line.  The tree comes from the oracle (it is
+bit-identical to the device tree, see test_gpu_tree.py); the results must match the
+reference's golden vectors: ntri exactly, the FP64 sums to 1e-12 normwise.
+
+Workgroups run one at a time (EMU_WORKERS=1) and eight at a time, which exercises
+the persistent unit queue of k_mp_finish with concurrent workgroups.
+"""
+import json
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import cases as C
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLD = os.path.join(HERE, 'golden')
+RTOL = 1e-12
+KEYS = ['zeta%d' % i for i in range(8)] + ['meand2', 'meanlogd2', 'meand3', 'meanlogd3',
+                                           'weight', 'weight_im', 'ntri']
+
+
+def normwise(a, b):
+    den = np.max(np.abs(b))
+    return 0. if den == 0 else np.max(np.abs(a - b)) / den
+
+
+@pytest.fixture(scope='module')
+def emu_exe():
+    if shutil.which('g++') is None:
+        pytest.skip('g++ is not available')
+    out = tempfile.mkdtemp(prefix='tc_emu_')
+    exe = os.path.join(out, 'emu_corr3')
+    cmd = ['g++', '-std=c++17', '-O1', '-g', '-fsanitize=address', '-fno-omit-frame-pointer',
+           '-ffp-contract=off', '-DEMU_CORR3_MAIN', '-I', os.path.join(HERE, 'native', 'hip_emu'),
+           '-I', os.path.join(ROOT, 'treecorr_amd', 'csrc'),
+           os.path.join(HERE, 'native', 'emu_corr3.cpp'), '-o', exe, '-pthread']
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    yield exe
+    shutil.rmtree(out, ignore_errors=True)
+
+
+def run_emu(exe, d, coords, tree, bn, workers):
+    tmp = tempfile.mkdtemp(prefix='tc_emu_run_')
+    try:
+        fin, fout = os.path.join(tmp, 'in.bin'), os.path.join(tmp, 'out.bin')
+        nn, ntop = len(tree['n']), len(tree['top'])
+        with open(fin, 'wb') as fh:
+            np.array([d, coords, nn, ntop, bn['nbins'], bn['max_n'], 0], np.int64).tofile(fh)
+            np.array([bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a']],
+                     np.float64).tofile(fh)
+            np.ascontiguousarray(tree['pos'], np.float64).tofile(fh)
+            for k in ('size', 'w'):
+                np.ascontiguousarray(tree[k], np.float32).tofile(fh)
+            np.ascontiguousarray(tree['g'], np.float32).tofile(fh)
+            np.ascontiguousarray(tree['n'], np.int64).tofile(fh)
+            for k in ('left', 'right', 'top'):
+                np.ascontiguousarray(tree[k], np.int32).tofile(fh)
+        env = dict(os.environ, EMU_WORKERS=str(workers), ASAN_OPTIONS='detect_leaks=0')
+        r = subprocess.run([exe, fin, fout], env=env, capture_output=True, timeout=600)
+        assert r.returncode == 0, r.stderr.decode()[-3000:]
+        ntot = bn['nbins'] ** 2 * (2 * bn['max_n'] + 1)
+        raw = np.fromfile(fout, np.float64, count=15 * ntot)
+        cnt = np.fromfile(fout, np.uint64, offset=15 * ntot * 8)
+        shape = (bn['nbins'], bn['nbins'], 2 * bn['max_n'] + 1)
+        out = {k: raw[i * ntot:(i + 1) * ntot].reshape(shape) for i, k in enumerate(KEYS)}
+        out['counters'] = dict(zip(('gn_calls', 'zeta_calls', 'zeta_rows', 'stages'),
+                                   (int(c) for c in cnt)))
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+RUNS = [(c, 8) for c in C.CORR3_CASES] + [(C.CORR3_CASES[0], 1), (C.CORR3_CASES[3], 1)]
+
+
+@pytest.mark.parametrize('case,workers', RUNS, ids=['%s-w%d' % (c[0], w) for c, w in RUNS])
+def test_emulated_multipole_matches_reference_golden(emu_exe, oracle_lib, case, workers):
+    O = oracle_lib
+    z = np.load(os.path.join(GOLD, case[0] + '.npz'), allow_pickle=False)
+    spec = json.loads(str(z['spec']))
+    cat, bn, extra = C.corr3_case(case)
+    assert C.checksum(cat) == spec['checksum']
+    d, coords = case[1], case[2]
+    f = O.Field(d, coords, cat['x'], cat['y'], cat['z'], k=cat.get('k'), g1=cat.get('g1'),
+                g2=cat.get('g2'), w=cat['w'], min_size=bn['min_size'], max_size=bn['max_size'],
+                min_top=extra['min_top'])
+    assert f.ntop == spec['ntop']
+    out = run_emu(emu_exe, d, coords, f.dump(), bn, workers)
+    np.testing.assert_array_equal(out['ntri'], z['ntri'])
+    for k in z.files:
+        if k not in ('spec', 'ntri') and z[k].size:
+            assert normwise(out[k], z[k]) < RTOL, k
+    ref = O.corr3_multipole(d, coords, f, bn['nbins'], bn['min_sep'], bn['max_sep'],
+                            bn['bin_size'], bn['b'], bn['a'], bn['max_n'])
+    assert out['counters'] == ref['counters']

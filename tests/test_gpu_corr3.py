@@ -10,6 +10,13 @@ import cases as C
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
 RTOL = 1e-12   # normwise relative tolerance for the FP64 sums; ntri is exact
 
+# The device multipole path is switched off in the library (TC_MULTIPOLE_ENABLED=0):
+# on gfx950 it returned wrong ntri and made an illegal memory access, twice, while the
+# same kernels under the CPU emulation (test_emu_multipole.py) match the reference.
+# These tests stay, skipped, until that cause is found.
+pytestmark = pytest.mark.skip(reason='device multipole path disabled: wrong results and an '
+                                     'illegal memory access on gfx950, cause not found yet')
+
 
 def normwise(a, b):
     den = np.max(np.abs(b))

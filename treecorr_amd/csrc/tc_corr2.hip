@@ -112,7 +112,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     long long nin, Item* __restrict__ out, unsigned long long* __restrict__ outcount, C2Params P,
     double* __restrict__ slot_hi, double* __restrict__ slot_lo,
     unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt) {
-    extern __shared__ double sh[];
+    HIP_DYNAMIC_SHARED(double, sh)
     const int nb = P.nbins;
     const int nh = NQ2 * nb;
     double* hist = sh;

@@ -59,7 +59,7 @@ enum { C3_GN = 0, C3_ZETA, C3_ROWS, C3_STAGES, NC3 = 4 };
 
 constexpr int MP_THREADS = 256;
 constexpr int MP_STACK = 96;               // DFS entries per workgroup
-constexpr long long MP_ARENA = 1 << 20;   // list entries per workgroup
+constexpr long long MP_ARENA0 = 1 << 20;  // initial list entries per workgroup (grown on demand)
 
 template <int C>
 __device__ __forceinline__ double dsq3(const DNode& a, const DNode& b) {
@@ -133,6 +133,7 @@ __global__ void k_node_sums(const DNode* nodes, int nn, NodeSum* sums, const int
         s.abswz = a * a + b * b;
     } else {
         const int L = c.child;
+        if (L >= nn - 1) { done_out[i] = 1; *changed = 1; sums[i] = NodeSum{0., 0., 0., 0.}; return; }
         if (!done_in[L] || !done_in[L + 1]) { done_out[i] = 0; return; }
         const NodeSum l = sums[L];
         const NodeSum r = sums[L + 1];
@@ -170,25 +171,54 @@ __device__ __forceinline__ int block_scan_excl(int v, int* sh, int& total) {
 
 struct Unit { int c1, off, len, pad; };
 
+// Error bits reported through `err` (all of them end the kernel cleanly).
+enum { E_POOL = 1, E_SPLITSTACK = 2, E_UNITS = 4, E_ARENA = 8, E_STACK = 16, E_INDEX = 32,
+       E_BUDGET = 64 };
+
+// Global data written by some waves of a workgroup and read by others after the
+// barrier (c2 lists in the pool and the arena, zeta output slots).  The workgroup
+// barrier alone waits only for LDS operations; the device-scope fences make this
+// thread's global stores complete before the barrier and drop stale L1 lines after it.
+__device__ __forceinline__ bool has_children(const DNode& c, int nnodes) {
+    return c.child >= 0 && c.child < nnodes - 1;
+}
+
+__device__ __forceinline__ void block_sync_global() {
+    __threadfence();
+    __syncthreads();
+    __threadfence();
+}
+
 // ---------------------------------------------------------------------------
 // Phase 1: multipoleSplit1 per top cell -> units
 // ---------------------------------------------------------------------------
+// Every node index read from memory is checked against nnodes before it is
+// dereferenced and every loop is bounded, so inconsistent data raises an error bit
+// instead of an out-of-range access.
 template <int C>
 __global__ __launch_bounds__(MP_THREADS) void k_mp_split1(
-    const DNode* __restrict__ nodes, const int* __restrict__ top, int ntop, C3Params P,
+    const DNode* __restrict__ nodes, int nnodes, const int* __restrict__ top, int ntop, C3Params P,
     int* __restrict__ pool, unsigned long long* pool_top, unsigned long long pool_cap,
     Unit* __restrict__ units, unsigned int* nunits, unsigned int unit_cap, int* err) {
     __shared__ int st_c1[MP_STACK], st_off[MP_STACK], st_len[MP_STACK];
     __shared__ int sh[MP_THREADS / 64];
     __shared__ unsigned long long s_base;
+    __shared__ int s_bad;
     int sp = 0;
     if (threadIdx.x == 0) { st_c1[0] = top[blockIdx.x]; st_off[0] = -1; st_len[0] = ntop; }
     sp = 1;
     __syncthreads();
-    while (sp > 0) {
+    for (int iter = 0; sp > 0; ++iter) {
         --sp;
         const int i1 = st_c1[sp], off = st_off[sp], len = st_len[sp];
+        if (threadIdx.x == 0) s_bad = 0;
         __syncthreads();
+        const bool ok = iter <= nnodes && (unsigned)i1 < (unsigned)nnodes && len >= 0 &&
+                        (off < 0 ? len == ntop : (unsigned long long)off + len <= pool_cap);
+        if (!ok) {
+            if (threadIdx.x == 0) atomicOr(err, iter > nnodes ? E_BUDGET : E_INDEX);
+            return;
+        }
         const DNode c1 = nodes[i1];
         const double s1 = c1.size;
         const int* list = off < 0 ? top : pool + off;
@@ -199,28 +229,42 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_split1(
             int t = base + threadIdx.x;
             int no = 0;
             if (t < len) {
-                const DNode c2 = nodes[list[t]];
-                const double s2 = c2.size;
-                const double rsq = dsq3<C>(c1, c2);
-                const double s1ps2 = s1 + s2;
-                if (!too_small3(P, rsq, s1ps2) && !too_large3(P, rsq, s1ps2)) {
-                    int k = -1; double r = 0, logr = 0;
-                    bool split = !single_bin3(P, rsq, s1ps2, k, r, logr);
-                    no = (split && s2 > s1) ? 2 : 1;
+                const int i2 = list[t];
+                if ((unsigned)i2 >= (unsigned)nnodes) {
+                    s_bad = 1;
+                } else {
+                    const DNode c2 = nodes[i2];
+                    const double s2 = c2.size;
+                    const double rsq = dsq3<C>(c1, c2);
+                    const double s1ps2 = s1 + s2;
+                    if (!too_small3(P, rsq, s1ps2) && !too_large3(P, rsq, s1ps2)) {
+                        int k = -1; double r = 0, logr = 0;
+                        bool split = !single_bin3(P, rsq, s1ps2, k, r, logr);
+                        if (split && s2 > s1) {
+                            if (!has_children(c2, nnodes)) s_bad = 1;
+                            else no = 2;
+                        } else {
+                            no = 1;
+                        }
+                    }
                 }
             }
             int tot;
             block_scan_excl(no, sh, tot);
             total_out += tot;
         }
+        if (s_bad) {
+            if (threadIdx.x == 0) atomicOr(err, E_INDEX);
+            return;
+        }
         if (threadIdx.x == 0) {
             s_base = atomicAdd(pool_top, (unsigned long long)total_out);
-            if (s_base + total_out > pool_cap) atomicOr(err, 1);
+            if (s_base + total_out > pool_cap) atomicOr(err, E_POOL);
         }
         __syncthreads();
         const unsigned long long obase = s_base;
         if (obase + total_out > pool_cap) return;
-        // pass 2: write outputs in list order
+        // pass 2: write outputs in list order (the same decisions as pass 1)
         int wpos = 0;
         for (int base = 0; base < len; base += blockDim.x) {
             int t = base + threadIdx.x;
@@ -240,27 +284,29 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_split1(
             }
             int tot;
             int ex = block_scan_excl(no, sh, tot);
-            if (no > 0) pool[obase + wpos + ex] = o0;
-            if (no > 1) pool[obase + wpos + ex + 1] = o1;
+            if (wpos + ex + no <= total_out) {
+                if (no > 0) pool[obase + wpos + ex] = o0;
+                if (no > 1) pool[obase + wpos + ex + 1] = o1;
+            }
             wpos += tot;
         }
-        __syncthreads();
+        // the children's passes read this list from other waves
+        block_sync_global();
         const int noff = (int)obase;
         if (s1 * s1 > P.maxbsq_eff) {
-            if (threadIdx.x == 0) {
-                if (sp + 2 > MP_STACK) atomicOr(err, 2);
-                else {
-                    st_c1[sp] = c1.child + 1; st_off[sp] = noff; st_len[sp] = total_out;
-                    st_c1[sp + 1] = c1.child; st_off[sp + 1] = noff; st_len[sp + 1] = total_out;
-                }
+            if (sp + 2 > MP_STACK || !has_children(c1, nnodes)) {
+                if (threadIdx.x == 0) atomicOr(err, sp + 2 > MP_STACK ? E_SPLITSTACK : E_INDEX);
+                return;
             }
-            __syncthreads();
-            if (sp + 2 > MP_STACK) return;
+            if (threadIdx.x == 0) {
+                st_c1[sp] = c1.child + 1; st_off[sp] = noff; st_len[sp] = total_out;
+                st_c1[sp + 1] = c1.child; st_off[sp + 1] = noff; st_len[sp + 1] = total_out;
+            }
             sp += 2;
         } else if (threadIdx.x == 0) {
             unsigned int u = atomicAdd(nunits, 1u);
             if (u < unit_cap) units[u] = Unit{i1, noff, total_out, 0};
-            else atomicOr(err, 4);
+            else atomicOr(err, E_UNITS);
         }
         __syncthreads();
     }
@@ -270,7 +316,7 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_split1(
 // Phase 2: multipoleFinish per unit
 // ---------------------------------------------------------------------------
 struct MPWork {
-    int* arena;            // per workgroup: MP_ARENA ints
+    int* arena;            // per workgroup: arena_cap ints
     double* scratch;       // per workgroup: MP_STACK * layout.total doubles
     int* st;               // per workgroup: MP_STACK * 8 ints (c1, off, len, mink, maxr_lo, maxr_hi)
 };
@@ -517,19 +563,28 @@ __device__ void zeta_block(const ScratchLayout& Lo, const double* sc, const DNod
     }
 }
 
+// Task record in LDS: every control value of the depth-first walk is read from
+// memory by thread 0 only and broadcast through LDS, so all waves of the workgroup
+// always agree on it.
+enum { T_C1 = 0, T_OFF, T_LEN, T_MINK, T_MAXR_LO, T_MAXR_HI, T_KIND, T_N };
+enum { TASK_NONE = 0, TASK_POP = 1, TASK_UNIT = 2, TASK_BAD = 3, TASK_STOP = 4 };
+
 template <int D, int C>
 __global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
-    const DNode* __restrict__ nodes, const NodeSum* __restrict__ nsum, const int* __restrict__ pool,
-    const Unit* __restrict__ units, unsigned int nunits, unsigned int* next_unit, C3Params P,
-    int* arena_all, double* scratch_all, int* stack_all, double* slots, int ntot,
-    unsigned long long* counters, int* err) {
-    extern __shared__ double sc[];
+    const DNode* __restrict__ nodes, int nnodes, const NodeSum* __restrict__ nsum,
+    const int* __restrict__ pool, unsigned long long pool_cap, const Unit* __restrict__ units,
+    unsigned int nunits, unsigned int* next_unit, unsigned long long* stage_count,
+    unsigned long long stage_budget, C3Params P, int* arena_all, long long arena_cap,
+    double* scratch_all,
+    int* stack_all, double* slots, int ntot, unsigned long long* counters, int* err) {
+    HIP_DYNAMIC_SHARED(double, sc)
     __shared__ int sh[MP_THREADS / 64];
-    __shared__ int s_unit, s_anysplit;
+    __shared__ int s_task[T_N];
+    __shared__ int s_anysplit, s_bad, s_stop;
     __shared__ double s_maxr;
     ScratchLayout Lo;
     Lo.init(P.nbins, P.maxn, D);
-    int* arena = arena_all + (size_t)blockIdx.x * MP_ARENA;
+    int* arena = arena_all + (size_t)blockIdx.x * arena_cap;
     double* sstack = scratch_all + (size_t)blockIdx.x * MP_STACK * Lo.total;
     int* st = stack_all + (size_t)blockIdx.x * MP_STACK * 8;
     double* slot = slots + (size_t)blockIdx.x * NOUT3 * ntot;
@@ -543,40 +598,70 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
     long long atop = 0;   // arena top (uniform)
     while (true) {
         if (!have) {
-            if (sp > 0) {
+            // next task: the DFS stack of this workgroup, else a new unit
+            if (threadIdx.x == 0) {
+                int kind = TASK_NONE;
+                if (sp > 0) {
+                    const int* e = st + (sp - 1) * 8;     // written by this same thread
+                    for (int q = 0; q < 6; ++q) s_task[q] = e[q];
+                    kind = TASK_POP;
+                } else {
+                    const unsigned int u = atomicAdd(next_unit, 1u);
+                    if (u < nunits) {
+                        const Unit un = units[u];
+                        s_task[T_C1] = un.c1; s_task[T_OFF] = un.off; s_task[T_LEN] = un.len;
+                        s_task[T_MINK] = P.nbins; s_task[T_MAXR_LO] = 0; s_task[T_MAXR_HI] = 0;
+                        kind = TASK_UNIT;
+                    }
+                }
+                if (kind != TASK_NONE) {
+                    const int c = s_task[T_C1], o = s_task[T_OFF], l = s_task[T_LEN];
+                    const int mk = s_task[T_MINK];
+                    const unsigned long long cap = kind == TASK_UNIT ? pool_cap : (unsigned long long)arena_cap;
+                    if ((unsigned)c >= (unsigned)nnodes || o < 0 || l < 0 ||
+                        (unsigned long long)o + l > cap || mk < 0 || mk > P.nbins)
+                        kind = TASK_BAD;
+                }
+                s_task[T_KIND] = kind;
+            }
+            __syncthreads();
+            const int kind = s_task[T_KIND];
+            if (kind == TASK_NONE) break;
+            if (kind == TASK_BAD) {
+                if (threadIdx.x == 0) atomicOr(err, E_INDEX);
+                break;
+            }
+            c1i = s_task[T_C1]; loff = s_task[T_OFF]; llen = s_task[T_LEN]; mink = s_task[T_MINK];
+            maxr = __hiloint2double(s_task[T_MAXR_HI], s_task[T_MAXR_LO]);
+            if (kind == TASK_POP) {
                 --sp;
-                const int* e = st + sp * 8;
-                c1i = e[0]; loff = e[1]; llen = e[2]; mink = e[3];
-                maxr = __hiloint2double(e[5], e[4]);
                 in_pool = false;
                 atop = (long long)loff + llen;
                 for (int t = threadIdx.x; t < Lo.total; t += blockDim.x)
-                    sc[t] = sstack[(size_t)sp * Lo.total + t];
-                __syncthreads();
-                have = 1;
+                    sc[t] = sstack[(size_t)sp * Lo.total + t];     // same thread wrote it
             } else {
-                if (threadIdx.x == 0) s_unit = (int)atomicAdd(next_unit, 1u);
-                __syncthreads();
-                const int u = s_unit;
-                __syncthreads();
-                if (u >= (int)nunits) break;
-                const Unit un = units[u];
-                c1i = un.c1; loff = un.off; llen = un.len; mink = P.nbins; maxr = 0.;
                 in_pool = true;
                 atop = 0;
                 for (int t = threadIdx.x; t < Lo.total; t += blockDim.x) sc[t] = 0.;
-                __syncthreads();
-                have = 1;
             }
+            __syncthreads();
+            have = 1;
         }
         // ---- one multipoleFinish stage for (c1, list) ----
         ++c_stage;
+        if (threadIdx.x == 0) {
+            s_anysplit = 0; s_maxr = 0.; s_bad = 0;
+            s_stop = atomicAdd(stage_count, 1ull) >= stage_budget;
+        }
+        __syncthreads();
+        if (s_stop) {
+            if (threadIdx.x == 0) atomicOr(err, E_BUDGET);
+            break;
+        }
         const DNode c1 = nodes[c1i];
         const double s1 = c1.size;
         const int* list = in_pool ? pool + loff : arena + loff;
         const double thresh1 = (1. - P.binsize) * maxr;
-        if (threadIdx.x == 0) { s_anysplit = 0; s_maxr = 0.; }
-        __syncthreads();
         const long long nlo = atop;      // new list starts at the arena top
         int nlen = 0;
         double lmax = 0.;
@@ -584,8 +669,10 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
         for (int base = 0; base < llen; base += blockDim.x) {
             const int t = base + threadIdx.x;
             int no = 0, o0 = 0, o1 = 0;
-            if (t < llen) {
-                const int i2 = list[t];
+            const int i2 = t < llen ? list[t] : 0;
+            if (t < llen && (unsigned)i2 >= (unsigned)nnodes) {
+                s_bad = 1;
+            } else if (t < llen) {
                 const DNode c2 = nodes[i2];
                 const double s2 = c2.size;
                 const double rsq = dsq3<C>(c1, c2);
@@ -599,8 +686,12 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
                                 logr = log(r);
                                 k = int((logr - P.logminsep) / P.binsize);
                             }
-                            ++c_gn;
-                            add_gn<D, C>(Lo, sc, c1, c2, nsum[i2], r, logr, k, P.maxn);
+                            if (k >= 0 && k < P.nbins) {
+                                ++c_gn;
+                                add_gn<D, C>(Lo, sc, c1, c2, nsum[i2], r, logr, k, P.maxn);
+                            } else {
+                                s_bad = 1;     // r at the very edge of the range: no bin
+                            }
                         }
                     } else {
                         const double rtot = r + s1ps2;
@@ -617,15 +708,20 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
                             sp2 = s2 * s2 > 0.3422 * bsq_eff;
                         }
                         if (sp1 && rtot > thresh1) lany = 1;
-                        if (sp2) { no = 2; o0 = c2.child; o1 = c2.child + 1; }
-                        else { no = 1; o0 = i2; }
+                        if (sp2) {
+                            if (has_children(c2, nnodes)) { no = 2; o0 = c2.child; o1 = c2.child + 1; }
+                            else s_bad = 1;
+                        } else {
+                            no = 1; o0 = i2;
+                        }
                     }
                 }
             }
             int tot;
             const int ex = block_scan_excl(no, sh, tot);
-            if (nlo + nlen + tot > MP_ARENA) { if (threadIdx.x == 0) atomicOr(err, 8); }
-            else {
+            if (nlo + nlen + tot > arena_cap) {
+                if (threadIdx.x == 0) atomicOr(err, E_ARENA);
+            } else {
                 if (no > 0) arena[nlo + nlen + ex] = o0;
                 if (no > 1) arena[nlo + nlen + ex + 1] = o1;
             }
@@ -639,8 +735,13 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
             if ((threadIdx.x & 63) == 0)
                 atomicMax((unsigned long long*)&s_maxr, (unsigned long long)__double_as_longlong(v));
         }
-        __syncthreads();
-        if (nlo + nlen > MP_ARENA) break;   // overflow (flagged above): stop cleanly
+        // the next stage reads the new list from other waves
+        block_sync_global();
+        if (s_bad) {
+            if (threadIdx.x == 0) atomicOr(err, E_INDEX);
+            break;
+        }
+        if (nlo + nlen > arena_cap) break;   // overflow (flagged above): stop cleanly
         const int anysplit1 = s_anysplit;
         const double nmaxr = s_maxr;
         __syncthreads();
@@ -649,23 +750,25 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
             if (nmaxr < P.maxsep) {
                 const int k = nmaxr < P.minsep ? 0
                               : int((log(nmaxr) - P.logminsep) / P.binsize) + 1;
-                if (k < mink) {
+                if (k >= 0 && k < mink) {
                     ++c_zeta; c_rows += (unsigned long long)(mink - k);
                     zeta_block<D>(Lo, sc, c1, k, mink, P, slot, ntot);
-                    __syncthreads();
+                    block_sync_global();
                     nmink = k;
                 }
             }
             if (anysplit1) {
-                if (sp >= MP_STACK) { if (threadIdx.x == 0) atomicOr(err, 16); break; }
-                int* e = st + sp * 8;
+                if (sp >= MP_STACK || !has_children(c1, nnodes)) {
+                    if (threadIdx.x == 0) atomicOr(err, sp >= MP_STACK ? E_STACK : E_INDEX);
+                    break;
+                }
                 if (threadIdx.x == 0) {
+                    int* e = st + sp * 8;
                     e[0] = c1.child + 1; e[1] = (int)nlo; e[2] = nlen; e[3] = nmink;
                     e[4] = __double2loint(nmaxr); e[5] = __double2hiint(nmaxr);
                 }
                 for (int t = threadIdx.x; t < Lo.total; t += blockDim.x)
                     sstack[(size_t)sp * Lo.total + t] = sc[t];
-                __syncthreads();
                 ++sp;
                 c1i = c1.child;
             }
@@ -674,7 +777,7 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
         } else {
             ++c_zeta; c_rows += (unsigned long long)mink;
             zeta_block<D>(Lo, sc, c1, 0, mink, P, slot, ntot);
-            __syncthreads();
+            block_sync_global();
             have = 0;
         }
     }
@@ -713,11 +816,12 @@ struct DBuf {
     }
 };
 
-using SplitFn = void (*)(const DNode*, const int*, int, C3Params, int*, unsigned long long*,
+using SplitFn = void (*)(const DNode*, int, const int*, int, C3Params, int*, unsigned long long*,
                          unsigned long long, Unit*, unsigned int*, unsigned int, int*);
-using FinishFn = void (*)(const DNode*, const NodeSum*, const int*, const Unit*, unsigned int,
-                          unsigned int*, C3Params, int*, double*, int*, double*, int,
-                          unsigned long long*, int*);
+using FinishFn = void (*)(const DNode*, int, const NodeSum*, const int*, unsigned long long,
+                          const Unit*, unsigned int, unsigned int*, unsigned long long*,
+                          unsigned long long, C3Params, int*, long long, double*, int*,
+                          double*, int, unsigned long long*, int*);
 
 static SplitFn pick_split(int C) {
     if (C == Flat) return &k_mp_split1<Flat>;
@@ -729,6 +833,18 @@ static FinishFn pick_finish(int D, int C) {
     if (D == NData) { if (C == Flat) TC_FF(NData, Flat); if (C == ThreeD) TC_FF(NData, ThreeD); TC_FF(NData, Sphere); }
     if (C == Flat) TC_FF(GData, Flat); if (C == ThreeD) TC_FF(GData, ThreeD); TC_FF(GData, Sphere);
 #undef TC_FF
+}
+
+static int set_err_bits(const char* phase, int bits) {
+    std::string m = std::string(phase) + " stopped:";
+    if (bits & E_POOL) m += " work pool overflow;";
+    if (bits & E_SPLITSTACK) m += " split stack overflow;";
+    if (bits & E_UNITS) m += " unit list overflow;";
+    if (bits & E_ARENA) m += " list arena overflow;";
+    if (bits & E_STACK) m += " depth-first stack overflow;";
+    if (bits & E_INDEX) m += " inconsistent tree or list data;";
+    if (bits & E_BUDGET) m += " stage budget exhausted;";
+    return fail(m);
 }
 
 // Runs the multipole auto-correlation of one device tree; writes NOUT3*ntot reduced
@@ -766,74 +882,104 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
         std::swap(din, dout);
     }
 
-    // phase 1: units
+    // phase 1: units.  The c2 lists of the split walk go to one pool; its size is
+    // not known beforehand, so an overflowing walk is repeated with a larger pool.
     unsigned long long pool_cap = std::max<unsigned long long>(1ull << 22, 64ull * nnodes);
-    pool_cap = std::min<unsigned long long>(pool_cap, 1ull << 31);
+    pool_cap = std::min<unsigned long long>(pool_cap, (1ull << 31) - 1);
     unsigned int unit_cap = (unsigned int)std::min<long long>(std::max(nnodes, 1024), 1 << 30);
     DBuf<int> pool, err;
     DBuf<unsigned long long> pool_top, counters;
     DBuf<Unit> units;
     DBuf<unsigned int> nunits, next_unit;
-    if (pool.alloc(pool_cap) || err.alloc(1) || pool_top.alloc(1) || counters.alloc(NC3) ||
-        units.alloc(unit_cap) || nunits.alloc(1) || next_unit.alloc(1))
+    if (err.alloc(1) || pool_top.alloc(1) || counters.alloc(NC3) || units.alloc(unit_cap) ||
+        nunits.alloc(1) || next_unit.alloc(1))
         return 2;
-    TC_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
-    TC_HIP(hipMemsetAsync(pool_top.p, 0, sizeof(unsigned long long), st));
-    TC_HIP(hipMemsetAsync(counters.p, 0, sizeof(unsigned long long) * NC3, st));
-    TC_HIP(hipMemsetAsync(nunits.p, 0, sizeof(unsigned int), st));
-    TC_HIP(hipMemsetAsync(next_unit.p, 0, sizeof(unsigned int), st));
-    hipLaunchKernelGGL(pick_split(C), dim3(ntop), dim3(MP_THREADS), 0, st, nodes, dtop, ntop, P,
-                       pool.p, pool_top.p, pool_cap, units.p, nunits.p, unit_cap, err.p);
-    TC_HIP(hipGetLastError());
     unsigned int hn = 0;
     int herr = 0;
-    TC_HIP(hipMemcpyAsync(&hn, nunits.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-    TC_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    TC_HIP(hipStreamSynchronize(st));
-    if (herr) return fail("multipole split phase overflowed its work pool");
+    for (;;) {
+        if (pool.alloc(pool_cap)) return 2;
+        TC_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+        TC_HIP(hipMemsetAsync(pool_top.p, 0, sizeof(unsigned long long), st));
+        TC_HIP(hipMemsetAsync(nunits.p, 0, sizeof(unsigned int), st));
+        hipLaunchKernelGGL(pick_split(C), dim3(ntop), dim3(MP_THREADS), 0, st, nodes, nnodes,
+                           dtop, ntop, P, pool.p, pool_top.p, pool_cap, units.p, nunits.p,
+                           unit_cap, err.p);
+        TC_HIP(hipGetLastError());
+        unsigned long long used = 0;
+        TC_HIP(hipMemcpyAsync(&hn, nunits.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipMemcpyAsync(&used, pool_top.p, sizeof(used), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipStreamSynchronize(st));
+        if (herr == E_POOL && pool_cap < (1ull << 31) - 1) {
+            pool_cap = std::min<unsigned long long>(std::max(2 * pool_cap, 2 * used),
+                                                    (1ull << 31) - 1);
+            continue;
+        }
+        break;
+    }
+    if (herr) return set_err_bits("multipole split phase", herr);
 
-    // phase 2
+    // phase 2, repeated with larger per-workgroup list arenas if one overflows
     const int nwg = (int)std::max<unsigned int>(1u, std::min<unsigned int>(hn, 512u));
     DBuf<int> arena, stk;
     DBuf<double> sstack, slots, red;
-    if (arena.alloc((size_t)nwg * MP_ARENA) || stk.alloc((size_t)nwg * MP_STACK * 8) ||
+    DBuf<unsigned long long> stage_count;
+    // No valid walk takes more stages than this: each stage accepts, splits or drops
+    // at least one (c1, c2) pair, and a pair's cells are refined at most to the leaves.
+    const unsigned long long stage_budget = 256ull * (unsigned long long)nnodes + (1ull << 20);
+    if (stage_count.alloc(1) || stk.alloc((size_t)nwg * MP_STACK * 8) ||
         sstack.alloc((size_t)nwg * MP_STACK * Lo.total) ||
         slots.alloc((size_t)nwg * NOUT3 * ntot) || red.alloc((size_t)NOUT3 * ntot))
         return 2;
-    TC_HIP(hipMemsetAsync(slots.p, 0, sizeof(double) * (size_t)nwg * NOUT3 * ntot, st));
     KernelStat& ks = kernel_stat(KS_MULTIPOLE);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timing = timing_enabled();
-    if (timing) {
-        TC_HIP(hipEventCreate(&e0));
-        TC_HIP(hipEventCreate(&e1));
-        TC_HIP(hipEventRecord(e0, st));
-    }
-    if (hn > 0) {
-        hipLaunchKernelGGL(ffn, dim3(nwg), dim3(MP_THREADS), shbytes, st, nodes, nsum.p, pool.p,
-                           units.p, hn, next_unit.p, P, arena.p, sstack.p, stk.p, slots.p, ntot,
-                           counters.p, err.p);
-        TC_HIP(hipGetLastError());
-    }
-    if (timing) TC_HIP(hipEventRecord(e1, st));
     const long long nred = (long long)NOUT3 * ntot;
+    long long arena_cap = MP_ARENA0;
+    for (;;) {
+        if (arena.alloc((size_t)nwg * arena_cap)) return 2;
+        TC_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+        TC_HIP(hipMemsetAsync(counters.p, 0, sizeof(unsigned long long) * NC3, st));
+        TC_HIP(hipMemsetAsync(next_unit.p, 0, sizeof(unsigned int), st));
+        TC_HIP(hipMemsetAsync(stage_count.p, 0, sizeof(unsigned long long), st));
+        TC_HIP(hipMemsetAsync(slots.p, 0, sizeof(double) * (size_t)nwg * NOUT3 * ntot, st));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timing) {
+            TC_HIP(hipEventCreate(&e0));
+            TC_HIP(hipEventCreate(&e1));
+            TC_HIP(hipEventRecord(e0, st));
+        }
+        if (hn > 0) {
+            hipLaunchKernelGGL(ffn, dim3(nwg), dim3(MP_THREADS), shbytes, st, nodes, nnodes,
+                               nsum.p, pool.p, pool_cap, units.p, hn, next_unit.p, stage_count.p,
+                               stage_budget, P, arena.p, arena_cap, sstack.p, stk.p, slots.p, ntot,
+                               counters.p, err.p);
+            TC_HIP(hipGetLastError());
+        }
+        if (timing) TC_HIP(hipEventRecord(e1, st));
+        TC_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipStreamSynchronize(st));
+        if (timing) {
+            float ms = 0.f;
+            TC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ks.ms += ms;
+            ks.launches += 1;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        if (herr == E_ARENA && arena_cap < (1ll << 27)) {
+            arena_cap *= 4;
+            continue;
+        }
+        break;
+    }
     hipLaunchKernelGGL(k_mp_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
                        slots.p, nwg, nred, red.p);
     TC_HIP(hipGetLastError());
     TC_HIP(hipMemcpyAsync(out, red.p, sizeof(double) * nred, hipMemcpyDeviceToHost, st));
     TC_HIP(hipMemcpyAsync(cnt, counters.p, sizeof(unsigned long long) * NC3,
                           hipMemcpyDeviceToHost, st));
-    TC_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));
-    if (timing) {
-        float ms = 0.f;
-        TC_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ks.ms += ms;
-        ks.launches += 1;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    }
-    if (herr) return fail("multipole finish phase overflowed its per-workgroup stack or arena");
+    if (herr) return set_err_bits("multipole finish phase", herr);
     return 0;
 }
 
