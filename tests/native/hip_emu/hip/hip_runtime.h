@@ -92,7 +92,8 @@ inline size_t stack_bytes() {
 
 struct Fiber {
     void* sp = nullptr;
-    std::vector<unsigned char> stack;
+    std::unique_ptr<unsigned char[]> stack;   // uninitialised; reused across blocks
+    size_t stack_size = 0;
     dim3 tid;
     unsigned flat = 0;
     bool done = false;
@@ -186,12 +187,15 @@ inline void run_block(Block& b, dim3 bidx, unsigned nt, size_t shmem) {
     b.lds.assign(shmem, kFill);
     for (unsigned t = 0; t < nt; ++t) {
         Fiber& f = b.fib[t];
-        if (f.stack.size() != stack_bytes()) f.stack.assign(stack_bytes(), 0);
+        if (f.stack_size != stack_bytes()) {
+            f.stack.reset(new unsigned char[stack_bytes()]);
+            f.stack_size = stack_bytes();
+        }
         f.done = false;
         f.waiting = false;
         f.flat = t;
         f.tid = dim3(t % blockDim.x, (t / blockDim.x) % blockDim.y, t / (blockDim.x * blockDim.y));
-        uintptr_t top = ((uintptr_t)(f.stack.data() + f.stack.size())) & ~(uintptr_t)15;
+        uintptr_t top = ((uintptr_t)(f.stack.get() + f.stack_size)) & ~(uintptr_t)15;
         void** sp = (void**)top;
         *--sp = nullptr;                       // fake return address of fiber_entry
         *--sp = (void*)&fiber_entry;           // where emu_ctx_switch returns to
@@ -208,7 +212,7 @@ inline void run_block(Block& b, dim3 bidx, unsigned nt, size_t shmem) {
             threadIdx = f.tid;
 #if EMU_ASAN
             void* fs = nullptr;
-            __sanitizer_start_switch_fiber(&fs, f.stack.data(), f.stack.size());
+            __sanitizer_start_switch_fiber(&fs, f.stack.get(), f.stack_size);
 #endif
             emu_ctx_switch(&b.sched_sp, f.sp);
 #if EMU_ASAN
@@ -245,6 +249,15 @@ T exchange(T v, int src) {
 }
 inline unsigned lane_id() { return tl_block->fib[tl_block->cur].flat % 64; }
 
+// One Block (with its fiber stacks) per worker slot, kept for the whole process.
+inline std::vector<std::unique_ptr<Block>>& block_pool(int workers) {
+    static std::vector<std::unique_ptr<Block>> blocks;
+    static std::mutex m;
+    std::lock_guard<std::mutex> g(m);
+    while ((int)blocks.size() < workers) blocks.emplace_back(new Block);
+    return blocks;
+}
+
 template <class F, class... A>
 void launch(F kernel, dim3 grid, dim3 block, size_t shmem, A... args) {
     gridDim = grid;
@@ -255,8 +268,9 @@ void launch(F kernel, dim3 grid, dim3 block, size_t shmem, A... args) {
     if (const char* e = std::getenv("EMU_WORKERS")) workers = std::max(1, std::atoi(e));
     workers = (int)std::min<long long>(workers, nblocks);
     std::atomic<long long> next{0};
-    auto work = [&] {
-        Block b;
+    std::vector<std::unique_ptr<Block>>& blocks = block_pool(workers);
+    auto work_on = [&](int wi) {
+        Block& b = *blocks[wi];
         b.body = [&] { kernel(args...); };
         for (long long i; (i = next.fetch_add(1)) < nblocks;) {
             dim3 bi((unsigned)(i % grid.x), (unsigned)((i / grid.x) % grid.y),
@@ -265,10 +279,10 @@ void launch(F kernel, dim3 grid, dim3 block, size_t shmem, A... args) {
         }
     };
     if (workers == 1) {
-        work();
+        work_on(0);
     } else {
         std::vector<std::thread> th;
-        for (int w = 0; w < workers; ++w) th.emplace_back(work);
+        for (int w = 0; w < workers; ++w) th.emplace_back(work_on, w);
         for (auto& t : th) t.join();
     }
 }

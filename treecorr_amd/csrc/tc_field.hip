@@ -77,14 +77,115 @@ __global__ void k_init_elems(Elems E, const double* x, const double* y, const do
 }
 
 // ---------------------------------------------------------------------------
-// Per-segment work: one wavefront per segment.
+// Per-segment work: one workgroup of NT threads per segment.
 // ---------------------------------------------------------------------------
-template <int D, int C>
-__global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, Elems E,
-                                               DNode* nodes, int* counters, TopEntry* tops,
-                                               BuildParams P, FieldInfo* finfo, int level0) {
-    const int s = blockIdx.x;
-    const int lane = threadIdx.x;
+// Reference-order sums.  Up to NC per-object values are summed over [start, end)
+// strictly in object order, one running sum per value, as BuildCellData and
+// finishAverages do (src/Cell.cpp:170-354).  The values of a batch of objects are
+// computed in parallel and staged in LDS as NC rows; lane c of wave 0 then adds row c
+// in order, so all NC sums advance with one dependent FP64 add per object.  With
+// NT > 64 the other waves stage batch b+1 while wave 0 adds batch b.
+constexpr long long SEG_WIDE_MAX = 2048;   // levels with at most this many segments use NT=256
+
+template <int NT> struct SegCfg {
+    static constexpr int B = NT > 64 ? 1024 : 128;   // objects per batch
+    static constexpr int NC = 5;                      // values per object, at most
+};
+
+template <int NT, int NC, class F>
+__device__ __forceinline__ void ordered_sums(long long start, long long end, double init,
+                                             double* stg, double* bcast, F value,
+                                             double (&out)[NC]) {
+    constexpr int B = SegCfg<NT>::B;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const long long n = end - start;
+    const int nbat = (int)((n + B - 1) / B);
+    // stagers: waves 1.. when there are several, else the single wave
+    const bool stager = NT > 64 ? wave > 0 : true;
+    const int st0 = NT > 64 ? t - 64 : t;
+    const int nst = NT > 64 ? NT - 64 : NT;
+    auto stage = [&](int bat, double* buf) {
+        const long long b0 = start + (long long)bat * B;
+        for (int j = st0; j < B; j += nst) {
+            const long long i = b0 + j;
+            if (i < end) {
+                double v[NC];
+                value(i, v);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) buf[c * B + j] = v[c];
+            }
+        }
+    };
+    double acc = init;
+    if (stager) stage(0, stg);
+    __syncthreads();
+    for (int bat = 0; bat < nbat; ++bat) {
+        double* cur = stg + (NT > 64 ? (bat & 1) * NC * B : 0);
+        if (NT > 64 && stager && bat + 1 < nbat) stage(bat + 1, stg + ((bat + 1) & 1) * NC * B);
+        if (wave == 0 && lane < NC) {
+            const long long cnt = end - (start + (long long)bat * B) < B
+                                      ? end - (start + (long long)bat * B) : B;
+            const double* row = cur + lane * B;
+            int j = 0;
+            for (; j + 8 <= cnt; j += 8) {
+                double r[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) r[q] = row[j + q];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc += r[q];
+            }
+            for (; j < cnt; ++j) acc += row[j];
+        }
+        __syncthreads();
+        if (NT == 64 && bat + 1 < nbat) {
+            stage(bat + 1, stg);
+            __syncthreads();
+        }
+    }
+    if (wave == 0 && lane < NC) bcast[lane] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = bcast[c];
+    __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* red) {
+    v = wave_max(v);
+    if (NT == 64) return v;
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int k = 1; k < NT / 64; ++k) m = fmax(m, red[k]);
+    __syncthreads();
+    return m;
+}
+template <int NT>
+__device__ __forceinline__ double block_min(double v, double* red) {
+    v = wave_min(v);
+    if (NT == 64) return v;
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int k = 1; k < NT / 64; ++k) m = fmin(m, red[k]);
+    __syncthreads();
+    return m;
+}
+
+template <int D, int C, int NT>
+__global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0, SegOut* sout,
+                                                Elems E, DNode* nodes, int* counters,
+                                                TopEntry* tops, BuildParams P, FieldInfo* finfo,
+                                                int level0) {
+    constexpr int B = SegCfg<NT>::B, NC = SegCfg<NT>::NC;
+    __shared__ double stg[(NT > 64 ? 2 : 1) * NC * B];
+    __shared__ double bcast[NC];
+    __shared__ double red[NT / 64];
+    __shared__ int s_int[2];
+    const long long s = seg0 + blockIdx.x;
+    const int tid = threadIdx.x;
     const Seg sg = segs[s];
     const long long start = sg.start, end = sg.end, n = end - start;
 
@@ -96,24 +197,15 @@ __global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, E
     if (n > 1 || level0) {
         // BuildCellData (src/Cell.cpp:170-200): sums start from the first object,
         // which -0.0 reproduces exactly (-0.0 + v == v for every v).
-        double ax = -0.0, ay = -0.0, az = -0.0, asw = -0.0, aww = -0.0;
-        for (long long c0 = start; c0 < end; c0 += 64) {
-            long long i = c0 + lane;
-            double vx = 0, vy = 0, vz = 0, vw = 0, vww = 0;
-            if (i < end) {
-                double wp = E.wp[i];
-                vx = E.x[i] * wp; vy = E.y[i] * wp; vz = E.z[i] * wp;
-                vw = wp; vww = double(E.w[i]);
-            }
-            const int cnt = (int)((end - c0) < 64 ? (end - c0) : 64);
-            for (int j = 0; j < cnt; ++j) {
-                ax += bcast_lane(vx, j);
-                ay += bcast_lane(vy, j);
-                if (C != Flat) az += bcast_lane(vz, j);
-                asw += bcast_lane(vw, j);
-                aww += bcast_lane(vww, j);
-            }
-        }
+        double sums[NC];
+        ordered_sums<NT, NC>(start, end, -0.0, stg, bcast,
+            [&](long long i, double (&v)[NC]) {
+                const double wp = E.wp[i];
+                v[0] = E.x[i] * wp; v[1] = E.y[i] * wp;
+                v[2] = C == Flat ? 0. : E.z[i] * wp;
+                v[3] = wp; v[4] = double(E.w[i]);
+            }, sums);
+        const double ax = sums[0], ay = sums[1], az = sums[2], asw = sums[3], aww = sums[4];
         wf = float(aww);
         if (asw != 0.) {
             cx = ax / asw; cy = ay / asw; cz = (C == Flat) ? 0. : az / asw;
@@ -123,7 +215,7 @@ __global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, E
         double smax = 0.;
         double lnx = INFINITY, lxx = -INFINITY, lny = INFINITY, lxy = -INFINITY;
         double lnz = INFINITY, lxz = -INFINITY;
-        for (long long i = start + lane; i < end; i += 64) {
+        for (long long i = start + tid; i < end; i += NT) {
             double px = E.x[i], py = E.y[i], pz = E.z[i];
             double dx = cx - px, dy = cy - py;
             double dsq;
@@ -134,11 +226,11 @@ __global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, E
             lny = fmin(lny, py); lxy = fmax(lxy, py);
             lnz = fmin(lnz, pz); lxz = fmax(lxz, pz);
         }
-        sizesq = wave_max(smax);
-        mnx = wave_min(lnx); mxx = wave_max(lxx);
-        mny = wave_min(lny); mxy = wave_max(lxy);
-        mnz = wave_min(lnz); mxz = wave_max(lxz);
-        if (level0 && lane == 0) {
+        sizesq = block_max<NT>(smax, red);
+        mnx = block_min<NT>(lnx, red); mxx = block_max<NT>(lxx, red);
+        mny = block_min<NT>(lny, red); mxy = block_max<NT>(lxy, red);
+        mnz = block_min<NT>(lnz, red); mxz = block_max<NT>(lxz, red);
+        if (level0 && tid == 0) {
             finfo->cx = cx; finfo->cy = cy; finfo->cz = cz; finfo->sizesq = sizesq;
         }
     }
@@ -150,13 +242,15 @@ __global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, E
     if (mode == SEG_TOP) {
         double ssq = (n == 1) ? 0. : sizesq;
         if (ssq == 0 || (ssq <= P.maxsizesq && sg.mintop <= 0) || sg.maxtop <= 0) {
-            if (lane == 0) {
+            if (tid == 0) {
                 node = atomicAdd(&counters[0], 1);
                 int t = atomicAdd(&counters[1], 1);
                 tops[t].key = sg.key;
                 tops[t].node = node;
+                s_int[0] = node;
             }
-            node = __shfl(node, 0, 64);
+            __syncthreads();
+            node = s_int[0];
             mode = SEG_BUILD;
         } else {
             split = true;
@@ -174,23 +268,20 @@ __global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, E
             // finishAverages (src/Cell.cpp:208-354)
             float fa = 0.f, fb = 0.f;
             if (D == KData) {
-                double sa = 0.;
-                for (long long c0 = start; c0 < end; c0 += 64) {
-                    long long i = c0 + lane;
-                    double v = (i < end) ? double(E.a[i]) : 0.;
-                    const int cnt = (int)((end - c0) < 64 ? (end - c0) : 64);
-                    for (int j = 0; j < cnt; ++j) sa += bcast_lane(v, j);
-                }
-                fa = float(sa);
+                double sk[NC];
+                ordered_sums<NT, NC>(start, end, 0., stg, bcast,
+                    [&](long long i, double (&v)[NC]) {
+                        v[0] = double(E.a[i]); v[1] = v[2] = v[3] = v[4] = 0.;
+                    }, sk);
+                fa = float(sk[0]);
             } else if (D == GData) {
-                double sr = 0., si = 0.;
                 double ccx = cx, ccy = cy, ccz = cz;
                 if (C == ThreeD) normalize_recip(ccx, ccy, ccz);
-                for (long long c0 = start; c0 < end; c0 += 64) {
-                    long long i = c0 + lane;
-                    double vr = 0., vi = 0.;
-                    if (i < end) {
-                        double gr = double(E.a[i]), gi = double(E.b[i]);
+                double sg2[NC];
+                ordered_sums<NT, NC>(start, end, 0., stg, bcast,
+                    [&](long long i, double (&v)[NC]) {
+                        const double gr = double(E.a[i]), gi = double(E.b[i]);
+                        double vr, vi;
                         if (C == Flat) {
                             vr = gr; vi = gi;
                         } else {
@@ -198,18 +289,18 @@ __global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, E
                             // evaluation order of the compiled reference.
                             double px = E.x[i], py = E.y[i], pz = E.z[i];
                             if (C == ThreeD) normalize_recip(px, py, pz);
-                            double z1 = cz, z2 = pz;
-                            double dx = ccx - px, dy = ccy - py, dz = ccz - pz;
-                            double dsq = (C == Sphere) ? (dx * dx + dy * dy) + dz * dz
-                                                       : (dy * dy + dz * dz) + dx * dx;
+                            const double z1 = cz, z2 = pz;
+                            const double dx = ccx - px, dy = ccy - py, dz = ccz - pz;
+                            const double dsq = (C == Sphere) ? (dx * dx + dy * dy) + dz * dz
+                                                             : (dy * dy + dz * dz) + dx * dx;
                             if (dsq < 1.e-12) {
                                 vr = gr; vi = gi;
                             } else {
-                                double cosA = (z1 - z2) + 0.5 * z2 * dsq;
-                                double sinA = ccy * px - ccx * py;
-                                double cosB = (z2 - z1) + 0.5 * z1 * dsq;
-                                double cosbeta = -cosA * cosB + sinA * sinA;
-                                double sinbeta = (C == Sphere)
+                                const double cosA = (z1 - z2) + 0.5 * z2 * dsq;
+                                const double sinA = ccy * px - ccx * py;
+                                const double cosB = (z2 - z1) + 0.5 * z1 * dsq;
+                                const double cosbeta = -cosA * cosB + sinA * sinA;
+                                const double sinbeta = (C == Sphere)
                                     ? (cosA + cosB) * sinA
                                     : ((z1 - z2) + (0.5 * z2 * dsq + cosB)) * sinA;
                                 double er, ei;
@@ -218,32 +309,28 @@ __global__ __launch_bounds__(64) void k_segment(const Seg* segs, SegOut* sout, E
                                 vi = gr * ei + gi * er;
                             }
                         }
-                    }
-                    const int cnt = (int)((end - c0) < 64 ? (end - c0) : 64);
-                    for (int j = 0; j < cnt; ++j) {
-                        sr += bcast_lane(vr, j);
-                        si += bcast_lane(vi, j);
-                    }
-                }
-                fa = float(sr); fb = float(si);
+                        v[0] = vr; v[1] = vi; v[2] = v[3] = v[4] = 0.;
+                    }, sg2);
+                fa = float(sg2[0]); fb = float(sg2[1]);
             }
             nd.x = cx; nd.y = cy; nd.z = cz;
             nd.w = wf; nd.a = fa; nd.b = fb; nd.n = (int)n;
             if (sizesq > P.minsizesq) {
                 nd.size = P.brute ? INFINITY : float(sqrt(sizesq));
                 split = true;
-                if (lane == 0) childL = atomicAdd(&counters[0], 2);
-                childL = __shfl(childL, 0, 64);
+                if (tid == 0) s_int[1] = atomicAdd(&counters[0], 2);
+                __syncthreads();
+                childL = s_int[1];
                 nd.child = childL;
             } else {
                 nd.size = 0.f;
                 nd.child = -1;
             }
         }
-        if (lane == 0) nodes[node] = nd;
+        if (tid == 0) nodes[node] = nd;
     }
 
-    if (lane == 0) {
+    if (tid == 0) {
         SegOut o;
         o.split = split ? 1 : 0;
         o.childL = childL;
@@ -550,9 +637,16 @@ static int build_tree_t(const double* x, const double* y, const double* z, const
     while (nseg > 0) {
         Seg* segs = bsegs[cur].as<Seg>();
         SegOut* sout = bsout.as<SegOut>();
-        hipLaunchKernelGGL((k_segment<D, C>), dim3((unsigned)nseg), dim3(64), 0, st, segs, sout,
-                           E[cur], T.nodes, bcnt.as<int>(), btops.as<TopEntry>(), P,
-                           bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0);
+        // Few large segments near the root: four waves each (three stage objects while
+        // one adds); many small ones further down: one wave each.
+        if (nseg <= SEG_WIDE_MAX)
+            hipLaunchKernelGGL((k_segment<D, C, 256>), dim3((unsigned)nseg), dim3(256), 0, st,
+                               segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
+                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0);
+        else
+            hipLaunchKernelGGL((k_segment<D, C, 64>), dim3((unsigned)nseg), dim3(64), 0, st,
+                               segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
+                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0);
         hipLaunchKernelGGL(k_seg_split_flags, grid(nseg), dim3(TPB), 0, st, sout, (int)nseg,
                            bsegflags.as<int>());
         exclusive_scan(bsegflags.as<int>(), nseg, bsegrank.as<int>(), bscan.as<int>());
