@@ -106,13 +106,19 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
     }
 }
 
+// Workgroup b takes the items [b*per_block, (b+1)*per_block) of the chunk and writes
+// the children of its split pairs, in item order, to its own region
+// out[b*4*per_block ...] (at most four children per item); bcount[b] gets their
+// number.  No global atomics: the regions are concatenated afterwards.
 template <int D, int C>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const DNode* __restrict__ n1, const DNode* __restrict__ n2, const Item* __restrict__ in,
-    long long nin, Item* __restrict__ out, unsigned long long* __restrict__ outcount, C2Params P,
+    long long nin, long long per_block, Item* __restrict__ out,
+    unsigned int* __restrict__ bcount, C2Params P,
     double* __restrict__ slot_hi, double* __restrict__ slot_lo,
     unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt) {
     HIP_DYNAMIC_SHARED(double, sh)
+    __shared__ int s_wsum[TRAV_THREADS / 64];
     const int nb = P.nbins;
     const int nh = NQ2 * nb;
     double* hist = sh;
@@ -122,14 +128,17 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
         reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
     __syncthreads();
 
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned int c_vis = 0, c_acc = 0, c_leaf = 0, c_ref = 0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < nin; base += stride) {
+    const long long b0 = (long long)blockIdx.x * per_block;
+    const long long b1 = b0 + per_block < nin ? b0 + per_block : nin;
+    Item* __restrict__ bout = out + (size_t)blockIdx.x * 4 * per_block;
+    unsigned int wpos = 0;    // children written by this workgroup so far (uniform)
+    for (long long base = b0; base < b1; base += blockDim.x) {
         const long long idx = base + threadIdx.x;
         Item o0{0, 0}, o1{0, 0}, o2{0, 0}, o3{0, 0};
         int no = 0;
-        if (idx < nin) {
+        if (idx < b1) {
             const Item it = in[idx];
             if (it.j < 0) {
                 // BaseCorr2::process2, src/Corr2.cpp:290-306
@@ -218,24 +227,30 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 }
             }
         }
-        // Wavefront compaction of the children: one atomic per wave.
+        // Workgroup-wide compaction of the children, in item order.
         int incl = no;
         for (int d = 1; d < 64; d <<= 1) {
             int y = __shfl_up(incl, d, 64);
             if (lane >= d) incl += y;
         }
-        const int total = __shfl(incl, 63, 64);
-        unsigned long long wbase = 0;
-        if (total > 0) {
-            if (lane == 63) wbase = atomicAdd(outcount, (unsigned long long)total);
-            wbase = __shfl(wbase, 63, 64);
-            const unsigned long long pos = wbase + (unsigned long long)(incl - no);
-            if (no > 0) out[pos] = o0;
-            if (no > 1) out[pos + 1] = o1;
-            if (no > 2) out[pos + 2] = o2;
-            if (no > 3) out[pos + 3] = o3;
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        unsigned int wbase = 0, tot = 0;
+#pragma unroll
+        for (int k = 0; k < TRAV_THREADS / 64; ++k) {
+            const unsigned int v = (unsigned int)s_wsum[k];
+            if (k < wave) wbase += v;
+            tot += v;
         }
+        __syncthreads();
+        const unsigned int pos = wpos + wbase + (unsigned int)(incl - no);
+        if (no > 0) bout[pos] = o0;
+        if (no > 1) bout[pos + 1] = o1;
+        if (no > 2) bout[pos + 2] = o2;
+        if (no > 3) bout[pos + 3] = o3;
+        wpos += tot;
     }
+    if (threadIdx.x == 0) bcount[blockIdx.x] = wpos;
     atomicAdd(&hcnt[CNT_VISITS], (unsigned long long)c_vis);
     atomicAdd(&hcnt[CNT_ACCEPTED], (unsigned long long)c_acc);
     atomicAdd(&hcnt[CNT_LEAF], (unsigned long long)c_leaf);
@@ -288,6 +303,54 @@ __global__ void k_reduce_slots(const double* slot_hi, const double* slot_lo,
     }
 }
 
+// Exclusive scan of the per-workgroup child counts (one workgroup of 1024 threads,
+// nblk <= TRAV_SLOTS); boff[nblk] = total.
+__global__ __launch_bounds__(1024) void k_scan_counts(const unsigned int* bcount, int nblk,
+                                                      unsigned long long* boff,
+                                                      unsigned long long* total) {
+    __shared__ unsigned long long wsum[16];
+    __shared__ unsigned long long carry_s;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t == 0) carry_s = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nblk; c0 += 1024) {
+        const int i = c0 + t;
+        const unsigned long long v = i < nblk ? bcount[i] : 0ull;
+        unsigned long long x = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            unsigned long long y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        unsigned long long wb = 0, tot = 0;
+        for (int k = 0; k < 16; ++k) {
+            if (k < wave) wb += wsum[k];
+            tot += wsum[k];
+        }
+        const unsigned long long carry = carry_s;
+        if (i < nblk) boff[i] = carry + wb + x - v;
+        __syncthreads();
+        if (t == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (t == 0) { boff[nblk] = carry_s; *total = carry_s; }
+}
+
+// Concatenates the per-workgroup child regions onto the stack: workgroup b copies its
+// bcount[b] items to dst[boff[b] ...].
+__global__ __launch_bounds__(256) void k_gather_children(const Item* __restrict__ out,
+                                                         long long per_block,
+                                                         const unsigned int* __restrict__ bcount,
+                                                         const unsigned long long* __restrict__ boff,
+                                                         Item* __restrict__ dst) {
+    const int b = blockIdx.x;
+    const unsigned int n = bcount[b];
+    const Item* src = out + (size_t)b * 4 * per_block;
+    Item* d = dst + boff[b];
+    for (unsigned int i = threadIdx.x; i < n; i += blockDim.x) d[i] = src[i];
+}
+
 __global__ void k_items_auto(const int* top, int ntop, Item* out) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long npair = (long long)ntop * (ntop - 1) / 2;
@@ -330,6 +393,8 @@ struct Corr2Scratch {
     unsigned long long *red_np = nullptr, *red_cnt = nullptr;
     unsigned long long* outcount = nullptr;
     unsigned long long* h_outcount = nullptr;
+    unsigned int* bcount = nullptr;
+    unsigned long long* boff = nullptr;
     int nb = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~Corr2Scratch() {
@@ -343,6 +408,8 @@ struct Corr2Scratch {
         if (red_np) (void)hipFree(red_np);
         if (red_cnt) (void)hipFree(red_cnt);
         if (outcount) (void)hipFree(outcount);
+        if (bcount) (void)hipFree(bcount);
+        if (boff) (void)hipFree(boff);
         if (h_outcount) (void)hipHostFree(h_outcount);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -364,7 +431,9 @@ static int ensure_scratch(Corr2Scratch& S, int nb) {
         TC_HIP(hipMalloc(&S.red_cnt, sizeof(unsigned long long) * NCNT));
         TC_HIP(hipMalloc(&S.outcount, sizeof(unsigned long long)));
         TC_HIP(hipHostMalloc((void**)&S.h_outcount, sizeof(unsigned long long)));
-        TC_HIP(hipMalloc(&S.outq, sizeof(Item) * 4 * TRAV_CHUNK));
+        TC_HIP(hipMalloc(&S.outq, sizeof(Item) * 4 * (TRAV_CHUNK + (long long)TRAV_THREADS * TRAV_SLOTS)));
+        TC_HIP(hipMalloc(&S.bcount, sizeof(unsigned int) * TRAV_SLOTS));
+        TC_HIP(hipMalloc(&S.boff, sizeof(unsigned long long) * (TRAV_SLOTS + 1)));
         TC_HIP(hipEventCreate(&S.ev0));
         TC_HIP(hipEventCreate(&S.ev1));
     }
@@ -386,8 +455,8 @@ static int ensure_stack(Corr2Scratch& S, long long need, long long keep) {
     return 0;
 }
 
-using TravFn = void (*)(const DNode*, const DNode*, const Item*, long long, Item*,
-                        unsigned long long*, C2Params, double*, double*, unsigned long long*,
+using TravFn = void (*)(const DNode*, const DNode*, const Item*, long long, long long, Item*,
+                        unsigned int*, C2Params, double*, double*, unsigned long long*,
                         unsigned long long*);
 
 static TravFn pick_traverse(int D, int C) {
@@ -432,13 +501,20 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
     while (top > 0) {
         const long long n = std::min(top, TRAV_CHUNK);
         const Item* inp = S.stack + (top - n);
-        TC_HIP(hipMemsetAsync(S.outcount, 0, sizeof(unsigned long long), st));
-        const int blocks = (int)std::min<long long>(TRAV_SLOTS, (n + TRAV_THREADS - 1) / TRAV_THREADS);
+        // contiguous ranges of whole 256-item rounds, one per workgroup
+        const long long rounds = (n + TRAV_THREADS - 1) / TRAV_THREADS;
+        const long long per_block =
+            ((rounds + TRAV_SLOTS - 1) / TRAV_SLOTS) * (long long)TRAV_THREADS;
+        const int blocks = (int)((n + per_block - 1) / per_block);
         if (timing) TC_HIP(hipEventRecord(S.ev0, st));
         hipLaunchKernelGGL(fn, dim3(blocks), dim3(TRAV_THREADS), shbytes, st, nodes1, n2, inp, n,
-                           S.outq, S.outcount, P, S.slot_hi, S.slot_lo, S.slot_np, S.slot_cnt);
+                           per_block, S.outq, S.bcount, P, S.slot_hi, S.slot_lo, S.slot_np,
+                           S.slot_cnt);
         TC_HIP(hipGetLastError());
         if (timing) TC_HIP(hipEventRecord(S.ev1, st));
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, S.bcount, blocks, S.boff,
+                           S.outcount);
+        TC_HIP(hipGetLastError());
         TC_HIP(hipMemcpyAsync(S.h_outcount, S.outcount, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, st));
         TC_HIP(hipStreamSynchronize(st));
@@ -452,8 +528,9 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
         top -= n;
         if (nout > 0) {
             if (ensure_stack(S, top + nout + 4 * TRAV_CHUNK, top)) return 2;
-            TC_HIP(hipMemcpyAsync(S.stack + top, S.outq, sizeof(Item) * nout,
-                                  hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_gather_children, dim3(blocks), dim3(256), 0, st, S.outq,
+                               per_block, S.bcount, S.boff, S.stack + top);
+            TC_HIP(hipGetLastError());
             top += nout;
         }
     }
