@@ -307,6 +307,28 @@ template <class T> T __shfl(T v, int s, int w = 64) {
     return emu::exchange(v, (l & ~(w - 1)) + (s & (w - 1)));
 }
 inline int __builtin_amdgcn_readlane(int v, int lane) { return emu::exchange(v, lane); }
+// Vote over the lanes of the wavefront that have not returned.
+inline unsigned long long __ballot(int pred) {
+    emu::Block* b = emu::tl_block;
+    const unsigned t = b->fib[b->cur].flat, w = t / 64;
+    b->xchg[t] = pred ? 1u : 0u;
+    emu::arrive(b->wave[w]);
+    unsigned long long m = 0;
+    const unsigned n = (unsigned)b->fib.size();
+    for (unsigned l = 0; l < 64 && w * 64 + l < n; ++l)
+        if (!b->fib[w * 64 + l].done && b->xchg[w * 64 + l]) m |= 1ull << l;
+    emu::arrive(b->wave[w]);
+    return m;
+}
+inline unsigned int __popcll(unsigned long long x) { return (unsigned int)__builtin_popcountll(x); }
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+// All lanes of the wavefront reach this point before any continues (on the GPU the
+// lanes run in lockstep; here it orders LDS writes and reads across lanes).
+inline void __builtin_amdgcn_wave_barrier() {
+    emu::Block* b = emu::tl_block;
+    emu::arrive(b->wave[b->fib[b->cur].flat / 64]);
+}
+inline void __builtin_amdgcn_fence(int, const char*) {}
 
 inline int __double2loint(double d) { uint64_t u; std::memcpy(&u, &d, 8); return (int)(uint32_t)u; }
 inline int __double2hiint(double d) { uint64_t u; std::memcpy(&u, &d, 8); return (int)(uint32_t)(u >> 32); }

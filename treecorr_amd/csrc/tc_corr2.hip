@@ -24,10 +24,11 @@ namespace tc {
 
 enum { Q_XI0 = 0, Q_XI1, Q_XI2, Q_XI3, Q_MEANR, Q_MEANLOGR, Q_WEIGHT, NQ2 = 7 };
 enum { CNT_VISITS = 0, CNT_ACCEPTED, CNT_LEAF, CNT_REFINED, NCNT = 4 };
+enum { CL_NONE = 0, CL_SPLIT, CL_SINGLE, CL_REFINE };   // process11 outcome classes
 
 constexpr int TRAV_THREADS = 256;
 constexpr int TRAV_SLOTS = 2048;          // persistent workgroups = accumulation slots
-constexpr long long TRAV_CHUNK = 1LL << 23;
+constexpr long long TRAV_CHUNK = 1LL << 25;
 
 template <int C>
 __device__ __forceinline__ double dist_sq(const DNode& a, const DNode& b) {
@@ -57,10 +58,12 @@ __device__ __forceinline__ void cmul(double& zr, double& zi, double er, double e
 
 // BaseCorr2::directProcess11 + Corr2::finishProcess + DirectHelper<0,1,3>
 // (src/Corr2.cpp:394-566) for one accepted pair, accumulated into LDS.
+// hist and hnp point at this lane's copy of the workgroup histogram; copies are
+// interleaved with stride R (entry e of copy c lives at e*R + c).
 template <int D, int C>
 __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, double rsq, int k,
                                          double r, double logr, const C2Params& P,
-                                         double* hist, unsigned long long* hnp) {
+                                         double* hist, unsigned long long* hnp, int R) {
     const int nb = P.nbins;
     if (k < 0) {
         r = sqrt(rsq);
@@ -69,14 +72,14 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
     }
     if (k == nb) --k;
     double ww = double(c1.w) * double(c2.w);
-    atomicAdd(&hist[Q_WEIGHT * nb + k], ww);
+    atomicAdd(&hist[(Q_WEIGHT * nb + k) * R], ww);
     if (!P.quick) {
-        atomicAdd(&hnp[k], (unsigned long long)c1.n * (unsigned long long)c2.n);
-        atomicAdd(&hist[Q_MEANR * nb + k], ww * r);
-        atomicAdd(&hist[Q_MEANLOGR * nb + k], ww * logr);
+        atomicAdd(&hnp[k * R], (unsigned long long)c1.n * (unsigned long long)c2.n);
+        atomicAdd(&hist[(Q_MEANR * nb + k) * R], ww * r);
+        atomicAdd(&hist[(Q_MEANLOGR * nb + k) * R], ww * logr);
     }
     if (D == KData) {
-        atomicAdd(&hist[Q_XI0 * nb + k], double(c1.a) * double(c2.a));
+        atomicAdd(&hist[(Q_XI0 * nb + k) * R], double(c1.a) * double(c2.a));
     } else if (D == GData) {
         double g1r = c1.a, g1i = c1.b, g2r = c2.a, g2i = c2.b;
         double er, ei;
@@ -99,10 +102,10 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
         }
         // both_complex_prod(g2, g1), include/ProjectHelper.h:84-101
         double rr = g2r * g1r, ri = g2r * g1i, ir = g2i * g1r, ii = g2i * g1i;
-        atomicAdd(&hist[Q_XI0 * nb + k], rr + ii);   // xip
-        atomicAdd(&hist[Q_XI1 * nb + k], ri - ir);   // xip_im
-        atomicAdd(&hist[Q_XI2 * nb + k], rr - ii);   // xim
-        atomicAdd(&hist[Q_XI3 * nb + k], ri + ir);   // xim_im
+        atomicAdd(&hist[(Q_XI0 * nb + k) * R], rr + ii);   // xip
+        atomicAdd(&hist[(Q_XI1 * nb + k) * R], ri - ir);   // xip_im
+        atomicAdd(&hist[(Q_XI2 * nb + k) * R], rr - ii);   // xim
+        atomicAdd(&hist[(Q_XI3 * nb + k) * R], ri + ir);   // xim_im
     }
 }
 
@@ -114,21 +117,25 @@ template <int D, int C>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const DNode* __restrict__ n1, const DNode* __restrict__ n2, const Item* __restrict__ in,
     long long nin, long long per_block, Item* __restrict__ out,
-    unsigned int* __restrict__ bcount, C2Params P,
+    unsigned int* __restrict__ bcount, C2Params P, int R,
     double* __restrict__ slot_hi, double* __restrict__ slot_lo,
     unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt) {
     HIP_DYNAMIC_SHARED(double, sh)
     __shared__ int s_wsum[TRAV_THREADS / 64];
     const int nb = P.nbins;
     const int nh = NQ2 * nb;
-    double* hist = sh;
-    unsigned long long* hnp = reinterpret_cast<unsigned long long*>(sh + nh);
-    unsigned long long* hcnt = hnp + nb;
-    for (int t = threadIdx.x; t < nh + nb + NCNT; t += blockDim.x)
+    // R interleaved copies of the histogram (R a power of two): lanes that land in the
+    // same bin in one instruction hit R different LDS banks instead of one address.
+    double* hist_all = sh;
+    unsigned long long* hnp_all = reinterpret_cast<unsigned long long*>(sh + (size_t)nh * R);
+    unsigned long long* hcnt = hnp_all + (size_t)nb * R;
+    for (int t = threadIdx.x; t < (nh + nb) * R + NCNT; t += blockDim.x)
         reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* hist = hist_all + (lane & (R - 1));
+    unsigned long long* hnp = hnp_all + (lane & (R - 1));
     unsigned int c_vis = 0, c_acc = 0, c_leaf = 0, c_ref = 0;
     const long long b0 = (long long)blockIdx.x * per_block;
     const long long b1 = b0 + per_block < nin ? b0 + per_block : nin;
@@ -149,80 +156,87 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                     no = 3;
                 }
             } else {
-                // BaseCorr2::process11, src/Corr2.cpp:308-389
+                // BaseCorr2::process11, src/Corr2.cpp:308-389.  The decision is taken in
+                // three steps so that every lane needs at most one sqrt, one log and one
+                // bin-index division, evaluated once per wavefront (the reference's
+                // expressions, unchanged: 0.5*log(rsq) for singleBin's refined test,
+                // log(sqrt(rsq)) for directProcess11).
                 ++c_vis;
                 const DNode c1 = n1[it.i];
                 const DNode c2 = n2[it.j];
+                // step 1: pruning and singleBin's cheap tests (include/BinType.h:93-151)
+                int cls = CL_NONE;
+                double rsq = 0., s1ps2 = 0., s1ps2sq = 0.;
+                const double s1 = c1.size, s2 = c2.size;
                 if (c1.w != 0.f && c2.w != 0.f) {
-                    const double s1 = c1.size, s2 = c2.size;
-                    const double rsq = dist_sq<C>(c1, c2);
-                    const double s1ps2 = s1 + s2;
+                    rsq = dist_sq<C>(c1, c2);
+                    s1ps2 = s1 + s2;
                     const bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
                                            rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
                     const bool too_large = rsq >= P.maxsepsq &&
                                            rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
                     if (!too_small && !too_large) {
-                        // BinTypeHelper<Log>::singleBin, include/BinType.h:93-151
-                        int k = -1;
-                        double r = 0., logr = 0.;
-                        bool single;
-                        if (s1ps2 == 0.) {
-                            single = true;
-                        } else {
-                            const double s1ps2sq = s1ps2 * s1ps2;
-                            if (s1ps2sq > P.asq * rsq) single = false;
-                            else if (s1ps2sq <= P.bsq * rsq) single = true;
-                            else if (s1ps2sq > 0.25 * ((P.binsize + P.b) * (P.binsize + P.b)) * rsq)
-                                single = false;
-                            else {
-                                ++c_ref;
-                                logr = 0.5 * log(rsq);
-                                const double kk = (logr - P.logminsep) / P.binsize;
-                                const int ik = int(kk);
-                                const double frackk = kk - ik;
-                                const double omf = 1. - frackk;
-                                const double f = (omf < frackk) ? omf : frackk;
-                                const double t1 = f * P.binsize + P.b;
-                                if (s1ps2sq > (t1 * t1) * rsq) single = false;
-                                else {
-                                    const double t2 = frackk * P.binsize + P.b - s1ps2sq / rsq;
-                                    if (s1ps2sq > (t2 * t2) * rsq) single = false;
-                                    else { single = true; k = ik; r = sqrt(rsq); }
-                                }
-                            }
-                        }
-                        if (single) {
-                            if (rsq >= P.minsepsq && rsq < P.maxsepsq) {
-                                ++c_acc;
-                                if (c1.child < 0 && c2.child < 0) ++c_leaf;
-                                bin_pair<D, C>(c1, c2, rsq, k, r, logr, P, hist, hnp);
-                            }
-                        } else {
-                            // CalcSplitSq, include/Split.h:82-101
-                            const double bsq_eff = rsq * (P.asq < P.bsq ? P.asq : P.bsq);
-                            bool sp1, sp2;
-                            if (s2 > s1) {
-                                sp2 = true;
-                                sp1 = (s2 > 2. * s1) ? false : (s1 * s1 > 0.3422 * bsq_eff);
-                            } else if (s1 > 2. * s2) {
-                                sp1 = true; sp2 = false;
-                            } else {
-                                sp1 = true;
-                                sp2 = s2 * s2 > 0.3422 * bsq_eff;
-                            }
-                            if (sp1 && sp2) {
-                                const int L1 = c1.child, L2 = c2.child;
-                                o0 = Item{L1, L2}; o1 = Item{L1, L2 + 1};
-                                o2 = Item{L1 + 1, L2}; o3 = Item{L1 + 1, L2 + 1};
-                                no = 4;
-                            } else if (sp1) {
-                                o0 = Item{c1.child, it.j}; o1 = Item{c1.child + 1, it.j};
-                                no = 2;
-                            } else {
-                                o0 = Item{it.i, c2.child}; o1 = Item{it.i, c2.child + 1};
-                                no = 2;
-                            }
-                        }
+                        s1ps2sq = s1ps2 * s1ps2;
+                        if (s1ps2 == 0.) cls = CL_SINGLE;
+                        else if (s1ps2sq > P.asq * rsq) cls = CL_SPLIT;
+                        else if (s1ps2sq <= P.bsq * rsq) cls = CL_SINGLE;
+                        else if (s1ps2sq > 0.25 * ((P.binsize + P.b) * (P.binsize + P.b)) * rsq)
+                            cls = CL_SPLIT;
+                        else cls = CL_REFINE;
+                    }
+                }
+                const bool inrange = rsq >= P.minsepsq && rsq < P.maxsepsq;
+                // step 2: r, log and the bin coordinate kk for the lanes that need them
+                double r = 0., logr = 0., kk = 0.;
+                if (cls == CL_REFINE || (cls == CL_SINGLE && inrange)) {
+                    r = sqrt(rsq);
+                    const double L = log(cls == CL_REFINE ? rsq : r);
+                    logr = cls == CL_REFINE ? 0.5 * L : L;
+                    kk = (logr - P.logminsep) / P.binsize;
+                }
+                // step 3: the refined test, then accept or split
+                int k = int(kk);
+                if (cls == CL_REFINE) {
+                    ++c_ref;
+                    const double frackk = kk - k;
+                    const double omf = 1. - frackk;
+                    const double f = (omf < frackk) ? omf : frackk;
+                    const double t1 = f * P.binsize + P.b;
+                    cls = CL_SINGLE;
+                    if (s1ps2sq > (t1 * t1) * rsq) cls = CL_SPLIT;
+                    else {
+                        const double t2 = frackk * P.binsize + P.b - s1ps2sq / rsq;
+                        if (s1ps2sq > (t2 * t2) * rsq) cls = CL_SPLIT;
+                    }
+                }
+                if (cls == CL_SINGLE && inrange) {
+                    ++c_acc;
+                    if (c1.child < 0 && c2.child < 0) ++c_leaf;
+                    bin_pair<D, C>(c1, c2, rsq, k, r, logr, P, hist, hnp, R);
+                } else if (cls == CL_SPLIT) {
+                    // CalcSplitSq, include/Split.h:82-101
+                    const double bsq_eff = rsq * (P.asq < P.bsq ? P.asq : P.bsq);
+                    bool sp1, sp2;
+                    if (s2 > s1) {
+                        sp2 = true;
+                        sp1 = (s2 > 2. * s1) ? false : (s1 * s1 > 0.3422 * bsq_eff);
+                    } else if (s1 > 2. * s2) {
+                        sp1 = true; sp2 = false;
+                    } else {
+                        sp1 = true;
+                        sp2 = s2 * s2 > 0.3422 * bsq_eff;
+                    }
+                    if (sp1 && sp2) {
+                        const int L1 = c1.child, L2 = c2.child;
+                        o0 = Item{L1, L2}; o1 = Item{L1, L2 + 1};
+                        o2 = Item{L1 + 1, L2}; o3 = Item{L1 + 1, L2 + 1};
+                        no = 4;
+                    } else if (sp1) {
+                        o0 = Item{c1.child, it.j}; o1 = Item{c1.child + 1, it.j};
+                        no = 2;
+                    } else {
+                        o0 = Item{it.i, c2.child}; o1 = Item{it.i, c2.child + 1};
+                        no = 2;
                     }
                 }
             }
@@ -259,7 +273,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     // Flush into this workgroup's slot with Neumaier compensation.
     const size_t so = (size_t)blockIdx.x * nh;
     for (int t = threadIdx.x; t < nh; t += blockDim.x) {
-        const double v = hist[t];
+        double v = 0.;
+        for (int c = 0; c < R; ++c) v += hist_all[(size_t)t * R + c];
         if (v == 0.) continue;
         const double s = slot_hi[so + t];
         const double tt = s + v;
@@ -267,7 +282,11 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
         slot_hi[so + t] = tt;
         slot_lo[so + t] += c;
     }
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) slot_np[(size_t)blockIdx.x * nb + t] += hnp[t];
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        unsigned long long v = 0;
+        for (int c = 0; c < R; ++c) v += hnp_all[(size_t)t * R + c];
+        slot_np[(size_t)blockIdx.x * nb + t] += v;
+    }
     if (threadIdx.x < NCNT) slot_cnt[(size_t)blockIdx.x * NCNT + threadIdx.x] += hcnt[threadIdx.x];
 }
 
@@ -456,7 +475,7 @@ static int ensure_stack(Corr2Scratch& S, long long need, long long keep) {
 }
 
 using TravFn = void (*)(const DNode*, const DNode*, const Item*, long long, long long, Item*,
-                        unsigned int*, C2Params, double*, double*, unsigned long long*,
+                        unsigned int*, C2Params, int, double*, double*, unsigned long long*,
                         unsigned long long*);
 
 static TravFn pick_traverse(int D, int C) {
@@ -475,8 +494,11 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
     hipStream_t st = stream();
     const int nb = P.nbins;
     const size_t nh = (size_t)NQ2 * nb;
-    const size_t shbytes = sizeof(double) * (nh + nb + NCNT);
-    if (shbytes > 64 * 1024) return fail("too many bins for the LDS histogram (nbins <= 1000)");
+    if (sizeof(double) * (nh + nb + NCNT) > 64 * 1024)
+        return fail("too many bins for the LDS histogram (nbins <= 1000)");
+    int R = 8;     // histogram copies: as many as fit in 48 KB, at most 8
+    while (R > 1 && sizeof(double) * ((nh + nb) * (size_t)R + NCNT) > 48 * 1024) R /= 2;
+    const size_t shbytes = sizeof(double) * ((nh + nb) * (size_t)R + NCNT);
     if (ensure_scratch(S, nb)) return 2;
     TC_HIP(hipMemsetAsync(S.slot_hi, 0, sizeof(double) * nh * TRAV_SLOTS, st));
     TC_HIP(hipMemsetAsync(S.slot_lo, 0, sizeof(double) * nh * TRAV_SLOTS, st));
@@ -508,7 +530,7 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
         const int blocks = (int)((n + per_block - 1) / per_block);
         if (timing) TC_HIP(hipEventRecord(S.ev0, st));
         hipLaunchKernelGGL(fn, dim3(blocks), dim3(TRAV_THREADS), shbytes, st, nodes1, n2, inp, n,
-                           per_block, S.outq, S.bcount, P, S.slot_hi, S.slot_lo, S.slot_np,
+                           per_block, S.outq, S.bcount, P, R, S.slot_hi, S.slot_lo, S.slot_np,
                            S.slot_cnt);
         TC_HIP(hipGetLastError());
         if (timing) TC_HIP(hipEventRecord(S.ev1, st));
