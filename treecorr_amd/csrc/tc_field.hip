@@ -178,7 +178,7 @@ template <int D, int C, int NT>
 __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0, SegOut* sout,
                                                 Elems E, DNode* nodes, int* counters,
                                                 TopEntry* tops, BuildParams P, FieldInfo* finfo,
-                                                int level0) {
+                                                int level0, long long skip_n) {
     constexpr int B = SegCfg<NT>::B, NC = SegCfg<NT>::NC;
     __shared__ double stg[(NT > 64 ? 2 : 1) * NC * B];
     __shared__ double bcast[NC];
@@ -188,6 +188,7 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
     const int tid = threadIdx.x;
     const Seg sg = segs[s];
     const long long start = sg.start, end = sg.end, n = end - start;
+    if (n <= skip_n) return;   // handled by k_segment_small
 
     double cx = E.x[start], cy = E.y[start], cz = E.z[start];
     float wf = E.w[start];
@@ -355,6 +356,160 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
         }
         sout[s] = o;
     }
+}
+
+// Segments of at most SEG_SMALL_N objects, one thread each (the deep levels hold up to
+// n/2 such segments).  The same sums in the same order as k_segment, computed by the
+// thread itself; k_segment skips these segments.
+constexpr long long SEG_SMALL_N = 32;
+
+template <int D, int C>
+__global__ __launch_bounds__(256) void k_segment_small(const Seg* segs, long long nseg,
+                                                       SegOut* sout, Elems E, DNode* nodes,
+                                                       int* counters, TopEntry* tops,
+                                                       BuildParams P) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const Seg sg = segs[s];
+    const long long start = sg.start, end = sg.end, n = end - start;
+    if (n > SEG_SMALL_N) return;
+    double cx = E.x[start], cy = E.y[start], cz = E.z[start];
+    float wf = E.w[start];
+    double sizesq = 0.;
+    double mnx = 0, mxx = 0, mny = 0, mxy = 0, mnz = 0, mxz = 0;
+    if (n > 1) {
+        // BuildCellData (src/Cell.cpp:170-200)
+        double ax = -0.0, ay = -0.0, az = -0.0, asw = -0.0, aww = -0.0;
+        for (long long i = start; i < end; ++i) {
+            const double wp = E.wp[i];
+            ax += E.x[i] * wp; ay += E.y[i] * wp;
+            if (C != Flat) az += E.z[i] * wp;
+            asw += wp; aww += double(E.w[i]);
+        }
+        wf = float(aww);
+        if (asw != 0.) {
+            cx = ax / asw; cy = ay / asw; cz = (C == Flat) ? 0. : az / asw;
+            if (C == Sphere) normalize_div(cx, cy, cz);
+        }
+        // CalculateSizeSq (src/Cell.cpp:89-100) and Bounds (include/Bounds.h)
+        mnx = mny = mnz = INFINITY; mxx = mxy = mxz = -INFINITY;
+        for (long long i = start; i < end; ++i) {
+            const double px = E.x[i], py = E.y[i], pz = E.z[i];
+            const double dx = cx - px, dy = cy - py;
+            double dsq;
+            if (C == Flat) dsq = dx * dx + dy * dy;
+            else { const double dz = cz - pz; dsq = dx * dx + dy * dy + dz * dz; }
+            sizesq = fmax(sizesq, dsq);
+            mnx = fmin(mnx, px); mxx = fmax(mxx, px);
+            mny = fmin(mny, py); mxy = fmax(mxy, py);
+            mnz = fmin(mnz, pz); mxz = fmax(mxz, pz);
+        }
+    }
+    // SetupTopLevelCells (src/Field.cpp:27-81) and BuildCell (src/Cell.cpp:556-615)
+    int mode = sg.mode;
+    int node = sg.node;
+    bool split = false;
+    if (mode == SEG_TOP) {
+        const double ssq = (n == 1) ? 0. : sizesq;
+        if (ssq == 0 || (ssq <= P.maxsizesq && sg.mintop <= 0) || sg.maxtop <= 0) {
+            node = atomicAdd(&counters[0], 1);
+            const int t = atomicAdd(&counters[1], 1);
+            tops[t].key = sg.key;
+            tops[t].node = node;
+            mode = SEG_BUILD;
+        } else {
+            split = true;
+        }
+    }
+    int childL = -1;
+    if (mode == SEG_BUILD) {
+        DNode nd;
+        if (n == 1) {
+            nd.x = E.x[start]; nd.y = E.y[start]; nd.z = E.z[start];
+            nd.w = E.w[start]; nd.a = E.a[start]; nd.b = E.b[start];
+            nd.size = 0.f; nd.child = -1; nd.n = 1;
+        } else {
+            // finishAverages (src/Cell.cpp:208-354)
+            float fa = 0.f, fb = 0.f;
+            if (D == KData) {
+                double sa = 0.;
+                for (long long i = start; i < end; ++i) sa += double(E.a[i]);
+                fa = float(sa);
+            } else if (D == GData) {
+                double sr = 0., si = 0.;
+                double ccx = cx, ccy = cy, ccz = cz;
+                if (C == ThreeD) normalize_recip(ccx, ccy, ccz);
+                for (long long i = start; i < end; ++i) {
+                    const double gr = double(E.a[i]), gi = double(E.b[i]);
+                    double vr, vi;
+                    if (C == Flat) {
+                        vr = gr; vi = gi;
+                    } else {
+                        // ParallelTransportSum (src/Cell.cpp:276-321), in the evaluation
+                        // order of the compiled reference
+                        double px = E.x[i], py = E.y[i], pz = E.z[i];
+                        if (C == ThreeD) normalize_recip(px, py, pz);
+                        const double z1 = cz, z2 = pz;
+                        const double dx = ccx - px, dy = ccy - py, dz = ccz - pz;
+                        const double dsq = (C == Sphere) ? (dx * dx + dy * dy) + dz * dz
+                                                         : (dy * dy + dz * dz) + dx * dx;
+                        if (dsq < 1.e-12) {
+                            vr = gr; vi = gi;
+                        } else {
+                            const double cosA = (z1 - z2) + 0.5 * z2 * dsq;
+                            const double sinA = ccy * px - ccx * py;
+                            const double cosB = (z2 - z1) + 0.5 * z1 * dsq;
+                            const double cosbeta = -cosA * cosB + sinA * sinA;
+                            const double sinbeta = (C == Sphere)
+                                ? (cosA + cosB) * sinA
+                                : ((z1 - z2) + (0.5 * z2 * dsq + cosB)) * sinA;
+                            double er, ei;
+                            expm2ialpha(cosbeta, sinbeta, er, ei);
+                            vr = gr * er - gi * ei;
+                            vi = gr * ei + gi * er;
+                        }
+                    }
+                    sr += vr; si += vi;
+                }
+                fa = float(sr); fb = float(si);
+            }
+            nd.x = cx; nd.y = cy; nd.z = cz;
+            nd.w = wf; nd.a = fa; nd.b = fb; nd.n = (int)n;
+            if (sizesq > P.minsizesq) {
+                nd.size = P.brute ? INFINITY : float(sqrt(sizesq));
+                split = true;
+                childL = atomicAdd(&counters[0], 2);
+                nd.child = childL;
+            } else {
+                nd.size = 0.f;
+                nd.child = -1;
+            }
+        }
+        nodes[node] = nd;
+    }
+    SegOut o;
+    o.split = split ? 1 : 0;
+    o.childL = childL;
+    o.child_mode = mode;
+    o.mid = start;
+    o.fallback = 0;
+    o.dim = 0;
+    o.value = 0.;
+    if (split) {
+        // Bounds<C>::getSplit (include/Bounds.h:65-66, 133-141)
+        int dim;
+        if (C == Flat) dim = (mxy - mny) > (mxx - mnx) ? 1 : 0;
+        else {
+            const double xr = mxx - mnx, yr = mxy - mny, zr = mxz - mnz;
+            dim = yr > xr ? (zr > yr ? 2 : 1) : (zr > xr ? 2 : 0);
+        }
+        o.dim = dim;
+        if (P.sm == Mean) o.value = dim == 2 ? cz : dim == 1 ? cy : cx;
+        else if (P.sm == Middle)
+            o.value = dim == 2 ? (mxz + mnz) / 2. : dim == 1 ? (mxy + mny) / 2. : (mxx + mnx) / 2.;
+        else o.fallback = 1;   // Median
+    }
+    sout[s] = o;
 }
 
 // ---------------------------------------------------------------------------
@@ -639,14 +794,20 @@ static int build_tree_t(const double* x, const double* y, const double* z, const
         SegOut* sout = bsout.as<SegOut>();
         // Few large segments near the root: four waves each (three stage objects while
         // one adds); many small ones further down: one wave each.
-        if (nseg <= SEG_WIDE_MAX)
+        if (nseg <= SEG_WIDE_MAX) {
             hipLaunchKernelGGL((k_segment<D, C, 256>), dim3((unsigned)nseg), dim3(256), 0, st,
                                segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
-                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0);
-        else
+                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0,
+                               0LL);
+        } else {
+            // many segments: the small ones one thread each, the rest one wave each
+            hipLaunchKernelGGL((k_segment_small<D, C>), grid(nseg), dim3(TPB), 0, st, segs, nseg,
+                               sout, E[cur], T.nodes, bcnt.as<int>(), btops.as<TopEntry>(), P);
             hipLaunchKernelGGL((k_segment<D, C, 64>), dim3((unsigned)nseg), dim3(64), 0, st,
                                segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
-                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0);
+                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0,
+                               SEG_SMALL_N);
+        }
         hipLaunchKernelGGL(k_seg_split_flags, grid(nseg), dim3(TPB), 0, st, sout, (int)nseg,
                            bsegflags.as<int>());
         exclusive_scan(bsegflags.as<int>(), nseg, bsegrank.as<int>(), bscan.as<int>());
