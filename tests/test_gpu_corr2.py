@@ -117,3 +117,28 @@ def test_gpu_finalized_api(gpu, oracle_lib):
     assert np.all((gg.meanr > gg.left_edges * 0.9) & (gg.meanr < gg.right_edges * 1.1))
     varg = c.varg
     np.testing.assert_allclose(gg.varxip, 2 * varg ** 2 / gg.weight, rtol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('chunk', [3000, 40000])
+def test_gpu_small_chunks_same_result(gpu, oracle_lib, chunk):
+    """The traversal's work stack across many launches (children read in place, the
+    excess moved onto the stack) gives the same sums as the oracle."""
+    from treecorr_amd import _lib
+    O = oracle_lib
+    cat, scale = C.make_catalog(C.SPHERE, 8000, 97, 0.3)
+    bn = C.binning(scale, 15, 0.005, 0.3, 0.1, 0.1)
+    _lib.check(_lib.lib().tc_set_traversal_chunk(chunk))
+    try:
+        out = raw_corr2(gpu, 4, 3, cat, None, bn, min_top=5)
+    finally:
+        _lib.check(_lib.lib().tc_set_traversal_chunk(0))
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=5)
+    f = O.Field(4, 3, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw)
+    ref = O.corr2(4, 3, f, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                  bn['b'], bn['a'], exact=True)
+    np.testing.assert_array_equal(out['npairs'], ref['npairs'])
+    for k in KEYS[:-1]:
+        assert normwise(out[k], ref[k]) < RTOL, k
+    for k in ('visits', 'accepted', 'leafpairs', 'refined'):
+        assert out['counters'][k] == ref['counters'][k], k

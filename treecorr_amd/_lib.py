@@ -50,6 +50,7 @@ SIGNATURES = {
     'tc_copy_calibration': (_i, [_vp, _vp, _ll]),
     'tc_kernel_stats': (_i, [_i, _dp, ctypes.POINTER(_ll)]),
     'tc_reset_kernel_stats': (_i, []),
+    'tc_set_traversal_chunk': (_i, [_ll]),
     'tc_corr3_create': (_i, [_i, _i, _d, _d, _i, _d, _d, _d, _i] + [_dp] * 19 +
                         [ctypes.POINTER(_vp)]),
     'tc_corr3_process_auto': (_i, [_vp, _vp, _i, _i, _i]),

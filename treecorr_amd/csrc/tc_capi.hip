@@ -329,6 +329,8 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
     return 0;
 }
 
+int tc_set_traversal_chunk(long long items) { return tc::set_traversal_chunk(items); }
+
 int tc_corr2_counters(const tc_corr2* c, long long* counters4) {
     if (!c || !counters4) return tc::fail("NULL argument");
     for (int k = 0; k < 4; ++k) counters4[k] = c->counters[k];

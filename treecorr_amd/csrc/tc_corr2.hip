@@ -28,7 +28,8 @@ enum { CL_NONE = 0, CL_SPLIT, CL_SINGLE, CL_REFINE };   // process11 outcome cla
 
 constexpr int TRAV_THREADS = 256;
 constexpr int TRAV_SLOTS = 2048;          // persistent workgroups = accumulation slots
-constexpr long long TRAV_CHUNK = 1LL << 25;
+constexpr long long TRAV_CHUNK = 1LL << 25;   // items per launch (buffers are sized for it)
+static long long g_chunk = TRAV_CHUNK;        // tc_set_traversal_chunk, for tests and tuning
 
 template <int C>
 __device__ __forceinline__ double dist_sq(const DNode& a, const DNode& b) {
@@ -124,15 +125,28 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
 // the children of its split pairs, in item order, to its own region
 // out[b*4*per_block ...] (at most four children per item); bcount[b] gets their
 // number.  No global atomics: the regions are concatenated afterwards.
+// Where a launch reads its items.  Items [0, nhead) come from the output regions of
+// the previous launch, read in place: region r holds off[r+1]-off[r] items at
+// in + r*stride, and the launch takes their concatenated items [first, first+nhead).
+// Items [nhead, nin) come from the contiguous run tail (the top of the stack).
+struct InView {
+    const Item* in;
+    const unsigned long long* off;
+    long long stride, first, nhead;
+    int nreg;
+    const Item* tail;
+};
+
 template <int D, int C>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
-    const DNode* __restrict__ n1, const DNode* __restrict__ n2, const Item* __restrict__ in,
+    const DNode* __restrict__ n1, const DNode* __restrict__ n2, InView V,
     long long nin, long long per_block, Item* __restrict__ out,
     unsigned int* __restrict__ bcount, C2Params P, int R,
     double* __restrict__ slot_hi, double* __restrict__ slot_lo,
     unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt) {
     HIP_DYNAMIC_SHARED(double, sh)
     __shared__ int s_wsum[TRAV_THREADS / 64];
+    __shared__ unsigned long long s_off[TRAV_SLOTS + 1];
     const int nb = P.nbins;
     const int nh = NQ2 * nb;
     // R interleaved copies of the histogram (R a power of two): lanes that land in the
@@ -142,6 +156,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     unsigned long long* hcnt = hnp_all + (size_t)nb * R;
     for (int t = threadIdx.x; t < (nh + nb) * R + NCNT; t += blockDim.x)
         reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
+    if (V.nhead > 0)
+        for (int t = threadIdx.x; t <= V.nreg; t += blockDim.x) s_off[t] = V.off[t];
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -152,12 +168,30 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const long long b1 = b0 + per_block < nin ? b0 + per_block : nin;
     Item* __restrict__ bout = out + (size_t)blockIdx.x * 4 * per_block;
     unsigned int wpos = 0;    // children written by this workgroup so far (uniform)
+    // region cursor of this thread's next item (items of a thread only move forward)
+    int reg = 0;
+    if (b0 + threadIdx.x < b1 && b0 + threadIdx.x < V.nhead) {
+        const unsigned long long g = (unsigned long long)(V.first + b0 + threadIdx.x);
+        int lo = 0, hi = V.nreg - 1;      // last region with s_off[r] <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        reg = lo;
+    }
     for (long long base = b0; base < b1; base += blockDim.x) {
         const long long idx = base + threadIdx.x;
         Item o0{0, 0}, o1{0, 0}, o2{0, 0}, o3{0, 0};
         int no = 0;
         if (idx < b1) {
-            const Item it = in[idx];
+            Item it;
+            if (idx < V.nhead) {
+                const unsigned long long g = (unsigned long long)(V.first + idx);
+                while (s_off[reg + 1] <= g) ++reg;
+                it = V.in[(long long)reg * V.stride + (long long)(g - s_off[reg])];
+            } else {
+                it = V.tail[idx - V.nhead];
+            }
             if (it.j < 0) {
                 // BaseCorr2::process2, src/Corr2.cpp:290-306
                 const DNode c = n1[it.i];
@@ -369,16 +403,20 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const unsigned int* bcount
 
 // Concatenates the per-workgroup child regions onto the stack: workgroup b copies its
 // bcount[b] items to dst[boff[b] ...].
+// Copies the concatenated items [0, limit) of the regions onto the stack (the rest is
+// read in place by the next launch): workgroup b copies its share to dst[boff[b] ...].
 __global__ __launch_bounds__(256) void k_gather_children(const Item* __restrict__ out,
                                                          long long per_block,
                                                          const unsigned int* __restrict__ bcount,
                                                          const unsigned long long* __restrict__ boff,
+                                                         unsigned long long limit,
                                                          Item* __restrict__ dst) {
     const int b = blockIdx.x;
-    const unsigned int n = bcount[b];
+    if (boff[b] >= limit) return;
+    const unsigned long long n = bcount[b] < limit - boff[b] ? bcount[b] : limit - boff[b];
     const Item* src = out + (size_t)b * 4 * per_block;
     Item* d = dst + boff[b];
-    for (unsigned int i = threadIdx.x; i < n; i += blockDim.x) d[i] = src[i];
+    for (unsigned long long i = threadIdx.x; i < n; i += blockDim.x) d[i] = src[i];
 }
 
 __global__ void k_items_auto(const int* top, int ntop, Item* out) {
@@ -416,20 +454,22 @@ __global__ void k_items_cross(const int* top1, int ntop1, const int* top2, int n
 struct Corr2Scratch {
     Item* stack = nullptr;
     long long cap = 0;
-    Item* outq = nullptr;
+    Item* outq = nullptr;      // two output region buffers (ping-pong)
+    Item* outq2 = nullptr;
     double *slot_hi = nullptr, *slot_lo = nullptr;
     unsigned long long *slot_np = nullptr, *slot_cnt = nullptr;
     double* red = nullptr;
     unsigned long long *red_np = nullptr, *red_cnt = nullptr;
     unsigned long long* outcount = nullptr;
     unsigned long long* h_outcount = nullptr;
-    unsigned int* bcount = nullptr;
-    unsigned long long* boff = nullptr;
+    unsigned int* bcount = nullptr;     // two count arrays (ping-pong), TRAV_SLOTS each
+    unsigned long long* boff = nullptr; // two offset arrays, TRAV_SLOTS+1 each
     int nb = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~Corr2Scratch() {
         if (stack) (void)hipFree(stack);
         if (outq) (void)hipFree(outq);
+        if (outq2) (void)hipFree(outq2);
         if (slot_hi) (void)hipFree(slot_hi);
         if (slot_lo) (void)hipFree(slot_lo);
         if (slot_np) (void)hipFree(slot_np);
@@ -462,8 +502,9 @@ static int ensure_scratch(Corr2Scratch& S, int nb) {
         TC_HIP(hipMalloc(&S.outcount, sizeof(unsigned long long)));
         TC_HIP(hipHostMalloc((void**)&S.h_outcount, sizeof(unsigned long long)));
         TC_HIP(hipMalloc(&S.outq, sizeof(Item) * 4 * (TRAV_CHUNK + (long long)TRAV_THREADS * TRAV_SLOTS)));
-        TC_HIP(hipMalloc(&S.bcount, sizeof(unsigned int) * TRAV_SLOTS));
-        TC_HIP(hipMalloc(&S.boff, sizeof(unsigned long long) * (TRAV_SLOTS + 1)));
+        TC_HIP(hipMalloc(&S.outq2, sizeof(Item) * 4 * (TRAV_CHUNK + (long long)TRAV_THREADS * TRAV_SLOTS)));
+        TC_HIP(hipMalloc(&S.bcount, sizeof(unsigned int) * 2 * TRAV_SLOTS));
+        TC_HIP(hipMalloc(&S.boff, sizeof(unsigned long long) * 2 * (TRAV_SLOTS + 1)));
         TC_HIP(hipEventCreate(&S.ev0));
         TC_HIP(hipEventCreate(&S.ev1));
     }
@@ -485,7 +526,7 @@ static int ensure_stack(Corr2Scratch& S, long long need, long long keep) {
     return 0;
 }
 
-using TravFn = void (*)(const DNode*, const DNode*, const Item*, long long, long long, Item*,
+using TravFn = void (*)(const DNode*, const DNode*, InView, long long, long long, Item*,
                         unsigned int*, C2Params, int, double*, double*, unsigned long long*,
                         unsigned long long*);
 
@@ -531,22 +572,56 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
     long long top = ninit;
     KernelStat& ks = kernel_stat(KS_TRAVERSE2);
     const bool timing = timing_enabled();
-    while (top > 0) {
-        const long long n = std::min(top, TRAV_CHUNK);
-        const Item* inp = S.stack + (top - n);
+    Item* outq[2] = {S.outq, S.outq2};
+    // children of the last launch, still in its output regions (newest work)
+    int cur = 0;                 // buffer the last launch wrote
+    long long reg_n = 0, reg_pb = 0;
+    int reg_blocks = 0;
+    for (;;) {
+        InView V{nullptr, nullptr, 0, 0, 0, 0, nullptr};
+        long long n = 0;
+        if (reg_n > 0) {
+            // LIFO: the newest children are processed next, in place; the oldest part
+            // beyond one chunk is moved onto the stack first.
+            n = std::min(reg_n, g_chunk);
+            const long long excess = reg_n - n;
+            if (excess > 0) {
+                if (ensure_stack(S, top + excess + 4 * TRAV_CHUNK, top)) return 2;
+                hipLaunchKernelGGL(k_gather_children, dim3(reg_blocks), dim3(256), 0, st,
+                                   outq[cur], reg_pb, S.bcount + cur * TRAV_SLOTS,
+                                   S.boff + cur * (TRAV_SLOTS + 1),
+                                   (unsigned long long)excess, S.stack + top);
+                TC_HIP(hipGetLastError());
+                top += excess;
+            }
+            // a short head is topped up with the newest stack items
+            const long long extra = excess > 0 ? 0 : std::min(top, g_chunk - n);
+            V = InView{outq[cur], S.boff + cur * (TRAV_SLOTS + 1), 4 * reg_pb, excess, n,
+                       reg_blocks, S.stack + (top - extra)};
+            n += extra;
+            top -= extra;
+            reg_n = 0;
+        } else if (top > 0) {
+            n = std::min(top, g_chunk);
+            V = InView{nullptr, nullptr, 0, 0, 0, 0, S.stack + (top - n)};
+            top -= n;
+        } else {
+            break;
+        }
+        const int nxt = 1 - cur;
         // contiguous ranges of whole 256-item rounds, one per workgroup
         const long long rounds = (n + TRAV_THREADS - 1) / TRAV_THREADS;
         const long long per_block =
             ((rounds + TRAV_SLOTS - 1) / TRAV_SLOTS) * (long long)TRAV_THREADS;
         const int blocks = (int)((n + per_block - 1) / per_block);
         if (timing) TC_HIP(hipEventRecord(S.ev0, st));
-        hipLaunchKernelGGL(fn, dim3(blocks), dim3(TRAV_THREADS), shbytes, st, nodes1, n2, inp, n,
-                           per_block, S.outq, S.bcount, P, R, S.slot_hi, S.slot_lo, S.slot_np,
-                           S.slot_cnt);
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(TRAV_THREADS), shbytes, st, nodes1, n2, V, n,
+                           per_block, outq[nxt], S.bcount + nxt * TRAV_SLOTS, P, R, S.slot_hi,
+                           S.slot_lo, S.slot_np, S.slot_cnt);
         TC_HIP(hipGetLastError());
         if (timing) TC_HIP(hipEventRecord(S.ev1, st));
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, S.bcount, blocks, S.boff,
-                           S.outcount);
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, S.bcount + nxt * TRAV_SLOTS,
+                           blocks, S.boff + nxt * (TRAV_SLOTS + 1), S.outcount);
         TC_HIP(hipGetLastError());
         TC_HIP(hipMemcpyAsync(S.h_outcount, S.outcount, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, st));
@@ -557,15 +632,10 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
             ks.ms += ms;
             ks.launches += 1;
         }
-        const long long nout = (long long)*S.h_outcount;
-        top -= n;
-        if (nout > 0) {
-            if (ensure_stack(S, top + nout + 4 * TRAV_CHUNK, top)) return 2;
-            hipLaunchKernelGGL(k_gather_children, dim3(blocks), dim3(256), 0, st, S.outq,
-                               per_block, S.bcount, S.boff, S.stack + top);
-            TC_HIP(hipGetLastError());
-            top += nout;
-        }
+        reg_n = (long long)*S.h_outcount;
+        reg_pb = per_block;
+        reg_blocks = blocks;
+        cur = nxt;
     }
     const int tot = (int)(nh + nb + NCNT);
     hipLaunchKernelGGL(k_reduce_slots, dim3((tot + 255) / 256), dim3(256), 0, st, S.slot_hi,
@@ -576,6 +646,12 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
     TC_HIP(hipMemcpyAsync(np, S.red_np, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, st));
     TC_HIP(hipMemcpyAsync(cnt, S.red_cnt, sizeof(unsigned long long) * NCNT, hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int set_traversal_chunk(long long items) {
+    if (items < 0) return fail("chunk must be >= 0");
+    g_chunk = items == 0 ? TRAV_CHUNK : std::min<long long>(std::max<long long>(items, 256), TRAV_CHUNK);
     return 0;
 }
 

@@ -27,6 +27,7 @@ struct C2Params {
     double logminsep, halfminsep, minsepsq, maxsepsq, bsq, asq;
 };
 
+int set_traversal_chunk(long long items);   // 0 restores the default
 struct Corr2Scratch;
 Corr2Scratch* new_corr2_scratch();
 void delete_corr2_scratch(Corr2Scratch*);
