@@ -34,6 +34,54 @@ hipStream_t stream() {
     return g_stream;
 }
 
+// ---- caching allocator for temporaries ----
+namespace {
+struct FreeBlock { void* p; size_t bytes; };
+std::mutex g_pool_m;
+std::vector<FreeBlock> g_pool;
+size_t g_pool_bytes = 0;
+constexpr size_t kPoolMax = size_t(24) << 30;   // bytes kept for reuse
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    {
+        std::lock_guard<std::mutex> g(g_pool_m);
+        // smallest cached block that fits and is not more than twice the request
+        int best = -1;
+        for (int i = 0; i < (int)g_pool.size(); ++i)
+            if (g_pool[i].bytes >= bytes && g_pool[i].bytes <= 2 * bytes &&
+                (best < 0 || g_pool[i].bytes < g_pool[best].bytes))
+                best = i;
+        if (best >= 0) {
+            FreeBlock b = g_pool[best];
+            g_pool.erase(g_pool.begin() + best);
+            g_pool_bytes -= b.bytes;
+            return b.p;
+        }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        // release the cache and try once more
+        std::lock_guard<std::mutex> g(g_pool_m);
+        for (auto& b : g_pool) (void)hipFree(b.p);
+        g_pool.clear();
+        g_pool_bytes = 0;
+        (void)hipGetLastError();
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    }
+    return p;
+}
+
+void dev_free(void* p, size_t bytes) {
+    if (!p) return;
+    if (bytes == 0) bytes = 256;
+    std::lock_guard<std::mutex> g(g_pool_m);
+    if (g_pool_bytes + bytes > kPoolMax) { (void)hipFree(p); return; }
+    g_pool.push_back(FreeBlock{p, bytes});
+    g_pool_bytes += bytes;
+}
+
 static bool g_timing = false;
 static KernelStat g_stats[KS_COUNT];
 bool timing_enabled() { return g_timing; }
@@ -58,10 +106,14 @@ struct tc_corr2 {
     C2Params P{};
     double* xi[4] = {nullptr, nullptr, nullptr, nullptr};
     double *meanr = nullptr, *meanlogr = nullptr, *weight = nullptr, *npairs = nullptr;
-    Corr2Scratch* S = nullptr;
     long long counters[4] = {0, 0, 0, 0};
-    ~tc_corr2() { if (S) delete_corr2_scratch(S); }
 };
+
+// One traversal scratch (work stack, output regions, accumulation slots) for the
+// process, reused by every correlation; library calls that run GPU work hold
+// g_run_m, so they are serialized.
+static std::mutex g_run_m;
+static Corr2Scratch* g_s2 = nullptr;
 
 namespace {
 
@@ -113,8 +165,9 @@ int make_field(bool device, int d, int coords, const double* x, const double* y,
         for (int k = 0; k < 7 && rc == 0; ++k) {
             if (!src[k]) continue;
             if (coords == Flat && k == 2) continue;
-            hipError_t e = hipMalloc(&dev[k], sizeof(double) * n);
-            if (e == hipSuccess) e = hipMemcpy(dev[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice);
+            dev[k] = static_cast<double*>(tc::dev_alloc(sizeof(double) * n));
+            if (!dev[k]) { rc = tc::fail("out of device memory staging catalog columns"); break; }
+            hipError_t e = hipMemcpy(dev[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice);
             if (e != hipSuccess) rc = tc::hip_fail(e, "staging catalog columns");
         }
     } else {
@@ -125,11 +178,12 @@ int make_field(bool device, int d, int coords, const double* x, const double* y,
     if (rc == 0) {
         f = new tc_field;
         f->d = d; f->coords = coords; f->nobj = n;
+        std::lock_guard<std::mutex> run_lock(g_run_m);
         rc = tc::build_tree(d, coords, dev[0], dev[1], dev[2], dev[3], dev[4], dev[5], dev[6], n,
                             minsize, maxsize, sm, brute, mintop, maxtop, f->T);
     }
     if (!device)
-        for (int k = 0; k < 7; ++k) if (dev[k]) (void)hipFree(dev[k]);
+        for (int k = 0; k < 7; ++k) if (dev[k]) tc::dev_free(dev[k], sizeof(double) * n);
     if (rc) { delete f; return rc; }
     *out = f;
     return 0;
@@ -242,7 +296,6 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
     P.bsq = b * b; P.asq = a * a;
     c->xi[0] = xi0; c->xi[1] = xi1; c->xi[2] = xi2; c->xi[3] = xi3;
     c->meanr = meanr; c->meanlogr = meanlogr; c->weight = weight; c->npairs = npairs;
-    c->S = new_corr2_scratch();
     *out = c;
     return 0;
 }
@@ -274,7 +327,9 @@ static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int 
     }
     std::vector<double> out((size_t)7 * nb);
     std::vector<unsigned long long> np(nb), cnt(4);
-    int rc = tc::corr2_run(*c->S, c->d, f1->coords, f1->T.nodes, f1->T.dtop, (int)f1->T.top.size(),
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    if (!g_s2) g_s2 = new_corr2_scratch();
+    int rc = tc::corr2_run(*g_s2, c->d, f1->coords, f1->T.nodes, f1->T.dtop, (int)f1->T.top.size(),
                            f2 ? f2->T.nodes : nullptr, f2 ? f2->T.dtop : nullptr,
                            f2 ? (int)f2->T.top.size() : 0, P, out.data(), np.data(), cnt.data());
     if (rc) return rc;
@@ -462,6 +517,7 @@ int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int met
     const int ntot = P.nbins * P.nbins * (2 * P.maxn + 1);
     std::vector<double> out((size_t)15 * ntot);
     unsigned long long cnt[4] = {0, 0, 0, 0};
+    std::lock_guard<std::mutex> run_lock(g_run_m);
     int rc = tc::corr3_run(c->d, f->coords, f->T.nodes, f->T.nnodes, f->T.dtop,
                            (int)f->T.top.size(), P, out.data(), cnt);
     if (rc) return rc;

@@ -47,6 +47,12 @@ int hip_fail(hipError_t e, const char* what);  // sets the error, returns TC_ERR
 
 hipStream_t stream();   // the library's stream (default priority)
 
+// Device memory for temporaries, kept for reuse across calls (a caching allocator:
+// blocks are returned to a pool instead of hipFree).  The pool holds at most a fixed
+// number of bytes; beyond that, freed blocks are released.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p, size_t bytes);
+
 // Timing of the dominant kernels, recorded with HIP events on stream() when enabled.
 struct KernelStat { double ms = 0; long long launches = 0; };
 bool timing_enabled();

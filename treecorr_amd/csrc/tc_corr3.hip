@@ -808,10 +808,13 @@ __global__ void k_mp_reduce(const double* slots, int nslots, long long n, double
 template <class T>
 struct DBuf {
     T* p = nullptr;
-    ~DBuf() { if (p) (void)hipFree(p); }
+    size_t bytes = 0;
+    ~DBuf() { dev_free(p, bytes); }
     int alloc(size_t n) {
-        if (p) { (void)hipFree(p); p = nullptr; }
-        TC_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)));
+        dev_free(p, bytes);
+        bytes = sizeof(T) * std::max<size_t>(n, 1);
+        p = static_cast<T*>(dev_alloc(bytes));
+        if (!p) { bytes = 0; return fail("out of device memory in the multipole engine"); }
         return 0;
     }
 };

@@ -123,16 +123,29 @@ __device__ __forceinline__ void ordered_sums(long long start, long long end, dou
         double* cur = stg + (NT > 64 ? (bat & 1) * NC * B : 0);
         if (NT > 64 && stager && bat + 1 < nbat) stage(bat + 1, stg + ((bat + 1) & 1) * NC * B);
         if (wave == 0 && lane < NC) {
-            const long long cnt = end - (start + (long long)bat * B) < B
-                                      ? end - (start + (long long)bat * B) : B;
+            const int cnt = (int)(end - (start + (long long)bat * B) < B
+                                      ? end - (start + (long long)bat * B) : B);
             const double* row = cur + lane * B;
+            // Software-pipelined: the next eight values are read while the current
+            // eight are added.  A row always has B entries, so reading ahead of cnt
+            // stays inside it (those values are never added).
+            const int cnt8 = cnt & ~7;
+            double ra[8], rb[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) ra[q] = row[q];
             int j = 0;
-            for (; j + 8 <= cnt; j += 8) {
-                double r[8];
+            while (j < cnt8) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) r[q] = row[j + q];
+                for (int q = 0; q < 8; ++q) rb[q] = row[(j + 8 + q) & (B - 1)];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) acc += r[q];
+                for (int q = 0; q < 8; ++q) acc += ra[q];
+                j += 8;
+                if (j >= cnt8) break;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) ra[q] = row[(j + 8 + q) & (B - 1)];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc += rb[q];
+                j += 8;
             }
             for (; j < cnt; ++j) acc += row[j];
         }
@@ -726,13 +739,16 @@ __global__ void k_children(const Seg* segs, const SegOut* sout, const int* segra
 // ---------------------------------------------------------------------------
 struct DevBuf {
     void* p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    size_t bytes = 0;
+    ~DevBuf() { dev_free(p, bytes); }
     template <class T> T* as() { return static_cast<T*>(p); }
 };
 
 static int dalloc(DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
-    TC_HIP(hipMalloc(&b.p, bytes));
+    b.p = dev_alloc(bytes);
+    if (!b.p) return fail("out of device memory building the tree");
+    b.bytes = bytes;
     return 0;
 }
 
