@@ -104,15 +104,24 @@ __device__ __forceinline__ void ordered_sums(long long start, long long end, dou
     const bool stager = NT > 64 ? wave > 0 : true;
     const int st0 = NT > 64 ? t - 64 : t;
     const int nst = NT > 64 ? NT - 64 : NT;
+    // Each stager handles PER objects of a batch; all their loads are issued before
+    // any value is written to LDS, so one memory latency covers the whole batch.
+    constexpr int NST = NT > 64 ? NT - 64 : NT;
+    constexpr int PER = (B + NST - 1) / NST;
     auto stage = [&](int bat, double* buf) {
         const long long b0 = start + (long long)bat * B;
-        for (int j = st0; j < B; j += nst) {
-            const long long i = b0 + j;
-            if (i < end) {
-                double v[NC];
-                value(i, v);
+        double v[PER][NC];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) buf[c * B + j] = v[c];
+        for (int q = 0; q < PER; ++q) {
+            const int j = st0 + q * nst;
+            if (j < B && b0 + j < end) value(b0 + j, v[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int j = st0 + q * nst;
+            if (j < B && b0 + j < end) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) buf[c * B + j] = v[q][c];
             }
         }
     };
