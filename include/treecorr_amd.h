@@ -91,6 +91,19 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
 int tc_corr2_process_auto(tc_corr2* c, tc_field* f, int dots, int quick, int metric);
 int tc_corr2_process_cross(tc_corr2* c, tc_field* f1, tc_field* f2, int dots, int quick,
                            int metric);
+/* All patch pairs of a patch-based correlation in one device traversal.  Replaces
+ * the Python loop of corr2base.py:855-1050 (_process_all_auto/_process_all_cross)
+ * that calls ProcessAuto/ProcessCross (src/Corr2.cpp:886-993) once per patch pair
+ * through _single_process12 (:836-853).  Pair p correlates fields1[p1[p]] with
+ * fields2[p2[p]]; with fields2 == NULL both sides are fields1 and a pair with
+ * p1[p] == p2[p] is that patch's auto-correlation.  A cross pair whose fields are
+ * entirely out of range is skipped as BaseCorr2::process does (:215-231).
+ * pair_out receives 8*nbins doubles per pair (xi0..xi3, meanr, meanlogr, weight,
+ * npairs; raw sums, zeros for unused xi), and the same sums are added to the
+ * correlation's arrays.  Results equal per-pair process calls to 1e-12. */
+int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
+                           tc_field* const* fields2, int nfield2, const int* p1, const int* p2,
+                           int npair, int quick, int metric, double* pair_out);
 /* BaseCorr2::triviallyZero / TriviallyZero, src/Corr2.cpp:590-601, 1107-1128. */
 int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, double y1,
                             double z1, double s1, double x2, double y2, double z2, double s2,

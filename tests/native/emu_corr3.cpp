@@ -19,6 +19,13 @@ KernelStat& kernel_stat(int which) {
     static KernelStat s[KS_COUNT];
     return s[which];
 }
+void* dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+}
+void dev_free(void* p, size_t) {
+    if (p) (void)hipFree(p);
+}
 }  // namespace tc
 
 extern "C" {

@@ -1,0 +1,286 @@
+"""Patch-based 2-point correlations on the GPU and their covariance estimates.
+
+Every patch pair of a patch-based process() runs in one device traversal
+(tc_corr2_process_pairs).  Checked here:
+  - each results[(i,j)] against the oracle run on the two patch catalogs (npairs
+    exactly, the FP64 sums to 1e-12 normwise), the set of stored pairs, NN's tot;
+  - with bin_slop = 0 (and angle_slop = 0 for shear, unit weights for counts: every
+    pair binned exactly, so sums do not depend on the tree), each jackknife row
+    against the oracle's correlation of the catalog with that patch removed, and the 'mean'/'match' rows against the combination of
+    whole-catalog runs those weights imply.  The covariance is then the textbook
+    jackknife sum over those independently computed rows.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases as C
+
+RTOL = 1e-12
+FLAT, THREED = 1, 2
+
+
+def normwise(a, b):
+    den = np.max(np.abs(b))
+    return 0. if den == 0 else np.max(np.abs(a - b)) / den
+
+
+def grid_labels(x, y, extent, ng):
+    ix = np.minimum((x / extent * ng).astype(int), ng - 1)
+    iy = np.minimum((y / extent * ng).astype(int), ng - 1)
+    return ix * ng + iy
+
+
+def sub(cat, sel):
+    return {k: (None if v is None else np.ascontiguousarray(v[sel])) for k, v in cat.items()}
+
+
+def oracle_raw(O, d, coords, c1, c2, bn, min_top):
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=min_top)
+
+    def fld(c):
+        return O.Field(d, coords, c['x'], c['y'], c['z'], k=c.get('k'), g1=c.get('g1'),
+                       g2=c.get('g2'), w=c['w'], **kw)
+    f1 = fld(c1)
+    f2 = fld(c2) if c2 is not None else None
+    return O.corr2(d, coords, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'],
+                   bn['bin_size'], bn['b'], bn['a'], exact=True)
+
+
+def front_catalog(tc, cat, coords, labels):
+    kw = dict(x=cat['x'], y=cat['y'], w=cat['w'], patch=labels)
+    if coords != FLAT:
+        kw['z'] = cat['z']
+    for k in ('k', 'g1', 'g2'):
+        if cat.get(k) is not None:
+            kw[k] = cat[k]
+    return tc.Catalog(**kw)
+
+
+def corr_obj(tc, d, bn, bin_slop, angle_slop, min_top, **kw):
+    cls = {0: tc.NNCorrelation, 1: tc.KKCorrelation, 4: tc.GGCorrelation}[d]
+    return cls(nbins=bn['nbins'], min_sep=bn['min_sep'], max_sep=bn['max_sep'],
+               bin_slop=bin_slop, angle_slop=angle_slop, min_top=min_top, **kw)
+
+
+def check_pair(res, ref, nxi):
+    np.testing.assert_array_equal(res.npairs, ref['npairs'])
+    for q in range(nxi):
+        assert normwise(res._xi[q], ref['xi%d' % q]) < RTOL, q
+    for k in ('meanr', 'meanlogr', 'weight'):
+        assert normwise(getattr(res, k), ref[k]) < RTOL, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d,coords,bs', [(4, THREED, 0.1), (1, FLAT, 0.5), (0, THREED, 0.0)])
+def test_gpu_patch_auto_pairs_match_oracle(gpu, oracle_lib, d, coords, bs):
+    O = oracle_lib
+    ng, min_top = 4, 3
+    cat, scale = C.make_catalog(coords, 6000, 300 + d + coords, 0.2)
+    labels = grid_labels(cat['x'], cat['y'], scale, ng)
+    bn = C.binning(scale, 12, 0.01, 0.3, bs, 0.1 if d == 4 else 1.0)
+    corr = corr_obj(gpu, d, bn, bs, bn['a'], min_top)
+    corr.process(front_catalog(gpu, cat, coords, labels), finalize=False)
+    npatch = ng * ng
+    assert corr.npatch1 == corr.npatch2 == npatch
+    nxi = {0: 0, 1: 1, 4: 4}[d]
+    tot_w = np.zeros(bn['nbins'])
+    tot_np = np.zeros(bn['nbins'])
+    nskip = 0
+    for i in range(npatch):
+        for j in range(i, npatch):
+            ci = sub(cat, labels == i)
+            cj = sub(cat, labels == j)
+            if i == j:
+                ref = oracle_raw(O, d, coords, ci, None, bn, min_top)
+            else:
+                ref = oracle_raw(O, d, coords, ci, cj, bn, min_top)
+            if (i, j) in corr.results:
+                check_pair(corr.results[(i, j)], ref, nxi)
+            else:
+                nskip += 1
+                assert i != j and not np.any(ref['weight']), (i, j)
+            tot_w += ref['weight']
+            tot_np += ref['npairs']
+    # far patch pairs are skipped before the traversal (NN keeps them with their tot)
+    if d == 0:
+        assert nskip == 0 and len(corr.results) == npatch * (npatch + 1) // 2
+        w = np.array([np.sum(cat['w'][labels == p]) for p in range(npatch)])
+        assert math.isclose(corr.tot, 0.5 * np.sum(w) ** 2, rel_tol=1e-12)
+    else:
+        assert nskip > 0
+    np.testing.assert_array_equal(corr.npairs, tot_np)
+    assert normwise(corr.weight, tot_w) < RTOL
+
+
+@pytest.mark.gpu
+def test_gpu_patch_cross_pairs_match_oracle(gpu, oracle_lib):
+    O = oracle_lib
+    ng, min_top = 3, 3
+    c1, scale = C.make_catalog(THREED, 4000, 321, 0.2)
+    c2, _ = C.make_catalog(THREED, 3000, 322, 0.0)
+    l1 = grid_labels(c1['x'], c1['y'], scale, ng)
+    l2 = grid_labels(c2['x'], c2['y'], scale, ng)
+    bn = C.binning(scale, 10, 0.01, 0.3, 0.2, 1.0)
+    kk = corr_obj(gpu, 1, bn, 0.2, 1.0, min_top)
+    kk.process(front_catalog(gpu, c1, THREED, l1), front_catalog(gpu, c2, THREED, l2),
+               finalize=False)
+    npatch = ng * ng
+    for i in range(npatch):
+        for j in range(npatch):
+            ref = oracle_raw(O, 1, THREED, sub(c1, l1 == i), sub(c2, l2 == j), bn, min_top)
+            if (i, j) in kk.results:
+                check_pair(kk.results[(i, j)], ref, 1)
+            else:
+                assert i != j and not np.any(ref['weight'])
+    for i in range(npatch):     # the i == j pairs are always stored
+        assert (i, i) in kk.results
+
+
+@pytest.mark.gpu
+def test_gpu_patch_pairs_small_chunks(gpu, oracle_lib):
+    """Batched patch pairs across many traversal launches."""
+    from treecorr_amd import _lib
+    O = oracle_lib
+    ng, min_top = 3, 4
+    cat, scale = C.make_catalog(THREED, 5000, 331, 0.3)
+    labels = grid_labels(cat['x'], cat['y'], scale, ng)
+    bn = C.binning(scale, 10, 0.01, 0.4, 0.1, 0.1)
+    gg = corr_obj(gpu, 4, bn, 0.1, 0.1, min_top)
+    _lib.check(_lib.lib().tc_set_traversal_chunk(3000))
+    try:
+        gg.process(front_catalog(gpu, cat, THREED, labels), finalize=False)
+    finally:
+        _lib.check(_lib.lib().tc_set_traversal_chunk(0))
+    for (i, j), res in gg.results.items():
+        ref = oracle_raw(O, 4, THREED, sub(cat, labels == i),
+                         None if i == j else sub(cat, labels == j), bn, min_top)
+        check_pair(res, ref, 4)
+
+
+def _xi_gg(raw):
+    w = raw['weight']
+    m = w != 0
+    xip = np.zeros_like(w)
+    xim = np.zeros_like(w)
+    xip[m] = raw['xi0'][m] / w[m]
+    xim[m] = raw['xi2'][m] / w[m]
+    return np.concatenate([xip, xim])
+
+
+@pytest.mark.gpu
+def test_gpu_jackknife_gg_rows_are_catalogs_without_patch(gpu, oracle_lib):
+    O = oracle_lib
+    ng, min_top = 3, 3
+    cat, scale = C.make_catalog(THREED, 3000, 341, 0.2)
+    labels = grid_labels(cat['x'], cat['y'], scale, ng)
+    # bin_slop = angle_slop = 0: every pair is binned and projected on its own, so the
+    # sums do not depend on the tree
+    bn = C.binning(scale, 8, 0.02, 0.3, 0.0, 0.0)
+    gg = corr_obj(gpu, 4, bn, 0.0, 0.0, min_top, var_method='jackknife')
+    gg.process(front_catalog(gpu, cat, THREED, labels))
+    npatch = ng * ng
+    rows = np.array([_xi_gg(oracle_raw(O, 4, THREED, sub(cat, labels != k), None, bn, min_top))
+                     for k in range(npatch)])
+    v, w = gg.build_cov_design_matrix('jackknife', cross_patch_weight='simple')
+    assert v.shape == rows.shape
+    for k in range(npatch):
+        assert normwise(v[k], rows[k]) < 1e-11, k
+    dv = rows - rows.mean(axis=0)
+    cov = (1. - 1. / npatch) * dv.T.dot(dv)
+    got = gg.estimate_cov('jackknife', cross_patch_weight='simple')
+    assert normwise(got, cov) < 1e-8
+    # varxip/varxim are the jackknife diagonal
+    np.testing.assert_allclose(gg.varxip, np.diag(got)[:bn['nbins']], rtol=1e-14)
+
+    # 'mean' and 'match' weight the pairs with one point in patch k by wc:
+    # S = (1-wc) S(catalog without k) + wc (S(all) - S(patch k alone)).
+    full = oracle_raw(O, 4, THREED, cat, None, bn, min_top)
+    for cpw in ('mean', 'match'):
+        wc = 0.5 if cpw == 'mean' else 1 - npatch / (2 + 2 ** 0.5 * (npatch - 1))
+        v, _ = gg.build_cov_design_matrix('jackknife', cross_patch_weight=cpw)
+        for k in range(npatch):
+            wo = oracle_raw(O, 4, THREED, sub(cat, labels != k), None, bn, min_top)
+            kk_ = oracle_raw(O, 4, THREED, sub(cat, labels == k), None, bn, min_top)
+            comb = {key: (1 - wc) * wo[key] + wc * (full[key] - kk_[key])
+                    for key in ('xi0', 'xi2', 'weight')}
+            assert normwise(v[k], _xi_gg(comb)) < 1e-11, (cpw, k)
+
+
+@pytest.mark.gpu
+def test_gpu_jackknife_nn_landy_szalay(gpu, oracle_lib):
+    """DD, RR, DR with patches; jackknife rows are Landy-Szalay on the catalogs with
+    one patch removed."""
+    O = oracle_lib
+    ng, min_top = 3, 3
+    dat, scale = C.make_catalog(FLAT, 2500, 351, 0.3)
+    ran, _ = C.make_catalog(FLAT, 5000, 352, 0.0)
+    # unit weights: a cell's weight (a float) is then exact, so with bin_slop = 0 the
+    # sums do not depend on the tree
+    dat['w'][:] = 1.
+    ran['w'][:] = 1.
+    ld = grid_labels(dat['x'], dat['y'], scale, ng)
+    lr = grid_labels(ran['x'], ran['y'], scale, ng)
+    bn = C.binning(scale, 8, 0.01, 0.25, 0.0, 1.0)
+    D = front_catalog(gpu, dat, FLAT, ld)
+    R = front_catalog(gpu, ran, FLAT, lr)
+    kw = dict(var_method='jackknife', cross_patch_weight='simple')
+    dd = corr_obj(gpu, 0, bn, 0.0, 1.0, min_top, **kw)
+    rr = corr_obj(gpu, 0, bn, 0.0, 1.0, min_top, **kw)
+    dr = corr_obj(gpu, 0, bn, 0.0, 1.0, min_top, **kw)
+    dd.process(D)
+    rr.process(R)
+    dr.process(D, R)
+    xi, varxi = dd.calculateXi(rr=rr, dr=dr)
+    npatch = ng * ng
+
+    def ls(dsel, rsel):
+        wd, wr = np.sum(dat['w'][dsel]), np.sum(ran['w'][rsel])
+        ddk = oracle_raw(O, 0, FLAT, sub(dat, dsel), None, bn, min_top)['weight']
+        rrk = oracle_raw(O, 0, FLAT, sub(ran, rsel), None, bn, min_top)['weight']
+        drk = oracle_raw(O, 0, FLAT, sub(dat, dsel), sub(ran, rsel), bn, min_top)['weight']
+        dd_tot, rr_tot, dr_tot = 0.5 * wd ** 2, 0.5 * wr ** 2, wd * wr
+        rrf, drf = dd_tot / rr_tot, dd_tot / dr_tot
+        den = rrk * rrf
+        out = ddk - 2. * drk * drf + den
+        den[den == 0] = 1.
+        return out / den
+
+    np.testing.assert_allclose(xi, ls(np.ones(len(ld), bool), np.ones(len(lr), bool)),
+                               rtol=1e-10, atol=1e-13)
+    rows = np.array([ls(ld != k, lr != k) for k in range(npatch)])
+    v, _ = dd.build_cov_design_matrix('jackknife')
+    for k in range(npatch):
+        assert normwise(v[k], rows[k]) < 1e-10, k
+    dv = rows - rows.mean(axis=0)
+    cov = (1. - 1. / npatch) * dv.T.dot(dv)
+    assert normwise(dd.cov, cov) < 1e-7
+    np.testing.assert_allclose(varxi, np.diag(dd.cov), rtol=1e-14)
+
+
+@pytest.mark.gpu
+def test_gpu_sample_cov_rows(gpu, oracle_lib):
+    """Sample covariance row k sums the stored pairs (k, j) (cross_patch_weight simple)."""
+    O = oracle_lib
+    ng, min_top = 3, 3
+    cat, scale = C.make_catalog(THREED, 3000, 361, 0.2)
+    labels = grid_labels(cat['x'], cat['y'], scale, ng)
+    bn = C.binning(scale, 8, 0.02, 0.3, 0.0, 1.0)
+    kk = corr_obj(gpu, 1, bn, 0.0, 1.0, min_top)
+    kk.process(front_catalog(gpu, cat, THREED, labels))
+    v, w = kk.build_cov_design_matrix('sample', cross_patch_weight='simple')
+    for k in range(ng * ng):
+        s = np.zeros(bn['nbins'])
+        ws = np.zeros(bn['nbins'])
+        for j in range(k, ng * ng):
+            if (k, j) in kk.results:
+                ref = oracle_raw(O, 1, THREED, sub(cat, labels == k),
+                                 None if j == k else sub(cat, labels == j), bn, min_top)
+                s += ref['xi0']
+                ws += ref['weight']
+        m = ws != 0
+        row = np.zeros_like(s)
+        row[m] = s[m] / ws[m]
+        assert normwise(v[k], row) < 1e-11, k
+        assert math.isclose(w[k], np.sum(ws), rel_tol=1e-12)

@@ -18,6 +18,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 _ll = ctypes.c_longlong
 _i = ctypes.c_int
+_ip = ctypes.POINTER(ctypes.c_int)
 _d = ctypes.c_double
 
 # name -> (restype, argtypes)
@@ -40,6 +41,8 @@ SIGNATURES = {
                         [ctypes.POINTER(_vp)]),
     'tc_corr2_process_auto': (_i, [_vp, _vp, _i, _i, _i]),
     'tc_corr2_process_cross': (_i, [_vp, _vp, _vp, _i, _i, _i]),
+    'tc_corr2_process_pairs': (_i, [_vp, ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp), _i, _ip,
+                                    _ip, _i, _i, _i, _dp]),
     'tc_corr2_trivially_zero': (_i, [_vp, _i, _i] + [_d] * 8 + [ctypes.POINTER(_i)]),
     'tc_corr2_counters': (_i, [_vp, ctypes.POINTER(_ll)]),
     'tc_corr2_destroy': (_i, [_vp]),

@@ -13,6 +13,7 @@ import sys
 import numpy as np
 
 from . import _lib
+from . import cov as _cov
 from .catalog import Catalog, calculateVarG, calculateVarK
 from .config import parse_unit
 from .field import coord_enum
@@ -20,6 +21,46 @@ from .util import set_omp_threads
 
 LOG = 0
 EUCLIDEAN = 1
+
+
+class _PairSums:
+    """Raw (unfinalized) sums of one patch pair.
+
+    TreeCorr keeps a Corr2 copy per patch pair in results[(i,j)]
+    (treecorr/corr2base.py:846-850, with NN's tot, nncorrelation.py:194-204); the
+    covariance estimators only read these arrays, so this holds just them.
+    row is the (8, nbins) block the engine writes per pair: xi0..xi3, meanr,
+    meanlogr, weight, npairs.
+    """
+    __slots__ = ('_xi', 'meanr', 'meanlogr', 'weight', 'npairs', 'tot')
+
+    def __init__(self, nbins, nxi, row=None, tot=0.):
+        if row is None:
+            row = np.zeros((8, nbins))
+        self._xi = [np.array(row[q]) if q < nxi else None for q in range(4)]
+        self.meanr = np.array(row[4])
+        self.meanlogr = np.array(row[5])
+        self.weight = np.array(row[6])
+        self.npairs = np.array(row[7])
+        self.tot = tot
+
+    @property
+    def nonzero(self):
+        # corr2base.py:766-770
+        return bool(np.any(self.weight))
+
+    _nonzero = nonzero
+
+    def __iadd__(self, other):
+        for a, b in zip(self._xi, other._xi):
+            if a is not None:
+                a += b
+        self.meanr += other.meanr
+        self.meanlogr += other.meanlogr
+        self.weight += other.weight
+        self.npairs += other.npairs
+        self.tot += other.tot
+        return self
 
 
 class Corr2:
@@ -104,6 +145,10 @@ class Corr2:
         if cfg.get('period') or cfg.get('xperiod') or cfg.get('yperiod') or cfg.get('zperiod'):
             raise NotImplementedError("periodic metrics are not supported by treecorr_amd")
         self.var_method = cfg.get('var_method', 'shot')
+        self.num_bootstrap = int(cfg.get('num_bootstrap', 500))
+        self.cross_patch_weight = cfg.get('cross_patch_weight')
+        if self.cross_patch_weight not in (None, 'simple', 'mean', 'match', 'geom'):
+            raise ValueError("Invalid cross_patch_weight %s" % self.cross_patch_weight)
         self.coords = None
         self.metric = None
         self.results = {}
@@ -227,18 +272,6 @@ class Corr2:
                                                        ctypes.byref(r)))
         return bool(r.value)
 
-    def _snapshot(self):
-        return dict(xi=[None if a is None else a.copy() for a in self._xi], meanr=self.meanr.copy(),
-                    meanlogr=self.meanlogr.copy(), weight=self.weight.copy(),
-                    npairs=self.npairs.copy())
-
-    def _delta_since(self, snap):
-        d = self._snapshot()
-        for key in ('meanr', 'meanlogr', 'weight', 'npairs'):
-            d[key] = d[key] - snap[key]
-        d['xi'] = [None if a is None else a - b for a, b in zip(d['xi'], snap['xi'])]
-        return d
-
     def process(self, cat1, cat2=None, metric=None, num_threads=None, comm=None, low_mem=False,
                 initialize=True, finalize=True, patch_method='global', corr_only=False):
         """Compute the correlation function (corr2base.py:1162-1272)."""
@@ -286,19 +319,55 @@ class Corr2:
     def _calculate_var(self, cats):
         return None
 
-    def _single_pair(self, c1, c2, ij, metric, num_threads, corr_only, force=False):
-        # corr2base.py:836-853: skip trivially-zero patch pairs, keep per-pair results
-        snap = self._snapshot()
-        if c2 is not None and not self._trivially_zero(c1, c2):
-            self.process_cross(c1, c2, metric=metric, num_threads=num_threads,
-                               corr_only=corr_only)
-        delta = self._delta_since(snap)
-        if np.any(delta['weight']) or force:
-            self.results[ij] = delta
-        self._add_tot(ij, c1, c2)
+    # ---- patch pairs (corr2base.py:836-1050), all in one device traversal ----
+    def _pair_tot(self, c1, c2):
+        """The tot a processed patch pair adds (only NN keeps one)."""
+        return 0.
+
+    def _tot_add(self, tot):
+        pass
 
     def _add_tot(self, ij, c1, c2):
+        # corr2base.py:772-774: a no-op except for NN
         pass
+
+    def _patch_fields(self, cats, brute):
+        min_size, max_size = self._get_minmax_size()
+        return [self._field(c, min_size=min_size, max_size=max_size,
+                            split_method=self.split_method, brute=brute, min_top=self.min_top,
+                            max_top=self.max_top, coords=self.coords) for c in cats]
+
+    def _run_pairs(self, fields1, fields2, jobs, corr_only):
+        """Raw sums of every job (i, j) over fields1[i] x fields2[j] (fields2 None: the
+        auto-correlation pairs of fields1), as an (njob, 8, nbins) array."""
+        out = np.zeros((len(jobs), 8, self.nbins))
+        p1 = np.ascontiguousarray([a for a, _ in jobs], dtype=np.int32)
+        p2 = np.ascontiguousarray([b for _, b in jobs], dtype=np.int32)
+        a1 = (ctypes.c_void_p * len(fields1))(*[f.data.value for f in fields1])
+        a2 = None if fields2 is None else \
+            (ctypes.c_void_p * len(fields2))(*[f.data.value for f in fields2])
+        ip = ctypes.POINTER(ctypes.c_int)
+        _lib.check(_lib.lib().tc_corr2_process_pairs(
+            self.corr, a1, len(fields1), a2, 0 if fields2 is None else len(fields2),
+            p1.ctypes.data_as(ip), p2.ctypes.data_as(ip), len(jobs), int(bool(corr_only)),
+            EUCLIDEAN, _lib.dptr(out)))
+        self._read_counters()
+        return out
+
+    def _record_pair(self, ij, row, c1, c2, force):
+        # corr2base.py:836-853 (_single_process12) after the pair was processed (row) or
+        # skipped as trivially zero (row None).  The sums themselves were already added
+        # to this object's arrays by the engine.
+        ps = _PairSums(self.nbins, self._nxi, row,
+                       tot=self._pair_tot(c1, c2) if row is not None else 0.)
+        if ps.nonzero or force:
+            if ij in self.results and self.results[ij].nonzero:
+                self.results[ij] += ps
+            else:
+                self.results[ij] = ps
+            self._tot_add(ps.tot)
+        else:
+            self._add_tot(ij, c1, c2)
 
     def _process_all_auto(self, cat1, metric, num_threads, corr_only):
         # corr2base.py:855-964 (global patch method, single process)
@@ -306,18 +375,37 @@ class Corr2:
             self.process_auto(cat1[0], metric=metric, num_threads=num_threads,
                               corr_only=corr_only)
             return
-        self.npatch1 = self.npatch2 = len(cat1)
+        if self.npatch1 == 1:
+            self.npatch1 = self.npatch2 = cat1[0].npatch if cat1[0].npatch != 1 else len(cat1)
         self._set_metric(metric, cat1[0].coords)
+        if num_threads is not None:
+            set_omp_threads(num_threads)
+        self._corr_only = corr_only
+        idx = [c._single_patch if c._single_patch is not None else ii
+               for ii, c in enumerate(cat1)]
+        order = []    # (ii, jj, processed) in the reference's loop order
         for ii, c1 in enumerate(cat1):
-            i = c1._single_patch if c1._single_patch is not None else ii
-            snap = self._snapshot()
-            self.process_auto(c1, metric=metric, num_threads=num_threads, corr_only=corr_only)
-            self.results[(i, i)] = self._delta_since(snap)
+            order.append((ii, ii, True))
             for jj in range(len(cat1) - 1, -1, -1):
-                c2 = cat1[jj]
-                j = c2._single_patch if c2._single_patch is not None else jj
-                if i < j:
-                    self._single_pair(c1, c2, (i, j), metric, num_threads, corr_only)
+                if idx[ii] < idx[jj]:
+                    order.append((ii, jj, not self._trivially_zero(c1, cat1[jj])))
+        jobs = [(a, b) for a, b, run in order if run]
+        fields = self._patch_fields(cat1, bool(self.brute))
+        out = self._run_pairs(fields, None, jobs, corr_only)
+        rows = {job: out[k] for k, job in enumerate(jobs)}
+        for ii, jj, run in order:
+            c1, c2 = cat1[ii], cat1[jj]
+            ij = (idx[ii], idx[jj])
+            if ii == jj:
+                ps = _PairSums(self.nbins, self._nxi, rows[(ii, ii)],
+                               tot=self._pair_tot(c1, None))
+                if ij not in self.results:
+                    self.results[ij] = ps
+                else:
+                    self.results[ij] += ps
+                self._tot_add(ps.tot)
+            else:
+                self._record_pair(ij, rows.get((ii, jj)), c1, c2, False)
 
     def _process_all_cross(self, cat1, cat2, metric, num_threads, corr_only):
         # corr2base.py:966-1050 (global patch method, single process)
@@ -325,15 +413,32 @@ class Corr2:
             self.process_cross(cat1[0], cat2[0], metric=metric, num_threads=num_threads,
                                corr_only=corr_only)
             return
-        self.npatch1, self.npatch2 = len(cat1), len(cat2)
+        if self.npatch1 == 1:
+            self.npatch1 = cat1[0].npatch if cat1[0].npatch != 1 else len(cat1)
+        if self.npatch2 == 1:
+            self.npatch2 = cat2[0].npatch if cat2[0].npatch != 1 else len(cat2)
+        if self.npatch1 != self.npatch2 and self.npatch1 != 1 and self.npatch2 != 1:
+            raise RuntimeError("Cross correlation requires both catalogs use the same patches.")
+        n1, n2 = self.npatch1, self.npatch2
         self._set_metric(metric, cat1[0].coords, cat2[0].coords)
-        n1, n2 = len(cat1), len(cat2)
-        for ii, c1 in enumerate(cat1):
-            i = c1._single_patch if c1._single_patch is not None else ii
-            for jj, c2 in enumerate(cat2):
-                j = c2._single_patch if c2._single_patch is not None else jj
-                self._single_pair(c1, c2, (i, j), metric, num_threads, corr_only,
-                                  force=(i == j or n1 == 1 or n2 == 1))
+        if num_threads is not None:
+            set_omp_threads(num_threads)
+        self._corr_only = corr_only
+        idx1 = [c._single_patch if c._single_patch is not None else ii
+                for ii, c in enumerate(cat1)]
+        idx2 = [c._single_patch if c._single_patch is not None else jj
+                for jj, c in enumerate(cat2)]
+        order = [(ii, jj, not self._trivially_zero(c1, c2))
+                 for ii, c1 in enumerate(cat1) for jj, c2 in enumerate(cat2)]
+        jobs = [(a, b) for a, b, run in order if run]
+        f1 = self._patch_fields(cat1, self.brute is True or self.brute == 1)
+        f2 = self._patch_fields(cat2, self.brute is True or self.brute == 2)
+        out = self._run_pairs(f1, f2, jobs, corr_only)
+        rows = {job: out[k] for k, job in enumerate(jobs)}
+        for ii, jj, run in order:
+            i, j = idx1[ii], idx2[jj]
+            self._record_pair((i, j), rows.get((ii, jj)), cat1[ii], cat2[jj],
+                              i == j or n1 == 1 or n2 == 1)
 
     def _finalize(self):
         # corr2base.py:1274-1294
@@ -364,15 +469,159 @@ class Corr2:
     def getWeight(self):
         return self.weight.ravel()
 
+    def getStat(self):
+        # corr2base.py:1052-1059
+        return self.xi.ravel()
+
     def _calculate_varxi(self, n):
-        # corr2base.py:1354-1358 with _cov_shot :2393-2408
+        # corr2base.py:1354-1358
         self._varxi = [np.zeros(self.nbins) for _ in range(n)]
         if self._var_num != 0:
-            v = self.getWeight().copy()
-            mask = v != 0
-            v[mask] = self._var_num / v[mask]
             for i in range(n):
-                self._varxi[i][:] = v[i * self.nbins:(i + 1) * self.nbins]
+                self._varxi[i][:] = self.cov_diag[i * self.nbins:(i + 1) * self.nbins]
+
+    # ---- covariance (corr2base.py:655-680, 1296-1352, 1360-1540, 1733-1850) ----
+    def _clear(self):
+        for a in self._xi:
+            if a is not None:
+                a[:] = 0
+        self.meanr[:] = 0
+        self.meanlogr[:] = 0
+        self.weight[:] = 0
+        self.npairs[:] = 0
+
+    def copy(self):
+        """A copy with its own arrays (and its own engine object, made on use)."""
+        ret = self.__class__.__new__(self.__class__)
+        for key, item in self.__dict__.items():
+            ret.__dict__[key] = item.copy() if isinstance(item, np.ndarray) else item
+        ret._xi = [None if a is None else a.copy() for a in self._xi]
+        ret._corr = None
+        return ret
+
+    @property
+    def nonzero(self):
+        return bool(np.any(self.weight))
+
+    def _sum(self, others, corr_only):
+        # corr2base.py:1333-1352: self = sum of w * other
+        def x(a, w):
+            return a if w == 1 else a * w
+        for q in range(4):
+            if self._xi[q] is not None:
+                np.sum([x(c._xi[q], w) for c, w in others], axis=0, out=self._xi[q])
+        np.sum([x(c.weight, w) for c, w in others], axis=0, out=self.weight)
+        if not corr_only:
+            np.sum([x(c.meanr, w) for c, w in others], axis=0, out=self.meanr)
+            np.sum([x(c.meanlogr, w) for c, w in others], axis=0, out=self.meanlogr)
+            np.sum([x(c.npairs, w) for c, w in others], axis=0, out=self.npairs)
+        self._varxi = None
+        self._cov = None
+
+    def _get_npatch(self):
+        return max(self.npatch1, self.npatch2)
+
+    def _calculate_xi_from_pairs(self, pairs, corr_only):
+        # corr2base.py:1737-1743
+        self._sum([(self.results[(i, j)], w) for i, j, w in pairs], corr_only)
+        self._finalize()
+
+    @property
+    def _ok(self):
+        # corr2base.py:1852-1859
+        ok = np.zeros((self.npatch1, self.npatch2), dtype=bool)
+        for (i, j) in self.results:
+            ok[i, j] = True
+        return ok
+
+    def _keep_ok(self, pairs):
+        ok = self._ok
+        return [(i, j, w) for i, j, w in pairs if ok[i, j] and w != 0]
+
+    def _jackknife_pairs(self, cross_patch_weight):
+        # JackknifePairIterator, corr2base.py:1787-1823: row k drops patch k
+        keys = list(self.results.keys())
+        n = self.npatch1 if self.npatch1 != 1 else self.npatch2
+        cpw = cross_patch_weight
+        rows = []
+        for k in range(n):
+            if self.npatch2 == 1:
+                rows.append([(i, j, 1) for i, j in keys if i != k])
+            elif self.npatch1 == 1:
+                rows.append([(i, j, 1) for i, j in keys if j != k])
+            elif cpw == 'simple':
+                assert self.npatch1 == self.npatch2
+                rows.append([(i, j, 1) for i, j in keys if i != k and j != k])
+            elif cpw in ('mean', 'match'):
+                assert self.npatch1 == self.npatch2
+                # match: eqn. 27 of Mohammad & Percival (2022), w = 1 - alpha
+                wc = 0.5 if cpw == 'mean' else \
+                    1 - self.npatch1 / (2 + 2 ** 0.5 * (self.npatch1 - 1))
+                rows.append([(i, j, wc if i == k or j == k else 1)
+                             for i, j in keys if i != k or j != k])
+            else:
+                raise ValueError(f"cross_patch_weight = {cpw} is invalid for jackknife")
+        return rows
+
+    def _sample_pairs(self, cross_patch_weight):
+        # SamplePairIterator, corr2base.py:1825-1850: row k keeps patch k
+        keys = list(self.results.keys())
+        n = self.npatch1 if self.npatch1 != 1 else self.npatch2
+        cpw = cross_patch_weight
+        rows = []
+        for k in range(n):
+            if self.npatch2 == 1:
+                rows.append([(i, j, 1) for i, j in keys if i == k])
+            elif self.npatch1 == 1:
+                rows.append([(i, j, 1) for i, j in keys if j == k])
+            elif cpw == 'simple':
+                assert self.npatch1 == self.npatch2
+                rows.append([(i, j, 1) for i, j in keys if i == k])
+            elif cpw == 'mean':
+                assert self.npatch1 == self.npatch2
+                rows.append([(i, j, 0.5 if i != k or j != k else 1)
+                             for i, j in keys if i == k or j == k])
+            else:
+                raise ValueError(f"cross_patch_weight = {cpw} is invalid for sample")
+        return rows
+
+    def _check_cpw(self, method, cross_patch_weight):
+        # corr2base.py:1931-1948 (the default's warning is not repeated here)
+        if cross_patch_weight is None:
+            cross_patch_weight = self.cross_patch_weight
+        if cross_patch_weight is None:
+            cross_patch_weight = 'simple'
+        return cross_patch_weight
+
+    def estimate_cov(self, method, *, func=None, comm=None, num_bootstrap=None,
+                     cross_patch_weight=None):
+        """Covariance estimate: 'shot', 'jackknife' or 'sample' (corr2base.py:1360-1482)."""
+        all_func = None if func is None else (lambda corrs: func(corrs[0]))
+        return _cov.estimate_multi_cov([self], method, func=all_func, comm=comm,
+                                       num_bootstrap=num_bootstrap,
+                                       cross_patch_weight=cross_patch_weight)
+
+    def build_cov_design_matrix(self, method, *, func=None, comm=None, num_bootstrap=None,
+                                cross_patch_weight=None):
+        """Design matrix behind a covariance estimate (corr2base.py:1484-1540)."""
+        all_func = None if func is None else (lambda corrs: func(corrs[0]))
+        return _cov.build_multi_cov_design_matrix([self], method, func=all_func, comm=comm,
+                                                  num_bootstrap=num_bootstrap,
+                                                  cross_patch_weight=cross_patch_weight)
+
+    @property
+    def cov(self):
+        # corr2base.py:660-669
+        if self._cov is None:
+            self._cov = self.estimate_cov(self.var_method)
+        return np.diag(self._cov) if self._cov.ndim == 1 else self._cov
+
+    @property
+    def cov_diag(self):
+        # corr2base.py:671-682
+        if self._cov is None:
+            self._cov = self.estimate_cov(self.var_method)
+        return self._cov if self._cov.ndim == 1 else self._cov.diagonal()
 
 
 class NNCorrelation(Corr2):
@@ -387,10 +636,24 @@ class NNCorrelation(Corr2):
         self.tot = 0.
         self.xi = None
         self.varxi = None
+        self._rr = self._dr = self._rd = None
+        self._rr_weight = None
 
     def clear(self):
         super().clear()
         self.tot = 0.
+
+    def _clear(self):
+        super()._clear()
+        self.tot = 0.
+
+    def copy(self):
+        # nncorrelation.py:97-108
+        ret = super().copy()
+        for nm in ('_rr', '_dr', '_rd'):
+            if getattr(self, nm) is not None:
+                setattr(ret, nm, getattr(self, nm).copy())
+        return ret
 
     def process_auto(self, cat, **kw):
         super().process_auto(cat, **kw)
@@ -400,15 +663,29 @@ class NNCorrelation(Corr2):
         super().process_cross(cat1, cat2, **kw)
         self.tot += cat1.sumw * cat2.sumw
 
-    def _add_tot(self, ij, c1, c2):
-        # nncorrelation.py:194-204: skipped pairs still count toward tot
-        pass
+    def _pair_tot(self, c1, c2):
+        # process_auto/process_cross add these (nncorrelation.py:138-155)
+        return 0.5 * c1.sumw ** 2 if c2 is None else c1.sumw * c2.sumw
 
-    def _single_pair(self, c1, c2, ij, metric, num_threads, corr_only, force=False):
-        tz = c2 is not None and self._trivially_zero(c1, c2)
-        super()._single_pair(c1, c2, ij, metric, num_threads, corr_only, force)
-        if tz or c2 is None:
-            self.tot += c1.sumw * (c2.sumw if c2 is not None else 0.)
+    def _tot_add(self, tot):
+        self.tot += tot
+
+    def _add_tot(self, ij, c1, c2):
+        # nncorrelation.py:194-204: a pair without weight still counts toward tot and
+        # stays in results with its tot.
+        tot = c1.sumw * c2.sumw
+        self.tot += tot
+        self.results[ij] = _PairSums(self.nbins, 0, None, tot=tot)
+
+    def _sum(self, others, corr_only):
+        # nncorrelation.py:179-192
+        tot = np.sum([c.tot * w for c, w in others])
+        others = [(c, w) for c, w in others if c.nonzero]
+        if len(others) == 0:
+            self._clear()
+        else:
+            super()._sum(others, corr_only)
+        self.tot = tot
 
     def finalize(self):
         self._finalize()
@@ -417,11 +694,28 @@ class NNCorrelation(Corr2):
         mean_np = np.mean(self.npairs)
         return 1 if mean_np == 0 else np.mean(self.weight) / mean_np
 
+    def getStat(self):
+        # nncorrelation.py:210-216
+        if self._rr is None:
+            raise RuntimeError("You need to call calculateXi before calling estimate_cov.")
+        return self.xi.ravel()
+
+    def getWeight(self):
+        # nncorrelation.py:218-228
+        if self._rr_weight is not None:
+            return self._rr_weight.ravel()
+        return self.tot
+
     def calculateXi(self, *, rr, dr=None, rd=None):
-        """xi from DD (self) and RR [, DR, RD] (nncorrelation.py:230-340), shot variance."""
+        """xi from DD (self) and RR [, DR, RD] and its variance by var_method
+        (nncorrelation.py:230-393)."""
         if rr.tot == 0:
             raise ValueError("rr has tot=0.")
         rrf = self.tot / rr.tot
+        if dr is not None and dr.tot == 0:
+            raise ValueError("dr has tot=0.")
+        if rd is not None and rd.tot == 0:
+            raise ValueError("rd has tot=0.")
         denom = rr.weight * rrf
         if dr is None and rd is None:
             xi = self.weight - denom
@@ -431,9 +725,8 @@ class NNCorrelation(Corr2):
             xi = self.weight - 2. * dr.weight * (self.tot / dr.tot) + denom
         else:
             xi = self.weight - dr.weight * (self.tot / dr.tot) - rd.weight * (self.tot / rd.tot) + denom
-        denom = denom.copy()
         denom[rr.weight == 0] = 1.
-        xi = xi / denom
+        self.xi = xi / denom
         ddw = self._mean_weight()
         rrw = rr._mean_weight()
         if dr is None and rd is None:
@@ -447,12 +740,82 @@ class NNCorrelation(Corr2):
                             (self.tot / dr.tot) * dr._mean_weight() / ddw + rrf * rrw / ddw)
         self._var_num = ddw * varxi_factor ** 2
         self._rr_weight = rr.weight * rrf
-        w = self._rr_weight.copy()
-        mask = w != 0
-        varxi = np.zeros(self.nbins)
-        varxi[mask] = self._var_num / w[mask]
-        self.xi, self.varxi = xi, varxi
-        return xi, varxi
+        self._rr, self._dr, self._rd = rr, dr, rd
+        if len(self.results) > 0:
+            # nncorrelation.py:349-385: same patches, and a zero entry for every pair
+            # that RR/DR/RD have and DD lacks.
+            if rr.npatch1 != 1 and rr.npatch2 != 1:
+                if rr.npatch1 != self.npatch1 or rr.npatch2 != self.npatch2:
+                    raise RuntimeError("If using patches, RR must be run with the same patches "
+                                       "as DD")
+            if dr is not None and (len(dr.results) == 0 or dr.npatch1 != self.npatch1 or
+                                   dr.npatch2 not in (self.npatch2, 1)):
+                raise RuntimeError("DR must be run with the same patches as DD")
+            if rd is not None and (len(rd.results) == 0 or rd.npatch2 != self.npatch2 or
+                                   rd.npatch1 not in (self.npatch1, 1)):
+                raise RuntimeError("RD must be run with the same patches as DD")
+            add_ij = set()
+            if rr.npatch1 != 1 and rr.npatch2 != 1:
+                add_ij.update(ij for ij in rr.results if ij not in self.results)
+            if dr is not None and dr.npatch2 != 1:
+                add_ij.update(ij for ij in dr.results if ij not in self.results)
+            if rd is not None and rd.npatch1 != 1:
+                add_ij.update(ij for ij in rd.results if ij not in self.results)
+            for ij in add_ij:
+                self.results[ij] = _PairSums(self.nbins, 0, None, tot=0.)
+        self._cov = self.estimate_cov(self.var_method)
+        self.varxi = self.cov_diag
+        return self.xi, self.varxi
+
+    def _calculate_xi_from_pairs(self, pairs, corr_only):
+        # nncorrelation.py:395-455
+        super()._calculate_xi_from_pairs(pairs, corr_only)
+        if self._rr is None:
+            return
+        dd = self.weight
+        if len(self._rr.results) > 0:
+            pairs1 = self._rr._keep_ok(pairs)
+            self._rr._calculate_xi_from_pairs(pairs1, corr_only=True)
+            dd_tot = self.tot
+        else:
+            # RR without patches: scale by the area fraction, with tot of the auto
+            # pairs proportional to area squared.
+            area_frac = np.sum([(self.results[i, j].tot * w) ** 0.5 for i, j, w in pairs
+                                if i == j])
+            area_frac /= np.sum([cij.tot ** 0.5 for (i, j), cij in self.results.items()
+                                 if i == j])
+            dd_tot = np.sum([self.results[ij].tot for ij in self.results])
+            dd_tot *= area_frac
+        rr = self._rr.weight
+        rrf = dd_tot / self._rr.tot
+        if self._dr is not None:
+            pairs2 = pairs
+            if self._dr.npatch2 == 1:
+                pairs2 = [(i, 0, w) for i, j, w in pairs2 if i == j]
+            pairs2 = self._dr._keep_ok(pairs2)
+            self._dr._calculate_xi_from_pairs(pairs2, corr_only=True)
+            dr = self._dr.weight
+            drf = dd_tot / self._dr.tot
+        if self._rd is not None:
+            pairs3 = pairs
+            if self._rd.npatch1 == 1 and not all([p[0] == 0 for p in pairs]):
+                pairs3 = [(0, j, w) for i, j, w in pairs3 if i == j]
+            pairs3 = self._rd._keep_ok(pairs3)
+            self._rd._calculate_xi_from_pairs(pairs3, corr_only=True)
+            rd = self._rd.weight
+            rdf = dd_tot / self._rd.tot
+        denom = rr * rrf
+        if self._dr is None and self._rd is None:
+            xi = dd - denom
+        elif self._rd is not None and self._dr is None:
+            xi = dd - 2. * rd * rdf + denom
+        elif self._dr is not None and self._rd is None:
+            xi = dd - 2. * dr * drf + denom
+        else:
+            xi = dd - dr * drf - rd * rdf + denom
+        denom[denom == 0] = 1
+        self.xi = xi / denom
+        self._rr_weight = denom
 
 
 class KKCorrelation(Corr2):
@@ -499,6 +862,10 @@ class GGCorrelation(Corr2):
         # zzcorrelation.py:76-88
         self._finalize()
         self._var_num = 2. * varg1 * varg2
+
+    def getStat(self):
+        # zzcorrelation.py:61-66
+        return np.concatenate([self.xip.ravel(), self.xim.ravel()])
 
     def getWeight(self):
         return np.concatenate([self.weight.ravel(), self.weight.ravel()])

@@ -300,6 +300,21 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
     return 0;
 }
 
+// BaseCorr2::process(f1,f2) early exit, src/Corr2.cpp:215-231: the two fields are
+// entirely closer than minsep or farther than maxsep.
+static bool fields_out_of_range(const C2Params& P, int coords, const tc_field* f1,
+                                const tc_field* f2) {
+    const FieldInfo& a = f1->T.info;
+    const FieldInfo& b = f2->T.info;
+    double dx = a.cx - b.cx, dy = a.cy - b.cy, dz = a.cz - b.cz;
+    double rsq = coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
+    double s1ps2 = std::sqrt(a.sizesq) + std::sqrt(b.sizesq);
+    bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
+                     rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
+    bool too_large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+    return too_small || too_large;
+}
+
 static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int metric) {
     if (!c || !f1) return tc::fail("NULL argument");
     if (metric != 1) {
@@ -313,18 +328,7 @@ static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int 
     P.quick = quick ? 1 : 0;
     const int nb = P.nbins;
     for (int k = 0; k < 4; ++k) c->counters[k] = 0;
-    if (f2) {
-        // BaseCorr2::process(f1,f2) early exit, src/Corr2.cpp:215-231
-        const FieldInfo& a = f1->T.info;
-        const FieldInfo& b = f2->T.info;
-        double dx = a.cx - b.cx, dy = a.cy - b.cy, dz = a.cz - b.cz;
-        double rsq = f1->coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
-        double s1ps2 = std::sqrt(a.sizesq) + std::sqrt(b.sizesq);
-        bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
-                         rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
-        bool too_large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
-        if (too_small || too_large) return 0;
-    }
+    if (f2 && fields_out_of_range(P, f1->coords, f1, f2)) return 0;
     std::vector<double> out((size_t)7 * nb);
     std::vector<unsigned long long> np(nb), cnt(4);
     std::lock_guard<std::mutex> run_lock(g_run_m);
@@ -356,6 +360,87 @@ int tc_corr2_process_cross(tc_corr2* c, tc_field* f1, tc_field* f2, int dots, in
     (void)dots;
     if (!f2) return tc::fail("NULL argument");
     return corr2_common(c, f1, f2, quick, metric);
+}
+
+int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
+                           tc_field* const* fields2, int nfield2, const int* p1, const int* p2,
+                           int npair, int quick, int metric, double* pair_out) {
+    if (!c || !fields1 || (npair > 0 && (!p1 || !p2 || !pair_out)) || npair < 0 || nfield1 <= 0)
+        return tc::fail("NULL or invalid argument");
+    if (metric != 1) {
+        set_error("only the Euclidean metric is implemented");
+        return TC_ERR_UNSUPPORTED;
+    }
+    const bool same = fields2 == nullptr;
+    if (!same && nfield2 <= 0) return tc::fail("nfield2 must be positive");
+    const int coords = fields1[0] ? fields1[0]->coords : 0;
+    auto check_side = [&](tc_field* const* fs, int n) -> int {
+        for (int k = 0; k < n; ++k) {
+            if (!fs[k]) return tc::fail("NULL field in the patch list");
+            if (fs[k]->d != c->d || fs[k]->coords != coords)
+                return tc::fail("field types do not match the correlation");
+        }
+        return 0;
+    };
+    if (int rc = check_side(fields1, nfield1)) return rc;
+    if (!same) if (int rc = check_side(fields2, nfield2)) return rc;
+    const int n2 = same ? nfield1 : nfield2;
+    for (int p = 0; p < npair; ++p)
+        if (p1[p] < 0 || p1[p] >= nfield1 || p2[p] < 0 || p2[p] >= n2)
+            return tc::fail("patch index out of range in the pair list");
+    if (int rc = check_gpu()) return rc;
+    C2Params P = c->P;
+    P.quick = quick ? 1 : 0;
+    const int nb = P.nbins;
+    for (int k = 0; k < 4; ++k) c->counters[k] = 0;
+    std::fill(pair_out, pair_out + (size_t)8 * nb * npair, 0.);
+    // the pairs that reach the traversal
+    std::vector<int> pi, pj, pa, which;
+    for (int p = 0; p < npair; ++p) {
+        const bool autop = same && p1[p] == p2[p];
+        tc_field* fa = fields1[p1[p]];
+        tc_field* fb = same ? fields1[p2[p]] : fields2[p2[p]];
+        if (!autop && fields_out_of_range(P, coords, fa, fb)) continue;
+        pi.push_back(p1[p]); pj.push_back(p2[p]); pa.push_back(autop ? 1 : 0);
+        which.push_back(p);
+    }
+    std::vector<const DeviceTree*> t1(nfield1), t2;
+    for (int k = 0; k < nfield1; ++k) t1[k] = &fields1[k]->T;
+    if (!same) {
+        t2.resize(nfield2);
+        for (int k = 0; k < nfield2; ++k) t2[k] = &fields2[k]->T;
+    }
+    const size_t nh = (size_t)7 * nb;
+    std::vector<double> out(nh * pi.size());
+    std::vector<unsigned long long> np((size_t)nb * pi.size()), cnt(4);
+    {
+        std::lock_guard<std::mutex> run_lock(g_run_m);
+        if (!g_s2) g_s2 = new_corr2_scratch();
+        if (int rc = tc::corr2_run_pairs(*g_s2, c->d, coords, t1, t2, pi, pj, pa, P, out.data(),
+                                         np.data(), cnt.data()))
+            return rc;
+    }
+    const int nxi = c->d == NData ? 0 : c->d == KData ? 1 : 4;
+    for (size_t q = 0; q < which.size(); ++q) {
+        const double* o = out.data() + nh * q;
+        double* r = pair_out + (size_t)8 * nb * which[q];
+        for (size_t t = 0; t < nh; ++t) r[t] = o[t];
+        for (int k = 0; k < nb; ++k) r[(size_t)7 * nb + k] = double(np[(size_t)nb * q + k]);
+    }
+    // totals, added in pair order
+    for (int p = 0; p < npair; ++p) {
+        const double* r = pair_out + (size_t)8 * nb * p;
+        for (int q = 0; q < nxi; ++q)
+            for (int k = 0; k < nb; ++k) c->xi[q][k] += r[(size_t)q * nb + k];
+        for (int k = 0; k < nb; ++k) {
+            c->meanr[k] += r[(size_t)4 * nb + k];
+            c->meanlogr[k] += r[(size_t)5 * nb + k];
+            c->weight[k] += r[(size_t)6 * nb + k];
+            c->npairs[k] += r[(size_t)7 * nb + k];
+        }
+    }
+    for (int k = 0; k < 4; ++k) c->counters[k] = (long long)cnt[k];
+    return 0;
 }
 
 int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, double y1,

@@ -6,13 +6,19 @@
 // applies the process11 decision (prune / accept into one bin / split 2 or 4
 // ways), accepted pairs are binned on the spot (directProcess11 + finishProcess,
 // src/Corr2.cpp:476-566) into a per-workgroup LDS histogram, and the children of
-// split pairs are compacted per wavefront (one atomic per wave) into the output
-// queue, which is pushed back onto the stack.  Popping the newest items first
-// keeps the stack depth bounded like the reference's recursion.
+// split pairs are written in item order to the workgroup's own output region; the
+// next launch reads them there.  Popping the newest items first keeps the stack
+// depth bounded like the reference's recursion.
 //
 // Accumulation: npairs in exact u64, every other sum in FP64 -- LDS atomics
 // within a launch, Neumaier-compensated per-workgroup slots across launches, and
 // a fixed-order compensated reduction over slots at the end.
+//
+// Patch batching (the jobs of a patch-based covariance, treecorr/corr2base.py:
+// 836-1050): the trees of all patches are concatenated, every item carries the
+// index of its patch pair, and all patch pairs run in the same launches; each
+// workgroup keeps LDS histograms for a few patch pairs at a time and adds them
+// to per-pair accumulators in HBM.
 #include "tc_engine.h"
 
 #include <algorithm>
@@ -121,52 +127,93 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
     }
 }
 
-// Workgroup b takes the items [b*per_block, (b+1)*per_block) of the chunk and writes
-// the children of its split pairs, in item order, to its own region
-// out[b*4*per_block ...] (at most four children per item); bcount[b] gets their
-// number.  No global atomics: the regions are concatenated afterwards.
+// A node pair of one patch pair in a batched run: p indexes the patch-pair list.
+struct alignas(16) ItemP { int i, j, p, pad; };
+
+template <class IT> struct ItemOps;
+template <> struct ItemOps<Item> {
+    static constexpr bool batched = false;
+    __device__ static Item make(int i, int j, const Item&) { return Item{i, j}; }
+    __device__ static int pair(const Item&) { return 0; }
+};
+template <> struct ItemOps<ItemP> {
+    static constexpr bool batched = true;
+    __device__ static ItemP make(int i, int j, const ItemP& parent) {
+        return ItemP{i, j, parent.p, 0};
+    }
+    __device__ static int pair(const ItemP& it) { return it.p; }
+};
+
+constexpr int PB_SLOTS = 4;   // patch pairs with an LDS histogram per workgroup
+
 // Where a launch reads its items.  Items [0, nhead) come from the output regions of
 // the previous launch, read in place: region r holds off[r+1]-off[r] items at
 // in + r*stride, and the launch takes their concatenated items [first, first+nhead).
 // Items [nhead, nin) come from the contiguous run tail (the top of the stack).
+template <class IT>
 struct InView {
-    const Item* in;
+    const IT* in;
     const unsigned long long* off;
     long long stride, first, nhead;
     int nreg;
-    const Item* tail;
+    const IT* tail;
 };
 
-template <int D, int C>
+// LDS histogram slot of patch pair p in this workgroup (claimed on first use), or -1
+// when all PB_SLOTS are taken by other pairs.
+__device__ __forceinline__ int pair_slot(int p, int* tags) {
+#pragma unroll
+    for (int s = 0; s < PB_SLOTS; ++s)
+        if (tags[s] == p) return s;
+#pragma unroll
+    for (int s = 0; s < PB_SLOTS; ++s) {
+        const int old = atomicCAS(&tags[s], -1, p);
+        if (old == -1 || old == p) return s;
+    }
+    return -1;
+}
+
+// Workgroup b takes the items [b*per_block, (b+1)*per_block) of the chunk and writes
+// the children of its split pairs, in item order, to its own region
+// out[b*4*per_block ...] (at most four children per item); bcount[b] gets their
+// number.  No global atomics: the regions are concatenated afterwards.
+template <int D, int C, class IT>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
-    const DNode* __restrict__ n1, const DNode* __restrict__ n2, InView V,
-    long long nin, long long per_block, Item* __restrict__ out,
+    const DNode* __restrict__ n1, const DNode* __restrict__ n2, InView<IT> V,
+    long long nin, long long per_block, IT* __restrict__ out,
     unsigned int* __restrict__ bcount, C2Params P, int R,
     double* __restrict__ slot_hi, double* __restrict__ slot_lo,
-    unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt) {
+    unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt,
+    double* __restrict__ pair_hist, unsigned long long* __restrict__ pair_np) {
+    using Ops = ItemOps<IT>;
     HIP_DYNAMIC_SHARED(double, sh)
     __shared__ int s_wsum[TRAV_THREADS / 64];
     __shared__ unsigned long long s_off[TRAV_SLOTS + 1];
+    __shared__ int s_tag[PB_SLOTS];
     const int nb = P.nbins;
     const int nh = NQ2 * nb;
     // R interleaved copies of the histogram (R a power of two): lanes that land in the
     // same bin in one instruction hit R different LDS banks instead of one address.
-    double* hist_all = sh;
-    unsigned long long* hnp_all = reinterpret_cast<unsigned long long*>(sh + (size_t)nh * R);
-    unsigned long long* hcnt = hnp_all + (size_t)nb * R;
-    for (int t = threadIdx.x; t < (nh + nb) * R + NCNT; t += blockDim.x)
+    // A batched run holds PB_SLOTS such histograms, one per patch pair.
+    const int nslot = Ops::batched ? PB_SLOTS : 1;
+    const size_t per_slot = (size_t)(nh + nb) * R;
+    unsigned long long* hcnt = reinterpret_cast<unsigned long long*>(sh + per_slot * nslot);
+    for (int t = threadIdx.x; t < (int)(per_slot * nslot) + NCNT; t += blockDim.x)
         reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
     if (V.nhead > 0)
         for (int t = threadIdx.x; t <= V.nreg; t += blockDim.x) s_off[t] = V.off[t];
+    if (threadIdx.x < PB_SLOTS) s_tag[threadIdx.x] = -1;
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* hist_all = sh;
+    unsigned long long* hnp_all = reinterpret_cast<unsigned long long*>(sh + (size_t)nh * R);
     double* hist = hist_all + (lane & (R - 1));
     unsigned long long* hnp = hnp_all + (lane & (R - 1));
     unsigned int c_vis = 0, c_acc = 0, c_leaf = 0, c_ref = 0;
     const long long b0 = (long long)blockIdx.x * per_block;
     const long long b1 = b0 + per_block < nin ? b0 + per_block : nin;
-    Item* __restrict__ bout = out + (size_t)blockIdx.x * 4 * per_block;
+    IT* __restrict__ bout = out + (size_t)blockIdx.x * 4 * per_block;
     unsigned int wpos = 0;    // children written by this workgroup so far (uniform)
     // region cursor of this thread's next item (items of a thread only move forward)
     int reg = 0;
@@ -181,10 +228,10 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     }
     for (long long base = b0; base < b1; base += blockDim.x) {
         const long long idx = base + threadIdx.x;
-        Item o0{0, 0}, o1{0, 0}, o2{0, 0}, o3{0, 0};
+        IT o0{}, o1{}, o2{}, o3{};
         int no = 0;
         if (idx < b1) {
-            Item it;
+            IT it;
             if (idx < V.nhead) {
                 const unsigned long long g = (unsigned long long)(V.first + idx);
                 while (s_off[reg + 1] <= g) ++reg;
@@ -197,7 +244,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 const DNode c = n1[it.i];
                 if (c.w != 0.f && double(c.size) > P.halfminsep) {
                     const int L = c.child;
-                    o0 = Item{L, -1}; o1 = Item{L + 1, -1}; o2 = Item{L, L + 1};
+                    o0 = Ops::make(L, -1, it); o1 = Ops::make(L + 1, -1, it);
+                    o2 = Ops::make(L, L + 1, it);
                     no = 3;
                 }
             } else {
@@ -257,7 +305,19 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 if (cls == CL_SINGLE && inrange) {
                     ++c_acc;
                     if (c1.child < 0 && c2.child < 0) ++c_leaf;
-                    bin_pair<D, C>(c1, c2, rsq, k, r, logr, P, hist, hnp, R);
+                    if (!Ops::batched) {
+                        bin_pair<D, C>(c1, c2, rsq, k, r, logr, P, hist, hnp, R);
+                    } else {
+                        const int pp = Ops::pair(it);
+                        const int sl = pair_slot(pp, s_tag);
+                        if (sl >= 0)
+                            bin_pair<D, C>(c1, c2, rsq, k, r, logr, P, hist + sl * per_slot,
+                                           hnp + sl * per_slot, R);
+                        else   // all LDS slots taken: straight into the pair's HBM sums
+                            bin_pair<D, C>(c1, c2, rsq, k, r, logr, P,
+                                           pair_hist + (size_t)pp * 2 * nh,
+                                           pair_np + (size_t)pp * nb, 1);
+                    }
                 } else if (cls == CL_SPLIT) {
                     // CalcSplitSq, include/Split.h:82-101
                     const double bsq_eff = rsq * (P.asq < P.bsq ? P.asq : P.bsq);
@@ -273,14 +333,14 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                     }
                     if (sp1 && sp2) {
                         const int L1 = c1.child, L2 = c2.child;
-                        o0 = Item{L1, L2}; o1 = Item{L1, L2 + 1};
-                        o2 = Item{L1 + 1, L2}; o3 = Item{L1 + 1, L2 + 1};
+                        o0 = Ops::make(L1, L2, it); o1 = Ops::make(L1, L2 + 1, it);
+                        o2 = Ops::make(L1 + 1, L2, it); o3 = Ops::make(L1 + 1, L2 + 1, it);
                         no = 4;
                     } else if (sp1) {
-                        o0 = Item{c1.child, it.j}; o1 = Item{c1.child + 1, it.j};
+                        o0 = Ops::make(c1.child, it.j, it); o1 = Ops::make(c1.child + 1, it.j, it);
                         no = 2;
                     } else {
-                        o0 = Item{it.i, c2.child}; o1 = Item{it.i, c2.child + 1};
+                        o0 = Ops::make(it.i, c2.child, it); o1 = Ops::make(it.i, c2.child + 1, it);
                         no = 2;
                     }
                 }
@@ -315,6 +375,35 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     atomicAdd(&hcnt[CNT_LEAF], (unsigned long long)c_leaf);
     atomicAdd(&hcnt[CNT_REFINED], (unsigned long long)c_ref);
     __syncthreads();
+    if (Ops::batched) {
+        // Add each used pair histogram to that pair's sums in HBM.  A pair's sums are
+        // a (hi, lo) pair of arrays: the exact rounding error of every addition into
+        // hi (TwoSum of the value it replaced and the addend) goes into lo.
+        if (threadIdx.x < NCNT) slot_cnt[(size_t)blockIdx.x * NCNT + threadIdx.x] += hcnt[threadIdx.x];
+        for (int sl = 0; sl < PB_SLOTS; ++sl) {
+            const int pp = s_tag[sl];
+            if (pp < 0) continue;
+            const double* hs = hist_all + sl * per_slot;
+            const unsigned long long* ns = hnp_all + sl * per_slot;
+            double* ph = pair_hist + (size_t)pp * 2 * nh;
+            for (int t = threadIdx.x; t < nh; t += blockDim.x) {
+                double v = 0.;
+                for (int c = 0; c < R; ++c) v += hs[(size_t)t * R + c];
+                if (v == 0.) continue;
+                const double old = atomicAdd(&ph[t], v);
+                const double sum = old + v;
+                const double bv = sum - old;
+                const double err = (old - (sum - bv)) + (v - bv);
+                if (err != 0.) atomicAdd(&ph[nh + t], err);
+            }
+            for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+                unsigned long long v = 0;
+                for (int c = 0; c < R; ++c) v += ns[(size_t)t * R + c];
+                if (v) atomicAdd(&pair_np[(size_t)pp * nb + t], v);
+            }
+        }
+        return;
+    }
     // Flush into this workgroup's slot with Neumaier compensation.
     const size_t so = (size_t)blockIdx.x * nh;
     for (int t = threadIdx.x; t < nh; t += blockDim.x) {
@@ -401,21 +490,20 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const unsigned int* bcount
     if (t == 0) { boff[nblk] = carry_s; *total = carry_s; }
 }
 
-// Concatenates the per-workgroup child regions onto the stack: workgroup b copies its
-// bcount[b] items to dst[boff[b] ...].
 // Copies the concatenated items [0, limit) of the regions onto the stack (the rest is
 // read in place by the next launch): workgroup b copies its share to dst[boff[b] ...].
-__global__ __launch_bounds__(256) void k_gather_children(const Item* __restrict__ out,
+template <class IT>
+__global__ __launch_bounds__(256) void k_gather_children(const IT* __restrict__ out,
                                                          long long per_block,
                                                          const unsigned int* __restrict__ bcount,
                                                          const unsigned long long* __restrict__ boff,
                                                          unsigned long long limit,
-                                                         Item* __restrict__ dst) {
+                                                         IT* __restrict__ dst) {
     const int b = blockIdx.x;
     if (boff[b] >= limit) return;
     const unsigned long long n = bcount[b] < limit - boff[b] ? bcount[b] : limit - boff[b];
-    const Item* src = out + (size_t)b * 4 * per_block;
-    Item* d = dst + boff[b];
+    const IT* src = out + (size_t)b * 4 * per_block;
+    IT* d = dst + boff[b];
     for (unsigned long long i = threadIdx.x; i < n; i += blockDim.x) d[i] = src[i];
 }
 
@@ -448,14 +536,66 @@ __global__ void k_items_cross(const int* top1, int ntop1, const int* top2, int n
     }
 }
 
+// Initial items of a batched run.  Patch pair p (pi[p], pj[p]) owns the items
+// [ioff[p], ioff[p+1]): for an auto pair (pauto[p] != 0) the process2 items of the
+// patch's top cells then its top-cell pairs a<b, otherwise all top-cell pairs of the
+// two patches.  Top cells are global node ids, patch q's at tops[toff[q] ...].
+__global__ void k_items_pairs(const int* __restrict__ pi, const int* __restrict__ pj,
+                              const int* __restrict__ pauto, const long long* __restrict__ ioff,
+                              int npair, const int* __restrict__ tops1,
+                              const int* __restrict__ toff1, const int* __restrict__ tops2,
+                              const int* __restrict__ toff2, ItemP* __restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ioff[npair]) return;
+    int lo = 0, hi = npair - 1;     // last pair with ioff[p] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ioff[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int p = lo;
+    long long q = t - ioff[p];
+    const int i = pi[p], j = pj[p];
+    const int* ta = tops1 + toff1[i];
+    const long long na = toff1[i + 1] - toff1[i];
+    if (pauto[p]) {
+        if (q < na) { out[t] = ItemP{ta[q], -1, p, 0}; return; }
+        q -= na;
+        // row-major over a < b: row a holds na-1-a pairs
+        long long a = (long long)((2. * na - 1 - sqrt((2. * na - 1) * (2. * na - 1) - 8. * double(q))) / 2);
+        if (a < 0) a = 0;
+        auto row_start = [&](long long r) { return r * (2 * na - r - 1) / 2; };
+        while (a > 0 && row_start(a) > q) --a;
+        while (row_start(a + 1) <= q) ++a;
+        const long long bb = a + 1 + (q - row_start(a));
+        out[t] = ItemP{ta[a], ta[bb], p, 0};
+    } else {
+        const int* tb = tops2 + toff2[j];
+        const long long nbt = toff2[j + 1] - toff2[j];
+        out[t] = ItemP{ta[q / nbt], tb[q % nbt], p, 0};
+    }
+}
+
+// Copies a patch's nodes into a concatenated array, moving child links by the
+// patch's offset.
+__global__ void k_concat_nodes(const DNode* __restrict__ src, int n, int offset,
+                               DNode* __restrict__ dst) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    DNode d = src[t];
+    if (d.child >= 0) d.child += offset;
+    dst[offset + t] = d;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Traversal buffers, kept by the library for reuse (sized in bytes: a batched run
+// stores 16-byte items, a single correlation 8-byte ones).
 struct Corr2Scratch {
-    Item* stack = nullptr;
-    long long cap = 0;
-    Item* outq = nullptr;      // two output region buffers (ping-pong)
-    Item* outq2 = nullptr;
+    void* stack = nullptr;
+    size_t stack_bytes = 0;
+    void* outq[2] = {nullptr, nullptr};     // output regions of consecutive launches
+    size_t outq_bytes = 0;
     double *slot_hi = nullptr, *slot_lo = nullptr;
     unsigned long long *slot_np = nullptr, *slot_cnt = nullptr;
     double* red = nullptr;
@@ -468,8 +608,7 @@ struct Corr2Scratch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ~Corr2Scratch() {
         if (stack) (void)hipFree(stack);
-        if (outq) (void)hipFree(outq);
-        if (outq2) (void)hipFree(outq2);
+        for (void* q : outq) if (q) (void)hipFree(q);
         if (slot_hi) (void)hipFree(slot_hi);
         if (slot_lo) (void)hipFree(slot_lo);
         if (slot_np) (void)hipFree(slot_np);
@@ -486,99 +625,93 @@ struct Corr2Scratch {
     }
 };
 
-static int ensure_scratch(Corr2Scratch& S, int nb) {
-    if (S.nb == nb && S.slot_hi) return 0;
-    const size_t nh = (size_t)NQ2 * nb;
-    if (S.slot_hi) { (void)hipFree(S.slot_hi); (void)hipFree(S.slot_lo); (void)hipFree(S.slot_np);
-                     (void)hipFree(S.red); (void)hipFree(S.red_np); }
-    TC_HIP(hipMalloc(&S.slot_hi, sizeof(double) * nh * TRAV_SLOTS));
-    TC_HIP(hipMalloc(&S.slot_lo, sizeof(double) * nh * TRAV_SLOTS));
-    TC_HIP(hipMalloc(&S.slot_np, sizeof(unsigned long long) * nb * TRAV_SLOTS));
-    TC_HIP(hipMalloc(&S.red, sizeof(double) * nh));
-    TC_HIP(hipMalloc(&S.red_np, sizeof(unsigned long long) * nb));
+static int ensure_scratch(Corr2Scratch& S, int nb, size_t item_bytes) {
+    if (S.nb != nb || !S.slot_hi) {
+        const size_t nh = (size_t)NQ2 * nb;
+        if (S.slot_hi) { (void)hipFree(S.slot_hi); (void)hipFree(S.slot_lo); (void)hipFree(S.slot_np);
+                         (void)hipFree(S.red); (void)hipFree(S.red_np); }
+        TC_HIP(hipMalloc(&S.slot_hi, sizeof(double) * nh * TRAV_SLOTS));
+        TC_HIP(hipMalloc(&S.slot_lo, sizeof(double) * nh * TRAV_SLOTS));
+        TC_HIP(hipMalloc(&S.slot_np, sizeof(unsigned long long) * nb * TRAV_SLOTS));
+        TC_HIP(hipMalloc(&S.red, sizeof(double) * nh));
+        TC_HIP(hipMalloc(&S.red_np, sizeof(unsigned long long) * nb));
+        S.nb = nb;
+    }
     if (!S.slot_cnt) {
         TC_HIP(hipMalloc(&S.slot_cnt, sizeof(unsigned long long) * NCNT * TRAV_SLOTS));
         TC_HIP(hipMalloc(&S.red_cnt, sizeof(unsigned long long) * NCNT));
         TC_HIP(hipMalloc(&S.outcount, sizeof(unsigned long long)));
         TC_HIP(hipHostMalloc((void**)&S.h_outcount, sizeof(unsigned long long)));
-        TC_HIP(hipMalloc(&S.outq, sizeof(Item) * 4 * (TRAV_CHUNK + (long long)TRAV_THREADS * TRAV_SLOTS)));
-        TC_HIP(hipMalloc(&S.outq2, sizeof(Item) * 4 * (TRAV_CHUNK + (long long)TRAV_THREADS * TRAV_SLOTS)));
         TC_HIP(hipMalloc(&S.bcount, sizeof(unsigned int) * 2 * TRAV_SLOTS));
         TC_HIP(hipMalloc(&S.boff, sizeof(unsigned long long) * 2 * (TRAV_SLOTS + 1)));
         TC_HIP(hipEventCreate(&S.ev0));
         TC_HIP(hipEventCreate(&S.ev1));
     }
-    S.nb = nb;
+    const size_t qb = item_bytes * 4 * (TRAV_CHUNK + (size_t)TRAV_THREADS * TRAV_SLOTS);
+    if (S.outq_bytes < qb) {
+        for (void*& q : S.outq) {
+            if (q) (void)hipFree(q);
+            TC_HIP(hipMalloc(&q, qb));
+        }
+        S.outq_bytes = qb;
+    }
     return 0;
 }
 
-static int ensure_stack(Corr2Scratch& S, long long need, long long keep) {
-    if (need <= S.cap) return 0;
-    long long cap = std::max<long long>(need, std::max<long long>(2 * S.cap, 1LL << 22));
-    Item* ns = nullptr;
-    TC_HIP(hipMalloc(&ns, sizeof(Item) * cap));
+static int ensure_stack(Corr2Scratch& S, size_t need, size_t keep) {
+    if (need <= S.stack_bytes) return 0;
+    size_t cap = std::max(need, std::max<size_t>(2 * S.stack_bytes, size_t(32) << 20));
+    void* ns = nullptr;
+    TC_HIP(hipMalloc(&ns, cap));
     if (S.stack && keep > 0)
-        TC_HIP(hipMemcpyAsync(ns, S.stack, sizeof(Item) * keep, hipMemcpyDeviceToDevice, stream()));
+        TC_HIP(hipMemcpyAsync(ns, S.stack, keep, hipMemcpyDeviceToDevice, stream()));
     TC_HIP(hipStreamSynchronize(stream()));
     if (S.stack) (void)hipFree(S.stack);
     S.stack = ns;
-    S.cap = cap;
+    S.stack_bytes = cap;
     return 0;
 }
 
-using TravFn = void (*)(const DNode*, const DNode*, InView, long long, long long, Item*,
+template <class IT>
+using TravFn = void (*)(const DNode*, const DNode*, InView<IT>, long long, long long, IT*,
                         unsigned int*, C2Params, int, double*, double*, unsigned long long*,
-                        unsigned long long*);
+                        unsigned long long*, double*, unsigned long long*);
 
-static TravFn pick_traverse(int D, int C) {
-#define TC_TF(DD, CC) return &k_traverse<DD, CC>
+template <class IT>
+static TravFn<IT> pick_traverse(int D, int C) {
+#define TC_TF(DD, CC) return &k_traverse<DD, CC, IT>
     if (D == NData) { if (C == Flat) TC_TF(NData, Flat); if (C == ThreeD) TC_TF(NData, ThreeD); TC_TF(NData, Sphere); }
     if (D == KData) { if (C == Flat) TC_TF(KData, Flat); if (C == ThreeD) TC_TF(KData, ThreeD); TC_TF(KData, Sphere); }
     if (C == Flat) TC_TF(GData, Flat); if (C == ThreeD) TC_TF(GData, ThreeD); TC_TF(GData, Sphere);
 #undef TC_TF
 }
 
-// Runs the whole traversal for one auto or cross correlation and writes the
-// reduced sums: out[NQ2*nb] doubles, np[nb], cnt[NCNT] (host arrays).
-int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dtop1, int ntop1,
-              const DNode* nodes2, const int* dtop2, int ntop2, const C2Params& P,
-              double* out, unsigned long long* np, unsigned long long* cnt) {
-    hipStream_t st = stream();
-    const int nb = P.nbins;
+// LDS histogram copies and bytes per workgroup for nslot pair histograms.
+static int hist_copies(int nb, int nslot, size_t& shbytes) {
     const size_t nh = (size_t)NQ2 * nb;
-    if (sizeof(double) * (nh + nb + NCNT) > 64 * 1024)
-        return fail("too many bins for the LDS histogram (nbins <= 1000)");
-    int R = 8;     // histogram copies: as many as fit in 48 KB, at most 8
-    while (R > 1 && sizeof(double) * ((nh + nb) * (size_t)R + NCNT) > 48 * 1024) R /= 2;
-    const size_t shbytes = sizeof(double) * ((nh + nb) * (size_t)R + NCNT);
-    if (ensure_scratch(S, nb)) return 2;
-    TC_HIP(hipMemsetAsync(S.slot_hi, 0, sizeof(double) * nh * TRAV_SLOTS, st));
-    TC_HIP(hipMemsetAsync(S.slot_lo, 0, sizeof(double) * nh * TRAV_SLOTS, st));
-    TC_HIP(hipMemsetAsync(S.slot_np, 0, sizeof(unsigned long long) * nb * TRAV_SLOTS, st));
-    TC_HIP(hipMemsetAsync(S.slot_cnt, 0, sizeof(unsigned long long) * NCNT * TRAV_SLOTS, st));
+    int R = 8;     // as many copies as fit in 48 KB, at most 8
+    while (R > 1 && sizeof(double) * ((nh + nb) * (size_t)R * nslot + NCNT) > 48 * 1024) R /= 2;
+    shbytes = sizeof(double) * ((nh + nb) * (size_t)R * nslot + NCNT);
+    return R;
+}
 
-    const bool autoc = (nodes2 == nullptr);
-    long long ninit = autoc ? (long long)ntop1 + (long long)ntop1 * (ntop1 - 1) / 2
-                            : (long long)ntop1 * ntop2;
-    if (ensure_stack(S, ninit + 4 * TRAV_CHUNK, 0)) return 2;
-    if (ninit > 0) {
-        dim3 g((unsigned)((ninit + 255) / 256));
-        if (autoc) hipLaunchKernelGGL(k_items_auto, g, dim3(256), 0, st, dtop1, ntop1, S.stack);
-        else hipLaunchKernelGGL(k_items_cross, g, dim3(256), 0, st, dtop1, ntop1, dtop2, ntop2, S.stack);
-        TC_HIP(hipGetLastError());
-    }
-    TravFn fn = pick_traverse(D, C);
-    const DNode* n2 = autoc ? nodes1 : nodes2;
+// The LIFO work-stack loop over launches; the stack holds the ninit initial items.
+template <class IT>
+static int traverse_stack(Corr2Scratch& S, TravFn<IT> fn, const DNode* n1, const DNode* n2,
+                          long long ninit, const C2Params& P, int R, size_t shbytes,
+                          double* pair_hist, unsigned long long* pair_np) {
+    hipStream_t st = stream();
+    IT* stack = static_cast<IT*>(S.stack);
+    IT* outq[2] = {static_cast<IT*>(S.outq[0]), static_cast<IT*>(S.outq[1])};
     long long top = ninit;
     KernelStat& ks = kernel_stat(KS_TRAVERSE2);
     const bool timing = timing_enabled();
-    Item* outq[2] = {S.outq, S.outq2};
     // children of the last launch, still in its output regions (newest work)
     int cur = 0;                 // buffer the last launch wrote
     long long reg_n = 0, reg_pb = 0;
     int reg_blocks = 0;
     for (;;) {
-        InView V{nullptr, nullptr, 0, 0, 0, 0, nullptr};
+        InView<IT> V{nullptr, nullptr, 0, 0, 0, 0, nullptr};
         long long n = 0;
         if (reg_n > 0) {
             // LIFO: the newest children are processed next, in place; the oldest part
@@ -586,24 +719,26 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
             n = std::min(reg_n, g_chunk);
             const long long excess = reg_n - n;
             if (excess > 0) {
-                if (ensure_stack(S, top + excess + 4 * TRAV_CHUNK, top)) return 2;
-                hipLaunchKernelGGL(k_gather_children, dim3(reg_blocks), dim3(256), 0, st,
+                if (ensure_stack(S, sizeof(IT) * (top + excess + 4 * TRAV_CHUNK),
+                                 sizeof(IT) * top)) return 2;
+                stack = static_cast<IT*>(S.stack);
+                hipLaunchKernelGGL(k_gather_children<IT>, dim3(reg_blocks), dim3(256), 0, st,
                                    outq[cur], reg_pb, S.bcount + cur * TRAV_SLOTS,
                                    S.boff + cur * (TRAV_SLOTS + 1),
-                                   (unsigned long long)excess, S.stack + top);
+                                   (unsigned long long)excess, stack + top);
                 TC_HIP(hipGetLastError());
                 top += excess;
             }
             // a short head is topped up with the newest stack items
             const long long extra = excess > 0 ? 0 : std::min(top, g_chunk - n);
-            V = InView{outq[cur], S.boff + cur * (TRAV_SLOTS + 1), 4 * reg_pb, excess, n,
-                       reg_blocks, S.stack + (top - extra)};
+            V = InView<IT>{outq[cur], S.boff + cur * (TRAV_SLOTS + 1), 4 * reg_pb, excess, n,
+                           reg_blocks, stack + (top - extra)};
             n += extra;
             top -= extra;
             reg_n = 0;
         } else if (top > 0) {
             n = std::min(top, g_chunk);
-            V = InView{nullptr, nullptr, 0, 0, 0, 0, S.stack + (top - n)};
+            V = InView<IT>{nullptr, nullptr, 0, 0, 0, 0, stack + (top - n)};
             top -= n;
         } else {
             break;
@@ -615,9 +750,9 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
             ((rounds + TRAV_SLOTS - 1) / TRAV_SLOTS) * (long long)TRAV_THREADS;
         const int blocks = (int)((n + per_block - 1) / per_block);
         if (timing) TC_HIP(hipEventRecord(S.ev0, st));
-        hipLaunchKernelGGL(fn, dim3(blocks), dim3(TRAV_THREADS), shbytes, st, nodes1, n2, V, n,
+        hipLaunchKernelGGL(fn, dim3(blocks), dim3(TRAV_THREADS), shbytes, st, n1, n2, V, n,
                            per_block, outq[nxt], S.bcount + nxt * TRAV_SLOTS, P, R, S.slot_hi,
-                           S.slot_lo, S.slot_np, S.slot_cnt);
+                           S.slot_lo, S.slot_np, S.slot_cnt, pair_hist, pair_np);
         TC_HIP(hipGetLastError());
         if (timing) TC_HIP(hipEventRecord(S.ev1, st));
         hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, S.bcount + nxt * TRAV_SLOTS,
@@ -637,6 +772,48 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
         reg_blocks = blocks;
         cur = nxt;
     }
+    return 0;
+}
+
+static int zero_slots(Corr2Scratch& S, int nb) {
+    hipStream_t st = stream();
+    const size_t nh = (size_t)NQ2 * nb;
+    TC_HIP(hipMemsetAsync(S.slot_hi, 0, sizeof(double) * nh * TRAV_SLOTS, st));
+    TC_HIP(hipMemsetAsync(S.slot_lo, 0, sizeof(double) * nh * TRAV_SLOTS, st));
+    TC_HIP(hipMemsetAsync(S.slot_np, 0, sizeof(unsigned long long) * nb * TRAV_SLOTS, st));
+    TC_HIP(hipMemsetAsync(S.slot_cnt, 0, sizeof(unsigned long long) * NCNT * TRAV_SLOTS, st));
+    return 0;
+}
+
+// Runs the whole traversal for one auto or cross correlation and writes the
+// reduced sums: out[NQ2*nb] doubles, np[nb], cnt[NCNT] (host arrays).
+int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dtop1, int ntop1,
+              const DNode* nodes2, const int* dtop2, int ntop2, const C2Params& P,
+              double* out, unsigned long long* np, unsigned long long* cnt) {
+    hipStream_t st = stream();
+    const int nb = P.nbins;
+    const size_t nh = (size_t)NQ2 * nb;
+    if (sizeof(double) * (nh + nb + NCNT) > 64 * 1024)
+        return fail("too many bins for the LDS histogram (nbins <= 1000)");
+    size_t shbytes = 0;
+    const int R = hist_copies(nb, 1, shbytes);
+    if (ensure_scratch(S, nb, sizeof(Item))) return 2;
+    if (zero_slots(S, nb)) return 2;
+    const bool autoc = (nodes2 == nullptr);
+    long long ninit = autoc ? (long long)ntop1 + (long long)ntop1 * (ntop1 - 1) / 2
+                            : (long long)ntop1 * ntop2;
+    if (ensure_stack(S, sizeof(Item) * (ninit + 4 * TRAV_CHUNK), 0)) return 2;
+    if (ninit > 0) {
+        dim3 g((unsigned)((ninit + 255) / 256));
+        Item* stack = static_cast<Item*>(S.stack);
+        if (autoc) hipLaunchKernelGGL(k_items_auto, g, dim3(256), 0, st, dtop1, ntop1, stack);
+        else hipLaunchKernelGGL(k_items_cross, g, dim3(256), 0, st, dtop1, ntop1, dtop2, ntop2, stack);
+        TC_HIP(hipGetLastError());
+    }
+    if (int rc = traverse_stack<Item>(S, pick_traverse<Item>(D, C), nodes1,
+                                      autoc ? nodes1 : nodes2, ninit, P, R, shbytes, nullptr,
+                                      nullptr))
+        return rc;
     const int tot = (int)(nh + nb + NCNT);
     hipLaunchKernelGGL(k_reduce_slots, dim3((tot + 255) / 256), dim3(256), 0, st, S.slot_hi,
                        S.slot_lo, S.slot_np, S.slot_cnt, TRAV_SLOTS, (int)nh, nb, S.red, S.red_np,
@@ -646,6 +823,133 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
     TC_HIP(hipMemcpyAsync(np, S.red_np, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, st));
     TC_HIP(hipMemcpyAsync(cnt, S.red_cnt, sizeof(unsigned long long) * NCNT, hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+// All patch pairs of a patch-based correlation in one traversal.  Side 1 has
+// npatch1 trees (side 2 npatch2, or side 1 again when trees2 is empty); pair p
+// correlates patch pi[p] of side 1 with patch pj[p] of side 2 (an auto pair when
+// pauto[p]).  Writes NQ2*nb sums and nb npairs per pair (host arrays, overwritten).
+int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const DeviceTree*>& trees1,
+                    const std::vector<const DeviceTree*>& trees2, const std::vector<int>& pi,
+                    const std::vector<int>& pj, const std::vector<int>& pauto, const C2Params& P,
+                    double* pair_out, unsigned long long* pair_npairs, unsigned long long* cnt) {
+    hipStream_t st = stream();
+    const int nb = P.nbins;
+    const size_t nh = (size_t)NQ2 * nb;
+    const int npair = (int)pi.size();
+    if (npair == 0) { for (int k = 0; k < NCNT; ++k) cnt[k] = 0; return 0; }
+    size_t shbytes = 0;
+    const int R = hist_copies(nb, PB_SLOTS, shbytes);
+    if (sizeof(double) * ((nh + nb) * PB_SLOTS + NCNT) > 64 * 1024)
+        return fail("too many bins for batched patch pairs (nbins <= 200)");
+    const bool same = trees2.empty();
+    const auto& t2 = same ? trees1 : trees2;
+    // concatenated nodes and top cells of each side
+    auto concat = [&](const std::vector<const DeviceTree*>& ts, DNode*& nodes, int*& tops,
+                      std::vector<int>& toff, size_t& nbytes, size_t& tbytes) -> int {
+        long long nn = 0, nt = 0;
+        for (auto* t : ts) { nn += t->nnodes; nt += (long long)t->top.size(); }
+        if (nn >= (1LL << 31) - 1) return fail("patches hold too many tree nodes together");
+        nbytes = sizeof(DNode) * std::max<long long>(nn, 1);
+        tbytes = sizeof(int) * std::max<long long>(nt, 1);
+        nodes = static_cast<DNode*>(dev_alloc(nbytes));
+        tops = static_cast<int*>(dev_alloc(tbytes));
+        if (!nodes || !tops) return fail("out of device memory concatenating patch trees");
+        std::vector<int> htops;
+        htops.reserve(nt);
+        toff.assign(1, 0);
+        int off = 0;
+        for (auto* t : ts) {
+            if (t->nnodes > 0)
+                hipLaunchKernelGGL(k_concat_nodes, dim3((t->nnodes + 255) / 256), dim3(256), 0,
+                                   st, t->nodes, t->nnodes, off, nodes);
+            for (int v : t->top) htops.push_back(v + off);
+            toff.push_back((int)htops.size());
+            off += t->nnodes;
+        }
+        TC_HIP(hipGetLastError());
+        if (!htops.empty())
+            TC_HIP(hipMemcpyAsync(tops, htops.data(), sizeof(int) * htops.size(),
+                                  hipMemcpyHostToDevice, st));
+        TC_HIP(hipStreamSynchronize(st));
+        return 0;
+    };
+    DNode *nodes1 = nullptr, *nodes2 = nullptr;
+    int *tops1 = nullptr, *tops2 = nullptr;
+    std::vector<int> toff1, toff2;
+    size_t nb1 = 0, tb1 = 0, nb2 = 0, tb2 = 0;
+    struct Freer {
+        std::vector<std::pair<void*, size_t>> v;
+        ~Freer() { for (auto& b : v) dev_free(b.first, b.second); }
+    } fr;
+    if (int rc = concat(trees1, nodes1, tops1, toff1, nb1, tb1)) return rc;
+    fr.v.push_back({nodes1, nb1}); fr.v.push_back({tops1, tb1});
+    if (same) { nodes2 = nodes1; tops2 = tops1; toff2 = toff1; }
+    else {
+        if (int rc = concat(t2, nodes2, tops2, toff2, nb2, tb2)) return rc;
+        fr.v.push_back({nodes2, nb2}); fr.v.push_back({tops2, tb2});
+    }
+    // item offsets of the pairs
+    std::vector<long long> ioff(npair + 1, 0);
+    for (int p = 0; p < npair; ++p) {
+        const long long na = toff1[pi[p] + 1] - toff1[pi[p]];
+        const long long nbt = toff2[pj[p] + 1] - toff2[pj[p]];
+        ioff[p + 1] = ioff[p] + (pauto[p] ? na + na * (na - 1) / 2 : na * nbt);
+    }
+    const long long ninit = ioff[npair];
+    // small device tables
+    const size_t pb = sizeof(int) * npair, ob = sizeof(long long) * (npair + 1);
+    const size_t f1b = sizeof(int) * toff1.size(), f2b = sizeof(int) * toff2.size();
+    int* dpi = static_cast<int*>(dev_alloc(pb));
+    int* dpj = static_cast<int*>(dev_alloc(pb));
+    int* dpa = static_cast<int*>(dev_alloc(pb));
+    long long* dio = static_cast<long long*>(dev_alloc(ob));
+    int* dt1 = static_cast<int*>(dev_alloc(f1b));
+    int* dt2 = static_cast<int*>(dev_alloc(f2b));
+    fr.v.push_back({dpi, pb}); fr.v.push_back({dpj, pb}); fr.v.push_back({dpa, pb});
+    fr.v.push_back({dio, ob}); fr.v.push_back({dt1, f1b}); fr.v.push_back({dt2, f2b});
+    if (!dpi || !dpj || !dpa || !dio || !dt1 || !dt2) return fail("out of device memory");
+    TC_HIP(hipMemcpyAsync(dpi, pi.data(), pb, hipMemcpyHostToDevice, st));
+    TC_HIP(hipMemcpyAsync(dpj, pj.data(), pb, hipMemcpyHostToDevice, st));
+    TC_HIP(hipMemcpyAsync(dpa, pauto.data(), pb, hipMemcpyHostToDevice, st));
+    TC_HIP(hipMemcpyAsync(dio, ioff.data(), ob, hipMemcpyHostToDevice, st));
+    TC_HIP(hipMemcpyAsync(dt1, toff1.data(), f1b, hipMemcpyHostToDevice, st));
+    TC_HIP(hipMemcpyAsync(dt2, toff2.data(), f2b, hipMemcpyHostToDevice, st));
+    // per-pair accumulators
+    const size_t hb = sizeof(double) * 2 * nh * npair, npb = sizeof(unsigned long long) * nb * npair;
+    double* ph = static_cast<double*>(dev_alloc(hb));
+    unsigned long long* pn = static_cast<unsigned long long*>(dev_alloc(npb));
+    fr.v.push_back({ph, hb}); fr.v.push_back({pn, npb});
+    if (!ph || !pn) return fail("out of device memory for the patch-pair sums");
+    TC_HIP(hipMemsetAsync(ph, 0, hb, st));
+    TC_HIP(hipMemsetAsync(pn, 0, npb, st));
+    if (ensure_scratch(S, nb, sizeof(ItemP))) return 2;
+    if (zero_slots(S, nb)) return 2;
+    if (ensure_stack(S, sizeof(ItemP) * (ninit + 4 * TRAV_CHUNK), 0)) return 2;
+    if (ninit > 0) {
+        hipLaunchKernelGGL(k_items_pairs, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st,
+                           dpi, dpj, dpa, dio, npair, tops1, dt1, tops2, dt2,
+                           static_cast<ItemP*>(S.stack));
+        TC_HIP(hipGetLastError());
+    }
+    if (int rc = traverse_stack<ItemP>(S, pick_traverse<ItemP>(D, C), nodes1, nodes2, ninit, P,
+                                       R, shbytes, ph, pn))
+        return rc;
+    // counters from the workgroup slots
+    const int tot = (int)(nh + nb + NCNT);
+    hipLaunchKernelGGL(k_reduce_slots, dim3((tot + 255) / 256), dim3(256), 0, st, S.slot_hi,
+                       S.slot_lo, S.slot_np, S.slot_cnt, TRAV_SLOTS, (int)nh, nb, S.red, S.red_np,
+                       S.red_cnt);
+    TC_HIP(hipGetLastError());
+    std::vector<double> hl(2 * nh * (size_t)npair);
+    TC_HIP(hipMemcpyAsync(hl.data(), ph, hb, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(pair_npairs, pn, npb, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(cnt, S.red_cnt, sizeof(unsigned long long) * NCNT, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipStreamSynchronize(st));
+    for (int p = 0; p < npair; ++p)
+        for (size_t t = 0; t < nh; ++t)
+            pair_out[p * nh + t] = hl[2 * nh * p + t] + hl[2 * nh * p + nh + t];
     return 0;
 }
 

@@ -34,6 +34,13 @@ void delete_corr2_scratch(Corr2Scratch*);
 int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dtop1, int ntop1,
               const DNode* nodes2, const int* dtop2, int ntop2, const C2Params& P, double* out,
               unsigned long long* np, unsigned long long* cnt);
+// All patch pairs (pi[p], pj[p]) of a patch-based correlation in one traversal;
+// trees2 empty means side 2 is side 1.  Per-pair sums (7*nbins doubles, nbins
+// npairs each) are written to host arrays.
+int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const DeviceTree*>& trees1,
+                    const std::vector<const DeviceTree*>& trees2, const std::vector<int>& pi,
+                    const std::vector<int>& pj, const std::vector<int>& pauto, const C2Params& P,
+                    double* pair_out, unsigned long long* pair_npairs, unsigned long long* cnt);
 
 // LogMultipole parameters of one 3-point correlation (BaseCorr3, src/Corr3.cpp:70-127).
 struct C3Params {
