@@ -188,6 +188,166 @@ def cpu_baseline(x, y, z, g1, g2, w, bn, gpu_counters, gpu_npairs, target_s=15.)
                        f"extrapolated by visit count = {t_full:.1f} s")}
 
 
+# ---------------------------------------------------------------------------
+# Patch workload: NN data x randoms and KK auto over 100 patches, jackknife
+# ---------------------------------------------------------------------------
+def lognormal_box(n, seed, box=1000., ngrid=256):
+    """n points in a box from a log-normal field: white noise on ngrid^3 smoothed by a
+    2-cell Gaussian, normalised to sigma 1, rho = exp(G - 1/2); cell occupation
+    proportional to rho, uniform within the cell."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    g = gaussian_filter(rng.standard_normal((ngrid,) * 3), 2.0, mode='wrap')
+    g /= g.std()
+    rho = np.exp(g - 0.5).ravel()
+    del g
+    cells = rng.choice(rho.size, size=n, p=rho / rho.sum())
+    ix, iy, iz = np.unravel_index(cells, (ngrid,) * 3)
+    h = box / ngrid
+    return ((ix + rng.random(n)) * h, (iy + rng.random(n)) * h, (iz + rng.random(n)) * h)
+
+
+def kmeans_labels(pts_fit, pts_all, npatch, seed, iters=10):
+    """Lloyd's k-means on a subsample, then nearest-centre labels for every point list."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    cen = pts_fit[rng.choice(len(pts_fit), npatch, replace=False)]
+    for it in range(iters):
+        lab = cKDTree(cen).query(pts_fit, workers=-1)[1]
+        print(f"# k-means iteration {it}", file=sys.stderr, flush=True)
+        for k in range(npatch):
+            m = lab == k
+            if np.any(m):
+                cen[k] = pts_fit[m].mean(axis=0)
+    tree = cKDTree(cen)
+    return [tree.query(p, workers=-1)[1] for p in pts_all]
+
+
+def patch_workload(nd, nr, npatch, seed=2026_12_03 + 3):
+    x, y, z = lognormal_box(nd, seed)
+    print("# log-normal data drawn", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed + 1)
+    k = rng.normal(0, 0.03, nd)
+    rx, ry, rz = (rng.uniform(0, 1000., nr) for _ in range(3))
+    dpts = np.column_stack([x, y, z])
+    rpts = np.column_stack([rx, ry, rz])
+    sub = dpts[rng.choice(nd, min(nd, 200_000), replace=False)]
+    ld, lr = kmeans_labels(sub, [dpts, rpts], npatch, seed + 2)
+    return dict(x=x, y=y, z=z, k=k, ld=ld, rx=rx, ry=ry, rz=rz, lr=lr)
+
+
+PATCH_BIN = dict(nbins=30, min_sep=1., max_sep=150., bin_slop=0.05, min_top=4)
+
+
+def patch_step(tc, D, R):
+    """process(D, R) for NN and process(D) for KK over all patch pairs, then the KK
+    jackknife covariance."""
+    nn = tc.NNCorrelation(angle_slop=0.1, var_method='jackknife', **PATCH_BIN)
+    kk = tc.KKCorrelation(angle_slop=1.0, var_method='jackknife', **PATCH_BIN)
+    nn.process(D, R)
+    kk.process(D)
+    cov = kk.estimate_cov('jackknife', cross_patch_weight='simple')
+    return nn, kk, cov
+
+
+def patch_cpu_baseline(tc, w, nn, kk, target_s=15.):
+    """The oracle on a sample of the jobs (patch-pair tree builds + traversals, all host
+    cores); the full-job time is extrapolated by the share of pairs sampled."""
+    from oracle import oracle as O
+    nthreads = int(os.environ.get('OMP_NUM_THREADS', '0')) or (os.cpu_count() or 1)
+    bn = PATCH_BIN
+    bin_size = math.log(bn['max_sep'] / bn['min_sep']) / bn['nbins']
+    jobs = [('nn', ij) for ij, r in nn.results.items() if r.nonzero] + \
+           [('kk', ij) for ij, r in kk.results.items() if r.nonzero]
+    rng = np.random.default_rng(7)
+    rng.shuffle(jobs)
+    tot = float(nn.npairs.sum() + kk.npairs.sum())
+    done, spent = 0., 0.
+    for kind, (i, j) in jobs:
+        a = 0.1 if kind == 'nn' else 1.0
+        b = bin_size * bn['bin_slop']
+        b1 = min(a, b)
+        kw = dict(min_size=bn['min_sep'] * b1 / (2 + 3 * b1), max_size=bn['max_sep'] * b1,
+                  min_top=bn['min_top'])
+        t0 = time.perf_counter()
+        di = w['ld'] == i
+        if kind == 'nn':
+            rj = w['lr'] == j
+            f1 = O.Field(O.NDATA, 2, w['x'][di], w['y'][di], w['z'][di], **kw)
+            f2 = O.Field(O.NDATA, 2, w['rx'][rj], w['ry'][rj], w['rz'][rj], **kw)
+            O.corr2(O.NDATA, 2, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bin_size,
+                    b, a, exact=False, nthreads=nthreads)
+            done += nn.results[(i, j)].npairs.sum()
+        else:
+            dj = w['ld'] == j
+            f1 = O.Field(O.KDATA, 2, w['x'][di], w['y'][di], w['z'][di], k=w['k'][di], **kw)
+            f2 = None if i == j else O.Field(O.KDATA, 2, w['x'][dj], w['y'][dj], w['z'][dj],
+                                             k=w['k'][dj], **kw)
+            O.corr2(O.KDATA, 2, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bin_size,
+                    b, a, exact=False, nthreads=nthreads)
+            done += kk.results[(i, j)].npairs.sum()
+        spent += time.perf_counter() - t0
+        if spent > target_s:
+            break
+    t_full = spent * tot / max(done, 1.)
+    return {"value": tot / t_full, "unit": "pairs/s", "cores": nthreads, "kind": "port",
+            "sample": (f"oracle (OpenMP restatement of TreeCorr's engine) on {100 * done / tot:.2f}% "
+                       f"of the pairs: random patch-pair jobs, each with its two tree builds "
+                       f"({spent:.1f} s); full-job time extrapolated by pair count = "
+                       f"{t_full:.0f} s; jackknife not included")}
+
+
+def run_patch_workload(args):
+    import treecorr_amd as tc
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    w = patch_workload(args.patch_nd, args.patch_nr, args.npatch)
+    print(f"# patch workload ready: {args.patch_nd} data, {args.patch_nr} randoms, "
+          f"{args.npatch} patches", file=sys.stderr, flush=True)
+
+    def cats():
+        return (tc.Catalog(x=w['x'], y=w['y'], z=w['z'], k=w['k'], patch=w['ld']),
+                tc.Catalog(x=w['rx'], y=w['ry'], z=w['rz'], patch=w['lr']))
+    for i in range(args.warmup):
+        patch_step(tc, *cats())
+        print(f"# warmup {i} done", file=sys.stderr, flush=True)
+    sets = [cats() for _ in range(args.steps)]
+    L.tc_reset_kernel_stats()
+    L.tc_set_timing(1)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        nn, kk, cov = patch_step(tc, *sets[i])
+        print(f"# step {i} done", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    L.tc_set_timing(0)
+    ms = ctypes.c_double()
+    launches = ctypes.c_longlong()
+    _lib.check(L.tc_kernel_stats(0, ctypes.byref(ms), ctypes.byref(launches)))
+    npairs = float(nn.npairs.sum() + kk.npairs.sum())
+    step_s = (t1 - t0) / args.steps
+    res = {
+        "metric": f"pairs/s (NN data x randoms + KK auto over {args.npatch} patches, jackknife)",
+        "value": npairs / step_s, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * step_s, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded; log-normal data in a 1000^3 box, uniform randoms, "
+                "k-means patches)",
+        "config": {"workload": f"NN {args.patch_nd} data x {args.patch_nr} randoms and KK auto "
+                               f"of the data, {args.npatch} patches (every patch pair in one "
+                               f"traversal), 30 log bins 1-150, bin_slop=0.05, KK jackknife "
+                               f"covariance; host catalogs, patch split and tree builds included",
+                   "nd": args.patch_nd, "nr": args.patch_nr, "npatch": args.npatch},
+        "traversal_ms_per_step": ms.value / args.steps,
+        "traversal_launches_per_step": launches.value / args.steps,
+        "jobs": {"nn_pairs_stored": len(nn.results), "kk_pairs_stored": len(kk.results)},
+        "counters": {"nn": nn.counters, "kk": kk.counters},
+        "cov_shape": list(cov.shape),
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = patch_cpu_baseline(tc, w, nn, kk)
+    print(json.dumps(res))
+
+
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
     path = os.path.join(ROOT, 'profiles', name)
@@ -209,6 +369,12 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run the plain copy kernel over MIB MiB (HBM counter calibration)')
+    ap.add_argument('--workload', choices=('gg_sphere', 'patch_cov_3d'), default='gg_sphere',
+                    help='gg_sphere: the headline GG workload; patch_cov_3d: NN data x randoms '
+                         'and KK auto over k-means patches with a jackknife covariance')
+    ap.add_argument('--patch-nd', type=int, default=4_000_000)
+    ap.add_argument('--patch-nr', type=int, default=40_000_000)
+    ap.add_argument('--npatch', type=int, default=100)
     args = ap.parse_args()
     if args.calibrate_copy:
         from treecorr_amd import _lib
@@ -223,6 +389,9 @@ def main():
         print(json.dumps({"error": "treecorr_amd runs on one GPU; --gpus must be 1 "
                                    "(the multi-GPU sharding is described, not built)"}))
         sys.exit(2)
+    if args.workload == 'patch_cov_3d':
+        run_patch_workload(args)
+        return
     from treecorr_amd import _lib
     L = _lib.lib()
     hip = Hip()

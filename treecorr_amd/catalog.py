@@ -147,7 +147,10 @@ class Catalog:
 
     @property
     def sumw(self):
-        return float(np.sum(self._w))
+        c = self.__dict__.setdefault('_cache', {})
+        if 'sumw' not in c:
+            c['sumw'] = float(np.sum(self._w))
+        return c['sumw']
 
     @property
     def sumw2(self):
@@ -176,32 +179,45 @@ class Catalog:
         return (self._weighted_var(self._g1) + self._weighted_var(self._g2)) / 2.
 
     def _get_center_size(self):
-        # catalog.py:1330-1341
-        mx, my = np.mean(self._x), np.mean(self._y)
-        mz = 0
-        dsq = (self._x - mx) ** 2 + (self._y - my) ** 2
-        if self._z is not None:
-            mz = np.mean(self._z)
-            dsq += (self._z - mz) ** 2
-        return mx, my, mz, np.max(dsq) ** 0.5
+        # catalog.py:1330-1341 (computed once per catalog)
+        if getattr(self, '_cen_s', None) is None:
+            mx, my = np.mean(self._x), np.mean(self._y)
+            mz = 0
+            dsq = (self._x - mx) ** 2 + (self._y - my) ** 2
+            if self._z is not None:
+                mz = np.mean(self._z)
+                dsq += (self._z - mz) ** 2
+            self._cen_s = (mx, my, mz, np.max(dsq) ** 0.5)
+        return self._cen_s
 
     # ---- patches ----
     def get_patches(self):
-        """A list with one Catalog per patch (or [self] without patches)."""
+        """A list with one Catalog per non-empty patch (or [self] without patches),
+        made once so that their fields are built once (catalog.py:2966-3050)."""
         if self._patch is None:
             return [self]
+        if getattr(self, '_patches', None) is not None:
+            return self._patches
         out = []
+        order = np.argsort(self._patch, kind='stable')
+        bounds = np.searchsorted(self._patch[order], np.arange(self._npatch + 1))
         for p in range(self._npatch):
-            idx = np.where(self._patch == p)[0]
+            idx = order[bounds[p]:bounds[p + 1]]
+            if len(idx) == 0:
+                continue
             sub = Catalog.__new__(Catalog)
             sub.__dict__.update(self.__dict__)
             sub._fields = {}
+            sub._cen_s = None
+            sub._cache = {}
+            sub._patches = None
             sub._select(idx)
             sub._patch = None
-            sub._npatch = 1
+            sub._npatch = self._npatch    # a patch catalog knows the full npatch
             sub._single_patch = p
             sub.name = '%s patch %d' % (self.name, p)
             out.append(sub)
+        self._patches = out
         return out
 
     _single_patch = None
