@@ -329,6 +329,9 @@ inline void __builtin_amdgcn_wave_barrier() {
     emu::arrive(b->wave[b->fib[b->cur].flat / 64]);
 }
 inline void __builtin_amdgcn_fence(int, const char*) {}
+// v_rcp_f64 is an estimate that callers refine with Newton steps; the exact quotient
+// is a valid estimate.
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 
 inline int __double2loint(double d) { uint64_t u; std::memcpy(&u, &d, 8); return (int)(uint32_t)u; }
 inline int __double2hiint(double d) { uint64_t u; std::memcpy(&u, &d, 8); return (int)(uint32_t)(u >> 32); }

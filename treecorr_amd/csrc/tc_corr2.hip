@@ -46,26 +46,88 @@ __device__ __forceinline__ double dist_sq(const DNode& a, const DNode& b) {
     return dx * dx + dy * dy + dz * dz;
 }
 
-// ProjectHelper<Sphere>::calculate_direction, include/ProjectHelper.h:272-345
+#ifndef TC_FAST_LOG
+#define TC_FAST_LOG 1
+#endif
+
+// Natural logarithm for the bin index and meanlogr: the published fdlibm method
+// (Sun's e_log.c): x = 2^k (1+f) with 1+f in [sqrt(2)/2, sqrt(2)), s = f/(2+f),
+// log(1+f) = f - f^2/2 + s (f^2/2 + R(s^2)) with R the degree-7 minimax polynomial of
+// that method, and k ln2 split in two parts.  Error below 1 ulp, like the host log
+// the reference calls; about a third of the instructions of the device library's
+// double-double log.  Only called with finite x > 0 (squared distances >= minsep^2).
+__device__ __forceinline__ double tc_log(double x) {
+#if TC_FAST_LOG
+    int e;
+    double m = frexp(x, &e);           // x = m 2^e, m in [0.5, 1)
+    if (m < 0.70710678118654752440) { m *= 2.; --e; }
+    const double f = m - 1.;
+    const double t = 2. + f;
+    double inv = __builtin_amdgcn_rcp(t);   // t in [1.7, 2.5): two Newton steps
+    inv = fma(fma(-t, inv, 1.), inv, inv);
+    inv = fma(fma(-t, inv, 1.), inv, inv);
+    const double sv = f * inv;
+    const double z = sv * sv, w = z * z;
+    const double t1 = w * fma(w, fma(w, 1.531383769920937332e-01, 2.222219843214978396e-01),
+                              3.999999999940941908e-01);
+    const double t2 = z * fma(w, fma(w, fma(w, 1.479819860511658591e-01,
+                                            1.818357216161805012e-01),
+                                     2.857142874366239149e-01),
+                              6.666666666666735130e-01);
+    const double R = t2 + t1;
+    const double hfsq = 0.5 * f * f;
+    const double dk = double(e);
+    return dk * 6.93147180369123816490e-01 -
+           ((hfsq - (sv * (hfsq + R) + dk * 1.90821492927058770002e-10)) - f);
+#else
+    return log(x);
+#endif
+}
+
+// ProjectHelper<Sphere>::calculate_direction, include/ProjectHelper.h:272-345.  dsq is
+// |p1 - p2|^2 of the (unit) positions, which the caller has already computed as the
+// pair's rsq with the same expression.
 __device__ __forceinline__ void sphere_direction(double x1, double y1, double z1, double x2,
-                                                 double y2, double z2, double& re, double& im) {
-    double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
-    double dsq = dx * dx + dy * dy + dz * dz;
+                                                 double y2, double z2, double dsq, double& re,
+                                                 double& im) {
     double cosA = (z1 - z2) + 0.5 * z2 * dsq;
     double sinA = y1 * x2 - x1 * y2;
     re = sinA;
     im = -cosA;
 }
 
+// 1/x for the projection factors (finite x > 0): hardware reciprocal estimate + two
+// Newton steps, within an ulp of the division.
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.), r, r);
+    return fma(fma(-x, r, 1.), r, r);
+}
+
 // calculate_expmsialpha<2> (include/ProjectHelper.h:45-47) for the shear projections:
-// conj(r^2)/|r|^2 with one reciprocal instead of two divisions.  The projected shears
-// then differ from the reference's by at most an ulp, inside the xi tolerance.
+// conj(r^2)/|r|^2 with a reciprocal instead of two divisions.  The projected shears
+// then differ from the reference's by an ulp or two, inside the xi tolerance.
 __device__ __forceinline__ void expm2ialpha_proj(double re, double im, double& er, double& ei) {
     double n = re * re + im * im;
     n = n > 0. ? n : 1.;
-    const double inv = 1. / n;
+    const double inv = recip(n);
     er = (re * re - im * im) * inv;
     ei = -(re * im + im * re) * inv;
+}
+
+// The two projection factors of a pair with one reciprocal.
+__device__ __forceinline__ void expm2ialpha_proj2(double r1, double i1, double r2, double i2,
+                                                  double& e1r, double& e1i, double& e2r,
+                                                  double& e2i) {
+    double n1 = r1 * r1 + i1 * i1, n2 = r2 * r2 + i2 * i2;
+    n1 = n1 > 0. ? n1 : 1.;
+    n2 = n2 > 0. ? n2 : 1.;
+    const double inv = recip(n1 * n2);
+    const double inv1 = n2 * inv, inv2 = n1 * inv;
+    e1r = (r1 * r1 - i1 * i1) * inv1;
+    e1i = -(r1 * i1 + i1 * r1) * inv1;
+    e2r = (r2 * r2 - i2 * i2) * inv2;
+    e2i = -(r2 * i2 + i2 * r2) * inv2;
 }
 
 __device__ __forceinline__ void cmul(double& zr, double& zi, double er, double ei) {
@@ -85,7 +147,7 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
     const int nb = P.nbins;
     if (k < 0) {
         r = sqrt(rsq);
-        logr = log(r);
+        logr = tc_log(r);
         k = int((logr - P.logminsep) / P.binsize);
     }
     if (k == nb) --k;
@@ -109,14 +171,19 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
         } else {
             // ProjectHelper<Sphere>::Project :370-388; ThreeD normalises first :573-589
             double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
-            if (C == ThreeD) { normalize_recip(x1, y1, z1); normalize_recip(x2, y2, z2); }
-            double dr, di;
-            sphere_direction(x1, y1, z1, x2, y2, z2, dr, di);
-            expm2ialpha_proj(dr, di, er, ei);
+            double dsq = rsq;    // Sphere positions are unit vectors already
+            if (C == ThreeD) {
+                normalize_recip(x1, y1, z1);
+                normalize_recip(x2, y2, z2);
+                const double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+                dsq = dx * dx + dy * dy + dz * dz;
+            }
+            double d2r, d2i, d1r, d1i, e1r, e1i;
+            sphere_direction(x1, y1, z1, x2, y2, z2, dsq, d2r, d2i);
+            sphere_direction(x2, y2, z2, x1, y1, z1, dsq, d1r, d1i);
+            expm2ialpha_proj2(d2r, d2i, d1r, d1i, er, ei, e1r, e1i);
             cmul(g2r, g2i, er, ei);
-            sphere_direction(x2, y2, z2, x1, y1, z1, dr, di);
-            expm2ialpha_proj(dr, di, er, ei);
-            cmul(g1r, g1i, er, ei);
+            cmul(g1r, g1i, e1r, e1i);
         }
         // both_complex_prod(g2, g1), include/ProjectHelper.h:84-101
         double rr = g2r * g1r, ri = g2r * g1i, ir = g2i * g1r, ii = g2i * g1i;
@@ -283,7 +350,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 double r = 0., logr = 0., kk = 0.;
                 if (cls == CL_REFINE || (cls == CL_SINGLE && inrange)) {
                     r = sqrt(rsq);
-                    const double L = log(cls == CL_REFINE ? rsq : r);
+                    const double L = tc_log(cls == CL_REFINE ? rsq : r);
                     logr = cls == CL_REFINE ? 0.5 * L : L;
                     kk = (logr - P.logminsep) / P.binsize;
                 }

@@ -87,9 +87,14 @@ __global__ void k_init_elems(Elems E, const double* x, const double* y, const do
 // NT > 64 the other waves stage batch b+1 while wave 0 adds batch b.
 constexpr long long SEG_WIDE_MAX = 2048;   // levels with at most this many segments use NT=256
 
+#ifndef TC_CHAIN_U
+#define TC_CHAIN_U 16
+#endif
+
 template <int NT> struct SegCfg {
     static constexpr int B = NT > 64 ? 1024 : 128;   // objects per batch
     static constexpr int NC = 5;                      // values per object, at most
+    static constexpr int U = NT > 64 ? TC_CHAIN_U : 8;   // chain read-ahead (power of 2)
 };
 
 template <int NT, int NC, class F>
@@ -135,26 +140,28 @@ __device__ __forceinline__ void ordered_sums(long long start, long long end, dou
             const int cnt = (int)(end - (start + (long long)bat * B) < B
                                       ? end - (start + (long long)bat * B) : B);
             const double* row = cur + lane * B;
-            // Software-pipelined: the next eight values are read while the current
-            // eight are added.  A row always has B entries, so reading ahead of cnt
-            // stays inside it (those values are never added).
-            const int cnt8 = cnt & ~7;
-            double ra[8], rb[8];
+            // Software-pipelined: the next U values are read while the current U are
+            // added, so an LDS read has U dependent additions of time to arrive.  A row
+            // always has B entries, so reading ahead of cnt stays inside it (those
+            // values are never added).
+            constexpr int U = SegCfg<NT>::U;
+            const int cntu = cnt & ~(U - 1);
+            double ra[U], rb[U];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) ra[q] = row[q];
+            for (int q = 0; q < U; ++q) ra[q] = row[q];
             int j = 0;
-            while (j < cnt8) {
+            while (j < cntu) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) rb[q] = row[(j + 8 + q) & (B - 1)];
+                for (int q = 0; q < U; ++q) rb[q] = row[(j + U + q) & (B - 1)];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) acc += ra[q];
-                j += 8;
-                if (j >= cnt8) break;
+                for (int q = 0; q < U; ++q) acc += ra[q];
+                j += U;
+                if (j >= cntu) break;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) ra[q] = row[(j + 8 + q) & (B - 1)];
+                for (int q = 0; q < U; ++q) ra[q] = row[(j + U + q) & (B - 1)];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) acc += rb[q];
-                j += 8;
+                for (int q = 0; q < U; ++q) acc += rb[q];
+                j += U;
             }
             for (; j < cnt; ++j) acc += row[j];
         }
