@@ -84,24 +84,26 @@ __global__ void k_init_elems(Elems E, const double* x, const double* y, const do
 // finishAverages do (src/Cell.cpp:170-354).  The values of a batch of objects are
 // computed in parallel and staged in LDS as NC rows; lane c of wave 0 then adds row c
 // in order, so all NC sums advance with one dependent FP64 add per object.  With
-// NT > 64 the other waves stage batch b+1 while wave 0 adds batch b.
+// NT > 64 the other waves stage batch b+1 while wave 0 adds batch b; the fewer the
+// segments (the longer the chains), the more staging waves (NT = 512 near the root).
 constexpr long long SEG_WIDE_MAX = 2048;   // levels with at most this many segments use NT=256
-
-#ifndef TC_CHAIN_U
-#define TC_CHAIN_U 16
-#endif
+constexpr long long SEG_TOP_MAX = 64;      // ... and with at most this many, NT=512
 
 template <int NT> struct SegCfg {
     static constexpr int B = NT > 64 ? 1024 : 128;   // objects per batch
     static constexpr int NC = 5;                      // values per object, at most
-    static constexpr int U = NT > 64 ? TC_CHAIN_U : 8;   // chain read-ahead (power of 2)
+    static constexpr int U = NT > 64 ? 32 : 8;        // chain read-ahead (power of 2)
+    // LDS batch buffers: with NBUF buffers the stagers fill batch b+NBUF-1 while the
+    // adding wave reads batch b (three buffers measured no faster than two)
+    static constexpr int NBUF = NT > 64 ? 2 : 1;
+    static constexpr int RS = B;   // row stride in LDS (padding measured no faster)
 };
 
 template <int NT, int NC, class F>
 __device__ __forceinline__ void ordered_sums(long long start, long long end, double init,
                                              double* stg, double* bcast, F value,
                                              double (&out)[NC]) {
-    constexpr int B = SegCfg<NT>::B;
+    constexpr int B = SegCfg<NT>::B, RS = SegCfg<NT>::RS;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const long long n = end - start;
     const int nbat = (int)((n + B - 1) / B);
@@ -126,20 +128,26 @@ __device__ __forceinline__ void ordered_sums(long long start, long long end, dou
             const int j = st0 + q * nst;
             if (j < B && b0 + j < end) {
 #pragma unroll
-                for (int c = 0; c < NC; ++c) buf[c * B + j] = v[q][c];
+                for (int c = 0; c < NC; ++c) buf[c * RS + j] = v[q][c];
             }
         }
     };
+    constexpr int NBUF = SegCfg<NT>::NBUF;
     double acc = init;
-    if (stager) stage(0, stg);
+    if (stager) {
+        stage(0, stg);
+        for (int b = 1; b < NBUF - 1 && b < nbat; ++b) stage(b, stg + b * NC * RS);
+    }
     __syncthreads();
     for (int bat = 0; bat < nbat; ++bat) {
-        double* cur = stg + (NT > 64 ? (bat & 1) * NC * B : 0);
-        if (NT > 64 && stager && bat + 1 < nbat) stage(bat + 1, stg + ((bat + 1) & 1) * NC * B);
+        double* cur = stg + (bat % NBUF) * NC * RS;
+        // buffer (bat+NBUF-1) % NBUF was read in the previous batch, before the barrier
+        if (NT > 64 && stager && bat + NBUF - 1 < nbat)
+            stage(bat + NBUF - 1, stg + ((bat + NBUF - 1) % NBUF) * NC * RS);
         if (wave == 0 && lane < NC) {
             const int cnt = (int)(end - (start + (long long)bat * B) < B
                                       ? end - (start + (long long)bat * B) : B);
-            const double* row = cur + lane * B;
+            const double* row = cur + lane * RS;
             // Software-pipelined: the next U values are read while the current U are
             // added, so an LDS read has U dependent additions of time to arrive.  A row
             // always has B entries, so reading ahead of cnt stays inside it (those
@@ -150,17 +158,24 @@ __device__ __forceinline__ void ordered_sums(long long start, long long end, dou
 #pragma unroll
             for (int q = 0; q < U; ++q) ra[q] = row[q];
             int j = 0;
+            // The scheduling barriers keep each group's reads issued before the
+            // additions of the previous group (left alone, the compiler sinks the reads
+            // next to their use and every group waits for a full LDS latency).
             while (j < cntu) {
 #pragma unroll
                 for (int q = 0; q < U; ++q) rb[q] = row[(j + U + q) & (B - 1)];
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int q = 0; q < U; ++q) acc += ra[q];
+                __builtin_amdgcn_sched_barrier(0);
                 j += U;
                 if (j >= cntu) break;
 #pragma unroll
                 for (int q = 0; q < U; ++q) ra[q] = row[(j + U + q) & (B - 1)];
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int q = 0; q < U; ++q) acc += rb[q];
+                __builtin_amdgcn_sched_barrier(0);
                 j += U;
             }
             for (; j < cnt; ++j) acc += row[j];
@@ -209,7 +224,7 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
                                                 TopEntry* tops, BuildParams P, FieldInfo* finfo,
                                                 int level0, long long skip_n) {
     constexpr int B = SegCfg<NT>::B, NC = SegCfg<NT>::NC;
-    __shared__ double stg[(NT > 64 ? 2 : 1) * NC * B];
+    __shared__ double stg[SegCfg<NT>::NBUF * NC * SegCfg<NT>::RS];
     __shared__ double bcast[NC];
     __shared__ double red[NT / 64];
     __shared__ int s_int[2];
@@ -826,7 +841,13 @@ static int build_tree_t(const double* x, const double* y, const double* z, const
         SegOut* sout = bsout.as<SegOut>();
         // Few large segments near the root: four waves each (three stage objects while
         // one adds); many small ones further down: one wave each.
-        if (nseg <= SEG_WIDE_MAX) {
+        if (nseg <= SEG_TOP_MAX) {
+            // the longest chains: more stager waves keep the adding wave fed
+            hipLaunchKernelGGL((k_segment<D, C, 512>), dim3((unsigned)nseg), dim3(512),
+                               0, st, segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
+                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0,
+                               0LL);
+        } else if (nseg <= SEG_WIDE_MAX) {
             hipLaunchKernelGGL((k_segment<D, C, 256>), dim3((unsigned)nseg), dim3(256), 0, st,
                                segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
                                btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0,
