@@ -321,6 +321,17 @@ inline unsigned long long __ballot(int pred) {
     return m;
 }
 inline unsigned int __popcll(unsigned long long x) { return (unsigned int)__builtin_popcountll(x); }
+// Set bits of the mask in the lanes below this one (low and high 32 lanes), plus acc.
+inline unsigned int __builtin_amdgcn_mbcnt_lo(unsigned int m, unsigned int acc) {
+    const unsigned l = (unsigned)emu::lane_id() & 63u;
+    const unsigned long long below = (1ull << l) - 1ull;
+    return acc + (unsigned)__builtin_popcount(m & (unsigned)(below & 0xffffffffull));
+}
+inline unsigned int __builtin_amdgcn_mbcnt_hi(unsigned int m, unsigned int acc) {
+    const unsigned l = (unsigned)emu::lane_id() & 63u;
+    const unsigned long long below = (1ull << l) - 1ull;
+    return acc + (unsigned)__builtin_popcount(m & (unsigned)(below >> 32));
+}
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 // All lanes of the wavefront reach this point before any continues (on the GPU the
 // lanes run in lockstep; here it orders LDS writes and reads across lanes).
@@ -329,6 +340,7 @@ inline void __builtin_amdgcn_wave_barrier() {
     emu::arrive(b->wave[b->fib[b->cur].flat / 64]);
 }
 inline void __builtin_amdgcn_fence(int, const char*) {}
+inline void __builtin_amdgcn_sched_barrier(int) {}   // an instruction-scheduling hint
 // v_rcp_f64 is an estimate that callers refine with Newton steps; the exact quotient
 // is a valid estimate.
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }

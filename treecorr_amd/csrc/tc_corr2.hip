@@ -50,6 +50,7 @@ __device__ __forceinline__ double dist_sq(const DNode& a, const DNode& b) {
 #define TC_FAST_LOG 1
 #endif
 
+
 // Natural logarithm for the bin index and meanlogr: the published fdlibm method
 // (Sun's e_log.c): x = 2^k (1+f) with 1+f in [sqrt(2)/2, sqrt(2)), s = f/(2+f),
 // log(1+f) = f - f^2/2 + s (f^2/2 + R(s^2)) with R the degree-7 minimax polynomial of
@@ -241,9 +242,11 @@ __device__ __forceinline__ int pair_slot(int p, int* tags) {
 }
 
 // Workgroup b takes the items [b*per_block, (b+1)*per_block) of the chunk and writes
-// the children of its split pairs, in item order, to its own region
-// out[b*4*per_block ...] (at most four children per item); bcount[b] gets their
-// number.  No global atomics: the regions are concatenated afterwards.
+// the children of its split pairs to its own region out[b*4*per_block ...] (at most
+// four children per item); bcount[b] gets their number.  Each wavefront claims room
+// for its children with one LDS atomic, so the waves of a workgroup never wait for
+// each other; the order of the children in a region does not matter.  No global
+// atomics: the regions are concatenated afterwards.
 template <int D, int C, class IT>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const DNode* __restrict__ n1, const DNode* __restrict__ n2, InView<IT> V,
@@ -254,7 +257,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     double* __restrict__ pair_hist, unsigned long long* __restrict__ pair_np) {
     using Ops = ItemOps<IT>;
     HIP_DYNAMIC_SHARED(double, sh)
-    __shared__ int s_wsum[TRAV_THREADS / 64];
+    __shared__ unsigned int s_pos;   // children written by this workgroup so far
     __shared__ unsigned long long s_off[TRAV_SLOTS + 1];
     __shared__ int s_tag[PB_SLOTS];
     const int nb = P.nbins;
@@ -270,9 +273,10 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     if (V.nhead > 0)
         for (int t = threadIdx.x; t <= V.nreg; t += blockDim.x) s_off[t] = V.off[t];
     if (threadIdx.x < PB_SLOTS) s_tag[threadIdx.x] = -1;
+    if (threadIdx.x == 0) s_pos = 0;
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     double* hist_all = sh;
     unsigned long long* hnp_all = reinterpret_cast<unsigned long long*>(sh + (size_t)nh * R);
     double* hist = hist_all + (lane & (R - 1));
@@ -281,7 +285,6 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const long long b0 = (long long)blockIdx.x * per_block;
     const long long b1 = b0 + per_block < nin ? b0 + per_block : nin;
     IT* __restrict__ bout = out + (size_t)blockIdx.x * 4 * per_block;
-    unsigned int wpos = 0;    // children written by this workgroup so far (uniform)
     // region cursor of this thread's next item (items of a thread only move forward)
     int reg = 0;
     if (b0 + threadIdx.x < b1 && b0 + threadIdx.x < V.nhead) {
@@ -293,19 +296,24 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
         }
         reg = lo;
     }
+    auto fetch = [&](long long idx) {
+        if (idx < V.nhead) {
+            const unsigned long long g = (unsigned long long)(V.first + idx);
+            while (s_off[reg + 1] <= g) ++reg;
+            return V.in[(long long)reg * V.stride + (long long)(g - s_off[reg])];
+        }
+        return V.tail[idx - V.nhead];
+    };
+    // The next item is loaded one round ahead, so its memory latency overlaps this
+    // round's work.
+    IT next{};
+    if (b0 + threadIdx.x < b1) next = fetch(b0 + threadIdx.x);
     for (long long base = b0; base < b1; base += blockDim.x) {
         const long long idx = base + threadIdx.x;
         IT o0{}, o1{}, o2{}, o3{};
         int no = 0;
+        const IT it = next;
         if (idx < b1) {
-            IT it;
-            if (idx < V.nhead) {
-                const unsigned long long g = (unsigned long long)(V.first + idx);
-                while (s_off[reg + 1] <= g) ++reg;
-                it = V.in[(long long)reg * V.stride + (long long)(g - s_off[reg])];
-            } else {
-                it = V.tail[idx - V.nhead];
-            }
             if (it.j < 0) {
                 // BaseCorr2::process2, src/Corr2.cpp:290-306
                 const DNode c = n1[it.i];
@@ -413,35 +421,33 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 }
             }
         }
-        // Workgroup-wide compaction of the children, in item order.
-        int incl = no;
-        for (int d = 1; d < 64; d <<= 1) {
-            int y = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += y;
+        if (idx + (long long)blockDim.x < b1) next = fetch(idx + (long long)blockDim.x);
+        // Wavefront compaction of the children: no <= 4 is three bits, so the lane's
+        // exclusive prefix is sum_k 2^k (lanes below with bit k set), from three votes.
+        const unsigned long long m0 = __ballot(no & 1), m1 = __ballot(no & 2),
+                                 m2 = __ballot(no & 4);
+        const unsigned int wtot = __popcll(m0) + 2 * __popcll(m1) + 4 * __popcll(m2);
+        if (wtot != 0) {      // uniform over the wavefront
+            unsigned int wbase = 0;
+            if (lane == 0) wbase = atomicAdd(&s_pos, wtot);
+            wbase = (unsigned int)__builtin_amdgcn_readlane((int)wbase, 0);
+            auto below = [](unsigned long long m) {
+                return __builtin_amdgcn_mbcnt_hi((unsigned int)(m >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((unsigned int)m, 0u));
+            };
+            const unsigned int pos = wbase + below(m0) + 2 * below(m1) + 4 * below(m2);
+            if (no > 0) bout[pos] = o0;
+            if (no > 1) bout[pos + 1] = o1;
+            if (no > 2) bout[pos + 2] = o2;
+            if (no > 3) bout[pos + 3] = o3;
         }
-        if (lane == 63) s_wsum[wave] = incl;
-        __syncthreads();
-        unsigned int wbase = 0, tot = 0;
-#pragma unroll
-        for (int k = 0; k < TRAV_THREADS / 64; ++k) {
-            const unsigned int v = (unsigned int)s_wsum[k];
-            if (k < wave) wbase += v;
-            tot += v;
-        }
-        __syncthreads();
-        const unsigned int pos = wpos + wbase + (unsigned int)(incl - no);
-        if (no > 0) bout[pos] = o0;
-        if (no > 1) bout[pos + 1] = o1;
-        if (no > 2) bout[pos + 2] = o2;
-        if (no > 3) bout[pos + 3] = o3;
-        wpos += tot;
     }
-    if (threadIdx.x == 0) bcount[blockIdx.x] = wpos;
     atomicAdd(&hcnt[CNT_VISITS], (unsigned long long)c_vis);
     atomicAdd(&hcnt[CNT_ACCEPTED], (unsigned long long)c_acc);
     atomicAdd(&hcnt[CNT_LEAF], (unsigned long long)c_leaf);
     atomicAdd(&hcnt[CNT_REFINED], (unsigned long long)c_ref);
     __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = s_pos;
     if (Ops::batched) {
         // Add each used pair histogram to that pair's sums in HBM.  A pair's sums are
         // a (hi, lo) pair of arrays: the exact rounding error of every addition into
