@@ -258,7 +258,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     using Ops = ItemOps<IT>;
     HIP_DYNAMIC_SHARED(double, sh)
     __shared__ unsigned int s_pos;   // children written by this workgroup so far
-    __shared__ unsigned long long s_off[TRAV_SLOTS + 1];
+    // region offsets of the in-place head (a launch's output stays below 2^32 items)
+    __shared__ unsigned int s_off[TRAV_SLOTS + 1];
     __shared__ int s_tag[PB_SLOTS];
     const int nb = P.nbins;
     const int nh = NQ2 * nb;
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     for (int t = threadIdx.x; t < (int)(per_slot * nslot) + NCNT; t += blockDim.x)
         reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
     if (V.nhead > 0)
-        for (int t = threadIdx.x; t <= V.nreg; t += blockDim.x) s_off[t] = V.off[t];
+        for (int t = threadIdx.x; t <= V.nreg; t += blockDim.x) s_off[t] = (unsigned int)V.off[t];
     if (threadIdx.x < PB_SLOTS) s_tag[threadIdx.x] = -1;
     if (threadIdx.x == 0) s_pos = 0;
     __syncthreads();
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     // region cursor of this thread's next item (items of a thread only move forward)
     int reg = 0;
     if (b0 + threadIdx.x < b1 && b0 + threadIdx.x < V.nhead) {
-        const unsigned long long g = (unsigned long long)(V.first + b0 + threadIdx.x);
+        const unsigned int g = (unsigned int)(V.first + b0 + threadIdx.x);
         int lo = 0, hi = V.nreg - 1;      // last region with s_off[r] <= g
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     }
     auto fetch = [&](long long idx) {
         if (idx < V.nhead) {
-            const unsigned long long g = (unsigned long long)(V.first + idx);
+            const unsigned int g = (unsigned int)(V.first + idx);
             while (s_off[reg + 1] <= g) ++reg;
             return V.in[(long long)reg * V.stride + (long long)(g - s_off[reg])];
         }
