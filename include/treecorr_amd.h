@@ -112,7 +112,7 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
  * accepted leaf-leaf pairs, refined single-bin tests (no reference counterpart;
  * used for roofline accounting). */
 int tc_corr2_counters(const tc_corr2* c, long long* counters4);
-/* Node pairs per traversal launch (default 2^25; clamped to [256, 2^25]; 0 restores
+/* Node pairs per traversal launch (default 2^26; clamped to [256, 2^26]; 0 restores
  * the default).  Results do not depend on it; tests use small values to exercise
  * the multi-launch work stack.  No reference counterpart. */
 int tc_set_traversal_chunk(long long items);
