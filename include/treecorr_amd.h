@@ -66,6 +66,19 @@ int tc_field_create_device(int data_type, int coords, const double* x, const dou
                            int brute, int mintop, int maxtop, tc_field** out);
 
 /* BaseField::getNObj / getSize / getNTopLevel, src/Field.cpp:624-628. */
+/* The fields of several patches in one call: objects [bounds[r], bounds[r+1]) of the
+ * columns form patch r (bounds[0] = 0, bounds[nroot] = nobj, every patch non-empty).
+ * All trees are built in one level-synchronous pass, the patches being the level-0
+ * segments, and share one node array.  Replaces one Field construction per patch
+ * catalog in TreeCorr's patch loop (treecorr/corr2base.py:855-1050 via
+ * catalog.py:2230-2316, src/Field.cpp:430-506); each out[r] is the same tree as
+ * tc_field_create on patch r's columns.  The shared node array is freed with the
+ * last of the nroot fields. */
+int tc_field_create_multi(int data_type, int coords, const double* x, const double* y,
+                          const double* z, const double* d1, const double* d2, const double* w,
+                          const double* wpos, long long nobj, const long long* bounds, int nroot,
+                          double minsize, double maxsize, int split_method, long long seed,
+                          int brute, int mintop, int maxtop, tc_field** out);
 int tc_field_nobj(const tc_field* f, long long* nobj);
 int tc_field_size(const tc_field* f, double* size);
 int tc_field_ntop(const tc_field* f, long long* ntop);
@@ -104,6 +117,13 @@ int tc_corr2_process_cross(tc_corr2* c, tc_field* f1, tc_field* f2, int dots, in
 int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
                            tc_field* const* fields2, int nfield2, const int* p1, const int* p2,
                            int npair, int quick, int metric, double* pair_out);
+/* Splits a catalog's columns by patch: a stable counting sort on host threads.
+ * Replaces the per-patch np.where + fancy indexing of Catalog.get_patches
+ * (treecorr/catalog.py:2966-3050).  out[c] receives cols[c] ordered by patch
+ * (objects of one patch keep their order); counts[p] the size of patch p.
+ * patch[i] must be in [0, npatch). */
+int tc_patch_split(const int* patch, long long n, int npatch, int ncol, const double* const* cols,
+                   double* const* out, long long* counts);
 /* BaseCorr2::triviallyZero / TriviallyZero, src/Corr2.cpp:590-601, 1107-1128. */
 int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, double y1,
                             double z1, double s1, double x2, double y2, double z2, double s2,

@@ -84,3 +84,37 @@ def test_binning_setup_matches_treecorr_rules():
         tca.NNCorrelation(min_sep=1., bin_size=0.1)
     with pytest.raises(ValueError):
         tca.NNCorrelation(min_sep=10., max_sep=1., nbins=3)
+
+
+def test_patch_split_is_a_stable_split_by_patch():
+    """Catalog.get_patches through tc_patch_split (host threads): every patch catalog
+    holds exactly the objects of np.where(patch == i), in their original order, as
+    TreeCorr's get_patches builds them (catalog.py:3017-3048); empty patches are left
+    out; a label outside [0, npatch) is an error."""
+    import treecorr_amd as tc
+    from treecorr_amd import _lib
+    rng = np.random.default_rng(5)
+    n = 300_000    # several host threads
+    x, y, z, w, k = (rng.random(n) for _ in range(5))
+    lab = rng.integers(0, 41, n)
+    lab[lab == 7] = 8      # patch 7 empty
+    cat = tc.Catalog(x=x, y=y, z=z, w=w, k=k, patch=lab)
+    ps = cat.get_patches()
+    assert len(ps) == 40 and cat.get_patches() is ps
+    assert [p._single_patch for p in ps] == [i for i in range(41) if i != 7]
+    for p in ps:
+        sel = np.where(lab == p._single_patch)[0]
+        for nm in ('x', 'y', 'z', 'w', 'k'):
+            np.testing.assert_array_equal(getattr(p, nm), getattr(cat, nm)[sel])
+        assert p.npatch == 41 and p.patch is None
+        assert math.isclose(p.sumw, float(np.sum(w[sel])), rel_tol=1e-12)
+    bad = np.array([0, 1, 5], dtype=np.int32)
+    col = np.zeros(3)
+    out = np.zeros(3)
+    dp = ctypes.POINTER(ctypes.c_double)
+    counts = np.zeros(3, dtype=np.int64)
+    rc = _lib.lib().tc_patch_split(bad.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 3, 3, 1,
+                                   (dp * 1)(col.ctypes.data_as(dp)),
+                                   (dp * 1)(out.ctypes.data_as(dp)),
+                                   counts.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+    assert rc == _lib.TC_ERR_INVALID

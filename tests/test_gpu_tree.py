@@ -103,3 +103,79 @@ def test_device_tree_degenerate_split_uses_median(gpu, oracle_lib):
     f, nd, ntop = device_tree(gpu, 4, 1, cat, kw)
     ref = O.Field(4, 1, x, y, None, g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw).dump()
     assert_same_tree(nd, ntop, ref)
+
+
+def multi_trees(d, coords, cat, labels, kw):
+    """tc_field_create_multi over the catalog ordered by label: one field per label."""
+    import ctypes
+    from treecorr_amd import _lib
+    from treecorr_amd.field import NODE_DTYPE, preorder
+    L = _lib.lib()
+    order = np.argsort(labels, kind='stable')
+    counts = np.bincount(labels)
+    assert np.all(counts > 0)
+    bounds = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    cols = {k: (None if cat.get(k) is None else np.ascontiguousarray(cat[k][order]))
+            for k in ('x', 'y', 'z', 'k', 'g1', 'g2', 'w')}
+    if coords == 1:
+        cols['z'] = None
+    d1 = cols['k'] if d == 1 else cols['g1'] if d == 4 else None
+    d2 = cols['g2'] if d == 4 else None
+    nroot = len(counts)
+    out = (ctypes.c_void_p * nroot)()
+    _lib.check(L.tc_field_create_multi(
+        d, coords, _lib.dptr(cols['x']), _lib.dptr(cols['y']), _lib.dptr(cols['z']),
+        _lib.dptr(d1), _lib.dptr(d2), _lib.dptr(cols['w']), None, len(order),
+        bounds.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), nroot, kw['min_size'],
+        kw['max_size'], kw['split_method'], 0, int(kw['brute']), kw['min_top'], kw['max_top'],
+        out))
+    try:
+        nn = ctypes.c_longlong()
+        _lib.check(L.tc_field_nnodes(ctypes.c_void_p(out[0]), ctypes.byref(nn)))
+        trees = []
+        for r in range(nroot):
+            h = ctypes.c_void_p(out[r])
+            nt = ctypes.c_longlong()
+            _lib.check(L.tc_field_ntop(h, ctypes.byref(nt)))
+            nodes = np.zeros(nn.value, dtype=NODE_DTYPE)
+            top = np.zeros(nt.value, dtype=np.int32)
+            _lib.check(L.tc_field_copy_nodes(h, nodes.ctypes.data_as(ctypes.c_void_p),
+                                             top.ctypes.data_as(ctypes.c_void_p)))
+            size = ctypes.c_double()
+            _lib.check(L.tc_field_size(h, ctypes.byref(size)))
+            trees.append((nodes[preorder(nodes, top)], len(top), size.value,
+                           {k: (None if v is None else v[bounds[r]:bounds[r + 1]])
+                            for k, v in cols.items()}))
+        return trees
+    finally:
+        for r in range(nroot):
+            L.tc_field_destroy(ctypes.c_void_p(out[r]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d,coords,n,npatch,check', [
+    (4, 3, 60000, 12, 12),      # 512-thread level 0
+    (0, 1, 40000, 300, 40),     # 256-thread level 0
+    (1, 2, 12000, 3000, 60),    # > 2048 roots: one thread or one wave per root at level 0
+])
+def test_multi_root_build_equals_per_patch_trees(gpu, oracle_lib, d, coords, n, npatch, check):
+    """All patch trees built in one pass (the patches as level-0 segments) are the
+    trees the oracle builds for each patch alone: same cells bit for bit, same top
+    cells, same field size; tiny patches (one or two objects) included."""
+    O = oracle_lib
+    cat, scale = C.make_catalog(coords, n, 500 + npatch, 0.3)
+    rng = np.random.default_rng(npatch)
+    labels = rng.integers(0, npatch, n)
+    labels[:npatch] = np.arange(npatch)      # every patch non-empty
+    kw = dict(min_size=scale * 2e-4, max_size=scale * 0.05, split_method=2, brute=False,
+              min_top=3, max_top=10)
+    trees = multi_trees(d, coords, cat, labels, kw)
+    counts = np.bincount(labels)
+    pick = set(rng.choice(npatch, min(check, npatch), replace=False).tolist())
+    pick |= set(np.where(counts <= 2)[0][:10].tolist())
+    for r in sorted(pick):
+        nd, ntop, size, c = trees[r]
+        f = O.Field(d, coords, c['x'], c['y'], c['z'], k=c['k'], g1=c['g1'], g2=c['g2'],
+                    w=c['w'], **kw)
+        assert_same_tree(nd, ntop, f.dump())
+        assert size == np.sqrt(f.sizesq), r

@@ -30,6 +30,8 @@ SIGNATURES = {
                                                      ctypes.POINTER(_vp)]),
     'tc_field_create_device': (_i, [_i, _i] + [_vp] * 7 + [_ll, _d, _d, _i, _ll, _i, _i, _i,
                                                             ctypes.POINTER(_vp)]),
+    'tc_field_create_multi': (_i, [_i, _i] + [_dp] * 7 + [_ll, ctypes.POINTER(_ll), _i, _d, _d,
+                                                          _i, _ll, _i, _i, _i, ctypes.POINTER(_vp)]),
     'tc_field_nobj': (_i, [_vp, ctypes.POINTER(_ll)]),
     'tc_field_size': (_i, [_vp, _dp]),
     'tc_field_ntop': (_i, [_vp, ctypes.POINTER(_ll)]),
@@ -43,6 +45,8 @@ SIGNATURES = {
     'tc_corr2_process_cross': (_i, [_vp, _vp, _vp, _i, _i, _i]),
     'tc_corr2_process_pairs': (_i, [_vp, ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp), _i, _ip,
                                     _ip, _i, _i, _i, _dp]),
+    'tc_patch_split': (_i, [_ip, _ll, _i, _i, ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                            ctypes.POINTER(_ll)]),
     'tc_corr2_trivially_zero': (_i, [_vp, _i, _i] + [_d] * 8 + [ctypes.POINTER(_i)]),
     'tc_corr2_counters': (_i, [_vp, ctypes.POINTER(_ll)]),
     'tc_corr2_destroy': (_i, [_vp]),

@@ -7,10 +7,16 @@ reference's GenerateXYZ (catalog.py:1609-1622), the derived sums/variances used
 at finalize (catalog.py:1200-1330), patches given as integer labels, and a
 one-entry cache of built fields per kind (catalog.py:2230-2316).
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
 from .config import parse_unit
+
+
+# The float64 columns a catalog may hold.
+_COLUMNS = ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2')
 
 
 class Catalog:
@@ -108,8 +114,7 @@ class Catalog:
             self._x, self._y, self._z = x, y, z
 
     def _select(self, idx):
-        for nm in ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2',
-                   '_patch'):
+        for nm in _COLUMNS + ('_patch',):
             a = getattr(self, nm)
             if a is not None:
                 setattr(self, nm, a[idx])
@@ -198,12 +203,27 @@ class Catalog:
             return [self]
         if getattr(self, '_patches', None) is not None:
             return self._patches
+        # One stable counting sort of every column by patch (native, host threads);
+        # each patch catalog then holds views of its contiguous slice.
+        names = [nm for nm in _COLUMNS if getattr(self, nm) is not None]
+        cols = [np.ascontiguousarray(getattr(self, nm), dtype=float) for nm in names]
+        split = [np.empty_like(c) for c in cols]
+        counts = np.zeros(self._npatch, dtype=np.int64)
+        labels = np.ascontiguousarray(self._patch, dtype=np.int32)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.check(_lib.lib().tc_patch_split(
+            labels.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(labels), self._npatch,
+            len(cols), (dp * len(cols))(*[c.ctypes.data_as(dp) for c in cols]),
+            (dp * len(cols))(*[c.ctypes.data_as(dp) for c in split]),
+            counts.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+        bounds = np.concatenate([[0], np.cumsum(counts)])
+        # kept for building all patch fields in one pass (field.build_patch_fields)
+        self._split_cols = dict(zip(names, split))
+        self._split_bounds = np.concatenate([[0], np.cumsum(counts[counts > 0])])
+        self._split_patches = [p for p in range(self._npatch) if counts[p] > 0]
         out = []
-        order = np.argsort(self._patch, kind='stable')
-        bounds = np.searchsorted(self._patch[order], np.arange(self._npatch + 1))
         for p in range(self._npatch):
-            idx = order[bounds[p]:bounds[p + 1]]
-            if len(idx) == 0:
+            if counts[p] == 0:
                 continue
             sub = Catalog.__new__(Catalog)
             sub.__dict__.update(self.__dict__)
@@ -211,7 +231,11 @@ class Catalog:
             sub._cen_s = None
             sub._cache = {}
             sub._patches = None
-            sub._select(idx)
+            sub._split_parent = self
+            for nm in ('_split_cols', '_split_bounds', '_split_patches'):
+                sub.__dict__.pop(nm, None)
+            for nm, a in zip(names, split):
+                setattr(sub, nm, a[bounds[p]:bounds[p + 1]])
             sub._patch = None
             sub._npatch = self._npatch    # a patch catalog knows the full npatch
             sub._single_patch = p
@@ -221,6 +245,7 @@ class Catalog:
         return out
 
     _single_patch = None
+    _split_parent = None     # the catalog whose get_patches made this patch catalog
 
     # ---- fields ----
     def _get_field(self, kind, cls, **kw):

@@ -16,7 +16,7 @@ from . import _lib
 from . import cov as _cov
 from .catalog import Catalog, calculateVarG, calculateVarK
 from .config import parse_unit
-from .field import coord_enum
+from .field import build_patch_fields, coord_enum
 from .util import set_omp_threads
 
 LOG = 0
@@ -333,9 +333,17 @@ class Corr2:
 
     def _patch_fields(self, cats, brute):
         min_size, max_size = self._get_minmax_size()
-        return [self._field(c, min_size=min_size, max_size=max_size,
-                            split_method=self.split_method, brute=brute, min_top=self.min_top,
-                            max_top=self.max_top, coords=self.coords) for c in cats]
+        kw = dict(min_size=min_size, max_size=max_size, split_method=self.split_method,
+                  brute=brute, min_top=self.min_top, max_top=self.max_top, coords=self.coords)
+        # fields already built for these parameters are reused
+        key = (self._letters[0], tuple(sorted(kw.items())))
+        cached = [c._fields.get(self._letters[0]) for c in cats]
+        if all(f is not None and f[0] == key for f in cached):
+            return [f[1] for f in cached]
+        fields = build_patch_fields(self._letters[0], cats, **kw)
+        if fields is not None:
+            return fields
+        return [self._field(c, **kw) for c in cats]
 
     def _run_pairs(self, fields1, fields2, jobs, corr_only):
         """Raw sums of every job (i, j) over fields1[i] x fields2[j] (fields2 None: the

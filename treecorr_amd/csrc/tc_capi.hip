@@ -10,7 +10,11 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <algorithm>
+#include <atomic>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 namespace tc {
@@ -91,12 +95,19 @@ KernelStat& kernel_stat(int which) { return g_stats[which]; }
 
 using namespace tc;
 
+// Node array shared by the trees of a multi-root build (freed with the last of them).
+struct SharedNodes {
+    DNode* p = nullptr;
+    ~SharedNodes() { if (p) (void)hipFree(p); }
+};
+
 struct tc_field {
     int d = 0, coords = 0;
     long long nobj = 0;
     DeviceTree T;
+    std::shared_ptr<SharedNodes> shared;   // set when T.nodes belongs to a multi-root build
     ~tc_field() {
-        if (T.nodes) (void)hipFree(T.nodes);
+        if (T.nodes && !shared) (void)hipFree(T.nodes);
         if (T.dtop) (void)hipFree(T.dtop);
     }
 };
@@ -221,6 +232,61 @@ int tc_field_create_device(int d, int coords, const double* x, const double* y, 
     (void)seed;
     return make_field(true, d, coords, x, y, z, d1, d2, w, wpos, nobj, minsize, maxsize, sm,
                       brute, mintop, maxtop, out);
+}
+
+int tc_field_create_multi(int d, int coords, const double* x, const double* y, const double* z,
+                          const double* d1, const double* d2, const double* w, const double* wpos,
+                          long long nobj, const long long* bounds, int nroot, double minsize,
+                          double maxsize, int sm, long long seed, int brute, int mintop,
+                          int maxtop, tc_field** out) {
+    (void)seed;
+    if (!out || !bounds || nroot <= 0) return tc::fail("NULL or invalid argument");
+    for (int r = 0; r < nroot; ++r) out[r] = nullptr;
+    if (int rc = check_gpu()) return rc;
+    if (int rc = validate_field_args(d, coords, x, y, z, d1, d2, w, nobj, sm, mintop, maxtop))
+        return rc;
+    std::vector<long long> bnd(bounds, bounds + nroot + 1);
+    if (bnd.front() != 0 || bnd.back() != nobj) return tc::fail("bounds must run from 0 to nobj");
+    for (int r = 0; r < nroot; ++r)
+        if (bnd[r + 1] <= bnd[r]) return tc::fail("every patch must hold objects");
+    const double* src[7] = {x, y, z, d1, d2, w, wpos};
+    double* dev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int rc = 0;
+    for (int k = 0; k < 7 && rc == 0; ++k) {
+        if (!src[k] || (coords == Flat && k == 2)) continue;
+        dev[k] = static_cast<double*>(tc::dev_alloc(sizeof(double) * nobj));
+        if (!dev[k]) { rc = tc::fail("out of device memory staging catalog columns"); break; }
+        hipError_t e = hipMemcpy(dev[k], src[k], sizeof(double) * nobj, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = tc::hip_fail(e, "staging catalog columns");
+    }
+    MultiTree M;
+    if (rc == 0) {
+        std::lock_guard<std::mutex> run_lock(g_run_m);
+        rc = tc::build_trees(d, coords, dev[0], dev[1], dev[2], dev[3], dev[4], dev[5], dev[6],
+                             nobj, bnd, minsize, maxsize, sm, brute, mintop, maxtop, M);
+    }
+    for (int k = 0; k < 7; ++k) if (dev[k]) tc::dev_free(dev[k], sizeof(double) * nobj);
+    auto shared = std::make_shared<SharedNodes>();
+    shared->p = M.nodes;
+    if (rc) return rc;
+    for (int r = 0; r < nroot && rc == 0; ++r) {
+        tc_field* f = new tc_field;
+        f->d = d; f->coords = coords; f->nobj = bnd[r + 1] - bnd[r];
+        f->shared = shared;
+        f->T.nodes = M.nodes;
+        f->T.nnodes = M.nnodes;
+        f->T.top = std::move(M.top[r]);
+        f->T.info = M.info[r];
+        hipError_t e = hipMalloc(&f->T.dtop, sizeof(int) * std::max<size_t>(1, f->T.top.size()));
+        if (e == hipSuccess)
+            e = hipMemcpy(f->T.dtop, f->T.top.data(), sizeof(int) * f->T.top.size(),
+                          hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = tc::hip_fail(e, "copying top cells");
+        out[r] = f;
+    }
+    if (rc)
+        for (int r = 0; r < nroot; ++r) { delete out[r]; out[r] = nullptr; }
+    return rc;
 }
 
 int tc_field_nobj(const tc_field* f, long long* nobj) {
@@ -470,6 +536,64 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
 }
 
 int tc_set_traversal_chunk(long long items) { return tc::set_traversal_chunk(items); }
+
+int tc_patch_split(const int* patch, long long n, int npatch, int ncol, const double* const* cols,
+                   double* const* out, long long* counts) {
+    if (!patch || n < 0 || npatch <= 0 || ncol < 0 || !counts || (ncol > 0 && (!cols || !out)))
+        return tc::fail("NULL or invalid argument");
+    for (int c = 0; c < ncol; ++c)
+        if (!cols[c] || !out[c]) return tc::fail("NULL column");
+    // Stable counting sort: thread t counts its contiguous chunk, the chunks' offsets
+    // follow in order within each patch, then every thread scatters its chunk in order.
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::max<long long>(1, std::min<long long>({(long long)std::min(hw, 16u),
+                                                                    n / 65536 + 1}));
+    std::vector<long long> cnt((size_t)nt * npatch, 0);
+    std::atomic<bool> bad{false};
+    auto chunk = [&](int t, long long& b, long long& e) {
+        b = n * t / nt;
+        e = n * (t + 1) / nt;
+    };
+    auto run = [&](auto fn) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(fn, t);
+        fn(0);
+        for (auto& x : th) x.join();
+    };
+    run([&](int t) {
+        long long b, e;
+        chunk(t, b, e);
+        long long* ct = cnt.data() + (size_t)t * npatch;
+        for (long long i = b; i < e; ++i) {
+            const int p = patch[i];
+            if (p < 0 || p >= npatch) { bad = true; return; }
+            ++ct[p];
+        }
+    });
+    if (bad) return tc::fail("patch index out of range");
+    long long run_off = 0;
+    for (int p = 0; p < npatch; ++p) {
+        long long tot = 0;
+        for (int t = 0; t < nt; ++t) {
+            long long& c = cnt[(size_t)t * npatch + p];
+            const long long k = c;
+            c = run_off + tot;       // becomes this chunk's next destination
+            tot += k;
+        }
+        counts[p] = tot;
+        run_off += tot;
+    }
+    run([&](int t) {
+        long long b, e;
+        chunk(t, b, e);
+        long long* dst = cnt.data() + (size_t)t * npatch;
+        for (long long i = b; i < e; ++i) {
+            const long long d = dst[patch[i]]++;
+            for (int c = 0; c < ncol; ++c) out[c][d] = cols[c][i];
+        }
+    });
+    return 0;
+}
 
 int tc_corr2_counters(const tc_corr2* c, long long* counters4) {
     if (!c || !counters4) return tc::fail("NULL argument");

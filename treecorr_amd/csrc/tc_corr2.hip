@@ -937,6 +937,28 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
                       std::vector<int>& toff, size_t& nbytes, size_t& tbytes) -> int {
         long long nn = 0, nt = 0;
         for (auto* t : ts) { nn += t->nnodes; nt += (long long)t->top.size(); }
+        // Trees of one multi-root build share their node array: use it as it is.
+        bool shared = true;
+        for (auto* t : ts) shared = shared && t->nodes == ts[0]->nodes;
+        if (shared) {
+            nbytes = 0;
+            nodes = ts[0]->nodes;
+            tbytes = sizeof(int) * std::max<long long>(nt, 1);
+            tops = static_cast<int*>(dev_alloc(tbytes));
+            if (!tops) return fail("out of device memory for the top cells");
+            std::vector<int> htops;
+            htops.reserve(nt);
+            toff.assign(1, 0);
+            for (auto* t : ts) {
+                htops.insert(htops.end(), t->top.begin(), t->top.end());
+                toff.push_back((int)htops.size());
+            }
+            if (!htops.empty())
+                TC_HIP(hipMemcpyAsync(tops, htops.data(), sizeof(int) * htops.size(),
+                                      hipMemcpyHostToDevice, st));
+            TC_HIP(hipStreamSynchronize(st));
+            return 0;
+        }
         if (nn >= (1LL << 31) - 1) return fail("patches hold too many tree nodes together");
         nbytes = sizeof(DNode) * std::max<long long>(nn, 1);
         tbytes = sizeof(int) * std::max<long long>(nt, 1);
@@ -971,11 +993,13 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
         ~Freer() { for (auto& b : v) dev_free(b.first, b.second); }
     } fr;
     if (int rc = concat(trees1, nodes1, tops1, toff1, nb1, tb1)) return rc;
-    fr.v.push_back({nodes1, nb1}); fr.v.push_back({tops1, tb1});
+    if (nb1) fr.v.push_back({nodes1, nb1});     // nb == 0: a shared array, not ours
+    fr.v.push_back({tops1, tb1});
     if (same) { nodes2 = nodes1; tops2 = tops1; toff2 = toff1; }
     else {
         if (int rc = concat(t2, nodes2, tops2, toff2, nb2, tb2)) return rc;
-        fr.v.push_back({nodes2, nb2}); fr.v.push_back({tops2, tb2});
+        if (nb2) fr.v.push_back({nodes2, nb2});
+        fr.v.push_back({tops2, tb2});
     }
     // item offsets of the pairs
     std::vector<long long> ioff(npair + 1, 0);

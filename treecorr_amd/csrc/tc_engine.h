@@ -20,6 +20,19 @@ int build_tree(int D, int C, const double* x, const double* y, const double* z, 
                double minsize, double maxsize, int sm, int brute, int mintop, int maxtop,
                DeviceTree& T);
 
+// Trees of several root ranges [bounds[r], bounds[r+1]) built together, sharing one
+// node array (node ids, child links and top lists are indices into it).
+struct MultiTree {
+    DNode* nodes = nullptr;     // owned by the caller after the build
+    int nnodes = 0;
+    std::vector<std::vector<int>> top;   // per root, in TreeCorr's order
+    std::vector<FieldInfo> info;         // per root
+};
+int build_trees(int D, int C, const double* x, const double* y, const double* z, const double* d1,
+                const double* d2, const double* w, const double* wpos, long long n,
+                const std::vector<long long>& bounds, double minsize, double maxsize, int sm,
+                int brute, int mintop, int maxtop, MultiTree& M);
+
 // Log-binning parameters of one 2-point correlation (BaseCorr2, src/Corr2.cpp:51-77).
 struct C2Params {
     int nbins, quick;

@@ -30,7 +30,7 @@ struct Seg {
     long long start, end;
     int node, mode, mintop, maxtop;
     unsigned long long key;
-    int depth, pad;
+    int depth, root;     // root: which tree of a multi-root build the segment belongs to
 };
 
 struct SegOut {
@@ -40,7 +40,7 @@ struct SegOut {
     int child_mode, pad;
 };
 
-struct TopEntry { unsigned long long key; int node, pad; };
+struct TopEntry { unsigned long long key; int node, root; };
 
 struct Elems {
     double *x, *y, *z, *wp;
@@ -60,7 +60,8 @@ struct BuildParams {
 template <int D, int C>
 __global__ void k_init_elems(Elems E, const double* x, const double* y, const double* z,
                              const double* d1, const double* d2, const double* w,
-                             const double* wpos, long long n) {
+                             const double* wpos, long long n, const long long* bounds,
+                             int nroot) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double px = x[i], py = y[i], pz = (C == Flat) ? 0. : z[i];
@@ -73,7 +74,13 @@ __global__ void k_init_elems(Elems E, const double* x, const double* y, const do
     if (D == KData) a = float(wi * d1[i]);
     if (D == GData) { a = float(wi * d1[i]); b = float(wi * d2[i]); }
     E.a[i] = a; E.b[i] = b;
-    E.seg[i] = 0;
+    // the root segment holding object i: bounds[r] <= i < bounds[r+1]
+    int lo = 0, hi = nroot - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bounds[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    E.seg[i] = lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -276,7 +283,8 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
         mny = block_min<NT>(lny, red); mxy = block_max<NT>(lxy, red);
         mnz = block_min<NT>(lnz, red); mxz = block_max<NT>(lxz, red);
         if (level0 && tid == 0) {
-            finfo->cx = cx; finfo->cy = cy; finfo->cz = cz; finfo->sizesq = sizesq;
+            // the roots are the level-0 segments, in order
+            finfo[s].cx = cx; finfo[s].cy = cy; finfo[s].cz = cz; finfo[s].sizesq = sizesq;
         }
     }
 
@@ -292,6 +300,7 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
                 int t = atomicAdd(&counters[1], 1);
                 tops[t].key = sg.key;
                 tops[t].node = node;
+                tops[t].root = sg.root;
                 s_int[0] = node;
             }
             __syncthreads();
@@ -411,7 +420,8 @@ template <int D, int C>
 __global__ __launch_bounds__(256) void k_segment_small(const Seg* segs, long long nseg,
                                                        SegOut* sout, Elems E, DNode* nodes,
                                                        int* counters, TopEntry* tops,
-                                                       BuildParams P) {
+                                                       BuildParams P, FieldInfo* finfo,
+                                                       int level0) {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg) return;
     const Seg sg = segs[s];
@@ -421,7 +431,7 @@ __global__ __launch_bounds__(256) void k_segment_small(const Seg* segs, long lon
     float wf = E.w[start];
     double sizesq = 0.;
     double mnx = 0, mxx = 0, mny = 0, mxy = 0, mnz = 0, mxz = 0;
-    if (n > 1) {
+    if (n > 1 || level0) {
         // BuildCellData (src/Cell.cpp:170-200)
         double ax = -0.0, ay = -0.0, az = -0.0, asw = -0.0, aww = -0.0;
         for (long long i = start; i < end; ++i) {
@@ -448,6 +458,9 @@ __global__ __launch_bounds__(256) void k_segment_small(const Seg* segs, long lon
             mny = fmin(mny, py); mxy = fmax(mxy, py);
             mnz = fmin(mnz, pz); mxz = fmax(mxz, pz);
         }
+        if (level0) {
+            finfo[s].cx = cx; finfo[s].cy = cy; finfo[s].cz = cz; finfo[s].sizesq = sizesq;
+        }
     }
     // SetupTopLevelCells (src/Field.cpp:27-81) and BuildCell (src/Cell.cpp:556-615)
     int mode = sg.mode;
@@ -460,6 +473,7 @@ __global__ __launch_bounds__(256) void k_segment_small(const Seg* segs, long lon
             const int t = atomicAdd(&counters[1], 1);
             tops[t].key = sg.key;
             tops[t].node = node;
+            tops[t].root = sg.root;
             mode = SEG_BUILD;
         } else {
             split = true;
@@ -758,7 +772,7 @@ __global__ void k_children(const Seg* segs, const SegOut* sout, const int* segra
     L.depth = R.depth = sg.depth + 1;
     L.key = sg.key;
     R.key = sg.key | (1ull << (62 - sg.depth));
-    L.pad = R.pad = 0;
+    L.root = R.root = sg.root;
     if (o.child_mode == SEG_BUILD) { L.node = o.childL; R.node = o.childL + 1; }
     else { L.node = R.node = -1; }
     next[2 * r] = L;
@@ -783,55 +797,66 @@ static int dalloc(DevBuf& b, size_t bytes) {
     return 0;
 }
 
+// Builds one tree per root range [bounds[r], bounds[r+1]) of the columns (each range
+// non-empty), all in the same level-synchronous pass: the roots are the level-0
+// segments.  The trees share one node array.
 template <int D, int C>
-static int build_tree_t(const double* x, const double* y, const double* z, const double* d1,
-                        const double* d2, const double* w, const double* wpos, long long n,
-                        double minsize, double maxsize, int sm, int brute, int mintop,
-                        int maxtop, DeviceTree& T) {
+static int build_trees_t(const double* x, const double* y, const double* z, const double* d1,
+                         const double* d2, const double* w, const double* wpos, long long n,
+                         const std::vector<long long>& bounds, double minsize, double maxsize,
+                         int sm, int brute, int mintop, int maxtop, MultiTree& M) {
     hipStream_t st = stream();
     const int TPB = 256;
     auto grid = [&](long long m) { return dim3((unsigned)((m + TPB - 1) / TPB)); };
-
-    DevBuf bx[2], by[2], bz[2], bwp[2], bw[2], ba[2], bb[2], bseg[2];
-    for (int k = 0; k < 2; ++k) {
-        if (dalloc(bx[k], n * 8) || dalloc(by[k], n * 8) || dalloc(bz[k], n * 8) ||
-            dalloc(bwp[k], n * 8) || dalloc(bw[k], n * 4) || dalloc(ba[k], n * 4) ||
-            dalloc(bb[k], n * 4) || dalloc(bseg[k], n * 4))
-            return 2;
-    }
+    const int nroot = (int)bounds.size() - 1;
+    const long long nb_scan = (n + 1 + SCAN_B - 1) / SCAN_B + 1;
+    DevBuf ecols[2][8];
     Elems E[2];
-    for (int k = 0; k < 2; ++k)
-        E[k] = Elems{bx[k].as<double>(), by[k].as<double>(), bz[k].as<double>(),
-                     bwp[k].as<double>(), bw[k].as<float>(), ba[k].as<float>(),
-                     bb[k].as<float>(), bseg[k].as<int>()};
-    DevBuf bflags, bpre, blpos, brpos, bdest, bnewseg, bord, bkey, bscan;
+    for (int k = 0; k < 2; ++k) {
+        for (int c = 0; c < 4; ++c) if (dalloc(ecols[k][c], n * 8)) return 2;
+        for (int c = 4; c < 8; ++c) if (dalloc(ecols[k][c], n * 4)) return 2;
+        E[k].x = ecols[k][0].as<double>(); E[k].y = ecols[k][1].as<double>();
+        E[k].z = ecols[k][2].as<double>(); E[k].wp = ecols[k][3].as<double>();
+        E[k].w = ecols[k][4].as<float>(); E[k].a = ecols[k][5].as<float>();
+        E[k].b = ecols[k][6].as<float>(); E[k].seg = ecols[k][7].as<int>();
+    }
+    DevBuf bflags, bpre, blpos, brpos, bdest, bnewseg, bord, bkey, bscan, bbounds;
     DevBuf bsegs[2], bsout, bsegflags, bsegrank, bcnt, btops, bfinfo;
-    long long nb_scan = (n + 1 + SCAN_B - 1) / SCAN_B + 1;
+    // top cells: at most one per object, and at most 2^maxtop per root
+    const long long maxtops = std::min<long long>(n, (long long)nroot << std::min(maxtop, 40));
+    const long long nsegmax = std::max<long long>(n, nroot);
     if (dalloc(bflags, n * 4) || dalloc(bpre, (n + 1) * 4) || dalloc(blpos, n * 4) ||
         dalloc(brpos, n * 4) || dalloc(bdest, n * 8) || dalloc(bnewseg, n * 4) ||
         dalloc(bord, n * 4) || dalloc(bkey, n * 8) || dalloc(bscan, nb_scan * 4) ||
-        dalloc(bsegs[0], n * sizeof(Seg)) || dalloc(bsegs[1], n * sizeof(Seg)) ||
-        dalloc(bsout, n * sizeof(SegOut)) || dalloc(bsegflags, n * 4) ||
-        dalloc(bsegrank, (n + 1) * 4) || dalloc(bcnt, 16) ||
-        dalloc(btops, sizeof(TopEntry) * (size_t)(std::min<long long>(n, 1LL << std::min(maxtop, 40)) + 1)) ||
-        dalloc(bfinfo, sizeof(FieldInfo)))
+        dalloc(bsegs[0], nsegmax * sizeof(Seg)) || dalloc(bsegs[1], nsegmax * sizeof(Seg)) ||
+        dalloc(bsout, nsegmax * sizeof(SegOut)) || dalloc(bsegflags, nsegmax * 4) ||
+        dalloc(bsegrank, (nsegmax + 1) * 4) || dalloc(bcnt, 16) ||
+        dalloc(btops, sizeof(TopEntry) * (size_t)(maxtops + 1)) ||
+        dalloc(bfinfo, sizeof(FieldInfo) * nroot) || dalloc(bbounds, sizeof(long long) * (nroot + 1)))
         return 2;
 
-    // Nodes: at most 2n-1 (each split adds two).
-    long long maxnodes = 2 * n;
-    TC_HIP(hipMalloc(&T.nodes, sizeof(DNode) * maxnodes));
+    // Nodes: at most 2n-1 per root's objects (each split adds two).
+    const long long maxnodes = 2 * n;
+    TC_HIP(hipMalloc(&M.nodes, sizeof(DNode) * maxnodes));
 
+    TC_HIP(hipMemcpyAsync(bbounds.p, bounds.data(), sizeof(long long) * (nroot + 1),
+                          hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL((k_init_elems<D, C>), grid(n), dim3(TPB), 0, st, E[0], x, y, z, d1, d2, w,
-                       wpos, n);
-    Seg root{};
-    root.start = 0; root.end = n; root.node = -1; root.mode = SEG_TOP;
-    root.mintop = mintop; root.maxtop = maxtop; root.key = 0; root.depth = 0;
-    TC_HIP(hipMemcpyAsync(bsegs[0].p, &root, sizeof(Seg), hipMemcpyHostToDevice, st));
+                       wpos, n, bbounds.as<long long>(), nroot);
+    std::vector<Seg> roots(nroot);
+    for (int r = 0; r < nroot; ++r) {
+        Seg& g = roots[r];
+        g = Seg{};
+        g.start = bounds[r]; g.end = bounds[r + 1]; g.node = -1; g.mode = SEG_TOP;
+        g.mintop = mintop; g.maxtop = maxtop; g.key = 0; g.depth = 0; g.root = r;
+    }
+    TC_HIP(hipMemcpyAsync(bsegs[0].p, roots.data(), sizeof(Seg) * nroot, hipMemcpyHostToDevice,
+                          st));
     TC_HIP(hipMemsetAsync(bcnt.p, 0, 16, st));
 
     BuildParams P{minsize * minsize, maxsize * maxsize, sm, brute};
     int cur = 0;
-    long long nseg = 1;
+    long long nseg = nroot;
     int level = 0;
     int* hcnt = nullptr;
     TC_HIP(hipHostMalloc((void**)&hcnt, 16));
@@ -839,26 +864,26 @@ static int build_tree_t(const double* x, const double* y, const double* z, const
     while (nseg > 0) {
         Seg* segs = bsegs[cur].as<Seg>();
         SegOut* sout = bsout.as<SegOut>();
-        // Few large segments near the root: four waves each (three stage objects while
-        // one adds); many small ones further down: one wave each.
+        const int l0 = level == 0 ? 1 : 0;
+        // Few large segments near the root: 512 or 256 threads each (the other waves
+        // stage objects while one adds); many small ones further down: one wave each.
         if (nseg <= SEG_TOP_MAX) {
             // the longest chains: more stager waves keep the adding wave fed
             hipLaunchKernelGGL((k_segment<D, C, 512>), dim3((unsigned)nseg), dim3(512),
-                               0, st, segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
-                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0,
-                               0LL);
+                               0, st, segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
+                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0, 0LL);
         } else if (nseg <= SEG_WIDE_MAX) {
             hipLaunchKernelGGL((k_segment<D, C, 256>), dim3((unsigned)nseg), dim3(256), 0, st,
-                               segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
-                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0,
-                               0LL);
+                               segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
+                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0, 0LL);
         } else {
             // many segments: the small ones one thread each, the rest one wave each
             hipLaunchKernelGGL((k_segment_small<D, C>), grid(nseg), dim3(TPB), 0, st, segs, nseg,
-                               sout, E[cur], T.nodes, bcnt.as<int>(), btops.as<TopEntry>(), P);
+                               sout, E[cur], M.nodes, bcnt.as<int>(), btops.as<TopEntry>(), P,
+                               bfinfo.as<FieldInfo>(), l0);
             hipLaunchKernelGGL((k_segment<D, C, 64>), dim3((unsigned)nseg), dim3(64), 0, st,
-                               segs, 0LL, sout, E[cur], T.nodes, bcnt.as<int>(),
-                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), level == 0 ? 1 : 0,
+                               segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
+                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0,
                                SEG_SMALL_N);
         }
         hipLaunchKernelGGL(k_seg_split_flags, grid(nseg), dim3(TPB), 0, st, sout, (int)nseg,
@@ -891,32 +916,56 @@ static int build_tree_t(const double* x, const double* y, const double* z, const
         if (level > 4 * 64 + 64) { rc = fail("tree build did not terminate"); break; }
     }
     int cnts[2];
+    M.info.resize(nroot);
     TC_HIP(hipMemcpyAsync(cnts, bcnt.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    TC_HIP(hipMemcpyAsync(&T.info, bfinfo.p, sizeof(FieldInfo), hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(M.info.data(), bfinfo.p, sizeof(FieldInfo) * nroot,
+                          hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));
     (void)hipHostFree(hcnt);
     if (rc) return rc;
-    T.nnodes = cnts[0];
+    M.nnodes = cnts[0];
     std::vector<TopEntry> tops(cnts[1]);
     TC_HIP(hipMemcpy(tops.data(), btops.p, sizeof(TopEntry) * cnts[1], hipMemcpyDeviceToHost));
-    std::sort(tops.begin(), tops.end(),
-              [](const TopEntry& a, const TopEntry& b) { return a.key < b.key; });
-    T.top.resize(tops.size());
-    for (size_t i = 0; i < tops.size(); ++i) T.top[i] = tops[i].node;
-    TC_HIP(hipMalloc(&T.dtop, sizeof(int) * std::max<size_t>(1, T.top.size())));
-    TC_HIP(hipMemcpy(T.dtop, T.top.data(), sizeof(int) * T.top.size(), hipMemcpyHostToDevice));
+    // each root's top cells in the reference's order (the path keys' order)
+    std::sort(tops.begin(), tops.end(), [](const TopEntry& a, const TopEntry& b) {
+        return a.root != b.root ? a.root < b.root : a.key < b.key;
+    });
+    M.top.assign(nroot, std::vector<int>());
+    for (const TopEntry& t : tops) M.top[t.root].push_back(t.node);
     return 0;
+}
+
+int build_trees(int D, int C, const double* x, const double* y, const double* z, const double* d1,
+                const double* d2, const double* w, const double* wpos, long long n,
+                const std::vector<long long>& bounds, double minsize, double maxsize, int sm,
+                int brute, int mintop, int maxtop, MultiTree& M) {
+    if (bounds.size() < 2 || bounds.front() != 0 || bounds.back() != n)
+        return fail("invalid root ranges");
+    for (size_t r = 0; r + 1 < bounds.size(); ++r)
+        if (bounds[r + 1] <= bounds[r]) return fail("every root range must hold objects");
+#define TC_BT(DD, CC) return build_trees_t<DD, CC>(x, y, z, d1, d2, w, wpos, n, bounds, minsize, maxsize, sm, brute, mintop, maxtop, M)
+    if (D == NData) { if (C == Flat) TC_BT(NData, Flat); if (C == ThreeD) TC_BT(NData, ThreeD); TC_BT(NData, Sphere); }
+    if (D == KData) { if (C == Flat) TC_BT(KData, Flat); if (C == ThreeD) TC_BT(KData, ThreeD); TC_BT(KData, Sphere); }
+    if (C == Flat) TC_BT(GData, Flat); if (C == ThreeD) TC_BT(GData, ThreeD); TC_BT(GData, Sphere);
+#undef TC_BT
 }
 
 int build_tree(int D, int C, const double* x, const double* y, const double* z, const double* d1,
                const double* d2, const double* w, const double* wpos, long long n,
                double minsize, double maxsize, int sm, int brute, int mintop, int maxtop,
                DeviceTree& T) {
-#define TC_BT(DD, CC) return build_tree_t<DD, CC>(x, y, z, d1, d2, w, wpos, n, minsize, maxsize, sm, brute, mintop, maxtop, T)
-    if (D == NData) { if (C == Flat) TC_BT(NData, Flat); if (C == ThreeD) TC_BT(NData, ThreeD); TC_BT(NData, Sphere); }
-    if (D == KData) { if (C == Flat) TC_BT(KData, Flat); if (C == ThreeD) TC_BT(KData, ThreeD); TC_BT(KData, Sphere); }
-    if (C == Flat) TC_BT(GData, Flat); if (C == ThreeD) TC_BT(GData, ThreeD); TC_BT(GData, Sphere);
-#undef TC_BT
+    MultiTree M;
+    const int rc = build_trees(D, C, x, y, z, d1, d2, w, wpos, n, {0, n}, minsize, maxsize, sm,
+                               brute, mintop, maxtop, M);
+    T.nodes = M.nodes;      // the tree owns the node array from here on
+    M.nodes = nullptr;
+    if (rc) return rc;
+    T.nnodes = M.nnodes;
+    T.top = std::move(M.top[0]);
+    T.info = M.info[0];
+    TC_HIP(hipMalloc(&T.dtop, sizeof(int) * std::max<size_t>(1, T.top.size())));
+    TC_HIP(hipMemcpy(T.dtop, T.top.data(), sizeof(int) * T.top.size(), hipMemcpyHostToDevice));
+    return 0;
 }
 
 }  // namespace tc

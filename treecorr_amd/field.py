@@ -34,7 +34,7 @@ class Field:
     _d = None
 
     def __init__(self, cat, *, min_size=0, max_size=None, split_method='mean', brute=False,
-                 min_top=None, max_top=10, coords=None, rng=None, logger=None):
+                 min_top=None, max_top=10, coords=None, rng=None, logger=None, _handle=None):
         self.ntot = cat.ntot
         self.min_size = float(min_size) if not brute else 0.
         self.max_size = float(max_size) if max_size is not None else np.inf
@@ -48,6 +48,9 @@ class Field:
         self.max_top = max_top
         self.coords = coords if coords is not None else cat.coords
         self._coords = coord_enum(self.coords)
+        if _handle is not None:     # built with the other patches, see build_patch_fields
+            self.data = _handle
+            return
         x = np.ascontiguousarray(cat.x, dtype=float)
         y = np.ascontiguousarray(cat.y, dtype=float)
         z = None if cat.z is None else np.ascontiguousarray(cat.z, dtype=float)
@@ -125,6 +128,47 @@ class GField(Field):
     def _payload(self, cat):
         return (np.ascontiguousarray(cat.g1, dtype=float),
                 np.ascontiguousarray(cat.g2, dtype=float))
+
+
+_KIND_CLS = {'N': 'NField', 'K': 'KField', 'G': 'GField'}
+
+
+def build_patch_fields(kind, cats, **kw):
+    """Fields of the patch catalogs cats in one device build (tc_field_create_multi),
+    when they are all the non-empty patches of one catalog, in order: the patch trees
+    are built in one level-synchronous pass over the parent's columns, already
+    ordered by patch.  Each field lands in its catalog's field cache, as
+    Catalog.get<kind>Field would build it.  Returns None when cats are not such a
+    list."""
+    parent = getattr(cats[0], '_split_parent', None)
+    if parent is None or any(getattr(c, '_split_parent', None) is not parent for c in cats):
+        return None
+    if [c._single_patch for c in cats] != parent._split_patches:
+        return None
+    cols = parent._split_cols
+    if kind == 'K' and cols.get('_k') is None:
+        raise TypeError("k is not defined for this catalog")
+    if kind == 'G' and (cols.get('_g1') is None or cols.get('_g2') is None):
+        raise TypeError("g1,g2 are not defined for this catalog")
+    cls = globals()[_KIND_CLS[kind]]
+    proto = cls(cats[0], _handle=ctypes.c_void_p(), **kw)   # the parameters as a field holds them
+    d1 = cols.get('_k') if kind == 'K' else cols.get('_g1') if kind == 'G' else None
+    d2 = cols.get('_g2') if kind == 'G' else None
+    bounds = np.ascontiguousarray(parent._split_bounds, dtype=np.int64)
+    n = int(bounds[-1])
+    out = (ctypes.c_void_p * len(cats))()
+    _lib.check(_lib.lib().tc_field_create_multi(
+        cls._d, proto._coords, _lib.dptr(cols['_x']), _lib.dptr(cols['_y']),
+        _lib.dptr(cols.get('_z')), _lib.dptr(d1), _lib.dptr(d2), _lib.dptr(cols['_w']),
+        _lib.dptr(cols.get('_wpos')), n, bounds.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+        len(cats), proto.min_size, proto.max_size, proto._sm, 0, int(proto.brute), proto.min_top,
+        proto.max_top, out))
+    proto.data = None
+    fields = [cls(c, _handle=ctypes.c_void_p(out[r]), **kw) for r, c in enumerate(cats)]
+    key = (kind, tuple(sorted(kw.items())))
+    for c, f in zip(cats, fields):
+        c._fields[kind] = (key, f)
+    return fields
 
 
 def preorder(nodes, top):
