@@ -623,42 +623,40 @@ __global__ void k_items_cross(const int* top1, int ntop1, const int* top2, int n
     }
 }
 
-// Initial items of a batched run.  Patch pair p (pi[p], pj[p]) owns the items
-// [ioff[p], ioff[p+1]): for an auto pair (pauto[p] != 0) the process2 items of the
-// patch's top cells then its top-cell pairs a<b, otherwise all top-cell pairs of the
-// two patches.  Top cells are global node ids, patch q's at tops[toff[q] ...].
-__global__ void k_items_pairs(const int* __restrict__ pi, const int* __restrict__ pj,
-                              const int* __restrict__ pauto, const long long* __restrict__ ioff,
-                              int npair, const int* __restrict__ tops1,
-                              const int* __restrict__ toff1, const int* __restrict__ tops2,
-                              const int* __restrict__ toff2, ItemP* __restrict__ out) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ioff[npair]) return;
-    int lo = 0, hi = npair - 1;     // last pair with ioff[p] <= t
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ioff[mid] <= t) lo = mid; else hi = mid - 1;
-    }
-    const int p = lo;
-    long long q = t - ioff[p];
+// Initial items of a batched run, one workgroup per patch pair.  Pair p (pi[p],
+// pj[p]) owns the items [ioff[p], ioff[p+1]): for an auto pair (pauto[p] != 0) the
+// process2 items of the patch's top cells then its top-cell pairs a<b, otherwise all
+// top-cell pairs of the two patches.  Top cells are global node ids, patch q's at
+// tops[toff[q] ...].
+__global__ __launch_bounds__(256) void k_items_pairs(
+    const int* __restrict__ pi, const int* __restrict__ pj, const int* __restrict__ pauto,
+    const long long* __restrict__ ioff, const int* __restrict__ tops1,
+    const int* __restrict__ toff1, const int* __restrict__ tops2,
+    const int* __restrict__ toff2, ItemP* __restrict__ out) {
+    const int p = blockIdx.x;
+    const long long o0 = ioff[p], cnt = ioff[p + 1] - o0;
     const int i = pi[p], j = pj[p];
     const int* ta = tops1 + toff1[i];
     const long long na = toff1[i + 1] - toff1[i];
-    if (pauto[p]) {
-        if (q < na) { out[t] = ItemP{ta[q], -1, p, 0}; return; }
-        q -= na;
-        // row-major over a < b: row a holds na-1-a pairs
-        long long a = (long long)((2. * na - 1 - sqrt((2. * na - 1) * (2. * na - 1) - 8. * double(q))) / 2);
-        if (a < 0) a = 0;
-        auto row_start = [&](long long r) { return r * (2 * na - r - 1) / 2; };
-        while (a > 0 && row_start(a) > q) --a;
-        while (row_start(a + 1) <= q) ++a;
-        const long long bb = a + 1 + (q - row_start(a));
-        out[t] = ItemP{ta[a], ta[bb], p, 0};
-    } else {
-        const int* tb = tops2 + toff2[j];
-        const long long nbt = toff2[j + 1] - toff2[j];
-        out[t] = ItemP{ta[q / nbt], tb[q % nbt], p, 0};
+    const bool autop = pauto[p] != 0;
+    const int* tb = tops2 + toff2[j];
+    const long long nbt = toff2[j + 1] - toff2[j];
+    auto row_start = [&](long long r) { return r * (2 * na - r - 1) / 2; };
+    for (long long q = threadIdx.x; q < cnt; q += blockDim.x) {
+        if (!autop) {
+            out[o0 + q] = ItemP{ta[q / nbt], tb[q % nbt], p, 0};
+        } else if (q < na) {
+            out[o0 + q] = ItemP{ta[q], -1, p, 0};
+        } else {
+            // row-major over a < b: row a holds na-1-a pairs
+            const long long u = q - na;
+            long long a = (long long)((2. * na - 1 -
+                                       sqrt((2. * na - 1) * (2. * na - 1) - 8. * double(u))) / 2);
+            if (a < 0) a = 0;
+            while (a > 0 && row_start(a) > u) --a;
+            while (row_start(a + 1) <= u) ++a;
+            out[o0 + q] = ItemP{ta[a], ta[a + 1 + (u - row_start(a))], p, 0};
+        }
     }
 }
 
@@ -1039,8 +1037,8 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
     if (zero_slots(S, nb)) return 2;
     if (ensure_stack(S, sizeof(ItemP) * (ninit + 4 * TRAV_CHUNK), 0)) return 2;
     if (ninit > 0) {
-        hipLaunchKernelGGL(k_items_pairs, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st,
-                           dpi, dpj, dpa, dio, npair, tops1, dt1, tops2, dt2,
+        hipLaunchKernelGGL(k_items_pairs, dim3((unsigned)npair), dim3(256), 0, st,
+                           dpi, dpj, dpa, dio, tops1, dt1, tops2, dt2,
                            static_cast<ItemP*>(S.stack));
         TC_HIP(hipGetLastError());
     }
