@@ -207,8 +207,10 @@ def lognormal_box(n, seed, box=1000., ngrid=256):
     return ((ix + rng.random(n)) * h, (iy + rng.random(n)) * h, (iz + rng.random(n)) * h)
 
 
-def kmeans_labels(pts_fit, pts_all, npatch, seed, iters=10):
-    """Lloyd's k-means on a subsample, then nearest-centre labels for every point list."""
+def kmeans_centers(pts_fit, npatch, seed, iters=10):
+    """Patch centres from Lloyd's k-means on a subsample (workload preparation; the
+    catalogs then get their patches from these centres, as with TreeCorr's
+    patch_centers)."""
     from scipy.spatial import cKDTree
     rng = np.random.default_rng(seed)
     cen = pts_fit[rng.choice(len(pts_fit), npatch, replace=False)]
@@ -219,8 +221,7 @@ def kmeans_labels(pts_fit, pts_all, npatch, seed, iters=10):
             m = lab == k
             if np.any(m):
                 cen[k] = pts_fit[m].mean(axis=0)
-    tree = cKDTree(cen)
-    return [tree.query(p, workers=-1)[1] for p in pts_all]
+    return cen
 
 
 def patch_workload(nd, nr, npatch, seed=2026_12_03 + 3):
@@ -229,11 +230,10 @@ def patch_workload(nd, nr, npatch, seed=2026_12_03 + 3):
     rng = np.random.default_rng(seed + 1)
     k = rng.normal(0, 0.03, nd)
     rx, ry, rz = (rng.uniform(0, 1000., nr) for _ in range(3))
-    dpts = np.column_stack([x, y, z])
-    rpts = np.column_stack([rx, ry, rz])
-    sub = dpts[rng.choice(nd, min(nd, 200_000), replace=False)]
-    ld, lr = kmeans_labels(sub, [dpts, rpts], npatch, seed + 2)
-    return dict(x=x, y=y, z=z, k=k, ld=ld, rx=rx, ry=ry, rz=rz, lr=lr)
+    pick = rng.choice(nd, min(nd, 200_000), replace=False)
+    sub = np.column_stack([x[pick], y[pick], z[pick]])
+    cen = kmeans_centers(sub, npatch, seed + 2)
+    return dict(x=x, y=y, z=z, k=k, rx=rx, ry=ry, rz=rz, centers=cen)
 
 
 PATCH_BIN = dict(nbins=30, min_sep=1., max_sep=150., bin_slop=0.05, min_top=4)
@@ -306,12 +306,14 @@ def run_patch_workload(args):
           f"{args.npatch} patches", file=sys.stderr, flush=True)
 
     def cats():
-        return (tc.Catalog(x=w['x'], y=w['y'], z=w['z'], k=w['k'], patch=w['ld']),
-                tc.Catalog(x=w['rx'], y=w['ry'], z=w['rz'], patch=w['lr']))
+        # patches from the centres (QuickAssign on the GPU), for both catalogs
+        return (tc.Catalog(x=w['x'], y=w['y'], z=w['z'], k=w['k'], patch_centers=w['centers']),
+                tc.Catalog(x=w['rx'], y=w['ry'], z=w['rz'], patch_centers=w['centers']))
     for i in range(args.warmup):
         patch_step(tc, *cats())
         print(f"# warmup {i} done", file=sys.stderr, flush=True)
     sets = [cats() for _ in range(args.steps)]
+    w['ld'], w['lr'] = sets[0][0].patch, sets[0][1].patch
     L.tc_reset_kernel_stats()
     L.tc_set_timing(1)
     t0 = time.perf_counter()
@@ -331,7 +333,7 @@ def run_patch_workload(args):
         "warmup": args.warmup, "ms_per_step": 1e3 * step_s, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded; log-normal data in a 1000^3 box, uniform randoms, "
-                "k-means patches)",
+                "patches from k-means centres, assigned on the GPU)",
         "config": {"workload": f"NN {args.patch_nd} data x {args.patch_nr} randoms and KK auto "
                                f"of the data, {args.npatch} patches (every patch pair in one "
                                f"traversal), 30 log bins 1-150, bin_slop=0.05, KK jackknife "

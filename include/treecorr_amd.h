@@ -117,6 +117,14 @@ int tc_corr2_process_cross(tc_corr2* c, tc_field* f1, tc_field* f2, int dots, in
 int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
                            tc_field* const* fields2, int nfield2, const int* p1, const int* p2,
                            int npair, int quick, int metric, double* pair_out);
+/* Patch of every object from the patch centres: the nearest centre, computed on the
+ * GPU.  Replaces QuickAssign, src/KMeans.cpp:890-942 (bound at src/Field.cpp:700), which
+ * Catalog._assign_patches calls for a catalog given patch_centers
+ * (treecorr/catalog.py:1556-1566).  centers holds npatch rows of (x,y) or, when z is
+ * not NULL, (x,y,z); the squared distance is (dx^2+dy^2)+dz^2 and the lowest index
+ * wins a tie, as in the reference.  patch receives int32 labels. */
+int tc_quick_assign(const double* centers, int npatch, const double* x, const double* y,
+                    const double* z, long long n, int* patch);
 /* Splits a catalog's columns by patch: a stable counting sort on host threads.
  * Replaces the per-patch np.where + fancy indexing of Catalog.get_patches
  * (treecorr/catalog.py:2966-3050).  out[c] receives cols[c] ordered by patch

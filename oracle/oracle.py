@@ -178,3 +178,24 @@ def corr3_multipole(d, coords, f, nbins, min_sep, max_sep, bin_size, b, a, max_n
     res = {k: v.reshape(shape) for k, v in out.items()}
     res['counters'] = dict(gn_calls=cnt[0], zeta_calls=cnt[1], zeta_rows=cnt[2], stages=cnt[3])
     return res
+
+
+def quick_assign(centers, x, y, z=None, chunk=1 << 16):
+    """QuickAssign (src/KMeans.cpp:890-942): for each object the index of the nearest
+    patch centre, squared distance (dx^2 + dy^2) + dz^2 in that order, the first
+    centre winning a tie (the reference keeps kmin unless rsq < min_rsq).  numpy
+    restatement, evaluated chunk by chunk."""
+    centers = np.asarray(centers, dtype=float)
+    dim = 2 if z is None else 3
+    centers = centers.reshape(-1, dim)
+    out = np.empty(len(x), dtype=np.int64)
+    for s in range(0, len(x), chunk):
+        e = min(len(x), s + chunk)
+        dx = x[s:e, None] - centers[None, :, 0]
+        dy = y[s:e, None] - centers[None, :, 1]
+        rsq = dx * dx + dy * dy
+        if z is not None:
+            dz = z[s:e, None] - centers[None, :, 2]
+            rsq = rsq + dz * dz
+        out[s:e] = np.argmin(rsq, axis=1)      # the first minimum
+    return out

@@ -284,3 +284,29 @@ def test_gpu_sample_cov_rows(gpu, oracle_lib):
         row[m] = s[m] / ws[m]
         assert normwise(v[k], row) < 1e-11, k
         assert math.isclose(w[k], np.sum(ws), rel_tol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dim', [2, 3])
+def test_gpu_quick_assign_matches_oracle(gpu, oracle_lib, dim):
+    """Catalog(patch_centers=...) labels every object with its nearest centre on the
+    GPU, exactly as the oracle's QuickAssign: more centres than one LDS batch, a
+    duplicated centre (the lower index wins the tie) and objects sitting on a centre."""
+    O = oracle_lib
+    coords = THREED if dim == 3 else FLAT
+    cat, scale = C.make_catalog(coords, 100_000, 371 + dim, 0.3)
+    rng = np.random.default_rng(dim)
+    npatch = 1500
+    cen = rng.uniform(0, scale, (npatch, dim))
+    cen[700] = cen[3]                       # a tie: patch 3 must win
+    for q in range(5):                      # objects exactly on centres
+        cat['x'][q], cat['y'][q] = cen[3 * q, 0], cen[3 * q, 1]
+        if dim == 3:
+            cat['z'][q] = cen[3 * q, 2]
+    kw = dict(x=cat['x'], y=cat['y'], w=cat['w'], patch_centers=cen)
+    if dim == 3:
+        kw['z'] = cat['z']
+    c = gpu.Catalog(**kw)
+    ref = O.quick_assign(cen, cat['x'], cat['y'], cat['z'] if dim == 3 else None)
+    np.testing.assert_array_equal(c.patch, ref)
+    assert c.npatch == npatch and not np.any(c.patch == 700)

@@ -23,7 +23,7 @@ class Catalog:
     def __init__(self, file_name=None, config=None, *, x=None, y=None, z=None, ra=None,
                  dec=None, r=None, w=None, wpos=None, k=None, g1=None, g2=None, patch=None,
                  x_units=None, y_units=None, ra_units=None, dec_units=None, flip_g1=False,
-                 flip_g2=False, keep_zero_weight=False, name='', **kwargs):
+                 flip_g2=False, keep_zero_weight=False, name='', patch_centers=None, **kwargs):
         if file_name is not None:
             raise NotImplementedError("treecorr_amd.Catalog takes arrays; file input is not part "
                                       "of the GPU engine")
@@ -39,6 +39,10 @@ class Catalog:
         self._w, self._wpos = arr(w), arr(wpos)
         self._k, self._g1, self._g2 = arr(k), arr(g1), arr(g2)
         self._patch = None if patch is None else np.asarray(patch, dtype=int).ravel()
+        if patch_centers is not None and patch is not None:
+            raise ValueError("Cannot provide both patch and patch_centers")
+        self._centers = None if patch_centers is None else \
+            np.ascontiguousarray(patch_centers, dtype=float)
 
         if self._x is not None or self._y is not None:
             if self._x is None or self._y is None:
@@ -98,7 +102,31 @@ class Catalog:
                 self._select(np.where(wpos != 0)[0])
 
         self._fields = {}
-        self._npatch = 1 if self._patch is None else int(self._patch.max()) + 1
+        if self._centers is not None:
+            self._assign_patches()
+        else:
+            self._npatch = 1 if self._patch is None else int(self._patch.max()) + 1
+
+    def _assign_patches(self):
+        # catalog.py:1466-1472, 1556-1566: the nearest of the given centres (QuickAssign
+        # on the GPU); npatch is the number of centres, empty patches included
+        c = self._centers
+        dim = 2 if self.coords == 'flat' else 3
+        if c.ndim != 2 or c.shape[1] != dim:
+            raise ValueError("Centers array has wrong shape.")
+        self._patch = np.empty(self.ntot, dtype=np.int32)
+        z = None if dim == 2 else np.ascontiguousarray(self._z, dtype=float)
+        ip = ctypes.POINTER(ctypes.c_int)
+        _lib.check(_lib.lib().tc_quick_assign(
+            _lib.dptr(c.ravel()), len(c), _lib.dptr(np.ascontiguousarray(self._x, dtype=float)),
+            _lib.dptr(np.ascontiguousarray(self._y, dtype=float)), _lib.dptr(z), self.ntot,
+            self._patch.ctypes.data_as(ip)))
+        self._patch = self._patch.astype(int)
+        self._npatch = len(c)
+
+    @property
+    def patch_centers(self):
+        return self._centers
 
     # ---- construction helpers ----
     def _generate_xyz(self):

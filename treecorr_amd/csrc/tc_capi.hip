@@ -537,6 +537,16 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
 
 int tc_set_traversal_chunk(long long items) { return tc::set_traversal_chunk(items); }
 
+int tc_quick_assign(const double* centers, int npatch, const double* x, const double* y,
+                    const double* z, long long n, int* patch) {
+    if (!centers || npatch <= 0 || !x || !y || n < 0 || (n > 0 && !patch))
+        return tc::fail("NULL or invalid argument");
+    if (n == 0) return 0;
+    if (int rc = check_gpu()) return rc;
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::quick_assign(centers, npatch, x, y, z, n, patch);
+}
+
 int tc_patch_split(const int* patch, long long n, int npatch, int ncol, const double* const* cols,
                    double* const* out, long long* counts) {
     if (!patch || n < 0 || npatch <= 0 || ncol < 0 || !counts || (ncol > 0 && (!cols || !out)))

@@ -40,6 +40,11 @@ struct C2Params {
     double logminsep, halfminsep, minsepsq, maxsepsq, bsq, asq;
 };
 
+// QuickAssign on the device: patch[i] = the nearest of npatch centres (2 or 3 values
+// each; z == nullptr means 2), host arrays in and out.
+int quick_assign(const double* centers, int npatch, const double* x, const double* y,
+                 const double* z, long long n, int* patch);
+
 int set_traversal_chunk(long long items);   // 0 restores the default
 struct Corr2Scratch;
 Corr2Scratch* new_corr2_scratch();
