@@ -179,3 +179,89 @@ def test_multi_root_build_equals_per_patch_trees(gpu, oracle_lib, d, coords, n, 
                     w=c['w'], **kw)
         assert_same_tree(nd, ntop, f.dump())
         assert size == np.sqrt(f.sizesq), r
+
+
+class _Speculation:
+    """tc_set_tree_speculation for the duration of a test; counts the exact rebuilds."""
+
+    def __init__(self, mode):
+        from treecorr_amd import _lib
+        self.L = _lib.lib()
+        self.mode = mode
+
+    def __enter__(self):
+        self.before = self.L.tc_set_tree_speculation(-1)
+        assert self.L.tc_set_tree_speculation(self.mode) == 0
+        return self
+
+    def __exit__(self, *exc):
+        self.redo = self.L.tc_set_tree_speculation(-1) - self.before
+        self.L.tc_set_tree_speculation(1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('mode', [3, 2])
+@pytest.mark.parametrize('d,coords,n,sm', [(4, 3, 60000, 2), (4, 2, 50000, 2),
+                                           (1, 2, 30000, 0), (0, 1, 40000, 2)])
+def test_speculative_build_matches_oracle(gpu, oracle_lib, mode, d, coords, n, sm):
+    """Top levels partitioned from approximate centroids, then checked exactly: the
+    tree is the oracle's bit for bit.  Mode 3 passes its check (random objects sit
+    nowhere near a split plane's last bits), so no rebuild runs; mode 2 forces the
+    rebuild, whose tree must be the same."""
+    O = oracle_lib
+    cat, scale = C.make_catalog(coords, n, 70 + d + coords, 0.3)
+    kw = dict(min_size=scale * 2e-4, max_size=scale * 0.02, split_method=sm, brute=False,
+              min_top=4, max_top=10)
+    with _Speculation(mode) as s:
+        f, nd, ntop = device_tree(gpu, d, coords, cat, kw)
+    assert s.redo == (1 if mode == 2 else 0)
+    ref = O.Field(d, coords, cat['x'], cat['y'], cat['z'], k=cat['k'], g1=cat['g1'],
+                  g2=cat['g2'], w=cat['w'], **kw)
+    assert_same_tree(nd, ntop, ref.dump())
+    import ctypes
+    from treecorr_amd import _lib
+    size = ctypes.c_double()
+    _lib.check(_lib.lib().tc_field_size(f.h, ctypes.byref(size)))
+    assert size.value == np.sqrt(ref.sizesq)
+
+
+@pytest.mark.gpu
+def test_speculative_multi_root_build(gpu, oracle_lib):
+    """Speculation over a multi-root build (12 patches at level 0)."""
+    O = oracle_lib
+    d, coords, n, npatch = 4, 3, 60000, 12
+    cat, scale = C.make_catalog(coords, n, 512, 0.3)
+    rng = np.random.default_rng(npatch)
+    labels = rng.integers(0, npatch, n)
+    kw = dict(min_size=scale * 2e-4, max_size=scale * 0.05, split_method=2, brute=False,
+              min_top=3, max_top=10)
+    with _Speculation(3) as s:
+        trees = multi_trees(d, coords, cat, labels, kw)
+    assert s.redo == 0
+    for r in (0, 5, 11):
+        nd, ntop, size, c = trees[r]
+        f = O.Field(d, coords, c['x'], c['y'], c['z'], k=c['k'], g1=c['g1'], g2=c['g2'],
+                    w=c['w'], **kw)
+        assert_same_tree(nd, ntop, f.dump())
+        assert size == np.sqrt(f.sizesq), r
+
+
+@pytest.mark.gpu
+def test_speculative_build_degenerate_split(gpu, oracle_lib):
+    """A mean split one ulp from a cluster of objects: whichever side the approximate
+    centroid puts them, the tree is the reference's (checked, rebuilt if it differs)."""
+    O = oracle_lib
+    rng = np.random.default_rng(5)
+    n = 300
+    x = np.repeat(rng.uniform(0, 1, 30), 10)
+    y = np.repeat(rng.uniform(0, 1e-3, 30), 10)
+    x[:10] = 1.0
+    x[10:20] = np.nextafter(1.0, 2.0)
+    y[:20] = 0.5
+    cat = dict(x=x, y=y, z=None, w=np.ones(n), k=None, g1=rng.normal(0, .2, n),
+               g2=rng.normal(0, .2, n))
+    kw = dict(min_size=0., max_size=np.inf, split_method=2, brute=False, min_top=2, max_top=10)
+    with _Speculation(3):
+        f, nd, ntop = device_tree(gpu, 4, 1, cat, kw)
+    ref = O.Field(4, 1, x, y, None, g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw).dump()
+    assert_same_tree(nd, ntop, ref)

@@ -37,6 +37,14 @@ hipStream_t stream() {
     });
     return g_stream;
 }
+static hipStream_t g_stream2 = nullptr;
+static std::once_flag g_stream2_once;
+hipStream_t stream2() {
+    std::call_once(g_stream2_once, [] {
+        if (hipStreamCreateWithFlags(&g_stream2, hipStreamNonBlocking) != hipSuccess) g_stream2 = nullptr;
+    });
+    return g_stream2;
+}
 
 // ---- caching allocator for temporaries ----
 namespace {
@@ -536,6 +544,7 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
 }
 
 int tc_set_traversal_chunk(long long items) { return tc::set_traversal_chunk(items); }
+int tc_set_tree_speculation(int mode) { return tc::set_tree_speculation(mode); }
 
 int tc_quick_assign(const double* centers, int npatch, const double* x, const double* y,
                     const double* z, long long n, int* patch) {

@@ -28,6 +28,11 @@ struct MultiTree {
     std::vector<std::vector<int>> top;   // per root, in TreeCorr's order
     std::vector<FieldInfo> info;         // per root
 };
+// Speculative centroids for the top tree levels: 0 off, 1 on for levels of large
+// segments (default), 2 on every top level with the check forced to fail, so the exact
+// rebuild runs, 3 on every top level (2 and 3 for tests).  A negative mode changes
+// nothing and returns the number of exact rebuilds so far.
+int set_tree_speculation(int mode);
 int build_trees(int D, int C, const double* x, const double* y, const double* z, const double* d1,
                 const double* d2, const double* w, const double* wpos, long long n,
                 const std::vector<long long>& bounds, double minsize, double maxsize, int sm,

@@ -19,6 +19,7 @@
 #include "tc_select.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -37,7 +38,25 @@ struct SegOut {
     long long mid;
     double value;
     int split, dim, fallback, childL;
-    int child_mode, pad;
+    int child_mode, node;     // node: the node this segment became (-1 if none)
+};
+
+// Speculative tree levels.  SPEC_FAST takes a segment's centroid from a parallel
+// (unordered) sum and drives the partition with it; SPEC_CHECK recomputes the
+// centroid in the reference's order from a snapshot of the same level, writes the
+// node records from it and flags any decision or partition side the approximate
+// centroid got wrong (the whole build is then redone without speculation).
+enum { SPEC_OFF = 0, SPEC_FAST = 1, SPEC_CHECK = 2 };
+constexpr int SPEC_NT = 256;            // threads of the slice kernels
+constexpr int SPEC_BLOCKS = 1024;       // slices per speculative level, at most
+constexpr int SPEC_LEVELS_MAX = 8;      // speculative levels per build, at most
+constexpr long long SPEC_MIN_N = 1 << 16;   // objects per segment worth speculating on
+
+struct SpecArgs {
+    const double* c1;   // SPEC_FAST: approximate centroids, 4 per segment
+    const double* p2;   // SPEC_FAST: per-slice size and bounds, 7 per slice
+    int k;              // SPEC_FAST: slices per segment
+    int* bad;           // SPEC_CHECK: set when a decision differs
 };
 
 struct TopEntry { unsigned long long key; int node, root; };
@@ -46,6 +65,20 @@ struct Elems {
     double *x, *y, *z, *wp;
     float *w, *a, *b;
     int* seg;
+};
+
+// A speculative level's snapshot for its exact pass: the segments, the decisions
+// taken from the approximate centroids and the objects in that level's order.
+struct SpecLevel {
+    const Seg* segs;
+    const SegOut* sout;
+    Elems E;
+    long long nseg;
+    int level0;
+};
+struct SpecBatch {
+    SpecLevel lv[SPEC_LEVELS_MAX];
+    int nlev;
 };
 
 struct BuildParams {
@@ -212,6 +245,19 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     __syncthreads();
     return m;
 }
+// Sum over the workgroup in no particular order (speculative centroids only).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (NT == 64) return v;
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int k = 1; k < NT / 64; ++k) m += red[k];
+    __syncthreads();
+    return m;
+}
 template <int NT>
 __device__ __forceinline__ double block_min(double v, double* red) {
     v = wave_min(v);
@@ -225,17 +271,33 @@ __device__ __forceinline__ double block_min(double v, double* red) {
     return m;
 }
 
-template <int D, int C, int NT>
-__global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0, SegOut* sout,
-                                                Elems E, DNode* nodes, int* counters,
-                                                TopEntry* tops, BuildParams P, FieldInfo* finfo,
-                                                int level0, long long skip_n) {
-    constexpr int B = SegCfg<NT>::B, NC = SegCfg<NT>::NC;
+// The centroid of summed (x w, y w, z w, w, w_float) values (BuildCellData,
+// src/Cell.cpp:170-200); cx, cy, cz hold the first object's position on entry and
+// keep it when the weights sum to zero.
+template <int C>
+__device__ __forceinline__ void centroid_from_sums(const double* sums, double& cx, double& cy,
+                                                   double& cz, float& wf) {
+    wf = float(sums[4]);
+    if (sums[3] != 0.) {
+        cx = sums[0] / sums[3]; cy = sums[1] / sums[3];
+        cz = (C == Flat) ? 0. : sums[2] / sums[3];
+        if (C == Sphere) normalize_div(cx, cy, cz);
+    }
+}
+
+// One segment's workgroup: its centroid, size and bounds, the top-cell and split
+// decisions, its node record and the split plane.
+template <int D, int C, int NT, int SPEC>
+__device__ __forceinline__ void segment_body(const long long s, const Seg* segs, SegOut* sout,
+                                             const Elems& E, DNode* nodes, int* counters,
+                                             TopEntry* tops, const BuildParams& P,
+                                             FieldInfo* finfo, int level0, long long skip_n,
+                                             const SpecArgs& sp) {
+    constexpr int NC = SegCfg<NT>::NC;
     __shared__ double stg[SegCfg<NT>::NBUF * NC * SegCfg<NT>::RS];
     __shared__ double bcast[NC];
     __shared__ double red[NT / 64];
     __shared__ int s_int[2];
-    const long long s = seg0 + blockIdx.x;
     const int tid = threadIdx.x;
     const Seg sg = segs[s];
     const long long start = sg.start, end = sg.end, n = end - start;
@@ -247,42 +309,50 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
     double mnx = 0, mxx = 0, mny = 0, mxy = 0, mnz = 0, mxz = 0;
 
     if (n > 1 || level0) {
-        // BuildCellData (src/Cell.cpp:170-200): sums start from the first object,
-        // which -0.0 reproduces exactly (-0.0 + v == v for every v).
-        double sums[NC];
-        ordered_sums<NT, NC>(start, end, -0.0, stg, bcast,
-            [&](long long i, double (&v)[NC]) {
-                const double wp = E.wp[i];
-                v[0] = E.x[i] * wp; v[1] = E.y[i] * wp;
-                v[2] = C == Flat ? 0. : E.z[i] * wp;
-                v[3] = wp; v[4] = double(E.w[i]);
-            }, sums);
-        const double ax = sums[0], ay = sums[1], az = sums[2], asw = sums[3], aww = sums[4];
-        wf = float(aww);
-        if (asw != 0.) {
-            cx = ax / asw; cy = ay / asw; cz = (C == Flat) ? 0. : az / asw;
-            if (C == Sphere) normalize_div(cx, cy, cz);
-        }
-        // CalculateSizeSq (src/Cell.cpp:89-100) and Bounds (include/Bounds.h).
         double smax = 0.;
         double lnx = INFINITY, lxx = -INFINITY, lny = INFINITY, lxy = -INFINITY;
         double lnz = INFINITY, lxz = -INFINITY;
-        for (long long i = start + tid; i < end; i += NT) {
-            double px = E.x[i], py = E.y[i], pz = E.z[i];
-            double dx = cx - px, dy = cy - py;
-            double dsq;
-            if (C == Flat) dsq = dx * dx + dy * dy;
-            else { double dz = cz - pz; dsq = dx * dx + dy * dy + dz * dz; }
-            smax = fmax(smax, dsq);
-            lnx = fmin(lnx, px); lxx = fmax(lxx, px);
-            lny = fmin(lny, py); lxy = fmax(lxy, py);
-            lnz = fmin(lnz, pz); lxz = fmax(lxz, pz);
+        if (SPEC == SPEC_FAST) {
+            // the approximate centroid and the partial sizes and bounds of the slices
+            const double* c1 = sp.c1 + 4 * s;
+            cx = c1[0]; cy = c1[1]; cz = c1[2];
+            for (int k = tid; k < sp.k; k += NT) {
+                const double* q = sp.p2 + ((size_t)s * sp.k + k) * 7;
+                smax = fmax(smax, q[0]);
+                lnx = fmin(lnx, q[1]); lxx = fmax(lxx, q[2]);
+                lny = fmin(lny, q[3]); lxy = fmax(lxy, q[4]);
+                lnz = fmin(lnz, q[5]); lxz = fmax(lxz, q[6]);
+            }
+        } else {
+            // BuildCellData (src/Cell.cpp:170-200): sums start from the first object,
+            // which -0.0 reproduces exactly (-0.0 + v == v for every v).
+            double sums[NC];
+            ordered_sums<NT, NC>(start, end, -0.0, stg, bcast,
+                [&](long long i, double (&v)[NC]) {
+                    const double wp = E.wp[i];
+                    v[0] = E.x[i] * wp; v[1] = E.y[i] * wp;
+                    v[2] = C == Flat ? 0. : E.z[i] * wp;
+                    v[3] = wp; v[4] = double(E.w[i]);
+                }, sums);
+            centroid_from_sums<C>(sums, cx, cy, cz, wf);
+            // CalculateSizeSq (src/Cell.cpp:89-100) and Bounds (include/Bounds.h).
+            for (long long i = start + tid; i < end; i += NT) {
+                double px = E.x[i], py = E.y[i], pz = E.z[i];
+                double dx = cx - px, dy = cy - py;
+                double dsq;
+                if (C == Flat) dsq = dx * dx + dy * dy;
+                else { double dz = cz - pz; dsq = dx * dx + dy * dy + dz * dz; }
+                smax = fmax(smax, dsq);
+                lnx = fmin(lnx, px); lxx = fmax(lxx, px);
+                lny = fmin(lny, py); lxy = fmax(lxy, py);
+                lnz = fmin(lnz, pz); lxz = fmax(lxz, pz);
+            }
         }
         sizesq = block_max<NT>(smax, red);
         mnx = block_min<NT>(lnx, red); mxx = block_max<NT>(lxx, red);
         mny = block_min<NT>(lny, red); mxy = block_max<NT>(lxy, red);
         mnz = block_min<NT>(lnz, red); mxz = block_max<NT>(lxz, red);
-        if (level0 && tid == 0) {
+        if (level0 && tid == 0 && SPEC != SPEC_FAST) {
             // the roots are the level-0 segments, in order
             finfo[s].cx = cx; finfo[s].cy = cy; finfo[s].cz = cz; finfo[s].sizesq = sizesq;
         }
@@ -292,10 +362,15 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
     int mode = sg.mode;
     int node = sg.node;
     bool split = false;
+    const SegOut fast = SPEC == SPEC_CHECK ? sout[s] : SegOut{};
+    bool bad = false;     // SPEC_CHECK: a decision differs from the speculative one
     if (mode == SEG_TOP) {
         double ssq = (n == 1) ? 0. : sizesq;
         if (ssq == 0 || (ssq <= P.maxsizesq && sg.mintop <= 0) || sg.maxtop <= 0) {
-            if (tid == 0) {
+            if (SPEC == SPEC_CHECK) {
+                bad = bad || fast.child_mode != SEG_BUILD;
+                node = fast.node;
+            } else if (tid == 0) {
                 node = atomicAdd(&counters[0], 1);
                 int t = atomicAdd(&counters[1], 1);
                 tops[t].key = sg.key;
@@ -303,15 +378,19 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
                 tops[t].root = sg.root;
                 s_int[0] = node;
             }
-            __syncthreads();
-            node = s_int[0];
+            if (SPEC != SPEC_CHECK) {
+                __syncthreads();
+                node = s_int[0];
+            }
             mode = SEG_BUILD;
         } else {
             split = true;
+            if (SPEC == SPEC_CHECK) bad = bad || fast.child_mode != SEG_TOP;
         }
     }
     int childL = -1;
-    if (mode == SEG_BUILD) {
+    if (mode == SEG_BUILD && SPEC == SPEC_CHECK && node < 0) bad = true;
+    if (mode == SEG_BUILD && !(SPEC == SPEC_CHECK && bad)) {
         DNode nd;
         if (n == 1) {
             long long i = start;
@@ -319,9 +398,11 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
             nd.w = E.w[i]; nd.a = E.a[i]; nd.b = E.b[i];
             nd.size = 0.f; nd.child = -1; nd.n = 1;
         } else {
-            // finishAverages (src/Cell.cpp:208-354)
+            // finishAverages (src/Cell.cpp:208-354); a speculative pass leaves the node
+            // record to SPEC_CHECK
             float fa = 0.f, fb = 0.f;
-            if (D == KData) {
+            if (SPEC == SPEC_FAST) {
+            } else if (D == KData) {
                 double sk[NC];
                 ordered_sums<NT, NC>(start, end, 0., stg, bcast,
                     [&](long long i, double (&v)[NC]) {
@@ -372,16 +453,51 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
             if (sizesq > P.minsizesq) {
                 nd.size = P.brute ? INFINITY : float(sqrt(sizesq));
                 split = true;
-                if (tid == 0) s_int[1] = atomicAdd(&counters[0], 2);
-                __syncthreads();
-                childL = s_int[1];
+                if (SPEC == SPEC_CHECK) {
+                    bad = bad || !fast.split || fast.childL < 0;
+                    childL = fast.childL;
+                } else {
+                    if (tid == 0) s_int[1] = atomicAdd(&counters[0], 2);
+                    __syncthreads();
+                    childL = s_int[1];
+                }
                 nd.child = childL;
             } else {
                 nd.size = 0.f;
                 nd.child = -1;
+                if (SPEC == SPEC_CHECK) bad = bad || fast.split;
             }
         }
-        if (tid == 0) nodes[node] = nd;
+        if (tid == 0 && SPEC != SPEC_FAST && !bad) nodes[node] = nd;
+    }
+
+    if (SPEC == SPEC_CHECK) {
+        // the partition side of every object under the exact split value must be the
+        // side the speculative value gave it
+        if (!bad && split) {
+            int dim;
+            if (C == Flat) dim = (mxy - mny) > (mxx - mnx) ? 1 : 0;
+            else {
+                double xr = mxx - mnx, yr = mxy - mny, zr = mxz - mnz;
+                dim = yr > xr ? (zr > yr ? 2 : 1) : (zr > xr ? 2 : 0);
+            }
+            bad = dim != fast.dim;
+            if (!bad && P.sm == Mean) {
+                const double v = dim == 2 ? cz : dim == 1 ? cy : cx;
+                int diff = 0;
+                for (long long i = start + tid; i < end; i += NT) {
+                    const double c = dim == 2 ? E.z[i] : dim == 1 ? E.y[i] : E.x[i];
+                    diff |= (c < v) != (c < fast.value);
+                }
+                bad = block_max<NT>(double(diff), red) != 0.;
+            } else if (!bad && P.sm == Middle) {
+                const double v = dim == 2 ? (mxz + mnz) / 2. : dim == 1 ? (mxy + mny) / 2.
+                                                                       : (mxx + mnx) / 2.;
+                bad = v != fast.value;
+            }
+        }
+        if (bad && tid == 0) atomicOr(sp.bad, 1);
+        return;
     }
 
     if (tid == 0) {
@@ -389,6 +505,7 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
         o.split = split ? 1 : 0;
         o.childL = childL;
         o.child_mode = mode;
+        o.node = mode == SEG_BUILD ? node : -1;
         o.mid = start;
         o.fallback = 0;
         o.dim = 0;
@@ -409,6 +526,115 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
         }
         sout[s] = o;
     }
+}
+
+template <int D, int C, int NT, int SPEC = SPEC_OFF>
+__global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0, SegOut* sout,
+                                                Elems E, DNode* nodes, int* counters,
+                                                TopEntry* tops, BuildParams P, FieldInfo* finfo,
+                                                int level0, long long skip_n, SpecArgs sp) {
+    segment_body<D, C, NT, SPEC>(seg0 + blockIdx.x, segs, sout, E, nodes, counters, tops, P,
+                                 finfo, level0, skip_n, sp);
+}
+
+// The exact pass of every speculative level in one launch, the root's segment first
+// (the longest chain): workgroup b is segment b of level 0, then level 1's, and so on.
+template <int D, int C>
+__global__ __launch_bounds__(512) void k_segment_check(SpecBatch B, DNode* nodes, BuildParams P,
+                                                       FieldInfo* finfo, int* bad) {
+    long long s = blockIdx.x;
+    int l = 0;
+    while (l + 1 < B.nlev && s >= B.lv[l].nseg) { s -= B.lv[l].nseg; ++l; }
+    const SpecLevel& L = B.lv[l];
+    SpecArgs sp{};
+    sp.bad = bad;
+    segment_body<D, C, 512, SPEC_CHECK>(s, L.segs, const_cast<SegOut*>(L.sout), L.E, nodes,
+                                        nullptr, nullptr, P, finfo, L.level0, 0, sp);
+}
+
+// Speculative centroids, sizes and bounds: each segment of the level is cut into K
+// slices, one workgroup each (a level near the root holds few segments, too few
+// workgroups to read its objects at the memory's rate).
+__device__ __forceinline__ void spec_slice(const Seg& sg, int k, int K, long long& a,
+                                           long long& b) {
+    const long long len = sg.end - sg.start;
+    a = sg.start + len * k / K;
+    b = sg.start + len * (k + 1) / K;
+}
+
+template <int C>
+__global__ __launch_bounds__(SPEC_NT) void k_spec_sums(const Seg* segs, Elems E, int K,
+                                                       double* p1) {
+    __shared__ double red[SPEC_NT / 64];
+    const int k = blockIdx.x;
+    const long long s = blockIdx.y;
+    long long a, b;
+    spec_slice(segs[s], k, K, a, b);
+    double part[5] = {0., 0., 0., 0., 0.};
+    for (long long i = a + threadIdx.x; i < b; i += SPEC_NT) {
+        const double wp = E.wp[i];
+        part[0] += E.x[i] * wp; part[1] += E.y[i] * wp;
+        part[2] += C == Flat ? 0. : E.z[i] * wp;
+        part[3] += wp; part[4] += double(E.w[i]);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const double v = block_sum<SPEC_NT>(part[c], red);
+        if (threadIdx.x == 0) p1[((size_t)s * K + k) * 5 + c] = v;
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(SPEC_NT) void k_spec_centroid(const Seg* segs, Elems E, int K,
+                                                           const double* p1, double* c1) {
+    __shared__ double red[SPEC_NT / 64];
+    const long long s = blockIdx.x;
+    double part[5] = {0., 0., 0., 0., 0.};
+    for (int k = threadIdx.x; k < K; k += SPEC_NT) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) part[c] += p1[((size_t)s * K + k) * 5 + c];
+    }
+    double sums[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sums[c] = block_sum<SPEC_NT>(part[c], red);
+    if (threadIdx.x == 0) {
+        const long long i = segs[s].start;
+        double cx = E.x[i], cy = E.y[i], cz = E.z[i];
+        float wf;
+        centroid_from_sums<C>(sums, cx, cy, cz, wf);
+        c1[4 * s] = cx; c1[4 * s + 1] = cy; c1[4 * s + 2] = cz; c1[4 * s + 3] = double(wf);
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(SPEC_NT) void k_spec_bounds(const Seg* segs, Elems E, int K,
+                                                         const double* c1, double* p2) {
+    __shared__ double red[SPEC_NT / 64];
+    const int k = blockIdx.x;
+    const long long s = blockIdx.y;
+    long long a, b;
+    spec_slice(segs[s], k, K, a, b);
+    const double cx = c1[4 * s], cy = c1[4 * s + 1], cz = c1[4 * s + 2];
+    double smax = 0.;
+    double lnx = INFINITY, lxx = -INFINITY, lny = INFINITY, lxy = -INFINITY;
+    double lnz = INFINITY, lxz = -INFINITY;
+    for (long long i = a + threadIdx.x; i < b; i += SPEC_NT) {
+        const double px = E.x[i], py = E.y[i], pz = E.z[i];
+        const double dx = cx - px, dy = cy - py;
+        double dsq;
+        if (C == Flat) dsq = dx * dx + dy * dy;
+        else { const double dz = cz - pz; dsq = dx * dx + dy * dy + dz * dz; }
+        smax = fmax(smax, dsq);
+        lnx = fmin(lnx, px); lxx = fmax(lxx, px);
+        lny = fmin(lny, py); lxy = fmax(lxy, py);
+        lnz = fmin(lnz, pz); lxz = fmax(lxz, pz);
+    }
+    const double v[7] = {block_max<SPEC_NT>(smax, red), block_min<SPEC_NT>(lnx, red),
+                         block_max<SPEC_NT>(lxx, red), block_min<SPEC_NT>(lny, red),
+                         block_max<SPEC_NT>(lxy, red), block_min<SPEC_NT>(lnz, red),
+                         block_max<SPEC_NT>(lxz, red)};
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 7; ++c) p2[((size_t)s * K + k) * 7 + c] = v[c];
 }
 
 // Segments of at most SEG_SMALL_N objects, one thread each (the deep levels hold up to
@@ -797,6 +1023,16 @@ static int dalloc(DevBuf& b, size_t bytes) {
     return 0;
 }
 
+static std::atomic<int> g_tree_spec{1};         // tc_set_tree_speculation
+static std::atomic<long long> g_tree_spec_redo{0};   // exact rebuilds after a failed check
+
+int set_tree_speculation(int mode) {
+    if (mode < 0) return (int)std::min<long long>(g_tree_spec_redo.load(), 1 << 30);
+    if (mode > 3) return fail("tree speculation mode must be 0 to 3");
+    g_tree_spec.store(mode);
+    return 0;
+}
+
 // Builds one tree per root range [bounds[r], bounds[r+1]) of the columns (each range
 // non-empty), all in the same level-synchronous pass: the roots are the level-0
 // segments.  The trees share one node array.
@@ -839,89 +1075,205 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
     const long long maxnodes = 2 * n;
     TC_HIP(hipMalloc(&M.nodes, sizeof(DNode) * maxnodes));
 
-    TC_HIP(hipMemcpyAsync(bbounds.p, bounds.data(), sizeof(long long) * (nroot + 1),
-                          hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL((k_init_elems<D, C>), grid(n), dim3(TPB), 0, st, E[0], x, y, z, d1, d2, w,
-                       wpos, n, bbounds.as<long long>(), nroot);
-    std::vector<Seg> roots(nroot);
-    for (int r = 0; r < nroot; ++r) {
-        Seg& g = roots[r];
-        g = Seg{};
-        g.start = bounds[r]; g.end = bounds[r + 1]; g.node = -1; g.mode = SEG_TOP;
-        g.mintop = mintop; g.maxtop = maxtop; g.key = 0; g.depth = 0; g.root = r;
-    }
-    TC_HIP(hipMemcpyAsync(bsegs[0].p, roots.data(), sizeof(Seg) * nroot, hipMemcpyHostToDevice,
-                          st));
-    TC_HIP(hipMemsetAsync(bcnt.p, 0, 16, st));
-
-    BuildParams P{minsize * minsize, maxsize * maxsize, sm, brute};
-    int cur = 0;
-    long long nseg = nroot;
-    int level = 0;
-    int* hcnt = nullptr;
-    TC_HIP(hipHostMalloc((void**)&hcnt, 16));
-    int rc = 0;
-    while (nseg > 0) {
-        Seg* segs = bsegs[cur].as<Seg>();
-        SegOut* sout = bsout.as<SegOut>();
-        const int l0 = level == 0 ? 1 : 0;
-        // Few large segments near the root: 512 or 256 threads each (the other waves
-        // stage objects while one adds); many small ones further down: one wave each.
-        if (nseg <= SEG_TOP_MAX) {
-            // the longest chains: more stager waves keep the adding wave fed
-            hipLaunchKernelGGL((k_segment<D, C, 512>), dim3((unsigned)nseg), dim3(512),
-                               0, st, segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
-                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0, 0LL);
-        } else if (nseg <= SEG_WIDE_MAX) {
-            hipLaunchKernelGGL((k_segment<D, C, 256>), dim3((unsigned)nseg), dim3(256), 0, st,
-                               segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
-                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0, 0LL);
-        } else {
-            // many segments: the small ones one thread each, the rest one wave each
-            hipLaunchKernelGGL((k_segment_small<D, C>), grid(nseg), dim3(TPB), 0, st, segs, nseg,
-                               sout, E[cur], M.nodes, bcnt.as<int>(), btops.as<TopEntry>(), P,
-                               bfinfo.as<FieldInfo>(), l0);
-            hipLaunchKernelGGL((k_segment<D, C, 64>), dim3((unsigned)nseg), dim3(64), 0, st,
-                               segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
-                               btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0,
-                               SEG_SMALL_N);
+    // Speculation (Mean and Middle splits): the levels near the root partition with
+    // centroids summed in parallel; their exact pass runs on the second stream from
+    // snapshots, writes the node records and flags any decision that differs.
+    const int spec_mode = g_tree_spec.load();
+    const bool spec_on = spec_mode != 0 && (sm == Mean || sm == Middle);
+    DevBuf bspec, bsnap[SPEC_LEVELS_MAX];
+    if (spec_on && dalloc(bspec, sizeof(double) * (4 * SEG_TOP_MAX + 12 * SPEC_BLOCKS))) return 2;
+    struct SpecGuard {   // declared after the buffers: the exact pass ends before they go
+        hipStream_t st2 = nullptr;
+        hipEvent_t ev = nullptr;
+        ~SpecGuard() {
+            if (st2) (void)hipStreamSynchronize(st2);
+            if (ev) (void)hipEventDestroy(ev);
         }
-        hipLaunchKernelGGL(k_seg_split_flags, grid(nseg), dim3(TPB), 0, st, sout, (int)nseg,
-                           bsegflags.as<int>());
-        exclusive_scan(bsegflags.as<int>(), nseg, bsegrank.as<int>(), bscan.as<int>());
-        TC_HIP(hipMemcpyAsync(hcnt, bsegrank.as<int>() + nseg, sizeof(int),
-                              hipMemcpyDeviceToHost, st));
-        hipLaunchKernelGGL(k_part_flags, grid(n), dim3(TPB), 0, st, E[cur], sout, n,
-                           bflags.as<int>());
-        exclusive_scan(bflags.as<int>(), n, bpre.as<int>(), bscan.as<int>());
-        hipLaunchKernelGGL(k_part_mid, grid(nseg), dim3(TPB), 0, st, segs, sout, (int)nseg,
-                           bpre.as<int>());
-        hipLaunchKernelGGL(k_part_median, grid(nseg), dim3(TPB), 0, st, segs, sout, (int)nseg,
-                           E[cur], bkey.as<double>(), bord.as<int>(), bdest.as<long long>());
-        hipLaunchKernelGGL(k_part_rank, grid(n), dim3(TPB), 0, st, E[cur], segs, sout, n,
-                           bflags.as<int>(), bpre.as<int>(), blpos.as<int>(), brpos.as<int>());
-        hipLaunchKernelGGL(k_part_dest, grid(n), dim3(TPB), 0, st, E[cur], segs, sout,
-                           bsegrank.as<int>(), n, bflags.as<int>(), bpre.as<int>(),
-                           blpos.as<int>(), brpos.as<int>(), bdest.as<long long>(),
-                           bnewseg.as<int>());
-        hipLaunchKernelGGL(k_permute, grid(n), dim3(TPB), 0, st, E[cur], E[1 - cur], n,
-                           bdest.as<long long>(), bnewseg.as<int>());
-        hipLaunchKernelGGL(k_children, grid(nseg), dim3(TPB), 0, st, segs, sout,
-                           bsegrank.as<int>(), (int)nseg, bsegs[1 - cur].as<Seg>());
-        TC_HIP(hipGetLastError());
+    } guard;
+    if (spec_on) {
+        guard.st2 = stream2();
+        if (!guard.st2) return fail("could not create the second stream");
+        TC_HIP(hipEventCreateWithFlags(&guard.ev, hipEventDisableTiming));
+    }
+    struct HostBuf {
+        int* p = nullptr;
+        ~HostBuf() { if (p) (void)hipHostFree(p); }
+    } hcnt_buf;
+    TC_HIP(hipHostMalloc((void**)&hcnt_buf.p, 16));
+    int* hcnt = hcnt_buf.p;
+    const BuildParams P{minsize * minsize, maxsize * maxsize, sm, brute};
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+
+    auto attempt = [&](bool spec) -> int {
+        TC_HIP(hipMemcpyAsync(bbounds.p, bounds.data(), sizeof(long long) * (nroot + 1),
+                              hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL((k_init_elems<D, C>), grid(n), dim3(TPB), 0, st, E[0], x, y, z, d1,
+                           d2, w, wpos, n, bbounds.as<long long>(), nroot);
+        std::vector<Seg> roots(nroot);
+        for (int r = 0; r < nroot; ++r) {
+            Seg& g = roots[r];
+            g = Seg{};
+            g.start = bounds[r]; g.end = bounds[r + 1]; g.node = -1; g.mode = SEG_TOP;
+            g.mintop = mintop; g.maxtop = maxtop; g.key = 0; g.depth = 0; g.root = r;
+        }
+        TC_HIP(hipMemcpyAsync(bsegs[0].p, roots.data(), sizeof(Seg) * nroot,
+                              hipMemcpyHostToDevice, st));
+        TC_HIP(hipMemsetAsync(bcnt.p, 0, 16, st));
+
+        SpecBatch SB{};
+        long long spec_segs = 0;
+        bool checked = false;
+        auto launch_check = [&]() -> int {
+            checked = true;
+            TC_HIP(hipEventRecord(guard.ev, st));
+            TC_HIP(hipStreamWaitEvent(guard.st2, guard.ev, 0));
+            hipLaunchKernelGGL((k_segment_check<D, C>), dim3((unsigned)spec_segs), dim3(512), 0,
+                               guard.st2, SB, M.nodes, P, bfinfo.as<FieldInfo>(),
+                               bcnt.as<int>() + 2);
+            TC_HIP(hipGetLastError());
+            return 0;
+        };
+        int cur = 0;
+        long long nseg = nroot;
+        int level = 0;
+        while (nseg > 0) {
+            Seg* segs = bsegs[cur].as<Seg>();
+            SegOut* sout = bsout.as<SegOut>();
+            const int l0 = level == 0 ? 1 : 0;
+            bool sp = spec && !checked && SB.nlev < SPEC_LEVELS_MAX && nseg <= SEG_TOP_MAX &&
+                      (spec_mode >= 2 || n / nseg >= SPEC_MIN_N);
+            const size_t segb = al(sizeof(Seg) * nseg), soutb = al(sizeof(SegOut) * nseg);
+            const size_t db = al(sizeof(double) * n), fb = al(sizeof(float) * n);
+            if (sp) {
+                DevBuf& sb = bsnap[SB.nlev];
+                if (!sb.p) {   // the first attempt allocates, a second reuses
+                    const size_t need = segb + soutb + 4 * db + 3 * fb;
+                    sb.p = dev_alloc(need);
+                    if (sb.p) sb.bytes = need;
+                    else sp = false;   // no room for the snapshot: exact from here on
+                }
+            }
+            if (!sp && SB.nlev > 0 && !checked && launch_check()) return 2;
+            if (sp) {
+                int K = (int)(SPEC_BLOCKS / nseg);
+                const long long per = n / nseg / 1024;   // slices of about 1024 objects or more
+                if (K > per) K = (int)per;
+                if (K < 1) K = 1;
+                double* c1 = bspec.as<double>();
+                double* p1 = c1 + 4 * SEG_TOP_MAX;
+                double* p2 = p1 + 5 * SPEC_BLOCKS;
+                hipLaunchKernelGGL((k_spec_sums<C>), dim3(K, (unsigned)nseg), dim3(SPEC_NT), 0,
+                                   st, segs, E[cur], K, p1);
+                hipLaunchKernelGGL((k_spec_centroid<C>), dim3((unsigned)nseg), dim3(SPEC_NT), 0,
+                                   st, segs, E[cur], K, p1, c1);
+                hipLaunchKernelGGL((k_spec_bounds<C>), dim3(K, (unsigned)nseg), dim3(SPEC_NT), 0,
+                                   st, segs, E[cur], K, c1, p2);
+                const SpecArgs sa{c1, p2, K, nullptr};
+                hipLaunchKernelGGL((k_segment<D, C, 512, SPEC_FAST>), dim3((unsigned)nseg),
+                                   dim3(512), 0, st, segs, 0LL, sout, E[cur], M.nodes,
+                                   bcnt.as<int>(), btops.as<TopEntry>(), P,
+                                   bfinfo.as<FieldInfo>(), l0, 0LL, sa);
+                char* q = static_cast<char*>(bsnap[SB.nlev].p);
+                SpecLevel& L = SB.lv[SB.nlev++];
+                Seg* ssegs = reinterpret_cast<Seg*>(q); q += segb;
+                SegOut* ssout = reinterpret_cast<SegOut*>(q); q += soutb;
+                Elems S{};
+                S.x = reinterpret_cast<double*>(q); q += db;
+                S.y = reinterpret_cast<double*>(q); q += db;
+                S.z = reinterpret_cast<double*>(q); q += db;
+                S.wp = reinterpret_cast<double*>(q); q += db;
+                S.w = reinterpret_cast<float*>(q); q += fb;
+                S.a = reinterpret_cast<float*>(q); q += fb;
+                S.b = reinterpret_cast<float*>(q);
+                const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
+                TC_HIP(hipMemcpyAsync(ssegs, segs, sizeof(Seg) * nseg, dd, st));
+                TC_HIP(hipMemcpyAsync(ssout, sout, sizeof(SegOut) * nseg, dd, st));
+                TC_HIP(hipMemcpyAsync(S.x, E[cur].x, sizeof(double) * n, dd, st));
+                TC_HIP(hipMemcpyAsync(S.y, E[cur].y, sizeof(double) * n, dd, st));
+                TC_HIP(hipMemcpyAsync(S.z, E[cur].z, sizeof(double) * n, dd, st));
+                TC_HIP(hipMemcpyAsync(S.wp, E[cur].wp, sizeof(double) * n, dd, st));
+                TC_HIP(hipMemcpyAsync(S.w, E[cur].w, sizeof(float) * n, dd, st));
+                TC_HIP(hipMemcpyAsync(S.a, E[cur].a, sizeof(float) * n, dd, st));
+                TC_HIP(hipMemcpyAsync(S.b, E[cur].b, sizeof(float) * n, dd, st));
+                L.segs = ssegs; L.sout = ssout; L.E = S; L.nseg = nseg; L.level0 = l0;
+                spec_segs += nseg;
+            } else if (nseg <= SEG_TOP_MAX) {
+                // Few large segments near the root: 512 threads each (the other waves
+                // stage objects while one adds); the longest chains need the most stagers.
+                hipLaunchKernelGGL((k_segment<D, C, 512>), dim3((unsigned)nseg), dim3(512),
+                                   0, st, segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
+                                   btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0, 0LL,
+                                   SpecArgs{});
+            } else if (nseg <= SEG_WIDE_MAX) {
+                hipLaunchKernelGGL((k_segment<D, C, 256>), dim3((unsigned)nseg), dim3(256), 0, st,
+                                   segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
+                                   btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0, 0LL,
+                                   SpecArgs{});
+            } else {
+                // many segments: the small ones one thread each, the rest one wave each
+                hipLaunchKernelGGL((k_segment_small<D, C>), grid(nseg), dim3(TPB), 0, st, segs,
+                                   nseg, sout, E[cur], M.nodes, bcnt.as<int>(),
+                                   btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0);
+                hipLaunchKernelGGL((k_segment<D, C, 64>), dim3((unsigned)nseg), dim3(64), 0, st,
+                                   segs, 0LL, sout, E[cur], M.nodes, bcnt.as<int>(),
+                                   btops.as<TopEntry>(), P, bfinfo.as<FieldInfo>(), l0,
+                                   SEG_SMALL_N, SpecArgs{});
+            }
+            hipLaunchKernelGGL(k_seg_split_flags, grid(nseg), dim3(TPB), 0, st, sout, (int)nseg,
+                               bsegflags.as<int>());
+            exclusive_scan(bsegflags.as<int>(), nseg, bsegrank.as<int>(), bscan.as<int>());
+            TC_HIP(hipMemcpyAsync(hcnt, bsegrank.as<int>() + nseg, sizeof(int),
+                                  hipMemcpyDeviceToHost, st));
+            hipLaunchKernelGGL(k_part_flags, grid(n), dim3(TPB), 0, st, E[cur], sout, n,
+                               bflags.as<int>());
+            exclusive_scan(bflags.as<int>(), n, bpre.as<int>(), bscan.as<int>());
+            hipLaunchKernelGGL(k_part_mid, grid(nseg), dim3(TPB), 0, st, segs, sout, (int)nseg,
+                               bpre.as<int>());
+            hipLaunchKernelGGL(k_part_median, grid(nseg), dim3(TPB), 0, st, segs, sout,
+                               (int)nseg, E[cur], bkey.as<double>(), bord.as<int>(),
+                               bdest.as<long long>());
+            hipLaunchKernelGGL(k_part_rank, grid(n), dim3(TPB), 0, st, E[cur], segs, sout, n,
+                               bflags.as<int>(), bpre.as<int>(), blpos.as<int>(),
+                               brpos.as<int>());
+            hipLaunchKernelGGL(k_part_dest, grid(n), dim3(TPB), 0, st, E[cur], segs, sout,
+                               bsegrank.as<int>(), n, bflags.as<int>(), bpre.as<int>(),
+                               blpos.as<int>(), brpos.as<int>(), bdest.as<long long>(),
+                               bnewseg.as<int>());
+            hipLaunchKernelGGL(k_permute, grid(n), dim3(TPB), 0, st, E[cur], E[1 - cur], n,
+                               bdest.as<long long>(), bnewseg.as<int>());
+            hipLaunchKernelGGL(k_children, grid(nseg), dim3(TPB), 0, st, segs, sout,
+                               bsegrank.as<int>(), (int)nseg, bsegs[1 - cur].as<Seg>());
+            TC_HIP(hipGetLastError());
+            TC_HIP(hipStreamSynchronize(st));
+            nseg = 2LL * hcnt[0];
+            cur = 1 - cur;
+            ++level;
+            if (level > 4 * 64 + 64) return fail("tree build did not terminate");
+        }
+        if (SB.nlev > 0 && !checked && launch_check()) return 2;
+        if (checked) TC_HIP(hipStreamSynchronize(guard.st2));
+        return 0;
+    };
+
+    int rc = attempt(spec_on);
+    if (rc == 0 && spec_on) {
+        int flag = 0;
+        TC_HIP(hipMemcpyAsync(&flag, bcnt.as<int>() + 2, sizeof(int), hipMemcpyDeviceToHost, st));
         TC_HIP(hipStreamSynchronize(st));
-        nseg = 2LL * hcnt[0];
-        cur = 1 - cur;
-        ++level;
-        if (level > 4 * 64 + 64) { rc = fail("tree build did not terminate"); break; }
+        if (flag || spec_mode == 2) {
+            // a speculative decision differed (or a test forces it): build again exactly
+            g_tree_spec_redo.fetch_add(1);
+            rc = attempt(false);
+        }
     }
     int cnts[2];
     M.info.resize(nroot);
-    TC_HIP(hipMemcpyAsync(cnts, bcnt.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    TC_HIP(hipMemcpyAsync(M.info.data(), bfinfo.p, sizeof(FieldInfo) * nroot,
-                          hipMemcpyDeviceToHost, st));
-    TC_HIP(hipStreamSynchronize(st));
-    (void)hipHostFree(hcnt);
+    if (rc == 0) {
+        TC_HIP(hipMemcpyAsync(cnts, bcnt.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipMemcpyAsync(M.info.data(), bfinfo.p, sizeof(FieldInfo) * nroot,
+                              hipMemcpyDeviceToHost, st));
+        TC_HIP(hipStreamSynchronize(st));
+    }
     if (rc) return rc;
     M.nnodes = cnts[0];
     std::vector<TopEntry> tops(cnts[1]);
