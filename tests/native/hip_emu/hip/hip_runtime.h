@@ -341,6 +341,7 @@ inline void __builtin_amdgcn_wave_barrier() {
 }
 inline void __builtin_amdgcn_fence(int, const char*) {}
 inline void __builtin_amdgcn_sched_barrier(int) {}   // an instruction-scheduling hint
+inline void __builtin_amdgcn_sched_group_barrier(int, int, int) {}
 // v_rcp_f64 is an estimate that callers refine with Newton steps; the exact quotient
 // is a valid estimate.
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }

@@ -37,13 +37,19 @@ hipStream_t stream() {
     });
     return g_stream;
 }
-static hipStream_t g_stream2 = nullptr;
-static std::once_flag g_stream2_once;
+static hipStream_t g_stream2 = nullptr, g_stream3 = nullptr;
+static std::once_flag g_stream2_once, g_stream3_once;
 hipStream_t stream2() {
     std::call_once(g_stream2_once, [] {
         if (hipStreamCreateWithFlags(&g_stream2, hipStreamNonBlocking) != hipSuccess) g_stream2 = nullptr;
     });
     return g_stream2;
+}
+hipStream_t stream3() {
+    std::call_once(g_stream3_once, [] {
+        if (hipStreamCreateWithFlags(&g_stream3, hipStreamNonBlocking) != hipSuccess) g_stream3 = nullptr;
+    });
+    return g_stream3;
 }
 
 // ---- caching allocator for temporaries ----

@@ -46,7 +46,8 @@ int fail(const std::string& msg);     // sets the error, returns TC_ERR_INVALID
 int hip_fail(hipError_t e, const char* what);  // sets the error, returns TC_ERR_HIP
 
 hipStream_t stream();   // the library's stream (default priority)
-hipStream_t stream2();  // a second stream for work that overlaps the first (default priority)
+hipStream_t stream2();  // second and third streams for work that overlaps the first
+hipStream_t stream3();  // (default priority)
 
 // Device memory for temporaries, kept for reuse across calls (a caching allocator:
 // blocks are returned to a pool instead of hipFree).  The pool holds at most a fixed

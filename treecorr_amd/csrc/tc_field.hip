@@ -335,18 +335,32 @@ __device__ __forceinline__ void segment_body(const long long s, const Seg* segs,
                     v[3] = wp; v[4] = double(E.w[i]);
                 }, sums);
             centroid_from_sums<C>(sums, cx, cy, cz, wf);
-            // CalculateSizeSq (src/Cell.cpp:89-100) and Bounds (include/Bounds.h).
-            for (long long i = start + tid; i < end; i += NT) {
-                double px = E.x[i], py = E.y[i], pz = E.z[i];
-                double dx = cx - px, dy = cy - py;
+            // CalculateSizeSq (src/Cell.cpp:89-100) and Bounds (include/Bounds.h).  Maxima
+            // and minima do not depend on the order, so each thread takes UB objects per
+            // round, all their loads in flight together (one workgroup reads the whole
+            // segment: near the root, millions of objects).
+            auto take = [&](double px, double py, double pz) {
+                const double dx = cx - px, dy = cy - py;
                 double dsq;
                 if (C == Flat) dsq = dx * dx + dy * dy;
-                else { double dz = cz - pz; dsq = dx * dx + dy * dy + dz * dz; }
+                else { const double dz = cz - pz; dsq = dx * dx + dy * dy + dz * dz; }
                 smax = fmax(smax, dsq);
                 lnx = fmin(lnx, px); lxx = fmax(lxx, px);
                 lny = fmin(lny, py); lxy = fmax(lxy, py);
                 lnz = fmin(lnz, pz); lxz = fmax(lxz, pz);
+            };
+            constexpr int UB = 4;
+            long long i = start + tid;
+            for (; i + (UB - 1) * NT < end; i += UB * NT) {
+                double px[UB], py[UB], pz[UB];
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    px[u] = E.x[i + u * NT]; py[u] = E.y[i + u * NT]; pz[u] = E.z[i + u * NT];
+                }
+#pragma unroll
+                for (int u = 0; u < UB; ++u) take(px[u], py[u], pz[u]);
             }
+            for (; i < end; i += NT) take(E.x[i], E.y[i], E.z[i]);
         }
         sizesq = block_max<NT>(smax, red);
         mnx = block_min<NT>(lnx, red); mxx = block_max<NT>(lxx, red);
@@ -484,10 +498,19 @@ __device__ __forceinline__ void segment_body(const long long s, const Seg* segs,
             bad = dim != fast.dim;
             if (!bad && P.sm == Mean) {
                 const double v = dim == 2 ? cz : dim == 1 ? cy : cx;
+                const double* cc = dim == 2 ? E.z : dim == 1 ? E.y : E.x;
                 int diff = 0;
-                for (long long i = start + tid; i < end; i += NT) {
-                    const double c = dim == 2 ? E.z[i] : dim == 1 ? E.y[i] : E.x[i];
-                    diff |= (c < v) != (c < fast.value);
+                if (v != fast.value) {
+                    constexpr int UB = 8;
+                    long long i = start + tid;
+                    for (; i + (UB - 1) * NT < end; i += UB * NT) {
+                        double c[UB];
+#pragma unroll
+                        for (int u = 0; u < UB; ++u) c[u] = cc[i + u * NT];
+#pragma unroll
+                        for (int u = 0; u < UB; ++u) diff |= (c[u] < v) != (c[u] < fast.value);
+                    }
+                    for (; i < end; i += NT) diff |= (cc[i] < v) != (cc[i] < fast.value);
                 }
                 bad = block_max<NT>(double(diff), red) != 0.;
             } else if (!bad && P.sm == Middle) {
@@ -1082,18 +1105,23 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
     const bool spec_on = spec_mode != 0 && (sm == Mean || sm == Middle);
     DevBuf bspec, bsnap[SPEC_LEVELS_MAX];
     if (spec_on && dalloc(bspec, sizeof(double) * (4 * SEG_TOP_MAX + 12 * SPEC_BLOCKS))) return 2;
-    struct SpecGuard {   // declared after the buffers: the exact pass ends before they go
-        hipStream_t st2 = nullptr;
-        hipEvent_t ev = nullptr;
+    struct SpecGuard {   // declared after the buffers: the exact passes end before they go
+        hipStream_t cs[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
         ~SpecGuard() {
-            if (st2) (void)hipStreamSynchronize(st2);
-            if (ev) (void)hipEventDestroy(ev);
+            for (int k = 0; k < 2; ++k) {
+                if (cs[k]) (void)hipStreamSynchronize(cs[k]);
+                if (ev[k]) (void)hipEventDestroy(ev[k]);
+            }
         }
     } guard;
     if (spec_on) {
-        guard.st2 = stream2();
-        if (!guard.st2) return fail("could not create the second stream");
-        TC_HIP(hipEventCreateWithFlags(&guard.ev, hipEventDisableTiming));
+        // the root level's exact pass on one stream, the other levels' on another
+        guard.cs[0] = stream2();
+        guard.cs[1] = stream3();
+        if (!guard.cs[0] || !guard.cs[1]) return fail("could not create the check streams");
+        for (int k = 0; k < 2; ++k)
+            TC_HIP(hipEventCreateWithFlags(&guard.ev[k], hipEventDisableTiming));
     }
     struct HostBuf {
         int* p = nullptr;
@@ -1120,17 +1148,21 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
                               hipMemcpyHostToDevice, st));
         TC_HIP(hipMemsetAsync(bcnt.p, 0, 16, st));
 
-        SpecBatch SB{};
+        SpecBatch SB{};          // speculative levels not yet handed to an exact pass
         long long spec_segs = 0;
-        bool checked = false;
+        int nspec = 0, nlaunch = 0;
+        bool done = false;       // no further speculative levels
         auto launch_check = [&]() -> int {
-            checked = true;
-            TC_HIP(hipEventRecord(guard.ev, st));
-            TC_HIP(hipStreamWaitEvent(guard.st2, guard.ev, 0));
+            hipStream_t cs = guard.cs[nlaunch];
+            hipEvent_t ev = guard.ev[nlaunch];
+            ++nlaunch;
+            TC_HIP(hipEventRecord(ev, st));
+            TC_HIP(hipStreamWaitEvent(cs, ev, 0));
             hipLaunchKernelGGL((k_segment_check<D, C>), dim3((unsigned)spec_segs), dim3(512), 0,
-                               guard.st2, SB, M.nodes, P, bfinfo.as<FieldInfo>(),
-                               bcnt.as<int>() + 2);
+                               cs, SB, M.nodes, P, bfinfo.as<FieldInfo>(), bcnt.as<int>() + 2);
             TC_HIP(hipGetLastError());
+            SB = SpecBatch{};
+            spec_segs = 0;
             return 0;
         };
         int cur = 0;
@@ -1140,12 +1172,12 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
             Seg* segs = bsegs[cur].as<Seg>();
             SegOut* sout = bsout.as<SegOut>();
             const int l0 = level == 0 ? 1 : 0;
-            bool sp = spec && !checked && SB.nlev < SPEC_LEVELS_MAX && nseg <= SEG_TOP_MAX &&
+            bool sp = spec && !done && nspec < SPEC_LEVELS_MAX && nseg <= SEG_TOP_MAX &&
                       (spec_mode >= 2 || n / nseg >= SPEC_MIN_N);
             const size_t segb = al(sizeof(Seg) * nseg), soutb = al(sizeof(SegOut) * nseg);
             const size_t db = al(sizeof(double) * n), fb = al(sizeof(float) * n);
             if (sp) {
-                DevBuf& sb = bsnap[SB.nlev];
+                DevBuf& sb = bsnap[nspec];
                 if (!sb.p) {   // the first attempt allocates, a second reuses
                     const size_t need = segb + soutb + 4 * db + 3 * fb;
                     sb.p = dev_alloc(need);
@@ -1153,7 +1185,10 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
                     else sp = false;   // no room for the snapshot: exact from here on
                 }
             }
-            if (!sp && SB.nlev > 0 && !checked && launch_check()) return 2;
+            if (!sp && nspec > 0 && !done) {
+                done = true;
+                if (SB.nlev > 0 && launch_check()) return 2;
+            }
             if (sp) {
                 int K = (int)(SPEC_BLOCKS / nseg);
                 const long long per = n / nseg / 1024;   // slices of about 1024 objects or more
@@ -1173,7 +1208,7 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
                                    dim3(512), 0, st, segs, 0LL, sout, E[cur], M.nodes,
                                    bcnt.as<int>(), btops.as<TopEntry>(), P,
                                    bfinfo.as<FieldInfo>(), l0, 0LL, sa);
-                char* q = static_cast<char*>(bsnap[SB.nlev].p);
+                char* q = static_cast<char*>(bsnap[nspec].p);
                 SpecLevel& L = SB.lv[SB.nlev++];
                 Seg* ssegs = reinterpret_cast<Seg*>(q); q += segb;
                 SegOut* ssout = reinterpret_cast<SegOut*>(q); q += soutb;
@@ -1197,6 +1232,8 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
                 TC_HIP(hipMemcpyAsync(S.b, E[cur].b, sizeof(float) * n, dd, st));
                 L.segs = ssegs; L.sout = ssout; L.E = S; L.nseg = nseg; L.level0 = l0;
                 spec_segs += nseg;
+                // the first level's exact pass (the longest chains) starts at once
+                if (++nspec == 1 && launch_check()) return 2;
             } else if (nseg <= SEG_TOP_MAX) {
                 // Few large segments near the root: 512 threads each (the other waves
                 // stage objects while one adds); the longest chains need the most stagers.
@@ -1250,8 +1287,8 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
             ++level;
             if (level > 4 * 64 + 64) return fail("tree build did not terminate");
         }
-        if (SB.nlev > 0 && !checked && launch_check()) return 2;
-        if (checked) TC_HIP(hipStreamSynchronize(guard.st2));
+        if (SB.nlev > 0 && launch_check()) return 2;
+        for (int k = 0; k < nlaunch; ++k) TC_HIP(hipStreamSynchronize(guard.cs[k]));
         return 0;
     };
 
