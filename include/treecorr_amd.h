@@ -149,7 +149,7 @@ int tc_set_traversal_chunk(long long items);
  * then write the node records and check every decision and every object's partition
  * side, and any difference makes the build start again without speculation, so the
  * trees are always the reference's.  mode 0: off; 1 (default): levels whose segments
- * hold 65536 objects or more; 3: every level of at most 64 segments; 2: as 3 with the
+ * hold 16384 objects or more; 3: every level of at most 64 segments; 2: as 3 with the
  * check forced to fail (2 and 3 are for tests).  Returns 0, or with mode < 0 changes
  * nothing and returns the number of exact rebuilds so far.  No reference counterpart:
  * TreeCorr builds its trees on the host (src/Cell.cpp:556-615). */

@@ -50,7 +50,7 @@ enum { SPEC_OFF = 0, SPEC_FAST = 1, SPEC_CHECK = 2 };
 constexpr int SPEC_NT = 256;            // threads of the slice kernels
 constexpr int SPEC_BLOCKS = 1024;       // slices per speculative level, at most
 constexpr int SPEC_LEVELS_MAX = 8;      // speculative levels per build, at most
-constexpr long long SPEC_MIN_N = 1 << 16;   // objects per segment worth speculating on
+constexpr long long SPEC_MIN_N = 1 << 14;   // objects per segment worth speculating on
 
 struct SpecArgs {
     const double* c1;   // SPEC_FAST: approximate centroids, 4 per segment
