@@ -562,17 +562,19 @@ __global__ __launch_bounds__(NT) void k_segment(const Seg* segs, long long seg0,
 
 // The exact pass of every speculative level in one launch, the root's segment first
 // (the longest chain): workgroup b is segment b of level 0, then level 1's, and so on.
+constexpr int CHECK_NT = 512;   // 1024 measured no faster
 template <int D, int C>
-__global__ __launch_bounds__(512) void k_segment_check(SpecBatch B, DNode* nodes, BuildParams P,
-                                                       FieldInfo* finfo, int* bad) {
+__global__ __launch_bounds__(CHECK_NT) void k_segment_check(SpecBatch B, DNode* nodes,
+                                                            BuildParams P, FieldInfo* finfo,
+                                                            int* bad) {
     long long s = blockIdx.x;
     int l = 0;
     while (l + 1 < B.nlev && s >= B.lv[l].nseg) { s -= B.lv[l].nseg; ++l; }
     const SpecLevel& L = B.lv[l];
     SpecArgs sp{};
     sp.bad = bad;
-    segment_body<D, C, 512, SPEC_CHECK>(s, L.segs, const_cast<SegOut*>(L.sout), L.E, nodes,
-                                        nullptr, nullptr, P, finfo, L.level0, 0, sp);
+    segment_body<D, C, CHECK_NT, SPEC_CHECK>(s, L.segs, const_cast<SegOut*>(L.sout), L.E, nodes,
+                                             nullptr, nullptr, P, finfo, L.level0, 0, sp);
 }
 
 // Speculative centroids, sizes and bounds: each segment of the level is cut into K
@@ -1158,7 +1160,7 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
             ++nlaunch;
             TC_HIP(hipEventRecord(ev, st));
             TC_HIP(hipStreamWaitEvent(cs, ev, 0));
-            hipLaunchKernelGGL((k_segment_check<D, C>), dim3((unsigned)spec_segs), dim3(512), 0,
+            hipLaunchKernelGGL((k_segment_check<D, C>), dim3((unsigned)spec_segs), dim3(CHECK_NT), 0,
                                cs, SB, M.nodes, P, bfinfo.as<FieldInfo>(), bcnt.as<int>() + 2);
             TC_HIP(hipGetLastError());
             SB = SpecBatch{};
