@@ -86,7 +86,10 @@ void* dev_alloc(size_t bytes) {
         g_pool.clear();
         g_pool_bytes = 0;
         (void)hipGetLastError();
-        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();   // the caller reports it (or does without)
+            return nullptr;
+        }
     }
     return p;
 }

@@ -51,6 +51,7 @@ constexpr int SPEC_NT = 256;            // threads of the slice kernels
 constexpr int SPEC_BLOCKS = 1024;       // slices per speculative level, at most
 constexpr int SPEC_LEVELS_MAX = 8;      // speculative levels per build, at most
 constexpr long long SPEC_MIN_N = 1 << 14;   // objects per segment worth speculating on
+constexpr size_t SPEC_SNAP_MAX = size_t(16) << 30;   // bytes of snapshots per build, at most
 
 struct SpecArgs {
     const double* c1;   // SPEC_FAST: approximate centroids, 4 per segment
@@ -1178,6 +1179,7 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
                       (spec_mode >= 2 || n / nseg >= SPEC_MIN_N);
             const size_t segb = al(sizeof(Seg) * nseg), soutb = al(sizeof(SegOut) * nseg);
             const size_t db = al(sizeof(double) * n), fb = al(sizeof(float) * n);
+            if (sp && (size_t)(nspec + 1) * (4 * db + 3 * fb) > SPEC_SNAP_MAX) sp = false;
             if (sp) {
                 DevBuf& sb = bsnap[nspec];
                 if (!sb.p) {   // the first attempt allocates, a second reuses
