@@ -265,3 +265,25 @@ def test_speculative_build_degenerate_split(gpu, oracle_lib):
         f, nd, ntop = device_tree(gpu, 4, 1, cat, kw)
     ref = O.Field(4, 1, x, y, None, g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw).dump()
     assert_same_tree(nd, ntop, ref)
+
+
+@pytest.mark.gpu
+def test_speculative_build_full_size_equals_exact_build(gpu):
+    """At the headline size (2,000,000 galaxies on the sphere, GG, the benchmark's tree
+    parameters) the default speculative build gives the same tree, node for node, as
+    the build without speculation, and its check passes (no rebuild)."""
+    cat, scale = C.make_catalog(C.SPHERE, 2000000, 2026, 0.3)
+    kw = dict(min_size=scale * 1e-4, max_size=scale * 0.01, split_method=2, brute=False,
+              min_top=4, max_top=10)
+    trees = []
+    for mode in (1, 0):
+        with _Speculation(mode) as s:
+            f, nd, ntop = device_tree(gpu, 4, 3, cat, kw)
+        assert s.redo == 0
+        trees.append((nd, ntop))
+        del f
+    (a, ta), (b, tb) = trees
+    assert ta == tb and len(a) == len(b)
+    for k in ('x', 'y', 'z', 'size', 'w', 'a', 'b', 'n'):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    np.testing.assert_array_equal(a['child'] < 0, b['child'] < 0)

@@ -115,7 +115,8 @@ using namespace tc;
 // Node array shared by the trees of a multi-root build (freed with the last of them).
 struct SharedNodes {
     DNode* p = nullptr;
-    ~SharedNodes() { if (p) (void)hipFree(p); }
+    size_t bytes = 0;
+    ~SharedNodes() { tc::dev_free(p, bytes); }
 };
 
 struct tc_field {
@@ -124,8 +125,8 @@ struct tc_field {
     DeviceTree T;
     std::shared_ptr<SharedNodes> shared;   // set when T.nodes belongs to a multi-root build
     ~tc_field() {
-        if (T.nodes && !shared) (void)hipFree(T.nodes);
-        if (T.dtop) (void)hipFree(T.dtop);
+        if (!shared) tc::dev_free(T.nodes, T.nodes_bytes);
+        tc::dev_free(T.dtop, T.dtop_bytes);
     }
 };
 
@@ -285,21 +286,28 @@ int tc_field_create_multi(int d, int coords, const double* x, const double* y, c
     for (int k = 0; k < 7; ++k) if (dev[k]) tc::dev_free(dev[k], sizeof(double) * nobj);
     auto shared = std::make_shared<SharedNodes>();
     shared->p = M.nodes;
+    shared->bytes = M.nodes_bytes;
     if (rc) return rc;
+    for (int r = 0; r < nroot; ++r) out[r] = nullptr;
     for (int r = 0; r < nroot && rc == 0; ++r) {
         tc_field* f = new tc_field;
         f->d = d; f->coords = coords; f->nobj = bnd[r + 1] - bnd[r];
         f->shared = shared;
         f->T.nodes = M.nodes;
+        f->T.nodes_bytes = M.nodes_bytes;
         f->T.nnodes = M.nnodes;
         f->T.top = std::move(M.top[r]);
         f->T.info = M.info[r];
-        hipError_t e = hipMalloc(&f->T.dtop, sizeof(int) * std::max<size_t>(1, f->T.top.size()));
-        if (e == hipSuccess)
-            e = hipMemcpy(f->T.dtop, f->T.top.data(), sizeof(int) * f->T.top.size(),
-                          hipMemcpyHostToDevice);
-        if (e != hipSuccess) rc = tc::hip_fail(e, "copying top cells");
+        f->T.dtop_bytes = sizeof(int) * std::max<size_t>(1, f->T.top.size());
+        f->T.dtop = static_cast<int*>(tc::dev_alloc(f->T.dtop_bytes));
         out[r] = f;
+        if (!f->T.dtop) {
+            rc = tc::fail("out of device memory for the top cells");
+            break;
+        }
+        const hipError_t e = hipMemcpy(f->T.dtop, f->T.top.data(),
+                                       sizeof(int) * f->T.top.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = tc::hip_fail(e, "copying top cells");
     }
     if (rc)
         for (int r = 0; r < nroot; ++r) { delete out[r]; out[r] = nullptr; }

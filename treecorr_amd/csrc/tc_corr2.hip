@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace tc {
@@ -691,19 +692,31 @@ struct Corr2Scratch {
     unsigned long long* boff = nullptr; // two offset arrays, TRAV_SLOTS+1 each
     int nb = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // sizes of the per-bin arrays (they depend on nb) and of the fixed ones
+    static size_t hist_bytes(int nb) { return sizeof(double) * (size_t)NQ2 * nb * TRAV_SLOTS; }
+    static size_t np_bytes(int nb) { return sizeof(unsigned long long) * (size_t)nb * TRAV_SLOTS; }
+    static constexpr size_t kCntBytes = sizeof(unsigned long long) * NCNT * TRAV_SLOTS;
+    static constexpr size_t kBcountBytes = sizeof(unsigned int) * 2 * TRAV_SLOTS;
+    static constexpr size_t kBoffBytes = sizeof(unsigned long long) * 2 * (TRAV_SLOTS + 1);
+    void free_bins() {
+        dev_free(slot_hi, hist_bytes(nb));
+        dev_free(slot_lo, hist_bytes(nb));
+        dev_free(slot_np, np_bytes(nb));
+        dev_free(red, sizeof(double) * (size_t)NQ2 * nb);
+        dev_free(red_np, sizeof(unsigned long long) * nb);
+        slot_hi = slot_lo = red = nullptr;
+        slot_np = red_np = nullptr;
+    }
     ~Corr2Scratch() {
-        if (stack) (void)hipFree(stack);
-        for (void* q : outq) if (q) (void)hipFree(q);
-        if (slot_hi) (void)hipFree(slot_hi);
-        if (slot_lo) (void)hipFree(slot_lo);
-        if (slot_np) (void)hipFree(slot_np);
-        if (slot_cnt) (void)hipFree(slot_cnt);
-        if (red) (void)hipFree(red);
-        if (red_np) (void)hipFree(red_np);
-        if (red_cnt) (void)hipFree(red_cnt);
-        if (outcount) (void)hipFree(outcount);
-        if (bcount) (void)hipFree(bcount);
-        if (boff) (void)hipFree(boff);
+        // the caching allocator keeps these for the next correlation
+        dev_free(stack, stack_bytes);
+        for (void* q : outq) dev_free(q, outq_bytes);
+        free_bins();
+        dev_free(slot_cnt, kCntBytes);
+        dev_free(red_cnt, sizeof(unsigned long long) * NCNT);
+        dev_free(outcount, sizeof(unsigned long long));
+        dev_free(bcount, kBcountBytes);
+        dev_free(boff, kBoffBytes);
         if (h_outcount) (void)hipHostFree(h_outcount);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -711,33 +724,40 @@ struct Corr2Scratch {
 };
 
 static int ensure_scratch(Corr2Scratch& S, int nb, size_t item_bytes) {
+    auto get = [](auto*& p, size_t bytes) {
+        p = static_cast<std::remove_reference_t<decltype(p)>>(dev_alloc(bytes));
+        return p != nullptr;
+    };
+    const char* oom = "out of device memory for the traversal";
     if (S.nb != nb || !S.slot_hi) {
         const size_t nh = (size_t)NQ2 * nb;
-        if (S.slot_hi) { (void)hipFree(S.slot_hi); (void)hipFree(S.slot_lo); (void)hipFree(S.slot_np);
-                         (void)hipFree(S.red); (void)hipFree(S.red_np); }
-        TC_HIP(hipMalloc(&S.slot_hi, sizeof(double) * nh * TRAV_SLOTS));
-        TC_HIP(hipMalloc(&S.slot_lo, sizeof(double) * nh * TRAV_SLOTS));
-        TC_HIP(hipMalloc(&S.slot_np, sizeof(unsigned long long) * nb * TRAV_SLOTS));
-        TC_HIP(hipMalloc(&S.red, sizeof(double) * nh));
-        TC_HIP(hipMalloc(&S.red_np, sizeof(unsigned long long) * nb));
+        S.free_bins();
         S.nb = nb;
+        if (!get(S.slot_hi, Corr2Scratch::hist_bytes(nb)) ||
+            !get(S.slot_lo, Corr2Scratch::hist_bytes(nb)) ||
+            !get(S.slot_np, Corr2Scratch::np_bytes(nb)) || !get(S.red, sizeof(double) * nh) ||
+            !get(S.red_np, sizeof(unsigned long long) * nb))
+            return fail(oom);
     }
     if (!S.slot_cnt) {
-        TC_HIP(hipMalloc(&S.slot_cnt, sizeof(unsigned long long) * NCNT * TRAV_SLOTS));
-        TC_HIP(hipMalloc(&S.red_cnt, sizeof(unsigned long long) * NCNT));
-        TC_HIP(hipMalloc(&S.outcount, sizeof(unsigned long long)));
+        if (!get(S.slot_cnt, Corr2Scratch::kCntBytes) ||
+            !get(S.red_cnt, sizeof(unsigned long long) * NCNT) ||
+            !get(S.outcount, sizeof(unsigned long long)) ||
+            !get(S.bcount, Corr2Scratch::kBcountBytes) || !get(S.boff, Corr2Scratch::kBoffBytes))
+            return fail(oom);
         TC_HIP(hipHostMalloc((void**)&S.h_outcount, sizeof(unsigned long long)));
-        TC_HIP(hipMalloc(&S.bcount, sizeof(unsigned int) * 2 * TRAV_SLOTS));
-        TC_HIP(hipMalloc(&S.boff, sizeof(unsigned long long) * 2 * (TRAV_SLOTS + 1)));
         TC_HIP(hipEventCreate(&S.ev0));
         TC_HIP(hipEventCreate(&S.ev1));
     }
     const size_t qb = item_bytes * 4 * (TRAV_CHUNK + (size_t)TRAV_THREADS * TRAV_SLOTS);
     if (S.outq_bytes < qb) {
         for (void*& q : S.outq) {
-            if (q) (void)hipFree(q);
-            TC_HIP(hipMalloc(&q, qb));
+            dev_free(q, S.outq_bytes);
+            q = nullptr;
         }
+        S.outq_bytes = 0;
+        for (void*& q : S.outq)
+            if (!(q = dev_alloc(qb))) return fail(oom);
         S.outq_bytes = qb;
     }
     return 0;
@@ -746,12 +766,12 @@ static int ensure_scratch(Corr2Scratch& S, int nb, size_t item_bytes) {
 static int ensure_stack(Corr2Scratch& S, size_t need, size_t keep) {
     if (need <= S.stack_bytes) return 0;
     size_t cap = std::max(need, std::max<size_t>(2 * S.stack_bytes, size_t(32) << 20));
-    void* ns = nullptr;
-    TC_HIP(hipMalloc(&ns, cap));
+    void* ns = dev_alloc(cap);
+    if (!ns) return fail("out of device memory for the traversal stack");
     if (S.stack && keep > 0)
         TC_HIP(hipMemcpyAsync(ns, S.stack, keep, hipMemcpyDeviceToDevice, stream()));
     TC_HIP(hipStreamSynchronize(stream()));
-    if (S.stack) (void)hipFree(S.stack);
+    dev_free(S.stack, S.stack_bytes);
     S.stack = ns;
     S.stack_bytes = cap;
     return 0;

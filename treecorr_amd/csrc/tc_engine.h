@@ -7,11 +7,15 @@
 namespace tc {
 
 // A tree resident in GPU memory.
+// Device arrays come from the library's caching allocator (dev_alloc/dev_free), so
+// they carry their sizes.
 struct DeviceTree {
     DNode* nodes = nullptr;
+    size_t nodes_bytes = 0;
     int nnodes = 0;
     std::vector<int> top;     // top-level node ids in TreeCorr's order
     int* dtop = nullptr;      // the same on the device
+    size_t dtop_bytes = 0;
     FieldInfo info{};
 };
 
@@ -23,7 +27,8 @@ int build_tree(int D, int C, const double* x, const double* y, const double* z, 
 // Trees of several root ranges [bounds[r], bounds[r+1]) built together, sharing one
 // node array (node ids, child links and top lists are indices into it).
 struct MultiTree {
-    DNode* nodes = nullptr;     // owned by the caller after the build
+    DNode* nodes = nullptr;     // owned by the caller after the build (dev_free)
+    size_t nodes_bytes = 0;
     int nnodes = 0;
     std::vector<std::vector<int>> top;   // per root, in TreeCorr's order
     std::vector<FieldInfo> info;         // per root

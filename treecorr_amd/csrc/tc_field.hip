@@ -1099,7 +1099,9 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
 
     // Nodes: at most 2n-1 per root's objects (each split adds two).
     const long long maxnodes = 2 * n;
-    TC_HIP(hipMalloc(&M.nodes, sizeof(DNode) * maxnodes));
+    M.nodes_bytes = sizeof(DNode) * maxnodes;
+    M.nodes = static_cast<DNode*>(dev_alloc(M.nodes_bytes));
+    if (!M.nodes) return fail("out of device memory for the tree nodes");
 
     // Speculation (Mean and Middle splits): the levels near the root partition with
     // centroids summed in parallel; their exact pass runs on the second stream from
@@ -1351,12 +1353,15 @@ int build_tree(int D, int C, const double* x, const double* y, const double* z, 
     const int rc = build_trees(D, C, x, y, z, d1, d2, w, wpos, n, {0, n}, minsize, maxsize, sm,
                                brute, mintop, maxtop, M);
     T.nodes = M.nodes;      // the tree owns the node array from here on
+    T.nodes_bytes = M.nodes_bytes;
     M.nodes = nullptr;
     if (rc) return rc;
     T.nnodes = M.nnodes;
     T.top = std::move(M.top[0]);
     T.info = M.info[0];
-    TC_HIP(hipMalloc(&T.dtop, sizeof(int) * std::max<size_t>(1, T.top.size())));
+    T.dtop_bytes = sizeof(int) * std::max<size_t>(1, T.top.size());
+    T.dtop = static_cast<int*>(dev_alloc(T.dtop_bytes));
+    if (!T.dtop) return fail("out of device memory for the top cells");
     TC_HIP(hipMemcpy(T.dtop, T.top.data(), sizeof(int) * T.top.size(), hipMemcpyHostToDevice));
     return 0;
 }
