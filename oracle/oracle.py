@@ -112,14 +112,23 @@ class Field:
         return out
 
 
+# codes of the mixed cross correlations in tco_corr2_sampled (tc_oracle.h Corr2Kind)
+MIXED = {(0, 1): 8, (0, 4): 9, (1, 4): 10}
+
+
 def corr2(d, coords, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, exact=True,
-          nthreads=0, row_stride=1):
+          nthreads=0, row_stride=1, d2=None):
     """Run the oracle's 2-point traversal; returns dict of arrays and counters.
 
+    d2 (the second field's type) differs from d for NK, NG and KG (cross only).
     row_stride > 1 processes only every row_stride-th top-level row (timing samples)."""
     out = {k: np.zeros(nbins) for k in
            ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')}
     cnt = (ctypes.c_longlong * 4)()
+    if d2 is not None and d2 != d:
+        if f2 is None:
+            raise ValueError('mixed correlations are cross correlations')
+        d = MIXED[(d, d2)]
     rc = lib().tco_corr2_sampled(f1.h, f2.h if f2 is not None else None, d, coords, nbins,
                                  float(min_sep), float(max_sep), float(bin_size), float(b),
                                  float(a), int(quick), int(exact), int(nthreads),

@@ -88,6 +88,7 @@ _FIELD = {(0, 1): 'NFieldFlat', (0, 2): 'NFieldThreeD', (0, 3): 'NFieldSphere',
           (1, 1): 'KFieldFlat', (1, 2): 'KFieldThreeD', (1, 3): 'KFieldSphere',
           (4, 1): 'GFieldFlat', (4, 2): 'GFieldThreeD', (4, 3): 'GFieldSphere'}
 _CORR = {0: 'NNCorr', 1: 'KKCorr', 4: 'GGCorr'}
+_CORR_MIXED = {(0, 1): 'NKCorr', (0, 4): 'NGCorr', (1, 4): 'KGCorr'}
 
 
 def make_field(d, coords, x, y, z=None, k=None, g1=None, g2=None, w=None, wpos=None,
@@ -111,8 +112,11 @@ def make_field(d, coords, x, y, z=None, k=None, g1=None, g2=None, w=None, wpos=N
                sm, 0, bool(brute), int(min_top), int(max_top))
 
 
-def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthreads=None):
-    """Run the reference 2-point engine (auto when f2 is None).  Returns raw sums."""
+def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthreads=None,
+          d2=None):
+    """Run the reference 2-point engine (auto when f2 is None).  Returns raw sums.
+    d2 != d selects NK, NG or KG (treecorr/nkcorrelation.py, ngcorrelation.py,
+    kgcorrelation.py: xi, and xi_im for a G second field)."""
     m = module()
     if nthreads is not None:
         m.SetOMPThreads(int(nthreads))
@@ -120,9 +124,11 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthre
            ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')}
     e = np.array([])
     xi = [out['xi0'], out['xi1'], out['xi2'], out['xi3']]
-    nxi = {0: 0, 1: 1, 4: 4}[d]
+    mixed = d2 is not None and d2 != d
+    nxi = (2 if d2 == 4 else 1) if mixed else {0: 0, 1: 1, 4: 4}[d]
     xi = [xi[i] if i < nxi else e for i in range(4)]
-    corr = getattr(m, _CORR[d])(m.Log, float(min_sep), float(max_sep), int(nbins),
+    name = _CORR_MIXED[(d, d2)] if mixed else _CORR[d]
+    corr = getattr(m, name)(m.Log, float(min_sep), float(max_sep), int(nbins),
                                 float(bin_size), float(b), float(a), -1.7976931348623157e308,
                                 1.7976931348623157e308, 0., 0., 0., *xi, out['meanr'],
                                 out['meanlogr'], out['weight'], out['npairs'])

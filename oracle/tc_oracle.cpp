@@ -371,6 +371,25 @@ struct Corr2Engine {
             acc.add(3, k, z1rz2i + z1iz2r);   // xim_im
             acc.add(0, k, z1rz2r + z1iz2i);   // xip
             acc.add(1, k, z1rz2i - z1iz2r);   // xip_im
+        } else if (D == XK_NK) {
+            // DirectHelper<1>, src/Corr2.cpp:408-420: getWK of an N cell is its w
+            // (include/ProjectHelper.h:75-80)
+            acc.add(0, k, double(c1.w) * double(c2.wk));
+        } else if (D == XK_NG || D == XK_KG) {
+            // DirectHelper<2>, src/Corr2.cpp:423-442: g2 projected onto the line to c1
+            // (ProjectHelper<C>::Project(c1,c2,z2), include/ProjectHelper.h:107-113,
+            // :348-355, :545-555), times -getWK(c1) (tangential, not radial)
+            std::complex<double> g2(c2.gr, c2.gi);
+            if (C == Flat) {
+                g2 = cmul(g2, expm2ialpha(std::complex<double>(c2.p.x - c1.p.x, c2.p.y - c1.p.y)));
+            } else {
+                Pt p1 = c1.p, p2 = c2.p;
+                if (C == ThreeD) { sphere_normalize_recip(p1); sphere_normalize_recip(p2); }
+                g2 = cmul(g2, expm2ialpha(sphere_direction(p1, p2)));
+            }
+            const double f = -(D == XK_NG ? double(c1.w) : double(c1.wk));
+            acc.add(0, k, g2.real() * f);
+            acc.add(1, k, g2.imag() * f);
         }
     }
 
@@ -530,6 +549,8 @@ void tco_field_dump(void* h, double* pos, float* size, float* w, float* wk, floa
 }
 
 // 2-point correlation over one tree (auto, f2 == NULL) or two trees (cross).
+// d: NData, KData or GData for NN, KK, GG; XK_NK, XK_NG, XK_KG for the mixed cross
+// correlations (f1 the N or K field, f2 the K or G field).
 // out arrays (nbins each, NULL allowed): xi0..xi3, meanr, meanlogr, weight, npairs.
 // Values are ADDED to the arrays.  counters[4]: node-pair visits, accepted pairs,
 // accepted leaf-leaf pairs, refined singleBin evaluations.  row_stride > 1 keeps
@@ -556,6 +577,12 @@ int tco_corr2_sampled(void* f1, void* f2, int d, int coords, int nbins, double m
         if (coords == Flat) TCO_RUN(KData, Flat); else if (coords == ThreeD) TCO_RUN(KData, ThreeD); else TCO_RUN(KData, Sphere);
     } else if (d == GData) {
         if (coords == Flat) TCO_RUN(GData, Flat); else if (coords == ThreeD) TCO_RUN(GData, ThreeD); else TCO_RUN(GData, Sphere);
+    } else if (d == XK_NK && t2) {
+        if (coords == Flat) TCO_RUN(XK_NK, Flat); else if (coords == ThreeD) TCO_RUN(XK_NK, ThreeD); else TCO_RUN(XK_NK, Sphere);
+    } else if (d == XK_NG && t2) {
+        if (coords == Flat) TCO_RUN(XK_NG, Flat); else if (coords == ThreeD) TCO_RUN(XK_NG, ThreeD); else TCO_RUN(XK_NG, Sphere);
+    } else if (d == XK_KG && t2) {
+        if (coords == Flat) TCO_RUN(XK_KG, Flat); else if (coords == ThreeD) TCO_RUN(XK_KG, ThreeD); else TCO_RUN(XK_KG, Sphere);
     } else {
         return 1;
     }

@@ -143,3 +143,36 @@ def test_oracle_against_literal_reference(oracle_lib):
     np.testing.assert_array_equal(o['npairs'], r['npairs'])
     for k in KEYS[:-1]:
         assert normwise(o[k], r[k]) < 1e-12, k
+
+
+@pytest.mark.parametrize('d1,d2', [(0, 1), (0, 4), (1, 4)])
+@pytest.mark.parametrize('coords', [1, 2, 3])
+def test_oracle_mixed_cross_against_literal_reference(oracle_lib, d1, d2, coords):
+    """NK, NG and KG (DirectHelper<1>/<2>, src/Corr2.cpp:408-442): the oracle's sums
+    against the compiled reference's NKCorr/NGCorr/KGCorr on the same trees' inputs,
+    flat, 3-d and spherical.  The xi sums are far from zero, so a dropped minus sign,
+    a projection at the wrong end or a swapped weight fails."""
+    from oracle import ref as R
+    if R.module() is None:
+        pytest.skip('oracle/_ref not built')
+    O = oracle_lib
+    cat1, scale = C.make_catalog(coords, 4000, 11 + coords, 0.3)
+    cat2, _ = C.make_catalog(coords, 5000, 21 + coords, 0.3)
+    bn = C.binning(scale, 12, 0.01, 0.5, 0.1, 0.1 if d2 == 4 else 1.0)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+
+    def cols(cat):
+        return dict(k=cat['k'], g1=cat['g1'], g2=cat['g2'], w=cat['w'])
+    r1 = R.make_field(d1, coords, cat1['x'], cat1['y'], cat1['z'], **cols(cat1), **kw)
+    r2 = R.make_field(d2, coords, cat2['x'], cat2['y'], cat2['z'], **cols(cat2), **kw)
+    r = R.corr2(d1, r1, r2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'],
+                bn['a'], d2=d2)
+    o1 = O.Field(d1, coords, cat1['x'], cat1['y'], cat1['z'], **cols(cat1), **kw)
+    o2 = O.Field(d2, coords, cat2['x'], cat2['y'], cat2['z'], **cols(cat2), **kw)
+    o = O.corr2(d1, coords, o1, o2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                bn['b'], bn['a'], d2=d2)
+    np.testing.assert_array_equal(o['npairs'], r['npairs'])
+    keys = ['xi0', 'meanr', 'meanlogr', 'weight'] + (['xi1'] if d2 == 4 else [])
+    for k in keys:
+        assert np.max(np.abs(r[k])) > 0, k
+        assert normwise(o[k], r[k]) < 1e-12, k
