@@ -94,8 +94,10 @@ int tc_field_destroy(tc_field* f);
 /* ---- 2-point: replace {NN,KK,GG}Corr(...) + processAuto/processCross ----
  * Reference: BuildCorr2 src/Corr2.cpp:842-864 bound at :1135-1145;
  * ProcessAuto/ProcessCross :886-993 bound at :1151-1157 (BaseCorr2::process,
- * :146-288).  d1 == d2 in {0,1,4}; bin_type Log; metric Euclidean.
- * xi1..xi3 may be NULL where the correlation has fewer xi arrays. */
+ * :146-288).  (d1, d2) is NN (0,0), KK (1,1), GG (4,4), or the mixed NK (0,1), NG (0,4),
+ * KG (1,4) (src/Corr2.cpp DirectHelper<1>/<2>, :408-442; cross only, f1/fields1 the
+ * d1 side); bin_type Log; metric Euclidean.  xi arrays: KK and NK xi0; NG and KG xi0
+ * (xi) and xi1 (xi_im); GG xi0..xi3; the unused ones may be NULL. */
 int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, int nbins,
                     double binsize, double b, double a, double minrpar, double maxrpar,
                     double xp, double yp, double zp, double* xi0, double* xi1, double* xi2,

@@ -47,13 +47,15 @@ class Field:
         return n.value
 
 
-def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False):
+def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, d2=None):
+    """d2 != d: the mixed NK, NG, KG cross correlations (f1 the d side)."""
     out = {k: np.zeros(nbins) for k in KEYS2}
-    nxi = {0: 0, 1: 1, 4: 4}[d]
+    d2 = d if d2 is None else d2
+    nxi = {0: 0, 1: 1, 4: 4}[d] if d2 == d else (2 if d2 == 4 else 1)
     xi = [_lib.dptr(out['xi%d' % i]) if i < nxi else None for i in range(4)]
     h = ctypes.c_void_p()
     L = _lib.lib()
-    _lib.check(L.tc_corr2_create(d, d, 0, float(min_sep), float(max_sep), int(nbins),
+    _lib.check(L.tc_corr2_create(d, d2, 0, float(min_sep), float(max_sep), int(nbins),
                                  float(bin_size), float(b), float(a), -1.7976931348623157e308,
                                  1.7976931348623157e308, 0., 0., 0., *xi,
                                  *[_lib.dptr(out[k]) for k in KEYS2[4:]], ctypes.byref(h)))
@@ -67,12 +69,8 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False):
         out['counters'] = dict(visits=c[0], accepted=c[1], leafpairs=c[2], refined=c[3])
     finally:
         L.tc_corr2_destroy(h)
-    if nxi == 0:
-        for i in range(4):
-            out['xi%d' % i] = np.array([])
-    elif nxi == 1:
-        for i in range(1, 4):
-            out['xi%d' % i] = np.array([])
+    for i in range(nxi, 4):
+        out['xi%d' % i] = np.array([])
     return out
 
 

@@ -142,3 +142,60 @@ def test_gpu_small_chunks_same_result(gpu, oracle_lib, chunk):
         assert normwise(out[k], ref[k]) < RTOL, k
     for k in ('visits', 'accepted', 'leafpairs', 'refined'):
         assert out['counters'][k] == ref['counters'][k], k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d1,d2', [(0, 1), (0, 4), (1, 4)])
+@pytest.mark.parametrize('coords', [1, 2, 3])
+def test_gpu_mixed_cross_matches_oracle(gpu, oracle_lib, d1, d2, coords):
+    """NK, NG and KG cross correlations (src/Corr2.cpp DirectHelper<1>/<2>) against the
+    oracle, itself pinned to the compiled reference: npairs and the node-pair counters
+    exact, the sums to RTOL."""
+    import capi
+    O = oracle_lib
+    cat1, scale = C.make_catalog(coords, 30000, 300 + 10 * d1 + d2 + coords, 0.3)
+    cat2, _ = C.make_catalog(coords, 40000, 400 + 10 * d1 + d2 + coords, 0.3)
+    bn = C.binning(scale, 15, 0.003, 0.3, 0.1, 0.1 if d2 == 4 else 1.0)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    f1 = capi.Field(d1, coords, cat1, brute=False, **kw)
+    f2 = capi.Field(d2, coords, cat2, brute=False, **kw)
+    out = capi.corr2(d1, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                     bn['b'], bn['a'], d2=d2)
+
+    def ofield(d, cat):
+        return O.Field(d, coords, cat['x'], cat['y'], cat['z'], k=cat['k'], g1=cat['g1'],
+                       g2=cat['g2'], w=cat['w'], **kw)
+    ref = O.corr2(d1, coords, ofield(d1, cat1), ofield(d2, cat2), bn['nbins'], bn['min_sep'],
+                  bn['max_sep'], bn['bin_size'], bn['b'], bn['a'], exact=True, d2=d2)
+    np.testing.assert_array_equal(out['npairs'], ref['npairs'])
+    keys = ['xi0', 'meanr', 'meanlogr', 'weight'] + (['xi1'] if d2 == 4 else [])
+    for k in keys:
+        assert np.max(np.abs(ref[k])) > 0, k
+        assert normwise(out[k], ref[k]) < RTOL, k
+    for k in ('visits', 'accepted', 'leafpairs', 'refined'):
+        assert out['counters'][k] == ref['counters'][k], k
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_rejects_auto_and_wrong_sides(gpu):
+    """A mixed correlation is cross only, with the d1 field first."""
+    import ctypes
+    import capi
+    from treecorr_amd import _lib
+    cat, scale = C.make_catalog(C.FLAT, 500, 5, 0.)
+    kw = dict(min_size=0., max_size=1., min_top=2, brute=False)
+    fn = capi.Field(0, 1, cat, **kw)
+    fg = capi.Field(4, 1, cat, **kw)
+    L = _lib.lib()
+    xi0, xi1, m, ml, w, npr = (np.zeros(5) for _ in range(6))
+    h = ctypes.c_void_p()
+    _lib.check(L.tc_corr2_create(0, 4, 0, 1., 10., 5, np.log(10.) / 5, 0.1, 1., -1e308, 1e308,
+                                 0., 0., 0., _lib.dptr(xi0), _lib.dptr(xi1), None, None,
+                                 _lib.dptr(m), _lib.dptr(ml), _lib.dptr(w), _lib.dptr(npr),
+                                 ctypes.byref(h)))
+    try:
+        assert L.tc_corr2_process_auto(h, fn.h, 0, 0, 1) != 0
+        assert L.tc_corr2_process_cross(h, fg.h, fn.h, 0, 0, 1) != 0
+        _lib.check(L.tc_corr2_process_cross(h, fn.h, fg.h, 0, 0, 1))
+    finally:
+        L.tc_corr2_destroy(h)

@@ -131,7 +131,8 @@ struct tc_field {
 };
 
 struct tc_corr2 {
-    int d = 0;
+    int d = 0, d2 = 0;     // field types of the two sides
+    int kind = 0;          // corr2_kind(d, d2): NData, KData, GData or XK_*
     C2Params P{};
     double* xi[4] = {nullptr, nullptr, nullptr, nullptr};
     double *meanr = nullptr, *meanlogr = nullptr, *weight = nullptr, *npairs = nullptr;
@@ -358,8 +359,9 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
     (void)minrpar; (void)maxrpar;
     if (!out) return tc::fail("out is NULL");
     *out = nullptr;
-    if (d1 != d2 || (d1 != NData && d1 != KData && d1 != GData)) {
-        set_error("only NN, KK and GG 2-point correlations are implemented");
+    const int kind = corr2_kind(d1, d2);
+    if (kind < 0 || (d1 != NData && d1 != KData && d1 != GData)) {
+        set_error("only NN, KK, GG, NK, NG and KG 2-point correlations are implemented");
         return TC_ERR_UNSUPPORTED;
     }
     if (bin_type != 0) {
@@ -374,10 +376,13 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
         return tc::fail("invalid binning");
     if (!weight || !meanr || !meanlogr || !npairs)
         return tc::fail("meanr, meanlogr, weight and npairs are required");
-    if (d1 == KData && !xi0) return tc::fail("KK needs xi0");
-    if (d1 == GData && (!xi0 || !xi1 || !xi2 || !xi3)) return tc::fail("GG needs xi0..xi3");
+    const int nxi = corr2_nxi(kind);
+    if ((nxi > 0 && !xi0) || (nxi > 1 && !xi1) || (nxi > 2 && (!xi2 || !xi3)))
+        return tc::fail("the correlation needs its xi arrays");
     tc_corr2* c = new tc_corr2;
     c->d = d1;
+    c->d2 = d2;
+    c->kind = kind;
     C2Params& P = c->P;
     P.nbins = nbins; P.quick = 0;
     P.minsep = minsep; P.maxsep = maxsep; P.binsize = binsize; P.b = b; P.a = a;
@@ -412,7 +417,8 @@ static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int 
         set_error("only the Euclidean metric is implemented");
         return TC_ERR_UNSUPPORTED;
     }
-    if (f1->d != c->d || (f2 && (f2->d != c->d || f2->coords != f1->coords)))
+    if (!f2 && c->d != c->d2) return tc::fail("a mixed correlation needs two fields");
+    if (f1->d != c->d || (f2 && (f2->d != c->d2 || f2->coords != f1->coords)))
         return tc::fail("field types do not match the correlation");
     if (int rc = check_gpu()) return rc;
     C2Params P = c->P;
@@ -424,11 +430,11 @@ static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int 
     std::vector<unsigned long long> np(nb), cnt(4);
     std::lock_guard<std::mutex> run_lock(g_run_m);
     if (!g_s2) g_s2 = new_corr2_scratch();
-    int rc = tc::corr2_run(*g_s2, c->d, f1->coords, f1->T.nodes, f1->T.dtop, (int)f1->T.top.size(),
+    int rc = tc::corr2_run(*g_s2, c->kind, f1->coords, f1->T.nodes, f1->T.dtop, (int)f1->T.top.size(),
                            f2 ? f2->T.nodes : nullptr, f2 ? f2->T.dtop : nullptr,
                            f2 ? (int)f2->T.top.size() : 0, P, out.data(), np.data(), cnt.data());
     if (rc) return rc;
-    const int nxi = c->d == NData ? 0 : c->d == KData ? 1 : 4;
+    const int nxi = corr2_nxi(c->kind);
     for (int q = 0; q < nxi; ++q)
         for (int k = 0; k < nb; ++k) c->xi[q][k] += out[(size_t)q * nb + k];
     for (int k = 0; k < nb; ++k) {
@@ -464,17 +470,18 @@ int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
     }
     const bool same = fields2 == nullptr;
     if (!same && nfield2 <= 0) return tc::fail("nfield2 must be positive");
+    if (same && c->d != c->d2) return tc::fail("a mixed correlation needs two field lists");
     const int coords = fields1[0] ? fields1[0]->coords : 0;
-    auto check_side = [&](tc_field* const* fs, int n) -> int {
+    auto check_side = [&](tc_field* const* fs, int n, int d) -> int {
         for (int k = 0; k < n; ++k) {
             if (!fs[k]) return tc::fail("NULL field in the patch list");
-            if (fs[k]->d != c->d || fs[k]->coords != coords)
+            if (fs[k]->d != d || fs[k]->coords != coords)
                 return tc::fail("field types do not match the correlation");
         }
         return 0;
     };
-    if (int rc = check_side(fields1, nfield1)) return rc;
-    if (!same) if (int rc = check_side(fields2, nfield2)) return rc;
+    if (int rc = check_side(fields1, nfield1, c->d)) return rc;
+    if (!same) if (int rc = check_side(fields2, nfield2, c->d2)) return rc;
     const int n2 = same ? nfield1 : nfield2;
     for (int p = 0; p < npair; ++p)
         if (p1[p] < 0 || p1[p] >= nfield1 || p2[p] < 0 || p2[p] >= n2)
@@ -507,11 +514,11 @@ int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
     {
         std::lock_guard<std::mutex> run_lock(g_run_m);
         if (!g_s2) g_s2 = new_corr2_scratch();
-        if (int rc = tc::corr2_run_pairs(*g_s2, c->d, coords, t1, t2, pi, pj, pa, P, out.data(),
+        if (int rc = tc::corr2_run_pairs(*g_s2, c->kind, coords, t1, t2, pi, pj, pa, P, out.data(),
                                          np.data(), cnt.data()))
             return rc;
     }
-    const int nxi = c->d == NData ? 0 : c->d == KData ? 1 : 4;
+    const int nxi = corr2_nxi(c->kind);
     for (size_t q = 0; q < which.size(); ++q) {
         const double* o = out.data() + nh * q;
         double* r = pair_out + (size_t)8 * nb * which[q];

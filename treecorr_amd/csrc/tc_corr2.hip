@@ -196,6 +196,35 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
         atomicAdd(&hist[(Q_XI1 * nb + k) * R], ri - ir);   // xip_im
         atomicAdd(&hist[(Q_XI2 * nb + k) * R], rr - ii);   // xim
         atomicAdd(&hist[(Q_XI3 * nb + k) * R], ri + ir);   // xim_im
+    } else if (D == XK_NK) {
+        // DirectHelper<1>, src/Corr2.cpp:408-420; getWK of an N cell is its w
+        // (include/ProjectHelper.h:75-80)
+        atomicAdd(&hist[(Q_XI0 * nb + k) * R], double(c1.w) * double(c2.a));
+    } else if (D == XK_NG || D == XK_KG) {
+        // DirectHelper<2>, src/Corr2.cpp:423-442: c2's shear projected onto the line to
+        // c1 (ProjectHelper<C>::Project(c1,c2,z2), include/ProjectHelper.h:107-113,
+        // :348-355, :545-555), times -getWK(c1)
+        double g2r = c2.a, g2i = c2.b;
+        double er, ei;
+        if (C == Flat) {
+            expm2ialpha_proj(c2.x - c1.x, c2.y - c1.y, er, ei);
+        } else {
+            double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
+            double dsq = rsq;
+            if (C == ThreeD) {
+                normalize_recip(x1, y1, z1);
+                normalize_recip(x2, y2, z2);
+                const double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+                dsq = dx * dx + dy * dy + dz * dz;
+            }
+            double d2r, d2i;
+            sphere_direction(x1, y1, z1, x2, y2, z2, dsq, d2r, d2i);
+            expm2ialpha_proj(d2r, d2i, er, ei);
+        }
+        cmul(g2r, g2i, er, ei);
+        const double f = -(D == XK_NG ? double(c1.w) : double(c1.a));
+        atomicAdd(&hist[(Q_XI0 * nb + k) * R], g2r * f);
+        atomicAdd(&hist[(Q_XI1 * nb + k) * R], g2i * f);
     }
 }
 
@@ -787,6 +816,9 @@ static TravFn<IT> pick_traverse(int D, int C) {
 #define TC_TF(DD, CC) return &k_traverse<DD, CC, IT>
     if (D == NData) { if (C == Flat) TC_TF(NData, Flat); if (C == ThreeD) TC_TF(NData, ThreeD); TC_TF(NData, Sphere); }
     if (D == KData) { if (C == Flat) TC_TF(KData, Flat); if (C == ThreeD) TC_TF(KData, ThreeD); TC_TF(KData, Sphere); }
+    if (D == XK_NK) { if (C == Flat) TC_TF(XK_NK, Flat); if (C == ThreeD) TC_TF(XK_NK, ThreeD); TC_TF(XK_NK, Sphere); }
+    if (D == XK_NG) { if (C == Flat) TC_TF(XK_NG, Flat); if (C == ThreeD) TC_TF(XK_NG, ThreeD); TC_TF(XK_NG, Sphere); }
+    if (D == XK_KG) { if (C == Flat) TC_TF(XK_KG, Flat); if (C == ThreeD) TC_TF(XK_KG, ThreeD); TC_TF(XK_KG, Sphere); }
     if (C == Flat) TC_TF(GData, Flat); if (C == ThreeD) TC_TF(GData, ThreeD); TC_TF(GData, Sphere);
 #undef TC_TF
 }
