@@ -42,13 +42,20 @@ def test_version_and_errors_without_gpu():
 
 
 def test_unsupported_corr2_reports_not_implemented():
+    """KN and GN have no reference class (treecorr has NK, NG, KG only); a missing
+    xi array for NG is an invalid argument."""
     from treecorr_amd import _lib
     L = _lib.lib()
     h = ctypes.c_void_p()
     a = np.zeros(5)
-    rc = L.tc_corr2_create(0, 1, 0, 1., 10., 5, math.log(10) / 5, 0.1, 0.1, 0., 0., 0., 0., 0.,
-                           None, None, None, None, *[_lib.dptr(a)] * 4, ctypes.byref(h))
-    assert rc == _lib.TC_ERR_UNSUPPORTED
+    for d1, d2 in ((1, 0), (4, 0), (4, 1)):
+        rc = L.tc_corr2_create(d1, d2, 0, 1., 10., 5, math.log(10) / 5, 0.1, 0.1, 0., 0., 0., 0.,
+                               0., _lib.dptr(a), _lib.dptr(a), None, None, *[_lib.dptr(a)] * 4,
+                               ctypes.byref(h))
+        assert rc == _lib.TC_ERR_UNSUPPORTED, (d1, d2)
+    rc = L.tc_corr2_create(0, 4, 0, 1., 10., 5, math.log(10) / 5, 0.1, 0.1, 0., 0., 0., 0., 0.,
+                           _lib.dptr(a), None, None, None, *[_lib.dptr(a)] * 4, ctypes.byref(h))
+    assert rc == _lib.TC_ERR_INVALID
 
 
 def test_generate_xyz_matches_sincos():
