@@ -199,3 +199,41 @@ def test_gpu_mixed_rejects_auto_and_wrong_sides(gpu):
         _lib.check(L.tc_corr2_process_cross(h, fn.h, fg.h, 0, 0, 1))
     finally:
         L.tc_corr2_destroy(h)
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_finalized_api(gpu):
+    """NG, NK and KG through the public classes: xi is the raw sum over the weight,
+    the shot-noise variance is var/weight (ngcorrelation.py, nkcorrelation.py,
+    kzcorrelation.py finalize), and calculateXi(rg=...) subtracts the random-point
+    correlation and adds its variance."""
+    def sph(cat):
+        return dict(ra=np.arctan2(cat['y'], cat['x']), dec=np.arcsin(cat['z']), ra_units='rad',
+                    dec_units='rad')
+    lc, _ = C.make_catalog(C.SPHERE, 8000, 601, 0.3)
+    sc, _ = C.make_catalog(C.SPHERE, 20000, 602, 0.3)
+    rc, _ = C.make_catalog(C.SPHERE, 16000, 603, 0.0)
+    lens = gpu.Catalog(**sph(lc), w=lc['w'], k=lc['k'])
+    src = gpu.Catalog(**sph(sc), g1=sc['g1'], g2=sc['g2'], k=sc['k'], w=sc['w'])
+    rand = gpu.Catalog(**sph(rc))
+    kw = dict(nbins=10, min_sep=1., max_sep=100., sep_units='arcmin', bin_slop=0.1)
+    ng = gpu.NGCorrelation(**kw)
+    ng.process(lens, src)
+    assert np.all(ng.npairs > 0) and np.all(np.isfinite(ng.xi)) and np.all(np.isfinite(ng.xi_im))
+    np.testing.assert_allclose(ng.varxi, src.varg / ng.weight, rtol=1e-12)
+    rg = gpu.NGCorrelation(**kw)
+    rg.process(rand, src)
+    raw = ng.xi.copy()
+    xi, xi_im, varxi = ng.calculateXi(rg=rg)
+    np.testing.assert_allclose(xi, raw - rg.xi, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(varxi, ng.raw_varxi + rg.varxi, rtol=1e-12)
+    np.testing.assert_array_equal(ng.raw_xi, raw)
+
+    nk = gpu.NKCorrelation(**kw)
+    nk.process(lens, src)
+    np.testing.assert_allclose(nk.varxi, src.vark / nk.weight, rtol=1e-12)
+    kg = gpu.KGCorrelation(**kw)
+    kg.process(lens, src)
+    np.testing.assert_allclose(kg.varxi, lens.vark * src.varg / kg.weight, rtol=1e-12)
+    with pytest.raises(TypeError):
+        kg.process(lens)

@@ -310,3 +310,43 @@ def test_gpu_quick_assign_matches_oracle(gpu, oracle_lib, dim):
     ref = O.quick_assign(cen, cat['x'], cat['y'], cat['z'] if dim == 3 else None)
     np.testing.assert_array_equal(c.patch, ref)
     assert c.npatch == npatch and not np.any(c.patch == 700)
+
+
+@pytest.mark.gpu
+def test_gpu_jackknife_ng_rows_are_catalogs_without_patch(gpu, oracle_lib):
+    """NG with patches (every patch pair in one traversal, the N fields on one side and
+    the G fields on the other): with bin_slop = angle_slop = 0, jackknife row k is
+    the oracle's NG of the lens and source catalogs without patch k."""
+    O = oracle_lib
+    ngrid, min_top = 3, 3
+    lc, scale = C.make_catalog(THREED, 2000, 351, 0.2)
+    sc, _ = C.make_catalog(THREED, 3000, 352, 0.2)
+    ll = grid_labels(lc['x'], lc['y'], scale, ngrid)
+    sl = grid_labels(sc['x'], sc['y'], scale, ngrid)
+    bn = C.binning(scale, 8, 0.02, 0.3, 0.0, 0.0)
+    ng = gpu.NGCorrelation(nbins=bn['nbins'], min_sep=bn['min_sep'], max_sep=bn['max_sep'],
+                           bin_slop=0., angle_slop=0., min_top=min_top, var_method='jackknife')
+    lens = {k: lc[k] for k in ('x', 'y', 'z', 'w')}
+    ng.process(front_catalog(gpu, lens, THREED, ll), front_catalog(gpu, sc, THREED, sl))
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=min_top)
+
+    def raw(c1, c2):
+        f1 = O.Field(0, THREED, c1['x'], c1['y'], c1['z'], w=c1['w'], **kw)
+        f2 = O.Field(4, THREED, c2['x'], c2['y'], c2['z'], g1=c2['g1'], g2=c2['g2'], w=c2['w'],
+                     **kw)
+        return O.corr2(0, THREED, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'],
+                       bn['bin_size'], bn['b'], bn['a'], exact=True, d2=4)
+    full = raw(lc, sc)
+    np.testing.assert_array_equal(ng.npairs, full['npairs'])
+    assert normwise(ng.raw_xi * ng.weight, full['xi0']) < 1e-11
+    npatch = ngrid * ngrid
+    rows = []
+    for k in range(npatch):
+        r = raw(sub(lc, ll != k), sub(sc, sl != k))
+        rows.append(np.where(r['weight'] != 0, r['xi0'] / np.where(r['weight'] != 0,
+                                                                   r['weight'], 1.), 0.))
+    rows = np.array(rows)
+    v, w = ng.build_cov_design_matrix('jackknife', cross_patch_weight='simple')
+    assert v.shape == rows.shape
+    for k in range(npatch):
+        assert normwise(v[k], rows[k]) < 1e-11, k

@@ -6,7 +6,8 @@ kernels behind a C ABI (include/treecorr_amd.h).
 """
 from .catalog import Catalog, calculateVarG, calculateVarK
 from .field import Field, NField, KField, GField
-from .corr2 import Corr2, NNCorrelation, KKCorrelation, GGCorrelation
+from .corr2 import (Corr2, NNCorrelation, KKCorrelation, GGCorrelation, NKCorrelation,
+                    NGCorrelation, KGCorrelation)
 from .util import set_omp_threads, get_omp_threads
 
 from .corr3 import Corr3, NNNCorrelation, GGGCorrelation

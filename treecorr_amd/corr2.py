@@ -68,6 +68,7 @@ class Corr2:
     _default_angle_slop = 0.1
     _letters = None
     _d = None
+    _d2 = None       # the second field's type where it differs (NK, NG, KG)
     _nxi = 0
 
     def __init__(self, config=None, *, logger=None, rng=None, **kwargs):
@@ -174,7 +175,7 @@ class Corr2:
             h = ctypes.c_void_p()
             xi = [_lib.dptr(a) if a is not None else None for a in self._xi]
             _lib.check(_lib.lib().tc_corr2_create(
-                self._d, self._d, LOG, self._min_sep, self._max_sep, self.nbins, self._bin_size,
+                self._d, self._d if self._d2 is None else self._d2, LOG, self._min_sep, self._max_sep, self.nbins, self._bin_size,
                 self.b, self.angle_slop, self.min_rpar, self.max_rpar, 0., 0., 0., *xi,
                 _lib.dptr(self.meanr), _lib.dptr(self.meanlogr), _lib.dptr(self.weight),
                 _lib.dptr(self.npairs), ctypes.byref(h)))
@@ -219,12 +220,14 @@ class Corr2:
         self._varxi = None
         self._cov = None
 
-    def _field(self, cat, **kw):
-        getter = getattr(cat, 'get%sField' % self._letters[0])
+    def _field(self, cat, side=0, **kw):
+        getter = getattr(cat, 'get%sField' % self._letters[side])
         return getter(**kw)
 
     def process_auto(self, cat, *, metric=None, num_threads=None, corr_only=False):
         """Accumulate the auto-correlation of one catalog (corr2base.py:1070-1111)."""
+        if self._d2 is not None:
+            raise TypeError("%s is a cross correlation: use process_cross" % type(self).__name__)
         self._set_metric(metric, cat.coords)
         if num_threads is not None:
             set_omp_threads(num_threads)
@@ -250,7 +253,7 @@ class Corr2:
                          split_method=self.split_method,
                          brute=self.brute is True or self.brute == 1,
                          min_top=self.min_top, max_top=self.max_top, coords=self.coords)
-        f2 = self._field(cat2, min_size=min_size, max_size=max_size,
+        f2 = self._field(cat2, 1, min_size=min_size, max_size=max_size,
                          split_method=self.split_method,
                          brute=self.brute is True or self.brute == 2,
                          min_top=self.min_top, max_top=self.max_top, coords=self.coords)
@@ -281,6 +284,8 @@ class Corr2:
             raise ValueError("Invalid patch_method %s" % patch_method)
         if patch_method == 'local':
             raise NotImplementedError("patch_method='local' is not implemented")
+        if cat2 is None and self._d2 is not None:
+            raise TypeError("%s.process needs cat2" % type(self).__name__)
         if initialize:
             self.clear()
             self._processed_cats1 = []
@@ -309,15 +314,22 @@ class Corr2:
             cats2 = self._processed_cats1 if auto else self._processed_cats2
             w2 = np.sum([c.sumw for c in cats2]) / np.sum([c.nobj for c in cats2])
             self._meanww = w1 * w2
+        # corr2base.py:1236-1271
         v1 = self._calculate_var(self._processed_cats1)
-        v2 = v1 if auto else self._calculate_var(self._processed_cats2)
+        v2 = v1 if auto else self._calculate_var2(self._processed_cats2)
         if v1 is None:
-            self.finalize()
+            if v2 is None:
+                self.finalize()
+            else:
+                self.finalize(v2)
         else:
             self.finalize(v1, v2)
 
     def _calculate_var(self, cats):
         return None
+
+    def _calculate_var2(self, cats):
+        return self._calculate_var(cats)
 
     # ---- patch pairs (corr2base.py:836-1050), all in one device traversal ----
     def _pair_tot(self, c1, c2):
@@ -331,19 +343,20 @@ class Corr2:
         # corr2base.py:772-774: a no-op except for NN
         pass
 
-    def _patch_fields(self, cats, brute):
+    def _patch_fields(self, cats, brute, side=0):
         min_size, max_size = self._get_minmax_size()
         kw = dict(min_size=min_size, max_size=max_size, split_method=self.split_method,
                   brute=brute, min_top=self.min_top, max_top=self.max_top, coords=self.coords)
+        letter = self._letters[side]
         # fields already built for these parameters are reused
-        key = (self._letters[0], tuple(sorted(kw.items())))
-        cached = [c._fields.get(self._letters[0]) for c in cats]
+        key = (letter, tuple(sorted(kw.items())))
+        cached = [c._fields.get(letter) for c in cats]
         if all(f is not None and f[0] == key for f in cached):
             return [f[1] for f in cached]
-        fields = build_patch_fields(self._letters[0], cats, **kw)
+        fields = build_patch_fields(letter, cats, **kw)
         if fields is not None:
             return fields
-        return [self._field(c, **kw) for c in cats]
+        return [self._field(c, side, **kw) for c in cats]
 
     def _run_pairs(self, fields1, fields2, jobs, corr_only):
         """Raw sums of every job (i, j) over fields1[i] x fields2[j] (fields2 None: the
@@ -440,7 +453,7 @@ class Corr2:
                  for ii, c1 in enumerate(cat1) for jj, c2 in enumerate(cat2)]
         jobs = [(a, b) for a, b, run in order if run]
         f1 = self._patch_fields(cat1, self.brute is True or self.brute == 1)
-        f2 = self._patch_fields(cat2, self.brute is True or self.brute == 2)
+        f2 = self._patch_fields(cat2, self.brute is True or self.brute == 2, 1)
         out = self._run_pairs(f1, f2, jobs, corr_only)
         rows = {job: out[k] for k, job in enumerate(jobs)}
         for ii, jj, run in order:
@@ -889,3 +902,161 @@ class GGCorrelation(Corr2):
         if self._varxi is None:
             self._calculate_varxi(2)
         return self._varxi[1]
+
+
+class _MixedCorrelation(Corr2):
+    """Count or scalar around points: NK, NG, KG (cross only).  xi (and xi_im for a
+    shear second field); NK and NG can subtract the same correlation around random
+    points (treecorr/nkcorrelation.py, nzcorrelation.py BaseNZCorrelation)."""
+    _default_angle_slop = 0.1
+
+    def __init__(self, config=None, *, logger=None, **kwargs):
+        super().__init__(config, logger=logger, **kwargs)
+        self._rz = None
+        self._comp_varxi = None
+        self.xi = self.raw_xi
+        if self._nxi > 1:
+            self.xi_im = self.raw_xi_im
+
+    raw_xi = property(lambda self: self._xi[0])
+    raw_xi_im = property(lambda self: self._xi[1])
+
+    def _use_raw(self):
+        self.xi = self.raw_xi
+        if self._nxi > 1:
+            self.xi_im = self.raw_xi_im
+
+    def copy(self):
+        ret = super().copy()
+        if self.xi is self.raw_xi:
+            ret._use_raw()
+        if self._rz is not None:
+            ret._rz = self._rz.copy()
+        return ret
+
+    @property
+    def raw_varxi(self):
+        if self._varxi is None:
+            self._calculate_varxi(1)
+        return self._varxi[0]
+
+    @property
+    def varxi(self):
+        return self.raw_varxi if self._comp_varxi is None else self._comp_varxi
+
+    def _clear(self):
+        super()._clear()
+        self._use_raw()
+        self._rz = None
+        self._comp_varxi = None
+
+    def _sum(self, others, corr_only):
+        super()._sum(others, corr_only)
+        self._use_raw()
+
+    def _compensate(self, rz):
+        # nkcorrelation.py:153-214, nzcorrelation.py:98-133: xi = D? - R?
+        if rz is None:
+            self._use_raw()
+            self._comp_varxi = None
+            return
+        self.xi = self.raw_xi - rz.xi
+        if self._nxi > 1:
+            self.xi_im = self.raw_xi_im - rz.xi_im
+        self._rz = rz
+        if rz.npatch1 not in (1, self.npatch1) or rz.npatch2 != self.npatch2:
+            raise RuntimeError("R%s must be run with the same patches as D%s"
+                               % (self._letters[1], self._letters[1]))
+        if len(self.results) > 0:
+            # patch pairs of rz missing here get zero entries so that every pair of
+            # the covariance rows is computed
+            for ij in rz.results:
+                if ij in self.results:
+                    continue
+                self.results[ij] = _PairSums(self.nbins, self._nxi, None)
+            self._cov = self.estimate_cov(self.var_method)
+            self._comp_varxi = np.zeros_like(self.rnom, dtype=float)
+            self._comp_varxi.ravel()[:] = self.cov_diag
+        else:
+            self._comp_varxi = self.raw_varxi + rz.varxi
+
+    def _calculate_xi_from_pairs(self, pairs, corr_only):
+        super()._calculate_xi_from_pairs(pairs, corr_only)
+        if self._rz is not None:
+            # an rz with npatch1 = 1 takes the pairs of patch j with itself
+            if self._rz.npatch1 == 1 and not all([p[0] == 0 for p in pairs]):
+                pairs = [(0, j, w) for i, j, w in pairs if i == j]
+            pairs = self._rz._keep_ok(pairs)
+            self._rz._calculate_xi_from_pairs(pairs, corr_only=True)
+            self.xi -= self._rz.xi
+
+
+class NKCorrelation(_MixedCorrelation):
+    """Count-scalar correlation, <kappa> around points (treecorr/nkcorrelation.py)."""
+    _letters = 'NK'
+    _d = 0
+    _d2 = 1
+    _nxi = 1
+    _default_angle_slop = 1
+
+    def _calculate_var(self, cats):
+        return None
+
+    def _calculate_var2(self, cats):
+        return calculateVarK(cats)
+
+    def finalize(self, vark):
+        # nkcorrelation.py:114-126
+        self._finalize()
+        self._var_num = vark
+        self._use_raw()
+
+    def calculateXi(self, *, rk=None):
+        """xi, or the compensated DK - RK given rk (nkcorrelation.py:153-214)."""
+        self._compensate(rk)
+        return self.xi, self.varxi
+
+
+class NGCorrelation(_MixedCorrelation):
+    """Count-shear correlation, tangential shear around points (treecorr/ngcorrelation.py)."""
+    _letters = 'NG'
+    _d = 0
+    _d2 = 4
+    _nxi = 2
+
+    def _calculate_var(self, cats):
+        return None
+
+    def _calculate_var2(self, cats):
+        return calculateVarG(cats)
+
+    def finalize(self, varg):
+        # nzcorrelation.py:236-247
+        self._finalize()
+        self._var_num = varg
+        self._use_raw()
+
+    def calculateXi(self, *, rg=None):
+        """xi, xi_im, or the compensated DG - RG given rg (ngcorrelation.py:100-135)."""
+        self._compensate(rg)
+        return self.xi, self.xi_im, self.varxi
+
+
+class KGCorrelation(_MixedCorrelation):
+    """Scalar-shear correlation (treecorr/kgcorrelation.py, kzcorrelation.py)."""
+    _letters = 'KG'
+    _d = 1
+    _d2 = 4
+    _nxi = 2
+
+    def _calculate_var(self, cats):
+        return calculateVarK(cats)
+
+    def _calculate_var2(self, cats):
+        return calculateVarG(cats)
+
+    def finalize(self, vark, varg):
+        # kzcorrelation.py:148-160 -> :51-53
+        self._finalize()
+        self._var_num = vark * varg
+        self._use_raw()
