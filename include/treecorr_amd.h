@@ -127,6 +127,18 @@ int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
  * wins a tie, as in the reference.  patch receives int32 labels. */
 int tc_quick_assign(const double* centers, int npatch, const double* x, const double* y,
                     const double* z, long long n, int* patch);
+/* Lloyd iterations of k-means on the GPU: KMeansRun (src/KMeans.cpp:773-833, bound
+ * at src/Field.cpp:633) with alt = false, which treecorr/field.py:552-560
+ * kmeans_refine_centers calls.  centers (npatch rows of dim = 2 or 3 values) is refined
+ * in place over the n objects (x, y[, z], weights w): each object joins its nearest
+ * centre, centres move to their objects' w-weighted means (normalised to the unit
+ * sphere when sphere != 0; an empty patch's centre becomes the origin, as in the
+ * reference), until the rms shift is below tol * size or max_iter steps ran; niter
+ * receives the steps.  The reference iterates over tree cells, whose float weights
+ * and summation order make its centres differ from these at the ~1e-7 level. */
+int tc_kmeans_run(double* centers, int npatch, int dim, const double* x, const double* y,
+                  const double* z, const double* w, long long n, int max_iter, double tol,
+                  double size, int sphere, int* niter);
 /* Splits a catalog's columns by patch: a stable counting sort on host threads.
  * Replaces the per-patch np.where + fancy indexing of Catalog.get_patches
  * (treecorr/catalog.py:2966-3050).  out[c] receives cols[c] ordered by patch

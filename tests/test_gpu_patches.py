@@ -350,3 +350,62 @@ def test_gpu_jackknife_ng_rows_are_catalogs_without_patch(gpu, oracle_lib):
     assert v.shape == rows.shape
     for k in range(npatch):
         assert normwise(v[k], rows[k]) < 1e-11, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('coords,npatch', [(3, 20), (1, 12), (2, 16), (3, 90)])
+def test_gpu_kmeans_matches_compiled_reference(gpu, coords, npatch):
+    """Field.run_kmeans's steps against the compiled reference from the same initial
+    centres (which are the reference's own for the same rng, see test_kmeans_init.py):
+    the reference iterates over tree cells with float weights, the GPU over the objects,
+    so the converged centres agree to ~1e-7 of the field size and the labels almost
+    everywhere."""
+    from oracle import ref as R
+    if R.module() is None:
+        pytest.skip('oracle/_ref not built')
+    cat, scale = C.make_catalog(coords, 20000, 700 + npatch, 0.3)
+    if coords == 3:
+        tcat = gpu.Catalog(ra=np.arctan2(cat['y'], cat['x']), dec=np.arcsin(cat['z']),
+                           ra_units='rad', dec_units='rad', w=cat['w'])
+    else:
+        kw = dict(x=cat['x'], y=cat['y'], w=cat['w'])
+        if coords == 2:
+            kw['z'] = cat['z']
+        tcat = gpu.Catalog(**kw)
+    max_top = int.bit_length(npatch) - 1
+    fld = tcat.getNField(max_top=max_top)
+    c0 = fld.kmeans_initialize_centers(npatch, rng=np.random.default_rng(5))
+    ours = c0.copy()
+    fld.kmeans_refine_centers(ours)
+    with pytest.raises(NotImplementedError):
+        fld.kmeans_refine_centers(c0.copy(), alt=True)
+    labels = fld.kmeans_assign_patches(ours)
+
+    zc = None if coords == 1 else tcat.z
+    rf = R.make_field(0, coords, tcat.x, tcat.y, zc, w=tcat.w, min_size=0., max_size=np.inf,
+                      min_top=0, max_top=max_top)
+    ref = c0.copy()
+    rf.KMeansRun(ref, npatch, 200, 1.e-5, False)
+    rlab = np.zeros(tcat.ntot, dtype=int)
+    rf.KMeansAssign(ref, npatch, rlab)
+    assert np.max(np.abs(ours - ref)) < 1e-5 * fld.size
+    assert np.mean(labels == rlab) > 0.995
+    assert len(np.unique(labels)) == npatch
+
+
+@pytest.mark.gpu
+def test_gpu_catalog_npatch_uses_kmeans(gpu):
+    """Catalog(npatch=N): k-means patches (catalog.py:1473-1484); every label is the
+    nearest centre and every patch holds objects."""
+    cat, scale = C.make_catalog(3, 30000, 808, 0.3)
+    tcat = gpu.Catalog(ra=np.arctan2(cat['y'], cat['x']), dec=np.arcsin(cat['z']),
+                       ra_units='rad', dec_units='rad', w=cat['w'], npatch=25,
+                       rng=np.random.default_rng(3))
+    assert tcat.npatch == 25
+    p = np.asarray(tcat.patch)
+    assert np.array_equal(np.unique(p), np.arange(25))
+    cen = tcat.patch_centers
+    pos = np.column_stack([tcat.x, tcat.y, tcat.z])
+    d = ((pos[:, None, :] - cen[None, :, :]) ** 2).sum(axis=2)
+    assert np.mean(np.argmin(d, axis=1) == p) > 0.999
+    assert len(tcat.get_patches()) == 25

@@ -102,8 +102,23 @@ class Catalog:
                 self._select(np.where(wpos != 0)[0])
 
         self._fields = {}
+        npatch = cfg.get('npatch')
         if self._centers is not None:
             self._assign_patches()
+        elif self._patch is None and npatch is not None and int(npatch) != 1:
+            # catalog.py:1473-1484: patches from k-means on an N field whose top level
+            # has about npatch cells
+            npatch = int(npatch)
+            if npatch > self.ntot:
+                raise ValueError("npatch is larger than the number of objects in catalog")
+            max_top = int.bit_length(npatch) - 1
+            c = 'spherical' if self._ra is not None else self.coords
+            fld = self.getNField(max_top=max_top, coords=c)
+            self._patch, self._centers = fld.run_kmeans(
+                npatch, init=cfg.get('kmeans_init', 'tree'), alt=bool(cfg.get('kmeans_alt', False)),
+                rng=cfg.get('rng'))
+            self._npatch = npatch
+            self._fields = {}
         else:
             self._npatch = 1 if self._patch is None else int(self._patch.max()) + 1
 

@@ -4,6 +4,7 @@
 #include "tc_engine.h"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 namespace tc {
@@ -83,6 +84,144 @@ int quick_assign(const double* centers, int npatch, const double* x, const doubl
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     release();
     if (e != hipSuccess) return hip_fail(e, "assigning patches");
+    return 0;
+}
+
+// One Lloyd step of k-means over the objects (KMeansRun, src/KMeans.cpp:773-833, with
+// the per-object assignment its FindCellsInPatches ends in): each object joins its
+// nearest centre (the lowest index on ties) and adds w*x, w*y, w*z and w to that
+// centre's sums.  Workgroups sum into LDS first when npatch <= KM_LDS_MAX, then add
+// to sums.
+constexpr int KM_LDS_MAX = 768;
+constexpr int KM_NS = 4;
+
+template <int DIM>
+__global__ __launch_bounds__(QA_THREADS) void k_kmeans_step(const double* __restrict__ centers,
+                                                            int npatch,
+                                                            const double* __restrict__ x,
+                                                            const double* __restrict__ y,
+                                                            const double* __restrict__ z,
+                                                            const double* __restrict__ w,
+                                                            long long n, double* __restrict__ sums) {
+    __shared__ double sc[QA_CHUNK * DIM];
+    __shared__ double acc[KM_LDS_MAX * KM_NS];
+    const bool lds = npatch <= KM_LDS_MAX;
+    if (lds)
+        for (int t = threadIdx.x; t < npatch * KM_NS; t += QA_THREADS) acc[t] = 0.;
+    const long long i = (long long)blockIdx.x * QA_THREADS + threadIdx.x;
+    const bool live = i < n;
+    const double px = live ? x[i] : 0., py = live ? y[i] : 0.;
+    const double pz = (DIM == 3 && live) ? z[i] : 0.;
+    int kmin = 0;
+    double best = 0.;
+    for (int c0 = 0; c0 < npatch; c0 += QA_CHUNK) {
+        const int nc = npatch - c0 < QA_CHUNK ? npatch - c0 : QA_CHUNK;
+        __syncthreads();
+        for (int t = threadIdx.x; t < nc * DIM; t += QA_THREADS) sc[t] = centers[(size_t)c0 * DIM + t];
+        __syncthreads();
+        for (int k = 0; k < nc; ++k) {
+            const double dx = px - sc[k * DIM], dy = py - sc[k * DIM + 1];
+            double dsq = dx * dx + dy * dy;
+            if (DIM == 3) {
+                const double dz = pz - sc[k * DIM + 2];
+                dsq = dsq + dz * dz;
+            }
+            if (c0 + k == 0 || dsq < best) { kmin = c0 + k; best = dsq; }
+        }
+    }
+    if (live) {
+        const double wi = w[i];
+        const double v[KM_NS] = {wi * px, wi * py, wi * pz, wi};
+        double* dst = lds ? acc : sums;
+#pragma unroll
+        for (int q = 0; q < KM_NS; ++q) atomicAdd(&dst[(size_t)kmin * KM_NS + q], v[q]);
+    }
+    if (lds) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < npatch * KM_NS; t += QA_THREADS)
+            if (acc[t] != 0.) atomicAdd(&sums[t], acc[t]);
+    }
+}
+
+int kmeans_run(double* centers, int npatch, int dim, const double* x, const double* y,
+               const double* z, const double* w, long long n, int max_iter, double tol,
+               double size, int sphere, int* niter) {
+    hipStream_t st = stream();
+    const size_t pb = sizeof(double) * (size_t)n;
+    const size_t cb = sizeof(double) * (size_t)npatch * dim;
+    const size_t sb = sizeof(double) * (size_t)npatch * KM_NS;
+    std::vector<std::pair<void*, size_t>> bufs;
+    auto alloc = [&](size_t bytes) {
+        void* p = dev_alloc(bytes);
+        if (p) bufs.push_back({p, bytes});
+        return p;
+    };
+    auto release = [&]() { for (auto& b : bufs) dev_free(b.first, b.second); };
+    double* dx = static_cast<double*>(alloc(pb));
+    double* dy = static_cast<double*>(alloc(pb));
+    double* dz = dim == 3 ? static_cast<double*>(alloc(pb)) : nullptr;
+    double* dw = static_cast<double*>(alloc(pb));
+    double* dc = static_cast<double*>(alloc(cb));
+    double* ds = static_cast<double*>(alloc(sb));
+    if (!dx || !dy || (dim == 3 && !dz) || !dw || !dc || !ds) {
+        release();
+        return fail("out of device memory for k-means");
+    }
+    std::vector<double> hs((size_t)npatch * KM_NS), hc(centers, centers + (size_t)npatch * dim);
+    std::vector<double> nc((size_t)npatch * dim);
+    const dim3 grid((unsigned)((n + QA_THREADS - 1) / QA_THREADS));
+    // one step: the sums of the assignment under the current centres
+    auto step = [&]() -> hipError_t {
+        hipError_t e = hipMemcpyAsync(dc, hc.data(), cb, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemsetAsync(ds, 0, sb, st);
+        if (e == hipSuccess) {
+            if (dim == 3) hipLaunchKernelGGL(k_kmeans_step<3>, grid, dim3(QA_THREADS), 0, st, dc, npatch, dx, dy, dz, dw, n, ds);
+            else hipLaunchKernelGGL(k_kmeans_step<2>, grid, dim3(QA_THREADS), 0, st, dc, npatch, dx, dy, dz, dw, n, ds);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(hs.data(), ds, sb, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        return e;
+    };
+    hipError_t e = hipMemcpyAsync(dx, x, pb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy, y, pb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && dim == 3) e = hipMemcpyAsync(dz, z, pb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dw, w, pb, hipMemcpyHostToDevice, st);
+    // rms shift / size < tol  <=>  total shift^2 < tol^2 size^2 npatch
+    const double tolsq = (tol * size) * (tol * size) * npatch;
+    int it = 0;
+    for (; e == hipSuccess && it < max_iter; ++it) {
+        e = step();
+        if (e != hipSuccess) break;
+        // UpdateCenters::finalize (src/KMeans.cpp:362-372): an empty patch's centre
+        // becomes the origin, as the reference's reset value
+        double shiftsq = 0.;
+        for (int k = 0; k < npatch; ++k) {
+            const double* sk = &hs[(size_t)k * KM_NS];
+            double c[3] = {0., 0., 0.};
+            if (sk[3] > 0.) {
+                for (int q = 0; q < dim; ++q) c[q] = sk[q] / sk[3];
+                if (sphere) {
+                    const double nrm = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                    if (nrm == 0.) { c[0] = 1.; c[1] = c[2] = 0.; }
+                    else for (int q = 0; q < 3; ++q) c[q] /= nrm;
+                }
+            }
+            double d = 0.;
+            for (int q = 0; q < dim; ++q) {
+                const double t = hc[(size_t)k * dim + q] - c[q];
+                d += t * t;
+                nc[(size_t)k * dim + q] = c[q];
+            }
+            shiftsq += d;
+        }
+        hc = nc;
+        if (shiftsq < tolsq) { ++it; break; }
+    }
+    release();
+    if (e != hipSuccess) return hip_fail(e, "running k-means");
+    std::copy(hc.begin(), hc.end(), centers);
+    if (niter) *niter = it;
     return 0;
 }
 

@@ -48,6 +48,7 @@ class Field:
         self.max_top = max_top
         self.coords = coords if coords is not None else cat.coords
         self._coords = coord_enum(self.coords)
+        self._cat = cat
         if _handle is not None:     # built with the other patches, see build_patch_fields
             self.data = _handle
             return
@@ -109,6 +110,66 @@ class Field:
         _lib.check(_lib.lib().tc_field_copy_nodes(self.data, nodes.ctypes.data_as(ctypes.c_void_p),
                                                   top.ctypes.data_as(ctypes.c_void_p)))
         return nodes, top
+
+
+    # ---- k-means patches (treecorr/field.py:286-590) ----
+    def run_kmeans(self, npatch, *, max_iter=200, tol=1.e-5, init='tree', alt=False, rng=None):
+        """Patch labels and centres from k-means (field.py:286-404)."""
+        centers = self.kmeans_initialize_centers(npatch, init=init, rng=rng)
+        self.kmeans_refine_centers(centers, max_iter=max_iter, tol=tol, alt=alt)
+        patches = self.kmeans_assign_patches(centers)
+        return patches, centers
+
+    def kmeans_initialize_centers(self, npatch, init='tree', *, rng=None):
+        """Initial centres from the tree or at random objects (field.py:406-452); the
+        same centres as the reference for the same rng."""
+        from . import kmeans
+        if npatch > self.ntot:
+            raise ValueError("Invalid npatch.  Cannot be greater than self.ntot.")
+        if npatch < 1:
+            raise ValueError("Invalid npatch.  Cannot be less than 1.")
+        seed = 0 if rng is None else int(rng.random() * 2**63)
+        T = kmeans.TreeView.from_device_nodes(*self.nodes())
+        if init == 'tree':
+            centers = kmeans.init_tree(T, int(npatch), seed)
+        elif init == 'random':
+            centers = kmeans.init_random(T, int(npatch), seed)
+        elif init == 'kmeans++':
+            raise NotImplementedError("init='kmeans++' is not implemented by treecorr_amd")
+        else:
+            raise ValueError("Invalid init: %s. " % init +
+                             "Must be one of 'tree', 'random', or 'kmeans++.'")
+        return centers[:, :2].copy() if self._coords == FLAT else centers
+
+    def _kmeans_objects(self):
+        # the tree's leaves: positions as the tree holds them (unit vectors on the
+        # sphere), weights as the cells hold them (float)
+        cat = self._cat
+        x = np.ascontiguousarray(cat.x, dtype=float)
+        y = np.ascontiguousarray(cat.y, dtype=float)
+        z = None if self._coords == FLAT else np.ascontiguousarray(cat.z, dtype=float)
+        if self._coords == SPHERE:
+            r = np.sqrt(x * x + y * y + z * z)
+            x, y, z = x / r, y / r, z / r
+        w = np.ascontiguousarray(np.asarray(cat.w, dtype=np.float32), dtype=float)
+        return x, y, z, w
+
+    def kmeans_refine_centers(self, centers, *, max_iter=200, tol=1.e-5, alt=False):
+        """Lloyd iterations on the GPU (field.py:454-560); centers is updated in place."""
+        from . import kmeans
+        if alt:
+            # the alternate algorithm penalises by an inertia the reference estimates per
+            # tree cell (0.75 w s^2, src/KMeans.cpp:396-407); not restated here
+            raise NotImplementedError("alt=True k-means is not implemented by treecorr_amd")
+        x, y, z, w = self._kmeans_objects()
+        return kmeans.refine(centers, x, y, z, w, self.size, self.coords, max_iter=max_iter,
+                             tol=tol)
+
+    def kmeans_assign_patches(self, centers):
+        """Each object's patch: its nearest centre (field.py:562-590)."""
+        from . import kmeans
+        x, y, z, _ = self._kmeans_objects()
+        return kmeans.assign(centers, x, y, z).astype(int)
 
 
 class NField(Field):

@@ -1,0 +1,185 @@
+"""k-means patches: Field.run_kmeans and its three steps (treecorr/field.py:286-530).
+
+The initial centres come from the field's tree on the host, restating the
+reference's own choices (src/KMeans.cpp InitializeCentersTree :28-109,
+InitializeCentersRand :111-157, SelectRandomFrom src/Corr2.cpp:622-662), with the
+same random stream: glibc's rand() seeded by srand(seed) (urand, src/Cell.cpp:59-82),
+so a seeded run starts from the reference's centres exactly.  The Lloyd iterations
+run on the GPU over the objects (tc_kmeans_run); the final labels are the nearest
+centres (tc_quick_assign).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+
+_LIBC = None
+_SEEDED = False
+RAND_MAX = 2147483647        # glibc
+
+
+def _libc():
+    global _LIBC
+    if _LIBC is None:
+        _LIBC = ctypes.CDLL(None)
+        _LIBC.rand.restype = ctypes.c_int
+        _LIBC.srand.argtypes = [ctypes.c_uint]
+    return _LIBC
+
+
+def urand(seed=0):
+    """urand (src/Cell.cpp:59-82): srand(seed) when seed != 0, else seeded from the
+    system's entropy once; then rand()/RAND_MAX."""
+    global _SEEDED
+    L = _libc()
+    if seed != 0:
+        L.srand(ctypes.c_uint(seed & 0xffffffff).value)
+        _SEEDED = True
+    elif not _SEEDED:
+        L.srand(int.from_bytes(os.urandom(4), 'little'))
+        _SEEDED = True
+    return L.rand() / RAND_MAX
+
+
+def select_random_from(m, n):
+    """n distinct integers from [0, m) (SelectRandomFrom, src/Corr2.cpp:622-662)."""
+    sel = [0] * n
+    if m > 3 * n:
+        # Floyd's algorithm: each new value goes to the next slot
+        chosen = set()
+        while len(chosen) < n:
+            j = int(urand() * m)
+            if j == m:
+                j = m - 1
+            if j not in chosen:
+                chosen.add(j)
+                sel[len(chosen) - 1] = j
+    else:
+        # Fisher-Yates through the first n
+        full = list(range(m))
+        for i in range(n):
+            j = int(urand() * (m - i)) + i
+            if j == m:
+                j = m - 1
+            full[i], full[j] = full[j], full[i]
+        sel = full[:n]
+    return sel
+
+
+class TreeView:
+    """A tree as arrays: pos (nnode x 3), n, left, right (-1 at leaves), top (the
+    top-level cells in TreeCorr's order)."""
+
+    def __init__(self, pos, n, left, right, top):
+        self.pos, self.n, self.left, self.right, self.top = pos, n, left, right, top
+
+    @classmethod
+    def from_device_nodes(cls, nodes, top):
+        """From the device layout (two children adjacent at child, child + 1)."""
+        pos = np.column_stack([nodes['x'], nodes['y'], nodes['z']])
+        left = np.asarray(nodes['child'], dtype=np.int64)
+        right = np.where(left >= 0, left + 1, -1)
+        return cls(pos, np.asarray(nodes['n'], dtype=np.int64), left, right,
+                   np.asarray(top, dtype=np.int64))
+
+
+def _init_tree_rec(T, centers, cell, first, ncenters):
+    # InitializeCentersTree (recursive), src/KMeans.cpp:28-59
+    if ncenters == 1:
+        centers[first] = T.pos[cell]
+    elif T.left[cell] >= 0:
+        m1 = ncenters // 2
+        m2 = ncenters - m1
+        if urand() < 0.5:
+            m1, m2 = m2, m1
+        _init_tree_rec(T, centers, T.left[cell], first, m1)
+        _init_tree_rec(T, centers, T.right[cell], first + m1, m2)
+    else:
+        for i in range(ncenters):
+            centers[first + i] = T.pos[cell] * (1. + urand() * 1.e-8)
+
+
+def init_tree(T, npatch, seed):
+    """Initial centres from the tree (InitializeCentersTree, src/KMeans.cpp:61-109).
+    Returns an (npatch, 3) array."""
+    centers = np.zeros((npatch, 3))
+    ncells = len(T.top)
+    urand(seed)
+    if ncells > npatch:
+        sel = select_random_from(ncells, npatch)
+        for i in range(npatch):
+            centers[i] = T.pos[T.top[sel[i]]]
+    else:
+        n1 = npatch // ncells
+        k2 = npatch % ncells
+        k1 = ncells - k2
+        nvalues = [n1] * k1 + [n1 + 1] * k2
+        for k in range(ncells, 1, -1):
+            j = int(urand() * k)
+            if j != k - 1:
+                nvalues[j], nvalues[k - 1] = nvalues[k - 1], nvalues[j]
+        first = 0
+        for k in range(ncells):
+            _init_tree_rec(T, centers, T.top[k], first, nvalues[k])
+            first += nvalues[k]
+    return centers
+
+
+def _leaf_number(T, cell, m):
+    # BaseCell::getLeafNumber: descend by the counts of the left child
+    while T.left[cell] >= 0:
+        nl = T.n[T.left[cell]]
+        if m < nl:
+            cell = T.left[cell]
+        else:
+            m -= nl
+            cell = T.right[cell]
+    return cell
+
+
+def init_random(T, npatch, seed):
+    """Initial centres at random objects (InitializeCentersRand, src/KMeans.cpp:111-157)."""
+    centers = np.zeros((npatch, 3))
+    ntot = int(sum(T.n[c] for c in T.top))
+    urand(seed)
+    index = select_random_from(ntot, npatch)
+    for i in range(npatch):
+        m = index[i]
+        for c in T.top:
+            if m < T.n[c]:
+                centers[i] = T.pos[_leaf_number(T, c, m)]
+                break
+            m -= T.n[c]
+        for i1 in range(i):
+            if np.array_equal(centers[i1], centers[i]):
+                centers[i] = centers[i] * (1. + urand() * 1.e-8)
+    return centers
+
+
+def refine(centers, x, y, z, w, size, coords, max_iter=200, tol=1.e-5):
+    """Lloyd iterations on the GPU (KMeansRun, src/KMeans.cpp:773-833): centres become
+    the w-weighted means of their objects (normalised to the unit sphere for spherical
+    coordinates) until the rms shift is below tol * size.  Returns the iterations run."""
+    dim = centers.shape[1]
+    c = np.ascontiguousarray(centers, dtype=float)
+    niter = ctypes.c_int()
+    _lib.check(_lib.lib().tc_kmeans_run(
+        _lib.dptr(c), c.shape[0], dim, _lib.dptr(x), _lib.dptr(y),
+        _lib.dptr(z) if dim == 3 else None, _lib.dptr(w), len(x), int(max_iter), float(tol),
+        float(size), 1 if coords == 'spherical' else 0, ctypes.byref(niter)))
+    centers[:] = c
+    return niter.value
+
+
+def assign(centers, x, y, z):
+    """Each object's nearest centre, the lowest index on ties."""
+    n = len(x)
+    out = np.empty(n, dtype=np.int32)
+    dim = centers.shape[1]
+    c = np.ascontiguousarray(centers, dtype=float)
+    _lib.check(_lib.lib().tc_quick_assign(_lib.dptr(c), c.shape[0], _lib.dptr(x), _lib.dptr(y),
+                                          _lib.dptr(z) if dim == 3 else None, n,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return out
