@@ -47,11 +47,12 @@ const char* tc_last_error(void);
 /* Number of visible GPUs (0 when no GPU is usable). */
 int tc_device_count(int* count);
 
-/* ---- Fields: replace {N,K,G}Field{Flat,Sphere,ThreeD}(...) ----
+/* ---- Fields: replace {N,K,Z,G}Field{Flat,Sphere,ThreeD}(...) ----
  * Reference: BuildNField/BuildKField/BuildAnyZField, src/Field.cpp:430-506,
  * bound at src/Field.cpp:684-690; the tree itself is Field<D,C>::Field +
  * BuildCells, src/Field.cpp:163-277 (built on the GPU here, eagerly).
- * d1 is k (K fields) or g1 (G fields); d2 is g2.  z and wpos may be NULL. */
+ * data_type 0 (N), 1 (K), 2 (Z, spin-0 complex) or 4 (G); d1 is k (K fields), z1 or g1;
+ * d2 is z2 or g2.  z and wpos may be NULL. */
 int tc_field_create(int data_type, int coords, const double* x, const double* y,
                     const double* z, const double* d1, const double* d2, const double* w,
                     const double* wpos, long long nobj, double minsize, double maxsize,
@@ -94,10 +95,11 @@ int tc_field_destroy(tc_field* f);
 /* ---- 2-point: replace {NN,KK,GG}Corr(...) + processAuto/processCross ----
  * Reference: BuildCorr2 src/Corr2.cpp:842-864 bound at :1135-1145;
  * ProcessAuto/ProcessCross :886-993 bound at :1151-1157 (BaseCorr2::process,
- * :146-288).  (d1, d2) is NN (0,0), KK (1,1), GG (4,4), or the mixed NK (0,1), NG (0,4),
- * KG (1,4) (src/Corr2.cpp DirectHelper<1>/<2>, :408-442; cross only, f1/fields1 the
- * d1 side); bin_type Log; metric Euclidean.  xi arrays: KK and NK xi0; NG and KG xi0
- * (xi) and xi1 (xi_im); GG xi0..xi3; the unused ones may be NULL. */
+ * :146-288).  (d1, d2) is NN (0,0), KK (1,1), ZZ (2,2), GG (4,4), or the mixed NK (0,1),
+ * NZ (0,2), NG (0,4), KZ (1,2), KG (1,4) (src/Corr2.cpp DirectHelper<1>/<2>, :408-442;
+ * cross only, f1/fields1 the d1 side); bin_type Log; metric Euclidean.  xi arrays: KK
+ * and NK xi0; NZ, NG, KZ and KG xi0 (xi) and xi1 (xi_im); ZZ and GG xi0..xi3; the unused
+ * ones may be NULL. */
 int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, int nbins,
                     double binsize, double b, double a, double minrpar, double maxrpar,
                     double xp, double yp, double zp, double* xi0, double* xi1, double* xi2,

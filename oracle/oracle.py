@@ -113,7 +113,7 @@ class Field:
 
 
 # codes of the mixed cross correlations in tco_corr2_sampled (tc_oracle.h Corr2Kind)
-MIXED = {(0, 1): 8, (0, 4): 9, (1, 4): 10}
+MIXED = {(0, 1): 8, (0, 4): 9, (1, 4): 10, (0, 2): 11, (1, 2): 12}
 
 
 def corr2(d, coords, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, exact=True,

@@ -86,9 +86,11 @@ def tree_dump(d, coords, x, y, z=None, k=None, g1=None, g2=None, w=None, wpos=No
 
 _FIELD = {(0, 1): 'NFieldFlat', (0, 2): 'NFieldThreeD', (0, 3): 'NFieldSphere',
           (1, 1): 'KFieldFlat', (1, 2): 'KFieldThreeD', (1, 3): 'KFieldSphere',
-          (4, 1): 'GFieldFlat', (4, 2): 'GFieldThreeD', (4, 3): 'GFieldSphere'}
-_CORR = {0: 'NNCorr', 1: 'KKCorr', 4: 'GGCorr'}
-_CORR_MIXED = {(0, 1): 'NKCorr', (0, 4): 'NGCorr', (1, 4): 'KGCorr'}
+          (4, 1): 'GFieldFlat', (4, 2): 'GFieldThreeD', (4, 3): 'GFieldSphere',
+          (2, 1): 'ZFieldFlat', (2, 2): 'ZFieldThreeD', (2, 3): 'ZFieldSphere'}
+_CORR = {0: 'NNCorr', 1: 'KKCorr', 2: 'ZZCorr', 4: 'GGCorr'}
+_CORR_MIXED = {(0, 1): 'NKCorr', (0, 4): 'NGCorr', (1, 4): 'KGCorr', (0, 2): 'NZCorr',
+               (1, 2): 'KZCorr'}
 
 
 def make_field(d, coords, x, y, z=None, k=None, g1=None, g2=None, w=None, wpos=None,
@@ -125,7 +127,7 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthre
     e = np.array([])
     xi = [out['xi0'], out['xi1'], out['xi2'], out['xi3']]
     mixed = d2 is not None and d2 != d
-    nxi = (2 if d2 == 4 else 1) if mixed else {0: 0, 1: 1, 4: 4}[d]
+    nxi = (2 if d2 in (2, 4) else 1) if mixed else {0: 0, 1: 1, 2: 4, 4: 4}[d]
     xi = [xi[i] if i < nxi else e for i in range(4)]
     name = _CORR_MIXED[(d, d2)] if mixed else _CORR[d]
     corr = getattr(m, name)(m.Log, float(min_sep), float(max_sep), int(nbins),

@@ -14,6 +14,7 @@
 template class Field<NData,Flat>; template class Field<NData,ThreeD>; template class Field<NData,Sphere>;
 template class Field<KData,Flat>; template class Field<KData,ThreeD>; template class Field<KData,Sphere>;
 template class Field<GData,Flat>; template class Field<GData,ThreeD>; template class Field<GData,Sphere>;
+template class Field<ZData,Flat>; template class Field<ZData,ThreeD>; template class Field<ZData,Sphere>;
 
 namespace {
 struct Dump {
@@ -32,6 +33,10 @@ int walk(const BaseCell<C>* c, Dump& d) {
     if (D == KData) wk = float(static_cast<const Cell<KData,C>*>(c)->getWK());
     if (D == GData) {
         std::complex<double> z = static_cast<const Cell<GData,C>*>(c)->getWZ();
+        gr = float(z.real()); gi = float(z.imag());
+    }
+    if (D == ZData) {
+        std::complex<double> z = static_cast<const Cell<ZData,C>*>(c)->getWZ();
         gr = float(z.real()); gi = float(z.imag());
     }
     d.wk.push_back(wk); d.g.push_back(gr); d.g.push_back(gi);
@@ -66,6 +71,7 @@ void* refdump_build(int D, int C, const double* x, const double* y, const double
 #define RD(DD, CC) return build<DD, CC>(x, y, z, d1, d2, w, wpos, n, minsize, maxsize, sm, brute, mintop, maxtop)
     if (D == NData) { if (C == Flat) RD(NData, Flat); if (C == ThreeD) RD(NData, ThreeD); RD(NData, Sphere); }
     if (D == KData) { if (C == Flat) RD(KData, Flat); if (C == ThreeD) RD(KData, ThreeD); RD(KData, Sphere); }
+    if (D == ZData) { if (C == Flat) RD(ZData, Flat); if (C == ThreeD) RD(ZData, ThreeD); RD(ZData, Sphere); }
     if (C == Flat) RD(GData, Flat); if (C == ThreeD) RD(GData, ThreeD); RD(GData, Sphere);
 #undef RD
 }

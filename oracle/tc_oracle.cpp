@@ -72,8 +72,10 @@ struct Builder {
             double s = 0.;
             for (size_t i = start; i < end; ++i) s += double(at(i).wk);
             nd.wk = float(s);
-        } else if (D == GData) {
-            if (C == Flat) {
+        } else if (D == GData || D == ZData) {
+            // spin 0 (Z) is never rotated: ParallelTransportSum multiplies by
+            // calculate_expmsialpha<0> = 1 (include/ProjectHelper.h:37-39)
+            if (C == Flat || D == ZData) {
                 double sr = 0., si = 0.;
                 for (size_t i = start; i < end; ++i) { sr += double(at(i).gr); si += double(at(i).gi); }
                 nd.gr = float(sr); nd.gi = float(si);
@@ -243,7 +245,7 @@ Tree* build_tree(int D, int C, const double* x, const double* y, const double* z
         it.wpos = wpos ? wpos[i] : w[i];
         it.idx = i;
         if (D == KData) it.wk = float(w[i] * d1[i]);
-        if (D == GData) { it.gr = float(w[i] * d1[i]); it.gi = float(w[i] * d2[i]); }
+        if (D == GData || D == ZData) { it.gr = float(w[i] * d1[i]); it.gi = float(w[i] * d2[i]); }
     }
     std::vector<int> perm(n);
     for (long i = 0; i < n; ++i) perm[i] = int(i);
@@ -371,6 +373,20 @@ struct Corr2Engine {
             acc.add(3, k, z1rz2i + z1iz2r);   // xim_im
             acc.add(0, k, z1rz2r + z1iz2i);   // xip
             acc.add(1, k, z1rz2i - z1iz2r);   // xip_im
+        } else if (D == ZData) {
+            // DirectHelper<3> for spin 0: no projection; both_complex_prod(z2, z1)
+            std::complex<double> g1(c1.gr, c1.gi), g2(c2.gr, c2.gi);
+            double z1rz2r = g2.real() * g1.real(), z1rz2i = g2.real() * g1.imag();
+            double z1iz2r = g2.imag() * g1.real(), z1iz2i = g2.imag() * g1.imag();
+            acc.add(2, k, z1rz2r - z1iz2i);
+            acc.add(3, k, z1rz2i + z1iz2r);
+            acc.add(0, k, z1rz2r + z1iz2i);
+            acc.add(1, k, z1rz2i - z1iz2r);
+        } else if (D == XK_NZ || D == XK_KZ) {
+            // DirectHelper<2> with a spin-0 second field: z2 times +getWK(c1)
+            const double f = D == XK_NZ ? double(c1.w) : double(c1.wk);
+            acc.add(0, k, double(c2.gr) * f);
+            acc.add(1, k, double(c2.gi) * f);
         } else if (D == XK_NK) {
             // DirectHelper<1>, src/Corr2.cpp:408-420: getWK of an N cell is its w
             // (include/ProjectHelper.h:75-80)
@@ -549,8 +565,8 @@ void tco_field_dump(void* h, double* pos, float* size, float* w, float* wk, floa
 }
 
 // 2-point correlation over one tree (auto, f2 == NULL) or two trees (cross).
-// d: NData, KData or GData for NN, KK, GG; XK_NK, XK_NG, XK_KG for the mixed cross
-// correlations (f1 the N or K field, f2 the K or G field).
+// d: NData, KData, ZData or GData for NN, KK, ZZ, GG; XK_NK, XK_NG, XK_KG, XK_NZ,
+// XK_KZ for the mixed cross correlations (f1 the N or K field, f2 the other).
 // out arrays (nbins each, NULL allowed): xi0..xi3, meanr, meanlogr, weight, npairs.
 // Values are ADDED to the arrays.  counters[4]: node-pair visits, accepted pairs,
 // accepted leaf-leaf pairs, refined singleBin evaluations.  row_stride > 1 keeps
@@ -577,6 +593,12 @@ int tco_corr2_sampled(void* f1, void* f2, int d, int coords, int nbins, double m
         if (coords == Flat) TCO_RUN(KData, Flat); else if (coords == ThreeD) TCO_RUN(KData, ThreeD); else TCO_RUN(KData, Sphere);
     } else if (d == GData) {
         if (coords == Flat) TCO_RUN(GData, Flat); else if (coords == ThreeD) TCO_RUN(GData, ThreeD); else TCO_RUN(GData, Sphere);
+    } else if (d == ZData) {
+        if (coords == Flat) TCO_RUN(ZData, Flat); else if (coords == ThreeD) TCO_RUN(ZData, ThreeD); else TCO_RUN(ZData, Sphere);
+    } else if (d == XK_NZ && t2) {
+        if (coords == Flat) TCO_RUN(XK_NZ, Flat); else if (coords == ThreeD) TCO_RUN(XK_NZ, ThreeD); else TCO_RUN(XK_NZ, Sphere);
+    } else if (d == XK_KZ && t2) {
+        if (coords == Flat) TCO_RUN(XK_KZ, Flat); else if (coords == ThreeD) TCO_RUN(XK_KZ, ThreeD); else TCO_RUN(XK_KZ, Sphere);
     } else if (d == XK_NK && t2) {
         if (coords == Flat) TCO_RUN(XK_NK, Flat); else if (coords == ThreeD) TCO_RUN(XK_NK, ThreeD); else TCO_RUN(XK_NK, Sphere);
     } else if (d == XK_NG && t2) {

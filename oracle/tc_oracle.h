@@ -15,10 +15,10 @@
 namespace tco {
 
 enum Coord { Flat = 1, ThreeD = 2, Sphere = 3 };
-enum DType { NData = 0, KData = 1, GData = 4 };   // values of include/Cell.h:34
+enum DType { NData = 0, KData = 1, ZData = 2, GData = 4 };   // include/Cell.h:34
 // The mixed 2-point correlations (cross only), as codes beside the DType values:
-// NK (src/Corr2.cpp DirectHelper<1>), NG and KG (DirectHelper<2>).
-enum Corr2Kind { XK_NK = 8, XK_NG = 9, XK_KG = 10 };
+// NK (src/Corr2.cpp DirectHelper<1>), NG, KG, NZ, KZ (DirectHelper<2>).
+enum Corr2Kind { XK_NK = 8, XK_NG = 9, XK_KG = 10, XK_NZ = 11, XK_KZ = 12 };
 enum SplitMethod { Middle = 0, Median = 1, Mean = 2, Random = 3 };
 
 struct Pt { double x = 0, y = 0, z = 0; };

@@ -16,12 +16,14 @@ from .config import parse_unit
 
 
 # The float64 columns a catalog may hold.
-_COLUMNS = ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2')
+_COLUMNS = ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2', '_z1',
+            '_z2')
 
 
 class Catalog:
     def __init__(self, file_name=None, config=None, *, x=None, y=None, z=None, ra=None,
-                 dec=None, r=None, w=None, wpos=None, k=None, g1=None, g2=None, patch=None,
+                 dec=None, r=None, w=None, wpos=None, k=None, g1=None, g2=None, z1=None,
+                 z2=None, patch=None,
                  x_units=None, y_units=None, ra_units=None, dec_units=None, flip_g1=False,
                  flip_g2=False, keep_zero_weight=False, name='', patch_centers=None, **kwargs):
         if file_name is not None:
@@ -38,6 +40,7 @@ class Catalog:
         self._ra, self._dec, self._r = arr(ra), arr(dec), arr(r)
         self._w, self._wpos = arr(w), arr(wpos)
         self._k, self._g1, self._g2 = arr(k), arr(g1), arr(g2)
+        self._z1, self._z2 = arr(z1), arr(z2)
         self._patch = None if patch is None else np.asarray(patch, dtype=int).ravel()
         if patch_centers is not None and patch is not None:
             raise ValueError("Cannot provide both patch and patch_centers")
@@ -67,11 +70,13 @@ class Catalog:
             raise TypeError("Either x,y or ra,dec must be given")
 
         n = len(self._x) if self._x is not None else len(self._ra)
-        for nm in ('_y', '_z', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2', '_patch'):
+        for nm in ('_y', '_z', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2', '_z1', '_z2',
+                   '_patch'):
             a = getattr(self, nm)
             if a is not None and len(a) != n:
                 raise ValueError("%s has the wrong length" % nm[1:])
-        for nm in ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2'):
+        for nm in ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2',
+                   '_z1', '_z2'):
             a = getattr(self, nm)
             if a is not None and np.any(np.isnan(a)):
                 raise ValueError("NaN values found in %s" % nm[1:])
@@ -174,6 +179,8 @@ class Catalog:
     k = property(lambda self: self._k)
     g1 = property(lambda self: self._g1)
     g2 = property(lambda self: self._g2)
+    z1 = property(lambda self: self._z1)
+    z2 = property(lambda self: self._z2)
     patch = property(lambda self: self._patch)
     npatch = property(lambda self: self._npatch)
     nontrivial_w = property(lambda self: self._nontrivial_w)
@@ -225,6 +232,11 @@ class Catalog:
     @property
     def varg(self):
         return (self._weighted_var(self._g1) + self._weighted_var(self._g2)) / 2.
+
+    @property
+    def varz(self):
+        # catalog.py:1237-1243
+        return (self._weighted_var(self._z1) + self._weighted_var(self._z2)) / 2.
 
     def _get_center_size(self):
         # catalog.py:1330-1341 (computed once per catalog)
@@ -315,6 +327,14 @@ class Catalog:
                                split_method=split_method, brute=brute, min_top=min_top,
                                max_top=max_top, coords=coords)
 
+    def getZField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
+                  min_top=None, max_top=10, coords=None):
+        if self._z1 is None or self._z2 is None:
+            raise TypeError("z1,z2 are not defined for this catalog")
+        return self._get_field('Z', 'ZField', min_size=min_size, max_size=max_size,
+                               split_method=split_method, brute=brute, min_top=min_top,
+                               max_top=max_top, coords=coords)
+
     def getGField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
                   min_top=None, max_top=10, coords=None):
         if self._g1 is None or self._g2 is None:
@@ -333,6 +353,21 @@ def calculateVarK(cat_list):
     use = w != 0
     meank = np.sum(w[use] * k[use]) / np.sum(w)
     return np.sum(w[use] ** 2 * (k[use] - meank) ** 2) / np.sum(w)
+
+
+def calculateVarZ(cat_list):
+    """The variance per component of the complex scalar (catalog.py:3402-3423)."""
+    cats = cat_list if isinstance(cat_list, list) else [cat_list]
+    if len(cats) == 1:
+        return cats[0].varz
+    w = np.concatenate([c.w for c in cats])
+    use = w != 0
+    out = 0.
+    for comp in ('z1', 'z2'):
+        v = np.concatenate([getattr(c, comp) for c in cats])
+        meanv = np.sum(w[use] * v[use]) / np.sum(w)
+        out += np.sum(w[use] ** 2 * (v[use] - meanv) ** 2) / np.sum(w)
+    return out / 2.
 
 
 def calculateVarG(cat_list):

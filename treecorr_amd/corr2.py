@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from . import cov as _cov
-from .catalog import Catalog, calculateVarG, calculateVarK
+from .catalog import Catalog, calculateVarG, calculateVarK, calculateVarZ
 from .config import parse_unit
 from .field import build_patch_fields, coord_enum
 from .util import set_omp_threads
@@ -1059,4 +1059,65 @@ class KGCorrelation(_MixedCorrelation):
         # kzcorrelation.py:148-160 -> :51-53
         self._finalize()
         self._var_num = vark * varg
+        self._use_raw()
+
+
+class ZZCorrelation(GGCorrelation):
+    """Spin-0 complex auto/cross correlation (treecorr/zzcorrelation.py): xip, xim as
+    GG's without the projection."""
+    _letters = 'ZZ'
+    _d = 2
+    _nxi = 4
+
+    def _calculate_var(self, cats):
+        return calculateVarZ(cats)
+
+    def finalize(self, varz1, varz2):
+        # zzcorrelation.py:76-88
+        self._finalize()
+        self._var_num = 2. * varz1 * varz2
+
+
+class NZCorrelation(_MixedCorrelation):
+    """Count-complex scalar correlation (treecorr/nzcorrelation.py)."""
+    _letters = 'NZ'
+    _d = 0
+    _d2 = 2
+    _nxi = 2
+
+    def _calculate_var(self, cats):
+        return None
+
+    def _calculate_var2(self, cats):
+        return calculateVarZ(cats)
+
+    def finalize(self, varz):
+        # nzcorrelation.py:65-67
+        self._finalize()
+        self._var_num = varz
+        self._use_raw()
+
+    def calculateXi(self, *, rz=None):
+        """xi, xi_im, or the compensated DZ - RZ given rz (nzcorrelation.py:98-133)."""
+        self._compensate(rz)
+        return self.xi, self.xi_im, self.varxi
+
+
+class KZCorrelation(_MixedCorrelation):
+    """Scalar-complex scalar correlation (treecorr/kzcorrelation.py)."""
+    _letters = 'KZ'
+    _d = 1
+    _d2 = 2
+    _nxi = 2
+
+    def _calculate_var(self, cats):
+        return calculateVarK(cats)
+
+    def _calculate_var2(self, cats):
+        return calculateVarZ(cats)
+
+    def finalize(self, vark, varz):
+        # kzcorrelation.py:51-53
+        self._finalize()
+        self._var_num = vark * varz
         self._use_raw()

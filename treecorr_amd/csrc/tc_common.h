@@ -17,21 +17,24 @@
 namespace tc {
 
 enum Coord { Flat = 1, ThreeD = 2, Sphere = 3 };       // include/Position.h:25
-enum DType { NData = 0, KData = 1, GData = 4 };        // include/Cell.h:34
+enum DType { NData = 0, KData = 1, ZData = 2, GData = 4 };   // include/Cell.h:34
 // The mixed 2-point correlations (cross only) as codes beside the DType values:
-// NK (src/Corr2.cpp DirectHelper<1>), NG and KG (DirectHelper<2>).
-enum Corr2Kind { XK_NK = 8, XK_NG = 9, XK_KG = 10 };
+// NK (src/Corr2.cpp DirectHelper<1>), NG, KG, NZ, KZ (DirectHelper<2>).
+enum Corr2Kind { XK_NK = 8, XK_NG = 9, XK_KG = 10, XK_NZ = 11, XK_KZ = 12 };
 __host__ __device__ inline int corr2_kind(int d1, int d2) {
     if (d1 == d2) return d1;
     if (d1 == NData && d2 == KData) return XK_NK;
     if (d1 == NData && d2 == GData) return XK_NG;
     if (d1 == KData && d2 == GData) return XK_KG;
+    if (d1 == NData && d2 == ZData) return XK_NZ;
+    if (d1 == KData && d2 == ZData) return XK_KZ;
     return -1;
 }
-// xi arrays of a 2-point correlation: NN none; KK, NK xi; NG, KG xi and xi_im;
-// GG xip, xip_im, xim, xim_im
+// xi arrays of a 2-point correlation: NN none; KK, NK xi; NG, KG, NZ, KZ xi and xi_im;
+// GG, ZZ xip, xip_im, xim, xim_im
 __host__ __device__ inline int corr2_nxi(int kind) {
-    return kind == NData ? 0 : kind == GData ? 4 : (kind == XK_NG || kind == XK_KG) ? 2 : 1;
+    return kind == NData ? 0 : (kind == GData || kind == ZData) ? 4
+         : (kind == XK_NG || kind == XK_KG || kind == XK_NZ || kind == XK_KZ) ? 2 : 1;
 }
 enum SplitMethod { Middle = 0, Median = 1, Mean = 2, Random = 3 };  // include/Cell.h:26
 

@@ -196,6 +196,19 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
         atomicAdd(&hist[(Q_XI1 * nb + k) * R], ri - ir);   // xip_im
         atomicAdd(&hist[(Q_XI2 * nb + k) * R], rr - ii);   // xim
         atomicAdd(&hist[(Q_XI3 * nb + k) * R], ri + ir);   // xim_im
+    } else if (D == ZData) {
+        // DirectHelper<3> for spin 0: no projection (exp(-i 0 alpha) = 1)
+        const double g1r = c1.a, g1i = c1.b, g2r = c2.a, g2i = c2.b;
+        double rr = g2r * g1r, ri = g2r * g1i, ir = g2i * g1r, ii = g2i * g1i;
+        atomicAdd(&hist[(Q_XI0 * nb + k) * R], rr + ii);   // xip
+        atomicAdd(&hist[(Q_XI1 * nb + k) * R], ri - ir);   // xip_im
+        atomicAdd(&hist[(Q_XI2 * nb + k) * R], rr - ii);   // xim
+        atomicAdd(&hist[(Q_XI3 * nb + k) * R], ri + ir);   // xim_im
+    } else if (D == XK_NZ || D == XK_KZ) {
+        // DirectHelper<2> for a spin-0 second field: z2 unprojected, times +getWK(c1)
+        const double f = D == XK_NZ ? double(c1.w) : double(c1.a);
+        atomicAdd(&hist[(Q_XI0 * nb + k) * R], double(c2.a) * f);
+        atomicAdd(&hist[(Q_XI1 * nb + k) * R], double(c2.b) * f);
     } else if (D == XK_NK) {
         // DirectHelper<1>, src/Corr2.cpp:408-420; getWK of an N cell is its w
         // (include/ProjectHelper.h:75-80)
@@ -819,6 +832,9 @@ static TravFn<IT> pick_traverse(int D, int C) {
     if (D == XK_NK) { if (C == Flat) TC_TF(XK_NK, Flat); if (C == ThreeD) TC_TF(XK_NK, ThreeD); TC_TF(XK_NK, Sphere); }
     if (D == XK_NG) { if (C == Flat) TC_TF(XK_NG, Flat); if (C == ThreeD) TC_TF(XK_NG, ThreeD); TC_TF(XK_NG, Sphere); }
     if (D == XK_KG) { if (C == Flat) TC_TF(XK_KG, Flat); if (C == ThreeD) TC_TF(XK_KG, ThreeD); TC_TF(XK_KG, Sphere); }
+    if (D == ZData) { if (C == Flat) TC_TF(ZData, Flat); if (C == ThreeD) TC_TF(ZData, ThreeD); TC_TF(ZData, Sphere); }
+    if (D == XK_NZ) { if (C == Flat) TC_TF(XK_NZ, Flat); if (C == ThreeD) TC_TF(XK_NZ, ThreeD); TC_TF(XK_NZ, Sphere); }
+    if (D == XK_KZ) { if (C == Flat) TC_TF(XK_KZ, Flat); if (C == ThreeD) TC_TF(XK_KZ, ThreeD); TC_TF(XK_KZ, Sphere); }
     if (C == Flat) TC_TF(GData, Flat); if (C == ThreeD) TC_TF(GData, ThreeD); TC_TF(GData, Sphere);
 #undef TC_TF
 }

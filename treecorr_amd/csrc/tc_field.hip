@@ -106,7 +106,7 @@ __global__ void k_init_elems(Elems E, const double* x, const double* y, const do
     E.wp[i] = wpos ? wpos[i] : wi;
     float a = 0.f, b = 0.f;
     if (D == KData) a = float(wi * d1[i]);
-    if (D == GData) { a = float(wi * d1[i]); b = float(wi * d2[i]); }
+    if (D == GData || D == ZData) { a = float(wi * d1[i]); b = float(wi * d2[i]); }
     E.a[i] = a; E.b[i] = b;
     // the root segment holding object i: bounds[r] <= i < bounds[r+1]
     int lo = 0, hi = nroot - 1;
@@ -424,7 +424,7 @@ __device__ __forceinline__ void segment_body(const long long s, const Seg* segs,
                         v[0] = double(E.a[i]); v[1] = v[2] = v[3] = v[4] = 0.;
                     }, sk);
                 fa = float(sk[0]);
-            } else if (D == GData) {
+            } else if (D == GData || D == ZData) {
                 double ccx = cx, ccy = cy, ccz = cz;
                 if (C == ThreeD) normalize_recip(ccx, ccy, ccz);
                 double sg2[NC];
@@ -432,7 +432,9 @@ __device__ __forceinline__ void segment_body(const long long s, const Seg* segs,
                     [&](long long i, double (&v)[NC]) {
                         const double gr = double(E.a[i]), gi = double(E.b[i]);
                         double vr, vi;
-                        if (C == Flat) {
+                        // spin 0 (Z) is never rotated: SimpleSum / ParallelTransportSum
+                        // with exp(-i 0 beta) = 1 (src/Cell.cpp:221-321)
+                        if (C == Flat || D == ZData) {
                             vr = gr; vi = gi;
                         } else {
                             // ParallelTransportSum (src/Cell.cpp:276-321), in the
@@ -745,14 +747,14 @@ __global__ __launch_bounds__(256) void k_segment_small(const Seg* segs, long lon
                 double sa = 0.;
                 for (long long i = start; i < end; ++i) sa += double(E.a[i]);
                 fa = float(sa);
-            } else if (D == GData) {
+            } else if (D == GData || D == ZData) {
                 double sr = 0., si = 0.;
                 double ccx = cx, ccy = cy, ccz = cz;
                 if (C == ThreeD) normalize_recip(ccx, ccy, ccz);
                 for (long long i = start; i < end; ++i) {
                     const double gr = double(E.a[i]), gi = double(E.b[i]);
                     double vr, vi;
-                    if (C == Flat) {
+                    if (C == Flat || D == ZData) {
                         vr = gr; vi = gi;
                     } else {
                         // ParallelTransportSum (src/Cell.cpp:276-321), in the evaluation
@@ -1341,6 +1343,7 @@ int build_trees(int D, int C, const double* x, const double* y, const double* z,
 #define TC_BT(DD, CC) return build_trees_t<DD, CC>(x, y, z, d1, d2, w, wpos, n, bounds, minsize, maxsize, sm, brute, mintop, maxtop, M)
     if (D == NData) { if (C == Flat) TC_BT(NData, Flat); if (C == ThreeD) TC_BT(NData, ThreeD); TC_BT(NData, Sphere); }
     if (D == KData) { if (C == Flat) TC_BT(KData, Flat); if (C == ThreeD) TC_BT(KData, ThreeD); TC_BT(KData, Sphere); }
+    if (D == ZData) { if (C == Flat) TC_BT(ZData, Flat); if (C == ThreeD) TC_BT(ZData, ThreeD); TC_BT(ZData, Sphere); }
     if (C == Flat) TC_BT(GData, Flat); if (C == ThreeD) TC_BT(GData, ThreeD); TC_BT(GData, Sphere);
 #undef TC_BT
 }

@@ -183,6 +183,15 @@ class KField(Field):
         return np.ascontiguousarray(cat.k, dtype=float), None
 
 
+class ZField(Field):
+    """Spin-0 complex field (treecorr/field.py ZField)."""
+    _d = 2
+
+    def _payload(self, cat):
+        return (np.ascontiguousarray(cat.z1, dtype=float),
+                np.ascontiguousarray(cat.z2, dtype=float))
+
+
 class GField(Field):
     _d = 4
 
@@ -191,7 +200,7 @@ class GField(Field):
                 np.ascontiguousarray(cat.g2, dtype=float))
 
 
-_KIND_CLS = {'N': 'NField', 'K': 'KField', 'G': 'GField'}
+_KIND_CLS = {'N': 'NField', 'K': 'KField', 'Z': 'ZField', 'G': 'GField'}
 
 
 def build_patch_fields(kind, cats, **kw):
@@ -213,8 +222,9 @@ def build_patch_fields(kind, cats, **kw):
         raise TypeError("g1,g2 are not defined for this catalog")
     cls = globals()[_KIND_CLS[kind]]
     proto = cls(cats[0], _handle=ctypes.c_void_p(), **kw)   # the parameters as a field holds them
-    d1 = cols.get('_k') if kind == 'K' else cols.get('_g1') if kind == 'G' else None
-    d2 = cols.get('_g2') if kind == 'G' else None
+    d1 = cols.get('_k') if kind == 'K' else cols.get('_g1') if kind == 'G' else \
+        cols.get('_z1') if kind == 'Z' else None
+    d2 = cols.get('_g2') if kind == 'G' else cols.get('_z2') if kind == 'Z' else None
     bounds = np.ascontiguousarray(parent._split_bounds, dtype=np.int64)
     n = int(bounds[-1])
     out = (ctypes.c_void_p * len(cats))()
