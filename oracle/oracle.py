@@ -113,7 +113,8 @@ class Field:
 
 
 # codes of the mixed cross correlations in tco_corr2_sampled (tc_oracle.h Corr2Kind)
-MIXED = {(0, 1): 8, (0, 4): 9, (1, 4): 10, (0, 2): 11, (1, 2): 12}
+MIXED = {(0, 1): 8, (0, 4): 9, (1, 4): 10, (0, 2): 11, (1, 2): 12, (0, 3): 13, (0, 5): 14,
+         (0, 6): 15, (1, 3): 16, (1, 5): 17, (1, 6): 18}
 
 
 def corr2(d, coords, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, exact=True,

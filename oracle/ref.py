@@ -87,10 +87,15 @@ def tree_dump(d, coords, x, y, z=None, k=None, g1=None, g2=None, w=None, wpos=No
 _FIELD = {(0, 1): 'NFieldFlat', (0, 2): 'NFieldThreeD', (0, 3): 'NFieldSphere',
           (1, 1): 'KFieldFlat', (1, 2): 'KFieldThreeD', (1, 3): 'KFieldSphere',
           (4, 1): 'GFieldFlat', (4, 2): 'GFieldThreeD', (4, 3): 'GFieldSphere',
-          (2, 1): 'ZFieldFlat', (2, 2): 'ZFieldThreeD', (2, 3): 'ZFieldSphere'}
-_CORR = {0: 'NNCorr', 1: 'KKCorr', 2: 'ZZCorr', 4: 'GGCorr'}
+          (2, 1): 'ZFieldFlat', (2, 2): 'ZFieldThreeD', (2, 3): 'ZFieldSphere',
+          (3, 1): 'VFieldFlat', (3, 2): 'VFieldThreeD', (3, 3): 'VFieldSphere',
+          (5, 1): 'TFieldFlat', (5, 2): 'TFieldThreeD', (5, 3): 'TFieldSphere',
+          (6, 1): 'QFieldFlat', (6, 2): 'QFieldThreeD', (6, 3): 'QFieldSphere'}
+_CORR = {0: 'NNCorr', 1: 'KKCorr', 2: 'ZZCorr', 3: 'VVCorr', 4: 'GGCorr', 5: 'TTCorr',
+         6: 'QQCorr'}
 _CORR_MIXED = {(0, 1): 'NKCorr', (0, 4): 'NGCorr', (1, 4): 'KGCorr', (0, 2): 'NZCorr',
-               (1, 2): 'KZCorr'}
+               (1, 2): 'KZCorr', (0, 3): 'NVCorr', (0, 5): 'NTCorr', (0, 6): 'NQCorr',
+               (1, 3): 'KVCorr', (1, 5): 'KTCorr', (1, 6): 'KQCorr'}
 
 
 def make_field(d, coords, x, y, z=None, k=None, g1=None, g2=None, w=None, wpos=None,
@@ -127,7 +132,7 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthre
     e = np.array([])
     xi = [out['xi0'], out['xi1'], out['xi2'], out['xi3']]
     mixed = d2 is not None and d2 != d
-    nxi = (2 if d2 in (2, 4) else 1) if mixed else {0: 0, 1: 1, 2: 4, 4: 4}[d]
+    nxi = (2 if d2 >= 2 else 1) if mixed else (0 if d == 0 else 1 if d == 1 else 4)
     xi = [xi[i] if i < nxi else e for i in range(4)]
     name = _CORR_MIXED[(d, d2)] if mixed else _CORR[d]
     corr = getattr(m, name)(m.Log, float(min_sep), float(max_sep), int(nbins),

@@ -15,12 +15,25 @@ template class Field<NData,Flat>; template class Field<NData,ThreeD>; template c
 template class Field<KData,Flat>; template class Field<KData,ThreeD>; template class Field<KData,Sphere>;
 template class Field<GData,Flat>; template class Field<GData,ThreeD>; template class Field<GData,Sphere>;
 template class Field<ZData,Flat>; template class Field<ZData,ThreeD>; template class Field<ZData,Sphere>;
+template class Field<VData,Flat>; template class Field<VData,ThreeD>; template class Field<VData,Sphere>;
+template class Field<TData,Flat>; template class Field<TData,ThreeD>; template class Field<TData,Sphere>;
+template class Field<QData,Flat>; template class Field<QData,ThreeD>; template class Field<QData,Sphere>;
 
 namespace {
 struct Dump {
     std::vector<double> pos; std::vector<float> size, w, wk, g;
     std::vector<long> n; std::vector<int> left, right, top;
 };
+
+// a complex field cell's w*z sum (C++11: no if constexpr)
+template <int D, int C> struct WZ {
+    static std::complex<double> get(const BaseCell<C>*) { return 0.; }
+};
+#define TC_WZ(DD) template <int C> struct WZ<DD, C> { \
+    static std::complex<double> get(const BaseCell<C>* c) { \
+        return static_cast<const Cell<DD,C>*>(c)->getWZ(); } };
+TC_WZ(ZData) TC_WZ(VData) TC_WZ(GData) TC_WZ(TData) TC_WZ(QData)
+#undef TC_WZ
 
 template <int D, int C>
 int walk(const BaseCell<C>* c, Dump& d) {
@@ -31,12 +44,8 @@ int walk(const BaseCell<C>* c, Dump& d) {
     d.size.push_back(float(c->getSize())); d.w.push_back(float(c->getW()));
     float wk = 0, gr = 0, gi = 0;
     if (D == KData) wk = float(static_cast<const Cell<KData,C>*>(c)->getWK());
-    if (D == GData) {
-        std::complex<double> z = static_cast<const Cell<GData,C>*>(c)->getWZ();
-        gr = float(z.real()); gi = float(z.imag());
-    }
-    if (D == ZData) {
-        std::complex<double> z = static_cast<const Cell<ZData,C>*>(c)->getWZ();
+    if (D == ZData || D == VData || D == GData || D == TData || D == QData) {
+        std::complex<double> z = WZ<D,C>::get(c);
         gr = float(z.real()); gi = float(z.imag());
     }
     d.wk.push_back(wk); d.g.push_back(gr); d.g.push_back(gi);
@@ -72,6 +81,9 @@ void* refdump_build(int D, int C, const double* x, const double* y, const double
     if (D == NData) { if (C == Flat) RD(NData, Flat); if (C == ThreeD) RD(NData, ThreeD); RD(NData, Sphere); }
     if (D == KData) { if (C == Flat) RD(KData, Flat); if (C == ThreeD) RD(KData, ThreeD); RD(KData, Sphere); }
     if (D == ZData) { if (C == Flat) RD(ZData, Flat); if (C == ThreeD) RD(ZData, ThreeD); RD(ZData, Sphere); }
+    if (D == VData) { if (C == Flat) RD(VData, Flat); if (C == ThreeD) RD(VData, ThreeD); RD(VData, Sphere); }
+    if (D == TData) { if (C == Flat) RD(TData, Flat); if (C == ThreeD) RD(TData, ThreeD); RD(TData, Sphere); }
+    if (D == QData) { if (C == Flat) RD(QData, Flat); if (C == ThreeD) RD(QData, ThreeD); RD(QData, Sphere); }
     if (C == Flat) RD(GData, Flat); if (C == ThreeD) RD(GData, ThreeD); RD(GData, Sphere);
 #undef RD
 }

@@ -72,7 +72,7 @@ struct Builder {
             double s = 0.;
             for (size_t i = start; i < end; ++i) s += double(at(i).wk);
             nd.wk = float(s);
-        } else if (D == GData || D == ZData) {
+        } else if (is_complex(D)) {
             // spin 0 (Z) is never rotated: ParallelTransportSum multiplies by
             // calculate_expmsialpha<0> = 1 (include/ProjectHelper.h:37-39)
             if (C == Flat || D == ZData) {
@@ -106,7 +106,7 @@ struct Builder {
                         double cosbeta = -cosA * cosB + sinA * sinA;
                         double sinbeta = C == Sphere ? (cosA + cosB) * sinA
                                                      : ((z1 - z2) + (0.5 * z2 * dsq + cosB)) * sinA;
-                        std::complex<double> e = expm2ialpha(std::complex<double>(cosbeta, sinbeta));
+                        std::complex<double> e = expmsialpha(spin_of(D), std::complex<double>(cosbeta, sinbeta));
                         std::complex<double> t = cmul(std::complex<double>(d.gr, d.gi), e);
                         sr += t.real(); si += t.imag();
                     }
@@ -245,7 +245,7 @@ Tree* build_tree(int D, int C, const double* x, const double* y, const double* z
         it.wpos = wpos ? wpos[i] : w[i];
         it.idx = i;
         if (D == KData) it.wk = float(w[i] * d1[i]);
-        if (D == GData || D == ZData) { it.gr = float(w[i] * d1[i]); it.gi = float(w[i] * d2[i]); }
+        if (is_complex(D)) { it.gr = float(w[i] * d1[i]); it.gi = float(w[i] * d2[i]); }
     }
     std::vector<int> perm(n);
     for (long i = 0; i < n; ++i) perm[i] = int(i);
@@ -354,17 +354,21 @@ struct Corr2Engine {
         }
         if (D == KData) {
             acc.add(0, k, double(c1.wk) * double(c2.wk));
-        } else if (D == GData) {
+        } else if (is_complex(D)) {
+            // DirectHelper<3>, src/Corr2.cpp:445-474, for a spin-s field with itself
+            const int S = spin_of(D);
             std::complex<double> g1(c1.gr, c1.gi), g2(c2.gr, c2.gi);
-            if (C == Flat) {   // ProjectHelper<Flat>::Project, ProjectHelper.h:126-136
+            if (S == 0) {
+            } else if (C == Flat) {   // ProjectHelper<Flat>::Project, ProjectHelper.h:126-147
                 std::complex<double> rr(c2.p.x - c1.p.x, c2.p.y - c1.p.y);
-                std::complex<double> e = expm2ialpha(rr);
+                std::complex<double> e = expmsialpha(S, rr);
                 g1 = cmul(g1, e); g2 = cmul(g2, e);
             } else {           // ProjectHelper<Sphere/ThreeD>::Project, :370-388, :573-589
                 Pt p1 = c1.p, p2 = c2.p;
                 if (C == ThreeD) { sphere_normalize_recip(p1); sphere_normalize_recip(p2); }
-                g2 = cmul(g2, expm2ialpha(sphere_direction(p1, p2)));
-                g1 = cmul(g1, expm2ialpha(sphere_direction(p2, p1)));
+                g2 = cmul(g2, expmsialpha(S, sphere_direction(p1, p2)));
+                g1 = cmul(g1, expmsialpha(S, sphere_direction(p2, p1)));
+                if (S == 1 || S == 3) g1 = -g1;   // odd spins: p1 on the left
             }
             // both_complex_prod(g2, g1), ProjectHelper.h:84-101
             double z1rz2r = g2.real() * g1.real(), z1rz2i = g2.real() * g1.imag();
@@ -373,37 +377,26 @@ struct Corr2Engine {
             acc.add(3, k, z1rz2i + z1iz2r);   // xim_im
             acc.add(0, k, z1rz2r + z1iz2i);   // xip
             acc.add(1, k, z1rz2i - z1iz2r);   // xip_im
-        } else if (D == ZData) {
-            // DirectHelper<3> for spin 0: no projection; both_complex_prod(z2, z1)
-            std::complex<double> g1(c1.gr, c1.gi), g2(c2.gr, c2.gi);
-            double z1rz2r = g2.real() * g1.real(), z1rz2i = g2.real() * g1.imag();
-            double z1iz2r = g2.imag() * g1.real(), z1iz2i = g2.imag() * g1.imag();
-            acc.add(2, k, z1rz2r - z1iz2i);
-            acc.add(3, k, z1rz2i + z1iz2r);
-            acc.add(0, k, z1rz2r + z1iz2i);
-            acc.add(1, k, z1rz2i - z1iz2r);
-        } else if (D == XK_NZ || D == XK_KZ) {
-            // DirectHelper<2> with a spin-0 second field: z2 times +getWK(c1)
-            const double f = D == XK_NZ ? double(c1.w) : double(c1.wk);
-            acc.add(0, k, double(c2.gr) * f);
-            acc.add(1, k, double(c2.gi) * f);
         } else if (D == XK_NK) {
             // DirectHelper<1>, src/Corr2.cpp:408-420: getWK of an N cell is its w
             // (include/ProjectHelper.h:75-80)
             acc.add(0, k, double(c1.w) * double(c2.wk));
-        } else if (D == XK_NG || D == XK_KG) {
-            // DirectHelper<2>, src/Corr2.cpp:423-442: g2 projected onto the line to c1
+        } else if (mixed_d2(D) >= 0) {
+            // DirectHelper<2>, src/Corr2.cpp:423-442: z2 projected onto the line to c1
             // (ProjectHelper<C>::Project(c1,c2,z2), include/ProjectHelper.h:107-113,
-            // :348-355, :545-555), times -getWK(c1) (tangential, not radial)
+            // :348-355, :545-555), times getWK(c1), negated for a shear (tangential)
+            const int d2 = mixed_d2(D), S = spin_of(d2);
             std::complex<double> g2(c2.gr, c2.gi);
-            if (C == Flat) {
-                g2 = cmul(g2, expm2ialpha(std::complex<double>(c2.p.x - c1.p.x, c2.p.y - c1.p.y)));
+            if (S == 0) {
+            } else if (C == Flat) {
+                g2 = cmul(g2, expmsialpha(S, std::complex<double>(c2.p.x - c1.p.x, c2.p.y - c1.p.y)));
             } else {
                 Pt p1 = c1.p, p2 = c2.p;
                 if (C == ThreeD) { sphere_normalize_recip(p1); sphere_normalize_recip(p2); }
-                g2 = cmul(g2, expm2ialpha(sphere_direction(p1, p2)));
+                g2 = cmul(g2, expmsialpha(S, sphere_direction(p1, p2)));
             }
-            const double f = -(D == XK_NG ? double(c1.w) : double(c1.wk));
+            double f = mixed_k1(D) ? double(c1.wk) : double(c1.w);
+            if (d2 == GData) f = -f;
             acc.add(0, k, g2.real() * f);
             acc.add(1, k, g2.imag() * f);
         }
@@ -565,8 +558,8 @@ void tco_field_dump(void* h, double* pos, float* size, float* w, float* wk, floa
 }
 
 // 2-point correlation over one tree (auto, f2 == NULL) or two trees (cross).
-// d: NData, KData, ZData or GData for NN, KK, ZZ, GG; XK_NK, XK_NG, XK_KG, XK_NZ,
-// XK_KZ for the mixed cross correlations (f1 the N or K field, f2 the other).
+// d: a DType for NN, KK, ZZ, VV, GG, TT, QQ; an XK_* code for the mixed cross
+// correlations (f1 the N or K field, f2 the other).
 // out arrays (nbins each, NULL allowed): xi0..xi3, meanr, meanlogr, weight, npairs.
 // Values are ADDED to the arrays.  counters[4]: node-pair visits, accepted pairs,
 // accepted leaf-leaf pairs, refined singleBin evaluations.  row_stride > 1 keeps
@@ -595,6 +588,16 @@ int tco_corr2_sampled(void* f1, void* f2, int d, int coords, int nbins, double m
         if (coords == Flat) TCO_RUN(GData, Flat); else if (coords == ThreeD) TCO_RUN(GData, ThreeD); else TCO_RUN(GData, Sphere);
     } else if (d == ZData) {
         if (coords == Flat) TCO_RUN(ZData, Flat); else if (coords == ThreeD) TCO_RUN(ZData, ThreeD); else TCO_RUN(ZData, Sphere);
+    } else if (d == VData) {
+        if (coords == Flat) TCO_RUN(VData, Flat); else if (coords == ThreeD) TCO_RUN(VData, ThreeD); else TCO_RUN(VData, Sphere);
+    } else if (d == TData) {
+        if (coords == Flat) TCO_RUN(TData, Flat); else if (coords == ThreeD) TCO_RUN(TData, ThreeD); else TCO_RUN(TData, Sphere);
+    } else if (d == QData) {
+        if (coords == Flat) TCO_RUN(QData, Flat); else if (coords == ThreeD) TCO_RUN(QData, ThreeD); else TCO_RUN(QData, Sphere);
+#define TCO_MX(KK) } else if (d == KK && t2) { \
+        if (coords == Flat) TCO_RUN(KK, Flat); else if (coords == ThreeD) TCO_RUN(KK, ThreeD); else TCO_RUN(KK, Sphere);
+    TCO_MX(XK_NV) TCO_MX(XK_NT) TCO_MX(XK_NQ) TCO_MX(XK_KV) TCO_MX(XK_KT) TCO_MX(XK_KQ)
+#undef TCO_MX
     } else if (d == XK_NZ && t2) {
         if (coords == Flat) TCO_RUN(XK_NZ, Flat); else if (coords == ThreeD) TCO_RUN(XK_NZ, ThreeD); else TCO_RUN(XK_NZ, Sphere);
     } else if (d == XK_KZ && t2) {

@@ -15,10 +15,27 @@
 namespace tco {
 
 enum Coord { Flat = 1, ThreeD = 2, Sphere = 3 };
-enum DType { NData = 0, KData = 1, ZData = 2, GData = 4 };   // include/Cell.h:34
+enum DType { NData = 0, KData = 1, ZData = 2, VData = 3, GData = 4, TData = 5, QData = 6 };
+// spin of a complex field type (include/ProjectHelper.h:63-72)
+inline int spin_of(int d) { return d == VData ? 1 : d == GData ? 2 : d == TData ? 3 : d == QData ? 4 : 0; }
+inline bool is_complex(int d) { return d == ZData || d == VData || d == GData || d == TData || d == QData; }
 // The mixed 2-point correlations (cross only), as codes beside the DType values:
 // NK (src/Corr2.cpp DirectHelper<1>), NG, KG, NZ, KZ (DirectHelper<2>).
-enum Corr2Kind { XK_NK = 8, XK_NG = 9, XK_KG = 10, XK_NZ = 11, XK_KZ = 12 };
+enum Corr2Kind { XK_NK = 8, XK_NG = 9, XK_KG = 10, XK_NZ = 11, XK_KZ = 12, XK_NV = 13,
+                 XK_NT = 14, XK_NQ = 15, XK_KV = 16, XK_KT = 17, XK_KQ = 18 };
+// the second field's type of a mixed kind
+inline int mixed_d2(int kind) {
+    switch (kind) {
+        case XK_NK: return KData;
+        case XK_NG: case XK_KG: return GData;
+        case XK_NZ: case XK_KZ: return ZData;
+        case XK_NV: case XK_KV: return VData;
+        case XK_NT: case XK_KT: return TData;
+        case XK_NQ: case XK_KQ: return QData;
+        default: return -1;
+    }
+}
+inline bool mixed_k1(int kind) { return kind == XK_KG || kind == XK_KZ || kind == XK_KV || kind == XK_KT || kind == XK_KQ; }
 enum SplitMethod { Middle = 0, Median = 1, Mean = 2, Random = 3 };
 
 struct Pt { double x = 0, y = 0, z = 0; };
@@ -111,6 +128,24 @@ inline std::complex<double> expm2ialpha(const std::complex<double>& r) {
 inline std::complex<double> cmul(const std::complex<double>& u, const std::complex<double>& v) {
     return std::complex<double>(u.real() * v.real() - u.imag() * v.imag(),
                                 u.real() * v.imag() + u.imag() * v.real());
+}
+// calculate_expmsialpha<s>, ProjectHelper.h:37-61, for s = 0..4
+inline std::complex<double> expmsialpha(int s, const std::complex<double>& r) {
+    if (s == 0) return std::complex<double>(1., 0.);
+    if (s == 2) return expm2ialpha(r);
+    if (s == 1) {                       // conj(r) / sqrt(|r|^2)
+        const double q = std::sqrt(safe_norm(r));
+        return std::complex<double>(r.real() / q, -r.imag() / q);
+    }
+    const std::complex<double> r2 = cmul(r, r);
+    if (s == 3) {                       // conj(r^3) / sqrt(|r^3|^2)
+        const std::complex<double> r3 = cmul(r2, r);
+        const double q = std::sqrt(safe_norm(r3));
+        return std::complex<double>(r3.real() / q, -r3.imag() / q);
+    }
+    const std::complex<double> r4 = cmul(r2, r2);   // s = 4: conj(r^4) / |r^2|^2
+    const double q = safe_norm(r2);
+    return std::complex<double>(r4.real() / q, -r4.imag() / q);
 }
 
 }  // namespace tco

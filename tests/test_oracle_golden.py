@@ -145,7 +145,9 @@ def test_oracle_against_literal_reference(oracle_lib):
         assert normwise(o[k], r[k]) < 1e-12, k
 
 
-@pytest.mark.parametrize('d1,d2', [(0, 1), (0, 4), (1, 4), (0, 2), (1, 2), (2, 2)])
+@pytest.mark.parametrize('d1,d2', [(0, 1), (0, 4), (1, 4), (0, 2), (1, 2), (2, 2), (3, 3),
+                                   (5, 5), (6, 6), (0, 3), (0, 5), (0, 6), (1, 3), (1, 5),
+                                   (1, 6)])
 @pytest.mark.parametrize('coords', [1, 2, 3])
 def test_oracle_mixed_cross_against_literal_reference(oracle_lib, d1, d2, coords):
     """NK, NG and KG (DirectHelper<1>/<2>, src/Corr2.cpp:408-442): the oracle's sums
@@ -172,26 +174,28 @@ def test_oracle_mixed_cross_against_literal_reference(oracle_lib, d1, d2, coords
     o = O.corr2(d1, coords, o1, o2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
                 bn['b'], bn['a'], d2=d2)
     np.testing.assert_array_equal(o['npairs'], r['npairs'])
-    keys = ['xi0', 'meanr', 'meanlogr', 'weight'] + (['xi1'] if d2 in (2, 4) else []) + \
+    keys = ['xi0', 'meanr', 'meanlogr', 'weight'] + (['xi1'] if d2 >= 2 else []) + \
         (['xi2', 'xi3'] if d1 == d2 else [])
     for k in keys:
         assert np.max(np.abs(r[k])) > 0, k
         assert normwise(o[k], r[k]) < 1e-12, k
 
 
+@pytest.mark.parametrize('d', [2, 3, 5, 6])
 @pytest.mark.parametrize('coords', [1, 2, 3])
-def test_oracle_z_tree_against_literal_reference(oracle_lib, coords):
-    """Z (spin-0 complex) fields: the oracle's cells against the compiled reference's,
-    bit for bit, the payload sums included (never rotated, src/Cell.cpp:221-354)."""
+def test_oracle_spin_tree_against_literal_reference(oracle_lib, d, coords):
+    """Z, V, T, Q (spin 0, 1, 3, 4 complex) fields: the oracle's cells against the
+    compiled reference's, bit for bit, the payload sums included (parallel transported
+    by exp(-i s beta) on the sphere, src/Cell.cpp:221-354)."""
     from oracle import ref as R
     if R.dumplib() is None:
         pytest.skip('oracle/_ref not built')
-    cat, scale = C.make_catalog(coords, 5000, 50 + coords, 0.3)
+    cat, scale = C.make_catalog(coords, 5000, 50 + coords + d, 0.3)
     kw = dict(min_size=scale * 1e-3, max_size=scale * 0.05, split_method=2, brute=False,
               min_top=3, max_top=10)
-    rd = R.tree_dump(2, coords, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
+    rd = R.tree_dump(d, coords, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
                      w=cat['w'], **kw)
-    od = oracle_lib.Field(2, coords, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
+    od = oracle_lib.Field(d, coords, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
                           w=cat['w'], **kw).dump()
     for k in ('pos', 'size', 'w', 'g', 'n', 'left', 'right', 'top'):
         np.testing.assert_array_equal(od[k], rd[k], err_msg=k)
