@@ -25,8 +25,8 @@ class Field:
     def __init__(self, d, coords, cat, min_size=0., max_size=np.inf, split_method=2, brute=False,
                  min_top=3, max_top=10):
         self.cols = [_c(cat['x']), _c(cat['y']), _c(cat.get('z')) if coords != 1 else None,
-                     _c(cat.get('k')) if d == 1 else _c(cat.get('g1')) if d in (2, 4) else None,
-                     _c(cat.get('g2')) if d in (2, 4) else None,
+                     _c(cat.get('k')) if d == 1 else _c(cat.get('g1')) if d >= 2 else None,
+                     _c(cat.get('g2')) if d >= 2 else None,
                      _c(cat['w']) if cat.get('w') is not None else np.ones(len(cat['x'])),
                      _c(cat.get('wpos'))]
         self.h = ctypes.c_void_p()
@@ -51,7 +51,7 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, d2=No
     """d2 != d: the mixed NK, NG, KG cross correlations (f1 the d side)."""
     out = {k: np.zeros(nbins) for k in KEYS2}
     d2 = d if d2 is None else d2
-    nxi = {0: 0, 1: 1, 2: 4, 4: 4}[d] if d2 == d else (2 if d2 in (2, 4) else 1)
+    nxi = (0 if d == 0 else 1 if d == 1 else 4) if d2 == d else (2 if d2 >= 2 else 1)
     xi = [_lib.dptr(out['xi%d' % i]) if i < nxi else None for i in range(4)]
     h = ctypes.c_void_p()
     L = _lib.lib()

@@ -145,7 +145,9 @@ def test_gpu_small_chunks_same_result(gpu, oracle_lib, chunk):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('d1,d2', [(0, 1), (0, 4), (1, 4), (0, 2), (1, 2), (2, 2)])
+@pytest.mark.parametrize('d1,d2', [(0, 1), (0, 4), (1, 4), (0, 2), (1, 2), (2, 2), (3, 3),
+                                   (5, 5), (6, 6), (0, 3), (0, 5), (0, 6), (1, 3), (1, 5),
+                                   (1, 6)])
 @pytest.mark.parametrize('coords', [1, 2, 3])
 def test_gpu_mixed_cross_matches_oracle(gpu, oracle_lib, d1, d2, coords):
     """NK, NG and KG cross correlations (src/Corr2.cpp DirectHelper<1>/<2>) against the
@@ -168,7 +170,7 @@ def test_gpu_mixed_cross_matches_oracle(gpu, oracle_lib, d1, d2, coords):
     ref = O.corr2(d1, coords, ofield(d1, cat1), ofield(d2, cat2), bn['nbins'], bn['min_sep'],
                   bn['max_sep'], bn['bin_size'], bn['b'], bn['a'], exact=True, d2=d2)
     np.testing.assert_array_equal(out['npairs'], ref['npairs'])
-    keys = ['xi0', 'meanr', 'meanlogr', 'weight'] + (['xi1'] if d2 in (2, 4) else []) + \
+    keys = ['xi0', 'meanr', 'meanlogr', 'weight'] + (['xi1'] if d2 >= 2 else []) + \
         (['xi2', 'xi3'] if d1 == d2 else [])
     for k in keys:
         assert np.max(np.abs(ref[k])) > 0, k
@@ -264,32 +266,44 @@ def test_gpu_gg_2m_matches_compiled_reference(gpu):
 
 
 @pytest.mark.gpu
-def test_gpu_z_finalized_api(gpu, oracle_lib):
-    """ZZ, NZ and KZ through the public classes: ZZ's raw sums equal the oracle's (the
-    catalog's z1, z2 as the field's complex values), xi is raw/weight, the shot variance
-    is 2 varz^2/weight (ZZ), varz/weight (NZ), vark varz/weight (KZ)."""
+@pytest.mark.parametrize('letter,d', [('Z', 2), ('V', 3), ('T', 5), ('Q', 6)])
+def test_gpu_spin_finalized_api(gpu, oracle_lib, letter, d):
+    """ZZ/VV/TT/QQ, NZ/NV/NT/NQ and KZ/KV/KT/KQ through the public classes: the auto
+    correlation's raw sums equal the oracle's (the catalog's two components as the
+    field's complex values), xi is raw/weight, the shot variance is 2 var^2/weight
+    (auto), var/weight (N), vark var/weight (K); the random-point compensation takes
+    its keyword after the field's letter."""
     O = oracle_lib
-    c1, scale = C.make_catalog(C.THREED, 12000, 611, 0.3)
-    c2, _ = C.make_catalog(C.THREED, 15000, 612, 0.3)
-    zc = gpu.Catalog(x=c2['x'], y=c2['y'], z=c2['z'], w=c2['w'], z1=c2['g1'], z2=c2['g2'])
+    c1, scale = C.make_catalog(C.THREED, 12000, 611 + d, 0.3)
+    c2, _ = C.make_catalog(C.THREED, 15000, 612 + d, 0.3)
+    lo = letter.lower()
+    kw2 = {lo + '1': c2['g1'], lo + '2': c2['g2']}
+    zc = gpu.Catalog(x=c2['x'], y=c2['y'], z=c2['z'], w=c2['w'], **kw2)
     lc = gpu.Catalog(x=c1['x'], y=c1['y'], z=c1['z'], w=c1['w'], k=c1['k'])
+    r3, _ = C.make_catalog(C.THREED, 8000, 613 + d, 0.0)
+    rc = gpu.Catalog(x=r3['x'], y=r3['y'], z=r3['z'])
+    var = getattr(zc, 'var' + lo)
     kw = dict(nbins=10, min_sep=scale * 0.01, max_sep=scale * 0.4, bin_slop=0.1, min_top=4)
-    zz = gpu.ZZCorrelation(**kw)
+    zz = getattr(gpu, letter * 2 + 'Correlation')(**kw)
     zz.process(zc)
-    np.testing.assert_allclose(zz.varxip, 2 * zc.varz ** 2 / zz.weight, rtol=1e-12)
+    np.testing.assert_allclose(zz.varxip, 2 * var ** 2 / zz.weight, rtol=1e-12)
     min_size, max_size = zz._get_minmax_size()
-    of = O.Field(2, C.THREED, c2['x'], c2['y'], c2['z'], g1=c2['g1'], g2=c2['g2'], w=c2['w'],
+    of = O.Field(d, C.THREED, c2['x'], c2['y'], c2['z'], g1=c2['g1'], g2=c2['g2'], w=c2['w'],
                  min_size=min_size, max_size=max_size, min_top=4)
-    ref = O.corr2(2, C.THREED, of, None, zz.nbins, zz._min_sep, zz._max_sep, zz._bin_size,
+    ref = O.corr2(d, C.THREED, of, None, zz.nbins, zz._min_sep, zz._max_sep, zz._bin_size,
                   zz.b, zz.angle_slop, exact=True)
     np.testing.assert_array_equal(zz.npairs, ref['npairs'])
     assert normwise(zz.xip * zz.weight, ref['xi0']) < RTOL
     assert normwise(zz.xim * zz.weight, ref['xi2']) < RTOL
 
-    nz = gpu.NZCorrelation(**kw)
+    nz = getattr(gpu, 'N' + letter + 'Correlation')(**kw)
     nz.process(lc, zc)
-    np.testing.assert_allclose(nz.varxi, zc.varz / nz.weight, rtol=1e-12)
-    assert np.all(np.isfinite(nz.xi_im))
-    kz = gpu.KZCorrelation(**kw)
+    np.testing.assert_allclose(nz.varxi, var / nz.weight, rtol=1e-12)
+    rz = getattr(gpu, 'N' + letter + 'Correlation')(**kw)
+    rz.process(rc, zc)
+    raw = nz.xi.copy()
+    xi, xi_im, varxi = nz.calculateXi(**{'r' + lo: rz})
+    np.testing.assert_allclose(xi, raw - rz.xi, rtol=1e-12, atol=1e-15)
+    kz = getattr(gpu, 'K' + letter + 'Correlation')(**kw)
     kz.process(lc, zc)
-    np.testing.assert_allclose(kz.varxi, lc.vark * zc.varz / kz.weight, rtol=1e-12)
+    np.testing.assert_allclose(kz.varxi, lc.vark * var / kz.weight, rtol=1e-12)

@@ -63,7 +63,8 @@ def test_device_tree_matches_reference_cells(gpu, case):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('d,coords,n', [(4, 3, 200000), (1, 2, 100000), (0, 1, 150000),
-                                        (2, 3, 100000), (2, 2, 60000)])
+                                        (2, 3, 100000), (2, 2, 60000), (3, 3, 80000),
+                                        (5, 2, 60000), (6, 3, 60000), (3, 1, 40000)])
 def test_device_tree_matches_oracle_large(gpu, oracle_lib, d, coords, n):
     O = oracle_lib
     cat, scale = C.make_catalog(coords, n, 40 + d, 0.3)
@@ -73,7 +74,7 @@ def test_device_tree_matches_oracle_large(gpu, oracle_lib, d, coords, n):
     ref = O.Field(d, coords, cat['x'], cat['y'], cat['z'], k=cat['k'], g1=cat['g1'],
                   g2=cat['g2'], w=cat['w'], **kw).dump()
     assert_same_tree(nd, ntop, ref)
-    if d in (2, 4):
+    if d >= 2:
         np.testing.assert_array_equal(nd['a'], ref['g'][:, 0])
         np.testing.assert_array_equal(nd['b'], ref['g'][:, 1])
     # Field center and size (Field::_center/_sizesq)

@@ -17,13 +17,16 @@ from .config import parse_unit
 
 # The float64 columns a catalog may hold.
 _COLUMNS = ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2', '_z1',
-            '_z2')
+            '_z2', '_v1', '_v2', '_t1', '_t2', '_q1', '_q2')
+# the complex fields' columns: spin 0, 1, 2, 3, 4 (treecorr/catalog.py)
+_COMPLEX = {'Z': ('z1', 'z2'), 'V': ('v1', 'v2'), 'G': ('g1', 'g2'), 'T': ('t1', 't2'),
+            'Q': ('q1', 'q2')}
 
 
 class Catalog:
     def __init__(self, file_name=None, config=None, *, x=None, y=None, z=None, ra=None,
                  dec=None, r=None, w=None, wpos=None, k=None, g1=None, g2=None, z1=None,
-                 z2=None, patch=None,
+                 z2=None, v1=None, v2=None, t1=None, t2=None, q1=None, q2=None, patch=None,
                  x_units=None, y_units=None, ra_units=None, dec_units=None, flip_g1=False,
                  flip_g2=False, keep_zero_weight=False, name='', patch_centers=None, **kwargs):
         if file_name is not None:
@@ -41,6 +44,9 @@ class Catalog:
         self._w, self._wpos = arr(w), arr(wpos)
         self._k, self._g1, self._g2 = arr(k), arr(g1), arr(g2)
         self._z1, self._z2 = arr(z1), arr(z2)
+        self._v1, self._v2 = arr(v1), arr(v2)
+        self._t1, self._t2 = arr(t1), arr(t2)
+        self._q1, self._q2 = arr(q1), arr(q2)
         self._patch = None if patch is None else np.asarray(patch, dtype=int).ravel()
         if patch_centers is not None and patch is not None:
             raise ValueError("Cannot provide both patch and patch_centers")
@@ -70,13 +76,11 @@ class Catalog:
             raise TypeError("Either x,y or ra,dec must be given")
 
         n = len(self._x) if self._x is not None else len(self._ra)
-        for nm in ('_y', '_z', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2', '_z1', '_z2',
-                   '_patch'):
+        for nm in _COLUMNS[1:] + ('_patch',):
             a = getattr(self, nm)
             if a is not None and len(a) != n:
                 raise ValueError("%s has the wrong length" % nm[1:])
-        for nm in ('_x', '_y', '_z', '_ra', '_dec', '_r', '_w', '_wpos', '_k', '_g1', '_g2',
-                   '_z1', '_z2'):
+        for nm in _COLUMNS:
             a = getattr(self, nm)
             if a is not None and np.any(np.isnan(a)):
                 raise ValueError("NaN values found in %s" % nm[1:])
@@ -181,6 +185,12 @@ class Catalog:
     g2 = property(lambda self: self._g2)
     z1 = property(lambda self: self._z1)
     z2 = property(lambda self: self._z2)
+    v1 = property(lambda self: self._v1)
+    v2 = property(lambda self: self._v2)
+    t1 = property(lambda self: self._t1)
+    t2 = property(lambda self: self._t2)
+    q1 = property(lambda self: self._q1)
+    q2 = property(lambda self: self._q2)
     patch = property(lambda self: self._patch)
     npatch = property(lambda self: self._npatch)
     nontrivial_w = property(lambda self: self._nontrivial_w)
@@ -233,10 +243,16 @@ class Catalog:
     def varg(self):
         return (self._weighted_var(self._g1) + self._weighted_var(self._g2)) / 2.
 
-    @property
-    def varz(self):
-        # catalog.py:1237-1243
-        return (self._weighted_var(self._z1) + self._weighted_var(self._z2)) / 2.
+    def _var_complex(self, letter):
+        # catalog.py:1229-1275: the mean of the two components' variances
+        a, b = _COMPLEX[letter]
+        return (self._weighted_var(getattr(self, '_' + a)) +
+                self._weighted_var(getattr(self, '_' + b))) / 2.
+
+    varz = property(lambda self: self._var_complex('Z'))
+    varv = property(lambda self: self._var_complex('V'))
+    vart = property(lambda self: self._var_complex('T'))
+    varq = property(lambda self: self._var_complex('Q'))
 
     def _get_center_size(self):
         # catalog.py:1330-1341 (computed once per catalog)
@@ -327,13 +343,35 @@ class Catalog:
                                split_method=split_method, brute=brute, min_top=min_top,
                                max_top=max_top, coords=coords)
 
+    def _get_complex_field(self, letter, **kw):
+        a, b = _COMPLEX[letter]
+        if getattr(self, '_' + a) is None or getattr(self, '_' + b) is None:
+            raise TypeError("%s,%s are not defined for this catalog" % (a, b))
+        return self._get_field(letter, letter + 'Field', **kw)
+
     def getZField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
                   min_top=None, max_top=10, coords=None):
-        if self._z1 is None or self._z2 is None:
-            raise TypeError("z1,z2 are not defined for this catalog")
-        return self._get_field('Z', 'ZField', min_size=min_size, max_size=max_size,
-                               split_method=split_method, brute=brute, min_top=min_top,
-                               max_top=max_top, coords=coords)
+        return self._get_complex_field('Z', min_size=min_size, max_size=max_size,
+                                       split_method=split_method, brute=brute,
+                                       min_top=min_top, max_top=max_top, coords=coords)
+
+    def getVField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
+                  min_top=None, max_top=10, coords=None):
+        return self._get_complex_field('V', min_size=min_size, max_size=max_size,
+                                       split_method=split_method, brute=brute,
+                                       min_top=min_top, max_top=max_top, coords=coords)
+
+    def getTField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
+                  min_top=None, max_top=10, coords=None):
+        return self._get_complex_field('T', min_size=min_size, max_size=max_size,
+                                       split_method=split_method, brute=brute,
+                                       min_top=min_top, max_top=max_top, coords=coords)
+
+    def getQField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
+                  min_top=None, max_top=10, coords=None):
+        return self._get_complex_field('Q', min_size=min_size, max_size=max_size,
+                                       split_method=split_method, brute=brute,
+                                       min_top=min_top, max_top=max_top, coords=coords)
 
     def getGField(self, *, min_size=0, max_size=None, split_method='mean', brute=False,
                   min_top=None, max_top=10, coords=None):
@@ -355,19 +393,36 @@ def calculateVarK(cat_list):
     return np.sum(w[use] ** 2 * (k[use] - meank) ** 2) / np.sum(w)
 
 
-def calculateVarZ(cat_list):
-    """The variance per component of the complex scalar (catalog.py:3402-3423)."""
+def _calculate_var_complex(cat_list, letter):
+    """The variance per component of a complex field over catalogs
+    (catalog.py:3402-3470)."""
     cats = cat_list if isinstance(cat_list, list) else [cat_list]
     if len(cats) == 1:
-        return cats[0].varz
+        return cats[0]._var_complex(letter)
     w = np.concatenate([c.w for c in cats])
     use = w != 0
     out = 0.
-    for comp in ('z1', 'z2'):
+    for comp in _COMPLEX[letter]:
         v = np.concatenate([getattr(c, comp) for c in cats])
         meanv = np.sum(w[use] * v[use]) / np.sum(w)
         out += np.sum(w[use] ** 2 * (v[use] - meanv) ** 2) / np.sum(w)
     return out / 2.
+
+
+def calculateVarZ(cat_list):
+    return _calculate_var_complex(cat_list, 'Z')
+
+
+def calculateVarV(cat_list):
+    return _calculate_var_complex(cat_list, 'V')
+
+
+def calculateVarT(cat_list):
+    return _calculate_var_complex(cat_list, 'T')
+
+
+def calculateVarQ(cat_list):
+    return _calculate_var_complex(cat_list, 'Q')
 
 
 def calculateVarG(cat_list):

@@ -14,7 +14,8 @@ import numpy as np
 
 from . import _lib
 from . import cov as _cov
-from .catalog import Catalog, calculateVarG, calculateVarK, calculateVarZ
+from .catalog import (Catalog, calculateVarG, calculateVarK, calculateVarQ, calculateVarT,
+                      calculateVarV, calculateVarZ)
 from .config import parse_unit
 from .field import build_patch_fields, coord_enum
 from .util import set_omp_threads
@@ -1097,9 +1098,14 @@ class NZCorrelation(_MixedCorrelation):
         self._var_num = varz
         self._use_raw()
 
-    def calculateXi(self, *, rz=None):
-        """xi, xi_im, or the compensated DZ - RZ given rz (nzcorrelation.py:98-133)."""
-        self._compensate(rz)
+    def calculateXi(self, **kw):
+        """xi, xi_im, or the compensated D? - R? given r? (nzcorrelation.py:98-133; the
+        keyword is rz, rv, rt or rq after the second field's letter)."""
+        name = 'r' + self._letters[1].lower()
+        for key in kw:
+            if key != name:
+                raise TypeError("calculateXi() got an unexpected keyword argument %r" % key)
+        self._compensate(kw.get(name))
         return self.xi, self.xi_im, self.varxi
 
 
@@ -1121,3 +1127,32 @@ class KZCorrelation(_MixedCorrelation):
         self._finalize()
         self._var_num = vark * varz
         self._use_raw()
+
+
+def _spin_classes():
+    """VV/TT/QQ (zzcorrelation.py-style, vv/tt/qqcorrelation.py), NV/NT/NQ
+    (nzcorrelation.py-style) and KV/KT/KQ (kzcorrelation.py-style) for the spin 1, 3
+    and 4 fields: the same finalize and variance rules as ZZ, NZ and KZ."""
+    out = {}
+    for letter, d, var in (('V', 3, calculateVarV), ('T', 5, calculateVarT),
+                           ('Q', 6, calculateVarQ)):
+        auto = type(letter * 2 + 'Correlation', (ZZCorrelation,), dict(
+            __doc__="%s%s correlation (treecorr/%s%scorrelation.py)" % (
+                letter, letter, letter.lower(), letter.lower()),
+            _letters=letter * 2, _d=d,
+            _calculate_var=lambda self, cats, var=var: var(cats)))
+        n = type('N' + letter + 'Correlation', (NZCorrelation,), dict(
+            __doc__="N%s correlation (treecorr/n%scorrelation.py)" % (letter, letter.lower()),
+            _letters='N' + letter, _d2=d,
+            _calculate_var2=lambda self, cats, var=var: var(cats)))
+        k = type('K' + letter + 'Correlation', (KZCorrelation,), dict(
+            __doc__="K%s correlation (treecorr/k%scorrelation.py)" % (letter, letter.lower()),
+            _letters='K' + letter, _d2=d,
+            _calculate_var2=lambda self, cats, var=var: var(cats)))
+        for c in (auto, n, k):
+            c.__module__ = __name__
+            out[c.__name__] = c
+    return out
+
+
+globals().update(_spin_classes())

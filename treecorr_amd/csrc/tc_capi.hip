@@ -159,8 +159,8 @@ int check_gpu() {
 int validate_field_args(int d, int coords, const double* x, const double* y, const double* z,
                         const double* d1, const double* d2, const double* w, long long n,
                         int sm, int mintop, int maxtop) {
-    if (d != NData && d != KData && d != ZData && d != GData)
-        return tc::fail("data_type must be 0 (N), 1 (K), 2 (Z) or 4 (G)");
+    if (d != NData && d != KData && !is_complex(d))
+        return tc::fail("data_type must be 0 (N), 1 (K), 2 (Z), 3 (V), 4 (G), 5 (T) or 6 (Q)");
     if (coords != Flat && coords != ThreeD && coords != Sphere)
         return tc::fail("coords must be 1 (Flat), 2 (ThreeD) or 3 (Sphere)");
     if (n <= 0) return tc::fail("a field needs at least one object");
@@ -170,7 +170,8 @@ int validate_field_args(int d, int coords, const double* x, const double* y, con
     if (coords != Flat && !z) return tc::fail("z is required for ThreeD and Sphere coordinates");
     if (d == KData && !d1) return tc::fail("k values are required for a K field");
     if (d == GData && (!d1 || !d2)) return tc::fail("g1 and g2 are required for a G field");
-    if (d == ZData && (!d1 || !d2)) return tc::fail("z1 and z2 are required for a Z field");
+    if (is_complex(d) && (!d1 || !d2))
+        return tc::fail("both components are required for a complex (Z, V, G, T, Q) field");
     if (sm == Random) {
         set_error("split_method='random' draws from the C library rand() and is not supported");
         return TC_ERR_UNSUPPORTED;
@@ -361,9 +362,9 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
     if (!out) return tc::fail("out is NULL");
     *out = nullptr;
     const int kind = corr2_kind(d1, d2);
-    if (kind < 0 || (d1 != NData && d1 != KData && d1 != ZData && d1 != GData)) {
-        set_error("only NN, KK, ZZ, GG, NK, NZ, NG, KZ and KG 2-point correlations are "
-                  "implemented");
+    if (kind < 0) {
+        set_error("2-point correlations: NN, KK, ZZ, VV, GG, TT, QQ, and N or K with K, Z, V, "
+                  "G, T or Q");
         return TC_ERR_UNSUPPORTED;
     }
     if (bin_type != 0) {

@@ -17,24 +17,48 @@
 namespace tc {
 
 enum Coord { Flat = 1, ThreeD = 2, Sphere = 3 };       // include/Position.h:25
-enum DType { NData = 0, KData = 1, ZData = 2, GData = 4 };   // include/Cell.h:34
+enum DType { NData = 0, KData = 1, ZData = 2, VData = 3, GData = 4, TData = 5, QData = 6 };
+                                                            // include/Cell.h:34
+// the complex field types and their spins (include/ProjectHelper.h:63-72)
+__host__ __device__ constexpr bool is_complex(int d) {
+    return d == ZData || d == VData || d == GData || d == TData || d == QData;
+}
+__host__ __device__ constexpr int spin_of(int d) {
+    return d == VData ? 1 : d == GData ? 2 : d == TData ? 3 : d == QData ? 4 : 0;
+}
 // The mixed 2-point correlations (cross only) as codes beside the DType values:
-// NK (src/Corr2.cpp DirectHelper<1>), NG, KG, NZ, KZ (DirectHelper<2>).
-enum Corr2Kind { XK_NK = 8, XK_NG = 9, XK_KG = 10, XK_NZ = 11, XK_KZ = 12 };
+// NK (src/Corr2.cpp DirectHelper<1>), N or K with a complex field (DirectHelper<2>).
+enum Corr2Kind { XK_NK = 8, XK_NG = 9, XK_KG = 10, XK_NZ = 11, XK_KZ = 12, XK_NV = 13,
+                 XK_NT = 14, XK_NQ = 15, XK_KV = 16, XK_KT = 17, XK_KQ = 18 };
+// the second field's type of a mixed kind with a complex second field (-1 otherwise)
+__host__ __device__ constexpr int mixed_d2(int kind) {
+    return kind == XK_NG || kind == XK_KG ? GData : kind == XK_NZ || kind == XK_KZ ? ZData
+         : kind == XK_NV || kind == XK_KV ? VData : kind == XK_NT || kind == XK_KT ? TData
+         : kind == XK_NQ || kind == XK_KQ ? QData : -1;
+}
+// whether the first field of a mixed kind is K (else N)
+__host__ __device__ constexpr bool mixed_k1(int kind) {
+    return kind == XK_KG || kind == XK_KZ || kind == XK_KV || kind == XK_KT || kind == XK_KQ;
+}
 __host__ __device__ inline int corr2_kind(int d1, int d2) {
-    if (d1 == d2) return d1;
+    if (d1 == d2) return (d1 == NData || d1 == KData || is_complex(d1)) ? d1 : -1;
     if (d1 == NData && d2 == KData) return XK_NK;
     if (d1 == NData && d2 == GData) return XK_NG;
     if (d1 == KData && d2 == GData) return XK_KG;
     if (d1 == NData && d2 == ZData) return XK_NZ;
     if (d1 == KData && d2 == ZData) return XK_KZ;
+    if (d1 == NData && d2 == VData) return XK_NV;
+    if (d1 == NData && d2 == TData) return XK_NT;
+    if (d1 == NData && d2 == QData) return XK_NQ;
+    if (d1 == KData && d2 == VData) return XK_KV;
+    if (d1 == KData && d2 == TData) return XK_KT;
+    if (d1 == KData && d2 == QData) return XK_KQ;
     return -1;
 }
 // xi arrays of a 2-point correlation: NN none; KK, NK xi; NG, KG, NZ, KZ xi and xi_im;
 // GG, ZZ xip, xip_im, xim, xim_im
 __host__ __device__ inline int corr2_nxi(int kind) {
-    return kind == NData ? 0 : (kind == GData || kind == ZData) ? 4
-         : (kind == XK_NG || kind == XK_KG || kind == XK_NZ || kind == XK_KZ) ? 2 : 1;
+    return kind == NData ? 0 : is_complex(kind) ? 4 : mixed_d2(kind) >= 0 ? 2 : 1;
 }
 enum SplitMethod { Middle = 0, Median = 1, Mean = 2, Random = 3 };  // include/Cell.h:26
 
@@ -115,6 +139,35 @@ __device__ __forceinline__ void expm2ialpha(double re, double im, double& er, do
     double ri = re * im + im * re;
     er = rr / n;
     ei = -ri / n;
+}
+
+// calculate_expmsialpha<S>, include/ProjectHelper.h:37-61: exp(-i S alpha) of the
+// direction r for spins 0..4 (S = 2 above).
+template <int S>
+__device__ __forceinline__ void expmsialpha(double re, double im, double& er, double& ei) {
+    if (S == 0) { er = 1.; ei = 0.; return; }
+    if (S == 2) { expm2ialpha(re, im, er, ei); return; }
+    if (S == 1) {            // conj(r) / sqrt(|r|^2)
+        double n = re * re + im * im;
+        n = n > 0. ? n : 1.;
+        const double q = sqrt(n);
+        er = re / q; ei = -im / q;
+        return;
+    }
+    const double r2r = re * re - im * im, r2i = re * im + im * re;
+    if (S == 3) {            // conj(r^3) / sqrt(|r^3|^2)
+        const double r3r = r2r * re - r2i * im, r3i = r2r * im + r2i * re;
+        double n = r3r * r3r + r3i * r3i;
+        n = n > 0. ? n : 1.;
+        const double q = sqrt(n);
+        er = r3r / q; ei = -r3i / q;
+        return;
+    }
+    // S == 4: conj(r^4) / |r^2|^2
+    const double r4r = r2r * r2r - r2i * r2i, r4i = r2r * r2i + r2i * r2r;
+    double n = r2r * r2r + r2i * r2i;
+    n = n > 0. ? n : 1.;
+    er = r4r / n; ei = -r4i / n;
 }
 
 // Position<Sphere>::normalize as compiled where a Position<Sphere> is built

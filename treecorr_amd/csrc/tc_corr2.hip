@@ -196,6 +196,66 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
         atomicAdd(&hist[(Q_XI1 * nb + k) * R], ri - ir);   // xip_im
         atomicAdd(&hist[(Q_XI2 * nb + k) * R], rr - ii);   // xim
         atomicAdd(&hist[(Q_XI3 * nb + k) * R], ri + ir);   // xim_im
+    } else if (D == VData || D == TData || D == QData) {
+        // DirectHelper<3> for spin 1, 3, 4 (the shear above has the fast form of spin 2):
+        // ProjectHelper<C>::Project(c1,c2,z1,z2), include/ProjectHelper.h:138-147,
+        // :370-388, :573-589; odd spins turn z1 by pi (p1 on the left)
+        constexpr int S = spin_of(D);
+        double g1r = c1.a, g1i = c1.b, g2r = c2.a, g2i = c2.b;
+        if (C == Flat) {
+            double er, ei;
+            expmsialpha<S>(c2.x - c1.x, c2.y - c1.y, er, ei);
+            cmul(g1r, g1i, er, ei);
+            cmul(g2r, g2i, er, ei);
+        } else {
+            double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
+            double dsq = rsq;
+            if (C == ThreeD) {
+                normalize_recip(x1, y1, z1);
+                normalize_recip(x2, y2, z2);
+                const double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+                dsq = dx * dx + dy * dy + dz * dz;
+            }
+            double d2r, d2i, d1r, d1i, e2r, e2i, e1r, e1i;
+            sphere_direction(x1, y1, z1, x2, y2, z2, dsq, d2r, d2i);
+            sphere_direction(x2, y2, z2, x1, y1, z1, dsq, d1r, d1i);
+            expmsialpha<S>(d2r, d2i, e2r, e2i);
+            expmsialpha<S>(d1r, d1i, e1r, e1i);
+            cmul(g2r, g2i, e2r, e2i);
+            cmul(g1r, g1i, e1r, e1i);
+            if (S == 1 || S == 3) { g1r = -g1r; g1i = -g1i; }
+        }
+        double rr = g2r * g1r, ri = g2r * g1i, ir = g2i * g1r, ii = g2i * g1i;
+        atomicAdd(&hist[(Q_XI0 * nb + k) * R], rr + ii);   // xip
+        atomicAdd(&hist[(Q_XI1 * nb + k) * R], ri - ir);   // xip_im
+        atomicAdd(&hist[(Q_XI2 * nb + k) * R], rr - ii);   // xim
+        atomicAdd(&hist[(Q_XI3 * nb + k) * R], ri + ir);   // xim_im
+    } else if (D == XK_NV || D == XK_NT || D == XK_NQ || D == XK_KV || D == XK_KT ||
+               D == XK_KQ) {
+        // DirectHelper<2> with a spin 1, 3 or 4 second field: z2 projected onto the line
+        // to c1, times +getWK(c1)
+        constexpr int S = spin_of(mixed_d2(D));
+        double g2r = c2.a, g2i = c2.b;
+        double er, ei;
+        if (C == Flat) {
+            expmsialpha<S>(c2.x - c1.x, c2.y - c1.y, er, ei);
+        } else {
+            double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
+            double dsq = rsq;
+            if (C == ThreeD) {
+                normalize_recip(x1, y1, z1);
+                normalize_recip(x2, y2, z2);
+                const double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+                dsq = dx * dx + dy * dy + dz * dz;
+            }
+            double d2r, d2i;
+            sphere_direction(x1, y1, z1, x2, y2, z2, dsq, d2r, d2i);
+            expmsialpha<S>(d2r, d2i, er, ei);
+        }
+        cmul(g2r, g2i, er, ei);
+        const double f = mixed_k1(D) ? double(c1.a) : double(c1.w);
+        atomicAdd(&hist[(Q_XI0 * nb + k) * R], g2r * f);
+        atomicAdd(&hist[(Q_XI1 * nb + k) * R], g2i * f);
     } else if (D == ZData) {
         // DirectHelper<3> for spin 0: no projection (exp(-i 0 alpha) = 1)
         const double g1r = c1.a, g1i = c1.b, g2r = c2.a, g2i = c2.b;
@@ -835,6 +895,10 @@ static TravFn<IT> pick_traverse(int D, int C) {
     if (D == ZData) { if (C == Flat) TC_TF(ZData, Flat); if (C == ThreeD) TC_TF(ZData, ThreeD); TC_TF(ZData, Sphere); }
     if (D == XK_NZ) { if (C == Flat) TC_TF(XK_NZ, Flat); if (C == ThreeD) TC_TF(XK_NZ, ThreeD); TC_TF(XK_NZ, Sphere); }
     if (D == XK_KZ) { if (C == Flat) TC_TF(XK_KZ, Flat); if (C == ThreeD) TC_TF(XK_KZ, ThreeD); TC_TF(XK_KZ, Sphere); }
+#define TC_TF3(KK) if (D == KK) { if (C == Flat) TC_TF(KK, Flat); if (C == ThreeD) TC_TF(KK, ThreeD); TC_TF(KK, Sphere); }
+    TC_TF3(VData) TC_TF3(TData) TC_TF3(QData)
+    TC_TF3(XK_NV) TC_TF3(XK_NT) TC_TF3(XK_NQ) TC_TF3(XK_KV) TC_TF3(XK_KT) TC_TF3(XK_KQ)
+#undef TC_TF3
     if (C == Flat) TC_TF(GData, Flat); if (C == ThreeD) TC_TF(GData, ThreeD); TC_TF(GData, Sphere);
 #undef TC_TF
 }

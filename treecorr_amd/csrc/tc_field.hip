@@ -106,7 +106,7 @@ __global__ void k_init_elems(Elems E, const double* x, const double* y, const do
     E.wp[i] = wpos ? wpos[i] : wi;
     float a = 0.f, b = 0.f;
     if (D == KData) a = float(wi * d1[i]);
-    if (D == GData || D == ZData) { a = float(wi * d1[i]); b = float(wi * d2[i]); }
+    if (is_complex(D)) { a = float(wi * d1[i]); b = float(wi * d2[i]); }
     E.a[i] = a; E.b[i] = b;
     // the root segment holding object i: bounds[r] <= i < bounds[r+1]
     int lo = 0, hi = nroot - 1;
@@ -424,7 +424,7 @@ __device__ __forceinline__ void segment_body(const long long s, const Seg* segs,
                         v[0] = double(E.a[i]); v[1] = v[2] = v[3] = v[4] = 0.;
                     }, sk);
                 fa = float(sk[0]);
-            } else if (D == GData || D == ZData) {
+            } else if (is_complex(D)) {
                 double ccx = cx, ccy = cy, ccz = cz;
                 if (C == ThreeD) normalize_recip(ccx, ccy, ccz);
                 double sg2[NC];
@@ -456,7 +456,7 @@ __device__ __forceinline__ void segment_body(const long long s, const Seg* segs,
                                     ? (cosA + cosB) * sinA
                                     : ((z1 - z2) + (0.5 * z2 * dsq + cosB)) * sinA;
                                 double er, ei;
-                                expm2ialpha(cosbeta, sinbeta, er, ei);
+                                expmsialpha<spin_of(D)>(cosbeta, sinbeta, er, ei);
                                 vr = gr * er - gi * ei;
                                 vi = gr * ei + gi * er;
                             }
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(256) void k_segment_small(const Seg* segs, long lon
                 double sa = 0.;
                 for (long long i = start; i < end; ++i) sa += double(E.a[i]);
                 fa = float(sa);
-            } else if (D == GData || D == ZData) {
+            } else if (is_complex(D)) {
                 double sr = 0., si = 0.;
                 double ccx = cx, ccy = cy, ccz = cz;
                 if (C == ThreeD) normalize_recip(ccx, ccy, ccz);
@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) void k_segment_small(const Seg* segs, long lon
                                 ? (cosA + cosB) * sinA
                                 : ((z1 - z2) + (0.5 * z2 * dsq + cosB)) * sinA;
                             double er, ei;
-                            expm2ialpha(cosbeta, sinbeta, er, ei);
+                            expmsialpha<spin_of(D)>(cosbeta, sinbeta, er, ei);
                             vr = gr * er - gi * ei;
                             vi = gr * ei + gi * er;
                         }
@@ -1344,6 +1344,9 @@ int build_trees(int D, int C, const double* x, const double* y, const double* z,
     if (D == NData) { if (C == Flat) TC_BT(NData, Flat); if (C == ThreeD) TC_BT(NData, ThreeD); TC_BT(NData, Sphere); }
     if (D == KData) { if (C == Flat) TC_BT(KData, Flat); if (C == ThreeD) TC_BT(KData, ThreeD); TC_BT(KData, Sphere); }
     if (D == ZData) { if (C == Flat) TC_BT(ZData, Flat); if (C == ThreeD) TC_BT(ZData, ThreeD); TC_BT(ZData, Sphere); }
+    if (D == VData) { if (C == Flat) TC_BT(VData, Flat); if (C == ThreeD) TC_BT(VData, ThreeD); TC_BT(VData, Sphere); }
+    if (D == TData) { if (C == Flat) TC_BT(TData, Flat); if (C == ThreeD) TC_BT(TData, ThreeD); TC_BT(TData, Sphere); }
+    if (D == QData) { if (C == Flat) TC_BT(QData, Flat); if (C == ThreeD) TC_BT(QData, ThreeD); TC_BT(QData, Sphere); }
     if (C == Flat) TC_BT(GData, Flat); if (C == ThreeD) TC_BT(GData, ThreeD); TC_BT(GData, Sphere);
 #undef TC_BT
 }

@@ -183,13 +183,35 @@ class KField(Field):
         return np.ascontiguousarray(cat.k, dtype=float), None
 
 
-class ZField(Field):
-    """Spin-0 complex field (treecorr/field.py ZField)."""
-    _d = 2
+class _ComplexField(Field):
+    _cols = None
 
     def _payload(self, cat):
-        return (np.ascontiguousarray(cat.z1, dtype=float),
-                np.ascontiguousarray(cat.z2, dtype=float))
+        return tuple(np.ascontiguousarray(getattr(cat, c), dtype=float) for c in self._cols)
+
+
+class ZField(_ComplexField):
+    """Spin-0 complex field (treecorr/field.py ZField)."""
+    _d = 2
+    _cols = ('z1', 'z2')
+
+
+class VField(_ComplexField):
+    """Spin-1 (vector) field (treecorr/field.py VField)."""
+    _d = 3
+    _cols = ('v1', 'v2')
+
+
+class TField(_ComplexField):
+    """Spin-3 field (treecorr/field.py TField)."""
+    _d = 5
+    _cols = ('t1', 't2')
+
+
+class QField(_ComplexField):
+    """Spin-4 field (treecorr/field.py QField)."""
+    _d = 6
+    _cols = ('q1', 'q2')
 
 
 class GField(Field):
@@ -200,7 +222,10 @@ class GField(Field):
                 np.ascontiguousarray(cat.g2, dtype=float))
 
 
-_KIND_CLS = {'N': 'NField', 'K': 'KField', 'Z': 'ZField', 'G': 'GField'}
+_KIND_CLS = {'N': 'NField', 'K': 'KField', 'Z': 'ZField', 'V': 'VField', 'G': 'GField',
+             'T': 'TField', 'Q': 'QField'}
+_PAYLOAD = {'K': ('_k', None), 'Z': ('_z1', '_z2'), 'V': ('_v1', '_v2'), 'G': ('_g1', '_g2'),
+            'T': ('_t1', '_t2'), 'Q': ('_q1', '_q2')}
 
 
 def build_patch_fields(kind, cats, **kw):
@@ -222,9 +247,11 @@ def build_patch_fields(kind, cats, **kw):
         raise TypeError("g1,g2 are not defined for this catalog")
     cls = globals()[_KIND_CLS[kind]]
     proto = cls(cats[0], _handle=ctypes.c_void_p(), **kw)   # the parameters as a field holds them
-    d1 = cols.get('_k') if kind == 'K' else cols.get('_g1') if kind == 'G' else \
-        cols.get('_z1') if kind == 'Z' else None
-    d2 = cols.get('_g2') if kind == 'G' else cols.get('_z2') if kind == 'Z' else None
+    n1, n2 = _PAYLOAD.get(kind, (None, None))
+    d1 = cols.get(n1) if n1 else None
+    d2 = cols.get(n2) if n2 else None
+    if (n1 and d1 is None) or (n2 and d2 is None):
+        raise TypeError("the %s field's columns are not defined for this catalog" % kind)
     bounds = np.ascontiguousarray(parent._split_bounds, dtype=np.int64)
     n = int(bounds[-1])
     out = (ctypes.c_void_p * len(cats))()
