@@ -28,10 +28,10 @@ extern "C" {
 #define TC_ERR_UNSUPPORTED 3
 
 /* Enumerations, values as in the reference:
- *   data types  N=0, K=1, G=4            (include/Cell.h:34 DataType)
+ *   data types  N=0, K=1, Z=2, V=3, G=4, T=5, Q=6 (include/Cell.h:34 DataType)
  *   coords      Flat=1, ThreeD=2, Sphere=3 (include/Position.h:25)
  *   bin types   Log=0, LogMultipole=5    (include/BinType.h:35)
- *   metrics     Euclidean=1              (include/Metric.h:29)
+ *   metrics     Euclidean=1, Periodic=6  (include/Metric.h:29)
  *   split       Middle=0, Median=1, Mean=2, Random=3 (include/Cell.h:26) */
 
 typedef struct tc_field tc_field;
@@ -95,11 +95,15 @@ int tc_field_destroy(tc_field* f);
 /* ---- 2-point: replace {NN,KK,GG}Corr(...) + processAuto/processCross ----
  * Reference: BuildCorr2 src/Corr2.cpp:842-864 bound at :1135-1145;
  * ProcessAuto/ProcessCross :886-993 bound at :1151-1157 (BaseCorr2::process,
- * :146-288).  (d1, d2) is NN (0,0), KK (1,1), ZZ (2,2), GG (4,4), or the mixed NK (0,1),
- * NZ (0,2), NG (0,4), KZ (1,2), KG (1,4) (src/Corr2.cpp DirectHelper<1>/<2>, :408-442;
- * cross only, f1/fields1 the d1 side); bin_type Log; metric Euclidean.  xi arrays: KK
- * and NK xi0; NZ, NG, KZ and KG xi0 (xi) and xi1 (xi_im); ZZ and GG xi0..xi3; the unused
- * ones may be NULL. */
+ * :146-288).  (d1, d2) is a field type with itself: NN (0,0), KK (1,1), ZZ (2,2), VV (3,3),
+ * GG (4,4), TT (5,5), QQ (6,6); or the mixed N or K (0 or 1) with K, Z, V, G, T or Q
+ * (src/Corr2.cpp DirectHelper<1>/<2>, :408-442; cross only, f1/fields1 the d1 side);
+ * bin_type Log.  xp, yp, zp are the periods of the
+ * Periodic metric (0 when unused).  The process calls take metric 1 (Euclidean) or 6
+ * (Periodic, flat or 3-d coordinates, include/Metric.h:813-838: every separation wrapped
+ * into [-p/2, p/2]).  xi arrays: KK and NK xi0; the other mixed kinds xi0 (xi) and xi1
+ * (xi_im); ZZ, VV, GG, TT, QQ xi0..xi3 (xip, xip_im, xim, xim_im); the unused ones may be
+ * NULL. */
 int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, int nbins,
                     double binsize, double b, double a, double minrpar, double maxrpar,
                     double xp, double yp, double zp, double* xi0, double* xi1, double* xi2,

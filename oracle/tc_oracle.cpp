@@ -289,6 +289,16 @@ struct Params2 {
     double logminsep, halfminsep, minsepsq, maxsepsq, bsq, asq;
 };
 
+// The Periodic metric's periods for the next 2-point runs (0: Euclidean); set by
+// tco_set_periodic.  MetricHelper<Periodic>::DistSq, include/Metric.h:831-838, wraps
+// each separation by Position::wrap, include/Position.h:106-112, :201-209.
+static double g_period[3] = {0., 0., 0.};
+static inline double wrap1(double d, double p) {
+    while (d > p / 2.) d -= p;
+    while (d < -p / 2.) d += p;
+    return d;
+}
+
 struct Acc2 {
     int nb; bool exact;
     std::vector<KSum> v;   // 8 quantities x nbins: xi0..xi3, meanr, meanlogr, weight, npairs
@@ -310,6 +320,10 @@ struct Corr2Engine {
 
     static double dist_sq(const Pt& a, const Pt& b) {   // Metric.h:143-147, :197-202
         Pt r{a.x - b.x, a.y - b.y, a.z - b.z};
+        if (g_period[0] > 0.) {
+            r.x = wrap1(r.x, g_period[0]); r.y = wrap1(r.y, g_period[1]);
+            if (C != Flat) r.z = wrap1(r.z, g_period[2]);
+        }
         return normsq(C, r);
     }
 
@@ -536,6 +550,9 @@ void* tco_field_new(int d, int coords, const double* x, const double* y, const d
 }
 
 void tco_field_free(void* h) { delete static_cast<Tree*>(h); }
+void tco_set_periodic(double xp, double yp, double zp) {
+    g_period[0] = xp; g_period[1] = yp; g_period[2] = zp;
+}
 long tco_field_nnodes(void* h) { return long(static_cast<Tree*>(h)->nodes.size()); }
 long tco_field_ntop(void* h) { return long(static_cast<Tree*>(h)->top.size()); }
 double tco_field_sizesq(void* h) { return static_cast<Tree*>(h)->sizesq; }

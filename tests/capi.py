@@ -47,8 +47,10 @@ class Field:
         return n.value
 
 
-def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, d2=None):
-    """d2 != d: the mixed NK, NG, KG cross correlations (f1 the d side)."""
+def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, d2=None,
+          period=None):
+    """d2 != d: the mixed cross correlations (f1 the d side); period (xp, yp, zp): the
+    Periodic metric."""
     out = {k: np.zeros(nbins) for k in KEYS2}
     d2 = d if d2 is None else d2
     nxi = (0 if d == 0 else 1 if d == 1 else 4) if d2 == d else (2 if d2 >= 2 else 1)
@@ -57,13 +59,15 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, d2=No
     L = _lib.lib()
     _lib.check(L.tc_corr2_create(d, d2, 0, float(min_sep), float(max_sep), int(nbins),
                                  float(bin_size), float(b), float(a), -1.7976931348623157e308,
-                                 1.7976931348623157e308, 0., 0., 0., *xi,
+                                 1.7976931348623157e308,
+                                 *(period if period is not None else (0., 0., 0.)), *xi,
                                  *[_lib.dptr(out[k]) for k in KEYS2[4:]], ctypes.byref(h)))
+    metric = 6 if period is not None else 1
     try:
         if f2 is None:
-            _lib.check(L.tc_corr2_process_auto(h, f1.h, 0, int(quick), 1))
+            _lib.check(L.tc_corr2_process_auto(h, f1.h, 0, int(quick), metric))
         else:
-            _lib.check(L.tc_corr2_process_cross(h, f1.h, f2.h, 0, int(quick), 1))
+            _lib.check(L.tc_corr2_process_cross(h, f1.h, f2.h, 0, int(quick), metric))
         c = (ctypes.c_longlong * 4)()
         _lib.check(L.tc_corr2_counters(h, c))
         out['counters'] = dict(visits=c[0], accepted=c[1], leafpairs=c[2], refined=c[3])

@@ -307,3 +307,52 @@ def test_gpu_spin_finalized_api(gpu, oracle_lib, letter, d):
     kz = getattr(gpu, 'K' + letter + 'Correlation')(**kw)
     kz.process(lc, zc)
     np.testing.assert_allclose(kz.varxi, lc.vark * var / kz.weight, rtol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d,coords,cross', [(0, 1, False), (4, 1, True), (1, 2, False),
+                                            (0, 2, True), (4, 2, False)])
+def test_gpu_periodic_matches_oracle(gpu, oracle_lib, d, coords, cross):
+    """The Periodic metric on the GPU against the oracle (itself pinned to the compiled
+    reference): npairs and the node-pair counters exact, the sums to RTOL."""
+    import capi
+    O = oracle_lib
+    cat, scale = C.make_catalog(coords, 30000, 120 + d + coords, 0.3)
+    cat2, _ = C.make_catalog(coords, 25000, 130 + d + coords, 0.3) if cross else (None, 0)
+    bn = C.binning(scale, 15, 0.003, 0.35, 0.1, 0.1 if d == 4 else 1.0)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    per = (scale, scale, scale)
+    f1 = capi.Field(d, coords, cat, brute=False, **kw)
+    f2 = capi.Field(d, coords, cat2, brute=False, **kw) if cross else None
+    out = capi.corr2(d, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                     bn['b'], bn['a'], period=per)
+
+    def ofield(c):
+        return O.Field(d, coords, c['x'], c['y'], c['z'], k=c['k'], g1=c['g1'], g2=c['g2'],
+                       w=c['w'], **kw)
+    ref = O.corr2(d, coords, ofield(cat), ofield(cat2) if cross else None, bn['nbins'],
+                  bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'], exact=True,
+                  period=per)
+    np.testing.assert_array_equal(out['npairs'], ref['npairs'])
+    for k in ['meanr', 'meanlogr', 'weight'] + (['xi0'] if d else []):
+        assert normwise(out[k], ref[k]) < RTOL, k
+    for k in ('visits', 'accepted', 'leafpairs', 'refined'):
+        assert out['counters'][k] == ref['counters'][k], k
+
+
+@pytest.mark.gpu
+def test_gpu_periodic_front_end(gpu, oracle_lib):
+    """NNCorrelation(metric='Periodic', period=L) through the public class: the pair
+    counts of the periodic box equal the oracle's."""
+    O = oracle_lib
+    cat, scale = C.make_catalog(C.THREED, 20000, 141, 0.3)
+    tcat = gpu.Catalog(x=cat['x'], y=cat['y'], z=cat['z'])
+    nn = gpu.NNCorrelation(nbins=10, min_sep=scale * 0.01, max_sep=scale * 0.3, bin_slop=0.,
+                           metric='Periodic', period=scale, min_top=4)
+    nn.process(tcat)
+    f = O.Field(0, C.THREED, cat['x'], cat['y'], cat['z'], min_size=0., max_size=0., min_top=4)
+    ref = O.corr2(0, C.THREED, f, None, nn.nbins, nn._min_sep, nn._max_sep, nn._bin_size,
+                  0., 1., exact=True, period=(scale, scale, scale))
+    np.testing.assert_array_equal(nn.npairs, ref['npairs'])
+    with pytest.raises(ValueError):
+        gpu.NNCorrelation(nbins=10, min_sep=1., max_sep=10., metric='Periodic').process(tcat)

@@ -21,7 +21,8 @@ from .field import build_patch_fields, coord_enum
 from .util import set_omp_threads
 
 LOG = 0
-EUCLIDEAN = 1
+EUCLIDEAN, PERIODIC = 1, 6          # include/Metric.h:29
+_METRIC_ENUM = {'Euclidean': EUCLIDEAN, 'Periodic': PERIODIC}
 
 
 class _PairSums:
@@ -144,8 +145,11 @@ class Corr2:
         self.max_rpar = sys.float_info.max
         if cfg.get('min_rpar') is not None or cfg.get('max_rpar') is not None:
             raise NotImplementedError("min_rpar/max_rpar are not supported by treecorr_amd")
-        if cfg.get('period') or cfg.get('xperiod') or cfg.get('yperiod') or cfg.get('zperiod'):
-            raise NotImplementedError("periodic metrics are not supported by treecorr_amd")
+        # corr2base.py:547-550: the Periodic metric's periods
+        period = float(cfg.get('period') or 0.)
+        self.xperiod = float(cfg.get('xperiod') or period)
+        self.yperiod = float(cfg.get('yperiod') or period)
+        self.zperiod = float(cfg.get('zperiod') or period)
         self.var_method = cfg.get('var_method', 'shot')
         self.num_bootstrap = int(cfg.get('num_bootstrap', 500))
         self.cross_patch_weight = cfg.get('cross_patch_weight')
@@ -176,8 +180,9 @@ class Corr2:
             h = ctypes.c_void_p()
             xi = [_lib.dptr(a) if a is not None else None for a in self._xi]
             _lib.check(_lib.lib().tc_corr2_create(
-                self._d, self._d if self._d2 is None else self._d2, LOG, self._min_sep, self._max_sep, self.nbins, self._bin_size,
-                self.b, self.angle_slop, self.min_rpar, self.max_rpar, 0., 0., 0., *xi,
+                self._d, self._d if self._d2 is None else self._d2, LOG, self._min_sep,
+                self._max_sep, self.nbins, self._bin_size, self.b, self.angle_slop,
+                self.min_rpar, self.max_rpar, self.xperiod, self.yperiod, self.zperiod, *xi,
                 _lib.dptr(self.meanr), _lib.dptr(self.meanlogr), _lib.dptr(self.weight),
                 _lib.dptr(self.npairs), ctypes.byref(h)))
             self._corr = h
@@ -193,8 +198,18 @@ class Corr2:
 
     def _set_metric(self, metric, coords1, coords2=None):
         metric = metric or self.config.get('metric', 'Euclidean')
-        if metric != 'Euclidean':
-            raise NotImplementedError("treecorr_amd implements the Euclidean metric")
+        if metric not in _METRIC_ENUM:
+            raise NotImplementedError("treecorr_amd implements the Euclidean and Periodic "
+                                      "metrics")
+        if metric == 'Periodic':
+            # corr2base.py:1576-1584 (coords.py): a box in flat or 3-d coordinates
+            if coords1 == 'spherical':
+                raise ValueError("Periodic metric is not valid for spherical coordinates")
+            if self.xperiod <= 0 or self.yperiod <= 0 or (coords1 == '3d' and
+                                                           self.zperiod <= 0):
+                raise ValueError("Periodic metric requires setting the period to use.")
+        elif self.xperiod != 0 or self.yperiod != 0 or self.zperiod != 0:
+            raise ValueError("period options are not valid for %s metric." % metric)
         if coords2 is not None and coords1 != coords2:
             raise ValueError("Cannot correlate catalogs with different coordinate systems.")
         if self.sep_units != '' and coords1 == '3d':
@@ -239,7 +254,7 @@ class Corr2:
                             split_method=self.split_method, brute=bool(self.brute),
                             min_top=self.min_top, max_top=self.max_top, coords=self.coords)
         _lib.check(_lib.lib().tc_corr2_process_auto(self.corr, field.data, 0, int(bool(corr_only)),
-                                                     EUCLIDEAN))
+                                                     _METRIC_ENUM[self.metric]))
         self._read_counters()
 
     def process_cross(self, cat1, cat2, *, metric=None, num_threads=None, corr_only=False):
@@ -259,7 +274,8 @@ class Corr2:
                          brute=self.brute is True or self.brute == 2,
                          min_top=self.min_top, max_top=self.max_top, coords=self.coords)
         _lib.check(_lib.lib().tc_corr2_process_cross(self.corr, f1.data, f2.data, 0,
-                                                      int(bool(corr_only)), EUCLIDEAN))
+                                                      int(bool(corr_only)),
+                                                      _METRIC_ENUM[self.metric]))
         self._read_counters()
 
     def _read_counters(self):
@@ -271,7 +287,8 @@ class Corr2:
         x1, y1, z1, s1 = c1._get_center_size()
         x2, y2, z2, s2 = c2._get_center_size()
         r = ctypes.c_int()
-        _lib.check(_lib.lib().tc_corr2_trivially_zero(self.corr, EUCLIDEAN, coord_enum(self.coords),
+        _lib.check(_lib.lib().tc_corr2_trivially_zero(self.corr, _METRIC_ENUM[self.metric],
+                                                       coord_enum(self.coords),
                                                        x1, y1, z1, s1, x2, y2, z2, s2,
                                                        ctypes.byref(r)))
         return bool(r.value)
@@ -372,7 +389,7 @@ class Corr2:
         _lib.check(_lib.lib().tc_corr2_process_pairs(
             self.corr, a1, len(fields1), a2, 0 if fields2 is None else len(fields2),
             p1.ctypes.data_as(ip), p2.ctypes.data_as(ip), len(jobs), int(bool(corr_only)),
-            EUCLIDEAN, _lib.dptr(out)))
+            _METRIC_ENUM[self.metric], _lib.dptr(out)))
         self._read_counters()
         return out
 

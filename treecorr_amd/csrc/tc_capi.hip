@@ -371,8 +371,8 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
         set_error("only Log binning is implemented for 2-point correlations");
         return TC_ERR_UNSUPPORTED;
     }
-    if (xp != 0. || yp != 0. || zp != 0.) {
-        set_error("periodic metrics are not implemented");
+    if (!(xp >= 0.) || !(yp >= 0.) || !(zp >= 0.)) {
+        set_error("the periods must be >= 0");
         return TC_ERR_UNSUPPORTED;
     }
     if (nbins <= 0 || !(minsep > 0.) || !(maxsep > minsep) || !(binsize > 0.))
@@ -393,10 +393,28 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
     P.logminsep = std::log(minsep); P.halfminsep = 0.5 * minsep;
     P.minsepsq = minsep * minsep; P.maxsepsq = maxsep * maxsep;
     P.bsq = b * b; P.asq = a * a;
+    P.periodic = 0; P.xp = xp; P.yp = yp; P.zp = zp;   // used with the Periodic metric
     c->xi[0] = xi0; c->xi[1] = xi1; c->xi[2] = xi2; c->xi[3] = xi3;
     c->meanr = meanr; c->meanlogr = meanlogr; c->weight = weight; c->npairs = npairs;
     *out = c;
     return 0;
+}
+
+// The metric of a process call (src/Corr2.cpp:886-993 dispatch on Metric, enum at
+// include/Metric.h:29): Euclidean (1), or Periodic (6) with the correlation's periods,
+// which wrap every separation (flat and 3-d coordinates).
+static int set_metric(const tc_corr2* c, int metric, int coords, C2Params& P) {
+    P.periodic = 0;
+    if (metric == 1) return 0;
+    if (metric == 6) {
+        if (coords == Sphere) return tc::fail("the Periodic metric needs flat or 3-d coordinates");
+        if (!(c->P.xp > 0.) || !(c->P.yp > 0.) || (coords == ThreeD && !(c->P.zp > 0.)))
+            return tc::fail("the Periodic metric needs positive periods");
+        P.periodic = 1;
+        return 0;
+    }
+    set_error("only the Euclidean and Periodic metrics are implemented");
+    return TC_ERR_UNSUPPORTED;
 }
 
 // BaseCorr2::process(f1,f2) early exit, src/Corr2.cpp:215-231: the two fields are
@@ -406,6 +424,10 @@ static bool fields_out_of_range(const C2Params& P, int coords, const tc_field* f
     const FieldInfo& a = f1->T.info;
     const FieldInfo& b = f2->T.info;
     double dx = a.cx - b.cx, dy = a.cy - b.cy, dz = a.cz - b.cz;
+    if (P.periodic) {
+        dx = wrap_period(dx, P.xp); dy = wrap_period(dy, P.yp);
+        if (coords != Flat) dz = wrap_period(dz, P.zp);
+    }
     double rsq = coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
     double s1ps2 = std::sqrt(a.sizesq) + std::sqrt(b.sizesq);
     bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
@@ -416,15 +438,12 @@ static bool fields_out_of_range(const C2Params& P, int coords, const tc_field* f
 
 static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int metric) {
     if (!c || !f1) return tc::fail("NULL argument");
-    if (metric != 1) {
-        set_error("only the Euclidean metric is implemented");
-        return TC_ERR_UNSUPPORTED;
-    }
     if (!f2 && c->d != c->d2) return tc::fail("a mixed correlation needs two fields");
     if (f1->d != c->d || (f2 && (f2->d != c->d2 || f2->coords != f1->coords)))
         return tc::fail("field types do not match the correlation");
-    if (int rc = check_gpu()) return rc;
     C2Params P = c->P;
+    if (int rc = set_metric(c, metric, f1->coords, P)) return rc;
+    if (int rc = check_gpu()) return rc;
     P.quick = quick ? 1 : 0;
     const int nb = P.nbins;
     for (int k = 0; k < 4; ++k) c->counters[k] = 0;
@@ -467,10 +486,6 @@ int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
                            int npair, int quick, int metric, double* pair_out) {
     if (!c || !fields1 || (npair > 0 && (!p1 || !p2 || !pair_out)) || npair < 0 || nfield1 <= 0)
         return tc::fail("NULL or invalid argument");
-    if (metric != 1) {
-        set_error("only the Euclidean metric is implemented");
-        return TC_ERR_UNSUPPORTED;
-    }
     const bool same = fields2 == nullptr;
     if (!same && nfield2 <= 0) return tc::fail("nfield2 must be positive");
     if (same && c->d != c->d2) return tc::fail("a mixed correlation needs two field lists");
@@ -489,8 +504,9 @@ int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
     for (int p = 0; p < npair; ++p)
         if (p1[p] < 0 || p1[p] >= nfield1 || p2[p] < 0 || p2[p] >= n2)
             return tc::fail("patch index out of range in the pair list");
-    if (int rc = check_gpu()) return rc;
     C2Params P = c->P;
+    if (int rc = set_metric(c, metric, coords, P)) return rc;
+    if (int rc = check_gpu()) return rc;
     P.quick = quick ? 1 : 0;
     const int nb = P.nbins;
     for (int k = 0; k < 4; ++k) c->counters[k] = 0;
@@ -548,10 +564,8 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
                             double z1, double s1, double x2, double y2, double z2, double s2,
                             int* result) {
     if (!c || !result) return tc::fail("NULL argument");
-    if (metric != 1) {
-        set_error("only the Euclidean metric is implemented");
-        return TC_ERR_UNSUPPORTED;
-    }
+    C2Params P = c->P;
+    if (int rc = set_metric(c, metric, coords, P)) return rc;
     // TriviallyZero3 builds Position<C>(x,y,z) (Sphere normalises) and calls
     // BaseCorr2::triviallyZero, src/Corr2.cpp:590-601, 1040-1048.
     if (coords == Sphere) {
@@ -563,9 +577,12 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
         norm(x1, y1, z1); norm(x2, y2, z2);
     }
     double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+    if (P.periodic) {
+        dx = wrap_period(dx, P.xp); dy = wrap_period(dy, P.yp);
+        if (coords != Flat) dz = wrap_period(dz, P.zp);
+    }
     double rsq = coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
     double s1ps2 = s1 + s2;
-    const C2Params& P = c->P;
     *result = (rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2)) ? 1 : 0;
     return 0;
 }

@@ -141,6 +141,14 @@ __device__ __forceinline__ void expm2ialpha(double re, double im, double& er, do
     ei = -ri / n;
 }
 
+// Position::wrap, include/Position.h:106-112, :201-209: a separation brought into
+// [-p/2, p/2] by whole periods
+__host__ __device__ inline double wrap_period(double d, double p) {
+    while (d > p / 2.) d -= p;
+    while (d < -p / 2.) d += p;
+    return d;
+}
+
 // calculate_expmsialpha<S>, include/ProjectHelper.h:37-61: exp(-i S alpha) of the
 // direction r for spins 0..4 (S = 2 above).
 template <int S>

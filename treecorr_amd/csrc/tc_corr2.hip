@@ -50,6 +50,17 @@ __device__ __forceinline__ double dist_sq(const DNode& a, const DNode& b) {
     return dx * dx + dy * dy + dz * dz;
 }
 
+// MetricHelper<Periodic>::DistSq, include/Metric.h:831-838 (and its ThreeD form): the
+// separation wrapped by the periods, then the Euclidean norm
+template <int C>
+__device__ __forceinline__ double dist_sq_periodic(const DNode& a, const DNode& b,
+                                                   const C2Params& P) {
+    const double dx = wrap_period(a.x - b.x, P.xp), dy = wrap_period(a.y - b.y, P.yp);
+    if (C == Flat) return dx * dx + dy * dy;
+    const double dz = wrap_period(a.z - b.z, P.zp);
+    return dx * dx + dy * dy + dz * dz;
+}
+
 #ifndef TC_FAST_LOG
 #define TC_FAST_LOG 1
 #endif
@@ -444,7 +455,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 double rsq = 0., s1ps2 = 0., s1ps2sq = 0.;
                 const double s1 = c1.size, s2 = c2.size;
                 if (c1.w != 0.f && c2.w != 0.f) {
-                    rsq = dist_sq<C>(c1, c2);
+                    rsq = P.periodic ? dist_sq_periodic<C>(c1, c2, P) : dist_sq<C>(c1, c2);
                     s1ps2 = s1 + s2;
                     const bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
                                            rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);

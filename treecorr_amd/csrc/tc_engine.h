@@ -48,14 +48,16 @@ struct C2Params {
     int nbins, quick;
     double minsep, maxsep, binsize, b, a;
     double logminsep, halfminsep, minsepsq, maxsepsq, bsq, asq;
+    int periodic;               // MetricHelper<Periodic>: separations wrapped by the periods
+    double xp, yp, zp;
 };
 
-// QuickAssign on the device: patch[i] = the nearest of npatch centres (2 or 3 values
-// each; z == nullptr means 2), host arrays in and out.
 // Lloyd iterations of k-means over the objects (centers in/out, npatch x dim).
 int kmeans_run(double* centers, int npatch, int dim, const double* x, const double* y,
                const double* z, const double* w, long long n, int max_iter, double tol,
                double size, int sphere, int* niter);
+// QuickAssign on the device: patch[i] = the nearest of npatch centres (2 or 3 values
+// each; z == nullptr means 2), host arrays in and out.
 int quick_assign(const double* centers, int npatch, const double* x, const double* y,
                  const double* z, long long n, int* patch);
 
