@@ -293,6 +293,9 @@ struct Params2 {
 // tco_set_periodic.  MetricHelper<Periodic>::DistSq, include/Metric.h:831-838, wraps
 // each separation by Position::wrap, include/Position.h:106-112, :201-209.
 static double g_period[3] = {0., 0., 0.};
+// The Arc metric for the next 2-point runs (tco_set_arc); MetricHelper<Arc>,
+// include/Metric.h:656-810.
+static int g_arc = 0;
 static inline double wrap1(double d, double p) {
     while (d > p / 2.) d -= p;
     while (d < -p / 2.) d += p;
@@ -318,6 +321,25 @@ struct Corr2Engine {
     bool quick;
     Corr2Engine(const Params2& p, const Tree& t1, const Tree& t2, bool q) : P(p), T1(t1), T2(t2), quick(q) {}
 
+    // MetricHelper<M>::DistSq; the Arc metric's 3-d form also scales the sizes to the
+    // unit sphere
+    static double dist_sq(const Pt& a, const Pt& b, double& s1, double& s2) {
+        if (g_arc) {
+            if (C == Sphere) {      // Metric.h:666-680: theta = 2 asin(L/2), L the chord
+                Pt r{a.x - b.x, a.y - b.y, a.z - b.z};
+                double th = 2. * std::asin(std::sqrt(normsq(ThreeD, r)) / 2.);
+                return th * th;
+            }
+            // Metric.h:701-721: theta = atan2(|a x b|, a.b); s /= |p|
+            Pt c{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+            double th = std::atan2(std::sqrt(normsq(ThreeD, c)),
+                                   a.x * b.x + a.y * b.y + a.z * b.z);
+            s1 /= std::sqrt(normsq(ThreeD, a));
+            s2 /= std::sqrt(normsq(ThreeD, b));
+            return th * th;
+        }
+        return dist_sq(a, b);
+    }
     static double dist_sq(const Pt& a, const Pt& b) {   // Metric.h:143-147, :197-202
         Pt r{a.x - b.x, a.y - b.y, a.z - b.z};
         if (g_period[0] > 0.) {
@@ -423,7 +445,7 @@ struct Corr2Engine {
         acc.visits++;
         if (c1.w == 0.f || c2.w == 0.f) return;
         double s1 = c1.size, s2 = c2.size;
-        double rsq = dist_sq(c1.p, c2.p);
+        double rsq = dist_sq(c1.p, c2.p, s1, s2);
         double s1ps2 = s1 + s2;
         // BinTypeHelper<Log>::tooSmallDist / tooLargeDist, BinType.h:70-82
         if (rsq < P.minsepsq && s1ps2 < P.minsep &&
@@ -485,7 +507,7 @@ static void run_corr2(const Params2& P, const Tree& t1, const Tree* t2, bool qui
     if (t2) {
         // BaseCorr2::process(f1,f2) early exit, src/Corr2.cpp:215-231
         double s1 = std::sqrt(t1.sizesq), s2 = std::sqrt(t2->sizesq);
-        double rsq = Corr2Engine<D, C>::dist_sq(t1.center, t2->center);
+        double rsq = Corr2Engine<D, C>::dist_sq(t1.center, t2->center, s1, s2);
         double s1ps2 = s1 + s2;
         bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep && rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
         bool too_large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
@@ -553,6 +575,7 @@ void tco_field_free(void* h) { delete static_cast<Tree*>(h); }
 void tco_set_periodic(double xp, double yp, double zp) {
     g_period[0] = xp; g_period[1] = yp; g_period[2] = zp;
 }
+void tco_set_arc(int on) { g_arc = on; }
 long tco_field_nnodes(void* h) { return long(static_cast<Tree*>(h)->nodes.size()); }
 long tco_field_ntop(void* h) { return long(static_cast<Tree*>(h)->top.size()); }
 double tco_field_sizesq(void* h) { return static_cast<Tree*>(h)->sizesq; }

@@ -48,9 +48,9 @@ class Field:
 
 
 def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, d2=None,
-          period=None):
+          period=None, arc=False):
     """d2 != d: the mixed cross correlations (f1 the d side); period (xp, yp, zp): the
-    Periodic metric."""
+    Periodic metric; arc: the Arc metric."""
     out = {k: np.zeros(nbins) for k in KEYS2}
     d2 = d if d2 is None else d2
     nxi = (0 if d == 0 else 1 if d == 1 else 4) if d2 == d else (2 if d2 >= 2 else 1)
@@ -62,7 +62,7 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, d2=No
                                  1.7976931348623157e308,
                                  *(period if period is not None else (0., 0., 0.)), *xi,
                                  *[_lib.dptr(out[k]) for k in KEYS2[4:]], ctypes.byref(h)))
-    metric = 6 if period is not None else 1
+    metric = 6 if period is not None else 4 if arc else 1
     try:
         if f2 is None:
             _lib.check(L.tc_corr2_process_auto(h, f1.h, 0, int(quick), metric))

@@ -51,6 +51,16 @@ def make_catalog(coords, n, seed, clumps=0.0, extent=None):
     return dict(x=x, y=y, z=z, w=w, k=k, g1=g1, g2=g2), scale
 
 
+def make_shell(n, seed, clumps=0.0):
+    """3-d positions: a spherical catalog's directions at radii uniform in [50, 150]
+    (for the Arc metric, whose separations are angles)."""
+    cat, scale = make_catalog(SPHERE, n, seed, clumps)
+    r = np.random.default_rng(seed + 1).uniform(50., 150., n)
+    for key in ('x', 'y', 'z'):
+        cat[key] = cat[key] * r
+    return cat, scale
+
+
 def checksum(cat):
     h = hashlib.sha256()
     for key in ('x', 'y', 'z', 'w', 'k', 'g1', 'g2'):

@@ -342,17 +342,80 @@ def test_gpu_periodic_matches_oracle(gpu, oracle_lib, d, coords, cross):
 
 @pytest.mark.gpu
 def test_gpu_periodic_front_end(gpu, oracle_lib):
-    """NNCorrelation(metric='Periodic', period=L) through the public class: the pair
-    counts of the periodic box equal the oracle's."""
+    """NNCorrelation(metric='Periodic', period=L) through the public class: trees to the
+    leaves (corr2base.py:1620-1624) and the pair counts of the periodic box equal the
+    oracle's."""
     O = oracle_lib
     cat, scale = C.make_catalog(C.THREED, 20000, 141, 0.3)
     tcat = gpu.Catalog(x=cat['x'], y=cat['y'], z=cat['z'])
-    nn = gpu.NNCorrelation(nbins=10, min_sep=scale * 0.01, max_sep=scale * 0.3, bin_slop=0.,
+    nn = gpu.NNCorrelation(nbins=10, min_sep=scale * 0.01, max_sep=scale * 0.3,
                            metric='Periodic', period=scale, min_top=4)
     nn.process(tcat)
     f = O.Field(0, C.THREED, cat['x'], cat['y'], cat['z'], min_size=0., max_size=0., min_top=4)
     ref = O.corr2(0, C.THREED, f, None, nn.nbins, nn._min_sep, nn._max_sep, nn._bin_size,
-                  0., 1., exact=True, period=(scale, scale, scale))
+                  nn.b, nn.angle_slop, exact=True, period=(scale, scale, scale))
     np.testing.assert_array_equal(nn.npairs, ref['npairs'])
     with pytest.raises(ValueError):
         gpu.NNCorrelation(nbins=10, min_sep=1., max_sep=10., metric='Periodic').process(tcat)
+    with pytest.raises(ValueError):
+        gpu.NNCorrelation(nbins=10, min_sep=1., max_sep=10., period=5.).process(tcat)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d,coords,cross', [(0, 3, False), (4, 3, True), (1, 3, True),
+                                            (0, 2, True), (4, 2, False), (1, 2, False)])
+def test_gpu_arc_matches_oracle(gpu, oracle_lib, d, coords, cross):
+    """The Arc metric on the GPU against the oracle (itself pinned to the compiled
+    reference): npairs and the node-pair counters exact, the sums to RTOL."""
+    import capi
+    O = oracle_lib
+    mk = C.make_shell if coords == C.THREED else \
+        (lambda n, s, cl: C.make_catalog(C.SPHERE, n, s, cl))
+    cat, scale = mk(30000, 150 + d + coords, 0.3)
+    cat2, _ = mk(25000, 160 + d + coords, 0.3) if cross else (None, 0)
+    bn = C.binning(scale, 15, 0.003, 0.5, 0.5, 0.1 if d == 4 else 1.0)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    f1 = capi.Field(d, coords, cat, brute=False, **kw)
+    f2 = capi.Field(d, coords, cat2, brute=False, **kw) if cross else None
+    out = capi.corr2(d, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                     bn['b'], bn['a'], arc=True)
+
+    def ofield(c):
+        return O.Field(d, coords, c['x'], c['y'], c['z'], k=c['k'], g1=c['g1'], g2=c['g2'],
+                       w=c['w'], **kw)
+    ref = O.corr2(d, coords, ofield(cat), ofield(cat2) if cross else None, bn['nbins'],
+                  bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'], exact=True,
+                  arc=True)
+    np.testing.assert_array_equal(out['npairs'], ref['npairs'])
+    assert ref['npairs'].sum() > 1e6
+    for k in ['meanr', 'meanlogr', 'weight'] + (['xi0'] if d else []):
+        assert normwise(out[k], ref[k]) < RTOL, k
+    for k in ('visits', 'accepted', 'leafpairs', 'refined'):
+        assert out['counters'][k] == ref['counters'][k], k
+
+
+@pytest.mark.gpu
+def test_gpu_arc_front_end(gpu, oracle_lib):
+    """NNCorrelation/GGCorrelation(metric='Arc') on 3-d positions with sep_units: the
+    pair counts and xi of the public classes equal the oracle's Arc run on trees to the
+    leaves, and meanr is the mean angle in the requested units."""
+    O = oracle_lib
+    cat, scale = C.make_shell(20000, 171, 0.3)
+    tcat = gpu.Catalog(x=cat['x'], y=cat['y'], z=cat['z'], g1=cat['g1'], g2=cat['g2'],
+                       w=cat['w'])
+    deg = np.pi / 180.
+    gg = gpu.GGCorrelation(nbins=10, min_sep=scale * 0.01 / deg, max_sep=scale * 0.4 / deg,
+                           sep_units='deg', metric='Arc', min_top=4)
+    gg.process(tcat)
+    f = O.Field(4, C.THREED, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
+                w=cat['w'], min_size=0., max_size=0., min_top=4)
+    ref = O.corr2(4, C.THREED, f, None, gg.nbins, gg._min_sep, gg._max_sep, gg._bin_size,
+                  gg.b, gg.angle_slop, exact=True, arc=True)
+    np.testing.assert_array_equal(gg.npairs, ref['npairs'])
+    assert normwise(gg.weight, ref['weight']) < RTOL
+    m = ref['weight'] > 0
+    np.testing.assert_allclose(gg.meanr[m], ref['meanr'][m] / ref['weight'][m] / deg,
+                               rtol=1e-12)
+    with pytest.raises(ValueError):
+        fcat = gpu.Catalog(x=cat['x'], y=cat['y'])
+        gpu.NNCorrelation(nbins=10, min_sep=1., max_sep=10., metric='Arc').process(fcat)

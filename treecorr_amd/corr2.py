@@ -21,8 +21,8 @@ from .field import build_patch_fields, coord_enum
 from .util import set_omp_threads
 
 LOG = 0
-EUCLIDEAN, PERIODIC = 1, 6          # include/Metric.h:29
-_METRIC_ENUM = {'Euclidean': EUCLIDEAN, 'Periodic': PERIODIC}
+EUCLIDEAN, ARC, PERIODIC = 1, 4, 6          # include/Metric.h:29
+_METRIC_ENUM = {'Euclidean': EUCLIDEAN, 'Arc': ARC, 'Periodic': PERIODIC}
 
 
 class _PairSums:
@@ -199,10 +199,13 @@ class Corr2:
     def _set_metric(self, metric, coords1, coords2=None):
         metric = metric or self.config.get('metric', 'Euclidean')
         if metric not in _METRIC_ENUM:
-            raise NotImplementedError("treecorr_amd implements the Euclidean and Periodic "
-                                      "metrics")
+            raise NotImplementedError("treecorr_amd implements the Euclidean, Arc and "
+                                      "Periodic metrics")
+        if metric == 'Arc' and coords1 not in ('spherical', '3d'):
+            # util.py:340-341
+            raise ValueError("Arc metric is only valid for catalogs with spherical positions.")
         if metric == 'Periodic':
-            # corr2base.py:1576-1584 (coords.py): a box in flat or 3-d coordinates
+            # corr2base.py:1566-1571: a box in flat or 3-d coordinates
             if coords1 == 'spherical':
                 raise ValueError("Periodic metric is not valid for spherical coordinates")
             if self.xperiod <= 0 or self.yperiod <= 0 or (coords1 == '3d' and
@@ -212,13 +215,15 @@ class Corr2:
             raise ValueError("period options are not valid for %s metric." % metric)
         if coords2 is not None and coords1 != coords2:
             raise ValueError("Cannot correlate catalogs with different coordinate systems.")
-        if self.sep_units != '' and coords1 == '3d':
+        if self.sep_units != '' and coords1 == '3d' and metric != 'Arc':
             raise ValueError("sep_units is invalid with 3d coordinates. ")
         self.coords = coords1
         self.metric = metric
 
     def _get_minmax_size(self):
-        # corr2base.py:1595-1625 (Log, Euclidean)
+        # corr2base.py:1595-1625 (Log): the other metrics go all the way to the leaves
+        if self.metric not in (None, 'Euclidean'):     # None: not yet set, the default
+            return 0., 0.
         b1 = min(self.angle_slop, self.b)
         min_size = self._min_sep * b1 / (2. + 3. * b1)
         max_size = self._max_sep * b1

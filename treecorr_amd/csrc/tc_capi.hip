@@ -393,7 +393,7 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
     P.logminsep = std::log(minsep); P.halfminsep = 0.5 * minsep;
     P.minsepsq = minsep * minsep; P.maxsepsq = maxsep * maxsep;
     P.bsq = b * b; P.asq = a * a;
-    P.periodic = 0; P.xp = xp; P.yp = yp; P.zp = zp;   // used with the Periodic metric
+    P.metric = MetEuclidean; P.xp = xp; P.yp = yp; P.zp = zp;   // periods: Periodic metric
     c->xi[0] = xi0; c->xi[1] = xi1; c->xi[2] = xi2; c->xi[3] = xi3;
     c->meanr = meanr; c->meanlogr = meanlogr; c->weight = weight; c->npairs = npairs;
     *out = c;
@@ -401,19 +401,25 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
 }
 
 // The metric of a process call (src/Corr2.cpp:886-993 dispatch on Metric, enum at
-// include/Metric.h:29): Euclidean (1), or Periodic (6) with the correlation's periods,
-// which wrap every separation (flat and 3-d coordinates).
+// include/Metric.h:29): Euclidean (1); Arc (4), the great-circle angle, for spherical
+// and 3-d coordinates (include/Metric.h:654-655); or Periodic (6) with the
+// correlation's periods, which wrap every separation (flat and 3-d coordinates).
 static int set_metric(const tc_corr2* c, int metric, int coords, C2Params& P) {
-    P.periodic = 0;
-    if (metric == 1) return 0;
-    if (metric == 6) {
+    P.metric = MetEuclidean;
+    if (metric == MetEuclidean) return 0;
+    if (metric == MetArc) {
+        if (coords == Flat) return tc::fail("the Arc metric needs spherical or 3-d coordinates");
+        P.metric = MetArc;
+        return 0;
+    }
+    if (metric == MetPeriodic) {
         if (coords == Sphere) return tc::fail("the Periodic metric needs flat or 3-d coordinates");
         if (!(c->P.xp > 0.) || !(c->P.yp > 0.) || (coords == ThreeD && !(c->P.zp > 0.)))
             return tc::fail("the Periodic metric needs positive periods");
-        P.periodic = 1;
+        P.metric = MetPeriodic;
         return 0;
     }
-    set_error("only the Euclidean and Periodic metrics are implemented");
+    set_error("only the Euclidean, Arc and Periodic metrics are implemented");
     return TC_ERR_UNSUPPORTED;
 }
 
@@ -423,13 +429,10 @@ static bool fields_out_of_range(const C2Params& P, int coords, const tc_field* f
                                 const tc_field* f2) {
     const FieldInfo& a = f1->T.info;
     const FieldInfo& b = f2->T.info;
-    double dx = a.cx - b.cx, dy = a.cy - b.cy, dz = a.cz - b.cz;
-    if (P.periodic) {
-        dx = wrap_period(dx, P.xp); dy = wrap_period(dy, P.yp);
-        if (coords != Flat) dz = wrap_period(dz, P.zp);
-    }
-    double rsq = coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
-    double s1ps2 = std::sqrt(a.sizesq) + std::sqrt(b.sizesq);
+    double s1 = std::sqrt(a.sizesq), s2 = std::sqrt(b.sizesq);
+    const double rsq = metric_dist_sq(P.metric, coords, a.cx, a.cy, a.cz, b.cx, b.cy, b.cz,
+                                      P.xp, P.yp, P.zp, s1, s2);
+    const double s1ps2 = s1 + s2;
     bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
                      rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
     bool too_large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
@@ -576,13 +579,9 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
         };
         norm(x1, y1, z1); norm(x2, y2, z2);
     }
-    double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
-    if (P.periodic) {
-        dx = wrap_period(dx, P.xp); dy = wrap_period(dy, P.yp);
-        if (coords != Flat) dz = wrap_period(dz, P.zp);
-    }
-    double rsq = coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
-    double s1ps2 = s1 + s2;
+    const double rsq = metric_dist_sq(P.metric, coords, x1, y1, z1, x2, y2, z2, P.xp, P.yp,
+                                      P.zp, s1, s2);
+    const double s1ps2 = s1 + s2;
     *result = (rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2)) ? 1 : 0;
     return 0;
 }

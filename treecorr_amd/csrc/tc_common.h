@@ -149,6 +149,40 @@ __host__ __device__ inline double wrap_period(double d, double p) {
     return d;
 }
 
+// The metrics of a 2-point run, values of the reference's enum (include/Metric.h:29).
+enum Metric { MetEuclidean = 1, MetArc = 4, MetPeriodic = 6 };
+
+// MetricHelper<M>::DistSq (include/Metric.h) for the metrics above:
+//   Euclidean  the squared norm of the separation (:143-147, :197-202);
+//   Arc        on the sphere (2 asin(L/2))^2, L the chord (:666-680); in 3-d
+//              atan2(|a x b|, a.b)^2, the sizes divided by |a|, |b| to put them on the
+//              unit sphere (:701-721);
+//   Periodic   each separation wrapped by its period, then the squared norm (:831-838).
+__host__ __device__ __forceinline__ double metric_dist_sq(int metric, int coords, double ax,
+                                                          double ay, double az, double bx,
+                                                          double by, double bz, double xp,
+                                                          double yp, double zp, double& s1,
+                                                          double& s2) {
+    if (metric == MetArc) {
+        if (coords == Sphere) {
+            const double dx = ax - bx, dy = ay - by, dz = az - bz;
+            const double th = 2. * asin(sqrt(dx * dx + dy * dy + dz * dz) / 2.);
+            return th * th;
+        }
+        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const double th = atan2(sqrt(cx * cx + cy * cy + cz * cz), ax * bx + ay * by + az * bz);
+        s1 /= sqrt(ax * ax + ay * ay + az * az);
+        s2 /= sqrt(bx * bx + by * by + bz * bz);
+        return th * th;
+    }
+    double dx = ax - bx, dy = ay - by, dz = az - bz;
+    if (metric == MetPeriodic) {
+        dx = wrap_period(dx, xp); dy = wrap_period(dy, yp);
+        if (coords != Flat) dz = wrap_period(dz, zp);
+    }
+    return coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
+}
+
 // calculate_expmsialpha<S>, include/ProjectHelper.h:37-61: exp(-i S alpha) of the
 // direction r for spins 0..4 (S = 2 above).
 template <int S>

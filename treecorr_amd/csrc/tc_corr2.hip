@@ -50,16 +50,6 @@ __device__ __forceinline__ double dist_sq(const DNode& a, const DNode& b) {
     return dx * dx + dy * dy + dz * dz;
 }
 
-// MetricHelper<Periodic>::DistSq, include/Metric.h:831-838 (and its ThreeD form): the
-// separation wrapped by the periods, then the Euclidean norm
-template <int C>
-__device__ __forceinline__ double dist_sq_periodic(const DNode& a, const DNode& b,
-                                                   const C2Params& P) {
-    const double dx = wrap_period(a.x - b.x, P.xp), dy = wrap_period(a.y - b.y, P.yp);
-    if (C == Flat) return dx * dx + dy * dy;
-    const double dz = wrap_period(a.z - b.z, P.zp);
-    return dx * dx + dy * dy + dz * dz;
-}
 
 #ifndef TC_FAST_LOG
 #define TC_FAST_LOG 1
@@ -156,7 +146,9 @@ __device__ __forceinline__ void cmul(double& zr, double& zi, double er, double e
 // (src/Corr2.cpp:394-566) for one accepted pair, accumulated into LDS.
 // hist and hnp point at this lane's copy of the workgroup histogram; copies are
 // interleaved with stride R (entry e of copy c lives at e*R + c).
-template <int D, int C>
+// GM: rsq comes from a metric other than Euclidean (the Arc metric's angle squared), so
+// the sphere projections take the chord length from the positions.
+template <int D, int C, bool GM>
 __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, double rsq, int k,
                                          double r, double logr, const C2Params& P,
                                          double* hist, unsigned long long* hnp, int R) {
@@ -187,7 +179,8 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
         } else {
             // ProjectHelper<Sphere>::Project :370-388; ThreeD normalises first :573-589
             double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
-            double dsq = rsq;    // Sphere positions are unit vectors already
+            // Sphere positions are unit vectors already: the chord is their separation
+            double dsq = GM ? dist_sq<Sphere>(c1, c2) : rsq;
             if (C == ThreeD) {
                 normalize_recip(x1, y1, z1);
                 normalize_recip(x2, y2, z2);
@@ -220,7 +213,7 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
             cmul(g2r, g2i, er, ei);
         } else {
             double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
-            double dsq = rsq;
+            double dsq = GM ? dist_sq<Sphere>(c1, c2) : rsq;
             if (C == ThreeD) {
                 normalize_recip(x1, y1, z1);
                 normalize_recip(x2, y2, z2);
@@ -252,7 +245,7 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
             expmsialpha<S>(c2.x - c1.x, c2.y - c1.y, er, ei);
         } else {
             double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
-            double dsq = rsq;
+            double dsq = GM ? dist_sq<Sphere>(c1, c2) : rsq;
             if (C == ThreeD) {
                 normalize_recip(x1, y1, z1);
                 normalize_recip(x2, y2, z2);
@@ -294,7 +287,7 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
             expm2ialpha_proj(c2.x - c1.x, c2.y - c1.y, er, ei);
         } else {
             double x1 = c1.x, y1 = c1.y, z1 = c1.z, x2 = c2.x, y2 = c2.y, z2 = c2.z;
-            double dsq = rsq;
+            double dsq = GM ? dist_sq<Sphere>(c1, c2) : rsq;
             if (C == ThreeD) {
                 normalize_recip(x1, y1, z1);
                 normalize_recip(x2, y2, z2);
@@ -364,7 +357,9 @@ __device__ __forceinline__ int pair_slot(int p, int* tags) {
 // for its children with one LDS atomic, so the waves of a workgroup never wait for
 // each other; the order of the children in a region does not matter.  No global
 // atomics: the regions are concatenated afterwards.
-template <int D, int C, class IT>
+// GM: the Arc and Periodic metrics (metric_dist_sq); the Euclidean instantiations
+// keep the plain distance, whose kernels need far fewer registers.
+template <int D, int C, class IT, bool GM>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
     const DNode* __restrict__ n1, const DNode* __restrict__ n2, InView<IT> V,
     long long nin, long long per_block, IT* __restrict__ out,
@@ -453,9 +448,11 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 // step 1: pruning and singleBin's cheap tests (include/BinType.h:93-151)
                 int cls = CL_NONE;
                 double rsq = 0., s1ps2 = 0., s1ps2sq = 0.;
-                const double s1 = c1.size, s2 = c2.size;
+                double s1 = c1.size, s2 = c2.size;    // the Arc metric rescales them in 3-d
                 if (c1.w != 0.f && c2.w != 0.f) {
-                    rsq = P.periodic ? dist_sq_periodic<C>(c1, c2, P) : dist_sq<C>(c1, c2);
+                    rsq = GM ? metric_dist_sq(P.metric, C, c1.x, c1.y, c1.z, c2.x, c2.y,
+                                              c2.z, P.xp, P.yp, P.zp, s1, s2)
+                             : dist_sq<C>(c1, c2);
                     s1ps2 = s1 + s2;
                     const bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
                                            rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
@@ -499,15 +496,15 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                     ++c_acc;
                     if (c1.child < 0 && c2.child < 0) ++c_leaf;
                     if (!Ops::batched) {
-                        bin_pair<D, C>(c1, c2, rsq, k, r, logr, P, hist, hnp, R);
+                        bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P, hist, hnp, R);
                     } else {
                         const int pp = Ops::pair(it);
                         const int sl = pair_slot(pp, s_tag);
                         if (sl >= 0)
-                            bin_pair<D, C>(c1, c2, rsq, k, r, logr, P, hist + sl * per_slot,
+                            bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P, hist + sl * per_slot,
                                            hnp + sl * per_slot, R);
                         else   // all LDS slots taken: straight into the pair's HBM sums
-                            bin_pair<D, C>(c1, c2, rsq, k, r, logr, P,
+                            bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P,
                                            pair_hist + (size_t)pp * 2 * nh,
                                            pair_np + (size_t)pp * nb, 1);
                     }
@@ -896,8 +893,8 @@ using TravFn = void (*)(const DNode*, const DNode*, InView<IT>, long long, long 
                         unsigned long long*, double*, unsigned long long*);
 
 template <class IT>
-static TravFn<IT> pick_traverse(int D, int C) {
-#define TC_TF(DD, CC) return &k_traverse<DD, CC, IT>
+static TravFn<IT> pick_traverse(int D, int C, bool gm) {
+#define TC_TF(DD, CC) return gm ? &k_traverse<DD, CC, IT, true> : &k_traverse<DD, CC, IT, false>
     if (D == NData) { if (C == Flat) TC_TF(NData, Flat); if (C == ThreeD) TC_TF(NData, ThreeD); TC_TF(NData, Sphere); }
     if (D == KData) { if (C == Flat) TC_TF(KData, Flat); if (C == ThreeD) TC_TF(KData, ThreeD); TC_TF(KData, Sphere); }
     if (D == XK_NK) { if (C == Flat) TC_TF(XK_NK, Flat); if (C == ThreeD) TC_TF(XK_NK, ThreeD); TC_TF(XK_NK, Sphere); }
@@ -1038,7 +1035,7 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
         else hipLaunchKernelGGL(k_items_cross, g, dim3(256), 0, st, dtop1, ntop1, dtop2, ntop2, stack);
         TC_HIP(hipGetLastError());
     }
-    if (int rc = traverse_stack<Item>(S, pick_traverse<Item>(D, C), nodes1,
+    if (int rc = traverse_stack<Item>(S, pick_traverse<Item>(D, C, P.metric != MetEuclidean), nodes1,
                                       autoc ? nodes1 : nodes2, ninit, P, R, shbytes, nullptr,
                                       nullptr))
         return rc;
@@ -1185,7 +1182,7 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
                            static_cast<ItemP*>(S.stack));
         TC_HIP(hipGetLastError());
     }
-    if (int rc = traverse_stack<ItemP>(S, pick_traverse<ItemP>(D, C), nodes1, nodes2, ninit, P,
+    if (int rc = traverse_stack<ItemP>(S, pick_traverse<ItemP>(D, C, P.metric != MetEuclidean), nodes1, nodes2, ninit, P,
                                        R, shbytes, ph, pn))
         return rc;
     // counters from the workgroup slots

@@ -48,8 +48,8 @@ struct C2Params {
     int nbins, quick;
     double minsep, maxsep, binsize, b, a;
     double logminsep, halfminsep, minsepsq, maxsepsq, bsq, asq;
-    int periodic;               // MetricHelper<Periodic>: separations wrapped by the periods
-    double xp, yp, zp;
+    int metric;                 // Metric: Euclidean, Arc or Periodic
+    double xp, yp, zp;          // the Periodic metric's periods
 };
 
 // Lloyd iterations of k-means over the objects (centers in/out, npatch x dim).
