@@ -31,7 +31,8 @@ extern "C" {
  *   data types  N=0, K=1, Z=2, V=3, G=4, T=5, Q=6 (include/Cell.h:34 DataType)
  *   coords      Flat=1, ThreeD=2, Sphere=3 (include/Position.h:25)
  *   bin types   Log=0, LogMultipole=5    (include/BinType.h:35)
- *   metrics     Euclidean=1, Arc=4, Periodic=6 (include/Metric.h:29)
+ *   metrics     Euclidean=1, Rperp=2, Rlens=3, Arc=4, OldRperp=5, Periodic=6
+ *               (include/Metric.h:29)
  *   split       Middle=0, Median=1, Mean=2, Random=3 (include/Cell.h:26) */
 
 typedef struct tc_field tc_field;
@@ -99,7 +100,10 @@ int tc_field_destroy(tc_field* f);
  * GG (4,4), TT (5,5), QQ (6,6); or the mixed N or K (0 or 1) with K, Z, V, G, T or Q
  * (src/Corr2.cpp DirectHelper<1>/<2>, :408-442; cross only, f1/fields1 the d1 side);
  * bin_type Log.  xp, yp, zp are the periods of the
- * Periodic metric (0 when unused).  The process calls take metric 1 (Euclidean), 4 (Arc,
+ * Periodic metric (0 when unused).  minrpar/maxrpar: the r_par range (3-d only; -DBL_MAX
+ * and DBL_MAX for none, BaseCorr2::nontrivialRPar, include/Corr2.h:58-62; ParHelper,
+ * include/Metric.h:37-106).  The process calls take metric 1 (Euclidean), 2 (Rperp), 3
+ * (Rlens) or 5 (OldRperp) (3-d only, include/Metric.h:268-651), 4 (Arc,
  * spherical or 3-d coordinates, include/Metric.h:656-810: the great-circle angle; in 3-d
  * atan2(|p1 x p2|, p1.p2) with the sizes scaled by 1/|p|) or 6 (Periodic, flat or 3-d
  * coordinates, include/Metric.h:813-838: every separation wrapped into [-p/2, p/2]).  xi arrays: KK and NK xi0; the other mixed kinds xi0 (xi) and xi1

@@ -41,6 +41,8 @@ def lib():
         L.tco_field_free.argtypes = [vp]
         L.tco_set_periodic.argtypes = [ctypes.c_double] * 3
         L.tco_set_arc.argtypes = [ctypes.c_int]
+        L.tco_set_metric3.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double]
         L.tco_field_nnodes.argtypes = [vp]
         L.tco_field_nnodes.restype = ctypes.c_long
         L.tco_field_ntop.argtypes = [vp]
@@ -120,12 +122,13 @@ MIXED = {(0, 1): 8, (0, 4): 9, (1, 4): 10, (0, 2): 11, (1, 2): 12, (0, 3): 13, (
 
 
 def corr2(d, coords, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, exact=True,
-          nthreads=0, row_stride=1, d2=None, period=None, arc=False):
+          nthreads=0, row_stride=1, d2=None, period=None, arc=False, metric3=0, rpar=None):
     """Run the oracle's 2-point traversal; returns dict of arrays and counters.
 
     d2 (the second field's type) differs from d for NK, NG and KG (cross only).
     row_stride > 1 processes only every row_stride-th top-level row (timing samples).
-    period (xp, yp, zp) selects the Periodic metric, arc=True the Arc metric."""
+    period (xp, yp, zp) selects the Periodic metric, arc=True the Arc metric, metric3 the
+    Rperp (2), Rlens (3) or OldRperp (5) metric; rpar (min, max) the r_par range."""
     out = {k: np.zeros(nbins) for k in
            ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')}
     cnt = (ctypes.c_longlong * 4)()
@@ -135,6 +138,8 @@ def corr2(d, coords, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=Fals
         d = MIXED[(d, d2)]
     lib().tco_set_periodic(*(period if period is not None else (0., 0., 0.)))
     lib().tco_set_arc(int(bool(arc)))
+    lib().tco_set_metric3(int(metric3), int(rpar is not None),
+                          *(rpar if rpar is not None else (0., 0.)))
     rc = lib().tco_corr2_sampled(f1.h, f2.h if f2 is not None else None, d, coords, nbins,
                                  float(min_sep), float(max_sep), float(bin_size), float(b),
                                  float(a), int(quick), int(exact), int(nthreads),
@@ -143,6 +148,7 @@ def corr2(d, coords, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=Fals
                                  int(row_stride))
     lib().tco_set_periodic(0., 0., 0.)
     lib().tco_set_arc(0)
+    lib().tco_set_metric3(0, 0, 0., 0.)
     if rc != 0:
         raise RuntimeError('oracle corr2 failed')
     out['counters'] = dict(visits=cnt[0], accepted=cnt[1], leafpairs=cnt[2], refined=cnt[3])

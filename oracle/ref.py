@@ -120,7 +120,7 @@ def make_field(d, coords, x, y, z=None, k=None, g1=None, g2=None, w=None, wpos=N
 
 
 def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthreads=None,
-          d2=None, period=None, arc=False):
+          d2=None, period=None, arc=False, metric3=0, rpar=None):
     """Run the reference 2-point engine (auto when f2 is None).  Returns raw sums.
     d2 != d selects NK, NG or KG (treecorr/nkcorrelation.py, ngcorrelation.py,
     kgcorrelation.py: xi, and xi_im for a G second field)."""
@@ -136,10 +136,13 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthre
     xi = [xi[i] if i < nxi else e for i in range(4)]
     name = _CORR_MIXED[(d, d2)] if mixed else _CORR[d]
     xp, yp, zp = period if period is not None else (0., 0., 0.)
-    metric = m.Periodic if period is not None else m.Arc if arc else m.Euclidean
+    metric = m.Periodic if period is not None else m.Arc if arc else \
+        {0: m.Euclidean, 2: m.Rperp, 3: m.Rlens, 5: m.OldRperp}[metric3]
+    minrpar, maxrpar = rpar if rpar is not None else (-1.7976931348623157e308,
+                                                      1.7976931348623157e308)
     corr = getattr(m, name)(m.Log, float(min_sep), float(max_sep), int(nbins),
-                            float(bin_size), float(b), float(a), -1.7976931348623157e308,
-                            1.7976931348623157e308, float(xp), float(yp), float(zp), *xi,
+                            float(bin_size), float(b), float(a), float(minrpar),
+                            float(maxrpar), float(xp), float(yp), float(zp), *xi,
                             out['meanr'], out['meanlogr'], out['weight'], out['npairs'])
     if f2 is None:
         corr.processAuto(f1, False, bool(quick), metric)

@@ -296,6 +296,67 @@ static double g_period[3] = {0., 0., 0.};
 // The Arc metric for the next 2-point runs (tco_set_arc); MetricHelper<Arc>,
 // include/Metric.h:656-810.
 static int g_arc = 0;
+// The line-of-sight metrics for the next 2-point runs (tco_set_metric3): Rperp (2),
+// Rlens (3), OldRperp (5), include/Metric.h:268-651; and the r_par range (ParHelper,
+// include/Metric.h:37-106), used when g_rpar_on.
+static int g_metric3 = 0;
+static int g_rpar_on = 0;
+static double g_minrpar = 0., g_maxrpar = 0.;
+static inline double dot3(const Pt& a, const Pt& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline Pt cross3(const Pt& a, const Pt& b) {
+    return Pt{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// ParHelper<1>::calculateRPar (r along the midpoint's direction); ParHelper<2> (OldRperp)
+// the difference of the distances
+static double calc_rpar(const Pt& a, const Pt& b) {
+    if (g_metric3 == 5) return std::sqrt(dot3(b, b)) - std::sqrt(dot3(a, a));
+    Pt r{b.x - a.x, b.y - a.y, b.z - a.z};
+    Pt L{(a.x + b.x) * 0.5, (a.y + b.y) * 0.5, (a.z + b.z) * 0.5};
+    return dot3(r, L) / std::sqrt(dot3(L, L));
+}
+struct PruneOut { bool skip; bool rin; };
+// BaseCorr2::process11's pruning, src/Corr2.cpp:326-346: the r_par range, then the bin
+// type's too-small/too-large tests (given) refined by the metric's own.
+static PruneOut metric_prune(const Pt& a, const Pt& b, double rsq, double s1ps2, bool bt_small,
+                             bool bt_large, double minsep, double minsepsq, double maxsep,
+                             double maxsepsq) {
+    PruneOut o{false, true};
+    double rpar = 0.;
+    if (g_rpar_on) {
+        rpar = calc_rpar(a, b);
+        if (rpar + s1ps2 < g_minrpar || rpar - s1ps2 > g_maxrpar) { o.skip = true; return o; }
+    }
+    bool small = bt_small, large = bt_large;
+    if (g_metric3 == 2) {                  // MetricHelper<Rperp>, Metric.h:458-476
+        Pt L{(a.x + b.x) * 0.5, (a.y + b.y) * 0.5, (a.z + b.z) * 0.5};
+        double nL = dot3(L, L);
+        if (small) {
+            if (rsq < nL) small = true;
+            else if (s1ps2 * s1ps2 > 4. * nL) small = false;
+            else {
+                double twoL = 2. * std::sqrt(nL);
+                double t = minsep * (1 - s1ps2 / twoL) - s1ps2;
+                small = rsq < t * t;
+            }
+        }
+        if (large) {
+            if (rsq < nL) large = true;
+            else {
+                double twoL = 2. * std::sqrt(nL);
+                double t = maxsep * (1 + s1ps2 / twoL) + s1ps2;
+                large = rsq > t * t;
+            }
+        }
+    } else if (g_metric3 == 5 && (small || large)) {   // MetricHelper<OldRperp>, :300-318
+        if (rpar == 0.) rpar = std::sqrt(dot3(b, b)) - std::sqrt(dot3(a, a));
+        double d3 = std::sqrt(rpar * rpar + rsq);
+        small = small && rsq + 2. * (d3 + std::fabs(rpar)) * s1ps2 < minsepsq;
+        large = large && rsq - 2. * (d3 + std::fabs(rpar)) * s1ps2 > maxsepsq;
+    }
+    if (small || large) { o.skip = true; return o; }
+    if (g_rpar_on) o.rin = !(rpar - s1ps2 < g_minrpar) && !(rpar + s1ps2 > g_maxrpar);
+    return o;
+}
 static inline double wrap1(double d, double p) {
     while (d > p / 2.) d -= p;
     while (d < -p / 2.) d += p;
@@ -324,6 +385,35 @@ struct Corr2Engine {
     // MetricHelper<M>::DistSq; the Arc metric's 3-d form also scales the sizes to the
     // unit sphere
     static double dist_sq(const Pt& a, const Pt& b, double& s1, double& s2) {
+        if (g_metric3 == 2) {     // MetricHelper<Rperp>::DistSq, include/Metric.h:416-434
+            Pt c = cross3(a, b);
+            Pt L{(a.x + b.x) * 0.5, (a.y + b.y) * 0.5, (a.z + b.z) * 0.5};
+            double nL = dot3(L, L);
+            double rperpsq = nL > 0. ? dot3(c, c) / nL : 4. * dot3(a, a);
+            double r1sq = dot3(a, a), r2sq = dot3(b, b);
+            if (r2sq > nL && s1 != 0) s1 *= std::sqrt(r2sq / nL);
+            if (r1sq > nL && s2 != 0) s2 *= std::sqrt(r1sq / nL);
+            return rperpsq;
+        }
+        if (g_metric3 == 5) {     // MetricHelper<OldRperp>::DistSq, include/Metric.h:277-293
+            Pt r{a.x - b.x, a.y - b.y, a.z - b.z};
+            double d3sq = dot3(r, r), r1sq = dot3(a, a), r2sq = dot3(b, b);
+            double rparsq = (r1sq - r2sq) * (r1sq - r2sq) / (r1sq + r2sq + 2. * std::sqrt(r1sq * r2sq));
+            const double big = std::numeric_limits<double>::max();
+            if (r1sq < r2sq) {
+                if (s1 != 0. && s1 < big) s1 *= (1. + 0.25 * (r2sq - r1sq) / r1sq);
+            } else {
+                if (s2 != 0. && s2 < big) s2 *= (1. + 0.25 * (r1sq - r2sq) / r2sq);
+            }
+            return std::fabs(d3sq - rparsq);
+        }
+        if (g_metric3 == 3) {     // MetricHelper<Rlens>::DistSq, include/Metric.h:582-589
+            Pt c = cross3(a, b);
+            double r2sq = dot3(b, b);
+            double rsq = dot3(c, c) / r2sq;
+            s2 *= std::sqrt(dot3(a, a) / r2sq);
+            return rsq;
+        }
         if (g_arc) {
             if (C == Sphere) {      // Metric.h:666-680: theta = 2 asin(L/2), L the chord
                 Pt r{a.x - b.x, a.y - b.y, a.z - b.z};
@@ -448,11 +538,20 @@ struct Corr2Engine {
         double rsq = dist_sq(c1.p, c2.p, s1, s2);
         double s1ps2 = s1 + s2;
         // BinTypeHelper<Log>::tooSmallDist / tooLargeDist, BinType.h:70-82
-        if (rsq < P.minsepsq && s1ps2 < P.minsep &&
-            rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2)) return;
-        if (rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2)) return;
+        const bool bt_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
+                              rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
+        const bool bt_large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+        bool rin = true;
+        if (g_metric3 || g_rpar_on) {
+            PruneOut po = metric_prune(c1.p, c2.p, rsq, s1ps2, bt_small, bt_large, P.minsep,
+                                       P.minsepsq, P.maxsep, P.maxsepsq);
+            if (po.skip) return;
+            rin = po.rin;
+        } else if (bt_small || bt_large) {
+            return;
+        }
         int k = -1; double r = 0, logr = 0;
-        if (single_bin(rsq, s1ps2, k, r, logr, acc)) {
+        if (rin && single_bin(rsq, s1ps2, k, r, logr, acc)) {
             if (rsq >= P.minsepsq && rsq < P.maxsepsq) direct(c1, c2, rsq, k, r, logr, acc);
             return;
         }
@@ -511,6 +610,11 @@ static void run_corr2(const Params2& P, const Tree& t1, const Tree* t2, bool qui
         double s1ps2 = s1 + s2;
         bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep && rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
         bool too_large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+        if (g_metric3 || g_rpar_on) {
+            too_small = metric_prune(t1.center, t2->center, rsq, s1ps2, too_small, too_large,
+                                     P.minsep, P.minsepsq, P.maxsep, P.maxsepsq).skip;
+            too_large = false;
+        }
         if (!(too_small || too_large)) {
             const long n2 = long(t2->top.size());
 #pragma omp parallel for schedule(dynamic) num_threads(nt)
@@ -576,6 +680,9 @@ void tco_set_periodic(double xp, double yp, double zp) {
     g_period[0] = xp; g_period[1] = yp; g_period[2] = zp;
 }
 void tco_set_arc(int on) { g_arc = on; }
+void tco_set_metric3(int metric, int rpar_on, double minrpar, double maxrpar) {
+    g_metric3 = metric; g_rpar_on = rpar_on; g_minrpar = minrpar; g_maxrpar = maxrpar;
+}
 long tco_field_nnodes(void* h) { return long(static_cast<Tree*>(h)->nodes.size()); }
 long tco_field_ntop(void* h) { return long(static_cast<Tree*>(h)->top.size()); }
 double tco_field_sizesq(void* h) { return static_cast<Tree*>(h)->sizesq; }

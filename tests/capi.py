@@ -48,21 +48,22 @@ class Field:
 
 
 def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, d2=None,
-          period=None, arc=False):
+          period=None, arc=False, metric3=0, rpar=None):
     """d2 != d: the mixed cross correlations (f1 the d side); period (xp, yp, zp): the
-    Periodic metric; arc: the Arc metric."""
+    Periodic metric; arc: the Arc metric; metric3: Rperp (2), Rlens (3) or OldRperp (5);
+    rpar: the (min, max) r_par range."""
     out = {k: np.zeros(nbins) for k in KEYS2}
     d2 = d if d2 is None else d2
     nxi = (0 if d == 0 else 1 if d == 1 else 4) if d2 == d else (2 if d2 >= 2 else 1)
     xi = [_lib.dptr(out['xi%d' % i]) if i < nxi else None for i in range(4)]
     h = ctypes.c_void_p()
     L = _lib.lib()
+    mnr, mxr = rpar if rpar is not None else (-1.7976931348623157e308, 1.7976931348623157e308)
     _lib.check(L.tc_corr2_create(d, d2, 0, float(min_sep), float(max_sep), int(nbins),
-                                 float(bin_size), float(b), float(a), -1.7976931348623157e308,
-                                 1.7976931348623157e308,
+                                 float(bin_size), float(b), float(a), float(mnr), float(mxr),
                                  *(period if period is not None else (0., 0., 0.)), *xi,
                                  *[_lib.dptr(out[k]) for k in KEYS2[4:]], ctypes.byref(h)))
-    metric = 6 if period is not None else 4 if arc else 1
+    metric = 6 if period is not None else 4 if arc else metric3 if metric3 else 1
     try:
         if f2 is None:
             _lib.check(L.tc_corr2_process_auto(h, f1.h, 0, int(quick), metric))

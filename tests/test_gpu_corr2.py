@@ -192,7 +192,8 @@ def test_gpu_mixed_rejects_auto_and_wrong_sides(gpu):
     L = _lib.lib()
     xi0, xi1, m, ml, w, npr = (np.zeros(5) for _ in range(6))
     h = ctypes.c_void_p()
-    _lib.check(L.tc_corr2_create(0, 4, 0, 1., 10., 5, np.log(10.) / 5, 0.1, 1., -1e308, 1e308,
+    big = 1.7976931348623157e308     # no r_par range
+    _lib.check(L.tc_corr2_create(0, 4, 0, 1., 10., 5, np.log(10.) / 5, 0.1, 1., -big, big,
                                  0., 0., 0., _lib.dptr(xi0), _lib.dptr(xi1), None, None,
                                  _lib.dptr(m), _lib.dptr(ml), _lib.dptr(w), _lib.dptr(npr),
                                  ctypes.byref(h)))
@@ -419,3 +420,68 @@ def test_gpu_arc_front_end(gpu, oracle_lib):
     with pytest.raises(ValueError):
         fcat = gpu.Catalog(x=cat['x'], y=cat['y'])
         gpu.NNCorrelation(nbins=10, min_sep=1., max_sep=10., metric='Arc').process(fcat)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('m3,d,cross,rpar', [(2, 0, False, None), (2, 4, True, (-20., 30.)),
+                                             (5, 0, True, (-20., 30.)), (5, 4, False, None),
+                                             (3, 4, True, None), (3, 0, True, (-20., 30.)),
+                                             (0, 4, False, (-1., 2.))])
+def test_gpu_los_metrics_match_oracle(gpu, oracle_lib, m3, d, cross, rpar):
+    """Rperp (2), Rlens (3), OldRperp (5) and the r_par range (0: with the Euclidean
+    metric) on the GPU against the oracle (itself pinned to the compiled reference):
+    npairs and the node-pair counters exact, the sums to RTOL."""
+    import capi
+    O = oracle_lib
+    cat, _ = C.make_shell(30000, 220 + m3 + d, 0.3)
+    cat2, _ = C.make_shell(25000, 230 + m3 + d, 0.3) if cross else (None, 0)
+    bn = C.binning(10., 15, 0.005, 0.5, 0.5, 0.1 if d == 4 else 1.0)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    f1 = capi.Field(d, 2, cat, brute=False, **kw)
+    f2 = capi.Field(d, 2, cat2, brute=False, **kw) if cross else None
+    out = capi.corr2(d, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                     bn['b'], bn['a'], metric3=m3, rpar=rpar)
+
+    def ofield(c):
+        return O.Field(d, 2, c['x'], c['y'], c['z'], k=c['k'], g1=c['g1'], g2=c['g2'],
+                       w=c['w'], **kw)
+    ref = O.corr2(d, 2, ofield(cat), ofield(cat2) if cross else None, bn['nbins'],
+                  bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'], exact=True,
+                  metric3=m3, rpar=rpar)
+    np.testing.assert_array_equal(out['npairs'], ref['npairs'])
+    assert ref['npairs'].sum() > 1e5
+    for k in ['meanr', 'meanlogr', 'weight'] + (['xi0'] if d else []):
+        assert normwise(out[k], ref[k]) < RTOL, k
+    for k in ('visits', 'accepted', 'leafpairs', 'refined'):
+        assert out['counters'][k] == ref['counters'][k], k
+
+
+@pytest.mark.gpu
+def test_gpu_rperp_front_end(gpu, oracle_lib):
+    """NNCorrelation(metric='Rperp', min_rpar, max_rpar) and NNCorrelation(metric='Rlens')
+    through the public classes: pair counts equal the oracle's on trees to the leaves;
+    the metrics' argument checks."""
+    O = oracle_lib
+    cat, _ = C.make_shell(20000, 241, 0.3)
+    cat2, _ = C.make_shell(15000, 242, 0.3)
+    t1 = gpu.Catalog(x=cat['x'], y=cat['y'], z=cat['z'])
+    t2 = gpu.Catalog(x=cat2['x'], y=cat2['y'], z=cat2['z'])
+    f1 = O.Field(0, 2, cat['x'], cat['y'], cat['z'], min_size=0., max_size=0., min_top=4)
+    f2 = O.Field(0, 2, cat2['x'], cat2['y'], cat2['z'], min_size=0., max_size=0., min_top=4)
+    for metric, m3, rpar, c2, o2 in (('Rperp', 2, (-10., 20.), None, None),
+                                     ('OldRperp', 5, None, t2, f2), ('Rlens', 3, None, t2, f2)):
+        kw = dict(min_rpar=rpar[0], max_rpar=rpar[1]) if rpar else {}
+        nn = gpu.NNCorrelation(nbins=10, min_sep=0.05, max_sep=5., metric=metric, min_top=4,
+                               **kw)
+        nn.process(t1, c2)
+        ref = O.corr2(0, 2, f1, o2, nn.nbins, nn._min_sep, nn._max_sep, nn._bin_size, nn.b,
+                      nn.angle_slop, exact=True, metric3=m3, rpar=rpar)
+        np.testing.assert_array_equal(nn.npairs, ref['npairs'])
+        assert ref['npairs'].sum() > 1e5
+    with pytest.raises(ValueError):
+        gpu.NNCorrelation(nbins=10, min_sep=0.05, max_sep=5., metric='Rlens').process(t1)
+    flat = gpu.Catalog(x=cat['x'], y=cat['y'])
+    with pytest.raises(ValueError):
+        gpu.NNCorrelation(nbins=10, min_sep=0.05, max_sep=5., metric='Rperp').process(flat)
+    with pytest.raises(ValueError):
+        gpu.NNCorrelation(nbins=10, min_sep=0.05, max_sep=5., min_rpar=0.).process(flat)

@@ -274,3 +274,44 @@ def test_oracle_arc_against_literal_reference(oracle_lib, d, coords, cross):
     for k in ['meanr', 'meanlogr', 'weight'] + (['xi0'] if d else []):
         assert normwise(o[k], r[k]) < 1e-12, k
     assert normwise(e['meanr'], r['meanr']) > 1e-8
+
+
+LOS_CASES = [(m3, d, cross, rpar) for m3 in (2, 5, 3, 0) for d in (0, 4)
+             for cross in (False, True) for rpar in (None, (-20., 30.))
+             if not (m3 == 3 and not cross) and not (m3 == 0 and rpar is None)]
+
+
+@pytest.mark.parametrize('m3,d,cross,rpar', LOS_CASES)
+def test_oracle_los_metrics_against_literal_reference(oracle_lib, m3, d, cross, rpar):
+    """The line-of-sight metrics Rperp (2), Rlens (3), OldRperp (5), and the r_par range
+    (also with the Euclidean metric, 0 here), include/Metric.h:37-651: oracle vs the
+    compiled reference on 3-d shells, npairs exact and sums within 1e-12.  The r_par range
+    removes about half the pairs, so a dropped range test fails."""
+    from oracle import ref as R
+    if R.module() is None:
+        pytest.skip('oracle/_ref not built')
+    O = oracle_lib
+    if m3 == 0:
+        rpar = (-1., 2.)     # 3-d separations below max_sep already bound |r_par| by 5
+    cat, _ = C.make_shell(6000, 200 + m3 + d, 0.3)
+    cat2, _ = C.make_shell(5000, 210 + m3 + d, 0.3)
+    bn = C.binning(10., 12, 0.01, 0.5, 0.5, 0.1 if d == 4 else 1.0)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+
+    def cols(c):
+        return dict(k=c['k'], g1=c['g1'], g2=c['g2'], w=c['w'])
+    rf2 = R.make_field(d, 2, cat2['x'], cat2['y'], cat2['z'], **cols(cat2), **kw) \
+        if cross else None
+    of2 = O.Field(d, 2, cat2['x'], cat2['y'], cat2['z'], **cols(cat2), **kw) if cross else None
+    rf1 = R.make_field(d, 2, cat['x'], cat['y'], cat['z'], **cols(cat), **kw)
+    of1 = O.Field(d, 2, cat['x'], cat['y'], cat['z'], **cols(cat), **kw)
+    args = (bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'])
+    r = R.corr2(d, rf1, rf2, *args, metric3=m3, rpar=rpar)
+    o = O.corr2(d, 2, of1, of2, *args, metric3=m3, rpar=rpar)
+    np.testing.assert_array_equal(o['npairs'], r['npairs'])
+    assert r['npairs'].sum() > 1e5
+    for k in ['meanr', 'meanlogr', 'weight'] + (['xi0'] if d else []):
+        assert normwise(o[k], r[k]) < 1e-12, k
+    if rpar is not None:
+        full = O.corr2(d, 2, of1, of2, *args, metric3=m3)
+        assert r['npairs'].sum() < 0.8 * full['npairs'].sum()

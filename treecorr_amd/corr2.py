@@ -21,8 +21,10 @@ from .field import build_patch_fields, coord_enum
 from .util import set_omp_threads
 
 LOG = 0
-EUCLIDEAN, ARC, PERIODIC = 1, 4, 6          # include/Metric.h:29
-_METRIC_ENUM = {'Euclidean': EUCLIDEAN, 'Arc': ARC, 'Periodic': PERIODIC}
+EUCLIDEAN, RPERP, RLENS, ARC, OLDRPERP, PERIODIC = 1, 2, 3, 4, 5, 6    # include/Metric.h:29
+# util.py:355-372; 'Rperp' is TreeCorr's default alias, 'FisherRperp'
+_METRIC_ENUM = {'Euclidean': EUCLIDEAN, 'Rperp': RPERP, 'FisherRperp': RPERP,
+                'OldRperp': OLDRPERP, 'Rlens': RLENS, 'Arc': ARC, 'Periodic': PERIODIC}
 
 
 class _PairSums:
@@ -141,10 +143,10 @@ class Corr2:
         self.bin_slop = float(bin_slop)
         self.b = self._bin_size * self.bin_slop
         self.brute = cfg.get('brute', False)
-        self.min_rpar = -sys.float_info.max
-        self.max_rpar = sys.float_info.max
-        if cfg.get('min_rpar') is not None or cfg.get('max_rpar') is not None:
-            raise NotImplementedError("min_rpar/max_rpar are not supported by treecorr_amd")
+        # corr2base.py: the r_par range (3-d coordinates)
+        mn, mx = cfg.get('min_rpar'), cfg.get('max_rpar')
+        self.min_rpar = float(mn) if mn is not None else -sys.float_info.max
+        self.max_rpar = float(mx) if mx is not None else sys.float_info.max
         # corr2base.py:547-550: the Periodic metric's periods
         period = float(cfg.get('period') or 0.)
         self.xperiod = float(cfg.get('xperiod') or period)
@@ -199,8 +201,16 @@ class Corr2:
     def _set_metric(self, metric, coords1, coords2=None):
         metric = metric or self.config.get('metric', 'Euclidean')
         if metric not in _METRIC_ENUM:
-            raise NotImplementedError("treecorr_amd implements the Euclidean, Arc and "
-                                      "Periodic metrics")
+            raise ValueError("Invalid metric %s" % metric)
+        if metric in ('Rperp', 'OldRperp', 'FisherRperp', 'Rlens') and coords1 != '3d':
+            # util.py:331-336
+            raise ValueError("%s metric is only valid for catalogs with 3d positions." % metric)
+        if metric == 'Rlens' and coords2 is None:
+            raise ValueError("Rlens metric is only valid for cross-correlations.")
+        if coords1 != '3d' and (self.min_rpar != -sys.float_info.max or
+                                self.max_rpar != sys.float_info.max):
+            # corr2base.py:1555-1559
+            raise ValueError("min_rpar/max_rpar are only valid for 3d coordinates")
         if metric == 'Arc' and coords1 not in ('spherical', '3d'):
             # util.py:340-341
             raise ValueError("Arc metric is only valid for catalogs with spherical positions.")

@@ -358,7 +358,6 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
                     double xp, double yp, double zp, double* xi0, double* xi1, double* xi2,
                     double* xi3, double* meanr, double* meanlogr, double* weight,
                     double* npairs, tc_corr2** out) {
-    (void)minrpar; (void)maxrpar;
     if (!out) return tc::fail("out is NULL");
     *out = nullptr;
     const int kind = corr2_kind(d1, d2);
@@ -394,6 +393,9 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
     P.minsepsq = minsep * minsep; P.maxsepsq = maxsep * maxsep;
     P.bsq = b * b; P.asq = a * a;
     P.metric = MetEuclidean; P.xp = xp; P.yp = yp; P.zp = zp;   // periods: Periodic metric
+    // BaseCorr2::nontrivialRPar, include/Corr2.h:58-62
+    P.minrpar = minrpar; P.maxrpar = maxrpar;
+    P.rpar_on = (minrpar != -1.7976931348623157e308 || maxrpar != 1.7976931348623157e308);
     c->xi[0] = xi0; c->xi[1] = xi1; c->xi[2] = xi2; c->xi[3] = xi3;
     c->meanr = meanr; c->meanlogr = meanlogr; c->weight = weight; c->npairs = npairs;
     *out = c;
@@ -406,7 +408,20 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
 // correlation's periods, which wrap every separation (flat and 3-d coordinates).
 static int set_metric(const tc_corr2* c, int metric, int coords, C2Params& P) {
     P.metric = MetEuclidean;
+    if (P.rpar_on && coords != ThreeD)
+        return tc::fail("min_rpar/max_rpar need 3-d coordinates");
+    if (P.rpar_on && (metric == MetArc || metric == MetPeriodic)) {
+        set_error("min_rpar/max_rpar are implemented with the Euclidean, Rperp, OldRperp and "
+                  "Rlens metrics");
+        return TC_ERR_UNSUPPORTED;
+    }
     if (metric == MetEuclidean) return 0;
+    if (los_metric(metric)) {
+        // Rperp, Rlens, OldRperp: 3-d only (include/Metric.h:271, :410, :570)
+        if (coords != ThreeD) return tc::fail("this metric needs 3-d coordinates");
+        P.metric = metric;
+        return 0;
+    }
     if (metric == MetArc) {
         if (coords == Flat) return tc::fail("the Arc metric needs spherical or 3-d coordinates");
         P.metric = MetArc;
@@ -419,7 +434,7 @@ static int set_metric(const tc_corr2* c, int metric, int coords, C2Params& P) {
         P.metric = MetPeriodic;
         return 0;
     }
-    set_error("only the Euclidean, Arc and Periodic metrics are implemented");
+    set_error("unknown metric");
     return TC_ERR_UNSUPPORTED;
 }
 
@@ -436,6 +451,13 @@ static bool fields_out_of_range(const C2Params& P, int coords, const tc_field* f
     bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
                      rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
     bool too_large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+    if (P.rpar_on || los_metric(P.metric)) {
+        bool rin;
+        too_small = metric_prune(P.metric, P.rpar_on, P.minrpar, P.maxrpar, a.cx, a.cy, a.cz,
+                                 b.cx, b.cy, b.cz, rsq, s1ps2, too_small, too_large, P.minsep,
+                                 P.minsepsq, P.maxsep, P.maxsepsq, rin);
+        too_large = false;
+    }
     return too_small || too_large;
 }
 
@@ -582,7 +604,13 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
     const double rsq = metric_dist_sq(P.metric, coords, x1, y1, z1, x2, y2, z2, P.xp, P.yp,
                                       P.zp, s1, s2);
     const double s1ps2 = s1 + s2;
-    *result = (rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2)) ? 1 : 0;
+    bool large = rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+    if (large && los_metric(P.metric)) {   // the metric's tooLargeDist, no r_par range
+        bool rin;
+        large = metric_prune(P.metric, 0, 0., 0., x1, y1, z1, x2, y2, z2, rsq, s1ps2, false,
+                             true, P.minsep, P.minsepsq, P.maxsep, P.maxsepsq, rin);
+    }
+    *result = large ? 1 : 0;
     return 0;
 }
 

@@ -150,7 +150,12 @@ __host__ __device__ inline double wrap_period(double d, double p) {
 }
 
 // The metrics of a 2-point run, values of the reference's enum (include/Metric.h:29).
-enum Metric { MetEuclidean = 1, MetArc = 4, MetPeriodic = 6 };
+enum Metric { MetEuclidean = 1, MetRperp = 2, MetRlens = 3, MetArc = 4, MetOldRperp = 5,
+              MetPeriodic = 6 };
+// the metrics measured across the line of sight (3-d only)
+__host__ __device__ inline bool los_metric(int m) {
+    return m == MetRperp || m == MetRlens || m == MetOldRperp;
+}
 
 // MetricHelper<M>::DistSq (include/Metric.h) for the metrics above:
 //   Euclidean  the squared norm of the separation (:143-147, :197-202);
@@ -163,6 +168,42 @@ __host__ __device__ __forceinline__ double metric_dist_sq(int metric, int coords
                                                           double by, double bz, double xp,
                                                           double yp, double zp, double& s1,
                                                           double& s2) {
+    if (metric == MetRperp) {
+        // MetricHelper<Rperp>::DistSq, include/Metric.h:416-434: |a x b|^2 / |L|^2 with L
+        // the midpoint; each size scaled by the farther object's distance over |L|
+        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const double lx = (ax + bx) * 0.5, ly = (ay + by) * 0.5, lz = (az + bz) * 0.5;
+        const double nl = lx * lx + ly * ly + lz * lz;
+        const double rperpsq = nl > 0. ? (cx * cx + cy * cy + cz * cz) / nl
+                                       : 4. * (ax * ax + ay * ay + az * az);
+        const double r1sq = ax * ax + ay * ay + az * az, r2sq = bx * bx + by * by + bz * bz;
+        if (r2sq > nl && s1 != 0) s1 *= sqrt(r2sq / nl);
+        if (r1sq > nl && s2 != 0) s2 *= sqrt(r1sq / nl);
+        return rperpsq;
+    }
+    if (metric == MetOldRperp) {
+        // MetricHelper<OldRperp>::DistSq, include/Metric.h:277-293: |a-b|^2 - (r1-r2)^2,
+        // the nearer object's size grown by (r2^2-r1^2)/(4 r1^2)
+        const double dx = ax - bx, dy = ay - by, dz = az - bz;
+        const double d3sq = dx * dx + dy * dy + dz * dz;
+        const double r1sq = ax * ax + ay * ay + az * az, r2sq = bx * bx + by * by + bz * bz;
+        const double rparsq = (r1sq - r2sq) * (r1sq - r2sq) / (r1sq + r2sq + 2. * sqrt(r1sq * r2sq));
+        if (r1sq < r2sq) {
+            if (s1 != 0. && s1 < 1.7976931348623157e308) s1 *= (1. + 0.25 * (r2sq - r1sq) / r1sq);
+        } else {
+            if (s2 != 0. && s2 < 1.7976931348623157e308) s2 *= (1. + 0.25 * (r1sq - r2sq) / r2sq);
+        }
+        return fabs(d3sq - rparsq);
+    }
+    if (metric == MetRlens) {
+        // MetricHelper<Rlens>::DistSq, include/Metric.h:582-589: the separation at the
+        // lens (the first object's) distance, the second size scaled to it
+        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const double r2sq = bx * bx + by * by + bz * bz;
+        const double rsq = (cx * cx + cy * cy + cz * cz) / r2sq;
+        s2 *= sqrt((ax * ax + ay * ay + az * az) / r2sq);
+        return rsq;
+    }
     if (metric == MetArc) {
         if (coords == Sphere) {
             const double dx = ax - bx, dy = ay - by, dz = az - bz;
@@ -181,6 +222,58 @@ __host__ __device__ __forceinline__ double metric_dist_sq(int metric, int coords
         if (coords != Flat) dz = wrap_period(dz, zp);
     }
     return coords == Flat ? dx * dx + dy * dy : dx * dx + dy * dy + dz * dz;
+}
+
+// BaseCorr2::process11's pruning, src/Corr2.cpp:326-346, for the line-of-sight metrics
+// and the r_par range (3-d): true when the pair is dropped, because r_par is outside
+// [minrpar, maxrpar] by more than s1+s2 (ParHelper<1>: r along the midpoint's direction;
+// ParHelper<2> for OldRperp: r2 - r1, include/Metric.h:37-106), or because the bin
+// type's too-small/too-large test (bt_small, bt_large) holds and the metric's own
+// (Rperp :458-476, OldRperp :300-318; the others always agree) confirms it.  rin: r_par
+// lies inside the range by s1+s2, which singleBin requires.
+__host__ __device__ inline bool metric_prune(int metric, int rpar_on, double minrpar,
+                                             double maxrpar, double ax, double ay, double az,
+                                             double bx, double by, double bz, double rsq,
+                                             double s1ps2, bool bt_small, bool bt_large,
+                                             double minsep, double minsepsq, double maxsep,
+                                             double maxsepsq, bool& rin) {
+    rin = true;
+    double rpar = 0.;
+    auto calc_rpar = [&]() {
+        if (metric == MetOldRperp)
+            return sqrt(bx * bx + by * by + bz * bz) - sqrt(ax * ax + ay * ay + az * az);
+        const double rx = bx - ax, ry = by - ay, rz = bz - az;
+        const double lx = (ax + bx) * 0.5, ly = (ay + by) * 0.5, lz = (az + bz) * 0.5;
+        return (rx * lx + ry * ly + rz * lz) / sqrt(lx * lx + ly * ly + lz * lz);
+    };
+    if (rpar_on) {
+        rpar = calc_rpar();
+        if (rpar + s1ps2 < minrpar || rpar - s1ps2 > maxrpar) return true;
+    }
+    bool small = bt_small, large = bt_large;
+    if (metric == MetRperp && (small || large)) {
+        const double lx = (ax + bx) * 0.5, ly = (ay + by) * 0.5, lz = (az + bz) * 0.5;
+        const double nl = lx * lx + ly * ly + lz * lz;
+        if (small && !(rsq < nl)) {
+            if (s1ps2 * s1ps2 > 4. * nl) small = false;
+            else {
+                const double t = minsep * (1 - s1ps2 / (2. * sqrt(nl))) - s1ps2;
+                small = rsq < t * t;
+            }
+        }
+        if (large && !(rsq < nl)) {
+            const double t = maxsep * (1 + s1ps2 / (2. * sqrt(nl))) + s1ps2;
+            large = rsq > t * t;
+        }
+    } else if (metric == MetOldRperp && (small || large)) {
+        if (rpar == 0.) rpar = calc_rpar();
+        const double d3 = sqrt(rpar * rpar + rsq);
+        small = small && rsq + 2. * (d3 + fabs(rpar)) * s1ps2 < minsepsq;
+        large = large && rsq - 2. * (d3 + fabs(rpar)) * s1ps2 > maxsepsq;
+    }
+    if (small || large) return true;
+    if (rpar_on) rin = !(rpar - s1ps2 < minrpar) && !(rpar + s1ps2 > maxrpar);
+    return false;
 }
 
 // calculate_expmsialpha<S>, include/ProjectHelper.h:37-61: exp(-i S alpha) of the

@@ -357,7 +357,7 @@ __device__ __forceinline__ int pair_slot(int p, int* tags) {
 // for its children with one LDS atomic, so the waves of a workgroup never wait for
 // each other; the order of the children in a region does not matter.  No global
 // atomics: the regions are concatenated afterwards.
-// GM: the Arc and Periodic metrics (metric_dist_sq); the Euclidean instantiations
+// GM: the other metrics and the r_par range (metric_dist_sq, metric_prune); the Euclidean instantiations
 // keep the plain distance, whose kernels need far fewer registers.
 template <int D, int C, class IT, bool GM>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
@@ -448,17 +448,27 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 // step 1: pruning and singleBin's cheap tests (include/BinType.h:93-151)
                 int cls = CL_NONE;
                 double rsq = 0., s1ps2 = 0., s1ps2sq = 0.;
-                double s1 = c1.size, s2 = c2.size;    // the Arc metric rescales them in 3-d
+                double s1 = c1.size, s2 = c2.size;    // some metrics rescale them
                 if (c1.w != 0.f && c2.w != 0.f) {
                     rsq = GM ? metric_dist_sq(P.metric, C, c1.x, c1.y, c1.z, c2.x, c2.y,
                                               c2.z, P.xp, P.yp, P.zp, s1, s2)
                              : dist_sq<C>(c1, c2);
                     s1ps2 = s1 + s2;
-                    const bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
-                                           rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
-                    const bool too_large = rsq >= P.maxsepsq &&
-                                           rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
-                    if (!too_small && !too_large) {
+                    bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
+                                     rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
+                    bool too_large = rsq >= P.maxsepsq &&
+                                     rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+                    bool rin = true;
+                    if (GM && (P.rpar_on || los_metric(P.metric))) {
+                        too_small = metric_prune(P.metric, P.rpar_on, P.minrpar, P.maxrpar,
+                                                 c1.x, c1.y, c1.z, c2.x, c2.y, c2.z, rsq,
+                                                 s1ps2, too_small, too_large, P.minsep,
+                                                 P.minsepsq, P.maxsep, P.maxsepsq, rin);
+                        too_large = false;
+                    }
+                    if (!too_small && !too_large && !rin) {
+                        cls = CL_SPLIT;     // singleBin needs r_par inside the range
+                    } else if (!too_small && !too_large) {
                         s1ps2sq = s1ps2 * s1ps2;
                         if (s1ps2 == 0.) cls = CL_SINGLE;
                         else if (s1ps2sq > P.asq * rsq) cls = CL_SPLIT;
@@ -1035,7 +1045,7 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
         else hipLaunchKernelGGL(k_items_cross, g, dim3(256), 0, st, dtop1, ntop1, dtop2, ntop2, stack);
         TC_HIP(hipGetLastError());
     }
-    if (int rc = traverse_stack<Item>(S, pick_traverse<Item>(D, C, P.metric != MetEuclidean), nodes1,
+    if (int rc = traverse_stack<Item>(S, pick_traverse<Item>(D, C, P.metric != MetEuclidean || P.rpar_on), nodes1,
                                       autoc ? nodes1 : nodes2, ninit, P, R, shbytes, nullptr,
                                       nullptr))
         return rc;
@@ -1182,7 +1192,7 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
                            static_cast<ItemP*>(S.stack));
         TC_HIP(hipGetLastError());
     }
-    if (int rc = traverse_stack<ItemP>(S, pick_traverse<ItemP>(D, C, P.metric != MetEuclidean), nodes1, nodes2, ninit, P,
+    if (int rc = traverse_stack<ItemP>(S, pick_traverse<ItemP>(D, C, P.metric != MetEuclidean || P.rpar_on), nodes1, nodes2, ninit, P,
                                        R, shbytes, ph, pn))
         return rc;
     // counters from the workgroup slots

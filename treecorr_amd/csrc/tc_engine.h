@@ -50,6 +50,8 @@ struct C2Params {
     double logminsep, halfminsep, minsepsq, maxsepsq, bsq, asq;
     int metric;                 // Metric: Euclidean, Arc or Periodic
     double xp, yp, zp;          // the Periodic metric's periods
+    int rpar_on;                // a [minrpar, maxrpar] range was given (3-d)
+    double minrpar, maxrpar;
 };
 
 // Lloyd iterations of k-means over the objects (centers in/out, npatch x dim).
