@@ -1,0 +1,94 @@
+// Runs the library's own device tree build (treecorr_amd/csrc/tc_field.hip) and the
+// device multipole engine (tc_corr3.hip) together on the CPU, through the HIP
+// emulation in hip_emu/: the node array, child links, counts and top list that the
+// multipole kernels read are the ones the device builder wrote.
+//
+//   emu_lib3 <input> <output>
+// input:  int64 header [D, C, n, nbins, maxn, quick, mintop, maxtop]
+//         double header [minsep, maxsep, binsize, b, a, minsize, maxsize]
+//         then x, y, z, g1, g2, w (n doubles each; z ignored for Flat).
+// output: 15*ntot f64 sums, 4 u64 counters, int64 nnodes, int64 ntop, then the nodes
+//         (nnodes DNode records of 48 bytes) and the top list (ntop int32).
+// Exit status 0 on success, 3 on an engine error (message on stderr).
+#include "tc_engine.h"
+
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+namespace tc {
+static std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+int fail(const std::string& m) { g_err = m; return 1; }
+int hip_fail(hipError_t, const char* what) { g_err = std::string("emulated HIP failure in ") + what; return 2; }
+hipStream_t stream() { return nullptr; }
+hipStream_t stream2() { static int s2; return (hipStream_t)&s2; }
+hipStream_t stream3() { static int s3; return (hipStream_t)&s3; }
+bool timing_enabled() { return false; }
+KernelStat& kernel_stat(int which) {
+    static KernelStat s[KS_COUNT];
+    return s[which];
+}
+void* dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    return hipMalloc(&p, bytes ? bytes : 1) == hipSuccess ? p : nullptr;
+}
+void dev_free(void* p, size_t) {
+    if (p) (void)hipFree(p);
+}
+}  // namespace tc
+
+int main(int argc, char** argv) {
+    using namespace tc;
+    if (argc != 3) return 2;
+    std::ifstream in(argv[1], std::ios::binary);
+    long long h[8];
+    double p[7];
+    in.read((char*)h, sizeof h);
+    in.read((char*)p, sizeof p);
+    const int D = (int)h[0], C = (int)h[1];
+    const long long n = h[2];
+    std::vector<double> col[6];
+    for (auto& c : col) {
+        c.resize(n);
+        in.read((char*)c.data(), 8 * n);
+    }
+    if (!in) return 2;
+    DeviceTree T;
+    const double* z = C == Flat ? nullptr : col[2].data();
+    const double* g1 = D == GData ? col[3].data() : nullptr;
+    const double* g2 = D == GData ? col[4].data() : nullptr;
+    int rc = build_tree(D, C, col[0].data(), col[1].data(), z, g1, g2, col[5].data(), nullptr, n,
+                        p[5], p[6], Mean, 0, (int)h[6], (int)h[7], T);
+    if (rc) {
+        std::fprintf(stderr, "emu_lib3: tree build: %s\n", g_err.c_str());
+        return 3;
+    }
+    C3Params P{};
+    P.nbins = (int)h[3]; P.maxn = (int)h[4]; P.D = D; P.quick = (int)h[5];
+    P.minsep = p[0]; P.maxsep = p[1]; P.binsize = p[2]; P.b = p[3]; P.a = p[4];
+    P.logminsep = std::log(P.minsep);
+    P.minsepsq = P.minsep * P.minsep; P.maxsepsq = P.maxsep * P.maxsep;
+    P.bsq = P.b * P.b; P.asq = P.a * P.a;
+    P.maxbsq_eff = P.maxsepsq * std::min(P.bsq, P.asq);
+    const long long ntot = (long long)P.nbins * P.nbins * (2 * P.maxn + 1);
+    std::vector<double> out(15 * ntot, 0.);
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    rc = corr3_run(D, C, T.nodes, T.nnodes, T.dtop, (int)T.top.size(), P, out.data(), cnt);
+    if (rc) {
+        std::fprintf(stderr, "emu_lib3: multipole: %s\n", g_err.c_str());
+        return 3;
+    }
+    std::ofstream o(argv[2], std::ios::binary);
+    o.write((const char*)out.data(), 8 * out.size());
+    o.write((const char*)cnt, sizeof cnt);
+    const long long nn = T.nnodes, nt = (long long)T.top.size();
+    o.write((const char*)&nn, 8);
+    o.write((const char*)&nt, 8);
+    o.write((const char*)T.nodes, sizeof(DNode) * nn);
+    o.write((const char*)T.top.data(), 4 * nt);
+    dev_free(T.nodes, T.nodes_bytes);
+    dev_free(T.dtop, T.dtop_bytes);
+    return o ? 0 : 2;
+}

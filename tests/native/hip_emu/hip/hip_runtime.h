@@ -203,9 +203,26 @@ inline void run_block(Block& b, dim3 bidx, unsigned nt, size_t shmem) {
         f.sp = sp;
     }
     unsigned left = nt;
+    // EMU_SHUFFLE=<seed>: between two barriers the fibers run in a fresh pseudo-random
+    // order on every sweep (not thread 0 first), so a value one thread writes and another
+    // reads without a barrier in between shows up as a wrong result.
+    static const unsigned long long shuffle_seed = [] {
+        const char* e = std::getenv("EMU_SHUFFLE");
+        return e ? std::strtoull(e, nullptr, 10) : 0ull;
+    }();
+    std::vector<unsigned> order(nt);
+    for (unsigned t = 0; t < nt; ++t) order[t] = t;
+    unsigned long long rs = shuffle_seed * 0x9E3779B97F4A7C15ull + bidx.x * 0xBF58476D1CE4E5B9ull + 1;
     while (left > 0) {
         bool progressed = false;
-        for (unsigned t = 0; t < nt; ++t) {
+        if (shuffle_seed) {
+            for (unsigned t = nt; t > 1; --t) {
+                rs ^= rs << 13; rs ^= rs >> 7; rs ^= rs << 17;
+                std::swap(order[t - 1], order[(unsigned)(rs % t)]);
+            }
+        }
+        for (unsigned q = 0; q < nt; ++q) {
+            const unsigned t = order[q];
             Fiber& f = b.fib[t];
             if (f.done || f.waiting) continue;
             b.cur = (int)t;

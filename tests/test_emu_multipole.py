@@ -3,9 +3,10 @@
 tests/native/hip_emu emulates the HIP subset the kernels use (fibers for the
 threads of a workgroup, barriers, cross-lane exchange, atomics) and the kernels are
 compiled with g++ and AddressSanitizer, so every out-of-bounds access of their
-indexing is reported with its source line.  The tree comes from the oracle (it is
-bit-identical to the device tree, see test_gpu_tree.py); the results must match the
-reference's golden vectors: ntri exactly, the FP64 sums to 1e-12 normwise.
+indexing is reported with its source line.  The first tests take the tree from the
+oracle; the last ones build it with the device tree builder itself (tc_field.hip under
+the same emulation).  The results must match the reference's golden vectors: ntri
+exactly, the FP64 sums to 1e-12 normwise.
 
 Workgroups run one at a time (EMU_WORKERS=1) and eight at a time, which exercises
 the persistent unit queue of k_mp_finish with concurrent workgroups.
@@ -96,6 +97,110 @@ def test_emulated_multipole_matches_reference_golden(emu_exe, oracle_lib, case, 
                 min_top=extra['min_top'])
     assert f.ntop == spec['ntop']
     out = run_emu(emu_exe, d, coords, f.dump(), bn, workers)
+    np.testing.assert_array_equal(out['ntri'], z['ntri'])
+    for k in z.files:
+        if k not in ('spec', 'ntri') and z[k].size:
+            assert normwise(out[k], z[k]) < RTOL, k
+    ref = O.corr3_multipole(d, coords, f, bn['nbins'], bn['min_sep'], bn['max_sep'],
+                            bn['bin_size'], bn['b'], bn['a'], bn['max_n'])
+    assert out['counters'] == ref['counters']
+
+
+# ---------------------------------------------------------------------------
+# The whole library on the CPU: the device tree builder (tc_field.hip) writes the node
+# array, child links, counts and top list that the multipole kernels (tc_corr3.hip) then
+# read, as on the GPU.  Fibers run in a shuffled order between barriers (EMU_SHUFFLE),
+# so a value one thread writes and another reads without a barrier shows up.
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope='module')
+def emu_lib_exe():
+    if shutil.which('g++') is None:
+        pytest.skip('g++ is not available')
+    out = tempfile.mkdtemp(prefix='tc_emu_lib_')
+    inc = ['-I', os.path.join(HERE, 'native', 'hip_emu'), '-I', os.path.join(ROOT, 'treecorr_amd', 'csrc')]
+    flags = ['g++', '-std=c++17', '-O1', '-ffp-contract=off', '-fsanitize=address',
+             '-fno-omit-frame-pointer']
+    objs = []
+    procs = []
+    for src in ('tc_field.hip', 'tc_corr3.hip'):
+        o = os.path.join(out, src + '.o')
+        procs.append(subprocess.Popen(flags + ['-x', 'c++', *inc, '-c',
+                                               os.path.join(ROOT, 'treecorr_amd', 'csrc', src),
+                                               '-o', o], stderr=subprocess.PIPE))
+        objs.append(o)
+    for p in procs:
+        _, e = p.communicate(timeout=600)
+        assert p.returncode == 0, e.decode()[-3000:]
+    exe = os.path.join(out, 'emu_lib3')
+    subprocess.run(flags + [*inc, os.path.join(HERE, 'native', 'emu_lib3.cpp'), *objs, '-o', exe,
+                            '-pthread'], check=True, capture_output=True, timeout=300)
+    yield exe
+    shutil.rmtree(out, ignore_errors=True)
+
+
+def run_emu_lib(exe, d, coords, cat, bn, min_top, env):
+    tmp = tempfile.mkdtemp(prefix='tc_emu_lib_run_')
+    try:
+        fin, fout = os.path.join(tmp, 'in.bin'), os.path.join(tmp, 'out.bin')
+        n = len(cat['x'])
+        with open(fin, 'wb') as fh:
+            np.array([d, coords, n, bn['nbins'], bn['max_n'], 0, min_top, 10], np.int64).tofile(fh)
+            np.array([bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'],
+                      bn['min_size'], bn['max_size']], np.float64).tofile(fh)
+            for k in ('x', 'y', 'z', 'g1', 'g2', 'w'):
+                v = cat.get(k)
+                (np.zeros(n) if v is None else np.ascontiguousarray(v, np.float64)).tofile(fh)
+        r = subprocess.run([exe, fin, fout], env=dict(os.environ, ASAN_OPTIONS='detect_leaks=0', **env),
+                           capture_output=True, timeout=600)
+        assert r.returncode == 0, r.stderr.decode()[-3000:]
+        ntot = bn['nbins'] ** 2 * (2 * bn['max_n'] + 1)
+        raw = open(fout, 'rb').read()
+        sums = np.frombuffer(raw, np.float64, count=15 * ntot)
+        o = 15 * ntot * 8
+        cnt = np.frombuffer(raw, np.uint64, count=4, offset=o)
+        nn, nt = np.frombuffer(raw, np.int64, count=2, offset=o + 32)
+        from treecorr_amd.field import NODE_DTYPE
+        nodes = np.frombuffer(raw, NODE_DTYPE, count=int(nn), offset=o + 48)
+        top = np.frombuffer(raw, np.int32, count=int(nt), offset=o + 48 + 48 * int(nn))
+        shape = (bn['nbins'], bn['nbins'], 2 * bn['max_n'] + 1)
+        out = {k: sums[i * ntot:(i + 1) * ntot].reshape(shape) for i, k in enumerate(KEYS)}
+        out['counters'] = dict(zip(('gn_calls', 'zeta_calls', 'zeta_rows', 'stages'),
+                                   (int(c) for c in cnt)))
+        return out, nodes, top
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+LIB_RUNS = [(c, {'EMU_WORKERS': '4', 'EMU_SHUFFLE': str(11 + i)}) for i, c in enumerate(C.CORR3_CASES)]
+
+
+@pytest.mark.parametrize('case,env', LIB_RUNS, ids=[c[0] for c, _ in LIB_RUNS])
+def test_emulated_library_multipole_on_device_tree(emu_lib_exe, oracle_lib, case, env):
+    """Device tree build + multipole, all under the emulation: the device tree equals the
+    oracle's node for node, ntri equals the reference's golden vector, the sums agree to
+    1e-12 and the Gn/zeta/stage counters equal the oracle's."""
+    from treecorr_amd.field import preorder
+    O = oracle_lib
+    z = np.load(os.path.join(GOLD, case[0] + '.npz'), allow_pickle=False)
+    spec = json.loads(str(z['spec']))
+    cat, bn, extra = C.corr3_case(case)
+    assert C.checksum(cat) == spec['checksum']
+    d, coords = case[1], case[2]
+    out, nodes, top = run_emu_lib(emu_lib_exe, d, coords, cat, bn, extra['min_top'], env)
+    f = O.Field(d, coords, cat['x'], cat['y'], cat['z'], g1=cat.get('g1'), g2=cat.get('g2'),
+                w=cat['w'], min_size=bn['min_size'], max_size=bn['max_size'],
+                min_top=extra['min_top'])
+    ref_tree = f.dump()
+    nd = nodes[preorder(nodes, top)]
+    assert len(top) == spec['ntop'] == len(ref_tree['top'])
+    np.testing.assert_array_equal(nd['x'], ref_tree['pos'][:, 0])
+    np.testing.assert_array_equal(nd['y'], ref_tree['pos'][:, 1])
+    np.testing.assert_array_equal(nd['size'], ref_tree['size'])
+    np.testing.assert_array_equal(nd['w'], ref_tree['w'])
+    np.testing.assert_array_equal(nd['n'], ref_tree['n'])
+    if d == 4:
+        np.testing.assert_array_equal(nd['a'], ref_tree['g'][:, 0])
+        np.testing.assert_array_equal(nd['b'], ref_tree['g'][:, 1])
     np.testing.assert_array_equal(out['ntri'], z['ntri'])
     for k in z.files:
         if k not in ('spec', 'ntri') and z[k].size:

@@ -1,4 +1,7 @@
-"""GPU multipole (NNN, GGG) vs the literal reference's golden vectors and the oracle."""
+"""GPU multipole (NNN, GGG) vs the literal reference's golden vectors and the oracle.
+
+Each test prints the case it runs (name, type, coordinates, size) before the device
+call, so that a failure that ends the process still names its case."""
 import json
 import os
 
@@ -9,13 +12,6 @@ import cases as C
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
 RTOL = 1e-12   # normwise relative tolerance for the FP64 sums; ntri is exact
-
-# The device multipole path is switched off in the library (TC_MULTIPOLE_ENABLED=0):
-# on gfx950 it returned wrong ntri and made an illegal memory access, twice, while the
-# same kernels under the CPU emulation (test_emu_multipole.py) match the reference.
-# These tests stay, skipped, until that cause is found.
-pytestmark = pytest.mark.skip(reason='device multipole path disabled: wrong results and an '
-                                     'illegal memory access on gfx950, cause not found yet')
 
 
 def normwise(a, b):
@@ -33,6 +29,7 @@ def test_gpu_multipole_matches_reference_golden(gpu, case):
     cat, bn, extra = C.corr3_case(case)
     assert C.checksum(cat) == spec['checksum']
     d, coords = case[1], case[2]
+    print('multipole golden case', case[0], 'd', d, 'coords', coords, 'n', len(cat['x']), flush=True)
     f = capi.Field(d, coords, cat, min_size=bn['min_size'], max_size=bn['max_size'],
                    min_top=extra['min_top'])
     assert f.ntop == spec['ntop']
@@ -52,6 +49,7 @@ def test_gpu_multipole_matches_oracle(gpu, oracle_lib, d, coords, n, max_n):
     O = oracle_lib
     cat, scale = C.make_catalog(coords, n, 60 + d + coords, 0.3)
     bn = capi.multipole_binning(scale, 12, 0.01, 0.2, max_n)
+    print('multipole oracle case d', d, 'coords', coords, 'n', n, 'max_n', max_n, flush=True)
     kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
     f = capi.Field(d, coords, cat, **kw)
     out = capi.corr3(d, f, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'],

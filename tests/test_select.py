@@ -13,3 +13,15 @@ def test_nth_element_matches_libstdcxx(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout + out.stderr
     assert '0 of' in out.stdout
+
+
+def test_wrap_period_matches_reference_loop_and_is_bounded(tmp_path):
+    """wrap_period (tc_common.h) is the reference's Position::wrap loop inside 1.5 periods
+    and ends for tiny periods and far-out separations (no spinning kernel)."""
+    exe = str(tmp_path / 'wrap_check')
+    subprocess.check_call(['g++', '-O2', '-std=c++17', '-I', os.path.join(HERE, 'native', 'hip_emu'),
+                           '-I', os.path.join(os.path.dirname(HERE), 'treecorr_amd', 'csrc'),
+                           os.path.join(HERE, 'native', 'wrap_check.cpp'), '-o', exe, '-pthread'])
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert 'in-box mismatches: 0 of' in out.stdout

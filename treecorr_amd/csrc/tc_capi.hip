@@ -746,16 +746,6 @@ int tc_reset_kernel_stats(void) {
 // ---------------------------------------------------------------------------
 // 3-point multipole
 // ---------------------------------------------------------------------------
-// The device multipole engine is off by default: on gfx950 its kernels return wrong
-// sums and make an illegal memory access, while the same kernels compiled for the
-// CPU emulation in tests/native/hip_emu match the reference exactly; the cause is not
-// found yet.  Building with -DTC_MULTIPOLE_ENABLED=1 switches it on (the CPU
-// emulation build does); otherwise tc_corr3_process_auto reports TC_ERR_UNSUPPORTED.
-#ifndef TC_MULTIPOLE_ENABLED
-#define TC_MULTIPOLE_ENABLED 0
-#endif
-static constexpr bool kMultipoleEnabled = TC_MULTIPOLE_ENABLED != 0;
-
 struct tc_corr3 {
     int d = 0;
     tc::C3Params P{};
@@ -818,10 +808,6 @@ int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int met
         return TC_ERR_UNSUPPORTED;
     }
     if (f->d != c->d) return tc::fail("field type does not match the correlation");
-    if (!kMultipoleEnabled) {
-        tc::set_error("the device multipole engine is disabled in this build");
-        return TC_ERR_UNSUPPORTED;
-    }
     if (int rc = check_gpu()) return rc;
     tc::C3Params P = c->P;
     P.quick = quick ? 1 : 0;

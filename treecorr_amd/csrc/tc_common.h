@@ -142,10 +142,14 @@ __device__ __forceinline__ void expm2ialpha(double re, double im, double& er, do
 }
 
 // Position::wrap, include/Position.h:106-112, :201-209: a separation brought into
-// [-p/2, p/2] by whole periods
+// [-p/2, p/2] by whole periods.  Inside |d| <= 1.5 p (every pair of a catalog that lies
+// in its box) this is the reference's subtraction loop, bit for bit.  A separation
+// further out is first reduced exactly to |d| < p by fmod, and both loops are bounded,
+// so a period below ulp(d) or positions far outside the box cannot make a kernel spin.
 __host__ __device__ inline double wrap_period(double d, double p) {
-    while (d > p / 2.) d -= p;
-    while (d < -p / 2.) d += p;
+    if (fabs(d) > 1.5 * p) d = fmod(d, p);
+    for (int i = 0; i < 4 && d > p / 2.; ++i) d -= p;
+    for (int i = 0; i < 4 && d < -p / 2.; ++i) d += p;
     return d;
 }
 

@@ -1,22 +1,29 @@
 // treecorr_amd: the 3-point multipole algorithm (Porth et al.) on the GPU,
 // auto-correlation of one field for NNN and GGG (LogMultipole binning).
 //
-// TreeCorr (src/Corr3.cpp:1113-2849) recurses per top cell on each CPU thread.
-// Here:
-//   1. k_mp_split1: one workgroup per top cell walks multipoleSplit1
-//      (:1374-1408, splitC2Cells :1323-1372) and emits independent "units":
-//      a c1 cell small enough for the largest bin plus its c2 list.
-//   2. k_mp_finish: persistent workgroups pull units and run multipoleFinish
-//      (:1548-1613) depth first.  The multipole scratch (Wn, Gn, the pair sums;
-//      include/MultipoleScratch.h) lives in LDS; a c1 split pushes a copy of it
-//      onto a per-workgroup stack in HBM (TreeCorr's mp.duplicate()).  Every stage
-//      processes its c2 list with one lane per c2 (splitC2CellsOrCalculateGn
-//      :1452-1546): accepted pairs add their Fourier moments into LDS with FP64
-//      atomics, split c2 cells are compacted in list order into the next list.
-//      Zeta rows [kstart, mink_zeta) are contracted (calculateZeta :1861-1974,
-//      MultipoleHelper<5> :2582-2849) by the whole workgroup into the
-//      workgroup's private output slot.
-//   3. k_mp_reduce: fixed-order compensated sum of the slots.
+// TreeCorr (src/Corr3.cpp:1113-2849) recurses per top cell on each CPU thread:
+// multipoleSplit1 splits c1 until it is small enough for the largest bin, then
+// multipoleFinish walks c1 depth first, accumulating the Fourier moments Gn of the
+// accepted c2 cells into a scratch (MultipoleScratch) that is copied whenever c1 is
+// split, and contracts finished rows into zeta.  Here every level of both recursions
+// is one breadth-first launch over all cells of the level, driven from the host:
+//   1. k_mp_split_level: one workgroup per (c1, c2 list) of multipoleSplit1
+//      (:1374-1408, splitC2Cells :1323-1372).  Children and the "units" (c1 small
+//      enough, with its c2 list) go to fixed per-task positions; a scan compacts them
+//      in task order, so every later order is independent of scheduling.
+//   2. k_mp_stage: one workgroup per multipoleFinish stage (:1548-1613,
+//      splitC2CellsOrCalculateGn :1452-1546).  The stage's scratch lives in a slot in
+//      HBM, is loaded into LDS, receives the accepted pairs' moments (calculateGn
+//      :1681-1828) and is written back.  A split of c1 hands the left child the slot
+//      and the right child a copy in a spare slot (mp.duplicate()).  A stage that can
+//      contract rows of zeta records a job (the slot and the row range).
+//   3. k_mp_zeta: the jobs of a level, contracted (calculateZeta :1861-1974,
+//      MultipoleHelper<5> :2582-2849) by one workgroup per (job chunk, row k2) into an
+//      LDS tile of the compressed outputs, written per chunk and summed in chunk order.
+//   4. Free slots of finished stages and unused spares return to a free list (scan).
+// The outputs are kept compressed (k3 >= k2, n >= 0: the other three quarters of the
+// reference's arrays are copies or conjugates) and expanded into TreeCorr's layout at
+// the end (k_mp_expand).
 #include "tc_engine.h"
 
 #include <algorithm>
@@ -58,8 +65,6 @@ enum { O_Z0 = 0, O_MEAND2 = 8, O_MEANLOGD2, O_MEAND3, O_MEANLOGD3, O_WEIGHT, O_W
 enum { C3_GN = 0, C3_ZETA, C3_ROWS, C3_STAGES, NC3 = 4 };
 
 constexpr int MP_THREADS = 256;
-constexpr int MP_STACK = 96;               // DFS entries per workgroup
-constexpr long long MP_ARENA0 = 1 << 20;  // initial list entries per workgroup (grown on demand)
 
 template <int C>
 __device__ __forceinline__ double dsq3(const DNode& a, const DNode& b) {
@@ -147,180 +152,6 @@ __global__ void k_node_sums(const DNode* nodes, int nn, NodeSum* sums, const int
     *changed = 1;
 }
 
-// ---------------------------------------------------------------------------
-// Block helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int block_scan_excl(int v, int* sh, int& total) {
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    int x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) sh[wv] = x;
-    __syncthreads();
-    int base = 0;
-    total = 0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-        if (k < wv) base += sh[k];
-        total += sh[k];
-    }
-    __syncthreads();
-    return base + x - v;
-}
-
-struct Unit { int c1, off, len, pad; };
-
-// Error bits reported through `err` (all of them end the kernel cleanly).
-enum { E_POOL = 1, E_SPLITSTACK = 2, E_UNITS = 4, E_ARENA = 8, E_STACK = 16, E_INDEX = 32,
-       E_BUDGET = 64 };
-
-// Global data written by some waves of a workgroup and read by others after the
-// barrier (c2 lists in the pool and the arena, zeta output slots).  The workgroup
-// barrier alone waits only for LDS operations; the device-scope fences make this
-// thread's global stores complete before the barrier and drop stale L1 lines after it.
-__device__ __forceinline__ bool has_children(const DNode& c, int nnodes) {
-    return c.child >= 0 && c.child < nnodes - 1;
-}
-
-__device__ __forceinline__ void block_sync_global() {
-    __threadfence();
-    __syncthreads();
-    __threadfence();
-}
-
-// ---------------------------------------------------------------------------
-// Phase 1: multipoleSplit1 per top cell -> units
-// ---------------------------------------------------------------------------
-// Every node index read from memory is checked against nnodes before it is
-// dereferenced and every loop is bounded, so inconsistent data raises an error bit
-// instead of an out-of-range access.
-template <int C>
-__global__ __launch_bounds__(MP_THREADS) void k_mp_split1(
-    const DNode* __restrict__ nodes, int nnodes, const int* __restrict__ top, int ntop, C3Params P,
-    int* __restrict__ pool, unsigned long long* pool_top, unsigned long long pool_cap,
-    Unit* __restrict__ units, unsigned int* nunits, unsigned int unit_cap, int* err) {
-    __shared__ int st_c1[MP_STACK], st_off[MP_STACK], st_len[MP_STACK];
-    __shared__ int sh[MP_THREADS / 64];
-    __shared__ unsigned long long s_base;
-    __shared__ int s_bad;
-    int sp = 0;
-    if (threadIdx.x == 0) { st_c1[0] = top[blockIdx.x]; st_off[0] = -1; st_len[0] = ntop; }
-    sp = 1;
-    __syncthreads();
-    for (int iter = 0; sp > 0; ++iter) {
-        --sp;
-        const int i1 = st_c1[sp], off = st_off[sp], len = st_len[sp];
-        if (threadIdx.x == 0) s_bad = 0;
-        __syncthreads();
-        const bool ok = iter <= nnodes && (unsigned)i1 < (unsigned)nnodes && len >= 0 &&
-                        (off < 0 ? len == ntop : (unsigned long long)off + len <= pool_cap);
-        if (!ok) {
-            if (threadIdx.x == 0) atomicOr(err, iter > nnodes ? E_BUDGET : E_INDEX);
-            return;
-        }
-        const DNode c1 = nodes[i1];
-        const double s1 = c1.size;
-        const int* list = off < 0 ? top : pool + off;
-        // splitC2Cells, src/Corr3.cpp:1323-1372
-        // pass 1: count outputs
-        int total_out = 0;
-        for (int base = 0; base < len; base += blockDim.x) {
-            int t = base + threadIdx.x;
-            int no = 0;
-            if (t < len) {
-                const int i2 = list[t];
-                if ((unsigned)i2 >= (unsigned)nnodes) {
-                    s_bad = 1;
-                } else {
-                    const DNode c2 = nodes[i2];
-                    const double s2 = c2.size;
-                    const double rsq = dsq3<C>(c1, c2);
-                    const double s1ps2 = s1 + s2;
-                    if (!too_small3(P, rsq, s1ps2) && !too_large3(P, rsq, s1ps2)) {
-                        int k = -1; double r = 0, logr = 0;
-                        bool split = !single_bin3(P, rsq, s1ps2, k, r, logr);
-                        if (split && s2 > s1) {
-                            if (!has_children(c2, nnodes)) s_bad = 1;
-                            else no = 2;
-                        } else {
-                            no = 1;
-                        }
-                    }
-                }
-            }
-            int tot;
-            block_scan_excl(no, sh, tot);
-            total_out += tot;
-        }
-        if (s_bad) {
-            if (threadIdx.x == 0) atomicOr(err, E_INDEX);
-            return;
-        }
-        if (threadIdx.x == 0) {
-            s_base = atomicAdd(pool_top, (unsigned long long)total_out);
-            if (s_base + total_out > pool_cap) atomicOr(err, E_POOL);
-        }
-        __syncthreads();
-        const unsigned long long obase = s_base;
-        if (obase + total_out > pool_cap) return;
-        // pass 2: write outputs in list order (the same decisions as pass 1)
-        int wpos = 0;
-        for (int base = 0; base < len; base += blockDim.x) {
-            int t = base + threadIdx.x;
-            int no = 0, o0 = 0, o1 = 0;
-            if (t < len) {
-                const int i2 = list[t];
-                const DNode c2 = nodes[i2];
-                const double s2 = c2.size;
-                const double rsq = dsq3<C>(c1, c2);
-                const double s1ps2 = s1 + s2;
-                if (!too_small3(P, rsq, s1ps2) && !too_large3(P, rsq, s1ps2)) {
-                    int k = -1; double r = 0, logr = 0;
-                    bool split = !single_bin3(P, rsq, s1ps2, k, r, logr);
-                    if (split && s2 > s1) { no = 2; o0 = c2.child; o1 = c2.child + 1; }
-                    else { no = 1; o0 = i2; }
-                }
-            }
-            int tot;
-            int ex = block_scan_excl(no, sh, tot);
-            if (wpos + ex + no <= total_out) {
-                if (no > 0) pool[obase + wpos + ex] = o0;
-                if (no > 1) pool[obase + wpos + ex + 1] = o1;
-            }
-            wpos += tot;
-        }
-        // the children's passes read this list from other waves
-        block_sync_global();
-        const int noff = (int)obase;
-        if (s1 * s1 > P.maxbsq_eff) {
-            if (sp + 2 > MP_STACK || !has_children(c1, nnodes)) {
-                if (threadIdx.x == 0) atomicOr(err, sp + 2 > MP_STACK ? E_SPLITSTACK : E_INDEX);
-                return;
-            }
-            if (threadIdx.x == 0) {
-                st_c1[sp] = c1.child + 1; st_off[sp] = noff; st_len[sp] = total_out;
-                st_c1[sp + 1] = c1.child; st_off[sp + 1] = noff; st_len[sp + 1] = total_out;
-            }
-            sp += 2;
-        } else if (threadIdx.x == 0) {
-            unsigned int u = atomicAdd(nunits, 1u);
-            if (u < unit_cap) units[u] = Unit{i1, noff, total_out, 0};
-            else atomicOr(err, E_UNITS);
-        }
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Phase 2: multipoleFinish per unit
-// ---------------------------------------------------------------------------
-struct MPWork {
-    int* arena;            // per workgroup: arena_cap ints
-    double* scratch;       // per workgroup: MP_STACK * layout.total doubles
-    int* st;               // per workgroup: MP_STACK * 8 ints (c1, off, len, mink, maxr_lo, maxr_hi)
-};
-
 // Corr3::calculateGn + MultipoleScratch::calculateGn for one accepted (c1,c2)
 template <int D, int C>
 __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, const DNode& c1,
@@ -402,404 +233,742 @@ __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, cons
         }
     }
 }
+// ---------------------------------------------------------------------------
+// Records of the two breadth-first recursions
+// ---------------------------------------------------------------------------
+// multipoleSplit1(c1, list): the list is pool[off, off + len)
+struct SplitTask { int c1, off, len, pad; };
+// multipoleFinish(c1, list, mp, mink_zeta, maxr): mp lives in slot `slot`; `spare` is a
+// free slot for the right child of a split; fresh: the slot's scratch starts at zero.
+struct MPTask {
+    int c1, off, len, mink;
+    double maxr;
+    int slot, spare, fresh, pad;
+};
+// calculateZeta(c1, mp, ks, mk) with mp in slot `slot` (c1 < 0: no job)
+struct MPJob { int c1, slot, ks, mk; };
 
-// Corr3::calculateZeta + MultipoleHelper<0>/<5>, written by the whole workgroup into
-// this workgroup's output slot.  Each (k2, k3>=k2, m>=0) triple is one work item; the
-// outputs it touches are disjoint from every other item's.
-template <int D>
-__device__ void zeta_block(const ScratchLayout& Lo, const double* sc, const DNode& c1,
-                           int kstart, int mink, const C3Params& P, double* slot, int ntot) {
-    const int nb = P.nbins, M = P.maxn, nn = 2 * M + 1;
-    const double w1 = c1.w, n1 = double(c1.n);
-    const double g1r = c1.a, g1i = c1.b;
-    // items: for k2 in [kstart, mink): (nb - k2) * (M + 1)
-    int nitems = 0;
-    for (int k2 = kstart; k2 < mink; ++k2) nitems += (nb - k2) * (M + 1);
-    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
-        int rem = it, k2 = kstart;
-        while (rem >= (nb - k2) * (M + 1)) { rem -= (nb - k2) * (M + 1); ++k2; }
-        const int k3 = k2 + rem / (M + 1);
-        const int m = rem % (M + 1);
-        const double np2 = sc[Lo.S(S_NPAIRS, k2)], sw2 = sc[Lo.S(S_SUMW, k2)];
-        if (k3 == k2) {
-            const int iz22 = (k2 * nb + k2) * nn + M;
-            if (m == 0 && !P.quick) {
-                const double swr = sc[Lo.S(S_SUMWR, k2)], swlr = sc[Lo.S(S_SUMWLOGR, k2)];
-                const double swwr = sc[Lo.S(S_SUMWWR, k2)], swwlr = sc[Lo.S(S_SUMWWLOGR, k2)];
-                slot[O_NTRI * ntot + iz22] += n1 * np2 * (np2 - 1);
-                slot[O_MEAND2 * ntot + iz22] += w1 * (sw2 * swr - swwr);
-                slot[O_MEANLOGD2 * ntot + iz22] += w1 * (sw2 * swlr - swwlr);
-                slot[O_MEAND3 * ntot + iz22] += w1 * (sw2 * swr - swwr);
-                slot[O_MEANLOGD3 * ntot + iz22] += w1 * (sw2 * swlr - swwlr);
-            }
-            const double wr = sc[Lo.W(k2, m)], wi = sc[Lo.W(k2, m) + 1];
-            const double www = w1 * ((wr * wr + wi * wi) - sc[Lo.S(S_SUMWW, k2)]);
-            slot[O_WEIGHT * ntot + iz22 + m] += www;
-            if (m > 0) slot[O_WEIGHT * ntot + iz22 - m] += www;
-        } else {
-            const int i23 = (k2 * nb + k3) * nn + M;
-            const int i32 = (k3 * nb + k2) * nn + M;
-            if (m == 0 && !P.quick) {
-                const double np3 = sc[Lo.S(S_NPAIRS, k3)], sw3 = sc[Lo.S(S_SUMW, k3)];
-                const double nnn = n1 * np2 * np3;
-                slot[O_NTRI * ntot + i23] += nnn;
-                slot[O_NTRI * ntot + i32] += nnn;
-                const double ww12 = w1 * sw2, ww13 = w1 * sw3;
-                const double wwwd2 = ww13 * sc[Lo.S(S_SUMWR, k2)];
-                slot[O_MEAND2 * ntot + i23] += wwwd2;
-                slot[O_MEAND3 * ntot + i32] += wwwd2;
-                const double wwwlogd2 = ww13 * sc[Lo.S(S_SUMWLOGR, k2)];
-                slot[O_MEANLOGD2 * ntot + i23] += wwwlogd2;
-                slot[O_MEANLOGD3 * ntot + i32] += wwwlogd2;
-                const double wwwd3 = ww12 * sc[Lo.S(S_SUMWR, k3)];
-                slot[O_MEAND3 * ntot + i23] += wwwd3;
-                slot[O_MEAND2 * ntot + i32] += wwwd3;
-                const double wwwlogd3 = ww12 * sc[Lo.S(S_SUMWLOGR, k3)];
-                slot[O_MEANLOGD3 * ntot + i23] += wwwlogd3;
-                slot[O_MEANLOGD2 * ntot + i32] += wwwlogd3;
-            }
-            // weight: w1 * Wn[k2,m] * conj(Wn[k3,m])
-            const double ar = w1 * sc[Lo.W(k2, m)], ai = w1 * sc[Lo.W(k2, m) + 1];
-            double xr, xi;
-            cmul3(ar, ai, sc[Lo.W(k3, m)], -sc[Lo.W(k3, m) + 1], xr, xi);
-            slot[O_WEIGHT * ntot + i23 + m] += xr;
-            slot[O_WEIGHT_IM * ntot + i23 + m] += xi;
-            slot[O_WEIGHT * ntot + i32 + m] += xr;
-            slot[O_WEIGHT_IM * ntot + i32 + m] -= xi;
-            if (m > 0) {
-                slot[O_WEIGHT * ntot + i23 - m] += xr;
-                slot[O_WEIGHT_IM * ntot + i23 - m] -= xi;
-                slot[O_WEIGHT * ntot + i32 - m] += xr;
-                slot[O_WEIGHT_IM * ntot + i32 - m] += xi;
+// Error bits reported through `err` (all of them end the kernel cleanly).
+enum { E_POOL = 1, E_INDEX = 2, E_SLOT = 4 };
+
+// Scratch slots in chunks of 2^SLOT_SHIFT; the chunk table is a device array.
+constexpr int SLOT_SHIFT = 12;
+constexpr int SLOT_MASK = (1 << SLOT_SHIFT) - 1;
+__device__ __forceinline__ double* slot_at(double* const* chunks, int s, int total) {
+    return chunks[s >> SLOT_SHIFT] + (size_t)(s & SLOT_MASK) * (size_t)total;
+}
+
+__device__ __forceinline__ bool has_children(const DNode& c, int nnodes) {
+    return c.child >= 0 && c.child < nnodes - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup helpers (every thread of the workgroup calls them)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int block_excl_scan(int v, int* sh, int& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    int base = 0;
+    total = 0;
+    for (int k = 0; k < nw; ++k) {
+        const int s = sh[k];
+        if (k < wv) base += s;
+        total += s;
+    }
+    __syncthreads();
+    return base + x - v;
+}
+
+__device__ __forceinline__ double block_max(double v, double* shd) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_max(v);
+    if (lane == 0) shd[wv] = v;
+    __syncthreads();
+    double m = shd[0];
+    for (int k = 1; k < nw; ++k) m = fmax(m, shd[k]);
+    __syncthreads();
+    return m;
+}
+
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v,
+                                                            unsigned long long* shl) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) shl[wv] = v;
+    __syncthreads();
+    unsigned long long s = 0;
+    for (int k = 0; k < nw; ++k) s += shl[k];
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// The per-c2 decisions
+// ---------------------------------------------------------------------------
+// splitC2Cells (src/Corr3.cpp:1323-1372): 0 dropped, 1 kept, 2 replaced by its children
+template <int C>
+__device__ __forceinline__ int split_class(const C3Params& P, const DNode& c1, double s1,
+                                           const DNode& c2) {
+    const double s2 = c2.size;
+    const double rsq = dsq3<C>(c1, c2);
+    const double s1ps2 = s1 + s2;
+    if (too_small3(P, rsq, s1ps2) || too_large3(P, rsq, s1ps2)) return 0;
+    int k = -1;
+    double r = 0., logr = 0.;
+    const bool split = !single_bin3(P, rsq, s1ps2, k, r, logr);
+    return (split && s2 > s1) ? 2 : 1;
+}
+
+// splitC2CellsOrCalculateGn (src/Corr3.cpp:1486-1545) for one c2: 0 dropped, 1 accepted
+// into bin k (with r, logr), 2 kept for the next stage (children if split2), with its
+// largest remaining separation rtot and whether c1 needs a split for it (CalcSplitSq,
+// include/Split.h:82-101).
+template <int C>
+__device__ __forceinline__ int finish_class(const C3Params& P, const DNode& c1, double s1,
+                                            const DNode& c2, int& k, double& r, double& logr,
+                                            double& rtot, bool& split1, bool& split2) {
+    const double s2 = c2.size;
+    const double rsq = dsq3<C>(c1, c2);
+    const double s1ps2 = s1 + s2;
+    if (too_small3(P, rsq, s1ps2) || too_large3(P, rsq, s1ps2)) return 0;
+    k = -1;
+    r = sqrt(rsq);
+    logr = 0.;
+    if (single_bin3(P, rsq, s1ps2, k, r, logr)) {
+        if (!(rsq >= P.minsepsq && rsq < P.maxsepsq)) return 0;
+        if (k < 0) {
+            logr = log(r);
+            k = int((logr - P.logminsep) / P.binsize);
+        }
+        return 1;
+    }
+    rtot = r + s1ps2;
+    const double bsq_eff = rsq * (P.asq < P.bsq ? P.asq : P.bsq);
+    if (s2 > s1) {
+        split2 = true;
+        split1 = (s2 > 2. * s1) ? false : (s1 * s1 > 0.3422 * bsq_eff);
+    } else if (s1 > 2. * s2) {
+        split1 = true;
+        split2 = false;
+    } else {
+        split1 = true;
+        split2 = s2 * s2 > 0.3422 * bsq_eff;
+    }
+    return 2;
+}
+
+// ---------------------------------------------------------------------------
+// Phase 1: one level of multipoleSplit1
+// ---------------------------------------------------------------------------
+__global__ void k_split_init(const int* __restrict__ top, int ntop, SplitTask* __restrict__ tasks,
+                             int* __restrict__ pool) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntop) return;
+    pool[i] = top[i];
+    tasks[i] = SplitTask{top[i], 0, ntop, 0};
+}
+
+template <int C>
+__global__ __launch_bounds__(MP_THREADS) void k_mp_split_level(
+    const DNode* __restrict__ nodes, int nnodes, const SplitTask* __restrict__ tasks,
+    int* __restrict__ pool, unsigned long long* pool_top, unsigned long long pool_cap, C3Params P,
+    SplitTask* __restrict__ child_out, SplitTask* __restrict__ unit_out, int* err) {
+    __shared__ int sh[MP_THREADS / 64];
+    __shared__ unsigned long long s_base;
+    __shared__ int s_bad;
+    const int t = blockIdx.x;
+    const SplitTask tk = tasks[t];
+    const SplitTask none{-1, 0, 0, 0};
+    const bool ok = (unsigned)tk.c1 < (unsigned)nnodes && tk.off >= 0 && tk.len >= 0 &&
+                    (unsigned long long)tk.off + (unsigned long long)tk.len <= pool_cap;
+    if (!ok) {
+        if (threadIdx.x == 0) {
+            atomicOr(err, E_INDEX);
+            child_out[2 * t] = none; child_out[2 * t + 1] = none; unit_out[t] = none;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    const DNode c1 = nodes[tk.c1];
+    const double s1 = c1.size;
+    const int* list = pool + tk.off;
+    int total = 0;
+    for (int base = 0; base < tk.len; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        int no = 0;
+        if (i < tk.len) {
+            const int i2 = list[i];
+            if ((unsigned)i2 >= (unsigned)nnodes) {
+                s_bad = 1;
+            } else {
+                const DNode c2 = nodes[i2];
+                const int cl = split_class<C>(P, c1, s1, c2);
+                if (cl == 2 && !has_children(c2, nnodes)) s_bad = 1;
+                else no = cl;
             }
         }
-        if (D != GData) continue;
-        // MultipoleHelper<5>::CalculateZeta, src/Corr3.cpp:2582-2849
-        auto Gr = [&](int k, int n) { return sc[Lo.G(k, n)]; };
-        auto Gi = [&](int k, int n) { return sc[Lo.G(k, n) + 1]; };
-        auto add8 = [&](int idx, double a0r, double a0i, double a1r, double a1i, double a2r,
-                        double a2i, double a3r, double a3i) {
-            slot[(O_Z0 + 0) * ntot + idx] += a0r; slot[(O_Z0 + 1) * ntot + idx] += a0i;
-            slot[(O_Z0 + 2) * ntot + idx] += a1r; slot[(O_Z0 + 3) * ntot + idx] += a1i;
-            slot[(O_Z0 + 4) * ntot + idx] += a2r; slot[(O_Z0 + 5) * ntot + idx] += a2i;
-            slot[(O_Z0 + 6) * ntot + idx] += a3r; slot[(O_Z0 + 7) * ntot + idx] += a3i;
-        };
-        const double g1cr = g1r, g1ci = -g1i;
-        if (k3 == k2) {
-            const int iz22 = (k2 * nb + k2) * nn + M;
-            double c0r, c0i, c1r, c1i, c2r, c2i;
-            cmul3(g1r, g1i, sc[Lo.GG(0, k2)], sc[Lo.GG(0, k2) + 1], c0r, c0i);
-            cmul3(g1cr, g1ci, sc[Lo.GG(1, k2)], sc[Lo.GG(1, k2) + 1], c1r, c1i);
-            cmul3(g1r, g1i, sc[Lo.GG(2, k2)], sc[Lo.GG(2, k2) + 1], c2r, c2i);
-            if (m == 0) {
-                double ar, ai, br, bi, tr, ti;
-                cmul3(g1r, g1i, Gr(k2, -1), Gi(k2, -1), ar, ai);     // g1G2m
-                cmul3(g1cr, g1ci, Gr(k2, 1), Gi(k2, 1), br, bi);     // g1cG2p
-                double x0r, x0i, x1r, x1i, x2r, x2i;
-                cmul3(ar, ai, Gr(k2, -1), Gi(k2, -1), tr, ti); x0r = tr - c0r; x0i = ti - c0i;
-                cmul3(br, bi, Gr(k2, 1), Gi(k2, 1), tr, ti); x1r = tr - c1r; x1i = ti - c1i;
-                cmul3(ar, ai, Gr(k2, 1), -Gi(k2, 1), tr, ti); x2r = tr - c2r; x2i = ti - c2i;
-                add8(iz22, x0r, x0i, x1r, x1i, x2r, x2i, x2r, x2i);
-            } else {
-                const double ppr = Gr(k2, m + 1), ppi = Gi(k2, m + 1);
-                const double pmr = Gr(k2, m - 1), pmi = Gi(k2, m - 1);
-                const double mpr = Gr(k2, -m + 1), mpi = Gi(k2, -m + 1);
-                const double mmr = Gr(k2, -m - 1), mmi = Gi(k2, -m - 1);
-                double ar, ai, br, bi, tr, ti;
-                cmul3(g1r, g1i, mmr, mmi, ar, ai);       // g1G2mm
-                cmul3(g1cr, g1ci, ppr, ppi, br, bi);     // g1cG2pp
-                double x0r, x0i, x1r, x1i, x2r, x2i, x3r, x3i;
-                cmul3(ar, ai, pmr, pmi, tr, ti); x0r = tr - c0r; x0i = ti - c0i;
-                cmul3(br, bi, mpr, mpi, tr, ti); x1r = tr - c1r; x1i = ti - c1i;
-                cmul3(br, -bi, pmr, pmi, tr, ti); x2r = tr - c2r; x2i = ti - c2i;
-                cmul3(ar, ai, mpr, -mpi, tr, ti); x3r = tr - c2r; x3i = ti - c2i;
-                add8(iz22 + m, x0r, x0i, x1r, x1i, x2r, x2i, x3r, x3i);
-                add8(iz22 - m, x0r, x0i, x1r, x1i, x3r, x3i, x2r, x2i);
-            }
+        int tot;
+        block_excl_scan(no, sh, tot);
+        total += tot;
+    }
+    if (threadIdx.x == 0) s_base = total > 0 ? atomicAdd(pool_top, (unsigned long long)total) : 0ull;
+    __syncthreads();
+    const unsigned long long obase = s_base;
+    const bool split_c1 = s1 * s1 > P.maxbsq_eff;
+    const int bad = s_bad ? E_INDEX : (obase + (unsigned long long)total > pool_cap ||
+                                       obase + (unsigned long long)total > 0x7fffffffull) ? E_POOL
+                    : (split_c1 && !has_children(c1, nnodes)) ? E_INDEX : 0;
+    if (bad) {
+        if (threadIdx.x == 0) {
+            atomicOr(err, bad);
+            child_out[2 * t] = none; child_out[2 * t + 1] = none; unit_out[t] = none;
+        }
+        return;
+    }
+    int wpos = 0;
+    for (int base = 0; base < tk.len; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        int no = 0, o0 = 0;
+        if (i < tk.len) {
+            const int i2 = list[i];
+            const DNode c2 = nodes[i2];
+            no = split_class<C>(P, c1, s1, c2);
+            o0 = no == 2 ? c2.child : i2;
+        }
+        int tot;
+        const int ex = block_excl_scan(no, sh, tot);
+        if (no >= 1) pool[obase + wpos + ex] = o0;
+        if (no == 2) pool[obase + wpos + ex + 1] = o0 + 1;
+        wpos += tot;
+    }
+    if (threadIdx.x == 0) {
+        const int noff = (int)obase;
+        if (split_c1) {
+            child_out[2 * t] = SplitTask{c1.child, noff, total, 0};
+            child_out[2 * t + 1] = SplitTask{c1.child + 1, noff, total, 0};
+            unit_out[t] = none;
         } else {
-            const int iz23 = (k2 * nb + k3) * nn + M;
-            const int iz32 = (k3 * nb + k2) * nn + M;
-            if (m == 0) {
-                double ar, ai, br, bi;
-                cmul3(g1r, g1i, Gr(k2, -1), Gi(k2, -1), ar, ai);     // g1G2m
-                cmul3(g1cr, g1ci, Gr(k2, 1), Gi(k2, 1), br, bi);     // g1cG2p
-                double a0r, a0i, a1r, a1i, a2r, a2i, a3r, a3i;
-                cmul3(ar, ai, Gr(k3, -1), Gi(k3, -1), a0r, a0i);
-                cmul3(br, bi, Gr(k3, 1), Gi(k3, 1), a1r, a1i);
-                cmul3(ar, ai, Gr(k3, 1), -Gi(k3, 1), a2r, a2i);
-                cmul3(br, -bi, Gr(k3, -1), Gi(k3, -1), a3r, a3i);
-                add8(iz23, a0r, a0i, a1r, a1i, a2r, a2i, a3r, a3i);
-                add8(iz32, a0r, a0i, a1r, a1i, a3r, a3i, a2r, a2i);
-            } else {
-                const double G2ppr = Gr(k2, m + 1), G2ppi = Gi(k2, m + 1);
-                const double G2pmr = Gr(k2, m - 1), G2pmi = Gi(k2, m - 1);
-                const double G2mpr = Gr(k2, -m + 1), G2mpi = Gi(k2, -m + 1);
-                const double G2mmr = Gr(k2, -m - 1), G2mmi = Gi(k2, -m - 1);
-                const double G3ppr = Gr(k3, m + 1), G3ppi = Gi(k3, m + 1);
-                const double G3pmr = Gr(k3, m - 1), G3pmi = Gi(k3, m - 1);
-                const double G3mpr = Gr(k3, -m + 1), G3mpi = Gi(k3, -m + 1);
-                const double G3mmr = Gr(k3, -m - 1), G3mmi = Gi(k3, -m - 1);
-                double pmR, pmI, cpR, cpI, mcR, mcI, mmR, mmI;
-                cmul3(g1r, g1i, G2pmr, G2pmi, pmR, pmI);        // g1G2pm
-                cmul3(g1cr, g1ci, G2ppr, G2ppi, cpR, cpI);      // g1cG2pp
-                cmul3(g1r, g1i, G2mpr, -G2mpi, mcR, mcI);       // g1G2mpc
-                cmul3(g1r, g1i, G2mmr, G2mmi, mmR, mmI);        // g1G2mm
-                double p0r, p0i, p1r, p1i, p2r, p2i, p3r, p3i;
-                cmul3(pmR, pmI, G3mmr, G3mmi, p0r, p0i);
-                cmul3(cpR, cpI, G3mpr, G3mpi, p1r, p1i);
-                cmul3(pmR, pmI, G3ppr, -G3ppi, p2r, p2i);
-                cmul3(mcR, mcI, G3mmr, G3mmi, p3r, p3i);
-                add8(iz23 + m, p0r, p0i, p1r, p1i, p2r, p2i, p3r, p3i);
-                add8(iz32 - m, p0r, p0i, p1r, p1i, p3r, p3i, p2r, p2i);
-                double q0r, q0i, q1r, q1i, q2r, q2i, q3r, q3i;
-                cmul3(mmR, mmI, G3pmr, G3pmi, q0r, q0i);
-                cmul3(mcR, -mcI, G3ppr, G3ppi, q1r, q1i);
-                cmul3(cpR, -cpI, G3pmr, G3pmi, q2r, q2i);
-                cmul3(mmR, mmI, G3mpr, -G3mpi, q3r, q3i);
-                add8(iz32 + m, q0r, q0i, q1r, q1i, q2r, q2i, q3r, q3i);
-                add8(iz23 - m, q0r, q0i, q1r, q1i, q3r, q3i, q2r, q2i);
-            }
+            child_out[2 * t] = none; child_out[2 * t + 1] = none;
+            unit_out[t] = SplitTask{tk.c1, noff, total, 0};
         }
     }
 }
 
-// Task record in LDS: every control value of the depth-first walk is read from
-// memory by thread 0 only and broadcast through LDS, so all waves of the workgroup
-// always agree on it.
-enum { T_C1 = 0, T_OFF, T_LEN, T_MINK, T_MAXR_LO, T_MAXR_HI, T_KIND, T_N };
-enum { TASK_NONE = 0, TASK_POP = 1, TASK_UNIT = 2, TASK_BAD = 3, TASK_STOP = 4 };
+// ---------------------------------------------------------------------------
+// Order-preserving compaction of fixed-position records (c1 < 0 or id < 0: empty)
+// ---------------------------------------------------------------------------
+constexpr int CP_TPB = 256, CP_ITEMS = 4, CP_BLOCK = CP_TPB * CP_ITEMS;
+
+__device__ __forceinline__ bool rec_valid(const SplitTask& r) { return r.c1 >= 0; }
+__device__ __forceinline__ bool rec_valid(const MPTask& r) { return r.c1 >= 0; }
+__device__ __forceinline__ bool rec_valid(const int& r) { return r >= 0; }
+__device__ __forceinline__ unsigned long long rec_len(const SplitTask& r) { return (unsigned)r.len; }
+__device__ __forceinline__ unsigned long long rec_len(const MPTask& r) { return (unsigned)r.len; }
+__device__ __forceinline__ unsigned long long rec_len(const int&) { return 0ull; }
+
+template <class R>
+__global__ __launch_bounds__(CP_TPB) void k_cp_count(const R* __restrict__ in, long long n,
+                                                      int* __restrict__ bcount,
+                                                      unsigned long long* __restrict__ blen) {
+    __shared__ int sh[CP_TPB / 64];
+    __shared__ unsigned long long shl[CP_TPB / 64];
+    const long long base = (long long)blockIdx.x * CP_BLOCK;
+    int c = 0;
+    unsigned long long l = 0;
+    for (int q = 0; q < CP_ITEMS; ++q) {
+        const long long i = base + q * CP_TPB + threadIdx.x;
+        if (i < n) {
+            const R r = in[i];
+            if (rec_valid(r)) { ++c; l += rec_len(r); }
+        }
+    }
+    int tot;
+    block_excl_scan(c, sh, tot);
+    const unsigned long long ltot = block_sum_u64(l, shl);
+    if (threadIdx.x == 0) { bcount[blockIdx.x] = tot; blen[blockIdx.x] = ltot; }
+}
+
+// one workgroup: exclusive offsets of the block counts; totals = {count, sum of len}
+__global__ __launch_bounds__(1024) void k_cp_scan(const int* __restrict__ bcount,
+                                                  const unsigned long long* __restrict__ blen,
+                                                  int nblk, int* __restrict__ boff,
+                                                  unsigned long long* __restrict__ totals) {
+    __shared__ int sh[1024 / 64];
+    __shared__ unsigned long long shl[1024 / 64];
+    const int per = (nblk + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int b0 = threadIdx.x * per;
+    int c = 0;
+    unsigned long long l = 0;
+    for (int b = b0; b < b0 + per && b < nblk; ++b) { c += bcount[b]; l += blen[b]; }
+    int tot;
+    const int ex = block_excl_scan(c, sh, tot);
+    const unsigned long long ltot = block_sum_u64(l, shl);
+    int run = ex;
+    for (int b = b0; b < b0 + per && b < nblk; ++b) { boff[b] = run; run += bcount[b]; }
+    if (threadIdx.x == 0) { totals[0] = (unsigned long long)tot; totals[1] = ltot; }
+}
+
+template <class R>
+__global__ __launch_bounds__(CP_TPB) void k_cp_scatter(const R* __restrict__ in, long long n,
+                                                        const int* __restrict__ boff,
+                                                        R* __restrict__ out, long long out_base) {
+    __shared__ int sh[CP_TPB / 64];
+    const long long base = (long long)blockIdx.x * CP_BLOCK;
+    long long run = out_base + boff[blockIdx.x];
+    for (int q = 0; q < CP_ITEMS; ++q) {
+        const long long i = base + q * CP_TPB + threadIdx.x;
+        R r{};
+        int v = 0;
+        if (i < n) {
+            r = in[i];
+            v = rec_valid(r) ? 1 : 0;
+        }
+        int tot;
+        const int ex = block_excl_scan(v, sh, tot);
+        if (v) out[run + ex] = r;
+        run += tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Phase 2: one level of multipoleFinish stages
+// ---------------------------------------------------------------------------
+__global__ void k_units_to_tasks(const SplitTask* __restrict__ units, int n, int nbins,
+                                 MPTask* __restrict__ tasks) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SplitTask u = units[i];
+    MPTask t;
+    t.c1 = u.c1; t.off = u.off; t.len = u.len; t.mink = nbins; t.maxr = 0.;
+    t.slot = -1; t.spare = -1; t.fresh = 1; t.pad = 0;
+    tasks[i] = t;
+}
+
+// Spare slot for every task (the right child of a split), and a slot for tasks that have
+// none yet, from the top of the free list in task order.
+__global__ void k_mp_prep(MPTask* __restrict__ tasks, int n, const int* __restrict__ free_ids,
+                          int free_top) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    MPTask t = tasks[i];
+    t.spare = free_ids[free_top - 1 - i];
+    if (t.slot < 0) { t.slot = free_ids[free_top - 1 - n - i]; t.fresh = 1; }
+    tasks[i] = t;
+}
 
 template <int D, int C>
-__global__ __launch_bounds__(MP_THREADS) void k_mp_finish(
+__global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
     const DNode* __restrict__ nodes, int nnodes, const NodeSum* __restrict__ nsum,
-    const int* __restrict__ pool, unsigned long long pool_cap, const Unit* __restrict__ units,
-    unsigned int nunits, unsigned int* next_unit, unsigned long long* stage_count,
-    unsigned long long stage_budget, C3Params P, int* arena_all, long long arena_cap,
-    double* scratch_all,
-    int* stack_all, double* slots, int ntot, unsigned long long* counters, int* err) {
+    const MPTask* __restrict__ tasks, const int* __restrict__ pool_in,
+    unsigned long long pool_in_cap, int* __restrict__ pool_out, unsigned long long* pool_out_top,
+    unsigned long long pool_out_cap, double* const* __restrict__ slots, int nslots, C3Params P,
+    MPTask* __restrict__ child_out, MPJob* __restrict__ job_out, int* __restrict__ ret_out,
+    unsigned long long* __restrict__ counters, int* err) {
     HIP_DYNAMIC_SHARED(double, sc)
     __shared__ int sh[MP_THREADS / 64];
-    __shared__ int s_task[T_N];
-    __shared__ int s_anysplit, s_bad, s_stop;
-    __shared__ double s_maxr;
+    __shared__ double shd[MP_THREADS / 64];
+    __shared__ unsigned long long shl[MP_THREADS / 64];
+    __shared__ unsigned long long s_base;
+    __shared__ int s_bad;
     ScratchLayout Lo;
     Lo.init(P.nbins, P.maxn, D);
-    int* arena = arena_all + (size_t)blockIdx.x * arena_cap;
-    double* sstack = scratch_all + (size_t)blockIdx.x * MP_STACK * Lo.total;
-    int* st = stack_all + (size_t)blockIdx.x * MP_STACK * 8;
-    double* slot = slots + (size_t)blockIdx.x * NOUT3 * ntot;
-    unsigned long long c_gn = 0, c_zeta = 0, c_rows = 0, c_stage = 0;
-
-    int sp = 0;           // DFS stack depth (uniform)
-    int have = 0;         // current task valid (uniform)
-    int c1i = 0, loff = 0, llen = 0, mink = 0;
-    bool in_pool = true;  // current list lives in the unit pool (else in the arena)
-    double maxr = 0.;
-    long long atop = 0;   // arena top (uniform)
-    while (true) {
-        if (!have) {
-            // next task: the DFS stack of this workgroup, else a new unit
-            if (threadIdx.x == 0) {
-                int kind = TASK_NONE;
-                if (sp > 0) {
-                    const int* e = st + (sp - 1) * 8;     // written by this same thread
-                    for (int q = 0; q < 6; ++q) s_task[q] = e[q];
-                    kind = TASK_POP;
-                } else {
-                    const unsigned int u = atomicAdd(next_unit, 1u);
-                    if (u < nunits) {
-                        const Unit un = units[u];
-                        s_task[T_C1] = un.c1; s_task[T_OFF] = un.off; s_task[T_LEN] = un.len;
-                        s_task[T_MINK] = P.nbins; s_task[T_MAXR_LO] = 0; s_task[T_MAXR_HI] = 0;
-                        kind = TASK_UNIT;
-                    }
-                }
-                if (kind != TASK_NONE) {
-                    const int c = s_task[T_C1], o = s_task[T_OFF], l = s_task[T_LEN];
-                    const int mk = s_task[T_MINK];
-                    const unsigned long long cap = kind == TASK_UNIT ? pool_cap : (unsigned long long)arena_cap;
-                    if ((unsigned)c >= (unsigned)nnodes || o < 0 || l < 0 ||
-                        (unsigned long long)o + l > cap || mk < 0 || mk > P.nbins)
-                        kind = TASK_BAD;
-                }
-                s_task[T_KIND] = kind;
-            }
-            __syncthreads();
-            const int kind = s_task[T_KIND];
-            if (kind == TASK_NONE) break;
-            if (kind == TASK_BAD) {
-                if (threadIdx.x == 0) atomicOr(err, E_INDEX);
-                break;
-            }
-            c1i = s_task[T_C1]; loff = s_task[T_OFF]; llen = s_task[T_LEN]; mink = s_task[T_MINK];
-            maxr = __hiloint2double(s_task[T_MAXR_HI], s_task[T_MAXR_LO]);
-            if (kind == TASK_POP) {
-                --sp;
-                in_pool = false;
-                atop = (long long)loff + llen;
-                for (int t = threadIdx.x; t < Lo.total; t += blockDim.x)
-                    sc[t] = sstack[(size_t)sp * Lo.total + t];     // same thread wrote it
-            } else {
-                in_pool = true;
-                atop = 0;
-                for (int t = threadIdx.x; t < Lo.total; t += blockDim.x) sc[t] = 0.;
-            }
-            __syncthreads();
-            have = 1;
-        }
-        // ---- one multipoleFinish stage for (c1, list) ----
-        ++c_stage;
+    const int t = blockIdx.x;
+    const MPTask tk = tasks[t];
+    MPTask none;
+    none.c1 = -1; none.off = 0; none.len = 0; none.mink = 0; none.maxr = 0.;
+    none.slot = -1; none.spare = -1; none.fresh = 0; none.pad = 0;
+    const MPJob nojob{-1, -1, 0, 0};
+    const bool ok = (unsigned)tk.c1 < (unsigned)nnodes && tk.off >= 0 && tk.len >= 0 &&
+                    (unsigned long long)tk.off + (unsigned long long)tk.len <= pool_in_cap &&
+                    (unsigned)tk.slot < (unsigned)nslots && (unsigned)tk.spare < (unsigned)nslots &&
+                    tk.slot != tk.spare && tk.mink >= 0 && tk.mink <= P.nbins;
+    if (!ok) {
         if (threadIdx.x == 0) {
-            s_anysplit = 0; s_maxr = 0.; s_bad = 0;
-            s_stop = atomicAdd(stage_count, 1ull) >= stage_budget;
+            atomicOr(err, E_SLOT);
+            child_out[2 * t] = none; child_out[2 * t + 1] = none; job_out[t] = nojob;
+            ret_out[2 * t] = -1; ret_out[2 * t + 1] = -1;
         }
-        __syncthreads();
-        if (s_stop) {
-            if (threadIdx.x == 0) atomicOr(err, E_BUDGET);
-            break;
-        }
-        const DNode c1 = nodes[c1i];
-        const double s1 = c1.size;
-        const int* list = in_pool ? pool + loff : arena + loff;
-        const double thresh1 = (1. - P.binsize) * maxr;
-        const long long nlo = atop;      // new list starts at the arena top
-        int nlen = 0;
-        double lmax = 0.;
-        int lany = 0;
-        for (int base = 0; base < llen; base += blockDim.x) {
-            const int t = base + threadIdx.x;
-            int no = 0, o0 = 0, o1 = 0;
-            const int i2 = t < llen ? list[t] : 0;
-            if (t < llen && (unsigned)i2 >= (unsigned)nnodes) {
+        return;
+    }
+    double* gs = slot_at(slots, tk.slot, Lo.total);
+    if (tk.fresh) {
+        for (int i = threadIdx.x; i < Lo.total; i += blockDim.x) sc[i] = 0.;
+    } else {
+        for (int i = threadIdx.x; i < Lo.total; i += blockDim.x) sc[i] = gs[i];
+    }
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    const DNode c1 = nodes[tk.c1];
+    const double s1 = c1.size;
+    const double thresh1 = (1. - P.binsize) * tk.maxr;   // BinTypeHelper<Log>::oneBinLessThan
+    const int* list = pool_in + tk.off;
+    // pass 1: accepted pairs into the scratch, the size of the next list, the largest
+    // remaining separation and whether any c2 needs c1 split
+    int nlen = 0, lany = 0;
+    double lmax = 0.;
+    unsigned long long c_gn = 0;
+    for (int base = 0; base < tk.len; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        int no = 0;
+        if (i < tk.len) {
+            const int i2 = list[i];
+            if ((unsigned)i2 >= (unsigned)nnodes) {
                 s_bad = 1;
-            } else if (t < llen) {
+            } else {
                 const DNode c2 = nodes[i2];
-                const double s2 = c2.size;
-                const double rsq = dsq3<C>(c1, c2);
-                const double s1ps2 = s1 + s2;
-                if (!too_small3(P, rsq, s1ps2) && !too_large3(P, rsq, s1ps2)) {
-                    int k = -1;
-                    double r = sqrt(rsq), logr = 0.;
-                    if (single_bin3(P, rsq, s1ps2, k, r, logr)) {
-                        if (rsq >= P.minsepsq && rsq < P.maxsepsq) {
-                            if (k < 0) {
-                                logr = log(r);
-                                k = int((logr - P.logminsep) / P.binsize);
-                            }
-                            if (k >= 0 && k < P.nbins) {
-                                ++c_gn;
-                                add_gn<D, C>(Lo, sc, c1, c2, nsum[i2], r, logr, k, P.maxn);
-                            } else {
-                                s_bad = 1;     // r at the very edge of the range: no bin
-                            }
-                        }
+                int k = -1;
+                double r = 0., logr = 0., rtot = 0.;
+                bool sp1 = false, sp2 = false;
+                const int cl = finish_class<C>(P, c1, s1, c2, k, r, logr, rtot, sp1, sp2);
+                if (cl == 1) {
+                    if (k >= 0 && k < P.nbins) {
+                        ++c_gn;
+                        add_gn<D, C>(Lo, sc, c1, c2, nsum[i2], r, logr, k, P.maxn);
                     } else {
-                        const double rtot = r + s1ps2;
-                        lmax = fmax(lmax, rtot);
-                        const double bsq_eff = rsq * (P.asq < P.bsq ? P.asq : P.bsq);
-                        bool sp1, sp2;
-                        if (s2 > s1) {
-                            sp2 = true;
-                            sp1 = (s2 > 2. * s1) ? false : (s1 * s1 > 0.3422 * bsq_eff);
-                        } else if (s1 > 2. * s2) {
-                            sp1 = true; sp2 = false;
-                        } else {
-                            sp1 = true;
-                            sp2 = s2 * s2 > 0.3422 * bsq_eff;
-                        }
-                        if (sp1 && rtot > thresh1) lany = 1;
-                        if (sp2) {
-                            if (has_children(c2, nnodes)) { no = 2; o0 = c2.child; o1 = c2.child + 1; }
-                            else s_bad = 1;
-                        } else {
-                            no = 1; o0 = i2;
-                        }
+                        s_bad = 1;     // r at the very edge of the range: no bin
                     }
+                } else if (cl == 2) {
+                    lmax = fmax(lmax, rtot);
+                    if (sp1 && rtot > thresh1) lany = 1;
+                    if (sp2 && !has_children(c2, nnodes)) s_bad = 1;
+                    no = sp2 ? 2 : 1;
                 }
+            }
+        }
+        int tot;
+        block_excl_scan(no, sh, tot);
+        nlen += tot;
+    }
+    const double nmaxr = block_max(lmax, shd);
+    int nany;
+    block_excl_scan(lany, sh, nany);
+    const unsigned long long gn_tot = block_sum_u64(c_gn, shl);
+    if (threadIdx.x == 0) s_base = nlen > 0 ? atomicAdd(pool_out_top, (unsigned long long)nlen) : 0ull;
+    __syncthreads();
+    const unsigned long long obase = s_base;
+    const bool split = nlen > 0 && nany > 0;
+    const int bad = s_bad ? E_INDEX
+                    : (obase + (unsigned long long)nlen > pool_out_cap ||
+                       obase + (unsigned long long)nlen > 0x7fffffffull) ? E_POOL
+                    : (split && !has_children(c1, nnodes)) ? E_INDEX : 0;
+    if (bad) {
+        if (threadIdx.x == 0) {
+            atomicOr(err, bad);
+            child_out[2 * t] = none; child_out[2 * t + 1] = none; job_out[t] = nojob;
+            ret_out[2 * t] = -1; ret_out[2 * t + 1] = -1;
+        }
+        return;
+    }
+    // pass 2: the next list, in list order (the same decisions as pass 1)
+    if (nlen > 0) {
+        int wpos = 0;
+        for (int base = 0; base < tk.len; base += blockDim.x) {
+            const int i = base + threadIdx.x;
+            int no = 0, o0 = 0;
+            if (i < tk.len) {
+                const int i2 = list[i];
+                const DNode c2 = nodes[i2];
+                int k = -1;
+                double r = 0., logr = 0., rtot = 0.;
+                bool sp1 = false, sp2 = false;
+                const int cl = finish_class<C>(P, c1, s1, c2, k, r, logr, rtot, sp1, sp2);
+                if (cl == 2) { no = sp2 ? 2 : 1; o0 = sp2 ? c2.child : i2; }
             }
             int tot;
-            const int ex = block_scan_excl(no, sh, tot);
-            if (nlo + nlen + tot > arena_cap) {
-                if (threadIdx.x == 0) atomicOr(err, E_ARENA);
-            } else {
-                if (no > 0) arena[nlo + nlen + ex] = o0;
-                if (no > 1) arena[nlo + nlen + ex + 1] = o1;
-            }
-            nlen += tot;
-        }
-        if (lany) atomicOr(&s_anysplit, 1);
-        // block max of the remaining separation (all values >= 0)
-        {
-            double v = lmax;
-            for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-            if ((threadIdx.x & 63) == 0)
-                atomicMax((unsigned long long*)&s_maxr, (unsigned long long)__double_as_longlong(v));
-        }
-        // the next stage reads the new list from other waves
-        block_sync_global();
-        if (s_bad) {
-            if (threadIdx.x == 0) atomicOr(err, E_INDEX);
-            break;
-        }
-        if (nlo + nlen > arena_cap) break;   // overflow (flagged above): stop cleanly
-        const int anysplit1 = s_anysplit;
-        const double nmaxr = s_maxr;
-        __syncthreads();
-        if (nlen > 0) {
-            int nmink = mink;
-            if (nmaxr < P.maxsep) {
-                const int k = nmaxr < P.minsep ? 0
-                              : int((log(nmaxr) - P.logminsep) / P.binsize) + 1;
-                if (k >= 0 && k < mink) {
-                    ++c_zeta; c_rows += (unsigned long long)(mink - k);
-                    zeta_block<D>(Lo, sc, c1, k, mink, P, slot, ntot);
-                    block_sync_global();
-                    nmink = k;
-                }
-            }
-            if (anysplit1) {
-                if (sp >= MP_STACK || !has_children(c1, nnodes)) {
-                    if (threadIdx.x == 0) atomicOr(err, sp >= MP_STACK ? E_STACK : E_INDEX);
-                    break;
-                }
-                if (threadIdx.x == 0) {
-                    int* e = st + sp * 8;
-                    e[0] = c1.child + 1; e[1] = (int)nlo; e[2] = nlen; e[3] = nmink;
-                    e[4] = __double2loint(nmaxr); e[5] = __double2hiint(nmaxr);
-                }
-                for (int t = threadIdx.x; t < Lo.total; t += blockDim.x)
-                    sstack[(size_t)sp * Lo.total + t] = sc[t];
-                ++sp;
-                c1i = c1.child;
-            }
-            loff = (int)nlo; llen = nlen; in_pool = false; mink = nmink; maxr = nmaxr;
-            atop = nlo + nlen;
-        } else {
-            ++c_zeta; c_rows += (unsigned long long)mink;
-            zeta_block<D>(Lo, sc, c1, 0, mink, P, slot, ntot);
-            block_sync_global();
-            have = 0;
+            const int ex = block_excl_scan(no, sh, tot);
+            if (no >= 1) pool_out[obase + wpos + ex] = o0;
+            if (no == 2) pool_out[obase + wpos + ex + 1] = o0 + 1;
+            wpos += tot;
         }
     }
-    atomicAdd(&counters[C3_GN], c_gn);
+    // multipoleFinish's decisions after splitC2CellsOrCalculateGn (src/Corr3.cpp:1574-1611)
+    MPJob job = nojob;
+    int nmink = tk.mink;
+    if (nlen > 0) {
+        if (nmaxr < P.maxsep) {
+            const int k = nmaxr < P.minsep ? 0 : int((log(nmaxr) - P.logminsep) / P.binsize) + 1;
+            if (k >= 0 && k < tk.mink) { job = MPJob{tk.c1, tk.slot, k, tk.mink}; nmink = k; }
+        }
+    } else {
+        job = MPJob{tk.c1, tk.slot, 0, tk.mink};
+    }
+    // the scratch after this stage: this level's zeta reads it from the slot; the right
+    // child of a split continues from a copy (mp.duplicate())
+    double* gsp = split ? slot_at(slots, tk.spare, Lo.total) : nullptr;
+    for (int i = threadIdx.x; i < Lo.total; i += blockDim.x) {
+        const double v = sc[i];
+        gs[i] = v;
+        if (split) gsp[i] = v;
+    }
     if (threadIdx.x == 0) {
-        atomicAdd(&counters[C3_ZETA], c_zeta);
-        atomicAdd(&counters[C3_ROWS], c_rows);
-        atomicAdd(&counters[C3_STAGES], c_stage);
+        job_out[t] = job;
+        MPTask a = none, b = none;
+        int r0 = -1, r1 = -1;
+        if (nlen > 0) {
+            a.off = (int)obase; a.len = nlen; a.mink = nmink; a.maxr = nmaxr;
+            if (split) {
+                b = a;
+                a.c1 = c1.child; a.slot = tk.slot;
+                b.c1 = c1.child + 1; b.slot = tk.spare;
+            } else {
+                a.c1 = tk.c1; a.slot = tk.slot;
+                r0 = tk.spare;
+            }
+        } else {
+            r0 = tk.spare;
+            r1 = tk.slot;       // free after this level's zeta
+        }
+        child_out[2 * t] = a; child_out[2 * t + 1] = b;
+        ret_out[2 * t] = r0; ret_out[2 * t + 1] = r1;
+        atomicAdd(&counters[C3_GN], gn_tot);
+        atomicAdd(&counters[C3_STAGES], 1ull);
+        if (job.c1 >= 0) {
+            atomicAdd(&counters[C3_ZETA], 1ull);
+            atomicAdd(&counters[C3_ROWS], (unsigned long long)(job.mk - job.ks));
+        }
     }
 }
 
-__global__ void k_mp_reduce(const double* slots, int nslots, long long n, double* out) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double s = 0., c = 0.;
-    for (int k = 0; k < nslots; ++k) {
-        const double v = slots[(size_t)k * n + t];
+// ---------------------------------------------------------------------------
+// Phase 3: zeta of the jobs of one level
+// ---------------------------------------------------------------------------
+// Compressed outputs per (k2, k3 >= k2, m >= 0) cell, NV doubles:
+//   [0,1]      w1 W(k2,m) W(k3,m)*  (k3 == k2: w1 (|W(k2,m)|^2 - sumww), real)
+//   GGG [2,9]  P0..P3: zeta0..3 at (k2, k3, +m); [10,17] Q0..Q3: zeta0..3 at (k3, k2, +m)
+//   last 5     ntri, meand2, meanlogd2, meand3, meanlogd3 at (k2, k3, n = 0) (m = 0 only)
+// The other entries of the reference's arrays follow from these (k_mp_expand).
+__host__ __device__ constexpr int zeta_nv(int D) { return D == GData ? 23 : 7; }
+
+template <int D>
+__device__ __forceinline__ void zeta_cell(const ScratchLayout& Lo, const double* __restrict__ sc,
+                                          const DNode& c1, int k2, int k3, int m,
+                                          const C3Params& P, double* __restrict__ v) {
+    const double w1 = c1.w, n1 = double(c1.n);
+    constexpr int NV = zeta_nv(D);
+    double* two = v + NV - 5;
+    // Corr3::calculateZeta, src/Corr3.cpp:1872-1940
+    if (k3 == k2) {
+        if (m == 0 && !P.quick) {
+            const double np2 = sc[Lo.S(S_NPAIRS, k2)], sw2 = sc[Lo.S(S_SUMW, k2)];
+            const double swr = sc[Lo.S(S_SUMWR, k2)], swlr = sc[Lo.S(S_SUMWLOGR, k2)];
+            const double swwr = sc[Lo.S(S_SUMWWR, k2)], swwlr = sc[Lo.S(S_SUMWWLOGR, k2)];
+            two[0] += n1 * np2 * (np2 - 1);
+            two[1] += w1 * (sw2 * swr - swwr);
+            two[2] += w1 * (sw2 * swlr - swwlr);
+            two[3] += w1 * (sw2 * swr - swwr);
+            two[4] += w1 * (sw2 * swlr - swwlr);
+        }
+        const double wr = sc[Lo.W(k2, m)], wi = sc[Lo.W(k2, m) + 1];
+        v[0] += w1 * ((wr * wr + wi * wi) - sc[Lo.S(S_SUMWW, k2)]);
+    } else {
+        if (m == 0 && !P.quick) {
+            const double np2 = sc[Lo.S(S_NPAIRS, k2)], sw2 = sc[Lo.S(S_SUMW, k2)];
+            const double np3 = sc[Lo.S(S_NPAIRS, k3)], sw3 = sc[Lo.S(S_SUMW, k3)];
+            const double ww12 = w1 * sw2, ww13 = w1 * sw3;
+            two[0] += n1 * np2 * np3;
+            two[1] += ww13 * sc[Lo.S(S_SUMWR, k2)];
+            two[2] += ww13 * sc[Lo.S(S_SUMWLOGR, k2)];
+            two[3] += ww12 * sc[Lo.S(S_SUMWR, k3)];
+            two[4] += ww12 * sc[Lo.S(S_SUMWLOGR, k3)];
+        }
+        const double ar = w1 * sc[Lo.W(k2, m)], ai = w1 * sc[Lo.W(k2, m) + 1];
+        double xr, xi;
+        cmul3(ar, ai, sc[Lo.W(k3, m)], -sc[Lo.W(k3, m) + 1], xr, xi);
+        v[0] += xr;
+        v[1] += xi;
+    }
+    if (D != GData) return;
+    // MultipoleHelper<5>::CalculateZeta, src/Corr3.cpp:2582-2849
+    auto Gr = [&](int k, int n) { return sc[Lo.G(k, n)]; };
+    auto Gi = [&](int k, int n) { return sc[Lo.G(k, n) + 1]; };
+    double* Pv = v + 2;
+    double* Qv = v + 10;
+    const double g1r = c1.a, g1i = c1.b;
+    const double g1cr = g1r, g1ci = -g1i;
+    if (k3 == k2) {
+        double c0r, c0i, c1r, c1i, c2r, c2i;
+        cmul3(g1r, g1i, sc[Lo.GG(0, k2)], sc[Lo.GG(0, k2) + 1], c0r, c0i);
+        cmul3(g1cr, g1ci, sc[Lo.GG(1, k2)], sc[Lo.GG(1, k2) + 1], c1r, c1i);
+        cmul3(g1r, g1i, sc[Lo.GG(2, k2)], sc[Lo.GG(2, k2) + 1], c2r, c2i);
+        if (m == 0) {
+            double ar, ai, br, bi, tr, ti;
+            cmul3(g1r, g1i, Gr(k2, -1), Gi(k2, -1), ar, ai);
+            cmul3(g1cr, g1ci, Gr(k2, 1), Gi(k2, 1), br, bi);
+            cmul3(ar, ai, Gr(k2, -1), Gi(k2, -1), tr, ti); Pv[0] += tr - c0r; Pv[1] += ti - c0i;
+            cmul3(br, bi, Gr(k2, 1), Gi(k2, 1), tr, ti); Pv[2] += tr - c1r; Pv[3] += ti - c1i;
+            cmul3(ar, ai, Gr(k2, 1), -Gi(k2, 1), tr, ti);
+            Pv[4] += tr - c2r; Pv[5] += ti - c2i;
+            Pv[6] += tr - c2r; Pv[7] += ti - c2i;
+        } else {
+            const double ppr = Gr(k2, m + 1), ppi = Gi(k2, m + 1);
+            const double pmr = Gr(k2, m - 1), pmi = Gi(k2, m - 1);
+            const double mpr = Gr(k2, -m + 1), mpi = Gi(k2, -m + 1);
+            const double mmr = Gr(k2, -m - 1), mmi = Gi(k2, -m - 1);
+            double ar, ai, br, bi, tr, ti;
+            cmul3(g1r, g1i, mmr, mmi, ar, ai);
+            cmul3(g1cr, g1ci, ppr, ppi, br, bi);
+            cmul3(ar, ai, pmr, pmi, tr, ti); Pv[0] += tr - c0r; Pv[1] += ti - c0i;
+            cmul3(br, bi, mpr, mpi, tr, ti); Pv[2] += tr - c1r; Pv[3] += ti - c1i;
+            cmul3(br, -bi, pmr, pmi, tr, ti); Pv[4] += tr - c2r; Pv[5] += ti - c2i;
+            cmul3(ar, ai, mpr, -mpi, tr, ti); Pv[6] += tr - c2r; Pv[7] += ti - c2i;
+        }
+    } else if (m == 0) {
+        double ar, ai, br, bi, tr, ti;
+        cmul3(g1r, g1i, Gr(k2, -1), Gi(k2, -1), ar, ai);
+        cmul3(g1cr, g1ci, Gr(k2, 1), Gi(k2, 1), br, bi);
+        cmul3(ar, ai, Gr(k3, -1), Gi(k3, -1), tr, ti); Pv[0] += tr; Pv[1] += ti;
+        cmul3(br, bi, Gr(k3, 1), Gi(k3, 1), tr, ti); Pv[2] += tr; Pv[3] += ti;
+        cmul3(ar, ai, Gr(k3, 1), -Gi(k3, 1), tr, ti); Pv[4] += tr; Pv[5] += ti;
+        cmul3(br, -bi, Gr(k3, -1), Gi(k3, -1), tr, ti); Pv[6] += tr; Pv[7] += ti;
+    } else {
+        const double G2ppr = Gr(k2, m + 1), G2ppi = Gi(k2, m + 1);
+        const double G2pmr = Gr(k2, m - 1), G2pmi = Gi(k2, m - 1);
+        const double G2mpr = Gr(k2, -m + 1), G2mpi = Gi(k2, -m + 1);
+        const double G2mmr = Gr(k2, -m - 1), G2mmi = Gi(k2, -m - 1);
+        const double G3ppr = Gr(k3, m + 1), G3ppi = Gi(k3, m + 1);
+        const double G3pmr = Gr(k3, m - 1), G3pmi = Gi(k3, m - 1);
+        const double G3mpr = Gr(k3, -m + 1), G3mpi = Gi(k3, -m + 1);
+        const double G3mmr = Gr(k3, -m - 1), G3mmi = Gi(k3, -m - 1);
+        double pmR, pmI, cpR, cpI, mcR, mcI, mmR, mmI, tr, ti;
+        cmul3(g1r, g1i, G2pmr, G2pmi, pmR, pmI);
+        cmul3(g1cr, g1ci, G2ppr, G2ppi, cpR, cpI);
+        cmul3(g1r, g1i, G2mpr, -G2mpi, mcR, mcI);
+        cmul3(g1r, g1i, G2mmr, G2mmi, mmR, mmI);
+        cmul3(pmR, pmI, G3mmr, G3mmi, tr, ti); Pv[0] += tr; Pv[1] += ti;
+        cmul3(cpR, cpI, G3mpr, G3mpi, tr, ti); Pv[2] += tr; Pv[3] += ti;
+        cmul3(pmR, pmI, G3ppr, -G3ppi, tr, ti); Pv[4] += tr; Pv[5] += ti;
+        cmul3(mcR, mcI, G3mmr, G3mmi, tr, ti); Pv[6] += tr; Pv[7] += ti;
+        cmul3(mmR, mmI, G3pmr, G3pmi, tr, ti); Qv[0] += tr; Qv[1] += ti;
+        cmul3(mcR, -mcI, G3ppr, G3ppi, tr, ti); Qv[2] += tr; Qv[3] += ti;
+        cmul3(cpR, -cpI, G3pmr, G3pmi, tr, ti); Qv[4] += tr; Qv[5] += ti;
+        cmul3(mmR, mmI, G3mpr, -G3mpi, tr, ti); Qv[6] += tr; Qv[7] += ti;
+    }
+}
+
+constexpr int ZT_THREADS = 256;
+constexpr int ZETA_JOBS = 2048;     // jobs per chunk
+
+// One workgroup per (job chunk, row k2, cell range): every thread owns fixed cells
+// (k3, m) of an LDS tile and adds each job's contribution to them; the tile is then
+// written to the chunk's partial sums.
+template <int D>
+__global__ __launch_bounds__(ZT_THREADS) void k_mp_zeta(
+    const MPJob* __restrict__ jobs, int njobs, const DNode* __restrict__ nodes,
+    double* const* __restrict__ slots, C3Params P, int cells_per_part,
+    double* __restrict__ partial) {
+    HIP_DYNAMIC_SHARED(double, tile)
+    constexpr int NV = zeta_nv(D);
+    const int q = blockIdx.x, k2 = blockIdx.y;
+    const int nb = P.nbins, M = P.maxn;
+    const int ncell = nb * (M + 1);
+    const int cbeg = blockIdx.z * cells_per_part;
+    const int cend = ncell < cbeg + cells_per_part ? ncell : cbeg + cells_per_part;
+    const int nmine = cend - cbeg;
+    for (int i = threadIdx.x; i < nmine * NV; i += blockDim.x) tile[i] = 0.;
+    __syncthreads();
+    ScratchLayout Lo;
+    Lo.init(nb, M, D);
+    const int j0 = q * ZETA_JOBS;
+    const int j1 = njobs < j0 + ZETA_JOBS ? njobs : j0 + ZETA_JOBS;
+    for (int j = j0; j < j1; ++j) {
+        const MPJob jb = jobs[j];
+        if (jb.c1 < 0 || k2 < jb.ks || k2 >= jb.mk) continue;
+        const double* sc = slot_at(slots, jb.slot, Lo.total);
+        const DNode c1 = nodes[jb.c1];
+        for (int c = threadIdx.x; c < nmine; c += blockDim.x) {
+            const int cell = cbeg + c;
+            const int k3 = cell / (M + 1), m = cell - k3 * (M + 1);
+            if (k3 < k2) continue;
+            zeta_cell<D>(Lo, sc, c1, k2, k3, m, P, &tile[c * NV]);
+        }
+    }
+    __syncthreads();
+    double* dst = partial + (((size_t)q * nb + k2) * ncell + cbeg) * NV;
+    for (int i = threadIdx.x; i < nmine * NV; i += blockDim.x) dst[i] = tile[i];
+}
+
+// total += sum over the chunks (in chunk order) of the partial sums, compensated
+__global__ void k_mp_zeta_reduce(const double* __restrict__ partial, int nchunks, long long n,
+                                 double* __restrict__ hi, double* __restrict__ lo) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double s = hi[e], c = lo[e];
+    for (int q = 0; q < nchunks; ++q) {
+        const double v = partial[(size_t)q * n + e];
         const double tt = s + v;
         c += (fabs(s) >= fabs(v)) ? (s - tt) + v : (v - tt) + s;
         s = tt;
     }
-    out[t] = s + c;
+    hi[e] = s;
+    lo[e] = c;
+}
+
+// The compressed sums -> TreeCorr's arrays zeta0..7, meand2, meanlogd2, meand3,
+// meanlogd3, weight, weight_im, ntri at index (a*nb + b)*(2M+1) + M + n.
+template <int D>
+__global__ void k_mp_expand(const double* __restrict__ hi, const double* __restrict__ lo,
+                            C3Params P, double* __restrict__ out) {
+    constexpr int NV = zeta_nv(D);
+    const int nb = P.nbins, M = P.maxn, nn = 2 * M + 1;
+    const long long ntot = (long long)nb * nb * nn;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntot) return;
+    const int a = (int)(idx / ((long long)nb * nn));
+    const int b = (int)((idx / nn) % nb);
+    const int n = (int)(idx % nn) - M;
+    const int m = n >= 0 ? n : -n;
+    const int k2 = a <= b ? a : b, k3 = a <= b ? b : a;
+    const long long cb = (((long long)k2 * nb + k3) * (M + 1) + m) * NV;
+    auto V = [&](int i) { return hi[cb + i] + lo[cb + i]; };
+    const long long cb0 = (((long long)k2 * nb + k3) * (M + 1)) * NV;
+    auto V0 = [&](int i) { return hi[cb0 + i] + lo[cb0 + i]; };
+    // weight, weight_im (src/Corr3.cpp:1917-1938)
+    double wre = V(0), wim = V(1);
+    if (a == b) wim = 0.;
+    else if (a < b) wim = n >= 0 ? wim : -wim;
+    else wim = n >= 0 ? -wim : wim;
+    out[O_WEIGHT * ntot + idx] = wre;
+    out[O_WEIGHT_IM * ntot + idx] = wim;
+    // ntri, meand2, meanlogd2, meand3, meanlogd3 at n = 0 (:1877-1907)
+    double t5[5] = {0., 0., 0., 0., 0.};
+    if (n == 0) {
+        for (int i = 0; i < 5; ++i) t5[i] = V0(NV - 5 + i);
+        if (a > b) {   // the (k3, k2) entries: d2 and d3 exchanged
+            double x = t5[1]; t5[1] = t5[3]; t5[3] = x;
+            x = t5[2]; t5[2] = t5[4]; t5[4] = x;
+        }
+    }
+    out[O_NTRI * ntot + idx] = t5[0];
+    out[O_MEAND2 * ntot + idx] = t5[1];
+    out[O_MEANLOGD2 * ntot + idx] = t5[2];
+    out[O_MEAND3 * ntot + idx] = t5[3];
+    out[O_MEANLOGD3 * ntot + idx] = t5[4];
+    if (D != GData) return;
+    // zeta0..3 as (re, im) pairs; perm exchanges zeta2 and zeta3
+    double z[8];
+    bool perm = false;
+    int base = 2;            // P
+    if (a == b) {
+        perm = n < 0;
+    } else if (a < b) {
+        if (n < 0) { base = 10; perm = true; }
+    } else {
+        if (n > 0) base = 10;
+        else perm = true;    // n = 0: P with zeta2, zeta3 exchanged; n < 0 as well
+    }
+    for (int i = 0; i < 8; ++i) z[i] = V(base + i);
+    if (perm) {
+        double x = z[4]; z[4] = z[6]; z[6] = x;
+        x = z[5]; z[5] = z[7]; z[7] = x;
+    }
+    for (int i = 0; i < 8; ++i) out[(O_Z0 + i) * ntot + idx] = z[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -808,47 +977,106 @@ __global__ void k_mp_reduce(const double* slots, int nslots, long long n, double
 template <class T>
 struct DBuf {
     T* p = nullptr;
+    size_t n = 0;       // elements
     size_t bytes = 0;
-    ~DBuf() { dev_free(p, bytes); }
-    int alloc(size_t n) {
-        dev_free(p, bytes);
-        bytes = sizeof(T) * std::max<size_t>(n, 1);
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    // A block goes back to the caching allocator only once no queued kernel can still
+    // use it (the allocator hands it out again at once).
+    ~DBuf() { release(); }
+    void release() {
+        if (p) {
+            (void)hipStreamSynchronize(stream());
+            dev_free(p, bytes);
+        }
+        p = nullptr; n = 0; bytes = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        bytes = sizeof(T) * std::max<size_t>(count, 1);
         p = static_cast<T*>(dev_alloc(bytes));
+        n = p ? std::max<size_t>(count, 1) : 0;
         if (!p) { bytes = 0; return fail("out of device memory in the multipole engine"); }
         return 0;
     }
+    // at least `count` elements; the first `keep` are preserved
+    int reserve(size_t count, size_t keep, hipStream_t st) {
+        if (count <= n) return 0;
+        const size_t want = std::max(count, n + n / 2);
+        T* q = static_cast<T*>(dev_alloc(sizeof(T) * want));
+        if (!q) return fail("out of device memory in the multipole engine");
+        if (p && keep) TC_HIP(hipMemcpyAsync(q, p, sizeof(T) * std::min(keep, n), hipMemcpyDeviceToDevice, st));
+        if (p) {
+            TC_HIP(hipStreamSynchronize(st));   // the copy has read the old block
+            dev_free(p, bytes);
+        }
+        p = q; n = want; bytes = sizeof(T) * want;
+        (void)keep;
+        return 0;
+    }
+    void swap(DBuf& o) { std::swap(p, o.p); std::swap(n, o.n); std::swap(bytes, o.bytes); }
 };
 
-using SplitFn = void (*)(const DNode*, int, const int*, int, C3Params, int*, unsigned long long*,
-                         unsigned long long, Unit*, unsigned int*, unsigned int, int*);
-using FinishFn = void (*)(const DNode*, int, const NodeSum*, const int*, unsigned long long,
-                          const Unit*, unsigned int, unsigned int*, unsigned long long*,
-                          unsigned long long, C3Params, int*, long long, double*, int*,
-                          double*, int, unsigned long long*, int*);
+using SplitFn = void (*)(const DNode*, int, const SplitTask*, int*, unsigned long long*,
+                         unsigned long long, C3Params, SplitTask*, SplitTask*, int*);
+using StageFn = void (*)(const DNode*, int, const NodeSum*, const MPTask*, const int*,
+                         unsigned long long, int*, unsigned long long*, unsigned long long,
+                         double* const*, int, C3Params, MPTask*, MPJob*, int*,
+                         unsigned long long*, int*);
+using ZetaFn = void (*)(const MPJob*, int, const DNode*, double* const*, C3Params, int, double*);
+using ExpandFn = void (*)(const double*, const double*, C3Params, double*);
 
 static SplitFn pick_split(int C) {
-    if (C == Flat) return &k_mp_split1<Flat>;
-    if (C == ThreeD) return &k_mp_split1<ThreeD>;
-    return &k_mp_split1<Sphere>;
+    if (C == Flat) return &k_mp_split_level<Flat>;
+    if (C == ThreeD) return &k_mp_split_level<ThreeD>;
+    return &k_mp_split_level<Sphere>;
 }
-static FinishFn pick_finish(int D, int C) {
-#define TC_FF(DD, CC) return &k_mp_finish<DD, CC>
-    if (D == NData) { if (C == Flat) TC_FF(NData, Flat); if (C == ThreeD) TC_FF(NData, ThreeD); TC_FF(NData, Sphere); }
-    if (C == Flat) TC_FF(GData, Flat); if (C == ThreeD) TC_FF(GData, ThreeD); TC_FF(GData, Sphere);
-#undef TC_FF
+static StageFn pick_stage(int D, int C) {
+#define TC_SF(DD, CC) return &k_mp_stage<DD, CC>
+    if (D == NData) { if (C == Flat) TC_SF(NData, Flat); if (C == ThreeD) TC_SF(NData, ThreeD); TC_SF(NData, Sphere); }
+    if (C == Flat) TC_SF(GData, Flat); if (C == ThreeD) TC_SF(GData, ThreeD); TC_SF(GData, Sphere);
+#undef TC_SF
 }
 
 static int set_err_bits(const char* phase, int bits) {
     std::string m = std::string(phase) + " stopped:";
-    if (bits & E_POOL) m += " work pool overflow;";
-    if (bits & E_SPLITSTACK) m += " split stack overflow;";
-    if (bits & E_UNITS) m += " unit list overflow;";
-    if (bits & E_ARENA) m += " list arena overflow;";
-    if (bits & E_STACK) m += " depth-first stack overflow;";
+    if (bits & E_POOL) m += " list pool overflow;";
     if (bits & E_INDEX) m += " inconsistent tree or list data;";
-    if (bits & E_BUDGET) m += " stage budget exhausted;";
+    if (bits & E_SLOT) m += " inconsistent stage record;";
     return fail(m);
 }
+
+// Scan compaction of n fixed-position records into out[out_base ...]; the count and the
+// sum of their list lengths land in totals[0], totals[1] (device, read later).
+template <class R>
+struct Compactor {
+    DBuf<int> bcount, boff;
+    DBuf<unsigned long long> blen;
+    int run(const R* in, long long n, R* out, long long out_base, unsigned long long* totals,
+            hipStream_t st) {
+        const int nblk = (int)std::max<long long>(1, (n + CP_BLOCK - 1) / CP_BLOCK);
+        if (bcount.reserve(nblk, 0, st) || boff.reserve(nblk, 0, st) || blen.reserve(nblk, 0, st))
+            return 2;
+        if (n > 0)
+            hipLaunchKernelGGL(k_cp_count<R>, dim3(nblk), dim3(CP_TPB), 0, st, in, n, bcount.p,
+                               blen.p);
+        else {
+            TC_HIP(hipMemsetAsync(bcount.p, 0, sizeof(int), st));
+            TC_HIP(hipMemsetAsync(blen.p, 0, sizeof(unsigned long long), st));
+        }
+        TC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_cp_scan, dim3(1), dim3(1024), 0, st, bcount.p, blen.p, nblk, boff.p,
+                           totals);
+        TC_HIP(hipGetLastError());
+        if (n > 0) {
+            hipLaunchKernelGGL(k_cp_scatter<R>, dim3(nblk), dim3(CP_TPB), 0, st, in, n, boff.p,
+                               out, out_base);
+            TC_HIP(hipGetLastError());
+        }
+        return 0;
+    }
+};
 
 // Runs the multipole auto-correlation of one device tree; writes NOUT3*ntot reduced
 // sums (zeta0..7, meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri) into
@@ -857,14 +1085,27 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
               const C3Params& P, double* out, unsigned long long* cnt) {
     hipStream_t st = stream();
     const int nb = P.nbins, M = P.maxn;
-    const int ntot = nb * nb * (2 * M + 1);
+    const long long ntot = (long long)nb * nb * (2 * M + 1);
     ScratchLayout Lo;
     Lo.init(nb, M, D);
     const size_t shbytes = sizeof(double) * Lo.total;
     if (shbytes > 150 * 1024) return fail("nbins and max_n too large for the LDS scratch");
-    FinishFn ffn = pick_finish(D, C);
-    TC_HIP(hipFuncSetAttribute((const void*)ffn, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (ntop <= 0 || nnodes <= 0) return fail("empty field");
+    StageFn sfn = pick_stage(D, C);
+    TC_HIP(hipFuncSetAttribute((const void*)sfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)shbytes));
+    const int NV = zeta_nv(D);
+    const long long ncell = (long long)nb * (M + 1);
+    const long long zmax_cells = (96 * 1024) / (8 * NV);      // LDS tile of at most 96 KB
+    const int ncpart = (int)((ncell + zmax_cells - 1) / zmax_cells);
+    const int cells_per_part = (int)((ncell + ncpart - 1) / ncpart);
+    const size_t zt_bytes = sizeof(double) * (size_t)cells_per_part * NV;
+    ZetaFn zfn = D == GData ? &k_mp_zeta<GData> : &k_mp_zeta<NData>;
+    TC_HIP(hipFuncSetAttribute((const void*)zfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)zt_bytes));
+    KernelStat& ks_stage = kernel_stat(KS_MULTIPOLE);
+    KernelStat& ks_zeta = kernel_stat(KS_ZETA);
+    const bool timing = timing_enabled();
 
     // node sums, bottom up: each launch finishes the nodes whose children are done
     DBuf<NodeSum> nsum;
@@ -872,117 +1113,198 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
     if (nsum.alloc(nnodes) || doneA.alloc(nnodes) || doneB.alloc(nnodes) || changed.alloc(1))
         return 2;
     TC_HIP(hipMemsetAsync(doneA.p, 0, sizeof(int) * nnodes, st));
-    int hchanged = 1;
-    int* din = doneA.p;
-    int* dout = doneB.p;
-    for (int it = 0; it < 4096 && hchanged; ++it) {
-        TC_HIP(hipMemsetAsync(changed.p, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_node_sums, dim3((nnodes + 255) / 256), dim3(256), 0, st, nodes,
-                           nnodes, nsum.p, din, dout, changed.p);
-        TC_HIP(hipGetLastError());
-        TC_HIP(hipMemcpyAsync(&hchanged, changed.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        TC_HIP(hipStreamSynchronize(st));
-        std::swap(din, dout);
-    }
-
-    // phase 1: units.  The c2 lists of the split walk go to one pool; its size is
-    // not known beforehand, so an overflowing walk is repeated with a larger pool.
-    unsigned long long pool_cap = std::max<unsigned long long>(1ull << 22, 64ull * nnodes);
-    pool_cap = std::min<unsigned long long>(pool_cap, (1ull << 31) - 1);
-    unsigned int unit_cap = (unsigned int)std::min<long long>(std::max(nnodes, 1024), 1 << 30);
-    DBuf<int> pool, err;
-    DBuf<unsigned long long> pool_top, counters;
-    DBuf<Unit> units;
-    DBuf<unsigned int> nunits, next_unit;
-    if (err.alloc(1) || pool_top.alloc(1) || counters.alloc(NC3) || units.alloc(unit_cap) ||
-        nunits.alloc(1) || next_unit.alloc(1))
-        return 2;
-    unsigned int hn = 0;
-    int herr = 0;
-    for (;;) {
-        if (pool.alloc(pool_cap)) return 2;
-        TC_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
-        TC_HIP(hipMemsetAsync(pool_top.p, 0, sizeof(unsigned long long), st));
-        TC_HIP(hipMemsetAsync(nunits.p, 0, sizeof(unsigned int), st));
-        hipLaunchKernelGGL(pick_split(C), dim3(ntop), dim3(MP_THREADS), 0, st, nodes, nnodes,
-                           dtop, ntop, P, pool.p, pool_top.p, pool_cap, units.p, nunits.p,
-                           unit_cap, err.p);
-        TC_HIP(hipGetLastError());
-        unsigned long long used = 0;
-        TC_HIP(hipMemcpyAsync(&hn, nunits.p, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        TC_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        TC_HIP(hipMemcpyAsync(&used, pool_top.p, sizeof(used), hipMemcpyDeviceToHost, st));
-        TC_HIP(hipStreamSynchronize(st));
-        if (herr == E_POOL && pool_cap < (1ull << 31) - 1) {
-            pool_cap = std::min<unsigned long long>(std::max(2 * pool_cap, 2 * used),
-                                                    (1ull << 31) - 1);
-            continue;
+    {
+        int hchanged = 1;
+        int* din = doneA.p;
+        int* dout = doneB.p;
+        for (int it = 0; it < 4096 && hchanged; ++it) {
+            TC_HIP(hipMemsetAsync(changed.p, 0, sizeof(int), st));
+            hipLaunchKernelGGL(k_node_sums, dim3((nnodes + 255) / 256), dim3(256), 0, st, nodes,
+                               nnodes, nsum.p, din, dout, changed.p);
+            TC_HIP(hipGetLastError());
+            TC_HIP(hipMemcpyAsync(&hchanged, changed.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            TC_HIP(hipStreamSynchronize(st));
+            std::swap(din, dout);
         }
-        break;
+        if (hchanged) return fail("node sums did not converge: the tree has a cycle");
     }
-    if (herr) return set_err_bits("multipole split phase", herr);
 
-    // phase 2, repeated with larger per-workgroup list arenas if one overflows
-    const int nwg = (int)std::max<unsigned int>(1u, std::min<unsigned int>(hn, 512u));
-    DBuf<int> arena, stk;
-    DBuf<double> sstack, slots, red;
-    DBuf<unsigned long long> stage_count;
-    // No valid walk takes more stages than this: each stage accepts, splits or drops
-    // at least one (c1, c2) pair, and a pair's cells are refined at most to the leaves.
-    const unsigned long long stage_budget = 256ull * (unsigned long long)nnodes + (1ull << 20);
-    if (stage_count.alloc(1) || stk.alloc((size_t)nwg * MP_STACK * 8) ||
-        sstack.alloc((size_t)nwg * MP_STACK * Lo.total) ||
-        slots.alloc((size_t)nwg * NOUT3 * ntot) || red.alloc((size_t)NOUT3 * ntot))
+    DBuf<int> err;
+    DBuf<unsigned long long> counters, tot, ptop;     // tot: 3 compaction totals x 2
+    if (err.alloc(1) || counters.alloc(NC3) || tot.alloc(6) || ptop.alloc(1)) return 2;
+    TC_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    TC_HIP(hipMemsetAsync(counters.p, 0, sizeof(unsigned long long) * NC3, st));
+
+    // ---- phase 1: multipoleSplit1, one launch per level ----
+    DBuf<int> spool;
+    DBuf<SplitTask> stA, stB, schild, sunit_out, units;
+    if (spool.alloc((size_t)ntop + 2ull * ntop * ntop) || stA.alloc(ntop) || units.alloc(1024))
         return 2;
-    KernelStat& ks = kernel_stat(KS_MULTIPOLE);
-    const bool timing = timing_enabled();
-    const long long nred = (long long)NOUT3 * ntot;
-    long long arena_cap = MP_ARENA0;
-    for (;;) {
-        if (arena.alloc((size_t)nwg * arena_cap)) return 2;
-        TC_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
-        TC_HIP(hipMemsetAsync(counters.p, 0, sizeof(unsigned long long) * NC3, st));
-        TC_HIP(hipMemsetAsync(next_unit.p, 0, sizeof(unsigned int), st));
-        TC_HIP(hipMemsetAsync(stage_count.p, 0, sizeof(unsigned long long), st));
-        TC_HIP(hipMemsetAsync(slots.p, 0, sizeof(double) * (size_t)nwg * NOUT3 * ntot, st));
-        hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipLaunchKernelGGL(k_split_init, dim3((ntop + 255) / 256), dim3(256), 0, st, dtop, ntop, stA.p,
+                       spool.p);
+    TC_HIP(hipGetLastError());
+    unsigned long long spool_top = (unsigned long long)ntop;
+    unsigned long long sumlen = (unsigned long long)ntop * ntop;
+    long long ntask = ntop, nunits = 0;
+    unsigned long long usum = 0;      // the units' list lengths (bounds the first next lists)
+    Compactor<SplitTask> cps;
+    Compactor<MPTask> cpt;
+    Compactor<int> cpi;
+    SplitFn splfn = pick_split(C);
+    for (int level = 0; ntask > 0; ++level) {
+        if (level > 200) return fail("multipole split phase: too many levels");
+        const unsigned long long need = spool_top + 2 * sumlen;
+        if (need > 0x7fffffffull) return fail("multipole split phase: list pool beyond 2^31 entries");
+        if (spool.reserve(need, spool_top, st) || schild.reserve(2 * ntask, 0, st) ||
+            sunit_out.reserve(ntask, 0, st) || stB.reserve(2 * ntask, 0, st) ||
+            units.reserve(nunits + ntask, nunits, st))
+            return 2;
+        TC_HIP(hipMemcpyAsync(ptop.p, &spool_top, sizeof(spool_top), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(splfn, dim3((unsigned)ntask), dim3(MP_THREADS), 0, st, nodes, nnodes,
+                           stA.p, spool.p, ptop.p, (unsigned long long)spool.n, P, schild.p,
+                           sunit_out.p, err.p);
+        TC_HIP(hipGetLastError());
+        if (cps.run(schild.p, 2 * ntask, stB.p, 0, tot.p, st) ||
+            cps.run(sunit_out.p, ntask, units.p, nunits, tot.p + 2, st))
+            return 2;
+        unsigned long long h[4];
+        int herr = 0;
+        TC_HIP(hipMemcpyAsync(h, tot.p, sizeof(h), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipMemcpyAsync(&spool_top, ptop.p, sizeof(spool_top), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipStreamSynchronize(st));
+        if (herr) return set_err_bits("multipole split phase", herr);
+        ntask = (long long)h[0];
+        sumlen = h[1];
+        nunits += (long long)h[2];
+        usum += h[3];
+        stA.swap(stB);
+    }
+
+    // ---- phase 2+3: multipoleFinish stages and zeta, one launch of each per level ----
+    DBuf<MPTask> tq, tq_next, tchild;
+    DBuf<MPJob> jobs;
+    DBuf<int> ret, free_ids;
+    DBuf<int> poolA, poolB;
+    DBuf<double> partial, zhi, zlo, red;
+    DBuf<double*> chunk_tab;
+    std::vector<double*> chunks;
+    struct ChunkGuard {
+        std::vector<double*>* c; size_t bytes;
+        ~ChunkGuard() {
+            (void)hipStreamSynchronize(stream());
+            for (double* p : *c) dev_free(p, bytes);
+        }
+    } cguard{&chunks, sizeof(double) * (size_t)Lo.total << SLOT_SHIFT};
+    const long long ncomp = (long long)nb * ncell * NV;
+    if (tq.alloc(std::max<long long>(nunits, 1)) || zhi.alloc(ncomp) || zlo.alloc(ncomp) ||
+        red.alloc((size_t)NOUT3 * ntot) || chunk_tab.alloc(4096) || free_ids.alloc(1 << SLOT_SHIFT))
+        return 2;
+    TC_HIP(hipMemsetAsync(zhi.p, 0, sizeof(double) * ncomp, st));
+    TC_HIP(hipMemsetAsync(zlo.p, 0, sizeof(double) * ncomp, st));
+    if (nunits > 0)
+        hipLaunchKernelGGL(k_units_to_tasks, dim3((unsigned)((nunits + 255) / 256)), dim3(256), 0,
+                           st, units.p, (int)nunits, nb, tq.p);
+    TC_HIP(hipGetLastError());
+    // the units' lists stay in the split pool, which is the first level's input pool
+    int* pin = spool.p;
+    unsigned long long pin_cap = spool_top;
+    bool pin_is_split = true;
+    int free_top = 0;
+    long long nslots = 0;
+    std::vector<int> new_ids;
+    ntask = nunits;
+    sumlen = usum;
+    for (int level = 0; ntask > 0; ++level) {
+        if (level > 100000) return fail("multipole finish phase: too many levels");
+        const long long pops = ntask * (level == 0 ? 2 : 1);
+        // enough free slots for the pops of this level
+        while (free_top < pops) {
+            if ((long long)chunks.size() >= 4096 || nslots + (1 << SLOT_SHIFT) > 0x7fffffffll)
+                return fail("multipole finish phase: too many scratch slots");
+            double* c = static_cast<double*>(dev_alloc(cguard.bytes));
+            if (!c) return fail("out of device memory for the multipole scratch slots");
+            chunks.push_back(c);
+            TC_HIP(hipMemcpyAsync(chunk_tab.p + chunks.size() - 1, &chunks.back(), sizeof(double*),
+                                  hipMemcpyHostToDevice, st));
+            if (free_ids.reserve(nslots + (1 << SLOT_SHIFT), free_top, st)) return 2;
+            new_ids.resize(1 << SLOT_SHIFT);
+            for (int i = 0; i < (1 << SLOT_SHIFT); ++i) new_ids[i] = (int)(nslots + (1 << SLOT_SHIFT) - 1 - i);
+            TC_HIP(hipMemcpyAsync(free_ids.p + free_top, new_ids.data(), sizeof(int) * new_ids.size(),
+                                  hipMemcpyHostToDevice, st));
+            TC_HIP(hipStreamSynchronize(st));     // new_ids is reused
+            free_top += 1 << SLOT_SHIFT;
+            nslots += 1 << SLOT_SHIFT;
+        }
+        DBuf<int>& pout = (!pin_is_split && pin == poolA.p) ? poolB : poolA;
+        const unsigned long long need = std::max<unsigned long long>(2 * sumlen, 1);
+        if (need > 0x7fffffffull) return fail("multipole finish phase: list pool beyond 2^31 entries");
+        if (pout.reserve(need, 0, st) || tchild.reserve(2 * ntask, 0, st) ||
+            tq_next.reserve(2 * ntask, 0, st) || jobs.reserve(ntask, 0, st) ||
+            ret.reserve(2 * ntask, 0, st))
+            return 2;
+        hipLaunchKernelGGL(k_mp_prep, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st,
+                           tq.p, (int)ntask, free_ids.p, free_top);
+        TC_HIP(hipGetLastError());
+        const int top_after = free_top - (int)pops;
+        const unsigned long long zero = 0;
+        TC_HIP(hipMemcpyAsync(ptop.p, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
         if (timing) {
-            TC_HIP(hipEventCreate(&e0));
-            TC_HIP(hipEventCreate(&e1));
+            TC_HIP(hipEventCreate(&e0)); TC_HIP(hipEventCreate(&e1)); TC_HIP(hipEventCreate(&e2));
             TC_HIP(hipEventRecord(e0, st));
         }
-        if (hn > 0) {
-            hipLaunchKernelGGL(ffn, dim3(nwg), dim3(MP_THREADS), shbytes, st, nodes, nnodes,
-                               nsum.p, pool.p, pool_cap, units.p, hn, next_unit.p, stage_count.p,
-                               stage_budget, P, arena.p, arena_cap, sstack.p, stk.p, slots.p, ntot,
-                               counters.p, err.p);
-            TC_HIP(hipGetLastError());
-        }
+        hipLaunchKernelGGL(sfn, dim3((unsigned)ntask), dim3(MP_THREADS), shbytes, st, nodes, nnodes,
+                           nsum.p, tq.p, pin, pin_cap, pout.p, ptop.p, (unsigned long long)pout.n,
+                           chunk_tab.p, (int)nslots, P, tchild.p, jobs.p, ret.p, counters.p, err.p);
+        TC_HIP(hipGetLastError());
         if (timing) TC_HIP(hipEventRecord(e1, st));
+        // this level's zeta jobs, in task order
+        const int nchunks = (int)((ntask + ZETA_JOBS - 1) / ZETA_JOBS);
+        if (partial.reserve((size_t)nchunks * ncomp, 0, st)) return 2;
+        hipLaunchKernelGGL(zfn, dim3(nchunks, nb, ncpart), dim3(ZT_THREADS), zt_bytes, st, jobs.p,
+                           (int)ntask, nodes, chunk_tab.p, P, cells_per_part, partial.p);
+        TC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mp_zeta_reduce, dim3((unsigned)((ncomp + 255) / 256)), dim3(256), 0, st,
+                           partial.p, nchunks, ncomp, zhi.p, zlo.p);
+        TC_HIP(hipGetLastError());
+        if (timing) TC_HIP(hipEventRecord(e2, st));
+        // next tasks, and the slots that became free, in task order
+        if (cpt.run(tchild.p, 2 * ntask, tq_next.p, 0, tot.p, st) ||
+            cpi.run(ret.p, 2 * ntask, free_ids.p, top_after, tot.p + 2, st))
+            return 2;
+        unsigned long long h[3] = {0, 0, 0};
+        int herr = 0;
+        unsigned long long used = 0;
+        TC_HIP(hipMemcpyAsync(h, tot.p, sizeof(h), hipMemcpyDeviceToHost, st));
         TC_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipMemcpyAsync(&used, ptop.p, sizeof(used), hipMemcpyDeviceToHost, st));
         TC_HIP(hipStreamSynchronize(st));
         if (timing) {
-            float ms = 0.f;
-            TC_HIP(hipEventElapsedTime(&ms, e0, e1));
-            ks.ms += ms;
-            ks.launches += 1;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            float ms1 = 0.f, ms2 = 0.f;
+            TC_HIP(hipEventElapsedTime(&ms1, e0, e1));
+            TC_HIP(hipEventElapsedTime(&ms2, e1, e2));
+            ks_stage.ms += ms1; ks_stage.launches += 1;
+            ks_zeta.ms += ms2; ks_zeta.launches += 1;
+            (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
         }
-        if (herr == E_ARENA && arena_cap < (1ll << 27)) {
-            arena_cap *= 4;
-            continue;
-        }
-        break;
+        if (herr) return set_err_bits("multipole finish phase", herr);
+        free_top = top_after + (int)h[2];
+        ntask = (long long)h[0];
+        sumlen = h[1];
+        tq.swap(tq_next);
+        pin = pout.p;
+        pin_cap = used;
+        pin_is_split = false;
+        if (level == 0) { spool.release(); units.release(); }   // the units' lists are consumed
     }
-    hipLaunchKernelGGL(k_mp_reduce, dim3((unsigned)((nred + 255) / 256)), dim3(256), 0, st,
-                       slots.p, nwg, nred, red.p);
+    ExpandFn efn = D == GData ? &k_mp_expand<GData> : &k_mp_expand<NData>;
+    hipLaunchKernelGGL(efn, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, zhi.p, zlo.p, P,
+                       red.p);
     TC_HIP(hipGetLastError());
-    TC_HIP(hipMemcpyAsync(out, red.p, sizeof(double) * nred, hipMemcpyDeviceToHost, st));
-    TC_HIP(hipMemcpyAsync(cnt, counters.p, sizeof(unsigned long long) * NC3,
-                          hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(out, red.p, sizeof(double) * NOUT3 * ntot, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(cnt, counters.p, sizeof(unsigned long long) * NC3, hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));
-    if (herr) return set_err_bits("multipole finish phase", herr);
     return 0;
 }
 
