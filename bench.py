@@ -93,6 +93,54 @@ def gg_binning():
                 min_top=4, max_top=10)
 
 
+def mp_flat_catalog(n=500_000, seed=2026_12_03 + 4):
+    """Multipole workload: 70% uniform + 30% from 500 Gaussian clumps (sigma = 3 arcmin)
+    over a 5 x 5 deg flat field (positions in arcmin), g1,g2 ~ N(0, 0.2^2) clipped to
+    |g| < 1, w ~ U[0.5, 1.5]."""
+    rng = np.random.default_rng(seed)
+    side = 300.
+    nu = int(round(0.7 * n))
+    x = np.empty(n)
+    y = np.empty(n)
+    x[:nu] = rng.uniform(0, side, nu)
+    y[:nu] = rng.uniform(0, side, nu)
+    cx = rng.uniform(0, side, 500)
+    cy = rng.uniform(0, side, 500)
+    filled = nu
+    while filled < n:
+        m = n - filled
+        k = rng.integers(0, 500, m)
+        xx = cx[k] + rng.normal(0, 3., m)
+        yy = cy[k] + rng.normal(0, 3., m)
+        ok = (xx >= 0) & (xx < side) & (yy >= 0) & (yy < side)
+        xx, yy = xx[ok], yy[ok]
+        x[filled:filled + len(xx)] = xx
+        y[filled:filled + len(yy)] = yy
+        filled += len(xx)
+    g1 = rng.normal(0, 0.2, n)
+    g2 = rng.normal(0, 0.2, n)
+    bad = g1 ** 2 + g2 ** 2 >= 1
+    while np.any(bad):
+        g1[bad] = rng.normal(0, 0.2, bad.sum())
+        g2[bad] = rng.normal(0, 0.2, bad.sum())
+        bad = g1 ** 2 + g2 ** 2 >= 1
+    w = rng.uniform(0.5, 1.5, n)
+    return x, y, g1, g2, w
+
+
+def mp_binning(max_n=30):
+    """LogMultipole, 20 log bins 5-100 arcmin (TreeCorr's defaults: b = min(bin_size, 0.1),
+    angle_slop = pi / (2 max_n + 1)); field sizes as corr3base.py:2934-2964."""
+    nbins, min_sep, max_sep = 20, 5., 100.
+    bin_size = math.log(max_sep / min_sep) / nbins
+    b = bin_size if bin_size <= 0.1 else 0.1
+    a = math.pi / (2 * max_n + 1)
+    b1 = min(a, b)
+    return dict(nbins=nbins, min_sep=min_sep, max_sep=max_sep, bin_size=bin_size, b=b, a=a,
+                max_n=max_n, min_size=2 * min_sep * b1 / (2 + 3 * b1), max_size=max_sep * b1,
+                min_top=4, max_top=10)
+
+
 # ---------------------------------------------------------------------------
 # Device memory (HIP runtime through ctypes)
 # ---------------------------------------------------------------------------

@@ -152,7 +152,34 @@ __global__ void k_node_sums(const DNode* nodes, int nn, NodeSum* sums, const int
     *changed = 1;
 }
 
-// Corr3::calculateGn + MultipoleScratch::calculateGn for one accepted (c1,c2)
+// Corr3::calculateGn + MultipoleScratch::calculateGn (src/Corr3.cpp:1681-1828) for one
+// accepted (c1,c2), added into the LDS scratch with FP64 atomics.
+//
+// The lanes of a wavefront usually hold pairs of the same bin k, so a plain loop over
+// m would make every lane of an atomic add hit the same address.  Each lane therefore
+// walks the multipoles in its own cyclic order, starting at m0 = lane mod (M+2): one
+// power sequence p = z^m (m = 0..M+1) gives W(k,m) += w p, G(k,m) += g p and
+// G(k,-m) += g conj(p) (|z| = 1, so conj(z)^m = conj(z^m)).  The reference multiplies
+// the powers up one at a time from m = 0; a lane here starts from z^m0 by repeated
+// squaring, so the terms differ from the reference's in the last bits only.
+__device__ __forceinline__ void cpow_int(double zr, double zi, int e, double& pr, double& pi) {
+    double br = zr, bi = zi;
+    pr = 1.;
+    pi = 0.;
+    for (int bit = 0; bit < 7; ++bit) {
+        if (e & 1) {
+            double tr, ti;
+            cmul3(pr, pi, br, bi, tr, ti);
+            pr = tr; pi = ti;
+        }
+        e >>= 1;
+        if (e == 0) break;
+        double tr, ti;
+        cmul3(br, bi, br, bi, tr, ti);
+        br = tr; bi = ti;
+    }
+}
+
 template <int D, int C>
 __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, const DNode& c1,
                                        const DNode& c2, const NodeSum& s2, double r,
@@ -181,15 +208,7 @@ __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, cons
         zr = dr / sn;
         zi = di / sn;
     }
-    atomicAdd(&sc[Lo.W(k, 0)], w);
-    double wr = w, wi = 0.;
-    for (int m = 1; m <= maxn; ++m) {
-        double tr, ti;
-        cmul3(wr, wi, zr, zi, tr, ti);
-        wr = tr; wi = ti;
-        atomicAdd(&sc[Lo.W(k, m)], wr);
-        atomicAdd(&sc[Lo.W(k, m) + 1], wi);
-    }
+    double gr = 0., gi = 0.;
     if (D == GData) {
         // ProjectWithSq<GData> (ProjectHelper.h:115-124, 357-368)
         double er, ei;
@@ -200,7 +219,6 @@ __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, cons
             direction3(x1, y1, z1, x2, y2, z2, dr, di);
             expm2ialpha(dr, di, er, ei);
         }
-        double gr, gi;
         cmul3(c2.a, c2.b, er, ei, gr, gi);
         double e2r, e2i;
         cmul3(er, ei, er, ei, e2r, e2i);
@@ -215,22 +233,33 @@ __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, cons
         atomicAdd(&sc[Lo.GG(1, k)], tr); atomicAdd(&sc[Lo.GG(1, k) + 1], ti);
         cmul3(s2.abswz, 0., zsr, -zsi, tr, ti);
         atomicAdd(&sc[Lo.GG(2, k)], tr); atomicAdd(&sc[Lo.GG(2, k) + 1], ti);
-        atomicAdd(&sc[Lo.G(k, 0)], gr);
-        atomicAdd(&sc[Lo.G(k, 0) + 1], gi);
-        double ar = gr, ai = gi;
-        for (int m = 1; m <= maxn + 1; ++m) {
-            cmul3(ar, ai, zr, zi, tr, ti);
-            ar = tr; ai = ti;
-            atomicAdd(&sc[Lo.G(k, m)], ar);
-            atomicAdd(&sc[Lo.G(k, m) + 1], ai);
+    }
+    // W(k, 0..M); G(k, -(M+1)..M+1) for GGG
+    const int nm = D == GData ? maxn + 2 : maxn + 1;
+    int m = (int)(threadIdx.x & 63u) % nm;
+    double pr, pi;
+    cpow_int(zr, zi, m, pr, pi);
+    for (int t = 0; t < nm; ++t) {
+        if (m <= maxn) {
+            atomicAdd(&sc[Lo.W(k, m)], w * pr);
+            atomicAdd(&sc[Lo.W(k, m) + 1], w * pi);
         }
-        ar = gr; ai = gi;
-        for (int m = 1; m <= maxn + 1; ++m) {
-            cmul3(ar, ai, zr, -zi, tr, ti);
-            ar = tr; ai = ti;
-            atomicAdd(&sc[Lo.G(k, -m)], ar);
-            atomicAdd(&sc[Lo.G(k, -m) + 1], ai);
+        if (D == GData) {
+            double tr, ti;
+            cmul3(gr, gi, pr, pi, tr, ti);
+            atomicAdd(&sc[Lo.G(k, m)], tr);
+            atomicAdd(&sc[Lo.G(k, m) + 1], ti);
+            if (m > 0) {
+                cmul3(gr, gi, pr, -pi, tr, ti);
+                atomicAdd(&sc[Lo.G(k, -m)], tr);
+                atomicAdd(&sc[Lo.G(k, -m) + 1], ti);
+            }
         }
+        // next multipole, cyclically
+        double tr, ti;
+        cmul3(pr, pi, zr, zi, tr, ti);
+        pr = tr; pi = ti;
+        if (++m == nm) { m = 0; pr = 1.; pi = 0.; }
     }
 }
 // ---------------------------------------------------------------------------
@@ -247,6 +276,8 @@ struct MPTask {
 };
 // calculateZeta(c1, mp, ks, mk) with mp in slot `slot` (c1 < 0: no job)
 struct MPJob { int c1, slot, ks, mk; };
+// rows [lo, hi) of a chunk of zeta jobs
+struct RowRange { int lo, hi; };
 
 // Error bits reported through `err` (all of them end the kernel cleanly).
 enum { E_POOL = 1, E_INDEX = 2, E_SLOT = 4 };
@@ -852,55 +883,137 @@ __device__ __forceinline__ void zeta_cell(const ScratchLayout& Lo, const double*
     }
 }
 
-constexpr int ZT_THREADS = 256;
-constexpr int ZETA_JOBS = 2048;     // jobs per chunk
+constexpr int ZT_THREADS = 512;
+constexpr int ZT_MR = 4;            // multipoles m per workgroup, at most
+constexpr int ZT_IPT = 2;           // cells per thread: mr * nbins (nbins + 1) / 2 <= ZT_IPT * ZT_THREADS
 
-// One workgroup per (job chunk, row k2, cell range): every thread owns fixed cells
-// (k3, m) of an LDS tile and adds each job's contribution to them; the tile is then
-// written to the chunk's partial sums.
+// One workgroup per (job chunk, range of mr multipoles m).  The chunk's jobs are
+// neighbours in task order, so their row ranges [ks, mk) are alike: the workgroup takes
+// the union [kmin, kmax) of its jobs' rows and every thread owns up to two cells
+// (k2, k3 >= k2, m) of those rows, their sums kept in registers.  For each job, the part
+// of its scratch these multipoles read (W(k, m), G(k, m-1..m+1), G(k, -m-1..-m+1) for
+// every bin, and the per-bin sums) is copied from the slot into LDS, at the slot's own
+// offsets; the cells whose row the job contracts (ks <= k2 < mk) add their terms.  The
+// sums are written to the chunk's partial array and the rows to chunk_rows.
 template <int D>
 __global__ __launch_bounds__(ZT_THREADS) void k_mp_zeta(
     const MPJob* __restrict__ jobs, int njobs, const DNode* __restrict__ nodes,
-    double* const* __restrict__ slots, C3Params P, int cells_per_part,
-    double* __restrict__ partial) {
-    HIP_DYNAMIC_SHARED(double, tile)
+    double* const* __restrict__ slots, C3Params P, int mr, int jpc, double* __restrict__ partial,
+    RowRange* __restrict__ chunk_rows) {
+    HIP_DYNAMIC_SHARED(double, sc)
+    __shared__ int s_kmin[ZT_THREADS / 64], s_kmax[ZT_THREADS / 64];
     constexpr int NV = zeta_nv(D);
-    const int q = blockIdx.x, k2 = blockIdx.y;
+    const int q = blockIdx.x;
+    const int m0 = blockIdx.y * mr;
     const int nb = P.nbins, M = P.maxn;
-    const int ncell = nb * (M + 1);
-    const int cbeg = blockIdx.z * cells_per_part;
-    const int cend = ncell < cbeg + cells_per_part ? ncell : cbeg + cells_per_part;
-    const int nmine = cend - cbeg;
-    for (int i = threadIdx.x; i < nmine * NV; i += blockDim.x) tile[i] = 0.;
-    __syncthreads();
     ScratchLayout Lo;
     Lo.init(nb, M, D);
-    const int j0 = q * ZETA_JOBS;
-    const int j1 = njobs < j0 + ZETA_JOBS ? njobs : j0 + ZETA_JOBS;
-    for (int j = j0; j < j1; ++j) {
+    const int j0 = q * jpc;
+    const int j1 = njobs < j0 + jpc ? njobs : j0 + jpc;
+    // the union of the chunk's row ranges
+    int kmin = nb, kmax = 0;
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
         const MPJob jb = jobs[j];
-        if (jb.c1 < 0 || k2 < jb.ks || k2 >= jb.mk) continue;
-        const double* sc = slot_at(slots, jb.slot, Lo.total);
-        const DNode c1 = nodes[jb.c1];
-        for (int c = threadIdx.x; c < nmine; c += blockDim.x) {
-            const int cell = cbeg + c;
-            const int k3 = cell / (M + 1), m = cell - k3 * (M + 1);
-            if (k3 < k2) continue;
-            zeta_cell<D>(Lo, sc, c1, k2, k3, m, P, &tile[c * NV]);
+        if (jb.c1 >= 0 && jb.ks < jb.mk) {
+            kmin = jb.ks < kmin ? jb.ks : kmin;
+            kmax = jb.mk > kmax ? jb.mk : kmax;
         }
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_xor(kmin, o, 64), b = __shfl_xor(kmax, o, 64);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    if ((threadIdx.x & 63) == 0) { s_kmin[threadIdx.x >> 6] = kmin; s_kmax[threadIdx.x >> 6] = kmax; }
     __syncthreads();
-    double* dst = partial + (((size_t)q * nb + k2) * ncell + cbeg) * NV;
-    for (int i = threadIdx.x; i < nmine * NV; i += blockDim.x) dst[i] = tile[i];
+    kmin = nb; kmax = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        kmin = s_kmin[w] < kmin ? s_kmin[w] : kmin;
+        kmax = s_kmax[w] > kmax ? s_kmax[w] : kmax;
+    }
+    if (threadIdx.x == 0 && blockIdx.y == 0) chunk_rows[q] = RowRange{kmin, kmax};
+    if (kmin >= kmax) return;            // no rows in this chunk (uniform)
+    // this thread's cells: c = threadIdx.x + u * ZT_THREADS, m fastest, then k3, then k2
+    int ck2[ZT_IPT], ck3[ZT_IPT], cm[ZT_IPT];
+#pragma unroll
+    for (int u = 0; u < ZT_IPT; ++u) {
+        const int c = (int)threadIdx.x + u * ZT_THREADS;
+        const int ml = c % mr;
+        int pr = c / mr;
+        int k2 = kmin;
+        while (k2 < kmax && pr >= nb - k2) { pr -= nb - k2; ++k2; }
+        ck2[u] = k2;
+        ck3[u] = k2 + pr;
+        cm[u] = (k2 < kmax && m0 + ml <= M) ? m0 + ml : -1;
+    }
+    double a0[NV], a1[NV];
+    static_assert(ZT_IPT == 2, "two cells per thread");
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { a0[v] = 0.; a1[v] = 0.; }
+    // the scratch entries these multipoles read: per bin W(k, m0..mhi), and for GGG
+    // G(k, gl..mhi+1) and G(k, -(mhi+1)..-gl); then all per-bin sums
+    const int mhi = (m0 + mr - 1 < M ? m0 + mr - 1 : M);
+    const int nw = 2 * (mhi - m0 + 1);
+    const int gl = m0 - 1 < 0 ? 0 : m0 - 1;
+    const int ng = D == GData ? 2 * (mhi + 1 - gl + 1) : 0;
+    const int per_k = nw + 2 * ng;
+    const int nsum = Lo.total - Lo.sums;
+    const int ncopy = nb * per_k + nsum;
+    for (int j = j0; j < j1; ++j) {
+        const MPJob jb = jobs[j];
+        if (jb.c1 < 0 || jb.ks >= jb.mk) continue;
+        const double* gsl = slot_at(slots, jb.slot, Lo.total);
+        __syncthreads();                 // the previous job's entries are no longer read
+        for (int e = threadIdx.x; e < ncopy; e += blockDim.x) {
+            int off;
+            if (e >= nb * per_k) {
+                off = Lo.sums + (e - nb * per_k);
+            } else {
+                const int k = e / per_k;
+                const int r = e - k * per_k;
+                if (r < nw) off = Lo.W(k, m0) + r;
+                else if (r < nw + ng) off = Lo.G(k, gl) + (r - nw);
+                else off = Lo.G(k, -(mhi + 1)) + (r - nw - ng);
+            }
+            sc[off] = gsl[off];
+        }
+        __syncthreads();
+        const DNode c1 = nodes[jb.c1];
+        if (cm[0] >= 0 && ck2[0] >= jb.ks && ck2[0] < jb.mk)
+            zeta_cell<D>(Lo, sc, c1, ck2[0], ck3[0], cm[0], P, a0);
+        if (cm[1] >= 0 && ck2[1] >= jb.ks && ck2[1] < jb.mk)
+            zeta_cell<D>(Lo, sc, c1, ck2[1], ck3[1], cm[1], P, a1);
+    }
+    const size_t ncomp_chunk = (size_t)nb * nb * (M + 1) * NV;
+    if (cm[0] >= 0) {
+        double* dst = partial + (size_t)q * ncomp_chunk +
+                      (((size_t)ck2[0] * nb + ck3[0]) * (M + 1) + cm[0]) * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) dst[v] = a0[v];
+    }
+    if (cm[1] >= 0) {
+        double* dst = partial + (size_t)q * ncomp_chunk +
+                      (((size_t)ck2[1] * nb + ck3[1]) * (M + 1) + cm[1]) * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) dst[v] = a1[v];
+    }
 }
 
-// total += sum over the chunks (in chunk order) of the partial sums, compensated
-__global__ void k_mp_zeta_reduce(const double* __restrict__ partial, int nchunks, long long n,
-                                 double* __restrict__ hi, double* __restrict__ lo) {
+// total += sum over the chunks (in chunk order) of the partial sums of the chunks whose
+// rows include this entry's row k2, compensated
+__global__ void k_mp_zeta_reduce(const double* __restrict__ partial,
+                                 const RowRange* __restrict__ chunk_rows, int nchunks, long long n,
+                                 long long row_stride, int nbins, double* __restrict__ hi,
+                                 double* __restrict__ lo) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
+    const int k2 = (int)(e / row_stride);
+    const int k3 = (int)((e % row_stride) / (row_stride / nbins));
+    if (k3 < k2) return;                 // not a cell of the compressed layout
     double s = hi[e], c = lo[e];
     for (int q = 0; q < nchunks; ++q) {
+        const RowRange rows = chunk_rows[q];
+        if (k2 < rows.lo || k2 >= rows.hi) continue;
         const double v = partial[(size_t)q * n + e];
         const double tt = s + v;
         c += (fabs(s) >= fabs(v)) ? (s - tt) + v : (v - tt) + s;
@@ -1024,7 +1137,8 @@ using StageFn = void (*)(const DNode*, int, const NodeSum*, const MPTask*, const
                          unsigned long long, int*, unsigned long long*, unsigned long long,
                          double* const*, int, C3Params, MPTask*, MPJob*, int*,
                          unsigned long long*, int*);
-using ZetaFn = void (*)(const MPJob*, int, const DNode*, double* const*, C3Params, int, double*);
+using ZetaFn = void (*)(const MPJob*, int, const DNode*, double* const*, C3Params, int, int, double*,
+                        RowRange*);
 using ExpandFn = void (*)(const double*, const double*, C3Params, double*);
 
 static SplitFn pick_split(int C) {
@@ -1096,11 +1210,12 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
                                (int)shbytes));
     const int NV = zeta_nv(D);
     const long long ncell = (long long)nb * (M + 1);
-    const long long zmax_cells = (96 * 1024) / (8 * NV);      // LDS tile of at most 96 KB
-    const int ncpart = (int)((ncell + zmax_cells - 1) / zmax_cells);
-    const int cells_per_part = (int)((ncell + ncpart - 1) / ncpart);
-    const size_t zt_bytes = sizeof(double) * (size_t)cells_per_part * NV;
+    const long long npair = (long long)nb * (nb + 1) / 2;
+    const int zmr = (int)std::min<long long>(ZT_MR, (long long)ZT_IPT * ZT_THREADS / npair);
+    if (zmr < 1) return fail("nbins too large for the zeta kernel's cells (at most 45 bins)");
+    const int nmr = (M + 1 + zmr - 1) / zmr;
     ZetaFn zfn = D == GData ? &k_mp_zeta<GData> : &k_mp_zeta<NData>;
+    const size_t zt_bytes = shbytes;    // the slot's layout, partly filled per job
     TC_HIP(hipFuncSetAttribute((const void*)zfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)zt_bytes));
     KernelStat& ks_stage = kernel_stat(KS_MULTIPOLE);
@@ -1187,6 +1302,7 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
     DBuf<int> ret, free_ids;
     DBuf<int> poolA, poolB;
     DBuf<double> partial, zhi, zlo, red;
+    DBuf<RowRange> chunk_rows;
     DBuf<double*> chunk_tab;
     std::vector<double*> chunks;
     struct ChunkGuard {
@@ -1260,13 +1376,22 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
         TC_HIP(hipGetLastError());
         if (timing) TC_HIP(hipEventRecord(e1, st));
         // this level's zeta jobs, in task order
-        const int nchunks = (int)((ntask + ZETA_JOBS - 1) / ZETA_JOBS);
-        if (partial.reserve((size_t)nchunks * ncomp, 0, st)) return 2;
-        hipLaunchKernelGGL(zfn, dim3(nchunks, nb, ncpart), dim3(ZT_THREADS), zt_bytes, st, jobs.p,
-                           (int)ntask, nodes, chunk_tab.p, P, cells_per_part, partial.p);
+        // enough chunks for about 4096 workgroups, at least 8 jobs each
+        const long long jpc = std::max<long long>(8, std::min<long long>(2048, (ntask * nmr + 4095) / 4096));
+        const int nchunks = (int)((ntask + jpc - 1) / jpc);
+        if (chunk_rows.reserve(nchunks, 0, st)) return 2;
+        if ((size_t)nchunks * ncomp > partial.n) {
+            // entries with k3 < k2 are never written: they stay zero
+            if (partial.reserve((size_t)nchunks * ncomp, 0, st)) return 2;
+            TC_HIP(hipMemsetAsync(partial.p, 0, sizeof(double) * partial.n, st));
+        }
+        hipLaunchKernelGGL(zfn, dim3(nchunks, nmr), dim3(ZT_THREADS), zt_bytes, st, jobs.p,
+                           (int)ntask, nodes, chunk_tab.p, P, zmr, (int)jpc, partial.p,
+                           chunk_rows.p);
         TC_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_mp_zeta_reduce, dim3((unsigned)((ncomp + 255) / 256)), dim3(256), 0, st,
-                           partial.p, nchunks, ncomp, zhi.p, zlo.p);
+                           partial.p, chunk_rows.p, nchunks, ncomp, (long long)nb * (M + 1) * NV,
+                           nb, zhi.p, zlo.p);
         TC_HIP(hipGetLastError());
         if (timing) TC_HIP(hipEventRecord(e2, st));
         // next tasks, and the slots that became free, in task order
