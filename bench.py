@@ -6,18 +6,24 @@ included, with the catalog columns already resident in HBM):
   70% uniform + 30% from 500 Gaussian clumps (sigma = 3 arcmin), g1,g2 ~ N(0, 0.2^2)
   clipped to |g| < 1, w ~ U[0.5, 1.5]; 25 log bins 1-200 arcmin, bin_slop = 0.1.
 Metric: pairs/s = sum(npairs) / wall time of the step.
+Multipole leg (the second half of the headline metric), under "multipole": GGG and
+NNN LogMultipole (max_n = 30, 20 log bins 5-100 arcmin) of 500,000 points, 70% uniform
++ 30% in 500 clumps (sigma = 3 arcmin) over a 5 x 5 deg flat field; triangles/s =
+sum(ntri at n = 0) / wall time of the step (tree build + multipole).
 
-Prints ONE JSON line (rank 0) with the roofline of the dominant kernel (the
-2-point traversal) measured with HIP events inside the timed region, and a CPU
-baseline: TreeCorr's own C++ engine (compiled from its sources into oracle/_ref) on
-all host cores, the whole job, with its sums compared to the GPU's (npairs exactly);
-without that build, the oracle's OpenMP restatement on a bounded sample.
+Prints ONE JSON line (rank 0) with the roofline of each leg's dominant kernel (the
+2-point traversal; the multipole's stage kernel) measured with HIP events inside the
+timed region, the plain GPU form beside the tuned one, and CPU baselines: TreeCorr's own
+C++ engine (compiled from its sources into oracle/_ref) on all host cores, the whole
+job, with its sums compared to the GPU's (npairs and ntri exactly); without that build,
+the oracle's OpenMP restatement on a bounded sample (GG only).
 """
 import argparse
 import ctypes
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -204,37 +210,75 @@ def gg_step(L, dev, n, bn, out):
 OPS_VISIT, OPS_ACCEPT_GG_SPHERE, OPS_REFINED, BYTES_VISIT = 25, 98, 25, 96
 
 
-def cpu_baseline_reference(x, y, z, g1, g2, w, bn, gpu_out):
+def host_cores():
+    """What the host offers and what the reference uses (its OpenMP thread count)."""
+    try:
+        nproc = int(subprocess.run(['nproc'], capture_output=True, text=True).stdout.strip())
+    except Exception:
+        nproc = None
+    return {"os_cpu_count": os.cpu_count(), "nproc": nproc,
+            "OMP_NUM_THREADS": os.environ.get('OMP_NUM_THREADS')}
+
+
+def normwise(a, b):
+    den = np.max(np.abs(b))
+    return 0. if den == 0 else float(np.max(np.abs(a - b)) / den)
+
+
+def cpu_baseline_reference(x, y, z, g1, g2, w, bn, gpu_out, runs=3):
     """The reference's own C++ engine (TreeCorr's _treecorr module compiled from its
     sources into oracle/_ref), the whole job on all host cores: tree build plus GG
-    auto-correlation of the 2M galaxies.  Its raw sums are compared with the GPU's from
-    the last timed step: npairs exactly, the FP64 sums normwise.  None when the module
-    is not built."""
+    auto-correlation of the 2M galaxies, the median of `runs` runs.  Its raw sums are
+    compared with the GPU's from the last timed step: npairs exactly, the FP64 sums
+    normwise.  A per-core figure comes from one thread on the galaxies of a 2.5 x 2.5 deg
+    corner of the cap (a bounded sample of the same workload).  None when the module is
+    not built."""
     from oracle import ref as R
     m = R.module()
     if m is None:
         return None
-    t0 = time.perf_counter()
-    f = R.make_field(4, 3, x, y, z, g1=g1, g2=g2, w=w, min_size=bn['min_size'],
-                     max_size=bn['max_size'], split_method=2, brute=False,
-                     min_top=bn['min_top'], max_top=bn['max_top'])
-    r = R.corr2(4, f, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'],
-                bn['a'])
-    t_full = time.perf_counter() - t0
+    nthreads = int(m.GetOMPThreads())
+    times = []
+    for i in range(runs):
+        t0 = time.perf_counter()
+        f = R.make_field(4, 3, x, y, z, g1=g1, g2=g2, w=w, min_size=bn['min_size'],
+                         max_size=bn['max_size'], split_method=2, brute=False,
+                         min_top=bn['min_top'], max_top=bn['max_top'])
+        r = R.corr2(4, f, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                    bn['b'], bn['a'])
+        times.append(time.perf_counter() - t0)
+        del f
+        print(f"# CPU reference run {i}: {times[-1]:.2f} s", file=sys.stderr, flush=True)
+    t_full = float(np.median(times))
     npairs = float(r['npairs'].sum())
-
-    def normwise(a, b):
-        den = np.max(np.abs(b))
-        return 0. if den == 0 else float(np.max(np.abs(a - b)) / den)
     parity = {"npairs_equal": bool(np.array_equal(gpu_out['npairs'], r['npairs'])),
               "sums_normwise_max": max(normwise(gpu_out[k], r[k]) for k in
                                        ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr',
                                         'weight'))}
-    return {"value": npairs / t_full, "unit": "pairs/s", "cores": int(m.GetOMPThreads()),
-            "kind": "reference",
-            "sample": (f"the whole job (no sampling): TreeCorr's C++ engine compiled from its "
-                       f"sources (oracle/_ref), tree build + GG auto of the same 2M-galaxy "
-                       f"catalog, {t_full:.1f} s"),
+    # one thread, a quarter by quarter corner (1/16 of the area)
+    ra = np.arctan2(y, x)
+    dec = np.arcsin(np.clip(z, -1., 1.))
+    deg = math.pi / 180.
+    sel = (ra < 2.5 * deg) & (dec < math.asin(math.sin(-5. * deg) + 0.25 * 2 * math.sin(5. * deg)))
+    m.SetOMPThreads(1)
+    t0 = time.perf_counter()
+    f1 = R.make_field(4, 3, x[sel], y[sel], z[sel], g1=g1[sel], g2=g2[sel], w=w[sel],
+                      min_size=bn['min_size'], max_size=bn['max_size'], split_method=2,
+                      brute=False, min_top=bn['min_top'], max_top=bn['max_top'])
+    r1 = R.corr2(4, f1, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                 bn['b'], bn['a'])
+    t1 = time.perf_counter() - t0
+    m.SetOMPThreads(nthreads)
+    return {"value": npairs / t_full, "unit": "pairs/s", "cores": nthreads,
+            "kind": "reference", "host": host_cores(),
+            "runs_s": times,
+            "sample": (f"the whole job (no sampling), median of {runs} runs: TreeCorr's C++ "
+                       f"engine compiled from its sources (oracle/_ref), tree build + GG auto of "
+                       f"the same 2M-galaxy catalog on {nthreads} OpenMP threads, "
+                       f"{t_full:.2f} s"),
+            "one_thread": {"value": float(r1['npairs'].sum()) / t1, "unit": "pairs/s",
+                           "sample": f"1 thread, the {int(sel.sum())} galaxies of a 2.5 x 2.5 deg "
+                                     f"corner, tree build + GG auto, {t1:.2f} s"},
             "parity_vs_gpu": parity}
 
 
@@ -270,6 +314,160 @@ def cpu_baseline(x, y, z, g1, g2, w, bn, gpu_counters, gpu_npairs, target_s=15.)
                        f"top-level row of the traversal ({t_samp:.2f} s, "
                        f"{100 * frac:.2f}% of the node-pair visits); full-job time "
                        f"extrapolated by visit count = {t_full:.1f} s")}
+
+
+# ---------------------------------------------------------------------------
+# Multipole leg: GGG (LogMultipole, max_n = 30) of 500k points, tree build included
+# ---------------------------------------------------------------------------
+KEYS3 = ('zeta0', 'zeta1', 'zeta2', 'zeta3', 'zeta4', 'zeta5', 'zeta6', 'zeta7', 'meand1',
+         'meanlogd1', 'meand2', 'meanlogd2', 'meand3', 'meanlogd3', 'meanu', 'meanv', 'weight',
+         'weight_im', 'ntri')
+
+
+# Algorithmic FP64 work of the multipole: a Gn call (accepted (c1, c2)) 24 M + 110 ops for
+# GGG, 8 M + 25 for NNN; the zeta contraction 8 flops per complex multiply-add, per zeta
+# cell (k2 row, k3 >= k2) and multipole n (2M+1 of them), for gam0..3 and the weight (5
+# outputs; NNN: the weight only).
+def gn_ops(d, M):
+    return 24 * M + 110 if d == 4 else 8 * M + 25
+
+
+def zeta_flops(d, M, cells):
+    return 8. * cells * (2 * M + 1) * (5 if d == 4 else 1)
+
+
+def mp_step(L, dev, n, bn, d, out):
+    from treecorr_amd import _lib
+    fh = ctypes.c_void_p()
+    g = d == 4
+    _lib.check(L.tc_field_create_device(d, 1, dev['x'], dev['y'], None,
+                                        dev['g1'] if g else None, dev['g2'] if g else None,
+                                        dev['w'], None, n, bn['min_size'], bn['max_size'], 2, 0, 0,
+                                        bn['min_top'], bn['max_top'], ctypes.byref(fh)))
+    for a in out.values():
+        a[:] = 0
+    h = ctypes.c_void_p()
+    ptrs = [_lib.dptr(out[k]) if (g or not k.startswith('zeta')) else None for k in KEYS3]
+    _lib.check(L.tc_corr3_create(d, 5, bn['min_sep'], bn['max_sep'], bn['nbins'],
+                                 bn['bin_size'], bn['b'], bn['a'], bn['max_n'], *ptrs,
+                                 ctypes.byref(h)))
+    _lib.check(L.tc_corr3_process_auto(h, fh, 0, 0, 1))
+    c = (ctypes.c_longlong * 4)()
+    _lib.check(L.tc_corr3_counters(h, c))
+    cells = ctypes.c_longlong()
+    _lib.check(L.tc_corr3_zeta_cells(h, ctypes.byref(cells)))
+    L.tc_corr3_destroy(h)
+    L.tc_field_destroy(fh)
+    return dict(gn_calls=c[0], zeta_calls=c[1], zeta_rows=c[2], stages=c[3],
+                zeta_cells=cells.value)
+
+
+def kernel_ms(L, which):
+    from treecorr_amd import _lib
+    ms = ctypes.c_double()
+    la = ctypes.c_longlong()
+    _lib.check(L.tc_kernel_stats(which, ctypes.byref(ms), ctypes.byref(la)))
+    return ms.value, la.value
+
+
+def timed_mp(L, hip, dev, n, bn, d, out, steps, warmup):
+    for _ in range(warmup):
+        mp_step(L, dev, n, bn, d, out)
+    L.tc_reset_kernel_stats()
+    L.tc_set_timing(1)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cnt = mp_step(L, dev, n, bn, d, out)
+    hip.sync()
+    t1 = time.perf_counter()
+    L.tc_set_timing(0)
+    return (t1 - t0) / steps, cnt, kernel_ms(L, 2), kernel_ms(L, 3)
+
+
+def cpu_baseline_mp(x, y, g1, g2, w, bn, d, gpu_out):
+    """TreeCorr's C++ engine (oracle/_ref) on the same catalog, all host cores, one run of
+    the whole job (tree build + multipole); ntri compared exactly, the sums normwise."""
+    from oracle import ref as R
+    m = R.module()
+    if m is None:
+        return None
+    t0 = time.perf_counter()
+    f = R.make_field(d, 1, x, y, None, g1=g1 if d == 4 else None, g2=g2 if d == 4 else None,
+                     w=w, min_size=bn['min_size'], max_size=bn['max_size'], split_method=2,
+                     brute=False, min_top=bn['min_top'], max_top=bn['max_top'])
+    r = R.corr3_multipole(d, f, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                          bn['b'], bn['a'], bn['max_n'])
+    t = time.perf_counter() - t0
+    M = bn['max_n']
+    keys = ['weight', 'weight_im', 'meand2', 'meanlogd2', 'meand3', 'meanlogd3'] + \
+        (['zeta%d' % i for i in range(8)] if d == 4 else [])
+    shape = r['ntri'].shape
+    ntri = float(r['ntri'][:, :, M].sum())
+    return {"value": ntri / t, "unit": "triangles/s", "cores": int(m.GetOMPThreads()),
+            "kind": "reference", "host": host_cores(), "seconds": t,
+            "sample": f"the whole job (no sampling), one run: TreeCorr's C++ engine compiled from "
+                      f"its sources (oracle/_ref), tree build + {'GGG' if d == 4 else 'NNN'} "
+                      f"multipole of the same catalog, {t:.1f} s",
+            "parity_vs_gpu": {
+                "ntri_equal": bool(np.array_equal(gpu_out['ntri'].reshape(shape), r['ntri'])),
+                "sums_normwise_max": max(normwise(gpu_out[k].reshape(shape), r[k]) for k in keys)}}
+
+
+def run_mp_leg(args, L, hip):
+    x, y, g1, g2, w = mp_flat_catalog(args.mp_n)
+    bn = mp_binning()
+    M = bn['max_n']
+    dev = {k: hip.upload(v) for k, v in dict(x=x, y=y, g1=g1, g2=g2, w=w).items()}
+    ntot = bn['nbins'] ** 2 * (2 * M + 1)
+    out = {k: np.zeros(ntot) for k in KEYS3}
+    res = {}
+    for d, name in ((4, 'GGG'), (0, 'NNN')):
+        print(f"# multipole {name} {args.mp_n}: tuned", file=sys.stderr, flush=True)
+        step_s, cnt, (st_ms, st_l), (zt_ms, zt_l) = timed_mp(L, hip, dev, args.mp_n, bn, d, out,
+                                                             args.steps, args.warmup)
+        ntri = float(out['ntri'].reshape(bn['nbins'], bn['nbins'], 2 * M + 1)[:, :, M].sum())
+        gpu_out = {k: out[k].copy() for k in out}
+        L.tc_set_multipole_plain(1)
+        print(f"# multipole {name}: plain form", file=sys.stderr, flush=True)
+        plain_s, _, (pst_ms, _), _ = timed_mp(L, hip, dev, args.mp_n, bn, d, out, 1, 0)
+        L.tc_set_multipole_plain(0)
+        flops = cnt['gn_calls'] * gn_ops(d, M)
+        st_s = st_ms * 1e-3 / args.steps
+        achieved = flops / st_s / 1e12 if st_s > 0 else 0.
+        zfl = zeta_flops(d, M, cnt['zeta_cells'])
+        zt_s = zt_ms * 1e-3 / args.steps
+        traffic = read_traffic(f'k_mp_stage_{name.lower()}_500k.txt')
+        leg = {
+            "workload": f"{name} LogMultipole max_n={M}, {args.mp_n} points (70% uniform + 30% "
+                        f"in 500 clumps) in a 5x5 deg flat field, 20 log bins 5-100 arcmin, "
+                        f"tree build included",
+            "value": ntri / step_s, "unit": "triangles/s", "ms_per_step": 1e3 * step_s,
+            "gn_pairs_per_s": cnt['gn_calls'] / step_s,
+            "plain_gpu": {"ms_per_step": 1e3 * plain_s, "value": ntri / plain_s,
+                          "stage_kernel_ms": pst_ms,
+                          "form": "Gn accumulation with every lane adding the multipoles in "
+                                  "order (lanes of one bin contend for each LDS address)"},
+            "tuned_over_plain": plain_s / step_s,
+            "counters": cnt,
+            "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                         "traffic": traffic, "kernel": f"k_mp_stage<{d},Flat>",
+                         "flops_per_step": flops, "kernel_ms_per_step": st_ms / args.steps,
+                         "launches_per_step": st_l / args.steps},
+            "zeta": {"kernel": f"k_mp_zeta<{d}> + k_mp_zeta_reduce",
+                     "kernel_ms_per_step": zt_ms / args.steps,
+                     "achieved_tflops": zfl / zt_s / 1e12 if zt_s > 0 else 0.,
+                     "flops_per_step": zfl},
+        }
+        if not args.no_cpu_baseline and d == 4:
+            print(f"# CPU baseline, {name} (the reference engine, all host cores)",
+                  file=sys.stderr, flush=True)
+            leg["cpu_baseline"] = cpu_baseline_mp(x, y, g1, g2, w, bn, d, gpu_out)
+        res[name] = leg
+    for v in dev.values():
+        hip.free(v)
+    return res
 
 
 # ---------------------------------------------------------------------------
@@ -461,6 +659,9 @@ def main():
     ap.add_argument('--patch-nd', type=int, default=4_000_000)
     ap.add_argument('--patch-nr', type=int, default=40_000_000)
     ap.add_argument('--npatch', type=int, default=100)
+    ap.add_argument('--mp-n', type=int, default=500_000,
+                    help='points of the multipole (GGG, NNN) leg of the headline line')
+    ap.add_argument('--no-mp', action='store_true', help='leave out the multipole leg')
     args = ap.parse_args()
     if args.calibrate_copy:
         from treecorr_amd import _lib
@@ -512,9 +713,22 @@ def main():
     kern_s = ms.value * 1e-3 / args.steps            # traversal kernel seconds per step
     achieved = flops / kern_s / 1e12 if kern_s > 0 else 0.
     launches_per_step = launches.value / args.steps
+    avg_launch_s = ms.value * 1e-3 / max(1, launches.value)
     traffic = read_traffic('k_traverse_gg_sphere_2M.txt')
+    gpu_out = {k: v.copy() for k, v in out.items()}
+    # the plain form: the tree built level by level in reference order (no speculative
+    # top levels); the traversal is the same
+    L.tc_set_tree_speculation(0)
+    hip.sync()
+    tp0 = time.perf_counter()
+    for i in range(args.steps):
+        gg_step(L, dev, args.n, bn, out)
+    hip.sync()
+    tp1 = time.perf_counter()
+    L.tc_set_tree_speculation(1)
+    plain_s = (tp1 - tp0) / args.steps
     res = {
-        "metric": "pairs/s (GG xi+/xi- 2M galaxies, bin_slop=0.1)",
+        "metric": "pairs/s (GG xi+/xi- 2M galaxies, bin_slop=0.1) + GGG multipole 500k",
         "value": value,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -530,27 +744,35 @@ def main():
                                "1-200 arcmin, bin_slop=0.1, tree build included",
                    "n": args.n, "nbins": nb, "bin_slop": 0.1, "min_top": bn['min_top'],
                    "max_top": bn['max_top']},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,
                      "kernel": "k_traverse<GData,Sphere>",
                      "flops_per_launch": flops / max(1., launches_per_step),
-                     "avg_launch_ms": ms.value / max(1, launches.value),
+                     "avg_launch_ms": 1e3 * avg_launch_s,
                      "launches_per_step": launches_per_step,
-                     "hbm_view_GBs": BYTES_VISIT * cnt['visits'] / kern_s / 1e9 if kern_s else 0.,
-                     "note": "FP64 peak 78.6 TF/s is the same for the vector ALUs this kernel "
-                             "uses and the matrix cores"},
+                     "hbm_GBs_from_counters": traffic / avg_launch_s / 1e9 if traffic else None,
+                     "note": "compute bound on the FP64 vector ALUs; priced at the 78.6 TF/s "
+                             "FP64 peak of MI355X"},
+        "plain_gpu": {"ms_per_step": 1e3 * plain_s, "value": npairs / plain_s,
+                      "form": "tree built level by level in reference order (no speculative "
+                              "top levels), same traversal"},
+        "tuned_over_plain": plain_s / ((t1 - t0) / args.steps),
         "counters": cnt,
         "traversal_ms_per_step": ms.value / args.steps,
     }
     if not args.no_cpu_baseline:
         print("# CPU baseline (the reference engine, all host cores)", file=sys.stderr,
               flush=True)
-        base = cpu_baseline_reference(x, y, z, g1, g2, w, bn, out)
+        base = cpu_baseline_reference(x, y, z, g1, g2, w, bn, gpu_out)
         res["cpu_baseline"] = base if base is not None else \
             cpu_baseline(x, y, z, g1, g2, w, bn, cnt, npairs)
+        if res["cpu_baseline"] is not None:
+            res["cpu_baseline"]["gpu_over_cpu"] = value / res["cpu_baseline"]["value"]
     for v in dev.values():
         hip.free(v)
+    if not args.no_mp:
+        res["multipole"] = run_mp_leg(args, L, hip)
     print(json.dumps(res))
 
 

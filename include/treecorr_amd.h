@@ -179,6 +179,12 @@ int tc_set_traversal_chunk(long long items);
  * nothing and returns the number of exact rebuilds so far.  No reference counterpart:
  * TreeCorr builds its trees on the host (src/Cell.cpp:556-615). */
 int tc_set_tree_speculation(int mode);
+/* The multipole's Gn accumulation in its plain form: every lane of a wavefront adds the
+ * multipoles m = 0, 1, ... in order, so lanes with pairs in the same bin contend for one
+ * LDS address (the untuned baseline of the benchmark).  on = 0 (default) restores the
+ * tuned form, in which each lane starts at its own multipole.  Results agree to the
+ * last bits.  Returns 0, or with on < 0 the current setting.  No reference counterpart. */
+int tc_set_multipole_plain(int on);
 int tc_corr2_destroy(tc_corr2* c);
 
 /* ---- 3-point multipole: replace {NNN,GGG}Corr(...) + processAuto ----
@@ -201,6 +207,10 @@ int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int met
 /* Work counters of the last process call: Gn calls, zeta calls, zeta rows,
  * multipoleFinish stages (no reference counterpart; roofline accounting). */
 int tc_corr3_counters(const tc_corr3* c, long long* counters4);
+/* Zeta cells of the last process call: the (k2, k3 >= k2) entries that its zeta
+ * contractions added to, summed over the contractions (the unit of the zeta work,
+ * times 2*max_n+1 multipoles).  No reference counterpart. */
+int tc_corr3_zeta_cells(const tc_corr3* c, long long* cells);
 int tc_corr3_destroy(tc_corr3* c);
 
 /* ---- Host utilities ---- */

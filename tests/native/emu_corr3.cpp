@@ -114,7 +114,7 @@ int main(int argc, char** argv) {
     if (!in) return 2;
     const long long ntot = h[4] * h[4] * (2 * h[5] + 1);
     std::vector<double> out(15 * ntot, 0.);
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     char err[256] = {0};
     const int rc = emu_corr3((int)h[0], (int)h[1], nn, pos.data(), size.data(), w.data(), g.data(),
                              n.data(), left.data(), right.data(), top.data(), (int)ntop, (int)h[4],
@@ -126,7 +126,7 @@ int main(int argc, char** argv) {
     }
     std::ofstream o(argv[2], std::ios::binary);
     o.write((const char*)out.data(), 8 * out.size());
-    o.write((const char*)cnt, sizeof cnt);
+    o.write((const char*)cnt, 4 * sizeof(unsigned long long));
     return o ? 0 : 2;
 }
 #endif

@@ -74,7 +74,7 @@ int main(int argc, char** argv) {
     P.maxbsq_eff = P.maxsepsq * std::min(P.bsq, P.asq);
     const long long ntot = (long long)P.nbins * P.nbins * (2 * P.maxn + 1);
     std::vector<double> out(15 * ntot, 0.);
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     rc = corr3_run(D, C, T.nodes, T.nnodes, T.dtop, (int)T.top.size(), P, out.data(), cnt);
     if (rc) {
         std::fprintf(stderr, "emu_lib3: multipole: %s\n", g_err.c_str());
@@ -82,7 +82,7 @@ int main(int argc, char** argv) {
     }
     std::ofstream o(argv[2], std::ios::binary);
     o.write((const char*)out.data(), 8 * out.size());
-    o.write((const char*)cnt, sizeof cnt);
+    o.write((const char*)cnt, 4 * sizeof(unsigned long long));
     const long long nn = T.nnodes, nt = (long long)T.top.size();
     o.write((const char*)&nn, 8);
     o.write((const char*)&nt, 8);

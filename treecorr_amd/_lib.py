@@ -47,6 +47,7 @@ SIGNATURES = {
                                     _ip, _i, _i, _i, _dp]),
     'tc_quick_assign': (_i, [_dp, _i, _dp, _dp, _dp, _ll, _ip]),
     'tc_set_tree_speculation': (_i, [_i]),
+    'tc_set_multipole_plain': (_i, [_i]),
     'tc_kmeans_run': (_i, [_dp, _i, _i, _dp, _dp, _dp, _dp, _ll, _i, _d, _d, _i,
                            ctypes.POINTER(ctypes.c_int)]),
     'tc_patch_split': (_i, [_ip, _ll, _i, _i, ctypes.POINTER(_dp), ctypes.POINTER(_dp),
@@ -66,6 +67,7 @@ SIGNATURES = {
                         [ctypes.POINTER(_vp)]),
     'tc_corr3_process_auto': (_i, [_vp, _vp, _i, _i, _i]),
     'tc_corr3_counters': (_i, [_vp, ctypes.POINTER(_ll)]),
+    'tc_corr3_zeta_cells': (_i, [_vp, ctypes.POINTER(_ll)]),
     'tc_corr3_destroy': (_i, [_vp]),
 }
 

@@ -616,6 +616,7 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
 
 int tc_set_traversal_chunk(long long items) { return tc::set_traversal_chunk(items); }
 int tc_set_tree_speculation(int mode) { return tc::set_tree_speculation(mode); }
+int tc_set_multipole_plain(int on) { return tc::set_multipole_plain(on); }
 
 int tc_quick_assign(const double* centers, int npatch, const double* x, const double* y,
                     const double* z, long long n, int* patch) {
@@ -752,7 +753,7 @@ struct tc_corr3 {
     double* z[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *meand2 = nullptr, *meanlogd2 = nullptr, *meand3 = nullptr, *meanlogd3 = nullptr;
     double *weight = nullptr, *weight_im = nullptr, *ntri = nullptr;
-    long long counters[4] = {0, 0, 0, 0};
+    long long counters[5] = {0, 0, 0, 0, 0};
 };
 
 extern "C" {
@@ -813,7 +814,7 @@ int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int met
     P.quick = quick ? 1 : 0;
     const int ntot = P.nbins * P.nbins * (2 * P.maxn + 1);
     std::vector<double> out((size_t)15 * ntot);
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     std::lock_guard<std::mutex> run_lock(g_run_m);
     int rc = tc::corr3_run(c->d, f->coords, f->T.nodes, f->T.nnodes, f->T.dtop,
                            (int)f->T.top.size(), P, out.data(), cnt);
@@ -825,13 +826,19 @@ int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int met
                       c->ntri};
     for (int q = 0; q < 7; ++q)
         for (int j = 0; j < ntot; ++j) dst[q][j] += out[(size_t)(8 + q) * ntot + j];
-    for (int k = 0; k < 4; ++k) c->counters[k] = (long long)cnt[k];
+    for (int k = 0; k < 5; ++k) c->counters[k] = (long long)cnt[k];
     return 0;
 }
 
 int tc_corr3_counters(const tc_corr3* c, long long* counters4) {
     if (!c || !counters4) return tc::fail("NULL argument");
     for (int k = 0; k < 4; ++k) counters4[k] = c->counters[k];
+    return 0;
+}
+
+int tc_corr3_zeta_cells(const tc_corr3* c, long long* cells) {
+    if (!c || !cells) return tc::fail("NULL argument");
+    *cells = c->counters[4];
     return 0;
 }
 

@@ -27,6 +27,7 @@
 #include "tc_engine.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -62,7 +63,8 @@ enum { S_NPAIRS = 0, S_SUMW, S_SUMWR, S_SUMWLOGR, S_SUMWW, S_SUMWWR, S_SUMWWLOGR
 // Output arrays per slot: zeta0..7, meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri
 enum { O_Z0 = 0, O_MEAND2 = 8, O_MEANLOGD2, O_MEAND3, O_MEANLOGD3, O_WEIGHT, O_WEIGHT_IM, O_NTRI,
        NOUT3 = 15 };
-enum { C3_GN = 0, C3_ZETA, C3_ROWS, C3_STAGES, NC3 = 4 };
+// counters: Gn calls, zeta calls, zeta rows, finish stages, zeta cells (k2 row, k3 >= k2)
+enum { C3_GN = 0, C3_ZETA, C3_ROWS, C3_STAGES, C3_CELLS, NC3 = 5 };
 
 constexpr int MP_THREADS = 256;
 
@@ -183,7 +185,7 @@ __device__ __forceinline__ void cpow_int(double zr, double zi, int e, double& pr
 template <int D, int C>
 __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, const DNode& c1,
                                        const DNode& c2, const NodeSum& s2, double r,
-                                       double logr, int k, int maxn) {
+                                       double logr, int k, int maxn, int plain) {
     const double n = double(c2.n), w = c2.w;
     atomicAdd(&sc[Lo.S(S_NPAIRS, k)], n);
     atomicAdd(&sc[Lo.S(S_SUMW, k)], w);
@@ -236,7 +238,7 @@ __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, cons
     }
     // W(k, 0..M); G(k, -(M+1)..M+1) for GGG
     const int nm = D == GData ? maxn + 2 : maxn + 1;
-    int m = (int)(threadIdx.x & 63u) % nm;
+    int m = plain ? 0 : (int)(threadIdx.x & 63u) % nm;
     double pr, pi;
     cpow_int(zr, zi, m, pr, pi);
     for (int t = 0; t < nm; ++t) {
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
                 if (cl == 1) {
                     if (k >= 0 && k < P.nbins) {
                         ++c_gn;
-                        add_gn<D, C>(Lo, sc, c1, c2, nsum[i2], r, logr, k, P.maxn);
+                        add_gn<D, C>(Lo, sc, c1, c2, nsum[i2], r, logr, k, P.maxn, P.plain);
                     } else {
                         s_bad = 1;     // r at the very edge of the range: no bin
                     }
@@ -763,6 +765,10 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
         if (job.c1 >= 0) {
             atomicAdd(&counters[C3_ZETA], 1ull);
             atomicAdd(&counters[C3_ROWS], (unsigned long long)(job.mk - job.ks));
+            // cells (k2, k3 >= k2) of the rows [ks, mk)
+            const int nb = P.nbins;
+            const long long a = nb - job.ks, b = nb - job.mk;
+            atomicAdd(&counters[C3_CELLS], (unsigned long long)((a * (a + 1) - b * (b + 1)) / 2));
         }
     }
 }
@@ -1192,12 +1198,22 @@ struct Compactor {
     }
 };
 
+static std::atomic<int> g_mp_plain{0};
+int set_multipole_plain(int on) {
+    if (on < 0) return g_mp_plain.load();
+    g_mp_plain.store(on ? 1 : 0);
+    return 0;
+}
+
 // Runs the multipole auto-correlation of one device tree; writes NOUT3*ntot reduced
 // sums (zeta0..7, meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri) into
-// out (host) and the work counters into cnt (4 values).
+// out (host) and the work counters into cnt (NC3 = 5 values: Gn calls, zeta calls, zeta
+// rows, stages, zeta cells).
 int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int ntop,
-              const C3Params& P, double* out, unsigned long long* cnt) {
+              const C3Params& P_in, double* out, unsigned long long* cnt) {
     hipStream_t st = stream();
+    C3Params P = P_in;
+    P.plain = g_mp_plain.load();
     const int nb = P.nbins, M = P.maxn;
     const long long ntot = (long long)nb * nb * (2 * M + 1);
     ScratchLayout Lo;

@@ -81,13 +81,17 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
 // LogMultipole parameters of one 3-point correlation (BaseCorr3, src/Corr3.cpp:70-127).
 struct C3Params {
     int nbins, maxn, D, quick;
+    int plain;      // Gn accumulation with every lane in multipole order (the untuned form)
+    int pad;
     double minsep, maxsep, binsize, b, a;
     double logminsep, minsepsq, maxsepsq, bsq, asq, maxbsq_eff;
 };
 
 // Multipole auto-correlation of one device tree: writes 15*ntot sums (zeta0..7,
-// meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri) and 4 counters.
+// meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri) and 5 counters (Gn
+// calls, zeta calls, zeta rows, stages, zeta cells).
 int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int ntop,
               const C3Params& P, double* out, unsigned long long* cnt);
+int set_multipole_plain(int on);   // < 0: returns the current setting
 
 }  // namespace tc
