@@ -150,6 +150,18 @@ int tc_quick_assign(const double* centers, int npatch, const double* x, const do
 int tc_kmeans_run(double* centers, int npatch, int dim, const double* x, const double* y,
                   const double* z, const double* w, long long n, int max_iter, double tol,
                   double size, int sphere, int* niter);
+/* tc_kmeans_run without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * tc_kmeans_run, except that centers, x, y, z and w are DEVICE pointers (centers is
+ * refined in place on the device).  The per-centre sums live in one fixed static device
+ * array of 8192 * 5 + 8 doubles (about 320 KB, held by each GPU the library runs on
+ * until the process ends); npatch above 8192 is rejected with an error.  It calls no
+ * hipMalloc/hipFree.  Calls that overlap on one GPU share that array and race: run them
+ * one after another. */
+int tc_kmeans_run_noalloc(double* centers, int npatch, int dim, const double* x,
+                          const double* y, const double* z, const double* w, long long n,
+                          int max_iter, double tol, double size, int sphere, int* niter);
 /* Splits a catalog's columns by patch: a stable counting sort on host threads.
  * Replaces the per-patch np.where + fancy indexing of Catalog.get_patches
  * (treecorr/catalog.py:2966-3050).  out[c] receives cols[c] ordered by patch

@@ -640,6 +640,18 @@ int tc_kmeans_run(double* centers, int npatch, int dim, const double* x, const d
                           niter);
 }
 
+int tc_kmeans_run_noalloc(double* centers, int npatch, int dim, const double* x,
+                          const double* y, const double* z, const double* w, long long n,
+                          int max_iter, double tol, double size, int sphere, int* niter) {
+    if (!centers || npatch <= 0 || (dim != 2 && dim != 3) || !x || !y || (dim == 3 && !z) ||
+        !w || n <= 0 || max_iter < 0 || !(tol >= 0.) || !(size >= 0.))
+        return tc::fail("NULL or invalid argument");
+    if (int rc = check_gpu()) return rc;
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::kmeans_run_noalloc(centers, npatch, dim, x, y, z, w, n, max_iter, tol, size,
+                                  sphere, niter);
+}
+
 int tc_patch_split(const int* patch, long long n, int npatch, int ncol, const double* const* cols,
                    double* const* out, long long* counts) {
     if (!patch || n < 0 || npatch <= 0 || ncol < 0 || !counts || (ncol > 0 && (!cols || !out)))

@@ -58,6 +58,11 @@ struct C2Params {
 int kmeans_run(double* centers, int npatch, int dim, const double* x, const double* y,
                const double* z, const double* w, long long n, int max_iter, double tol,
                double size, int sphere, int* niter);
+// kmeans_run on device-resident columns and centres, with a fixed static scratch array
+// (no allocation): at most 8192 centres.
+int kmeans_run_noalloc(double* centers, int npatch, int dim, const double* x, const double* y,
+                       const double* z, const double* w, long long n, int max_iter, double tol,
+                       double size, int sphere, int* niter);
 // QuickAssign on the device: patch[i] = the nearest of npatch centres (2 or 3 values
 // each; z == nullptr means 2), host arrays in and out.
 int quick_assign(const double* centers, int npatch, const double* x, const double* y,

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace tc {
@@ -221,6 +223,106 @@ int kmeans_run(double* centers, int npatch, int dim, const double* x, const doub
     release();
     if (e != hipSuccess) return hip_fail(e, "running k-means");
     std::copy(hc.begin(), hc.end(), centers);
+    if (niter) *niter = it;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// k-means without allocation: the same Lloyd iterations on device-resident columns,
+// for callers that cannot allocate (HIP graph capture, serving loops).  The per-centre
+// sums and the centre update live in a fixed static device array, `scratch`.
+// ---------------------------------------------------------------------------
+constexpr int KM_NOALLOC_MAX = 8192;                       // centres at most
+// per-centre sums (KM_NS each), then per-centre squared shifts, then the total shift
+constexpr size_t KM_SCRATCH_DOUBLES = (size_t)KM_NOALLOC_MAX * (KM_NS + 1) + 8;
+static __device__ __align__(256) double scratch[KM_SCRATCH_DOUBLES];
+
+__global__ void k_kmeans_zero(double* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.;
+}
+
+// UpdateCenters::finalize (src/KMeans.cpp:362-372), as kmeans_run does it on the host:
+// each centre moves to its objects' w-weighted mean (normalised for the sphere; an empty
+// patch's centre becomes the origin), then one thread adds the squared shifts in centre
+// order.  The sums are cleared for the next step.
+__global__ void k_kmeans_update(double* __restrict__ centers, int npatch, int dim, int sphere,
+                                double* __restrict__ sums, double* __restrict__ dsq,
+                                double* __restrict__ shiftsq) {
+    for (int k = threadIdx.x; k < npatch; k += blockDim.x) {
+        double* sk = &sums[(size_t)k * KM_NS];
+        double c[3] = {0., 0., 0.};
+        if (sk[3] > 0.) {
+            for (int q = 0; q < dim; ++q) c[q] = sk[q] / sk[3];
+            if (sphere) {
+                const double nrm = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                if (nrm == 0.) { c[0] = 1.; c[1] = c[2] = 0.; }
+                else for (int q = 0; q < 3; ++q) c[q] /= nrm;
+            }
+        }
+        double d = 0.;
+        for (int q = 0; q < dim; ++q) {
+            const double t = centers[(size_t)k * dim + q] - c[q];
+            d += t * t;
+            centers[(size_t)k * dim + q] = c[q];
+        }
+        dsq[k] = d;
+        for (int q = 0; q < KM_NS; ++q) sk[q] = 0.;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.;
+        for (int k = 0; k < npatch; ++k) s += dsq[k];
+        *shiftsq = s;
+    }
+}
+
+namespace {
+std::mutex g_km_m;
+double* g_km_scratch[64] = {nullptr};     // the array's address on each device
+}  // namespace
+
+int kmeans_run_noalloc(double* centers, int npatch, int dim, const double* x, const double* y,
+                       const double* z, const double* w, long long n, int max_iter, double tol,
+                       double size, int sphere, int* niter) {
+    if (npatch > KM_NOALLOC_MAX)
+        return fail("k-means without allocation: at most 8192 centres (npatch = " +
+                    std::to_string(npatch) + ")");
+    int dev = 0;
+    TC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail("k-means without allocation: device index beyond 63");
+    double* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_km_m);
+        if (!g_km_scratch[dev]) {
+            void* p = nullptr;
+            TC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_km_scratch[dev] = static_cast<double*>(p);
+        }
+        ptr = g_km_scratch[dev];
+    }
+    double* sums = ptr;
+    double* dsq = ptr + (size_t)KM_NOALLOC_MAX * KM_NS;
+    double* shift = dsq + KM_NOALLOC_MAX;
+    hipStream_t st = stream();
+    const dim3 grid((unsigned)((n + QA_THREADS - 1) / QA_THREADS));
+    const int nz = npatch * KM_NS;
+    hipLaunchKernelGGL(k_kmeans_zero, dim3((nz + 255) / 256), dim3(256), 0, st, sums, nz);
+    TC_HIP(hipGetLastError());
+    const double tolsq = (tol * size) * (tol * size) * npatch;
+    int it = 0;
+    for (; it < max_iter; ++it) {
+        if (dim == 3) hipLaunchKernelGGL(k_kmeans_step<3>, grid, dim3(QA_THREADS), 0, st, centers, npatch, x, y, z, w, n, sums);
+        else hipLaunchKernelGGL(k_kmeans_step<2>, grid, dim3(QA_THREADS), 0, st, centers, npatch, x, y, z, w, n, sums);
+        TC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_kmeans_update, dim3(1), dim3(256), 0, st, centers, npatch, dim, sphere,
+                           sums, dsq, shift);
+        TC_HIP(hipGetLastError());
+        double shiftsq = 0.;
+        TC_HIP(hipMemcpyAsync(&shiftsq, shift, sizeof(double), hipMemcpyDeviceToHost, st));
+        TC_HIP(hipStreamSynchronize(st));
+        if (shiftsq < tolsq) { ++it; break; }
+    }
     if (niter) *niter = it;
     return 0;
 }
