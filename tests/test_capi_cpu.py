@@ -125,3 +125,11 @@ def test_patch_split_is_a_stable_split_by_patch():
                                    (dp * 1)(out.ctypes.data_as(dp)),
                                    counts.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
     assert rc == _lib.TC_ERR_INVALID
+
+
+def test_inverted_rpar_range_is_rejected():
+    """corr2base.py:545-546: min_rpar > max_rpar raises ValueError (host-side check)."""
+    import pytest
+    import treecorr_amd as tca
+    with pytest.raises(ValueError, match='min_rpar'):
+        tca.NNCorrelation(nbins=5, min_sep=1., max_sep=10., min_rpar=5., max_rpar=-5.)

@@ -147,6 +147,8 @@ class Corr2:
         mn, mx = cfg.get('min_rpar'), cfg.get('max_rpar')
         self.min_rpar = float(mn) if mn is not None else -sys.float_info.max
         self.max_rpar = float(mx) if mx is not None else sys.float_info.max
+        if self.min_rpar > self.max_rpar:      # corr2base.py:545-546
+            raise ValueError("min_rpar must be <= max_rpar")
         # corr2base.py:547-550: the Periodic metric's periods
         period = float(cfg.get('period') or 0.)
         self.xperiod = float(cfg.get('xperiod') or period)
