@@ -460,6 +460,36 @@ def run_mp_leg(args, L, hip):
                      "achieved_tflops": zfl / zt_s / 1e12 if zt_s > 0 else 0.,
                      "flops_per_step": zfl},
         }
+        # the zeta contraction on the matrix cores, against the FP64 contraction above
+        nn = 2 * M + 1
+        shape = (bn['nbins'], bn['nbins'], nn)
+        pairs = [('weight', 'weight_im')] + ([('zeta%d' % (2 * i), 'zeta%d' % (2 * i + 1))
+                                               for i in range(4)] if d == 4 else [])
+        leg["zeta_matrix_cores"] = {}
+        for mode, mname in ((1, 'fp32_mfma'), (2, 'bf16x3_mfma')):
+            L.tc_set_zeta_mode(mode)
+            print(f"# multipole {name}: zeta on the matrix cores, {mname}", file=sys.stderr,
+                  flush=True)
+            try:
+                m_s, _, _, (mzt_ms, _) = timed_mp(L, hip, dev, args.mp_n, bn, d, out, args.steps, 1)
+            finally:
+                L.tc_set_zeta_mode(0)
+            errs = {}
+            for kr, ki in pairs:
+                zr = (gpu_out[kr] + 1j * gpu_out[ki]).reshape(shape)
+                zg = (out[kr] + 1j * out[ki]).reshape(shape)
+                per_n = []
+                for i in range(nn):
+                    den = np.max(np.abs(zr[:, :, i]))
+                    per_n.append(float(np.max(np.abs(zg[:, :, i] - zr[:, :, i])) / den)
+                                 if den > 0 else 0.)
+                errs[kr] = {"normwise_max_over_n": max(per_n), "normwise_per_n": per_n}
+            leg["zeta_matrix_cores"][mname] = {
+                "ms_per_step": 1e3 * m_s, "zeta_kernels_ms_per_step": mzt_ms / args.steps,
+                "ntri_equal": bool(np.array_equal(out['ntri'], gpu_out['ntri'])),
+                "error_vs_fp64": errs,
+                "error_definition": "per multipole n: max over (k2,k3) of |Z - Z_fp64| / max "
+                                    "over (k2,k3) of |Z_fp64|, Z the complex weight or gam_i"}
         if not args.no_cpu_baseline and d == 4:
             print(f"# CPU baseline, {name} (the reference engine, all host cores)",
                   file=sys.stderr, flush=True)

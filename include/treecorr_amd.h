@@ -197,6 +197,15 @@ int tc_set_tree_speculation(int mode);
  * tuned form, in which each lane starts at its own multipole.  Results agree to the
  * last bits.  Returns 0, or with on < 0 the current setting.  No reference counterpart. */
 int tc_set_multipole_plain(int on);
+/* Precision of the multipole's zeta contraction for the cells k3 != k2 (the weight and
+ * zeta0..7 sums; the diagonal cells, ntri and the mean distances always stay FP64):
+ * mode 0 (default) FP64 on the vector ALUs, results within 1e-12 of the reference;
+ * 1: FP32 operands on 16x16x4 FP32 MFMAs, FP32 sums per block of 32 centres added in
+ * FP64; 2: each FP32 operand split into three BF16 parts, six products per pair on
+ * 16x16x32 BF16 MFMAs, the same block sums.  Modes 1 and 2 need nbins <= 32 (else 0 is
+ * used); their error against mode 0 is about 1e-7 of each multipole's largest value.
+ * Returns 0, or with mode < 0 the current mode.  No reference counterpart. */
+int tc_set_zeta_mode(int mode);
 int tc_corr2_destroy(tc_corr2* c);
 
 /* ---- 3-point multipole: replace {NNN,GGG}Corr(...) + processAuto ----

@@ -15,6 +15,8 @@ int fail(const std::string& m) { g_err = m; return 1; }
 int hip_fail(hipError_t, const char* what) { g_err = std::string("emulated HIP failure in ") + what; return 2; }
 hipStream_t stream() { return nullptr; }
 bool timing_enabled() { return false; }
+// the matrix-core zeta (tc_zeta_mfma.hip) is GPU code: the emulation runs the FP64 one
+ZetaMfmaFn pick_zeta_mfma(int, int, int, int& threads, size_t& bytes) { threads = 0; bytes = 0; return nullptr; }
 KernelStat& kernel_stat(int which) {
     static KernelStat s[KS_COUNT];
     return s[which];

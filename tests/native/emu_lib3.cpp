@@ -10,7 +10,7 @@
 // output: 15*ntot f64 sums, 4 u64 counters, int64 nnodes, int64 ntop, then the nodes
 //         (nnodes DNode records of 48 bytes) and the top list (ntop int32).
 // Exit status 0 on success, 3 on an engine error (message on stderr).
-#include "tc_engine.h"
+#include "tc_mp.h"
 
 #include <cstdio>
 #include <fstream>
@@ -26,6 +26,8 @@ hipStream_t stream() { return nullptr; }
 hipStream_t stream2() { static int s2; return (hipStream_t)&s2; }
 hipStream_t stream3() { static int s3; return (hipStream_t)&s3; }
 bool timing_enabled() { return false; }
+// the matrix-core zeta (tc_zeta_mfma.hip) is GPU code: the emulation runs the FP64 one
+ZetaMfmaFn pick_zeta_mfma(int, int, int, int& threads, size_t& bytes) { threads = 0; bytes = 0; return nullptr; }
 KernelStat& kernel_stat(int which) {
     static KernelStat s[KS_COUNT];
     return s[which];

@@ -87,3 +87,45 @@ def test_gpu_ggg_public_api(gpu):
     # W_{-n} = conj(W_n) symmetry of the weight
     np.testing.assert_allclose(ggg.weightr[:, :, ::-1], ggg.weightr.transpose(1, 0, 2), rtol=1e-12,
                                atol=1e-9 * np.max(np.abs(ggg.weightr)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('mode', [1, 2], ids=['fp32_mfma', 'bf16x3_mfma'])
+@pytest.mark.parametrize('d,coords,n,max_n', [(4, 1, 30000, 16), (0, 1, 30000, 12),
+                                              (4, 3, 20000, 10)])
+def test_gpu_multipole_zeta_on_matrix_cores(gpu, mode, d, coords, n, max_n):
+    """The zeta contraction on the matrix cores (FP32, or BF16 split in three) against
+    the FP64 contraction of the same run: for every multipole n, each complex output
+    (weight + i weight_im, and gam0..3 = zeta0 + i zeta1, ...) within 1e-6 of its largest
+    modulus over (k2, k3) (normwise); ntri and the mean distances, which stay FP64,
+    unchanged to 1e-12."""
+    import capi
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    cat, scale = C.make_catalog(coords, n, 90 + d + coords, 0.3)
+    bn = capi.multipole_binning(scale, 12, 0.01, 0.2, max_n)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    print('zeta mode', mode, 'd', d, 'coords', coords, 'n', n, 'max_n', max_n, flush=True)
+    f = capi.Field(d, coords, cat, **kw)
+    args = (d, f, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'],
+            bn['max_n'])
+    ref = capi.corr3(*args)
+    assert L.tc_set_zeta_mode(mode) == 0
+    try:
+        got = capi.corr3(*args)
+    finally:
+        L.tc_set_zeta_mode(0)
+    np.testing.assert_array_equal(got['ntri'], ref['ntri'])
+    for k in ('meand2', 'meanlogd2', 'meand3', 'meanlogd3'):
+        assert normwise(got[k], ref[k]) < RTOL, k
+    pairs = [('weight', 'weight_im')] + ([('zeta%d' % (2 * i), 'zeta%d' % (2 * i + 1))
+                                           for i in range(4)] if d == 4 else [])
+    for kr, ki in pairs:
+        zr = ref[kr] + 1j * ref[ki]
+        zg = got[kr] + 1j * got[ki]
+        for i in range(2 * max_n + 1):
+            den = np.max(np.abs(zr[:, :, i]))
+            if den == 0:
+                continue
+            err = np.max(np.abs(zg[:, :, i] - zr[:, :, i])) / den
+            assert err < 1e-6, (kr, i - max_n, err)

@@ -48,6 +48,7 @@ SIGNATURES = {
     'tc_quick_assign': (_i, [_dp, _i, _dp, _dp, _dp, _ll, _ip]),
     'tc_set_tree_speculation': (_i, [_i]),
     'tc_set_multipole_plain': (_i, [_i]),
+    'tc_set_zeta_mode': (_i, [_i]),
     'tc_kmeans_run': (_i, [_dp, _i, _i, _dp, _dp, _dp, _dp, _ll, _i, _d, _d, _i,
                            ctypes.POINTER(ctypes.c_int)]),
     'tc_kmeans_run_noalloc': (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _ll, _i, _d, _d, _i,

@@ -617,6 +617,7 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
 int tc_set_traversal_chunk(long long items) { return tc::set_traversal_chunk(items); }
 int tc_set_tree_speculation(int mode) { return tc::set_tree_speculation(mode); }
 int tc_set_multipole_plain(int on) { return tc::set_multipole_plain(on); }
+int tc_set_zeta_mode(int mode) { return tc::set_zeta_mode(mode); }
 
 int tc_quick_assign(const double* centers, int npatch, const double* x, const double* y,
                     const double* z, long long n, int* patch) {

@@ -24,7 +24,7 @@
 // The outputs are kept compressed (k3 >= k2, n >= 0: the other three quarters of the
 // reference's arrays are copies or conjugates) and expanded into TreeCorr's layout at
 // the end (k_mp_expand).
-#include "tc_engine.h"
+#include "tc_mp.h"
 
 #include <algorithm>
 #include <atomic>
@@ -33,33 +33,6 @@
 
 namespace tc {
 
-struct NodeSum {          // BaseCell::calculateSumWSq / SumWZSq / SumAbsWZSq (src/Cell.cpp:143-168)
-    double wsq, wzr, wzi, abswz;
-};
-
-// Scratch layout (doubles) in LDS, for nb bins and maxn M:
-//   Wn  : nb*(M+1) complex           TreeCorr's Windex(k)+n (MultipoleScratch.h:80-81)
-//   Gn  : nb*(2M+3) complex (GGG)    Gindex(k)+n, n in [-(M+1), M+1] (:334-335)
-//   sums: npairs sumw sumwr sumwlogr sumww sumwwr sumwwlogr (7*nb)
-//   gg  : sumwwgg0,1,2 (GGG) 3*nb complex
-struct ScratchLayout {
-    int nb, M, D;
-    int wn, gn, sums, gg, total;
-    __host__ __device__ void init(int nb_, int M_, int D_) {
-        nb = nb_; M = M_; D = D_;
-        wn = 0;
-        gn = wn + 2 * nb * (M + 1);
-        sums = gn + (D == GData ? 2 * nb * (2 * M + 3) : 0);
-        gg = sums + 7 * nb;
-        total = gg + (D == GData ? 6 * nb : 0);
-    }
-    __device__ int W(int k, int m) const { return wn + 2 * (k * (M + 1) + m); }
-    __device__ int G(int k, int m) const { return gn + 2 * (k * (2 * M + 3) + (M + 1) + m); }
-    __device__ int S(int q, int k) const { return sums + q * nb + k; }
-    __device__ int GG(int q, int k) const { return gg + 2 * (q * nb + k); }
-};
-
-enum { S_NPAIRS = 0, S_SUMW, S_SUMWR, S_SUMWLOGR, S_SUMWW, S_SUMWWR, S_SUMWWLOGR };
 // Output arrays per slot: zeta0..7, meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri
 enum { O_Z0 = 0, O_MEAND2 = 8, O_MEANLOGD2, O_MEAND3, O_MEANLOGD3, O_WEIGHT, O_WEIGHT_IM, O_NTRI,
        NOUT3 = 15 };
@@ -103,12 +76,6 @@ __device__ __forceinline__ bool too_small3(const C3Params& P, double rsq, double
 }
 __device__ __forceinline__ bool too_large3(const C3Params& P, double rsq, double s1ps2) {
     return rsq >= P.maxsepsq && rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
-}
-
-__device__ __forceinline__ void cmul3(double ar, double ai, double br, double bi, double& cr,
-                                      double& ci) {
-    cr = ar * br - ai * bi;
-    ci = ar * bi + ai * br;
 }
 
 // ProjectHelper<Sphere>::calculate_direction (include/ProjectHelper.h:272-345)
@@ -276,21 +243,6 @@ struct MPTask {
     double maxr;
     int slot, spare, fresh, pad;
 };
-// calculateZeta(c1, mp, ks, mk) with mp in slot `slot` (c1 < 0: no job)
-struct MPJob { int c1, slot, ks, mk; };
-// rows [lo, hi) of a chunk of zeta jobs
-struct RowRange { int lo, hi; };
-
-// Error bits reported through `err` (all of them end the kernel cleanly).
-enum { E_POOL = 1, E_INDEX = 2, E_SLOT = 4 };
-
-// Scratch slots in chunks of 2^SLOT_SHIFT; the chunk table is a device array.
-constexpr int SLOT_SHIFT = 12;
-constexpr int SLOT_MASK = (1 << SLOT_SHIFT) - 1;
-__device__ __forceinline__ double* slot_at(double* const* chunks, int s, int total) {
-    return chunks[s >> SLOT_SHIFT] + (size_t)(s & SLOT_MASK) * (size_t)total;
-}
-
 __device__ __forceinline__ bool has_children(const DNode& c, int nnodes) {
     return c.child >= 0 && c.child < nnodes - 1;
 }
@@ -781,8 +733,6 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
 //   GGG [2,9]  P0..P3: zeta0..3 at (k2, k3, +m); [10,17] Q0..Q3: zeta0..3 at (k3, k2, +m)
 //   last 5     ntri, meand2, meanlogd2, meand3, meanlogd3 at (k2, k3, n = 0) (m = 0 only)
 // The other entries of the reference's arrays follow from these (k_mp_expand).
-__host__ __device__ constexpr int zeta_nv(int D) { return D == GData ? 23 : 7; }
-
 template <int D>
 __device__ __forceinline__ void zeta_cell(const ScratchLayout& Lo, const double* __restrict__ sc,
                                           const DNode& c1, int k2, int k3, int m,
@@ -905,12 +855,13 @@ template <int D>
 __global__ __launch_bounds__(ZT_THREADS) void k_mp_zeta(
     const MPJob* __restrict__ jobs, int njobs, const DNode* __restrict__ nodes,
     double* const* __restrict__ slots, C3Params P, int mr, int jpc, double* __restrict__ partial,
-    RowRange* __restrict__ chunk_rows) {
+    RowRange* __restrict__ chunk_rows, int nq, int nsub, int diag_only) {
     HIP_DYNAMIC_SHARED(double, sc)
     __shared__ int s_kmin[ZT_THREADS / 64], s_kmax[ZT_THREADS / 64];
     constexpr int NV = zeta_nv(D);
-    const int q = blockIdx.x;
-    const int m0 = blockIdx.y * mr;
+    int q, sub;
+    if (!xcd_chunk(nq, nsub, q, sub)) return;
+    const int m0 = sub * mr;
     const int nb = P.nbins, M = P.maxn;
     ScratchLayout Lo;
     Lo.init(nb, M, D);
@@ -937,7 +888,7 @@ __global__ __launch_bounds__(ZT_THREADS) void k_mp_zeta(
         kmin = s_kmin[w] < kmin ? s_kmin[w] : kmin;
         kmax = s_kmax[w] > kmax ? s_kmax[w] : kmax;
     }
-    if (threadIdx.x == 0 && blockIdx.y == 0) chunk_rows[q] = RowRange{kmin, kmax};
+    if (threadIdx.x == 0 && sub == 0) chunk_rows[q] = RowRange{kmin, kmax};
     if (kmin >= kmax) return;            // no rows in this chunk (uniform)
     // this thread's cells: c = threadIdx.x + u * ZT_THREADS, m fastest, then k3, then k2
     int ck2[ZT_IPT], ck3[ZT_IPT], cm[ZT_IPT];
@@ -951,6 +902,9 @@ __global__ __launch_bounds__(ZT_THREADS) void k_mp_zeta(
         ck2[u] = k2;
         ck3[u] = k2 + pr;
         cm[u] = (k2 < kmax && m0 + ml <= M) ? m0 + ml : -1;
+        // with the contraction on the matrix cores, this kernel keeps only the n = 0 sums
+        // (ntri, meand2, ...), which stay FP64 (mode 2: launched for m = 0 alone)
+        if (diag_only && cm[u] > 0) cm[u] = -1;
     }
     double a0[NV], a1[NV];
     static_assert(ZT_IPT == 2, "two cells per thread");
@@ -994,14 +948,16 @@ __global__ __launch_bounds__(ZT_THREADS) void k_mp_zeta(
     if (cm[0] >= 0) {
         double* dst = partial + (size_t)q * ncomp_chunk +
                       (((size_t)ck2[0] * nb + ck3[0]) * (M + 1) + cm[0]) * NV;
+        const int v0 = diag_only ? NV - 5 : 0;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) dst[v] = a0[v];
+        for (int v = 0; v < NV; ++v) if (v >= v0) dst[v] = a0[v];
     }
     if (cm[1] >= 0) {
         double* dst = partial + (size_t)q * ncomp_chunk +
                       (((size_t)ck2[1] * nb + ck3[1]) * (M + 1) + cm[1]) * NV;
+        const int v0 = diag_only ? NV - 5 : 0;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) dst[v] = a1[v];
+        for (int v = 0; v < NV; ++v) if (v >= v0) dst[v] = a1[v];
     }
 }
 
@@ -1144,7 +1100,7 @@ using StageFn = void (*)(const DNode*, int, const NodeSum*, const MPTask*, const
                          double* const*, int, C3Params, MPTask*, MPJob*, int*,
                          unsigned long long*, int*);
 using ZetaFn = void (*)(const MPJob*, int, const DNode*, double* const*, C3Params, int, int, double*,
-                        RowRange*);
+                        RowRange*, int, int, int);
 using ExpandFn = void (*)(const double*, const double*, C3Params, double*);
 
 static SplitFn pick_split(int C) {
@@ -1198,6 +1154,14 @@ struct Compactor {
     }
 };
 
+static std::atomic<int> g_zeta_mode{0};
+int set_zeta_mode(int mode) {
+    if (mode < 0) return g_zeta_mode.load();
+    if (mode > 2) return fail("zeta mode must be 0 (FP64), 1 (FP32 MFMA) or 2 (split-BF16 MFMA)");
+    g_zeta_mode.store(mode);
+    return 0;
+}
+
 static std::atomic<int> g_mp_plain{0};
 int set_multipole_plain(int on) {
     if (on < 0) return g_mp_plain.load();
@@ -1234,6 +1198,13 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
     const size_t zt_bytes = shbytes;    // the slot's layout, partly filled per job
     TC_HIP(hipFuncSetAttribute((const void*)zfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)zt_bytes));
+    // the off-diagonal zeta cells on the matrix cores (at most 32 bins)
+    const int zmode = nb <= 32 ? g_zeta_mode.load() : 0;
+    int mf_threads = 0;
+    size_t mf_bytes = 0;
+    ZetaMfmaFn mfn = pick_zeta_mfma(D, zmode, nb, mf_threads, mf_bytes);
+    if (mfn) TC_HIP(hipFuncSetAttribute((const void*)mfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)mf_bytes));
     KernelStat& ks_stage = kernel_stat(KS_MULTIPOLE);
     KernelStat& ks_zeta = kernel_stat(KS_ZETA);
     const bool timing = timing_enabled();
@@ -1401,10 +1372,20 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
             if (partial.reserve((size_t)nchunks * ncomp, 0, st)) return 2;
             TC_HIP(hipMemsetAsync(partial.p, 0, sizeof(double) * partial.n, st));
         }
-        hipLaunchKernelGGL(zfn, dim3(nchunks, nmr), dim3(ZT_THREADS), zt_bytes, st, jobs.p,
-                           (int)ntask, nodes, chunk_tab.p, P, zmr, (int)jpc, partial.p,
-                           chunk_rows.p);
-        TC_HIP(hipGetLastError());
+        {
+            // FP64: every cell; with the matrix cores, only the n = 0 sums, for m = 0
+            const int zr_mr = mfn ? 1 : zmr, zr_sub = mfn ? 1 : nmr;
+            const int qpad = (nchunks + 7) / 8 * 8;
+            hipLaunchKernelGGL(zfn, dim3((unsigned)(qpad * zr_sub)), dim3(ZT_THREADS), zt_bytes, st,
+                               jobs.p, (int)ntask, nodes, chunk_tab.p, P, zr_mr, (int)jpc, partial.p,
+                               chunk_rows.p, nchunks, zr_sub, mfn ? 1 : 0);
+            TC_HIP(hipGetLastError());
+            if (mfn)
+                hipLaunchKernelGGL(mfn, dim3((unsigned)(qpad * (M + 1))), dim3(mf_threads), mf_bytes,
+                                   st, jobs.p, (int)ntask, nodes, chunk_tab.p, P, (int)jpc,
+                                   partial.p, nchunks);
+            TC_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_mp_zeta_reduce, dim3((unsigned)((ncomp + 255) / 256)), dim3(256), 0, st,
                            partial.p, chunk_rows.p, nchunks, ncomp, (long long)nb * (M + 1) * NV,
                            nb, zhi.p, zlo.p);

@@ -98,5 +98,7 @@ struct C3Params {
 int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int ntop,
               const C3Params& P, double* out, unsigned long long* cnt);
 int set_multipole_plain(int on);   // < 0: returns the current setting
+// zeta contraction: 0 FP64 (default), 1 FP32 MFMA, 2 split-BF16 MFMA; < 0 returns it
+int set_zeta_mode(int mode);
 
 }  // namespace tc
