@@ -692,6 +692,8 @@ def main():
     ap.add_argument('--mp-n', type=int, default=500_000,
                     help='points of the multipole (GGG, NNN) leg of the headline line')
     ap.add_argument('--no-mp', action='store_true', help='leave out the multipole leg')
+    ap.add_argument('--only-mp', action='store_true',
+                    help='run only the multipole leg (its own JSON line, for profiling)')
     args = ap.parse_args()
     if args.calibrate_copy:
         from treecorr_amd import _lib
@@ -712,6 +714,9 @@ def main():
     from treecorr_amd import _lib
     L = _lib.lib()
     hip = Hip()
+    if args.only_mp:
+        print(json.dumps({"multipole": run_mp_leg(args, L, hip)}))
+        return
     ra, dec, g1, g2, w = gg_sphere_catalog(args.n)
     x, y, z = radec_to_xyz(ra, dec)
     bn = gg_binning()

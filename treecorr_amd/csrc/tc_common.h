@@ -153,6 +153,114 @@ __host__ __device__ inline double wrap_period(double d, double p) {
     return d;
 }
 
+// ---------------------------------------------------------------------------
+// Bin-edge guard: the correctly rounded log for decisions close to an edge
+// ---------------------------------------------------------------------------
+// A pair whose bin coordinate kk = (log r - log minsep) / binsize lies within a few ulp
+// of an integer, or whose singleBin refined test (include/BinType.h:128-147) lies within
+// a few ulp of its threshold, lands differently for two logs that differ in the last
+// bit.  The reference takes std::log from the host libm, which rounds correctly for all
+// but about 1.5e-4 of its arguments; the device logs (the fdlibm-method tc_log, the
+// device library's log) are within one ulp.  Such pairs therefore take their log again
+// from log_cr: the approximation y0 corrected by one Newton step on exp(y) = x,
+// log x = y0 + (x - e^y0) / e^y0 + O(2^-104 y0^2), with e^y0 in double-double
+// arithmetic (reduction by ln 2 held in three parts, then a 24-term Taylor series on
+// |r| <= ln2/2), so the sum carries about 100 bits before its final rounding.
+struct DD { double hi, lo; };
+__device__ __forceinline__ DD dd_fast(double a, double b) {
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ __forceinline__ DD dd_two_sum(double a, double b) {
+    const double s = a + b, bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DD dd_prod(double a, double b) {
+    const double p = a * b;
+    return {p, fma(a, b, -p)};
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    DD s = dd_two_sum(a.hi, b.hi);
+    const DD t = dd_two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = dd_fast(s.hi, s.lo);
+    s.lo += t.lo;
+    return dd_fast(s.hi, s.lo);
+}
+__device__ __forceinline__ DD dd_mul(DD a, DD b) {
+    DD p = dd_prod(a.hi, b.hi);
+    p.lo += a.hi * b.lo + a.lo * b.hi;
+    return dd_fast(p.hi, p.lo);
+}
+__device__ __forceinline__ DD dd_div(DD a, double b) {
+    const double q = a.hi / b;
+    const DD p = dd_prod(q, b);
+    return dd_fast(q, ((a.hi - p.hi) - p.lo + a.lo) / b);
+}
+// e^y in double-double for |y| < 700
+__device__ inline DD exp_dd(double y) {
+    const double n = rint(y * 0x1.71547652b82fep+0);
+    const DD a = dd_prod(n, 0x1.62e42fefa39efp-1), b = dd_prod(n, 0x1.abc9e3b39803fp-56);
+    DD r = dd_add(DD{y, 0.}, DD{-a.hi, -a.lo});
+    r = dd_add(r, DD{-b.hi, -b.lo});
+    r = dd_add(r, DD{-n * 0x1.7b57a079a1934p-111, 0.});
+    DD s{1., 0.}, t{1., 0.};
+    for (int i = 1; i <= 24; ++i) {
+        t = dd_div(dd_mul(t, r), double(i));
+        s = dd_add(s, t);
+    }
+    const int e = int(n);
+    return {ldexp(s.hi, e), ldexp(s.lo, e)};
+}
+// log x rounded correctly (to within 2^-50 ulp of a rounding midpoint), from y0, any
+// approximation of log x within a few ulp
+__device__ inline double log_cr(double x, double y0) {
+    const DD e = exp_dd(y0);
+    const double d = x - e.hi;                    // exact: x and e.hi are within 2x
+    return y0 + (d - e.lo) / e.hi;
+}
+// The change of kk that a two-ulp change of logr can make, with the rounding of the
+// subtraction and the division on top, and a factor of two to spare.
+__device__ __forceinline__ double kk_slack(double logr, double logminsep, double binsize,
+                                           double kk) {
+    return 2e-15 * ((fabs(logr) + fabs(logminsep)) / binsize + fabs(kk) + 1.);
+}
+__device__ __forceinline__ bool near_int(double kk, double slack) {
+    return fabs(kk - rint(kk)) <= slack;
+}
+// directProcess11's bin index int((log x - logminsep) / binsize) from lx, a device log of
+// x, with the guard; logr receives the log that decided it
+__device__ __forceinline__ int guarded_bin(double x, double lx, double logminsep,
+                                           double binsize, double& logr) {
+    double kk = (lx - logminsep) / binsize;
+    if (near_int(kk, kk_slack(lx, logminsep, binsize, kk))) {
+        lx = log_cr(x, lx);
+        kk = (lx - logminsep) / binsize;
+    }
+    logr = lx;
+    return int(kk);
+}
+// singleBin's refined test at bin coordinate kk (include/BinType.h:128-147), the
+// reference's expressions unchanged: true when the pair fits one bin.  near is set when
+// a comparison is within what a change of kk by dkk could turn over.
+__device__ __forceinline__ bool refined_fits(double kk, double s1ps2sq, double rsq,
+                                             double binsize, double b, double dkk,
+                                             bool& near) {
+    const int ik = int(kk);
+    const double frackk = kk - ik;
+    const double omf = 1. - frackk;
+    const double f = (omf < frackk) ? omf : frackk;
+    const double dt = dkk * binsize;
+    const double t1 = f * binsize + b;
+    const double m1 = (t1 * t1) * rsq;
+    near = near || fabs(s1ps2sq - m1) <= (2. * fabs(t1) + dt) * dt * rsq + 1e-15 * m1;
+    if (s1ps2sq > m1) return false;
+    const double t2 = frackk * binsize + b - s1ps2sq / rsq;
+    const double m2 = (t2 * t2) * rsq;
+    near = near || fabs(s1ps2sq - m2) <= (2. * fabs(t2) + dt) * dt * rsq + 1e-15 * m2;
+    return !(s1ps2sq > m2);
+}
+
 // The metrics of a 2-point run, values of the reference's enum (include/Metric.h:29).
 enum Metric { MetEuclidean = 1, MetRperp = 2, MetRlens = 3, MetArc = 4, MetOldRperp = 5,
               MetPeriodic = 6 };

@@ -59,9 +59,10 @@ __device__ __forceinline__ double dist_sq(const DNode& a, const DNode& b) {
 // Natural logarithm for the bin index and meanlogr: the published fdlibm method
 // (Sun's e_log.c): x = 2^k (1+f) with 1+f in [sqrt(2)/2, sqrt(2)), s = f/(2+f),
 // log(1+f) = f - f^2/2 + s (f^2/2 + R(s^2)) with R the degree-7 minimax polynomial of
-// that method, and k ln2 split in two parts.  Error below 1 ulp, like the host log
-// the reference calls; about a third of the instructions of the device library's
-// double-double log.  Only called with finite x > 0 (squared distances >= minsep^2).
+// that method, and k ln2 split in two parts.  Error below 1 ulp; about a third of the
+// instructions of the device library's double-double log.  A bin decision that a
+// last-bit difference from the host libm's log could turn over is taken again with
+// log_cr.  Only called with finite x > 0 (squared distances >= minsep^2).
 __device__ __forceinline__ double tc_log(double x) {
 #if TC_FAST_LOG
     int e;
@@ -153,11 +154,6 @@ __device__ __forceinline__ void bin_pair(const DNode& c1, const DNode& c2, doubl
                                          double r, double logr, const C2Params& P,
                                          double* hist, unsigned long long* hnp, int R) {
     const int nb = P.nbins;
-    if (k < 0) {
-        r = sqrt(rsq);
-        logr = tc_log(r);
-        k = int((logr - P.logminsep) / P.binsize);
-    }
     if (k == nb) --k;
     double ww = double(c1.w) * double(c2.w);
     atomicAdd(&hist[(Q_WEIGHT * nb + k) * R], ww);
@@ -481,25 +477,31 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 const bool inrange = rsq >= P.minsepsq && rsq < P.maxsepsq;
                 // step 2: r, log and the bin coordinate kk for the lanes that need them
                 double r = 0., logr = 0., kk = 0.;
+                int k = 0;
                 if (cls == CL_REFINE || (cls == CL_SINGLE && inrange)) {
                     r = sqrt(rsq);
-                    const double L = tc_log(cls == CL_REFINE ? rsq : r);
+                    const double x = cls == CL_REFINE ? rsq : r;
+                    double L = tc_log(x);
                     logr = cls == CL_REFINE ? 0.5 * L : L;
                     kk = (logr - P.logminsep) / P.binsize;
-                }
-                // step 3: the refined test, then accept or split
-                int k = int(kk);
-                if (cls == CL_REFINE) {
-                    ++c_ref;
-                    const double frackk = kk - k;
-                    const double omf = 1. - frackk;
-                    const double f = (omf < frackk) ? omf : frackk;
-                    const double t1 = f * P.binsize + P.b;
-                    cls = CL_SINGLE;
-                    if (s1ps2sq > (t1 * t1) * rsq) cls = CL_SPLIT;
-                    else {
-                        const double t2 = frackk * P.binsize + P.b - s1ps2sq / rsq;
-                        if (s1ps2sq > (t2 * t2) * rsq) cls = CL_SPLIT;
+                    // step 3: the refined test, then accept or split.  A decision within
+                    // a few ulp of a bin edge or of the refined test's threshold is taken
+                    // again with the correctly rounded log, as the host libm rounds it.
+                    const double sl = kk_slack(logr, P.logminsep, P.binsize, kk);
+                    bool near = near_int(kk, sl), fits = true;
+                    if (cls == CL_REFINE)
+                        fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, sl, near);
+                    if (near) {
+                        L = log_cr(x, L);
+                        logr = cls == CL_REFINE ? 0.5 * L : L;
+                        kk = (logr - P.logminsep) / P.binsize;
+                        if (cls == CL_REFINE)
+                            fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, 0., near);
+                    }
+                    k = int(kk);
+                    if (cls == CL_REFINE) {
+                        ++c_ref;
+                        cls = fits ? CL_SINGLE : CL_SPLIT;
                     }
                 }
                 if (cls == CL_SINGLE && inrange) {

@@ -57,16 +57,22 @@ __device__ __forceinline__ bool single_bin3(const C3Params& P, double rsq, doubl
     if (s1ps2sq > P.asq * rsq) return false;
     if (s1ps2sq <= P.bsq * rsq) return true;
     if (s1ps2sq > 0.25 * ((P.binsize + P.b) * (P.binsize + P.b)) * rsq) return false;
-    logr = 0.5 * log(rsq);
-    const double kk = (logr - P.logminsep) / P.binsize;
+    // a decision within a few ulp of a bin edge or of the refined test's threshold is
+    // taken again with the correctly rounded log, as the host libm rounds it
+    double L = log(rsq);
+    logr = 0.5 * L;
+    double kk = (logr - P.logminsep) / P.binsize;
+    const double sl = kk_slack(logr, P.logminsep, P.binsize, kk);
+    bool near = near_int(kk, sl);
+    bool fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, sl, near);
+    if (near) {
+        L = log_cr(rsq, L);
+        logr = 0.5 * L;
+        kk = (logr - P.logminsep) / P.binsize;
+        fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, 0., near);
+    }
     ik = int(kk);
-    const double frackk = kk - ik;
-    const double omf = 1. - frackk;
-    const double f = (omf < frackk) ? omf : frackk;
-    const double t1 = f * P.binsize + P.b;
-    if (s1ps2sq > (t1 * t1) * rsq) return false;
-    const double t2 = frackk * P.binsize + P.b - s1ps2sq / rsq;
-    if (s1ps2sq > (t2 * t2) * rsq) return false;
+    if (!fits) return false;
     r = sqrt(rsq);
     return true;
 }
@@ -327,10 +333,7 @@ __device__ __forceinline__ int finish_class(const C3Params& P, const DNode& c1, 
     logr = 0.;
     if (single_bin3(P, rsq, s1ps2, k, r, logr)) {
         if (!(rsq >= P.minsepsq && rsq < P.maxsepsq)) return 0;
-        if (k < 0) {
-            logr = log(r);
-            k = int((logr - P.logminsep) / P.binsize);
-        }
+        if (k < 0) k = guarded_bin(r, log(r), P.logminsep, P.binsize, logr);
         return 1;
     }
     rtot = r + s1ps2;
@@ -678,7 +681,9 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
     int nmink = tk.mink;
     if (nlen > 0) {
         if (nmaxr < P.maxsep) {
-            const int k = nmaxr < P.minsep ? 0 : int((log(nmaxr) - P.logminsep) / P.binsize) + 1;
+            double lm = 0.;
+            const int k = nmaxr < P.minsep ? 0
+                        : guarded_bin(nmaxr, log(nmaxr), P.logminsep, P.binsize, lm) + 1;
             if (k >= 0 && k < tk.mink) { job = MPJob{tk.c1, tk.slot, k, tk.mink}; nmink = k; }
         }
     } else {
