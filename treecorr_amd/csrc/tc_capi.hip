@@ -392,6 +392,7 @@ int tc_corr2_create(int d1, int d2, int bin_type, double minsep, double maxsep, 
     P.logminsep = std::log(minsep); P.halfminsep = 0.5 * minsep;
     P.minsepsq = minsep * minsep; P.maxsepsq = maxsep * maxsep;
     P.bsq = b * b; P.asq = a * a;
+    P.inv_binsize = 1. / binsize;
     P.metric = MetEuclidean; P.xp = xp; P.yp = yp; P.zp = zp;   // periods: Periodic metric
     // BaseCorr2::nontrivialRPar, include/Corr2.h:58-62
     P.minrpar = minrpar; P.maxrpar = maxrpar;

@@ -220,10 +220,13 @@ __device__ inline double log_cr(double x, double y0) {
     return y0 + (d - e.lo) / e.hi;
 }
 // The change of kk that a two-ulp change of logr can make, with the rounding of the
-// subtraction and the division on top, and a factor of two to spare.
-__device__ __forceinline__ double kk_slack(double logr, double logminsep, double binsize,
+// subtraction and of the division (or of the product with a rounded 1/binsize) on top,
+// an absolute 4e-16 of logr for 0.5 log(rsq) in place of log(sqrt(rsq)) (the rounding of
+// the sqrt moves its log by at most 2^-53), and a factor of two to spare.
+__device__ __forceinline__ double kk_slack(double logr, double logminsep, double inv_binsize,
                                            double kk) {
-    return 2e-15 * ((fabs(logr) + fabs(logminsep)) / binsize + fabs(kk) + 1.);
+    return (2e-15 * (fabs(logr) + fabs(logminsep)) + 4e-16) * inv_binsize +
+           2e-15 * (fabs(kk) + 1.);
 }
 __device__ __forceinline__ bool near_int(double kk, double slack) {
     return fabs(kk - rint(kk)) <= slack;
@@ -233,7 +236,7 @@ __device__ __forceinline__ bool near_int(double kk, double slack) {
 __device__ __forceinline__ int guarded_bin(double x, double lx, double logminsep,
                                            double binsize, double& logr) {
     double kk = (lx - logminsep) / binsize;
-    if (near_int(kk, kk_slack(lx, logminsep, binsize, kk))) {
+    if (near_int(kk, kk_slack(lx, logminsep, 1. / binsize, kk))) {
         lx = log_cr(x, lx);
         kk = (lx - logminsep) / binsize;
     }

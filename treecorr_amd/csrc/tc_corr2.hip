@@ -103,6 +103,20 @@ __device__ __forceinline__ void sphere_direction(double x1, double y1, double z1
     im = -cosA;
 }
 
+// sqrt(x) for meanr (finite x > 0, not denormal): the hardware reciprocal square root
+// estimate, two Goldschmidt steps and a final Newton correction, within an ulp of the
+// correctly rounded root.  No decision depends on it.
+__device__ __forceinline__ double sqrt_fast(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double e = fma(-g, h, 0.5);
+    g = fma(g, e, g); h = fma(h, e, h);
+    e = fma(-g, h, 0.5);
+    g = fma(g, e, g); h = fma(h, e, h);
+    const double d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
 // 1/x for the projection factors (finite x > 0): hardware reciprocal estimate + two
 // Newton steps, within an ulp of the division.
 __device__ __forceinline__ double recip(double x) {
@@ -413,19 +427,20 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
         }
         return V.tail[idx - V.nhead];
     };
-    // The next item is loaded one round ahead, so its memory latency overlaps this
-    // round's work.
-    IT next{};
-    if (b0 + threadIdx.x < b1) next = fetch(b0 + threadIdx.x);
-    for (long long base = b0; base < b1; base += blockDim.x) {
-        const long long idx = base + threadIdx.x;
+    // Software pipeline over two register sets: while a round works on one item and
+    // its nodes, the nodes of the next round's item and the item two rounds ahead are
+    // loaded into the other set, so their memory latency overlaps this round's work.
+    // The loop is unrolled by two so that the sets alternate without register copies.
+    // (A process2 item, j < 0, loads node 0 of the second tree in place of its unused
+    // second node; items past the end load nothing.)
+    const long long bd = blockDim.x;
+    auto do_round = [&](const IT it, const DNode& c1, const DNode& c2, long long idx) {
         IT o0{}, o1{}, o2{}, o3{};
         int no = 0;
-        const IT it = next;
         if (idx < b1) {
             if (it.j < 0) {
                 // BaseCorr2::process2, src/Corr2.cpp:290-306
-                const DNode c = n1[it.i];
+                const DNode& c = c1;
                 if (c.w != 0.f && double(c.size) > P.halfminsep) {
                     const int L = c.child;
                     o0 = Ops::make(L, -1, it); o1 = Ops::make(L + 1, -1, it);
@@ -439,8 +454,6 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 // expressions, unchanged: 0.5*log(rsq) for singleBin's refined test,
                 // log(sqrt(rsq)) for directProcess11).
                 ++c_vis;
-                const DNode c1 = n1[it.i];
-                const DNode c2 = n2[it.j];
                 // step 1: pruning and singleBin's cheap tests (include/BinType.h:93-151)
                 int cls = CL_NONE;
                 double rsq = 0., s1ps2 = 0., s1ps2sq = 0.;
@@ -475,25 +488,28 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                     }
                 }
                 const bool inrange = rsq >= P.minsepsq && rsq < P.maxsepsq;
-                // step 2: r, log and the bin coordinate kk for the lanes that need them
+                // step 2: the log and the bin coordinate kk for the lanes that need them.
+                // Both classes take 0.5 log(rsq) times the rounded 1/binsize; away from an
+                // edge and from the refined test's thresholds that decides as the
+                // reference's expressions do (singleBin's 0.5*log(rsq), directProcess11's
+                // log(sqrt(rsq)), each divided by binsize).
                 double r = 0., logr = 0., kk = 0.;
                 int k = 0;
                 if (cls == CL_REFINE || (cls == CL_SINGLE && inrange)) {
-                    r = sqrt(rsq);
-                    const double x = cls == CL_REFINE ? rsq : r;
-                    double L = tc_log(x);
-                    logr = cls == CL_REFINE ? 0.5 * L : L;
-                    kk = (logr - P.logminsep) / P.binsize;
+                    const double L = tc_log(rsq);
+                    logr = 0.5 * L;
+                    kk = (logr - P.logminsep) * P.inv_binsize;
                     // step 3: the refined test, then accept or split.  A decision within
                     // a few ulp of a bin edge or of the refined test's threshold is taken
-                    // again with the correctly rounded log, as the host libm rounds it.
-                    const double sl = kk_slack(logr, P.logminsep, P.binsize, kk);
+                    // again by the reference's own expression with the correctly rounded
+                    // log, as the host libm rounds it, and the division.
+                    const double sl = kk_slack(logr, P.logminsep, P.inv_binsize, kk);
                     bool near = near_int(kk, sl), fits = true;
                     if (cls == CL_REFINE)
                         fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, sl, near);
                     if (near) {
-                        L = log_cr(x, L);
-                        logr = cls == CL_REFINE ? 0.5 * L : L;
+                        logr = cls == CL_REFINE ? 0.5 * log_cr(rsq, L)
+                                                : log_cr(sqrt(rsq), logr);
                         kk = (logr - P.logminsep) / P.binsize;
                         if (cls == CL_REFINE)
                             fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, 0., near);
@@ -505,6 +521,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                     }
                 }
                 if (cls == CL_SINGLE && inrange) {
+                    r = sqrt_fast(rsq);
                     ++c_acc;
                     if (c1.child < 0 && c2.child < 0) ++c_leaf;
                     if (!Ops::batched) {
@@ -548,7 +565,6 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
                 }
             }
         }
-        if (idx + (long long)blockDim.x < b1) next = fetch(idx + (long long)blockDim.x);
         // Wavefront compaction of the children: no <= 4 is three bits, so the lane's
         // exclusive prefix is sum_k 2^k (lanes below with bit k set), from three votes.
         const unsigned long long m0 = __ballot(no & 1), m1 = __ballot(no & 2),
@@ -568,6 +584,35 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
             if (no > 2) bout[pos + 2] = o2;
             if (no > 3) bout[pos + 3] = o3;
         }
+    };
+    IT itA{}, itB{};
+    DNode a1{}, a2{}, q1{}, q2{};
+    const long long t0 = b0 + threadIdx.x;
+    if (t0 < b1) {
+        itA = fetch(t0);
+        a1 = n1[itA.i];
+        a2 = n2[itA.j >= 0 ? itA.j : 0];
+    }
+    if (t0 + bd < b1) itB = fetch(t0 + bd);
+    for (long long base = b0; base < b1; base += 2 * bd) {
+        const long long idx = base + threadIdx.x;
+        // round on set A; loads for set B (the item of the next round is itB)
+        const IT curA = itA;
+        if (idx + bd < b1) {
+            q1 = n1[itB.i];
+            q2 = n2[itB.j >= 0 ? itB.j : 0];
+        }
+        if (idx + 2 * bd < b1) itA = fetch(idx + 2 * bd);
+        do_round(curA, a1, a2, idx);
+        if (base + bd >= b1) break;     // uniform over the workgroup
+        // round on set B; loads for set A
+        const IT curB = itB;
+        if (idx + 2 * bd < b1) {
+            a1 = n1[itA.i];
+            a2 = n2[itA.j >= 0 ? itA.j : 0];
+        }
+        if (idx + 3 * bd < b1) itB = fetch(idx + 3 * bd);
+        do_round(curB, q1, q2, idx + bd);
     }
     atomicAdd(&hcnt[CNT_VISITS], (unsigned long long)c_vis);
     atomicAdd(&hcnt[CNT_ACCEPTED], (unsigned long long)c_acc);

@@ -62,7 +62,7 @@ __device__ __forceinline__ bool single_bin3(const C3Params& P, double rsq, doubl
     double L = log(rsq);
     logr = 0.5 * L;
     double kk = (logr - P.logminsep) / P.binsize;
-    const double sl = kk_slack(logr, P.logminsep, P.binsize, kk);
+    const double sl = kk_slack(logr, P.logminsep, 1. / P.binsize, kk);
     bool near = near_int(kk, sl);
     bool fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, sl, near);
     if (near) {

@@ -48,6 +48,7 @@ struct C2Params {
     int nbins, quick;
     double minsep, maxsep, binsize, b, a;
     double logminsep, halfminsep, minsepsq, maxsepsq, bsq, asq;
+    double inv_binsize;         // 1/binsize rounded, for bin coordinates away from an edge
     int metric;                 // Metric: Euclidean, Arc or Periodic
     double xp, yp, zp;          // the Periodic metric's periods
     int rpar_on;                // a [minrpar, maxrpar] range was given (3-d)
