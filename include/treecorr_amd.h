@@ -181,6 +181,24 @@ int tc_corr2_counters(const tc_corr2* c, long long* counters4);
  * the default).  Results do not depend on it; tests use small values to exercise
  * the multi-launch work stack.  No reference counterpart. */
 int tc_set_traversal_chunk(long long items);
+/* How the 2-point traversal runs.  mode 1 (default): one persistent launch per
+ * correlation; each workgroup keeps its own LIFO stack of node pairs and takes jobs
+ * (initial node pairs, src/Corr2.cpp:158-203) by a shared counter, as the reference's
+ * OpenMP schedule(dynamic) does.  2: the same launch with a static, equal-count
+ * partition of the jobs.  0: chunked launches driven from the host (one read-back per
+ * launch).  Results do not depend on the mode (npairs identical, sums to rounding).
+ * Returns 0, or with mode < 0 the current mode.  No reference counterpart. */
+int tc_set_traversal_mode(int mode);
+/* Persistent traversals redone with chunked launches because a workgroup's stack would
+ * have overflowed (expected 0).  No reference counterpart. */
+long long tc_traversal_fallbacks(void);
+/* Parameters of the persistent traversal.  levels: stack room of each workgroup, in
+ * levels of 4 x 256 node pairs (0, the default: the two trees' depths + 4).  min_jobs:
+ * fewer initial node pairs than this are first split by chunked launches until there
+ * are at least this many jobs (0, the default: 2^26).  Tests set
+ * small values to exercise the fallback and the persistent launch on small inputs.
+ * No reference counterpart. */
+int tc_set_persist_params(int levels, long long min_jobs);
 /* Speculative top tree levels.  The partitions near the root proceed from centroids
  * summed in parallel while the reference-order sums run on a second stream; those
  * then write the node records and check every decision and every object's partition

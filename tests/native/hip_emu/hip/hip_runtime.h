@@ -362,6 +362,7 @@ inline void __builtin_amdgcn_sched_group_barrier(int, int, int) {}
 // v_rcp_f64 is an estimate that callers refine with Newton steps; the exact quotient
 // is a valid estimate.
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+inline double __builtin_amdgcn_rsq(double x) { return 1.0 / std::sqrt(x); }
 
 inline int __double2loint(double d) { uint64_t u; std::memcpy(&u, &d, 8); return (int)(uint32_t)u; }
 inline int __double2hiint(double d) { uint64_t u; std::memcpy(&u, &d, 8); return (int)(uint32_t)(u >> 32); }
@@ -460,6 +461,17 @@ inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
+// an emulated device of 8 compute units holding 2 workgroups each (EMU_CUS overrides 8)
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 };
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
+    const char* e = std::getenv("EMU_CUS");
+    *v = e ? std::atoi(e) : 8;
+    return hipSuccess;
+}
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) {
+    *n = 2;
+    return hipSuccess;
+}
 
 #define HIP_DYNAMIC_SHARED(type, var) type* var = (type*)emu::dyn_lds();
 #define hipLaunchKernelGGL(K, G, B, S, ST, ...) ((void)(ST), emu::launch(K, dim3(G), dim3(B), (size_t)(S), __VA_ARGS__))

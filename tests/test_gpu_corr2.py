@@ -128,11 +128,14 @@ def test_gpu_small_chunks_same_result(gpu, oracle_lib, chunk):
     O = oracle_lib
     cat, scale = C.make_catalog(C.SPHERE, 8000, 97, 0.3)
     bn = C.binning(scale, 15, 0.005, 0.3, 0.1, 0.1)
-    _lib.check(_lib.lib().tc_set_traversal_chunk(chunk))
+    L = _lib.lib()
+    _lib.check(L.tc_set_traversal_chunk(chunk))
+    _lib.check(L.tc_set_traversal_mode(0))
     try:
         out = raw_corr2(gpu, 4, 3, cat, None, bn, min_top=5)
     finally:
-        _lib.check(_lib.lib().tc_set_traversal_chunk(0))
+        _lib.check(L.tc_set_traversal_chunk(0))
+        _lib.check(L.tc_set_traversal_mode(1))
     kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=5)
     f = O.Field(4, 3, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw)
     ref = O.corr2(4, 3, f, None, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
@@ -485,3 +488,41 @@ def test_gpu_rperp_front_end(gpu, oracle_lib):
         gpu.NNCorrelation(nbins=10, min_sep=0.05, max_sep=5., metric='Rperp').process(flat)
     with pytest.raises(ValueError):
         gpu.NNCorrelation(nbins=10, min_sep=0.05, max_sep=5., min_rpar=0.).process(flat)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('mode,levels,min_jobs', [(0, 0, 0), (1, 0, 0), (1, 0, 1), (2, 0, 1),
+                                                   (1, 1, 1), (2, 1, 1)])
+@pytest.mark.parametrize('cross', [False, True])
+def test_gpu_traversal_modes_same_result(gpu, oracle_lib, mode, levels, min_jobs, cross):
+    """Chunked launches, the persistent launch with jobs by a shared counter or by a
+    static equal-count partition (from the initial node pairs, min_jobs 1, or after the
+    chunked split into 65536 jobs), and the persistent launch with a stack too small (the
+    fallback to chunked launches): npairs and the visit counters equal the oracle's."""
+    from treecorr_amd import _lib
+    O = oracle_lib
+    L = _lib.lib()
+    cat, scale = C.make_catalog(C.SPHERE, 12000, 98, 0.3)
+    cat2, _ = C.make_catalog(C.SPHERE, 9000, 99, 0.3) if cross else (None, None)
+    bn = C.binning(scale, 15, 0.005, 0.3, 0.1, 0.1)
+    _lib.check(L.tc_set_traversal_mode(mode))
+    _lib.check(L.tc_set_persist_params(levels, min_jobs))
+    fb0 = L.tc_traversal_fallbacks()
+    try:
+        out = raw_corr2(gpu, 4, 3, cat, cat2, bn, min_top=3)
+        assert L.tc_set_traversal_mode(-1) == mode
+    finally:
+        _lib.check(L.tc_set_traversal_mode(1))
+        _lib.check(L.tc_set_persist_params(0, 0))
+    assert L.tc_traversal_fallbacks() - fb0 == (1 if levels and mode else 0)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=3)
+    f = O.Field(4, 3, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw)
+    f2 = None if cat2 is None else O.Field(4, 3, cat2['x'], cat2['y'], cat2['z'], g1=cat2['g1'],
+                                           g2=cat2['g2'], w=cat2['w'], **kw)
+    ref = O.corr2(4, 3, f, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'],
+                  bn['b'], bn['a'], exact=True)
+    np.testing.assert_array_equal(out['npairs'], ref['npairs'])
+    for k in KEYS[:-1]:
+        assert normwise(out[k], ref[k]) < RTOL, k
+    for k in ('visits', 'accepted', 'leafpairs', 'refined'):
+        assert out['counters'][k] == ref['counters'][k], k

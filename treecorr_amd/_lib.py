@@ -47,6 +47,9 @@ SIGNATURES = {
                                     _ip, _i, _i, _i, _dp]),
     'tc_quick_assign': (_i, [_dp, _i, _dp, _dp, _dp, _ll, _ip]),
     'tc_set_tree_speculation': (_i, [_i]),
+    'tc_set_traversal_mode': (_i, [_i]),
+    'tc_traversal_fallbacks': (_ll, []),
+    'tc_set_persist_params': (_i, [_i, _ll]),
     'tc_set_multipole_plain': (_i, [_i]),
     'tc_set_zeta_mode': (_i, [_i]),
     'tc_kmeans_run': (_i, [_dp, _i, _i, _dp, _dp, _dp, _dp, _ll, _i, _d, _d, _i,

@@ -299,6 +299,7 @@ int tc_field_create_multi(int d, int coords, const double* x, const double* y, c
         f->T.nodes = M.nodes;
         f->T.nodes_bytes = M.nodes_bytes;
         f->T.nnodes = M.nnodes;
+        f->T.depth = M.depth;
         f->T.top = std::move(M.top[r]);
         f->T.info = M.info[r];
         f->T.dtop_bytes = sizeof(int) * std::max<size_t>(1, f->T.top.size());
@@ -479,8 +480,9 @@ static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int 
     std::lock_guard<std::mutex> run_lock(g_run_m);
     if (!g_s2) g_s2 = new_corr2_scratch();
     int rc = tc::corr2_run(*g_s2, c->kind, f1->coords, f1->T.nodes, f1->T.dtop, (int)f1->T.top.size(),
-                           f2 ? f2->T.nodes : nullptr, f2 ? f2->T.dtop : nullptr,
-                           f2 ? (int)f2->T.top.size() : 0, P, out.data(), np.data(), cnt.data());
+                           f1->T.depth, f2 ? f2->T.nodes : nullptr, f2 ? f2->T.dtop : nullptr,
+                           f2 ? (int)f2->T.top.size() : 0, f2 ? f2->T.depth : 0, P, out.data(),
+                           np.data(), cnt.data());
     if (rc) return rc;
     const int nxi = corr2_nxi(c->kind);
     for (int q = 0; q < nxi; ++q)
@@ -616,6 +618,14 @@ int tc_corr2_trivially_zero(tc_corr2* c, int metric, int coords, double x1, doub
 }
 
 int tc_set_traversal_chunk(long long items) { return tc::set_traversal_chunk(items); }
+int tc_set_traversal_mode(int mode) {
+    if (mode < 0) return tc::traversal_mode();
+    return tc::set_traversal_mode(mode);
+}
+long long tc_traversal_fallbacks(void) { return tc::persist_fallbacks(); }
+int tc_set_persist_params(int levels, long long min_jobs) {
+    return tc::set_persist_params(levels, min_jobs);
+}
 int tc_set_tree_speculation(int mode) { return tc::set_tree_speculation(mode); }
 int tc_set_multipole_plain(int on) { return tc::set_multipole_plain(on); }
 int tc_set_zeta_mode(int mode) { return tc::set_zeta_mode(mode); }

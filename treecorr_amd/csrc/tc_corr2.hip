@@ -22,6 +22,7 @@
 #include "tc_engine.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -39,6 +40,11 @@ constexpr int TRAV_SLOTS = 2048;          // persistent workgroups = accumulatio
 // items, 4.3 GB for 8-byte items).  2^26 measured 3.5% faster than 2^25 on the 2M GG
 // workload (half the launches, less copying onto the stack); 2^27 gained 1% more.
 constexpr long long TRAV_CHUNK = 1LL << 26;
+// Jobs a persistent launch wants to start from: the initial node pairs are split
+// breadth first by chunked launches until there are this many, so that no job's subtree
+// is more than a small part of one workgroup's share (measured on GG 2M: 2^20 jobs
+// 158 ms, 2^22 78 ms, 2^24 43.4 ms, 2^26 41.2 ms of traversal; chunked launches 42.5).
+constexpr long long PERSIST_MIN_JOBS = 1LL << 26;
 static long long g_chunk = TRAV_CHUNK;        // tc_set_traversal_chunk, for tests and tuning
 
 template <int C>
@@ -361,265 +367,195 @@ __device__ __forceinline__ int pair_slot(int p, int* tags) {
     return -1;
 }
 
-// Workgroup b takes the items [b*per_block, (b+1)*per_block) of the chunk and writes
-// the children of its split pairs to its own region out[b*4*per_block ...] (at most
-// four children per item); bcount[b] gets their number.  Each wavefront claims room
-// for its children with one LDS atomic, so the waves of a workgroup never wait for
-// each other; the order of the children in a region does not matter.  No global
-// atomics: the regions are concatenated afterwards.
-// GM: the other metrics and the r_par range (metric_dist_sq, metric_prune); the Euclidean instantiations
-// keep the plain distance, whose kernels need far fewer registers.
-template <int D, int C, class IT, bool GM>
-__global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
-    const DNode* __restrict__ n1, const DNode* __restrict__ n2, InView<IT> V,
-    long long nin, long long per_block, IT* __restrict__ out,
-    unsigned int* __restrict__ bcount, C2Params P, int R,
-    double* __restrict__ slot_hi, double* __restrict__ slot_lo,
-    unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt,
-    double* __restrict__ pair_hist, unsigned long long* __restrict__ pair_np) {
-    using Ops = ItemOps<IT>;
-    HIP_DYNAMIC_SHARED(double, sh)
-    __shared__ unsigned int s_pos;   // children written by this workgroup so far
-    // region offsets of the in-place head (a launch's output stays below 2^32 items)
-    __shared__ unsigned int s_off[TRAV_SLOTS + 1];
-    __shared__ int s_tag[PB_SLOTS];
-    const int nb = P.nbins;
-    const int nh = NQ2 * nb;
-    // R interleaved copies of the histogram (R a power of two): lanes that land in the
-    // same bin in one instruction hit R different LDS banks instead of one address.
-    // A batched run holds PB_SLOTS such histograms, one per patch pair.
-    const int nslot = Ops::batched ? PB_SLOTS : 1;
-    const size_t per_slot = (size_t)(nh + nb) * R;
-    unsigned long long* hcnt = reinterpret_cast<unsigned long long*>(sh + per_slot * nslot);
-    for (int t = threadIdx.x; t < (int)(per_slot * nslot) + NCNT; t += blockDim.x)
-        reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
-    if (V.nhead > 0)
-        for (int t = threadIdx.x; t <= V.nreg; t += blockDim.x) s_off[t] = (unsigned int)V.off[t];
-    if (threadIdx.x < PB_SLOTS) s_tag[threadIdx.x] = -1;
-    if (threadIdx.x == 0) s_pos = 0;
-    __syncthreads();
+// The per-workgroup state of a traversal round: this lane's copy of the LDS histogram
+// (and of the patch-pair histograms of a batched run), and the visit counters.
+template <class IT>
+struct VisitCtx {
+    double* hist;
+    unsigned long long* hnp;
+    int R;
+    size_t per_slot;
+    int* s_tag;
+    double* pair_hist;
+    unsigned long long* pair_np;
+    unsigned int c_vis, c_acc, c_leaf, c_ref;
+    int lane;
+};
 
-    const int lane = threadIdx.x & 63;
-    double* hist_all = sh;
-    unsigned long long* hnp_all = reinterpret_cast<unsigned long long*>(sh + (size_t)nh * R);
-    double* hist = hist_all + (lane & (R - 1));
-    unsigned long long* hnp = hnp_all + (lane & (R - 1));
-    unsigned int c_vis = 0, c_acc = 0, c_leaf = 0, c_ref = 0;
-    const long long b0 = (long long)blockIdx.x * per_block;
-    const long long b1 = b0 + per_block < nin ? b0 + per_block : nin;
-    IT* __restrict__ bout = out + (size_t)blockIdx.x * 4 * per_block;
-    // region cursor of this thread's next item (items of a thread only move forward)
-    int reg = 0;
-    if (b0 + threadIdx.x < b1 && b0 + threadIdx.x < V.nhead) {
-        const unsigned int g = (unsigned int)(V.first + b0 + threadIdx.x);
-        int lo = 0, hi = V.nreg - 1;      // last region with s_off[r] <= g
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= g) lo = mid; else hi = mid - 1;
-        }
-        reg = lo;
-    }
-    auto fetch = [&](long long idx) {
-        if (idx < V.nhead) {
-            const unsigned int g = (unsigned int)(V.first + idx);
-            while (s_off[reg + 1] <= g) ++reg;
-            return V.in[(long long)reg * V.stride + (long long)(g - s_off[reg])];
-        }
-        return V.tail[idx - V.nhead];
-    };
-    // Software pipeline over two register sets: while a round works on one item and
-    // its nodes, the nodes of the next round's item and the item two rounds ahead are
-    // loaded into the other set, so their memory latency overlaps this round's work.
-    // The loop is unrolled by two so that the sets alternate without register copies.
-    // (A process2 item, j < 0, loads node 0 of the second tree in place of its unused
-    // second node; items past the end load nothing.)
-    const long long bd = blockDim.x;
-    auto do_round = [&](const IT it, const DNode& c1, const DNode& c2, long long idx) {
-        IT o0{}, o1{}, o2{}, o3{};
-        int no = 0;
-        if (idx < b1) {
-            if (it.j < 0) {
-                // BaseCorr2::process2, src/Corr2.cpp:290-306
-                const DNode& c = c1;
-                if (c.w != 0.f && double(c.size) > P.halfminsep) {
-                    const int L = c.child;
-                    o0 = Ops::make(L, -1, it); o1 = Ops::make(L + 1, -1, it);
-                    o2 = Ops::make(L, L + 1, it);
-                    no = 3;
+// One round of the traversal for one lane: the process2 / process11 decision for the
+// lane's item (valid: the lane has one), the accepted pair binned into the LDS
+// histogram, and the children of a split pair written with the rest of the wavefront's
+// children to bout[*s_pos ...] (s_pos an LDS counter the wavefront claims room from).
+template <int D, int C, class IT, bool GM>
+__device__ __forceinline__ void visit_round(VisitCtx<IT>& X, const C2Params& P, const IT it,
+                                            const DNode& c1, const DNode& c2, bool valid,
+                                            IT* __restrict__ bout, unsigned int* s_pos) {
+    using Ops = ItemOps<IT>;
+    IT o0{}, o1{}, o2{}, o3{};
+    int no = 0;
+    if (valid) {
+        if (it.j < 0) {
+            // BaseCorr2::process2, src/Corr2.cpp:290-306
+            const DNode& c = c1;
+            if (c.w != 0.f && double(c.size) > P.halfminsep) {
+                const int L = c.child;
+                o0 = Ops::make(L, -1, it); o1 = Ops::make(L + 1, -1, it);
+                o2 = Ops::make(L, L + 1, it);
+                no = 3;
+            }
+        } else {
+            // BaseCorr2::process11, src/Corr2.cpp:308-389.  The decision is taken in
+            // three steps so that every lane needs at most one sqrt, one log and one
+            // bin-index division, evaluated once per wavefront (the reference's
+            // expressions, unchanged: 0.5*log(rsq) for singleBin's refined test,
+            // log(sqrt(rsq)) for directProcess11).
+            ++X.c_vis;
+            // step 1: pruning and singleBin's cheap tests (include/BinType.h:93-151)
+            int cls = CL_NONE;
+            double rsq = 0., s1ps2 = 0., s1ps2sq = 0.;
+            double s1 = c1.size, s2 = c2.size;    // some metrics rescale them
+            if (c1.w != 0.f && c2.w != 0.f) {
+                rsq = GM ? metric_dist_sq(P.metric, C, c1.x, c1.y, c1.z, c2.x, c2.y,
+                                          c2.z, P.xp, P.yp, P.zp, s1, s2)
+                         : dist_sq<C>(c1, c2);
+                s1ps2 = s1 + s2;
+                bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
+                                 rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
+                bool too_large = rsq >= P.maxsepsq &&
+                                 rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
+                bool rin = true;
+                if (GM && (P.rpar_on || los_metric(P.metric))) {
+                    too_small = metric_prune(P.metric, P.rpar_on, P.minrpar, P.maxrpar,
+                                             c1.x, c1.y, c1.z, c2.x, c2.y, c2.z, rsq,
+                                             s1ps2, too_small, too_large, P.minsep,
+                                             P.minsepsq, P.maxsep, P.maxsepsq, rin);
+                    too_large = false;
                 }
-            } else {
-                // BaseCorr2::process11, src/Corr2.cpp:308-389.  The decision is taken in
-                // three steps so that every lane needs at most one sqrt, one log and one
-                // bin-index division, evaluated once per wavefront (the reference's
-                // expressions, unchanged: 0.5*log(rsq) for singleBin's refined test,
-                // log(sqrt(rsq)) for directProcess11).
-                ++c_vis;
-                // step 1: pruning and singleBin's cheap tests (include/BinType.h:93-151)
-                int cls = CL_NONE;
-                double rsq = 0., s1ps2 = 0., s1ps2sq = 0.;
-                double s1 = c1.size, s2 = c2.size;    // some metrics rescale them
-                if (c1.w != 0.f && c2.w != 0.f) {
-                    rsq = GM ? metric_dist_sq(P.metric, C, c1.x, c1.y, c1.z, c2.x, c2.y,
-                                              c2.z, P.xp, P.yp, P.zp, s1, s2)
-                             : dist_sq<C>(c1, c2);
-                    s1ps2 = s1 + s2;
-                    bool too_small = rsq < P.minsepsq && s1ps2 < P.minsep &&
-                                     rsq < (P.minsep - s1ps2) * (P.minsep - s1ps2);
-                    bool too_large = rsq >= P.maxsepsq &&
-                                     rsq >= (P.maxsep + s1ps2) * (P.maxsep + s1ps2);
-                    bool rin = true;
-                    if (GM && (P.rpar_on || los_metric(P.metric))) {
-                        too_small = metric_prune(P.metric, P.rpar_on, P.minrpar, P.maxrpar,
-                                                 c1.x, c1.y, c1.z, c2.x, c2.y, c2.z, rsq,
-                                                 s1ps2, too_small, too_large, P.minsep,
-                                                 P.minsepsq, P.maxsep, P.maxsepsq, rin);
-                        too_large = false;
-                    }
-                    if (!too_small && !too_large && !rin) {
-                        cls = CL_SPLIT;     // singleBin needs r_par inside the range
-                    } else if (!too_small && !too_large) {
-                        s1ps2sq = s1ps2 * s1ps2;
-                        if (s1ps2 == 0.) cls = CL_SINGLE;
-                        else if (s1ps2sq > P.asq * rsq) cls = CL_SPLIT;
-                        else if (s1ps2sq <= P.bsq * rsq) cls = CL_SINGLE;
-                        else if (s1ps2sq > 0.25 * ((P.binsize + P.b) * (P.binsize + P.b)) * rsq)
-                            cls = CL_SPLIT;
-                        else cls = CL_REFINE;
-                    }
+                if (!too_small && !too_large && !rin) {
+                    cls = CL_SPLIT;     // singleBin needs r_par inside the range
+                } else if (!too_small && !too_large) {
+                    s1ps2sq = s1ps2 * s1ps2;
+                    if (s1ps2 == 0.) cls = CL_SINGLE;
+                    else if (s1ps2sq > P.asq * rsq) cls = CL_SPLIT;
+                    else if (s1ps2sq <= P.bsq * rsq) cls = CL_SINGLE;
+                    else if (s1ps2sq > 0.25 * ((P.binsize + P.b) * (P.binsize + P.b)) * rsq)
+                        cls = CL_SPLIT;
+                    else cls = CL_REFINE;
                 }
-                const bool inrange = rsq >= P.minsepsq && rsq < P.maxsepsq;
-                // step 2: the log and the bin coordinate kk for the lanes that need them.
-                // Both classes take 0.5 log(rsq) times the rounded 1/binsize; away from an
-                // edge and from the refined test's thresholds that decides as the
-                // reference's expressions do (singleBin's 0.5*log(rsq), directProcess11's
-                // log(sqrt(rsq)), each divided by binsize).
-                double r = 0., logr = 0., kk = 0.;
-                int k = 0;
-                if (cls == CL_REFINE || (cls == CL_SINGLE && inrange)) {
-                    const double L = tc_log(rsq);
-                    logr = 0.5 * L;
-                    kk = (logr - P.logminsep) * P.inv_binsize;
-                    // step 3: the refined test, then accept or split.  A decision within
-                    // a few ulp of a bin edge or of the refined test's threshold is taken
-                    // again by the reference's own expression with the correctly rounded
-                    // log, as the host libm rounds it, and the division.
-                    const double sl = kk_slack(logr, P.logminsep, P.inv_binsize, kk);
-                    bool near = near_int(kk, sl), fits = true;
+            }
+            const bool inrange = rsq >= P.minsepsq && rsq < P.maxsepsq;
+            // step 2: the log and the bin coordinate kk for the lanes that need them.
+            // Both classes take 0.5 log(rsq) times the rounded 1/binsize; away from an
+            // edge and from the refined test's thresholds that decides as the
+            // reference's expressions do (singleBin's 0.5*log(rsq), directProcess11's
+            // log(sqrt(rsq)), each divided by binsize).
+            double r = 0., logr = 0., kk = 0.;
+            int k = 0;
+            if (cls == CL_REFINE || (cls == CL_SINGLE && inrange)) {
+                const double L = tc_log(rsq);
+                logr = 0.5 * L;
+                kk = (logr - P.logminsep) * P.inv_binsize;
+                // step 3: the refined test, then accept or split.  A decision within
+                // a few ulp of a bin edge or of the refined test's threshold is taken
+                // again by the reference's own expression with the correctly rounded
+                // log, as the host libm rounds it, and the division.
+                const double sl = kk_slack(logr, P.logminsep, P.inv_binsize, kk);
+                bool near = near_int(kk, sl), fits = true;
+                if (cls == CL_REFINE)
+                    fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, sl, near);
+                if (near) {
+                    logr = cls == CL_REFINE ? 0.5 * log_cr(rsq, L)
+                                            : log_cr(sqrt(rsq), logr);
+                    kk = (logr - P.logminsep) / P.binsize;
                     if (cls == CL_REFINE)
-                        fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, sl, near);
-                    if (near) {
-                        logr = cls == CL_REFINE ? 0.5 * log_cr(rsq, L)
-                                                : log_cr(sqrt(rsq), logr);
-                        kk = (logr - P.logminsep) / P.binsize;
-                        if (cls == CL_REFINE)
-                            fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, 0., near);
-                    }
-                    k = int(kk);
-                    if (cls == CL_REFINE) {
-                        ++c_ref;
-                        cls = fits ? CL_SINGLE : CL_SPLIT;
-                    }
+                        fits = refined_fits(kk, s1ps2sq, rsq, P.binsize, P.b, 0., near);
                 }
-                if (cls == CL_SINGLE && inrange) {
-                    r = sqrt_fast(rsq);
-                    ++c_acc;
-                    if (c1.child < 0 && c2.child < 0) ++c_leaf;
-                    if (!Ops::batched) {
-                        bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P, hist, hnp, R);
-                    } else {
-                        const int pp = Ops::pair(it);
-                        const int sl = pair_slot(pp, s_tag);
-                        if (sl >= 0)
-                            bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P, hist + sl * per_slot,
-                                           hnp + sl * per_slot, R);
-                        else   // all LDS slots taken: straight into the pair's HBM sums
-                            bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P,
-                                           pair_hist + (size_t)pp * 2 * nh,
-                                           pair_np + (size_t)pp * nb, 1);
-                    }
-                } else if (cls == CL_SPLIT) {
-                    // CalcSplitSq, include/Split.h:82-101
-                    const double bsq_eff = rsq * (P.asq < P.bsq ? P.asq : P.bsq);
-                    bool sp1, sp2;
-                    if (s2 > s1) {
-                        sp2 = true;
-                        sp1 = (s2 > 2. * s1) ? false : (s1 * s1 > 0.3422 * bsq_eff);
-                    } else if (s1 > 2. * s2) {
-                        sp1 = true; sp2 = false;
-                    } else {
-                        sp1 = true;
-                        sp2 = s2 * s2 > 0.3422 * bsq_eff;
-                    }
-                    if (sp1 && sp2) {
-                        const int L1 = c1.child, L2 = c2.child;
-                        o0 = Ops::make(L1, L2, it); o1 = Ops::make(L1, L2 + 1, it);
-                        o2 = Ops::make(L1 + 1, L2, it); o3 = Ops::make(L1 + 1, L2 + 1, it);
-                        no = 4;
-                    } else if (sp1) {
-                        o0 = Ops::make(c1.child, it.j, it); o1 = Ops::make(c1.child + 1, it.j, it);
-                        no = 2;
-                    } else {
-                        o0 = Ops::make(it.i, c2.child, it); o1 = Ops::make(it.i, c2.child + 1, it);
-                        no = 2;
-                    }
+                k = int(kk);
+                if (cls == CL_REFINE) {
+                    ++X.c_ref;
+                    cls = fits ? CL_SINGLE : CL_SPLIT;
+                }
+            }
+            if (cls == CL_SINGLE && inrange) {
+                r = sqrt_fast(rsq);
+                ++X.c_acc;
+                if (c1.child < 0 && c2.child < 0) ++X.c_leaf;
+                if (!Ops::batched) {
+                    bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P, X.hist, X.hnp, X.R);
+                } else {
+                    const int pp = Ops::pair(it);
+                    const int sl = pair_slot(pp, X.s_tag);
+                    if (sl >= 0)
+                        bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P, X.hist + sl * X.per_slot,
+                                       X.hnp + sl * X.per_slot, X.R);
+                    else   // all LDS slots taken: straight into the pair's HBM sums
+                        bin_pair<D, C, GM>(c1, c2, rsq, k, r, logr, P,
+                                       X.pair_hist + (size_t)pp * 2 * NQ2 * P.nbins,
+                                       X.pair_np + (size_t)pp * P.nbins, 1);
+                }
+            } else if (cls == CL_SPLIT) {
+                // CalcSplitSq, include/Split.h:82-101
+                const double bsq_eff = rsq * (P.asq < P.bsq ? P.asq : P.bsq);
+                bool sp1, sp2;
+                if (s2 > s1) {
+                    sp2 = true;
+                    sp1 = (s2 > 2. * s1) ? false : (s1 * s1 > 0.3422 * bsq_eff);
+                } else if (s1 > 2. * s2) {
+                    sp1 = true; sp2 = false;
+                } else {
+                    sp1 = true;
+                    sp2 = s2 * s2 > 0.3422 * bsq_eff;
+                }
+                if (sp1 && sp2) {
+                    const int L1 = c1.child, L2 = c2.child;
+                    o0 = Ops::make(L1, L2, it); o1 = Ops::make(L1, L2 + 1, it);
+                    o2 = Ops::make(L1 + 1, L2, it); o3 = Ops::make(L1 + 1, L2 + 1, it);
+                    no = 4;
+                } else if (sp1) {
+                    o0 = Ops::make(c1.child, it.j, it); o1 = Ops::make(c1.child + 1, it.j, it);
+                    no = 2;
+                } else {
+                    o0 = Ops::make(it.i, c2.child, it); o1 = Ops::make(it.i, c2.child + 1, it);
+                    no = 2;
                 }
             }
         }
-        // Wavefront compaction of the children: no <= 4 is three bits, so the lane's
-        // exclusive prefix is sum_k 2^k (lanes below with bit k set), from three votes.
-        const unsigned long long m0 = __ballot(no & 1), m1 = __ballot(no & 2),
-                                 m2 = __ballot(no & 4);
-        const unsigned int wtot = __popcll(m0) + 2 * __popcll(m1) + 4 * __popcll(m2);
-        if (wtot != 0) {      // uniform over the wavefront
-            unsigned int wbase = 0;
-            if (lane == 0) wbase = atomicAdd(&s_pos, wtot);
-            wbase = (unsigned int)__builtin_amdgcn_readlane((int)wbase, 0);
-            auto below = [](unsigned long long m) {
-                return __builtin_amdgcn_mbcnt_hi((unsigned int)(m >> 32),
-                                                 __builtin_amdgcn_mbcnt_lo((unsigned int)m, 0u));
-            };
-            const unsigned int pos = wbase + below(m0) + 2 * below(m1) + 4 * below(m2);
-            if (no > 0) bout[pos] = o0;
-            if (no > 1) bout[pos + 1] = o1;
-            if (no > 2) bout[pos + 2] = o2;
-            if (no > 3) bout[pos + 3] = o3;
-        }
-    };
-    IT itA{}, itB{};
-    DNode a1{}, a2{}, q1{}, q2{};
-    const long long t0 = b0 + threadIdx.x;
-    if (t0 < b1) {
-        itA = fetch(t0);
-        a1 = n1[itA.i];
-        a2 = n2[itA.j >= 0 ? itA.j : 0];
     }
-    if (t0 + bd < b1) itB = fetch(t0 + bd);
-    for (long long base = b0; base < b1; base += 2 * bd) {
-        const long long idx = base + threadIdx.x;
-        // round on set A; loads for set B (the item of the next round is itB)
-        const IT curA = itA;
-        if (idx + bd < b1) {
-            q1 = n1[itB.i];
-            q2 = n2[itB.j >= 0 ? itB.j : 0];
-        }
-        if (idx + 2 * bd < b1) itA = fetch(idx + 2 * bd);
-        do_round(curA, a1, a2, idx);
-        if (base + bd >= b1) break;     // uniform over the workgroup
-        // round on set B; loads for set A
-        const IT curB = itB;
-        if (idx + 2 * bd < b1) {
-            a1 = n1[itA.i];
-            a2 = n2[itA.j >= 0 ? itA.j : 0];
-        }
-        if (idx + 3 * bd < b1) itB = fetch(idx + 3 * bd);
-        do_round(curB, q1, q2, idx + bd);
+    // Wavefront compaction of the children: no <= 4 is three bits, so the lane's
+    // exclusive prefix is sum_k 2^k (lanes below with bit k set), from three votes.
+    const unsigned long long m0 = __ballot(no & 1), m1 = __ballot(no & 2),
+                             m2 = __ballot(no & 4);
+    const unsigned int wtot = __popcll(m0) + 2 * __popcll(m1) + 4 * __popcll(m2);
+    if (wtot != 0) {      // uniform over the wavefront
+        unsigned int wbase = 0;
+        if (X.lane == 0) wbase = atomicAdd(s_pos, wtot);
+        wbase = (unsigned int)__builtin_amdgcn_readlane((int)wbase, 0);
+        auto below = [](unsigned long long m) {
+            return __builtin_amdgcn_mbcnt_hi((unsigned int)(m >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((unsigned int)m, 0u));
+        };
+        const unsigned int pos = wbase + below(m0) + 2 * below(m1) + 4 * below(m2);
+        if (no > 0) bout[pos] = o0;
+        if (no > 1) bout[pos + 1] = o1;
+        if (no > 2) bout[pos + 2] = o2;
+        if (no > 3) bout[pos + 3] = o3;
     }
-    atomicAdd(&hcnt[CNT_VISITS], (unsigned long long)c_vis);
-    atomicAdd(&hcnt[CNT_ACCEPTED], (unsigned long long)c_acc);
-    atomicAdd(&hcnt[CNT_LEAF], (unsigned long long)c_leaf);
-    atomicAdd(&hcnt[CNT_REFINED], (unsigned long long)c_ref);
-    __syncthreads();
-    if (threadIdx.x == 0) bcount[blockIdx.x] = s_pos;
+}
+
+// End of a traversal launch: the workgroup's visit counters and LDS histograms go to
+// its accumulation slot (a single correlation: Neumaier-compensated, slot blockIdx.x)
+// or, in a batched run, to the patch pairs' sums in HBM.
+template <class IT>
+__device__ __forceinline__ void flush_sums(double* hist_all, unsigned long long* hnp_all,
+                                           unsigned long long* hcnt, int nh, int nb, int R,
+                                           size_t per_slot, const int* s_tag,
+                                           double* __restrict__ slot_hi,
+                                           double* __restrict__ slot_lo,
+                                           unsigned long long* __restrict__ slot_np,
+                                           unsigned long long* __restrict__ slot_cnt,
+                                           double* __restrict__ pair_hist,
+                                           unsigned long long* __restrict__ pair_np) {
+    using Ops = ItemOps<IT>;
     if (Ops::batched) {
         // Add each used pair histogram to that pair's sums in HBM.  A pair's sums are
         // a (hi, lo) pair of arrays: the exact rounding error of every addition into
@@ -667,6 +603,225 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
         slot_np[(size_t)blockIdx.x * nb + t] += v;
     }
     if (threadIdx.x < NCNT) slot_cnt[(size_t)blockIdx.x * NCNT + threadIdx.x] += hcnt[threadIdx.x];
+}
+
+// Workgroup b takes the items [b*per_block, (b+1)*per_block) of the chunk and writes
+// the children of its split pairs to its own region out[b*4*per_block ...] (at most
+// four children per item); bcount[b] gets their number.  Each wavefront claims room
+// for its children with one LDS atomic, so the waves of a workgroup never wait for
+// each other; the order of the children in a region does not matter.  No global
+// atomics: the regions are concatenated afterwards.
+// GM: the other metrics and the r_par range (metric_dist_sq, metric_prune); the Euclidean instantiations
+// keep the plain distance, whose kernels need far fewer registers.
+template <int D, int C, class IT, bool GM>
+__global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
+    const DNode* __restrict__ n1, const DNode* __restrict__ n2, InView<IT> V,
+    long long nin, long long per_block, IT* __restrict__ out,
+    unsigned int* __restrict__ bcount, C2Params P, int R,
+    double* __restrict__ slot_hi, double* __restrict__ slot_lo,
+    unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt,
+    double* __restrict__ pair_hist, unsigned long long* __restrict__ pair_np) {
+    using Ops = ItemOps<IT>;
+    HIP_DYNAMIC_SHARED(double, sh)
+    __shared__ unsigned int s_pos;   // children written by this workgroup so far
+    // region offsets of the in-place head (a launch's output stays below 2^32 items)
+    __shared__ unsigned int s_off[TRAV_SLOTS + 1];
+    __shared__ int s_tag[PB_SLOTS];
+    const int nb = P.nbins;
+    const int nh = NQ2 * nb;
+    // R interleaved copies of the histogram (R a power of two): lanes that land in the
+    // same bin in one instruction hit R different LDS banks instead of one address.
+    // A batched run holds PB_SLOTS such histograms, one per patch pair.
+    const int nslot = Ops::batched ? PB_SLOTS : 1;
+    const size_t per_slot = (size_t)(nh + nb) * R;
+    unsigned long long* hcnt = reinterpret_cast<unsigned long long*>(sh + per_slot * nslot);
+    for (int t = threadIdx.x; t < (int)(per_slot * nslot) + NCNT; t += blockDim.x)
+        reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
+    if (V.nhead > 0)
+        for (int t = threadIdx.x; t <= V.nreg; t += blockDim.x) s_off[t] = (unsigned int)V.off[t];
+    if (threadIdx.x < PB_SLOTS) s_tag[threadIdx.x] = -1;
+    if (threadIdx.x == 0) s_pos = 0;
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    double* hist_all = sh;
+    unsigned long long* hnp_all = reinterpret_cast<unsigned long long*>(sh + (size_t)nh * R);
+    double* hist = hist_all + (lane & (R - 1));
+    unsigned long long* hnp = hnp_all + (lane & (R - 1));
+    VisitCtx<IT> X{hist, hnp, R, per_slot, s_tag, pair_hist, pair_np, 0u, 0u, 0u, 0u, lane};
+    const long long b0 = (long long)blockIdx.x * per_block;
+    const long long b1 = b0 + per_block < nin ? b0 + per_block : nin;
+    IT* __restrict__ bout = out + (size_t)blockIdx.x * 4 * per_block;
+    // region cursor of this thread's next item (items of a thread only move forward)
+    int reg = 0;
+    if (b0 + threadIdx.x < b1 && b0 + threadIdx.x < V.nhead) {
+        const unsigned int g = (unsigned int)(V.first + b0 + threadIdx.x);
+        int lo = 0, hi = V.nreg - 1;      // last region with s_off[r] <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        reg = lo;
+    }
+    auto fetch = [&](long long idx) {
+        if (idx < V.nhead) {
+            const unsigned int g = (unsigned int)(V.first + idx);
+            while (s_off[reg + 1] <= g) ++reg;
+            return V.in[(long long)reg * V.stride + (long long)(g - s_off[reg])];
+        }
+        return V.tail[idx - V.nhead];
+    };
+    // Software pipeline over two register sets: while a round works on one item and
+    // its nodes, the nodes of the next round's item and the item two rounds ahead are
+    // loaded into the other set, so their memory latency overlaps this round's work.
+    // The loop is unrolled by two so that the sets alternate without register copies.
+    // (A process2 item, j < 0, loads node 0 of the second tree in place of its unused
+    // second node; items past the end load nothing.)
+    const long long bd = blockDim.x;
+    IT itA{}, itB{};
+    DNode a1{}, a2{}, q1{}, q2{};
+    const long long t0 = b0 + threadIdx.x;
+    if (t0 < b1) {
+        itA = fetch(t0);
+        a1 = n1[itA.i];
+        a2 = n2[itA.j >= 0 ? itA.j : 0];
+    }
+    if (t0 + bd < b1) itB = fetch(t0 + bd);
+    for (long long base = b0; base < b1; base += 2 * bd) {
+        const long long idx = base + threadIdx.x;
+        // round on set A; loads for set B (the item of the next round is itB)
+        const IT curA = itA;
+        if (idx + bd < b1) {
+            q1 = n1[itB.i];
+            q2 = n2[itB.j >= 0 ? itB.j : 0];
+        }
+        if (idx + 2 * bd < b1) itA = fetch(idx + 2 * bd);
+        visit_round<D, C, IT, GM>(X, P, curA, a1, a2, idx < b1, bout, &s_pos);
+        if (base + bd >= b1) break;     // uniform over the workgroup
+        // round on set B; loads for set A
+        const IT curB = itB;
+        if (idx + 2 * bd < b1) {
+            a1 = n1[itA.i];
+            a2 = n2[itA.j >= 0 ? itA.j : 0];
+        }
+        if (idx + 3 * bd < b1) itB = fetch(idx + 3 * bd);
+        visit_round<D, C, IT, GM>(X, P, curB, q1, q2, idx + bd < b1, bout, &s_pos);
+    }
+    atomicAdd(&hcnt[CNT_VISITS], (unsigned long long)X.c_vis);
+    atomicAdd(&hcnt[CNT_ACCEPTED], (unsigned long long)X.c_acc);
+    atomicAdd(&hcnt[CNT_LEAF], (unsigned long long)X.c_leaf);
+    atomicAdd(&hcnt[CNT_REFINED], (unsigned long long)X.c_ref);
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = s_pos;
+    flush_sums<IT>(hist_all, hnp_all, hcnt, nh, nb, R, per_slot, s_tag, slot_hi, slot_lo,
+                   slot_np, slot_cnt, pair_hist, pair_np);
+}
+
+// Persistent form of the traversal: one launch runs a whole correlation.  The grid is
+// the number of workgroups the chip holds at once; each keeps a LIFO stack of items in
+// its own HBM region stk[0, cap), pops up to TRAV_THREADS items a round (the newest
+// first) and pushes the children of its split pairs back on top, so the stack stays
+// shallow as the reference's recursion does.  When fewer than TRAV_THREADS items are
+// left it takes more jobs (initial items): in dynamic mode the next ones of the job
+// list by one atomic counter (as the reference's OpenMP schedule(dynamic) hands out
+// top-level pairs, src/Corr2.cpp:158-203), in static mode the next ones of its own
+// contiguous, equal-count share of the list.  A workgroup leaves when its stack and its
+// jobs are exhausted; no workgroup ever waits for another.  A round whose children
+// might not fit the stack sets *flag and ends the workgroup instead; the caller then
+// discards the sums and runs the chunked traversal.
+template <int D, int C, class IT, bool GM>
+__global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
+    const DNode* __restrict__ n1, const DNode* __restrict__ n2, const IT* __restrict__ jobs,
+    long long njobs, unsigned long long* __restrict__ job_next, int static_part,
+    IT* __restrict__ stacks, long long cap, unsigned int* __restrict__ flag, C2Params P, int R,
+    double* __restrict__ slot_hi, double* __restrict__ slot_lo,
+    unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt,
+    double* __restrict__ pair_hist, unsigned long long* __restrict__ pair_np) {
+    using Ops = ItemOps<IT>;
+    HIP_DYNAMIC_SHARED(double, sh)
+    __shared__ unsigned int s_pos;           // children written this round
+    __shared__ long long s_size;             // items on the stack
+    __shared__ long long s_g0, s_gn;         // jobs taken this round
+    __shared__ long long s_snext, s_send;    // static mode: the rest of this share
+    __shared__ int s_tag[PB_SLOTS];
+    const int nb = P.nbins;
+    const int nh = NQ2 * nb;
+    const int nslot = Ops::batched ? PB_SLOTS : 1;
+    const size_t per_slot = (size_t)(nh + nb) * R;
+    unsigned long long* hcnt = reinterpret_cast<unsigned long long*>(sh + per_slot * nslot);
+    for (int t = threadIdx.x; t < (int)(per_slot * nslot) + NCNT; t += blockDim.x)
+        reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
+    if (threadIdx.x < PB_SLOTS) s_tag[threadIdx.x] = -1;
+    if (threadIdx.x == 0) {
+        s_size = 0;
+        s_snext = njobs * (long long)blockIdx.x / gridDim.x;
+        s_send = njobs * (long long)(blockIdx.x + 1) / gridDim.x;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    double* hist_all = sh;
+    unsigned long long* hnp_all = reinterpret_cast<unsigned long long*>(sh + (size_t)nh * R);
+    VisitCtx<IT> X{hist_all + (lane & (R - 1)), hnp_all + (lane & (R - 1)), R, per_slot, s_tag,
+                   pair_hist, pair_np, 0u, 0u, 0u, 0u, lane};
+    IT* __restrict__ stk = stacks + (size_t)blockIdx.x * cap;
+    const long long bd = blockDim.x;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            long long g0 = 0, gn = 0;
+            const long long need = bd - s_size;
+            if (need > 0) {
+                if (static_part) {
+                    g0 = s_snext;
+                    gn = need < s_send - s_snext ? need : s_send - s_snext;
+                    s_snext += gn;
+                } else {
+                    // a batch of at most a sixteenth of a fair share, so the jobs left
+                    // near the end stay spread over the workgroups
+                    long long want = njobs / (16LL * gridDim.x);
+                    want = want < 1 ? 1 : (want < need ? want : need);
+                    const unsigned long long a = atomicAdd(job_next, (unsigned long long)want);
+                    if (a < (unsigned long long)njobs) {
+                        g0 = (long long)a;
+                        gn = want < njobs - g0 ? want : njobs - g0;
+                    }
+                }
+            }
+            s_g0 = g0;
+            s_gn = gn;
+            s_pos = 0;
+        }
+        __syncthreads();
+        const long long size0 = s_size, gn = s_gn;
+        if (threadIdx.x < gn) stk[size0 + threadIdx.x] = jobs[s_g0 + threadIdx.x];
+        const long long size = size0 + gn;
+        if (size == 0) break;                       // uniform: nothing left anywhere
+        const long long n = size < bd ? size : bd;
+        const long long base = size - n;
+        if (base + 4 * n > cap) {                   // uniform
+            if (threadIdx.x == 0) atomicOr(flag, 1u);
+            break;
+        }
+        __syncthreads();                            // the jobs taken are on the stack
+        const bool valid = threadIdx.x < n;
+        IT it{};
+        DNode c1{}, c2{};
+        if (valid) {
+            it = stk[base + threadIdx.x];
+            c1 = n1[it.i];
+            c2 = n2[it.j >= 0 ? it.j : 0];
+        }
+        __syncthreads();                            // every item read before children overwrite them
+        visit_round<D, C, IT, GM>(X, P, it, c1, c2, valid, stk + base, &s_pos);
+        __syncthreads();
+        if (threadIdx.x == 0) s_size = base + s_pos;
+    }
+    atomicAdd(&hcnt[CNT_VISITS], (unsigned long long)X.c_vis);
+    atomicAdd(&hcnt[CNT_ACCEPTED], (unsigned long long)X.c_acc);
+    atomicAdd(&hcnt[CNT_LEAF], (unsigned long long)X.c_leaf);
+    atomicAdd(&hcnt[CNT_REFINED], (unsigned long long)X.c_ref);
+    __syncthreads();
+    flush_sums<IT>(hist_all, hnp_all, hcnt, nh, nb, R, per_slot, s_tag, slot_hi, slot_lo,
+                   slot_np, slot_cnt, pair_hist, pair_np);
 }
 
 // Fixed-order compensated reduction of the per-workgroup slots.
@@ -856,6 +1011,9 @@ struct Corr2Scratch {
     unsigned long long* outcount = nullptr;
     unsigned long long* h_outcount = nullptr;
     unsigned int* bcount = nullptr;     // two count arrays (ping-pong), TRAV_SLOTS each
+    void* pstack = nullptr;             // per-workgroup stacks of the persistent traversal
+    size_t pstack_bytes = 0;
+    unsigned long long* pctl = nullptr; // persistent traversal: job counter, overflow flag
     unsigned long long* boff = nullptr; // two offset arrays, TRAV_SLOTS+1 each
     int nb = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -884,6 +1042,8 @@ struct Corr2Scratch {
         dev_free(outcount, sizeof(unsigned long long));
         dev_free(bcount, kBcountBytes);
         dev_free(boff, kBoffBytes);
+        dev_free(pstack, pstack_bytes);
+        dev_free(pctl, 2 * sizeof(unsigned long long));
         if (h_outcount) (void)hipHostFree(h_outcount);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -916,7 +1076,7 @@ static int ensure_scratch(Corr2Scratch& S, int nb, size_t item_bytes) {
         TC_HIP(hipEventCreate(&S.ev0));
         TC_HIP(hipEventCreate(&S.ev1));
     }
-    const size_t qb = item_bytes * 4 * (TRAV_CHUNK + (size_t)TRAV_THREADS * TRAV_SLOTS);
+    const size_t qb = item_bytes * 4 * ((size_t)g_chunk + (size_t)TRAV_THREADS * TRAV_SLOTS);
     if (S.outq_bytes < qb) {
         for (void*& q : S.outq) {
             dev_free(q, S.outq_bytes);
@@ -979,9 +1139,12 @@ static int hist_copies(int nb, int nslot, size_t& shbytes) {
 
 // The LIFO work-stack loop over launches; the stack holds the ninit initial items.
 template <class IT>
+// With until > 0 the loop stops as soon as at least `until` items are pending, with all
+// of them on the stack; *left receives their number (the jobs of a persistent launch).
 static int traverse_stack(Corr2Scratch& S, TravFn<IT> fn, const DNode* n1, const DNode* n2,
                           long long ninit, const C2Params& P, int R, size_t shbytes,
-                          double* pair_hist, unsigned long long* pair_np) {
+                          double* pair_hist, unsigned long long* pair_np,
+                          long long until = 0, long long* left = nullptr) {
     hipStream_t st = stream();
     IT* stack = static_cast<IT*>(S.stack);
     IT* outq[2] = {static_cast<IT*>(S.outq[0]), static_cast<IT*>(S.outq[1])};
@@ -993,6 +1156,21 @@ static int traverse_stack(Corr2Scratch& S, TravFn<IT> fn, const DNode* n1, const
     long long reg_n = 0, reg_pb = 0;
     int reg_blocks = 0;
     for (;;) {
+        if (until > 0 && top + reg_n >= until) {
+            // enough pending work: everything onto the stack, for the persistent launch
+            if (reg_n > 0) {
+                if (ensure_stack(S, sizeof(IT) * (top + reg_n), sizeof(IT) * top)) return 2;
+                stack = static_cast<IT*>(S.stack);
+                hipLaunchKernelGGL(k_gather_children<IT>, dim3(reg_blocks), dim3(256), 0, st,
+                                   outq[cur], reg_pb, S.bcount + cur * TRAV_SLOTS,
+                                   S.boff + cur * (TRAV_SLOTS + 1), (unsigned long long)reg_n,
+                                   stack + top);
+                TC_HIP(hipGetLastError());
+                top += reg_n;
+            }
+            *left = top;
+            return 0;
+        }
         InView<IT> V{nullptr, nullptr, 0, 0, 0, 0, nullptr};
         long long n = 0;
         if (reg_n > 0) {
@@ -1054,6 +1232,98 @@ static int traverse_stack(Corr2Scratch& S, TravFn<IT> fn, const DNode* n1, const
         reg_blocks = blocks;
         cur = nxt;
     }
+    if (left) *left = 0;
+    return 0;
+}
+
+template <class IT>
+using PersistFn = void (*)(const DNode*, const DNode*, const IT*, long long, unsigned long long*,
+                           int, IT*, long long, unsigned int*, C2Params, int, double*, double*,
+                           unsigned long long*, unsigned long long*, double*, unsigned long long*);
+
+template <class IT>
+static PersistFn<IT> pick_persist(int D, int C, bool gm) {
+#define TC_TF(DD, CC) return gm ? &k_traverse_persist<DD, CC, IT, true> : &k_traverse_persist<DD, CC, IT, false>
+#define TC_TF3(KK) if (D == KK) { if (C == Flat) TC_TF(KK, Flat); if (C == ThreeD) TC_TF(KK, ThreeD); TC_TF(KK, Sphere); }
+    TC_TF3(NData) TC_TF3(KData) TC_TF3(XK_NK) TC_TF3(XK_NG) TC_TF3(XK_KG) TC_TF3(ZData)
+    TC_TF3(XK_NZ) TC_TF3(XK_KZ) TC_TF3(VData) TC_TF3(TData) TC_TF3(QData)
+    TC_TF3(XK_NV) TC_TF3(XK_NT) TC_TF3(XK_NQ) TC_TF3(XK_KV) TC_TF3(XK_KT) TC_TF3(XK_KQ)
+    if (C == Flat) TC_TF(GData, Flat); if (C == ThreeD) TC_TF(GData, ThreeD); TC_TF(GData, Sphere);
+#undef TC_TF3
+#undef TC_TF
+}
+
+// How a correlation's traversal runs (tc_set_traversal_mode): the persistent kernel
+// taking jobs by a shared counter (default), the persistent kernel with a static
+// equal-count partition of the jobs, or the chunked launches driven from the host.
+static int g_trav_mode = TRAV_PERSIST_DYNAMIC;
+std::atomic<long long> g_persist_fallbacks{0};   // persistent runs redone chunked
+static int g_persist_levels = 0;    // stack levels per workgroup (0: from the tree depths)
+static long long g_persist_min_jobs = PERSIST_MIN_JOBS;
+int set_persist_params(int levels, long long min_jobs) {
+    if (levels < 0 || min_jobs < 0) return fail("levels and min_jobs must be >= 0");
+    g_persist_levels = levels;
+    g_persist_min_jobs = min_jobs == 0 ? PERSIST_MIN_JOBS : min_jobs;
+    return 0;
+}
+int set_traversal_mode(int mode) {
+    if (mode < TRAV_CHUNKED || mode > TRAV_PERSIST_STATIC) return fail("unknown traversal mode");
+    g_trav_mode = mode;
+    return 0;
+}
+int traversal_mode() { return g_trav_mode; }
+
+// One launch of the persistent traversal over the njobs items at jobs.  depth_sum bounds
+// the chain of splits below a job (the depths of the two trees); each workgroup's stack
+// holds 4 * TRAV_THREADS items per level of it.  Sets overflow when a workgroup's stack
+// would not have held its children (the sums are then incomplete).
+template <class IT>
+static int traverse_persist(Corr2Scratch& S, PersistFn<IT> fn, const DNode* n1, const DNode* n2,
+                            const IT* jobs, long long njobs, int depth_sum, bool static_part,
+                            const C2Params& P, int R, size_t shbytes, double* pair_hist,
+                            unsigned long long* pair_np, bool& overflow) {
+    hipStream_t st = stream();
+    overflow = false;
+    if (njobs <= 0) return 0;
+    int dev = 0, ncu = 0, per_cu = 0;
+    TC_HIP(hipGetDevice(&dev));
+    TC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    TC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn),
+                                                        TRAV_THREADS, shbytes));
+    const int grid = std::max(1, std::min(TRAV_SLOTS, std::max(1, per_cu) * ncu));
+    const long long cap = 4LL * TRAV_THREADS * (g_persist_levels > 0 ? g_persist_levels : depth_sum + 4);
+    const size_t bytes = sizeof(IT) * (size_t)cap * grid;
+    if (S.pstack_bytes < bytes) {
+        dev_free(S.pstack, S.pstack_bytes);
+        S.pstack_bytes = 0;
+        S.pstack = dev_alloc(bytes);
+        if (!S.pstack) return fail("out of device memory for the traversal stacks");
+        S.pstack_bytes = bytes;
+    }
+    if (!S.pctl) {
+        S.pctl = static_cast<unsigned long long*>(dev_alloc(2 * sizeof(unsigned long long)));
+        if (!S.pctl) return fail("out of device memory for the traversal");
+    }
+    TC_HIP(hipMemsetAsync(S.pctl, 0, 2 * sizeof(unsigned long long), st));
+    KernelStat& ks = kernel_stat(KS_TRAVERSE2);
+    const bool timing = timing_enabled();
+    if (timing) TC_HIP(hipEventRecord(S.ev0, st));
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(TRAV_THREADS), shbytes, st, n1, n2, jobs, njobs,
+                       S.pctl, static_part ? 1 : 0, static_cast<IT*>(S.pstack), cap,
+                       reinterpret_cast<unsigned int*>(S.pctl + 1), P, R, S.slot_hi, S.slot_lo,
+                       S.slot_np, S.slot_cnt, pair_hist, pair_np);
+    TC_HIP(hipGetLastError());
+    if (timing) TC_HIP(hipEventRecord(S.ev1, st));
+    TC_HIP(hipMemcpyAsync(S.h_outcount, S.pctl + 1, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st));
+    TC_HIP(hipStreamSynchronize(st));
+    if (timing) {
+        float ms = 0.f;
+        TC_HIP(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+        ks.ms += ms;
+        ks.launches += 1;
+    }
+    overflow = (*S.h_outcount & 0xffffffffull) != 0;
     return 0;
 }
 
@@ -1070,8 +1340,8 @@ static int zero_slots(Corr2Scratch& S, int nb) {
 // Runs the whole traversal for one auto or cross correlation and writes the
 // reduced sums: out[NQ2*nb] doubles, np[nb], cnt[NCNT] (host arrays).
 int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dtop1, int ntop1,
-              const DNode* nodes2, const int* dtop2, int ntop2, const C2Params& P,
-              double* out, unsigned long long* np, unsigned long long* cnt) {
+              int depth1, const DNode* nodes2, const int* dtop2, int ntop2, int depth2,
+              const C2Params& P, double* out, unsigned long long* np, unsigned long long* cnt) {
     hipStream_t st = stream();
     const int nb = P.nbins;
     const size_t nh = (size_t)NQ2 * nb;
@@ -1092,10 +1362,44 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
         else hipLaunchKernelGGL(k_items_cross, g, dim3(256), 0, st, dtop1, ntop1, dtop2, ntop2, stack);
         TC_HIP(hipGetLastError());
     }
-    if (int rc = traverse_stack<Item>(S, pick_traverse<Item>(D, C, P.metric != MetEuclidean || P.rpar_on), nodes1,
-                                      autoc ? nodes1 : nodes2, ninit, P, R, shbytes, nullptr,
-                                      nullptr))
-        return rc;
+    const bool gm = P.metric != MetEuclidean || P.rpar_on;
+    bool chunked = g_trav_mode == TRAV_CHUNKED;
+    if (!chunked) {
+        // The initial node pairs are first split by chunked launches (breadth first,
+        // while fewer than a chunk are pending) until there are g_persist_min_jobs jobs;
+        // then one persistent launch takes them all.
+        long long njobs = ninit;
+        if (ninit < g_persist_min_jobs)
+            if (int rc = traverse_stack<Item>(S, pick_traverse<Item>(D, C, gm), nodes1,
+                                              autoc ? nodes1 : nodes2, ninit, P, R, shbytes,
+                                              nullptr, nullptr, g_persist_min_jobs, &njobs))
+                return rc;
+        bool overflow = false;
+        if (int rc = traverse_persist<Item>(S, pick_persist<Item>(D, C, gm), nodes1,
+                                            autoc ? nodes1 : nodes2,
+                                            static_cast<const Item*>(S.stack), njobs,
+                                            depth1 + (autoc ? depth1 : depth2),
+                                            g_trav_mode == TRAV_PERSIST_STATIC, P, R, shbytes,
+                                            nullptr, nullptr, overflow))
+            return rc;
+        if (overflow) {      // a stack would have overflowed: start again, chunked
+            g_persist_fallbacks.fetch_add(1);
+            if (zero_slots(S, nb)) return 2;
+            if (ninit > 0) {
+                dim3 g((unsigned)((ninit + 255) / 256));
+                Item* stack = static_cast<Item*>(S.stack);
+                if (autoc) hipLaunchKernelGGL(k_items_auto, g, dim3(256), 0, st, dtop1, ntop1, stack);
+                else hipLaunchKernelGGL(k_items_cross, g, dim3(256), 0, st, dtop1, ntop1, dtop2, ntop2, stack);
+                TC_HIP(hipGetLastError());
+            }
+            chunked = true;
+        }
+    }
+    if (chunked)
+        if (int rc = traverse_stack<Item>(S, pick_traverse<Item>(D, C, gm), nodes1,
+                                          autoc ? nodes1 : nodes2, ninit, P, R, shbytes, nullptr,
+                                          nullptr))
+            return rc;
     const int tot = (int)(nh + nb + NCNT);
     hipLaunchKernelGGL(k_reduce_slots, dim3(tot), dim3(64), 0, st, S.slot_hi,
                        S.slot_lo, S.slot_np, S.slot_cnt, TRAV_SLOTS, (int)nh, nb, S.red, S.red_np,
@@ -1258,6 +1562,8 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
             pair_out[p * nh + t] = hl[2 * nh * p + t] + hl[2 * nh * p + nh + t];
     return 0;
 }
+
+long long persist_fallbacks() { return g_persist_fallbacks.load(); }
 
 int set_traversal_chunk(long long items) {
     if (items < 0) return fail("chunk must be >= 0");

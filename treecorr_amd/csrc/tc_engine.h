@@ -13,6 +13,7 @@ struct DeviceTree {
     DNode* nodes = nullptr;
     size_t nodes_bytes = 0;
     int nnodes = 0;
+    int depth = 0;            // levels of the tree (a leaf at level depth-1 at most)
     std::vector<int> top;     // top-level node ids in TreeCorr's order
     int* dtop = nullptr;      // the same on the device
     size_t dtop_bytes = 0;
@@ -30,6 +31,7 @@ struct MultiTree {
     DNode* nodes = nullptr;     // owned by the caller after the build (dev_free)
     size_t nodes_bytes = 0;
     int nnodes = 0;
+    int depth = 0;                       // levels of the deepest tree
     std::vector<std::vector<int>> top;   // per root, in TreeCorr's order
     std::vector<FieldInfo> info;         // per root
 };
@@ -70,12 +72,19 @@ int quick_assign(const double* centers, int npatch, const double* x, const doubl
                  const double* z, long long n, int* patch);
 
 int set_traversal_chunk(long long items);   // 0 restores the default
+// Traversal modes: chunked launches driven from the host, or one persistent launch whose
+// workgroups take jobs by a shared counter or from static equal-count shares.
+enum { TRAV_CHUNKED = 0, TRAV_PERSIST_DYNAMIC = 1, TRAV_PERSIST_STATIC = 2 };
+int set_traversal_mode(int mode);
+int traversal_mode();
+long long persist_fallbacks();
+int set_persist_params(int levels, long long min_jobs);   // 0, 0: the defaults
 struct Corr2Scratch;
 Corr2Scratch* new_corr2_scratch();
 void delete_corr2_scratch(Corr2Scratch*);
 int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dtop1, int ntop1,
-              const DNode* nodes2, const int* dtop2, int ntop2, const C2Params& P, double* out,
-              unsigned long long* np, unsigned long long* cnt);
+              int depth1, const DNode* nodes2, const int* dtop2, int ntop2, int depth2,
+              const C2Params& P, double* out, unsigned long long* np, unsigned long long* cnt);
 // All patch pairs (pi[p], pj[p]) of a patch-based correlation in one traversal;
 // trees2 empty means side 2 is side 1.  Per-pair sums (7*nbins doubles, nbins
 // npairs each) are written to host arrays.

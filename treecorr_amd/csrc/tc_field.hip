@@ -1139,6 +1139,7 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
     const BuildParams P{minsize * minsize, maxsize * maxsize, sm, brute};
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
 
+    int nlevels = 0;
     auto attempt = [&](bool spec) -> int {
         TC_HIP(hipMemcpyAsync(bbounds.p, bounds.data(), sizeof(long long) * (nroot + 1),
                               hipMemcpyHostToDevice, st));
@@ -1295,6 +1296,7 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
             ++level;
             if (level > 4 * 64 + 64) return fail("tree build did not terminate");
         }
+        nlevels = level;
         if (SB.nlev > 0 && launch_check()) return 2;
         for (int k = 0; k < nlaunch; ++k) TC_HIP(hipStreamSynchronize(guard.cs[k]));
         return 0;
@@ -1321,6 +1323,7 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
     }
     if (rc) return rc;
     M.nnodes = cnts[0];
+    M.depth = nlevels;
     std::vector<TopEntry> tops(cnts[1]);
     TC_HIP(hipMemcpy(tops.data(), btops.p, sizeof(TopEntry) * cnts[1], hipMemcpyDeviceToHost));
     // each root's top cells in the reference's order (the path keys' order)
@@ -1363,6 +1366,7 @@ int build_tree(int D, int C, const double* x, const double* y, const double* z, 
     M.nodes = nullptr;
     if (rc) return rc;
     T.nnodes = M.nnodes;
+    T.depth = M.depth;
     T.top = std::move(M.top[0]);
     T.info = M.info[0];
     T.dtop_bytes = sizeof(int) * std::max<size_t>(1, T.top.size());
