@@ -186,12 +186,48 @@ int tc_set_traversal_chunk(long long items);
  * (initial node pairs, src/Corr2.cpp:158-203) by a shared counter, as the reference's
  * OpenMP schedule(dynamic) does.  2: the same launch with a static, equal-count
  * partition of the jobs.  0: chunked launches driven from the host (one read-back per
- * launch).  Results do not depend on the mode (npairs identical, sums to rounding).
+ * launch).  3: static shares of equal work as the cost model predicts it; 4: jobs
+ * handed out by the counter in order of decreasing predicted work (3 and 4 need
+ * tc_set_cost_model).  Results do not depend on the mode (npairs identical, sums to
+ * rounding).
  * Returns 0, or with mode < 0 the current mode.  No reference counterpart. */
 int tc_set_traversal_mode(int mode);
 /* Persistent traversals redone with chunked launches because a workgroup's stack would
  * have overflowed (expected 0).  No reference counterpart. */
 long long tc_traversal_fallbacks(void);
+/* The learned cost model of the persistent traversal's jobs (no reference counterpart:
+ * TreeCorr hands out its top-level pairs with OpenMP's schedule(dynamic),
+ * src/Corr2.cpp:158-203).  A perceptron of three FP32 layers (nfeatures inputs, two
+ * hidden layers of 64 ReLU units, one output) predicts y = log2(1 + items processed) of
+ * a job from its two nodes; trained on the GPU (forward, backward and Adam kernels of
+ * this library) on the traversal's own job logs.  Traversal modes 3 and 4 use it:
+ * static shares of equal predicted work, or jobs handed out in order of decreasing
+ * predicted work.  Results never depend on it. */
+typedef struct tc_cost_model tc_cost_model;
+int tc_cost_model_create(unsigned long long seed, tc_cost_model** out);
+int tc_cost_model_destroy(tc_cost_model* m);
+int tc_cost_model_sizes(int* nfeatures, int* nparams);
+/* params (nparams floats: W1[64][nf], b1[64], W2[64][64], b2[64], W3[64], b3) are read
+ * (set = 0) or written (set = 1). */
+int tc_cost_model_params(tc_cost_model* m, float* params, int set);
+int tc_cost_model_predict(tc_cost_model* m, const float* X, long long n, float* yhat);
+/* The gradient of the mean squared error over the n samples (X: n x nfeatures), and
+ * that loss. */
+int tc_cost_model_grad(tc_cost_model* m, const float* X, const float* y, long long n,
+                       float* grad, double* loss);
+/* steps Adam steps on mini-batches of `batch` samples drawn from the n; loss: the mean
+ * squared error over all n afterwards. */
+int tc_cost_model_train(tc_cost_model* m, const float* X, const float* y, long long n, int steps,
+                        int batch, float lr, double* loss);
+/* The model traversal modes 3 and 4 use (null clears it). */
+int tc_set_cost_model(tc_cost_model* m);
+/* With max_samples > 0, each following persistent 2-point traversal (NN, KK or GG, the
+ * Euclidean metric) counts the items of every job and keeps the features and counts of
+ * an evenly strided sample of at most max_samples jobs; 0 stops logging and clears the
+ * log.  tc_job_log_fetch copies at most cap samples (X: cap x nfeatures, visits: cap)
+ * and sets n to the number kept. */
+int tc_set_job_log(long long max_samples);
+int tc_job_log_fetch(float* X, float* visits, long long cap, long long* n);
 /* Parameters of the persistent traversal.  levels: stack room of each workgroup, in
  * levels of 4 x 256 node pairs (0, the default: the two trees' depths + 4).  min_jobs:
  * fewer initial node pairs than this are first split by chunked launches until there

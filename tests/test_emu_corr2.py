@@ -40,7 +40,7 @@ def emu_lib2_exe():
     flags = ['g++', '-std=c++17', '-O1', '-ffp-contract=off', '-fsanitize=address',
              '-fno-omit-frame-pointer']
     objs, procs = [], []
-    for src in ('tc_field.hip', 'tc_corr2.hip'):
+    for src in ('tc_field.hip', 'tc_corr2.hip', 'tc_cost.hip'):
         o = os.path.join(out, src + '.o')
         procs.append(subprocess.Popen(flags + ['-x', 'c++', *inc, '-c',
                                                os.path.join(ROOT, 'treecorr_amd', 'csrc', src),

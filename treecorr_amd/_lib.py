@@ -50,6 +50,20 @@ SIGNATURES = {
     'tc_set_traversal_mode': (_i, [_i]),
     'tc_traversal_fallbacks': (_ll, []),
     'tc_set_persist_params': (_i, [_i, _ll]),
+    'tc_cost_model_create': (_i, [ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
+    'tc_cost_model_destroy': (_i, [_vp]),
+    'tc_cost_model_sizes': (_i, [ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    'tc_cost_model_params': (_i, [_vp, ctypes.POINTER(ctypes.c_float), _i]),
+    'tc_cost_model_predict': (_i, [_vp, ctypes.POINTER(ctypes.c_float), _ll,
+                                   ctypes.POINTER(ctypes.c_float)]),
+    'tc_cost_model_grad': (_i, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                _ll, ctypes.POINTER(ctypes.c_float), _dp]),
+    'tc_cost_model_train': (_i, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                 _ll, _i, _i, ctypes.c_float, _dp]),
+    'tc_set_cost_model': (_i, [_vp]),
+    'tc_set_job_log': (_i, [_ll]),
+    'tc_job_log_fetch': (_i, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _ll,
+                              ctypes.POINTER(_ll)]),
     'tc_set_multipole_plain': (_i, [_i]),
     'tc_set_zeta_mode': (_i, [_i]),
     'tc_kmeans_run': (_i, [_dp, _i, _i, _dp, _dp, _dp, _dp, _ll, _i, _d, _d, _i,

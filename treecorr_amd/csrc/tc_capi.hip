@@ -144,6 +144,7 @@ struct tc_corr2 {
 // g_run_m, so they are serialized.
 static std::mutex g_run_m;
 static Corr2Scratch* g_s2 = nullptr;
+static tc_cost_model* g_cost_model_set = nullptr;   // the model tc_set_cost_model installed
 
 namespace {
 
@@ -623,6 +624,76 @@ int tc_set_traversal_mode(int mode) {
     return tc::set_traversal_mode(mode);
 }
 long long tc_traversal_fallbacks(void) { return tc::persist_fallbacks(); }
+struct tc_cost_model {
+    tc::CostModel* m = nullptr;
+    ~tc_cost_model() { tc::cost_model_delete(m); }
+};
+
+int tc_cost_model_create(unsigned long long seed, tc_cost_model** out) {
+    if (!out) return tc::fail("out is required");
+    *out = nullptr;
+    if (int rc = check_gpu()) return rc;
+    tc::CostModel* m = tc::cost_model_new(seed);
+    if (!m) return TC_ERR_HIP;
+    *out = new tc_cost_model;
+    (*out)->m = m;
+    return 0;
+}
+int tc_cost_model_destroy(tc_cost_model* m) {
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    if (g_cost_model_set == m) {
+        tc::set_cost_model(nullptr);
+        g_cost_model_set = nullptr;
+    }
+    delete m;
+    return 0;
+}
+int tc_cost_model_sizes(int* nfeatures, int* nparams) {
+    if (nfeatures) *nfeatures = tc::CM_NF;
+    if (nparams) *nparams = tc::CM_NP;
+    return 0;
+}
+int tc_cost_model_params(tc_cost_model* m, float* params, int set) {
+    if (!m || !params) return tc::fail("model and params are required");
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::cost_model_params(m->m, params, set != 0);
+}
+int tc_cost_model_predict(tc_cost_model* m, const float* X, long long n, float* yhat) {
+    if (!m || (n > 0 && (!X || !yhat))) return tc::fail("model, X and yhat are required");
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::cost_model_predict(m->m, X, n, yhat);
+}
+int tc_cost_model_grad(tc_cost_model* m, const float* X, const float* y, long long n,
+                       float* grad, double* loss) {
+    if (!m || !X || !y || !grad || !loss || n <= 0) return tc::fail("invalid arguments");
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::cost_model_grad(m->m, X, y, n, grad, loss);
+}
+int tc_cost_model_train(tc_cost_model* m, const float* X, const float* y, long long n, int steps,
+                        int batch, float lr, double* loss) {
+    if (!m || !X || !y || !loss || n <= 0) return tc::fail("invalid arguments");
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::cost_model_train(m->m, X, y, n, steps, batch, lr, loss);
+}
+int tc_set_cost_model(tc_cost_model* m) {
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    tc::set_cost_model(m ? m->m : nullptr);
+    g_cost_model_set = m;
+    return 0;
+}
+int tc_set_job_log(long long max_samples) {
+    if (max_samples < 0) return tc::fail("max_samples must be >= 0");
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    tc::set_job_log(max_samples);
+    return 0;
+}
+int tc_job_log_fetch(float* X, float* visits, long long cap, long long* n) {
+    if (!n) return tc::fail("n is required");
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    *n = tc::job_log_fetch(X, visits, cap);
+    return 0;
+}
+
 int tc_set_persist_params(int levels, long long min_jobs) {
     return tc::set_persist_params(levels, min_jobs);
 }

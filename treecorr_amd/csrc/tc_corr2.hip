@@ -338,6 +338,19 @@ template <> struct ItemOps<ItemP> {
     __device__ static int pair(const ItemP& it) { return it.p; }
 };
 
+// A node pair of a logged run: job is the index of the job (initial node pair) whose
+// subtree it belongs to, so that every job's visits can be counted.
+struct alignas(16) ItemJ { int i, j, job, pad; };
+template <> struct ItemOps<ItemJ> {
+    static constexpr bool batched = false;
+    __device__ static ItemJ make(int i, int j, const ItemJ& parent) {
+        return ItemJ{i, j, parent.job, 0};
+    }
+    __device__ static int pair(const ItemJ&) { return 0; }
+};
+template <class IT> struct IsLogged { static constexpr bool value = false; };
+template <> struct IsLogged<ItemJ> { static constexpr bool value = true; };
+
 constexpr int PB_SLOTS = 4;   // patch pairs with an LDS histogram per workgroup
 
 // Where a launch reads its items.  Items [0, nhead) come from the output regions of
@@ -380,6 +393,7 @@ struct VisitCtx {
     unsigned long long* pair_np;
     unsigned int c_vis, c_acc, c_leaf, c_ref;
     int lane;
+    unsigned int* job_visits = nullptr;   // a logged run: items processed per job
 };
 
 // One round of the traversal for one lane: the process2 / process11 decision for the
@@ -393,6 +407,8 @@ __device__ __forceinline__ void visit_round(VisitCtx<IT>& X, const C2Params& P, 
     using Ops = ItemOps<IT>;
     IT o0{}, o1{}, o2{}, o3{};
     int no = 0;
+    if constexpr (IsLogged<IT>::value)
+        if (valid) atomicAdd(&X.job_visits[it.job], 1u);
     if (valid) {
         if (it.j < 0) {
             // BaseCorr2::process2, src/Corr2.cpp:290-306
@@ -725,7 +741,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
 // left it takes more jobs (initial items): in dynamic mode the next ones of the job
 // list by one atomic counter (as the reference's OpenMP schedule(dynamic) hands out
 // top-level pairs, src/Corr2.cpp:158-203), in static mode the next ones of its own
-// contiguous, equal-count share of the list.  A workgroup leaves when its stack and its
+// contiguous share of the list (equal counts, or the shares of equal predicted work
+// that the cost model gives, bounds[b] to bounds[b+1]).  A workgroup leaves when its stack and its
 // jobs are exhausted; no workgroup ever waits for another.  A round whose children
 // might not fit the stack sets *flag and ends the workgroup instead; the caller then
 // discards the sums and runs the chunked traversal.
@@ -733,10 +750,11 @@ template <int D, int C, class IT, bool GM>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
     const DNode* __restrict__ n1, const DNode* __restrict__ n2, const IT* __restrict__ jobs,
     long long njobs, unsigned long long* __restrict__ job_next, int static_part,
-    IT* __restrict__ stacks, long long cap, unsigned int* __restrict__ flag, C2Params P, int R,
-    double* __restrict__ slot_hi, double* __restrict__ slot_lo,
-    unsigned long long* __restrict__ slot_np, unsigned long long* __restrict__ slot_cnt,
-    double* __restrict__ pair_hist, unsigned long long* __restrict__ pair_np) {
+    const long long* __restrict__ bounds, IT* __restrict__ stacks, long long cap,
+    unsigned int* __restrict__ flag, C2Params P, int R, double* __restrict__ slot_hi,
+    double* __restrict__ slot_lo, unsigned long long* __restrict__ slot_np,
+    unsigned long long* __restrict__ slot_cnt, double* __restrict__ pair_hist,
+    unsigned long long* __restrict__ pair_np, unsigned int* __restrict__ job_visits) {
     using Ops = ItemOps<IT>;
     HIP_DYNAMIC_SHARED(double, sh)
     __shared__ unsigned int s_pos;           // children written this round
@@ -754,15 +772,16 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
     if (threadIdx.x < PB_SLOTS) s_tag[threadIdx.x] = -1;
     if (threadIdx.x == 0) {
         s_size = 0;
-        s_snext = njobs * (long long)blockIdx.x / gridDim.x;
-        s_send = njobs * (long long)(blockIdx.x + 1) / gridDim.x;
+        // static shares: jobs [bounds[b], bounds[b+1]), or equal counts without bounds
+        s_snext = bounds ? bounds[blockIdx.x] : njobs * (long long)blockIdx.x / gridDim.x;
+        s_send = bounds ? bounds[blockIdx.x + 1] : njobs * (long long)(blockIdx.x + 1) / gridDim.x;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
     double* hist_all = sh;
     unsigned long long* hnp_all = reinterpret_cast<unsigned long long*>(sh + (size_t)nh * R);
     VisitCtx<IT> X{hist_all + (lane & (R - 1)), hnp_all + (lane & (R - 1)), R, per_slot, s_tag,
-                   pair_hist, pair_np, 0u, 0u, 0u, 0u, lane};
+                   pair_hist, pair_np, 0u, 0u, 0u, 0u, lane, job_visits};
     IT* __restrict__ stk = stacks + (size_t)blockIdx.x * cap;
     const long long bd = blockDim.x;
     for (;;) {
@@ -823,6 +842,147 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
     flush_sums<IT>(hist_all, hnp_all, hcnt, nh, nb, R, per_slot, s_tag, slot_hi, slot_lo,
                    slot_np, slot_cnt, pair_hist, pair_np);
 }
+
+// The cost model's features of job k (a node pair, or a cell of a process2 job), in the
+// units of the binning: the two sizes and counts, the separation, the split ratio, the
+// job's kind, the bin slop and width and whether the positions are 3-d (flat 0).
+template <int C>
+__global__ void k_job_features(const DNode* __restrict__ n1, const DNode* __restrict__ n2,
+                               const Item* __restrict__ jobs, long long njobs, C2Params P,
+                               float* __restrict__ feat) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= njobs) return;
+    const Item it = jobs[k];
+    const DNode c1 = n1[it.i];
+    const bool p2 = it.j < 0;
+    const DNode c2 = p2 ? c1 : n2[it.j];
+    const double d = p2 ? 0. : sqrt(dist_sq<C>(c1, c2));
+    const double s1 = c1.size, s2 = c2.size;
+    const double dm = d > 1e-3 * P.minsep ? d : 1e-3 * P.minsep;
+    float* f = feat + k * CM_NF;
+    f[0] = (float)log2(1. + s1 / P.minsep);
+    f[1] = (float)log2(1. + s2 / P.minsep);
+    f[2] = (float)log2((double)(c1.n > 0 ? c1.n : 1));
+    f[3] = (float)log2((double)(c2.n > 0 ? c2.n : 1));
+    f[4] = (float)log2(dm / P.minsep);
+    f[5] = (float)fmin((s1 + s2) / dm, 8.);
+    f[6] = p2 ? 1.f : 0.f;
+    f[7] = (float)log2(1. + d / P.maxsep);
+    f[8] = (float)log2(P.b > 1e-6 ? P.b : 1e-6);
+    f[9] = (float)log2(P.binsize);
+    f[10] = C == Flat ? 0.f : 1.f;
+}
+
+__global__ void k_jobs_logged(const Item* __restrict__ jobs, long long njobs,
+                              ItemJ* __restrict__ out) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < njobs) out[k] = ItemJ{jobs[k].i, jobs[k].j, (int)k, 0};
+}
+
+// Every stride-th job of a logged run: its features and items processed.
+__global__ void k_log_sample(const float* __restrict__ feat, const unsigned int* __restrict__ vis,
+                             long long njobs, long long stride, long long nout,
+                             float* __restrict__ fout, float* __restrict__ vout) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nout) return;
+    const long long src = k * stride;
+    for (int f = 0; f < CM_NF; ++f) fout[k * CM_NF + f] = feat[src * CM_NF + f];
+    vout[k] = (float)vis[src];
+}
+
+// Predicted work of each job, 2^y - 1 (the model predicts y = log2(1 + visits)).
+__global__ void k_pred_cost(const float* __restrict__ y, long long n, double* __restrict__ c) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        const double v = exp2((double)fminf(fmaxf(y[k], 0.f), 60.f)) - 1.;
+        c[k] = v;
+    }
+}
+
+// Predicted work per tile of COST_TILE jobs (a fixed-order tree sum per tile).
+constexpr int COST_TILE = 4096;
+__global__ __launch_bounds__(256) void k_cost_tiles(const double* __restrict__ c, long long n,
+                                                    double* __restrict__ tiles) {
+    __shared__ double s_sum[256];
+    const long long t0 = (long long)blockIdx.x * COST_TILE;
+    double a = 0.;
+    for (int k = threadIdx.x; k < COST_TILE; k += 256)
+        if (t0 + k < n) a += c[t0 + k];
+    s_sum[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tiles[blockIdx.x] = s_sum[0];
+}
+
+// bounds[w] for w = 1..G-1: the first job of tile tile[w] at which the running sum of
+// predicted work within the tile exceeds rem[w] (the part of w's target in that tile);
+// bounds[0] = 0 and bounds[G] = n.  Targets increase with w, so the bounds do too.
+__global__ void k_cost_bounds(const double* __restrict__ c, long long n, int G,
+                              const long long* __restrict__ tile, const double* __restrict__ rem,
+                              long long* __restrict__ bounds) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w > G) return;
+    if (w == 0) { bounds[0] = 0; return; }
+    if (w == G) { bounds[G] = n; return; }
+    const long long t0 = tile[w] * COST_TILE;
+    const long long t1 = t0 + COST_TILE < n ? t0 + COST_TILE : n;
+    double a = 0.;
+    long long k = t0;
+    while (k < t1) {
+        a += c[k];
+        if (a > rem[w]) break;
+        ++k;
+    }
+    bounds[w] = k;
+}
+
+// Longest-first order of the jobs by predicted y in steps of 1/4 (256 classes): a
+// counting sort.  Key 255 - 4y, so the largest predictions come first.
+__device__ __forceinline__ int lpt_key(float y) {
+    const float q = fminf(fmaxf(y * 4.f, 0.f), 255.f);
+    return 255 - (int)q;
+}
+// Per-workgroup class counts (LDS) added to the global counts.
+__global__ __launch_bounds__(256) void k_lpt_count(const float* __restrict__ y, long long n,
+                                                   unsigned long long* __restrict__ cnt) {
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x)
+        atomicAdd(&h[lpt_key(y[k])], 1u);
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+// Scatter: cursor[key] starts at the class's offset; each workgroup reserves room for
+// its jobs of every class with one atomic per class, then places them (the order
+// within a class does not matter).
+__global__ __launch_bounds__(256) void k_lpt_scatter(const float* __restrict__ y,
+                                                     const Item* __restrict__ jobs, long long n,
+                                                     unsigned long long* __restrict__ cursor,
+                                                     Item* __restrict__ out) {
+    __shared__ unsigned int h[256];
+    __shared__ unsigned long long base[256];
+    const long long k0 = (long long)blockIdx.x * 4096;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < 4096; t += 256)
+        if (k0 + t < n) atomicAdd(&h[lpt_key(y[k0 + t])], 1u);
+    __syncthreads();
+    base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&cursor[threadIdx.x], (unsigned long long)h[threadIdx.x]) : 0;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < 4096; t += 256)
+        if (k0 + t < n) {
+            const int key = lpt_key(y[k0 + t]);
+            const unsigned int r = atomicAdd(&h[key], 1u);
+            out[base[key] + r] = jobs[k0 + t];
+        }
+}
+
 
 // Fixed-order compensated reduction of the per-workgroup slots.
 // One wavefront per output entry (nh sums, nb npairs, NCNT counters, in that order):
@@ -1014,6 +1174,24 @@ struct Corr2Scratch {
     void* pstack = nullptr;             // per-workgroup stacks of the persistent traversal
     size_t pstack_bytes = 0;
     unsigned long long* pctl = nullptr; // persistent traversal: job counter, overflow flag
+    // the learned schedule and the job logs: features, predictions, predicted work and
+    // its prefix, share bounds, sort keys, reordered jobs, logged jobs and their visits
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+        int get(size_t b) {
+            if (b <= bytes) return 0;
+            dev_free(p, bytes);
+            bytes = 0;
+            p = dev_alloc(b ? b : 1);
+            if (!p) return fail("out of device memory for the traversal schedule");
+            bytes = b;
+            return 0;
+        }
+        template <class T> T* as() const { return static_cast<T*>(p); }
+        ~Buf() { dev_free(p, bytes); }
+    };
+    Buf feat, pred, cost, prefix, bounds, keys, jobs2, jobsJ, jvis, logX, logV;
     unsigned long long* boff = nullptr; // two offset arrays, TRAV_SLOTS+1 each
     int nb = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1238,8 +1416,9 @@ static int traverse_stack(Corr2Scratch& S, TravFn<IT> fn, const DNode* n1, const
 
 template <class IT>
 using PersistFn = void (*)(const DNode*, const DNode*, const IT*, long long, unsigned long long*,
-                           int, IT*, long long, unsigned int*, C2Params, int, double*, double*,
-                           unsigned long long*, unsigned long long*, double*, unsigned long long*);
+                           int, const long long*, IT*, long long, unsigned int*, C2Params, int,
+                           double*, double*, unsigned long long*, unsigned long long*, double*,
+                           unsigned long long*, unsigned int*);
 
 template <class IT>
 static PersistFn<IT> pick_persist(int D, int C, bool gm) {
@@ -1251,6 +1430,17 @@ static PersistFn<IT> pick_persist(int D, int C, bool gm) {
     if (C == Flat) TC_TF(GData, Flat); if (C == ThreeD) TC_TF(GData, ThreeD); TC_TF(GData, Sphere);
 #undef TC_TF3
 #undef TC_TF
+}
+
+// The logged form (items carry their job) for the correlation kinds the cost model is
+// trained on: NN, KK and GG with the Euclidean metric.
+static PersistFn<ItemJ> pick_persist_logged(int D, int C) {
+#define TC_TF(DD, CC) return &k_traverse_persist<DD, CC, ItemJ, false>
+#define TC_TF3(KK) if (D == KK) { if (C == Flat) TC_TF(KK, Flat); if (C == ThreeD) TC_TF(KK, ThreeD); TC_TF(KK, Sphere); }
+    TC_TF3(NData) TC_TF3(KData) TC_TF3(GData)
+#undef TC_TF3
+#undef TC_TF
+    return nullptr;
 }
 
 // How a correlation's traversal runs (tc_set_traversal_mode): the persistent kernel
@@ -1267,11 +1457,41 @@ int set_persist_params(int levels, long long min_jobs) {
     return 0;
 }
 int set_traversal_mode(int mode) {
-    if (mode < TRAV_CHUNKED || mode > TRAV_PERSIST_STATIC) return fail("unknown traversal mode");
+    if (mode < TRAV_CHUNKED || mode > TRAV_PERSIST_LEARNED_DYNAMIC)
+        return fail("unknown traversal mode");
     g_trav_mode = mode;
     return 0;
 }
 int traversal_mode() { return g_trav_mode; }
+
+// The cost model of the learned modes, and the job log: when g_log_max > 0 the next
+// persistent traversal counts every job's items and keeps the features and counts of
+// an evenly strided sample of at most g_log_max jobs.
+static const CostModel* g_cost_model = nullptr;
+void set_cost_model(const CostModel* m) { g_cost_model = m; }
+static long long g_log_max = 0;
+static std::vector<float> g_log_X, g_log_V;
+void set_job_log(long long max_samples) {
+    g_log_max = max_samples;
+    g_log_X.clear();
+    g_log_V.clear();
+}
+long long job_log_fetch(float* X, float* V, long long cap) {
+    const long long n = std::min<long long>(cap, (long long)g_log_V.size());
+    if (X) std::copy(g_log_X.begin(), g_log_X.begin() + n * CM_NF, X);
+    if (V) std::copy(g_log_V.begin(), g_log_V.begin() + n, V);
+    return (long long)g_log_V.size();
+}
+
+// Workgroups of a persistent launch: as many as the chip holds at once.
+static int persist_grid(const void* fn, size_t shbytes, int& grid) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    TC_HIP(hipGetDevice(&dev));
+    TC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    TC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TRAV_THREADS, shbytes));
+    grid = std::max(1, std::min(TRAV_SLOTS, std::max(1, per_cu) * ncu));
+    return 0;
+}
 
 // One launch of the persistent traversal over the njobs items at jobs.  depth_sum bounds
 // the chain of splits below a job (the depths of the two trees); each workgroup's stack
@@ -1280,17 +1500,14 @@ int traversal_mode() { return g_trav_mode; }
 template <class IT>
 static int traverse_persist(Corr2Scratch& S, PersistFn<IT> fn, const DNode* n1, const DNode* n2,
                             const IT* jobs, long long njobs, int depth_sum, bool static_part,
-                            const C2Params& P, int R, size_t shbytes, double* pair_hist,
-                            unsigned long long* pair_np, bool& overflow) {
+                            const long long* bounds, const C2Params& P, int R, size_t shbytes,
+                            double* pair_hist, unsigned long long* pair_np,
+                            unsigned int* job_visits, bool& overflow) {
     hipStream_t st = stream();
     overflow = false;
     if (njobs <= 0) return 0;
-    int dev = 0, ncu = 0, per_cu = 0;
-    TC_HIP(hipGetDevice(&dev));
-    TC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    TC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn),
-                                                        TRAV_THREADS, shbytes));
-    const int grid = std::max(1, std::min(TRAV_SLOTS, std::max(1, per_cu) * ncu));
+    int grid = 0;
+    if (int rc = persist_grid(reinterpret_cast<const void*>(fn), shbytes, grid)) return rc;
     const long long cap = 4LL * TRAV_THREADS * (g_persist_levels > 0 ? g_persist_levels : depth_sum + 4);
     const size_t bytes = sizeof(IT) * (size_t)cap * grid;
     if (S.pstack_bytes < bytes) {
@@ -1309,9 +1526,9 @@ static int traverse_persist(Corr2Scratch& S, PersistFn<IT> fn, const DNode* n1, 
     const bool timing = timing_enabled();
     if (timing) TC_HIP(hipEventRecord(S.ev0, st));
     hipLaunchKernelGGL(fn, dim3(grid), dim3(TRAV_THREADS), shbytes, st, n1, n2, jobs, njobs,
-                       S.pctl, static_part ? 1 : 0, static_cast<IT*>(S.pstack), cap,
+                       S.pctl, static_part ? 1 : 0, bounds, static_cast<IT*>(S.pstack), cap,
                        reinterpret_cast<unsigned int*>(S.pctl + 1), P, R, S.slot_hi, S.slot_lo,
-                       S.slot_np, S.slot_cnt, pair_hist, pair_np);
+                       S.slot_np, S.slot_cnt, pair_hist, pair_np, job_visits);
     TC_HIP(hipGetLastError());
     if (timing) TC_HIP(hipEventRecord(S.ev1, st));
     TC_HIP(hipMemcpyAsync(S.h_outcount, S.pctl + 1, sizeof(unsigned long long),
@@ -1324,6 +1541,123 @@ static int traverse_persist(Corr2Scratch& S, PersistFn<IT> fn, const DNode* n1, 
         ks.launches += 1;
     }
     overflow = (*S.h_outcount & 0xffffffffull) != 0;
+    return 0;
+}
+
+// The learned schedules and the job log start from the jobs' features (S.feat).  The
+// learned static mode gives workgroup w the jobs [bounds[w], bounds[w+1]) of equal
+// predicted work; the learned dynamic mode hands out the jobs in order of decreasing
+// predicted work (jobs then points at the reordered list).
+static int schedule_jobs(Corr2Scratch& S, int D, int C, bool gm, const DNode* n1,
+                         const DNode* n2, const Item*& jobs, long long njobs, const C2Params& P,
+                         size_t shbytes, const long long*& bounds) {
+    hipStream_t st = stream();
+    const bool learned = g_trav_mode == TRAV_PERSIST_LEARNED_STATIC ||
+                         g_trav_mode == TRAV_PERSIST_LEARNED_DYNAMIC;
+    if (learned && !g_cost_model)
+        return fail("a learned traversal mode needs a cost model (tc_set_cost_model)");
+    if ((!learned && g_log_max <= 0) || njobs <= 0) return 0;
+    if (S.feat.get(sizeof(float) * CM_NF * njobs)) return 2;
+    const unsigned g = (unsigned)((njobs + 255) / 256);
+    if (C == Flat) hipLaunchKernelGGL(k_job_features<Flat>, dim3(g), dim3(256), 0, st, n1, n2, jobs, njobs, P, S.feat.as<float>());
+    else if (C == ThreeD) hipLaunchKernelGGL(k_job_features<ThreeD>, dim3(g), dim3(256), 0, st, n1, n2, jobs, njobs, P, S.feat.as<float>());
+    else hipLaunchKernelGGL(k_job_features<Sphere>, dim3(g), dim3(256), 0, st, n1, n2, jobs, njobs, P, S.feat.as<float>());
+    TC_HIP(hipGetLastError());
+    if (!learned) return 0;
+    if (S.pred.get(sizeof(float) * njobs)) return 2;
+    if (int rc = cost_model_predict_device(g_cost_model, S.feat.as<float>(), njobs,
+                                           S.pred.as<float>(), st))
+        return rc;
+    if (g_trav_mode == TRAV_PERSIST_LEARNED_STATIC) {
+        int grid = 0;
+        if (int rc = persist_grid(reinterpret_cast<const void*>(pick_persist<Item>(D, C, gm)),
+                                  shbytes, grid))
+            return rc;
+        const long long ntile = (njobs + COST_TILE - 1) / COST_TILE;
+        if (S.cost.get(sizeof(double) * njobs) || S.prefix.get(sizeof(double) * ntile) ||
+            S.bounds.get(sizeof(long long) * (grid + 1)) ||
+            S.keys.get((sizeof(long long) + sizeof(double)) * (grid + 1)))
+            return 2;
+        hipLaunchKernelGGL(k_pred_cost, dim3(g), dim3(256), 0, st, S.pred.as<float>(), njobs,
+                           S.cost.as<double>());
+        hipLaunchKernelGGL(k_cost_tiles, dim3((unsigned)ntile), dim3(256), 0, st,
+                           S.cost.as<double>(), njobs, S.prefix.as<double>());
+        TC_HIP(hipGetLastError());
+        std::vector<double> tiles(ntile);
+        TC_HIP(hipMemcpyAsync(tiles.data(), S.prefix.p, sizeof(double) * ntile,
+                              hipMemcpyDeviceToHost, st));
+        TC_HIP(hipStreamSynchronize(st));
+        // the tile of each share's end and the rest of its target inside that tile
+        std::vector<double> pre(ntile + 1, 0.);
+        for (long long t = 0; t < ntile; ++t) pre[t + 1] = pre[t] + tiles[t];
+        std::vector<long long> tile(grid + 1, 0);
+        std::vector<double> rem(grid + 1, 0.);
+        for (int w = 1; w < grid; ++w) {
+            const double target = pre[ntile] * (double)w / (double)grid;
+            long long t = (long long)(std::upper_bound(pre.begin() + 1, pre.end(), target) -
+                                      (pre.begin() + 1));
+            if (t >= ntile) t = ntile - 1;
+            tile[w] = t;
+            rem[w] = target - pre[t];
+        }
+        long long* dtile = S.keys.as<long long>();
+        double* drem = reinterpret_cast<double*>(dtile + (grid + 1));
+        TC_HIP(hipMemcpyAsync(dtile, tile.data(), sizeof(long long) * (grid + 1),
+                              hipMemcpyHostToDevice, st));
+        TC_HIP(hipMemcpyAsync(drem, rem.data(), sizeof(double) * (grid + 1),
+                              hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_cost_bounds, dim3((grid + 256) / 256), dim3(256), 0, st,
+                           S.cost.as<double>(), njobs, grid, dtile, drem,
+                           S.bounds.as<long long>());
+        TC_HIP(hipGetLastError());
+        TC_HIP(hipStreamSynchronize(st));     // tile and rem are host arrays the copies read
+        bounds = S.bounds.as<long long>();
+        return 0;
+    }
+    if (S.keys.get(sizeof(unsigned long long) * 256) || S.jobs2.get(sizeof(Item) * njobs))
+        return 2;
+    unsigned long long* cnt = S.keys.as<unsigned long long>();
+    TC_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 256, st));
+    hipLaunchKernelGGL(k_lpt_count, dim3((unsigned)std::min<long long>(g, 4096)), dim3(256), 0, st,
+                       S.pred.as<float>(), njobs, cnt);
+    TC_HIP(hipGetLastError());
+    unsigned long long h[256];
+    TC_HIP(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
+    TC_HIP(hipStreamSynchronize(st));
+    unsigned long long off = 0;
+    for (int k = 0; k < 256; ++k) {
+        const unsigned long long c = h[k];
+        h[k] = off;
+        off += c;
+    }
+    TC_HIP(hipMemcpyAsync(cnt, h, sizeof h, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_lpt_scatter, dim3((unsigned)((njobs + 4095) / 4096)), dim3(256), 0, st,
+                       S.pred.as<float>(), jobs, njobs, cnt, S.jobs2.as<Item>());
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipStreamSynchronize(st));     // h is a host array the copy above reads
+    jobs = S.jobs2.as<Item>();
+    return 0;
+}
+
+// An evenly strided sample of at most g_log_max jobs of a logged run, to the host.
+static int keep_job_log(Corr2Scratch& S, long long njobs) {
+    hipStream_t st = stream();
+    if (njobs <= 0) return 0;
+    const long long stride = std::max<long long>(1, (njobs + g_log_max - 1) / g_log_max);
+    const long long nout = (njobs + stride - 1) / stride;
+    if (S.logX.get(sizeof(float) * CM_NF * nout) || S.logV.get(sizeof(float) * nout)) return 2;
+    hipLaunchKernelGGL(k_log_sample, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, st,
+                       S.feat.as<float>(), S.jvis.as<unsigned int>(), njobs, stride, nout,
+                       S.logX.as<float>(), S.logV.as<float>());
+    TC_HIP(hipGetLastError());
+    const size_t o = g_log_V.size();
+    g_log_X.resize((o + nout) * CM_NF);
+    g_log_V.resize(o + nout);
+    TC_HIP(hipMemcpyAsync(g_log_X.data() + o * CM_NF, S.logX.p, sizeof(float) * CM_NF * nout,
+                          hipMemcpyDeviceToHost, st));
+    TC_HIP(hipMemcpyAsync(g_log_V.data() + o, S.logV.p, sizeof(float) * nout,
+                          hipMemcpyDeviceToHost, st));
+    TC_HIP(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -1375,13 +1709,37 @@ int corr2_run(Corr2Scratch& S, int D, int C, const DNode* nodes1, const int* dto
                                               nullptr, nullptr, g_persist_min_jobs, &njobs))
                 return rc;
         bool overflow = false;
-        if (int rc = traverse_persist<Item>(S, pick_persist<Item>(D, C, gm), nodes1,
-                                            autoc ? nodes1 : nodes2,
-                                            static_cast<const Item*>(S.stack), njobs,
-                                            depth1 + (autoc ? depth1 : depth2),
-                                            g_trav_mode == TRAV_PERSIST_STATIC, P, R, shbytes,
-                                            nullptr, nullptr, overflow))
+        const DNode* n2 = autoc ? nodes1 : nodes2;
+        const int dsum = depth1 + (autoc ? depth1 : depth2);
+        const bool stat = g_trav_mode == TRAV_PERSIST_STATIC ||
+                          g_trav_mode == TRAV_PERSIST_LEARNED_STATIC;
+        const Item* jobs = static_cast<const Item*>(S.stack);
+        const long long* bounds = nullptr;
+        if (int rc = schedule_jobs(S, D, C, gm, nodes1, n2, jobs, njobs, P, shbytes, bounds))
             return rc;
+        if (g_log_max > 0) {
+            PersistFn<ItemJ> fl = gm ? nullptr : pick_persist_logged(D, C);
+            if (!fl) return fail("job logs cover NN, KK and GG with the Euclidean metric");
+            if (S.jobsJ.get(sizeof(ItemJ) * std::max<long long>(njobs, 1)) ||
+                S.jvis.get(sizeof(unsigned int) * std::max<long long>(njobs, 1)))
+                return 2;
+            const unsigned g = (unsigned)((njobs + 255) / 256);
+            if (njobs > 0)
+                hipLaunchKernelGGL(k_jobs_logged, dim3(g), dim3(256), 0, st, jobs, njobs,
+                                   S.jobsJ.as<ItemJ>());
+            TC_HIP(hipMemsetAsync(S.jvis.p, 0, sizeof(unsigned int) * njobs, st));
+            if (int rc = traverse_persist<ItemJ>(S, fl, nodes1, n2, S.jobsJ.as<ItemJ>(), njobs,
+                                                 dsum, stat, bounds, P, R, shbytes, nullptr,
+                                                 nullptr, S.jvis.as<unsigned int>(), overflow))
+                return rc;
+            if (!overflow)
+                if (int rc = keep_job_log(S, njobs)) return rc;
+        } else {
+            if (int rc = traverse_persist<Item>(S, pick_persist<Item>(D, C, gm), nodes1, n2, jobs,
+                                                njobs, dsum, stat, bounds, P, R, shbytes, nullptr,
+                                                nullptr, nullptr, overflow))
+                return rc;
+        }
         if (overflow) {      // a stack would have overflowed: start again, chunked
             g_persist_fallbacks.fetch_add(1);
             if (zero_slots(S, nb)) return 2;

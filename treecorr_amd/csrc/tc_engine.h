@@ -71,10 +71,32 @@ int kmeans_run_noalloc(double* centers, int npatch, int dim, const double* x, co
 int quick_assign(const double* centers, int npatch, const double* x, const double* y,
                  const double* z, long long n, int* patch);
 
+// The learned cost model of the traversal's jobs (tc_cost.hip): a perceptron of
+// CM_NF inputs, two hidden layers of CM_H ReLU units and one output, FP32.
+constexpr int CM_NF = 11, CM_H = 64;
+constexpr int CM_NP = CM_H * CM_NF + CM_H + CM_H * CM_H + CM_H + CM_H + 1;
+struct CostModel;
+CostModel* cost_model_new(unsigned long long seed);
+void cost_model_delete(CostModel* m);
+int cost_model_params(CostModel* m, float* host, bool set);
+int cost_model_predict(CostModel* m, const float* X, long long n, float* yhat);
+int cost_model_predict_device(const CostModel* m, const float* dX, long long n, float* dyhat,
+                              hipStream_t st);
+int cost_model_grad(CostModel* m, const float* X, const float* Y, long long n, float* grad,
+                    double* loss);
+int cost_model_train(CostModel* m, const float* X, const float* Y, long long n, int steps,
+                     int batch, float lr, double* loss);
+
 int set_traversal_chunk(long long items);   // 0 restores the default
 // Traversal modes: chunked launches driven from the host, or one persistent launch whose
 // workgroups take jobs by a shared counter or from static equal-count shares.
-enum { TRAV_CHUNKED = 0, TRAV_PERSIST_DYNAMIC = 1, TRAV_PERSIST_STATIC = 2 };
+// The learned modes take the cost model's predictions: static shares of equal predicted
+// work, or jobs handed out by the counter in order of decreasing predicted work.
+enum { TRAV_CHUNKED = 0, TRAV_PERSIST_DYNAMIC = 1, TRAV_PERSIST_STATIC = 2,
+       TRAV_PERSIST_LEARNED_STATIC = 3, TRAV_PERSIST_LEARNED_DYNAMIC = 4 };
+void set_cost_model(const CostModel* m);
+void set_job_log(long long max_samples);     // 0: off
+long long job_log_fetch(float* X, float* V, long long cap);   // returns the samples kept
 int set_traversal_mode(int mode);
 int traversal_mode();
 long long persist_fallbacks();
