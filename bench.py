@@ -662,6 +662,219 @@ def run_patch_workload(args):
     print(json.dumps(res))
 
 
+# ---------------------------------------------------------------------------
+# Learned scheduler workload: KK of 20M points at (RA, Dec, r); the cost model of the
+# persistent traversal trained on job logs of seeded runs shaped like the other workloads
+# ---------------------------------------------------------------------------
+def corr_step(L, D, C, dev, n, bn, out, min_top=None):
+    """Tree build from device columns + one 2-point auto correlation through the C ABI;
+    returns the work counters (out receives the raw sums)."""
+    from treecorr_amd import _lib
+    fh = ctypes.c_void_p()
+    _lib.check(L.tc_field_create_device(D, C, dev['x'], dev['y'], dev.get('z'), dev.get('k'),
+                                        dev.get('g2'), dev.get('w'), None, n, bn['min_size'],
+                                        bn['max_size'], 2, 0, 0,
+                                        bn['min_top'] if min_top is None else min_top,
+                                        bn['max_top'], ctypes.byref(fh)))
+    ch = ctypes.c_void_p()
+    arr = [out[k] for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr', 'weight', 'npairs')]
+    for a in arr:
+        a[:] = 0
+    _lib.check(L.tc_corr2_create(D, D, 0, bn['min_sep'], bn['max_sep'], bn['nbins'],
+                                 bn['bin_size'], bn['b'], bn['a'], -1.7976931348623157e308,
+                                 1.7976931348623157e308, 0., 0., 0.,
+                                 *[_lib.dptr(a) for a in arr], ctypes.byref(ch)))
+    _lib.check(L.tc_corr2_process_auto(ch, fh, 0, 0, 1))
+    c = (ctypes.c_longlong * 4)()
+    _lib.check(L.tc_corr2_counters(ch, c))
+    L.tc_corr2_destroy(ch)
+    L.tc_field_destroy(fh)
+    return dict(visits=c[0], accepted=c[1], leafpairs=c[2], refined=c[3])
+
+
+def log_binning(nbins, min_sep, max_sep, bin_slop, angle_slop=0.1, min_top=4, max_top=10):
+    bin_size = math.log(max_sep / min_sep) / nbins
+    b = bin_size * bin_slop
+    b1 = min(angle_slop, b)
+    return dict(nbins=nbins, min_sep=min_sep, max_sep=max_sep, bin_size=bin_size, b=b,
+                a=angle_slop, min_size=min_sep * b1 / (2 + 3 * b1), max_size=max_sep * b1,
+                min_top=min_top, max_top=max_top)
+
+
+def lightcone_catalog(n, seed, ra_deg=30., dec_deg=30., rmin=500., rmax=2000., cell=8.):
+    """n points at (RA, Dec, r): RA uniform over ra_deg, Dec uniform on the sphere over
+    dec_deg (a cap about the x axis), r in [rmin, rmax]; the density a log-normal field
+    (white noise on a grid of `cell` Mpc smoothed by a 2-cell Gaussian, sigma_ln 1).
+    kappa = 0.02 x the smoothed Gaussian field at the point + N(0, 0.05^2).  Returns
+    x, y, z (Mpc) and kappa."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    ha, hd = math.radians(ra_deg) / 2, math.radians(dec_deg) / 2
+    lo = np.array([rmin * math.cos(ha) * math.cos(hd), -rmax * math.sin(ha), -rmax * math.sin(hd)])
+    hi = np.array([rmax, rmax * math.sin(ha), rmax * math.sin(hd)])
+    ng = np.maximum(1, np.ceil((hi - lo) / cell).astype(int))
+    g = gaussian_filter(rng.standard_normal(tuple(ng)), 2.0, mode='wrap')
+    g /= g.std()
+    rho = np.exp(g - 0.5).ravel()
+    # the cap fills about 43% of its box: draw 2.6 n candidates, keep those inside
+    xs, ys, zs, gs = [], [], [], []
+    kept = 0
+    while kept < n:
+        m = int(2.6 * (n - kept)) + 1000
+        cells = rng.choice(rho.size, size=m, p=rho / rho.sum())
+        ix, iy, iz = np.unravel_index(cells, tuple(ng))
+        x = lo[0] + (ix + rng.random(m)) * cell
+        y = lo[1] + (iy + rng.random(m)) * cell
+        z = lo[2] + (iz + rng.random(m)) * cell
+        r = np.sqrt(x * x + y * y + z * z)
+        ra = np.arctan2(y, x)
+        sd = z / r
+        ok = (r >= rmin) & (r < rmax) & (np.abs(ra) < ha) & (np.abs(sd) < math.sin(hd))
+        xs.append(x[ok]); ys.append(y[ok]); zs.append(z[ok]); gs.append(g.ravel()[cells[ok]])
+        kept += int(ok.sum())
+    x, y, z, gv = (np.concatenate(a)[:n] for a in (xs, ys, zs, gs))
+    kappa = 0.02 * gv + rng.normal(0, 0.05, n)
+    return x, y, z, kappa
+
+
+def training_runs():
+    """Seeded runs shaped like the other workloads (seeds of their own): NN flat, GG
+    sphere, KK 3-d log-normal box, KK flat clumpy field.  Yields (D, C, columns, binning)."""
+    rng = np.random.default_rng(2026_12_03 + 50)
+    n1 = 100_000
+    yield (0, 1, dict(x=rng.uniform(0, 1000., n1), y=rng.uniform(0, 1000., n1), w=np.ones(n1)),
+           log_binning(20, 1., 100., 0.0))
+    ra, dec, g1, g2, w = gg_sphere_catalog(500_000, seed=2026_12_03 + 51)
+    x, y, z = radec_to_xyz(ra, dec)
+    yield (4, 3, dict(x=x, y=y, z=z, k=g1, g2=g2, w=w), gg_binning())
+    x, y, z = lognormal_box(1_000_000, 2026_12_03 + 52, ngrid=128)
+    k = np.random.default_rng(53).normal(0, 0.03, len(x))
+    yield (1, 2, dict(x=x, y=y, z=z, k=k, w=np.ones(len(x))), log_binning(30, 1., 150., 0.05, 1.0))
+    x, y, g1, g2, w = mp_flat_catalog(500_000, seed=2026_12_03 + 54)
+    yield (1, 1, dict(x=x, y=y, k=g1, w=w), log_binning(20, 5., 100., 0.1))
+
+
+def run_learned_workload(args):
+    from treecorr_amd import _lib, cost
+    L = _lib.lib()
+    hip = Hip()
+    res = {"workload": "learned scheduler", "metric": "pairs/s (KK 3-d, 20M points at RA/Dec/r)"}
+    # 1. job logs of the training runs (two job granularities each)
+    Xs, ys = [], []
+    t0 = time.perf_counter()
+    for D, C, cols, bn in training_runs():
+        n = len(cols['x'])
+        dev = {k: hip.upload(np.ascontiguousarray(v, np.float64)) for k, v in cols.items()}
+        out = {k: np.zeros(bn['nbins']) for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr',
+                                                  'meanlogr', 'weight', 'npairs')}
+        for mj in (1 << 18, 1 << 22):
+            _lib.check(L.tc_set_persist_params(0, mj))
+            cost.set_mode(cost.DYNAMIC)
+            cost.start_log(150_000)
+            corr_step(L, D, C, dev, n, bn, out)
+            X, v = cost.stop_log()
+            Xs.append(X)
+            ys.append(cost.target(v))
+        for p in dev.values():
+            hip.free(p)
+        print(f"# logged D={D} C={C} n={n}: {sum(len(a) for a in ys)} jobs so far",
+              file=sys.stderr, flush=True)
+    _lib.check(L.tc_set_persist_params(0, 0))
+    X = np.concatenate(Xs)
+    y = np.concatenate(ys)
+    t1 = time.perf_counter()
+    model = cost.CostModel(seed=7)
+    hip.sync()
+    t2 = time.perf_counter()
+    loss = model.train(X, y, steps=args.train_steps, batch=4096, lr=2e-3)
+    hip.sync()
+    t3 = time.perf_counter()
+    res["training"] = {"samples": int(len(y)), "runs": "NN flat 100k, GG sphere 500k, "
+                       "KK 3-d log-normal 1M, KK flat clumpy 500k (seeds of their own); jobs "
+                       "logged at 2^18 and 2^22 jobs per run",
+                       "logging_s": t1 - t0, "train_s": t3 - t2, "steps": args.train_steps,
+                       "batch": 4096, "mse_log2": loss, "target_var": float(np.var(y))}
+    print(f"# trained: mse {loss:.3f} (target variance {np.var(y):.3f}) in {t3 - t2:.2f} s",
+          file=sys.stderr, flush=True)
+    # 2. the 20M-point catalog
+    x, yy, z, kap = lightcone_catalog(args.lc_n, 2026_12_03 + 5)
+    bn = log_binning(40, 0.5, 200., 0.1, 0.1)
+    dev = {k: hip.upload(v) for k, v in dict(x=x, y=yy, z=z, k=kap, w=np.ones(len(x))).items()}
+    n = len(x)
+    out = {k: np.zeros(bn['nbins']) for k in ('xi0', 'xi1', 'xi2', 'xi3', 'meanr', 'meanlogr',
+                                              'weight', 'npairs')}
+    print(f"# light-cone catalog: {n} points", file=sys.stderr, flush=True)
+    # held-out prediction error: a logged run on this catalog
+    print("# logged run on the light cone ...", file=sys.stderr, flush=True)
+    cost.set_mode(cost.DYNAMIC)
+    cost.start_log(2_000_000)
+    tl = time.perf_counter()
+    corr_step(L, 1, 2, dev, n, bn, out)
+    Xh, vh = cost.stop_log()
+    print(f"# logged run on the light cone: {time.perf_counter() - tl:.2f} s, "
+          f"npairs {out['npairs'].sum():.4g}", file=sys.stderr, flush=True)
+    pred = model.predict(Xh)
+    th = cost.target(vh)
+    resid = pred - th
+    res["prediction"] = {"jobs_sampled": int(len(th)), "r2": float(1 - np.var(resid) / np.var(th)),
+                         "corr": float(np.corrcoef(pred, th)[0, 1]),
+                         "mean_abs_err_log2": float(np.mean(np.abs(resid))),
+                         "median_abs_err_log2": float(np.median(np.abs(resid)))}
+    # 3. the modes, timed (tree build included, as each is one process call)
+    model.install()
+    modes = [(cost.STATIC, "static equal-count shares (fallback and judge)"),
+             (cost.LEARNED_STATIC, "static shares of equal predicted work"),
+             (cost.DYNAMIC, "jobs by a shared counter, natural order"),
+             (cost.LEARNED_DYNAMIC, "jobs by a shared counter, longest predicted first"),
+             (cost.CHUNKED, "chunked launches driven from the host")]
+    ref = None
+    res["modes"] = {}
+    for mode, what in modes:
+        cost.set_mode(mode)
+        for i in range(args.warmup):
+            corr_step(L, 1, 2, dev, n, bn, out)
+            print(f"# mode {mode}: warm-up {i} done", file=sys.stderr, flush=True)
+        L.tc_reset_kernel_stats()
+        L.tc_set_timing(1)
+        hip.sync()
+        ta = time.perf_counter()
+        for i in range(args.steps):
+            cnt = corr_step(L, 1, 2, dev, n, bn, out)
+            print(f"# mode {mode}: step {i} done", file=sys.stderr, flush=True)
+        hip.sync()
+        tb = time.perf_counter()
+        L.tc_set_timing(0)
+        ms = ctypes.c_double()
+        la = ctypes.c_longlong()
+        _lib.check(L.tc_kernel_stats(0, ctypes.byref(ms), ctypes.byref(la)))
+        got = {k: v.copy() for k, v in out.items()}
+        if ref is None:
+            ref = got
+        ent = {"what": what, "ms_per_step": 1e3 * (tb - ta) / args.steps,
+               "traversal_ms_per_step": ms.value / args.steps,
+               "pairs_per_s": float(got['npairs'].sum()) / ((tb - ta) / args.steps),
+               "npairs_equal_to_static": bool(np.array_equal(got['npairs'], ref['npairs'])),
+               "sums_normwise_vs_static": max(normwise(got[k], ref[k]) for k in
+                                              ('xi0', 'meanr', 'meanlogr', 'weight'))}
+        res["modes"][str(mode)] = ent
+        print(f"# mode {mode}: {ent['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
+    cost.set_mode(cost.DYNAMIC)
+    cost.clear_model()
+    res["counters"] = cnt
+    st = res["modes"][str(cost.STATIC)]["traversal_ms_per_step"]
+    res["learned_static_speedup_over_static"] = st / res["modes"][str(cost.LEARNED_STATIC)]["traversal_ms_per_step"]
+    res["learned_dynamic_speedup_over_dynamic"] = (
+        res["modes"][str(cost.DYNAMIC)]["traversal_ms_per_step"] /
+        res["modes"][str(cost.LEARNED_DYNAMIC)]["traversal_ms_per_step"])
+    res["config"] = {"workload": f"KK 3-d auto, {n} points at (RA, Dec, r): RA, Dec over a 30 x 30 "
+                                 f"deg cap, r 500-2000 Mpc from a log-normal density field, "
+                                 f"kappa = smoothed field + N(0, 0.05^2); 40 log bins 0.5-200 Mpc, "
+                                 f"bin_slop 0.1; tree build included", "n": n}
+    res["dtype"] = "f64 (model FP32)"
+    res["data"] = "synthetic (seeded)"
+    print(json.dumps(res))
+
+
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
     path = os.path.join(ROOT, 'profiles', name)
@@ -683,9 +896,14 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run the plain copy kernel over MIB MiB (HBM counter calibration)')
-    ap.add_argument('--workload', choices=('gg_sphere', 'patch_cov_3d'), default='gg_sphere',
+    ap.add_argument('--workload', choices=('gg_sphere', 'patch_cov_3d', 'learned_kk3d'),
+                    default='gg_sphere',
                     help='gg_sphere: the headline GG workload; patch_cov_3d: NN data x randoms '
-                         'and KK auto over k-means patches with a jackknife covariance')
+                         'and KK auto over k-means patches with a jackknife covariance; '
+                         'learned_kk3d: the traversal cost model trained on job logs, then '
+                         'KK of 20M points at RA/Dec/r in every traversal mode')
+    ap.add_argument('--lc-n', type=int, default=20_000_000)
+    ap.add_argument('--train-steps', type=int, default=4000)
     ap.add_argument('--patch-nd', type=int, default=4_000_000)
     ap.add_argument('--patch-nr', type=int, default=40_000_000)
     ap.add_argument('--npatch', type=int, default=100)
@@ -710,6 +928,9 @@ def main():
         sys.exit(2)
     if args.workload == 'patch_cov_3d':
         run_patch_workload(args)
+        return
+    if args.workload == 'learned_kk3d':
+        run_learned_workload(args)
         return
     from treecorr_amd import _lib
     L = _lib.lib()

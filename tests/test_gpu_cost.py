@@ -78,7 +78,7 @@ def test_gpu_adam_steps_match_reference(gpu):
     m.params = p
     X, y = _data(rng, 500)
     batch, lr, seed = 256, 1e-2, 9
-    m.train(X, y, steps=3, batch=batch, lr=lr)
+    m.train(X, y, steps=3, batch=batch, lr=lr, standardize=False)
     q = p.astype(np.float64)
     mm = np.zeros(R.NP)
     vv = np.zeros(R.NP)
@@ -88,6 +88,30 @@ def test_gpu_adam_steps_match_reference(gpu):
         q, mm, vv = R.adam(q, g, mm, vv, t + 1, lr)
     got = m.params.astype(np.float64)
     assert np.max(np.abs(got - q)) < 2e-4 * np.max(np.abs(q)), np.max(np.abs(got - q))
+
+
+@pytest.mark.gpu
+def test_gpu_standardized_training_reads_raw_features(gpu):
+    """Training on standardized features, folded into the first layer afterwards, gives
+    the same predictions on raw features as the unfolded network on standardized ones."""
+    from treecorr_amd import cost
+    rng = np.random.default_rng(14)
+    m = cost.CostModel(seed=4)
+    X, y = _data(rng, 5000)
+    X = X * np.linspace(0.5, 20., R.NF, dtype=np.float32) + np.linspace(-30., 30., R.NF,
+                                                                         dtype=np.float32)
+    m.train(X, y, steps=200, batch=1024, lr=3e-3)
+    p = m.params.astype(np.float64)
+    W1 = p[R.O_W1:R.O_B1].reshape(R.H, R.NF) * m._sd
+    b1 = p[R.O_B1:R.O_W2] + p[R.O_W1:R.O_B1].reshape(R.H, R.NF) @ m._mu
+    q = p.copy()
+    q[R.O_W1:R.O_B1] = W1.ravel()
+    q[R.O_B1:R.O_W2] = b1
+    ref, _ = R.forward(q, (X - m._mu) / m._sd)
+    np.testing.assert_allclose(m.predict(X), ref, rtol=1e-4, atol=1e-4 * np.max(np.abs(ref)))
+    # a second call trains on from the same scaling
+    m.train(X, y, steps=50, batch=1024, lr=3e-3)
+    assert m._folded
 
 
 @pytest.mark.gpu

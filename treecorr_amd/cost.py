@@ -89,14 +89,46 @@ class CostModel:
                                                   ctypes.byref(loss)))
         return g, loss.value
 
-    def train(self, X, y, steps=2000, batch=4096, lr=2e-3):
-        """Adam on mini-batches; returns the mean squared error over all samples."""
+    def train(self, X, y, steps=2000, batch=4096, lr=2e-3, standardize=True):
+        """Adam on mini-batches; returns the mean squared error over all samples.
+
+        With standardize (the default), the GPU trains on features shifted and scaled to
+        mean 0 and variance 1 (the statistics of the first training set), and the scaling
+        is then folded into the first layer (W1 / sd, b1 - W1 mu / sd), so that the model
+        reads the traversal's raw features."""
         X = _f32(X).reshape(-1, self.nfeatures)
         y = _f32(y)
+        if standardize:
+            if getattr(self, '_mu', None) is None:
+                self._mu = X.astype(np.float64).mean(0)
+                self._sd = X.astype(np.float64).std(0) + 1e-6
+            self._fold(unfold=True)
+            X = _f32((X - self._mu) / self._sd)
         loss = ctypes.c_double()
         _lib.check(_lib.lib().tc_cost_model_train(self._h, _ptr(X), _ptr(y), len(X), steps, batch,
                                                    ctypes.c_float(lr), ctypes.byref(loss)))
+        if standardize:
+            self._fold(unfold=False)
         return loss.value
+
+    def _fold(self, unfold):
+        """Fold the feature scaling into the first layer, or take it out again."""
+        if getattr(self, '_mu', None) is None or getattr(self, '_folded', False) != unfold:
+            return
+        nf, h = self.nfeatures, 64
+        p = self.params.astype(np.float64)
+        W1 = p[:h * nf].reshape(h, nf)
+        b1 = p[h * nf:h * nf + h]
+        if unfold:      # W = W1 sd, b = b1 + W1 mu
+            b1 = b1 + W1 @ self._mu
+            W1 = W1 * self._sd
+        else:           # W1 = W / sd, b1 = b - W mu / sd
+            W1 = W1 / self._sd
+            b1 = b1 - W1 @ self._mu
+        p[:h * nf] = W1.ravel()
+        p[h * nf:h * nf + h] = b1
+        self.params = p.astype(np.float32)
+        self._folded = not unfold
 
     def install(self):
         """Use this model in the learned traversal modes."""
