@@ -961,20 +961,27 @@ def main():
     ms = ctypes.c_double()
     launches = ctypes.c_longlong()
     _lib.check(L.tc_kernel_stats(0, ctypes.byref(ms), ctypes.byref(launches)))
+    pms, plaunch = kernel_ms(L, 4)                    # the persistent launches alone
+    pc = (ctypes.c_longlong * 4)()
+    _lib.check(L.tc_persist_counters(pc))             # the last step's persistent launch
     npairs = float(out['npairs'].sum())
     ms_per_step = 1e3 * (t1 - t0) / args.steps
     value = npairs / ((t1 - t0) / args.steps)
     flops = (OPS_VISIT * cnt['visits'] + OPS_ACCEPT_GG_SPHERE * cnt['accepted'] +
              OPS_REFINED * cnt['refined'])
-    kern_s = ms.value * 1e-3 / args.steps            # traversal kernel seconds per step
-    achieved = flops / kern_s / 1e12 if kern_s > 0 else 0.
+    # roofline of the dominant kernel, k_traverse_persist: its own counters (the chunked
+    # split before it is counted apart) over its average launch time
+    flops_p = OPS_VISIT * pc[0] + OPS_ACCEPT_GG_SPHERE * pc[1] + OPS_REFINED * pc[3]
+    avg_launch_s = pms * 1e-3 / max(1, plaunch)
+    achieved = flops_p / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.
     launches_per_step = launches.value / args.steps
-    avg_launch_s = ms.value * 1e-3 / max(1, launches.value)
-    traffic = read_traffic('k_traverse_gg_sphere_2M.txt')
+    traffic = read_traffic('k_traverse_persist_gg_sphere_2M.txt')
     gpu_out = {k: v.copy() for k, v in out.items()}
     # the plain form: the tree built level by level in reference order (no speculative
-    # top levels); the traversal is the same
+    # top levels) and the traversal in chunked launches driven from the host (one
+    # read-back per launch)
     L.tc_set_tree_speculation(0)
+    L.tc_set_traversal_mode(0)
     hip.sync()
     tp0 = time.perf_counter()
     for i in range(args.steps):
@@ -982,6 +989,7 @@ def main():
     hip.sync()
     tp1 = time.perf_counter()
     L.tc_set_tree_speculation(1)
+    L.tc_set_traversal_mode(1)
     plain_s = (tp1 - tp0) / args.steps
     res = {
         "metric": "pairs/s (GG xi+/xi- 2M galaxies, bin_slop=0.1) + GGG multipole 500k",
@@ -1003,19 +1011,24 @@ def main():
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic,
-                     "kernel": "k_traverse<GData,Sphere>",
-                     "flops_per_launch": flops / max(1., launches_per_step),
+                     "kernel": "k_traverse_persist<GData,Sphere,Item>",
+                     "flops_per_launch": flops_p,
+                     "counters_per_launch": dict(zip(('visits', 'accepted', 'leafpairs',
+                                                      'refined'), (int(v) for v in pc))),
                      "avg_launch_ms": 1e3 * avg_launch_s,
-                     "launches_per_step": launches_per_step,
+                     "launches_per_step": plaunch / args.steps,
                      "hbm_GBs_from_counters": traffic / avg_launch_s / 1e9 if traffic else None,
                      "note": "compute bound on the FP64 vector ALUs; priced at the 78.6 TF/s "
                              "FP64 peak of MI355X"},
         "plain_gpu": {"ms_per_step": 1e3 * plain_s, "value": npairs / plain_s,
                       "form": "tree built level by level in reference order (no speculative "
-                              "top levels), same traversal"},
+                              "top levels); traversal in chunked launches with one host "
+                              "read-back each (no persistent launch)"},
         "tuned_over_plain": plain_s / ((t1 - t0) / args.steps),
         "counters": cnt,
         "traversal_ms_per_step": ms.value / args.steps,
+        "traversal_launches_per_step": launches_per_step,
+        "whole_traversal_tflops": flops / (ms.value * 1e-3 / args.steps) / 1e12,
     }
     if not args.no_cpu_baseline:
         print("# CPU baseline (the reference engine, all host cores)", file=sys.stderr,

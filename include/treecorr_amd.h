@@ -195,6 +195,9 @@ int tc_set_traversal_mode(int mode);
 /* Persistent traversals redone with chunked launches because a workgroup's stack would
  * have overflowed (expected 0).  No reference counterpart. */
 long long tc_traversal_fallbacks(void);
+/* Work counters (as tc_corr2_counters) of the last persistent launch alone, without the
+ * chunked split before it; for roofline accounting.  No reference counterpart. */
+int tc_persist_counters(long long* counters4);
 /* The learned cost model of the persistent traversal's jobs (no reference counterpart:
  * TreeCorr hands out its top-level pairs with OpenMP's schedule(dynamic),
  * src/Corr2.cpp:158-203).  A perceptron of three FP32 layers (nfeatures inputs, two
@@ -298,8 +301,9 @@ int tc_set_num_threads(int n);
 int tc_get_num_threads(int* n);
 
 /* Kernel timing with HIP events on the library stream (bench/profiling only;
- * no reference counterpart).  which: 0 = 2-pt traversal, 1 = tree build,
- * 2 = multipole Gn, 3 = zeta contraction. */
+ * no reference counterpart).  which: 0 = 2-pt traversal (all its launches), 1 = tree
+ * build, 2 = multipole Gn, 3 = zeta contraction, 4 = the persistent traversal launches
+ * alone. */
 int tc_set_timing(int on);
 /* Streams n doubles from src to dst (device pointers) with a plain copy kernel;
  * used only to calibrate HBM traffic counters (FETCH_SIZE/WRITE_SIZE). */

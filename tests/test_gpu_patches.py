@@ -139,20 +139,35 @@ def test_gpu_patch_cross_pairs_match_oracle(gpu, oracle_lib):
 
 
 @pytest.mark.gpu
-def test_gpu_patch_pairs_small_chunks(gpu, oracle_lib):
-    """Batched patch pairs across many traversal launches."""
+@pytest.mark.parametrize('mode,levels,min_jobs,chunk',
+                         [(0, 0, 0, 3000), (1, 0, 2, 0), (1, 0, 2000, 3000), (2, 0, 2, 0),
+                          (1, 1, 2, 3000)],
+                         ids=['chunked', 'persistent', 'persistent-split', 'static',
+                              'overflow'])
+def test_gpu_patch_pairs_traversal_modes(gpu, oracle_lib, mode, levels, min_jobs, chunk):
+    """Batched patch pairs across many chunked launches, in one persistent launch (from
+    the initial node pairs or after a chunked split; every workgroup meets many patch
+    pairs and recycles its LDS histograms), from static shares, and through the fallback
+    of a persistent launch whose stacks are too small."""
     from treecorr_amd import _lib
     O = oracle_lib
+    L = _lib.lib()
     ng, min_top = 3, 4
     cat, scale = C.make_catalog(THREED, 5000, 331, 0.3)
     labels = grid_labels(cat['x'], cat['y'], scale, ng)
     bn = C.binning(scale, 10, 0.01, 0.4, 0.1, 0.1)
     gg = corr_obj(gpu, 4, bn, 0.1, 0.1, min_top)
-    _lib.check(_lib.lib().tc_set_traversal_chunk(3000))
+    _lib.check(L.tc_set_traversal_chunk(chunk))
+    _lib.check(L.tc_set_traversal_mode(mode))
+    _lib.check(L.tc_set_persist_params(levels, min_jobs))
+    fb0 = L.tc_traversal_fallbacks()
     try:
         gg.process(front_catalog(gpu, cat, THREED, labels), finalize=False)
     finally:
-        _lib.check(_lib.lib().tc_set_traversal_chunk(0))
+        _lib.check(L.tc_set_traversal_chunk(0))
+        _lib.check(L.tc_set_traversal_mode(1))
+        _lib.check(L.tc_set_persist_params(0, 0))
+    assert L.tc_traversal_fallbacks() - fb0 == (1 if levels else 0)
     for (i, j), res in gg.results.items():
         ref = oracle_raw(O, 4, THREED, sub(cat, labels == i),
                          None if i == j else sub(cat, labels == j), bn, min_top)

@@ -49,6 +49,7 @@ SIGNATURES = {
     'tc_set_tree_speculation': (_i, [_i]),
     'tc_set_traversal_mode': (_i, [_i]),
     'tc_traversal_fallbacks': (_ll, []),
+    'tc_persist_counters': (_i, [ctypes.POINTER(_ll)]),
     'tc_set_persist_params': (_i, [_i, _ll]),
     'tc_cost_model_create': (_i, [ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
     'tc_cost_model_destroy': (_i, [_vp]),

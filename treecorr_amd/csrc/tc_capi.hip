@@ -624,6 +624,11 @@ int tc_set_traversal_mode(int mode) {
     return tc::set_traversal_mode(mode);
 }
 long long tc_traversal_fallbacks(void) { return tc::persist_fallbacks(); }
+int tc_persist_counters(long long* counters4) {
+    if (!counters4) return tc::fail("counters4 is required");
+    tc::persist_counters(counters4);
+    return 0;
+}
 struct tc_cost_model {
     tc::CostModel* m = nullptr;
     ~tc_cost_model() { tc::cost_model_delete(m); }

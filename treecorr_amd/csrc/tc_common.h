@@ -101,7 +101,7 @@ void dev_free(void* p, size_t bytes);
 struct KernelStat { double ms = 0; long long launches = 0; };
 bool timing_enabled();
 KernelStat& kernel_stat(int which);
-enum { KS_TRAVERSE2 = 0, KS_TREE = 1, KS_MULTIPOLE = 2, KS_ZETA = 3, KS_COUNT = 4 };
+enum { KS_TRAVERSE2 = 0, KS_TREE = 1, KS_MULTIPOLE = 2, KS_ZETA = 3, KS_PERSIST = 4, KS_COUNT = 5 };
 
 }  // namespace tc
 

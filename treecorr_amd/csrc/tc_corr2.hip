@@ -40,6 +40,8 @@ constexpr int TRAV_SLOTS = 2048;          // persistent workgroups = accumulatio
 // items, 4.3 GB for 8-byte items).  2^26 measured 3.5% faster than 2^25 on the 2M GG
 // workload (half the launches, less copying onto the stack); 2^27 gained 1% more.
 constexpr long long TRAV_CHUNK = 1LL << 26;
+// Persistent launch control words: job counter, overflow flag, the launch's counters.
+constexpr int PCTL_WORDS = 2 + 4;
 // Jobs a persistent launch wants to start from: the initial node pairs are split
 // breadth first by chunked launches until there are this many, so that no job's subtree
 // is more than a small part of one workgroup's share (measured on GG 2M: 2^20 jobs
@@ -558,6 +560,38 @@ __device__ __forceinline__ void visit_round(VisitCtx<IT>& X, const C2Params& P, 
     }
 }
 
+// Adds each used patch-pair histogram of the workgroup to that pair's sums in HBM.  A
+// pair's sums are a (hi, lo) pair of arrays: the exact rounding error of every addition
+// into hi (TwoSum of the value it replaced and the addend) goes into lo.
+__device__ __forceinline__ void flush_pair_hists(const double* hist_all,
+                                                 const unsigned long long* hnp_all, int nh,
+                                                 int nb, int R, size_t per_slot,
+                                                 const int* s_tag, double* __restrict__ pair_hist,
+                                                 unsigned long long* __restrict__ pair_np) {
+    for (int sl = 0; sl < PB_SLOTS; ++sl) {
+        const int pp = s_tag[sl];
+        if (pp < 0) continue;
+        const double* hs = hist_all + sl * per_slot;
+        const unsigned long long* ns = hnp_all + sl * per_slot;
+        double* ph = pair_hist + (size_t)pp * 2 * nh;
+        for (int t = threadIdx.x; t < nh; t += blockDim.x) {
+            double v = 0.;
+            for (int c = 0; c < R; ++c) v += hs[(size_t)t * R + c];
+            if (v == 0.) continue;
+            const double old = atomicAdd(&ph[t], v);
+            const double sum = old + v;
+            const double bv = sum - old;
+            const double err = (old - (sum - bv)) + (v - bv);
+            if (err != 0.) atomicAdd(&ph[nh + t], err);
+        }
+        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+            unsigned long long v = 0;
+            for (int c = 0; c < R; ++c) v += ns[(size_t)t * R + c];
+            if (v) atomicAdd(&pair_np[(size_t)pp * nb + t], v);
+        }
+    }
+}
+
 // End of a traversal launch: the workgroup's visit counters and LDS histograms go to
 // its accumulation slot (a single correlation: Neumaier-compensated, slot blockIdx.x)
 // or, in a batched run, to the patch pairs' sums in HBM.
@@ -573,35 +607,10 @@ __device__ __forceinline__ void flush_sums(double* hist_all, unsigned long long*
                                            unsigned long long* __restrict__ pair_np) {
     using Ops = ItemOps<IT>;
     if (Ops::batched) {
-        // Add each used pair histogram to that pair's sums in HBM.  A pair's sums are
-        // a (hi, lo) pair of arrays: the exact rounding error of every addition into
-        // hi (TwoSum of the value it replaced and the addend) goes into lo.
         if (threadIdx.x < NCNT) slot_cnt[(size_t)blockIdx.x * NCNT + threadIdx.x] += hcnt[threadIdx.x];
-        for (int sl = 0; sl < PB_SLOTS; ++sl) {
-            const int pp = s_tag[sl];
-            if (pp < 0) continue;
-            const double* hs = hist_all + sl * per_slot;
-            const unsigned long long* ns = hnp_all + sl * per_slot;
-            double* ph = pair_hist + (size_t)pp * 2 * nh;
-            for (int t = threadIdx.x; t < nh; t += blockDim.x) {
-                double v = 0.;
-                for (int c = 0; c < R; ++c) v += hs[(size_t)t * R + c];
-                if (v == 0.) continue;
-                const double old = atomicAdd(&ph[t], v);
-                const double sum = old + v;
-                const double bv = sum - old;
-                const double err = (old - (sum - bv)) + (v - bv);
-                if (err != 0.) atomicAdd(&ph[nh + t], err);
-            }
-            for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-                unsigned long long v = 0;
-                for (int c = 0; c < R; ++c) v += ns[(size_t)t * R + c];
-                if (v) atomicAdd(&pair_np[(size_t)pp * nb + t], v);
-            }
-        }
+        flush_pair_hists(hist_all, hnp_all, nh, nb, R, per_slot, s_tag, pair_hist, pair_np);
         return;
-    }
-    // Flush into this workgroup's slot with Neumaier compensation.
+    }    // Flush into this workgroup's slot with Neumaier compensation.
     const size_t so = (size_t)blockIdx.x * nh;
     for (int t = threadIdx.x; t < nh; t += blockDim.x) {
         double v = 0.;
@@ -754,7 +763,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
     unsigned int* __restrict__ flag, C2Params P, int R, double* __restrict__ slot_hi,
     double* __restrict__ slot_lo, unsigned long long* __restrict__ slot_np,
     unsigned long long* __restrict__ slot_cnt, double* __restrict__ pair_hist,
-    unsigned long long* __restrict__ pair_np, unsigned int* __restrict__ job_visits) {
+    unsigned long long* __restrict__ pair_np, unsigned int* __restrict__ job_visits,
+    unsigned long long* __restrict__ launch_cnt) {
     using Ops = ItemOps<IT>;
     HIP_DYNAMIC_SHARED(double, sh)
     __shared__ unsigned int s_pos;           // children written this round
@@ -762,6 +772,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
     __shared__ long long s_g0, s_gn;         // jobs taken this round
     __shared__ long long s_snext, s_send;    // static mode: the rest of this share
     __shared__ int s_tag[PB_SLOTS];
+    __shared__ int s_full;                   // a batched round found no free pair slot
     const int nb = P.nbins;
     const int nh = NQ2 * nb;
     const int nslot = Ops::batched ? PB_SLOTS : 1;
@@ -808,6 +819,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
             s_g0 = g0;
             s_gn = gn;
             s_pos = 0;
+            s_full = 0;
         }
         __syncthreads();
         const long long size0 = s_size, gn = s_gn;
@@ -830,6 +842,22 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
             c2 = n2[it.j >= 0 ? it.j : 0];
         }
         __syncthreads();                            // every item read before children overwrite them
+        if constexpr (Ops::batched) {
+            // A workgroup meets many patch pairs over its life: when a round's items
+            // find no free LDS histogram, the pairs' histograms go to HBM and the slots
+            // are claimed anew (a lane still without one adds straight to HBM).
+            if (valid && pair_slot(Ops::pair(it), s_tag) < 0) s_full = 1;
+            __syncthreads();
+            if (s_full) {                           // uniform
+                flush_pair_hists(hist_all, hnp_all, nh, nb, R, per_slot, s_tag, pair_hist,
+                                 pair_np);
+                __syncthreads();
+                for (int t = threadIdx.x; t < (int)(per_slot * nslot); t += blockDim.x)
+                    reinterpret_cast<unsigned long long*>(sh)[t] = 0ull;
+                if (threadIdx.x < PB_SLOTS) s_tag[threadIdx.x] = -1;
+                __syncthreads();
+            }
+        }
         visit_round<D, C, IT, GM>(X, P, it, c1, c2, valid, stk + base, &s_pos);
         __syncthreads();
         if (threadIdx.x == 0) s_size = base + s_pos;
@@ -839,6 +867,8 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
     atomicAdd(&hcnt[CNT_LEAF], (unsigned long long)X.c_leaf);
     atomicAdd(&hcnt[CNT_REFINED], (unsigned long long)X.c_ref);
     __syncthreads();
+    // this launch's own counters too (the chunked split before it has its own)
+    if (threadIdx.x < NCNT) atomicAdd(&launch_cnt[threadIdx.x], hcnt[threadIdx.x]);
     flush_sums<IT>(hist_all, hnp_all, hcnt, nh, nb, R, per_slot, s_tag, slot_hi, slot_lo,
                    slot_np, slot_cnt, pair_hist, pair_np);
 }
@@ -1221,7 +1251,7 @@ struct Corr2Scratch {
         dev_free(bcount, kBcountBytes);
         dev_free(boff, kBoffBytes);
         dev_free(pstack, pstack_bytes);
-        dev_free(pctl, 2 * sizeof(unsigned long long));
+        dev_free(pctl, PCTL_WORDS * sizeof(unsigned long long));
         if (h_outcount) (void)hipHostFree(h_outcount);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -1418,7 +1448,7 @@ template <class IT>
 using PersistFn = void (*)(const DNode*, const DNode*, const IT*, long long, unsigned long long*,
                            int, const long long*, IT*, long long, unsigned int*, C2Params, int,
                            double*, double*, unsigned long long*, unsigned long long*, double*,
-                           unsigned long long*, unsigned int*);
+                           unsigned long long*, unsigned int*, unsigned long long*);
 
 template <class IT>
 static PersistFn<IT> pick_persist(int D, int C, bool gm) {
@@ -1469,6 +1499,10 @@ int traversal_mode() { return g_trav_mode; }
 // an evenly strided sample of at most g_log_max jobs.
 static const CostModel* g_cost_model = nullptr;
 void set_cost_model(const CostModel* m) { g_cost_model = m; }
+static long long g_persist_cnt[NCNT] = {0, 0, 0, 0};   // the last persistent launch's counters
+void persist_counters(long long* c) {
+    for (int k = 0; k < NCNT; ++k) c[k] = g_persist_cnt[k];
+}
 static long long g_log_max = 0;
 static std::vector<float> g_log_X, g_log_V;
 void set_job_log(long long max_samples) {
@@ -1518,29 +1552,33 @@ static int traverse_persist(Corr2Scratch& S, PersistFn<IT> fn, const DNode* n1, 
         S.pstack_bytes = bytes;
     }
     if (!S.pctl) {
-        S.pctl = static_cast<unsigned long long*>(dev_alloc(2 * sizeof(unsigned long long)));
+        S.pctl = static_cast<unsigned long long*>(dev_alloc(PCTL_WORDS * sizeof(unsigned long long)));
         if (!S.pctl) return fail("out of device memory for the traversal");
     }
-    TC_HIP(hipMemsetAsync(S.pctl, 0, 2 * sizeof(unsigned long long), st));
+    TC_HIP(hipMemsetAsync(S.pctl, 0, PCTL_WORDS * sizeof(unsigned long long), st));
     KernelStat& ks = kernel_stat(KS_TRAVERSE2);
+    KernelStat& kp = kernel_stat(KS_PERSIST);
     const bool timing = timing_enabled();
     if (timing) TC_HIP(hipEventRecord(S.ev0, st));
     hipLaunchKernelGGL(fn, dim3(grid), dim3(TRAV_THREADS), shbytes, st, n1, n2, jobs, njobs,
                        S.pctl, static_part ? 1 : 0, bounds, static_cast<IT*>(S.pstack), cap,
                        reinterpret_cast<unsigned int*>(S.pctl + 1), P, R, S.slot_hi, S.slot_lo,
-                       S.slot_np, S.slot_cnt, pair_hist, pair_np, job_visits);
+                       S.slot_np, S.slot_cnt, pair_hist, pair_np, job_visits, S.pctl + 2);
     TC_HIP(hipGetLastError());
     if (timing) TC_HIP(hipEventRecord(S.ev1, st));
-    TC_HIP(hipMemcpyAsync(S.h_outcount, S.pctl + 1, sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, st));
+    unsigned long long ctl[PCTL_WORDS];
+    TC_HIP(hipMemcpyAsync(ctl, S.pctl, sizeof ctl, hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));
     if (timing) {
         float ms = 0.f;
         TC_HIP(hipEventElapsedTime(&ms, S.ev0, S.ev1));
         ks.ms += ms;
         ks.launches += 1;
+        kp.ms += ms;
+        kp.launches += 1;
     }
-    overflow = (*S.h_outcount & 0xffffffffull) != 0;
+    overflow = (ctl[1] & 0xffffffffull) != 0;
+    for (int k = 0; k < NCNT; ++k) g_persist_cnt[k] = (long long)ctl[2 + k];
     return 0;
 }
 
@@ -1901,9 +1939,47 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
                            static_cast<ItemP*>(S.stack));
         TC_HIP(hipGetLastError());
     }
-    if (int rc = traverse_stack<ItemP>(S, pick_traverse<ItemP>(D, C, P.metric != MetEuclidean || P.rpar_on), nodes1, nodes2, ninit, P,
-                                       R, shbytes, ph, pn))
-        return rc;
+    const bool gm = P.metric != MetEuclidean || P.rpar_on;
+    bool chunked = g_trav_mode == TRAV_CHUNKED;
+    if (!chunked) {
+        // as in corr2_run: a breadth-first chunked split into enough jobs (half as many:
+        // the items are twice as large), then every patch pair's jobs in one persistent
+        // launch (the learned modes are for single correlations; a batched run takes its
+        // jobs by the shared counter in them)
+        long long njobs = ninit;
+        if (ninit < g_persist_min_jobs / 2)
+            if (int rc = traverse_stack<ItemP>(S, pick_traverse<ItemP>(D, C, gm), nodes1, nodes2,
+                                               ninit, P, R, shbytes, ph, pn,
+                                               g_persist_min_jobs / 2, &njobs))
+                return rc;
+        int depth = 0;
+        for (auto* t : trees1) depth = std::max(depth, t->depth);
+        int depth2 = 0;
+        for (auto* t : t2) depth2 = std::max(depth2, t->depth);
+        bool overflow = false;
+        if (int rc = traverse_persist<ItemP>(S, pick_persist<ItemP>(D, C, gm), nodes1, nodes2,
+                                             static_cast<const ItemP*>(S.stack), njobs,
+                                             depth + depth2, g_trav_mode == TRAV_PERSIST_STATIC,
+                                             nullptr, P, R, shbytes, ph, pn, nullptr, overflow))
+            return rc;
+        if (overflow) {      // a stack would have overflowed: start again, chunked
+            g_persist_fallbacks.fetch_add(1);
+            if (zero_slots(S, nb)) return 2;
+            TC_HIP(hipMemsetAsync(ph, 0, hb, st));
+            TC_HIP(hipMemsetAsync(pn, 0, npb, st));
+            if (ninit > 0) {
+                hipLaunchKernelGGL(k_items_pairs, dim3((unsigned)npair), dim3(256), 0, st,
+                                   dpi, dpj, dpa, dio, tops1, dt1, tops2, dt2,
+                                   static_cast<ItemP*>(S.stack));
+                TC_HIP(hipGetLastError());
+            }
+            chunked = true;
+        }
+    }
+    if (chunked)
+        if (int rc = traverse_stack<ItemP>(S, pick_traverse<ItemP>(D, C, gm), nodes1, nodes2,
+                                           ninit, P, R, shbytes, ph, pn))
+            return rc;
     // counters from the workgroup slots
     const int tot = (int)(nh + nb + NCNT);
     hipLaunchKernelGGL(k_reduce_slots, dim3(tot), dim3(64), 0, st, S.slot_hi,

@@ -100,6 +100,7 @@ long long job_log_fetch(float* X, float* V, long long cap);   // returns the sam
 int set_traversal_mode(int mode);
 int traversal_mode();
 long long persist_fallbacks();
+void persist_counters(long long* c4);   // visits, accepted, leaf, refined of the last persistent launch
 int set_persist_params(int levels, long long min_jobs);   // 0, 0: the defaults
 struct Corr2Scratch;
 Corr2Scratch* new_corr2_scratch();
