@@ -198,6 +198,16 @@ long long tc_traversal_fallbacks(void);
 /* Work counters (as tc_corr2_counters) of the last persistent launch alone, without the
  * chunked split before it; for roofline accounting.  No reference counterpart. */
 int tc_persist_counters(long long* counters4);
+/* Patch-based covariances (treecorr/corr2base.py:2321-2538): every row of a jackknife,
+ * sample, bootstrap or marked-bootstrap design matrix is a weighted sum of patch-pair
+ * sums, R = W S (W: nrow x npair row weights, S: npair x nq stacked pair sums, R: nrow x
+ * nq; host arrays, row major), replacing the reference's per-row Python sums
+ * (_make_cov_design_matrix_core :2321-2368, Corr2._sum :1333-1352); and the covariance of
+ * the finished rows V (nrow x ncol), C = sum_r w_r (v_r - mean)^T (v_r - mean) with mean
+ * the plain row mean and w null for unit weights (the v.conj().T.dot(v) of :2432-2538).
+ * FP64, a fixed summation order per element. */
+int tc_cov_rowsums(int nrow, int npair, int nq, const double* W, const double* S, double* R);
+int tc_cov_gram(int nrow, int ncol, const double* V, const double* w, double* C);
 /* The learned cost model of the persistent traversal's jobs (no reference counterpart:
  * TreeCorr hands out its top-level pairs with OpenMP's schedule(dynamic),
  * src/Corr2.cpp:158-203).  A perceptron of three FP32 layers (nfeatures inputs, two

@@ -62,6 +62,8 @@ SIGNATURES = {
     'tc_cost_model_train': (_i, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                  _ll, _i, _i, ctypes.c_float, _dp]),
     'tc_set_cost_model': (_i, [_vp]),
+    'tc_cov_rowsums': (_i, [_i, _i, _i, _dp, _dp, _dp]),
+    'tc_cov_gram': (_i, [_i, _i, _dp, _dp, _dp]),
     'tc_set_job_log': (_i, [_ll]),
     'tc_job_log_fetch': (_i, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _ll,
                               ctypes.POINTER(_ll)]),

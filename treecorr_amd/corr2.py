@@ -156,6 +156,7 @@ class Corr2:
         self.zperiod = float(cfg.get('zperiod') or period)
         self.var_method = cfg.get('var_method', 'shot')
         self.num_bootstrap = int(cfg.get('num_bootstrap', 500))
+        self._rng = rng
         self.cross_patch_weight = cfg.get('cross_patch_weight')
         if self.cross_patch_weight not in (None, 'simple', 'mean', 'match', 'geom'):
             raise ValueError("Invalid cross_patch_weight %s" % self.cross_patch_weight)
@@ -561,6 +562,12 @@ class Corr2:
 
     def _sum(self, others, corr_only):
         # corr2base.py:1333-1352: self = sum of w * other
+        if _cov._SUM_HOOK is not None:      # a design matrix's rows, formed on the GPU
+            _cov._SUM_HOOK(self, others, corr_only)
+            self._varxi = None
+            self._cov = None
+            return
+
         def x(a, w):
             return a if w == 1 else a * w
         for q in range(4):
@@ -641,6 +648,70 @@ class Corr2:
                 raise ValueError(f"cross_patch_weight = {cpw} is invalid for sample")
         return rows
 
+    @property
+    def rng(self):
+        # corr2base.py:579-582: the bootstrap draws come from this numpy Generator
+        if self._rng is None:
+            self._rng = np.random.default_rng()
+        return self._rng
+
+    def _marked_pairs(self, index, cross_patch_weight):
+        # MarkedPairIterator, corr2base.py:1861-1884: the patches of the first catalog at
+        # their multiplicity in index, each with all its pairs
+        ok = self._ok
+        if self.npatch2 == 1:
+            return [(i, 0, 1) for i in index if ok[i, 0]]
+        if self.npatch1 == 1:
+            return [(0, j, 1) for j in index if ok[0, j]]
+        assert self.npatch1 == self.npatch2
+        idx, counts = np.unique(index, return_counts=True)
+        if cross_patch_weight == 'simple':
+            return [(i, j, w) for i, w in zip(idx, counts) for j in range(self.npatch2)
+                    if ok[i, j]]
+        if cross_patch_weight == 'mean':
+            out = []
+            for i, w in zip(idx, counts):
+                for j in range(self.npatch2):
+                    if i == j:
+                        if ok[i, j]:
+                            out.append((i, j, w))
+                    elif ok[i, j]:
+                        out.append((i, j, w / 2))
+                    elif ok[j, i]:
+                        out.append((j, i, w / 2))
+            return out
+        raise ValueError(f"cross_patch_weight = {cross_patch_weight} is invalid for "
+                         "marked_bootstrap")
+
+    def _bootstrap_pairs(self, index, cross_patch_weight):
+        # BootstrapPairIterator, corr2base.py:1886-1928: the auto pairs of the drawn patches
+        # at their multiplicity, and the cross pairs between drawn patches weighted by the
+        # product (simple), mean (mean, over all pairs) or geometric mean (geom) of the
+        # two multiplicities
+        ok = self._ok
+        if self.npatch2 == 1:
+            return [(i, 0, 1) for i in index if ok[i, 0]]
+        if self.npatch1 == 1:
+            return [(0, j, 1) for j in index if ok[0, j]]
+        assert self.npatch1 == self.npatch2
+        idx, counts = np.unique(index, return_counts=True)
+        out = [(i, i, w) for i, w in zip(idx, counts) if ok[i, i]]
+        cpw = cross_patch_weight
+        if cpw == 'simple':
+            out += [(i, j, w1 * w2) for i, w1 in zip(idx, counts) for j, w2 in zip(idx, counts)
+                    if ok[i, j] and i != j]
+        elif cpw == 'mean':
+            mult = dict(zip(idx, counts))
+            out += [(i, j, (mult.get(i, 0) + mult.get(j, 0)) / 2)
+                    for i in range(self.npatch1) for j in range(self.npatch2)
+                    if ok[i, j] and i != j]
+        elif cpw == 'geom':
+            out += [(i, j, (w1 * w2) ** 0.5) for i, w1 in zip(idx, counts)
+                    for j, w2 in zip(idx, counts) if ok[i, j] and i != j]
+        else:
+            raise ValueError(f"cross_patch_weight = {cpw} is invalid for bootstrap")
+        return out
+
     def _check_cpw(self, method, cross_patch_weight):
         # corr2base.py:1931-1948 (the default's warning is not repeated here)
         if cross_patch_weight is None:
@@ -651,7 +722,8 @@ class Corr2:
 
     def estimate_cov(self, method, *, func=None, comm=None, num_bootstrap=None,
                      cross_patch_weight=None):
-        """Covariance estimate: 'shot', 'jackknife' or 'sample' (corr2base.py:1360-1482)."""
+        """Covariance estimate: 'shot', 'jackknife', 'sample', 'bootstrap' or
+        'marked_bootstrap' (corr2base.py:1360-1482)."""
         all_func = None if func is None else (lambda corrs: func(corrs[0]))
         return _cov.estimate_multi_cov([self], method, func=all_func, comm=comm,
                                        num_bootstrap=num_bootstrap,

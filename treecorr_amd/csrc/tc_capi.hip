@@ -699,6 +699,21 @@ int tc_job_log_fetch(float* X, float* visits, long long cap, long long* n) {
     return 0;
 }
 
+int tc_cov_rowsums(int nrow, int npair, int nq, const double* W, const double* S, double* R) {
+    if (nrow < 0 || npair < 0 || nq < 0) return tc::fail("sizes must be >= 0");
+    if ((nrow > 0 && nq > 0) && (!R || (npair > 0 && (!W || !S))))
+        return tc::fail("W, S and R are required");
+    if (int rc = check_gpu()) return rc;
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::cov_rowsums(nrow, npair, nq, W, S, R);
+}
+int tc_cov_gram(int nrow, int ncol, const double* V, const double* w, double* C) {
+    if (!V || !C) return tc::fail("V and C are required");
+    if (int rc = check_gpu()) return rc;
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::cov_gram(nrow, ncol, V, w, C);
+}
+
 int tc_set_persist_params(int levels, long long min_jobs) {
     return tc::set_persist_params(levels, min_jobs);
 }

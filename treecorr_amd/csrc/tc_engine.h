@@ -87,6 +87,11 @@ int cost_model_grad(CostModel* m, const float* X, const float* Y, long long n, f
 int cost_model_train(CostModel* m, const float* X, const float* Y, long long n, int steps,
                      int batch, float lr, double* loss);
 
+// Covariance linear algebra (tc_cov.hip): R = W S (the design matrix's rows from the
+// patch-pair sums) and C = sum_r w_r (v_r - mean)^T (v_r - mean); host arrays, row major.
+int cov_rowsums(int nrow, int npair, int nq, const double* W, const double* S, double* R);
+int cov_gram(int nrow, int ncol, const double* V, const double* w, double* C);
+
 int set_traversal_chunk(long long items);   // 0 restores the default
 // Traversal modes: chunked launches driven from the host, or one persistent launch whose
 // workgroups take jobs by a shared counter or from static equal-count shares.
