@@ -248,6 +248,17 @@ int tc_job_log_fetch(float* X, float* visits, long long cap, long long* n);
  * small values to exercise the fallback and the persistent launch on small inputs.
  * No reference counterpart. */
 int tc_set_persist_params(int levels, long long min_jobs);
+/* A persistent workgroup that runs short takes 1/div of a fair share of the jobs at a
+ * time from the shared counter (default 16: the queue depth the autotuner varies).
+ * No reference counterpart. */
+int tc_set_persist_grab(int div);
+/* Graph mode: each persistent launch (its control words cleared, then the kernel) is
+ * captured as a hipGraph on first use and replayed whenever a launch has the same
+ * arguments again (the arguments of a graph's kernels are fixed at capture; up to 8
+ * signatures are kept).  on < 0 returns the current setting.  tc_graph_stats reports the
+ * captures and replays so far.  No reference counterpart. */
+int tc_set_graph_mode(int on);
+int tc_graph_stats(long long* captures, long long* replays);
 /* Speculative top tree levels.  The partitions near the root proceed from centroids
  * summed in parallel while the reference-order sums run on a second stream; those
  * then write the node records and check every decision and every object's partition

@@ -1,4 +1,5 @@
 """GPU 2-point correlations vs the literal reference's golden vectors and vs the oracle."""
+import ctypes
 import json
 import math
 import os
@@ -526,3 +527,66 @@ def test_gpu_traversal_modes_same_result(gpu, oracle_lib, mode, levels, min_jobs
         assert normwise(out[k], ref[k]) < RTOL, k
     for k in ('visits', 'accepted', 'leafpairs', 'refined'):
         assert out['counters'][k] == ref['counters'][k], k
+
+
+@pytest.mark.gpu
+def test_gpu_graph_mode_replays_match_eager(gpu, oracle_lib):
+    """Graph mode: the persistent launch captured once and replayed; every replay, with
+    eager launches between them, gives the eager launch's npairs and counters, and the
+    captures and replays are counted."""
+    import capi
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    cat, scale = C.make_catalog(C.SPHERE, 20000, 131, 0.3)
+    bn = C.binning(scale, 12, 0.005, 0.3, 0.1, 0.1)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], brute=False, min_top=3)
+    f1 = capi.Field(4, 3, cat, **kw)
+    args = (bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'])
+    _lib.check(L.tc_set_persist_params(0, 2000))
+    try:
+        eager = capi.corr2(4, f1, None, *args)
+        _lib.check(L.tc_set_graph_mode(1))
+        assert L.tc_set_graph_mode(-1) == 1
+        c0, r0 = ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.check(L.tc_graph_stats(ctypes.byref(c0), ctypes.byref(r0)))
+        outs = []
+        for rep in range(3):
+            outs.append(capi.corr2(4, f1, None, *args))
+            _lib.check(L.tc_set_graph_mode(0))
+            between = capi.corr2(4, f1, None, *args)      # eager work between replays
+            np.testing.assert_array_equal(between['npairs'], eager['npairs'])
+            _lib.check(L.tc_set_graph_mode(1))
+        c1, r1 = ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.check(L.tc_graph_stats(ctypes.byref(c1), ctypes.byref(r1)))
+    finally:
+        _lib.check(L.tc_set_graph_mode(0))
+        _lib.check(L.tc_set_persist_params(0, 0))
+    assert c1.value - c0.value >= 1 and r1.value - r0.value >= 1
+    assert (c1.value - c0.value) + (r1.value - r0.value) == 3
+    for out in outs:
+        np.testing.assert_array_equal(out['npairs'], eager['npairs'])
+        assert out['counters'] == eager['counters']
+        for k in KEYS[:-1]:
+            assert normwise(out[k], eager[k]) < 1e-13, k
+
+
+@pytest.mark.gpu
+def test_gpu_autotune_keeps_results(gpu):
+    import capi
+    from treecorr_amd import _lib, tune
+    cat, scale = C.make_catalog(C.SPHERE, 15000, 132, 0.3)
+    bn = C.binning(scale, 12, 0.005, 0.3, 0.1, 0.1)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], brute=False, min_top=3)
+    f1 = capi.Field(4, 3, cat, **kw)
+    args = (bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'])
+    ref = capi.corr2(4, f1, None, *args)
+    outs = []
+    try:
+        best, times = tune.autotune(lambda: outs.append(capi.corr2(4, f1, None, *args)),
+                                    min_jobs=(500, 5000), grab_div=(2, 32))
+    finally:
+        tune.set_params()
+    assert best in times and len(times) == 4 and all(t > 0 for t in times.values())
+    for out in outs:
+        np.testing.assert_array_equal(out['npairs'], ref['npairs'])
+        assert out['counters'] == ref['counters']

@@ -51,6 +51,9 @@ SIGNATURES = {
     'tc_traversal_fallbacks': (_ll, []),
     'tc_persist_counters': (_i, [ctypes.POINTER(_ll)]),
     'tc_set_persist_params': (_i, [_i, _ll]),
+    'tc_set_persist_grab': (_i, [_i]),
+    'tc_set_graph_mode': (_i, [_i]),
+    'tc_graph_stats': (_i, [ctypes.POINTER(_ll), ctypes.POINTER(_ll)]),
     'tc_cost_model_create': (_i, [ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
     'tc_cost_model_destroy': (_i, [_vp]),
     'tc_cost_model_sizes': (_i, [ctypes.POINTER(_i), ctypes.POINTER(_i)]),

@@ -714,6 +714,24 @@ int tc_cov_gram(int nrow, int ncol, const double* V, const double* w, double* C)
     return tc::cov_gram(nrow, ncol, V, w, C);
 }
 
+int tc_set_persist_grab(int div) {
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::set_persist_grab(div);
+}
+int tc_set_graph_mode(int on) {
+    static int cur = 0;
+    if (on < 0) return cur;
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    cur = on ? 1 : 0;
+    tc::set_graph_mode(cur != 0);
+    return 0;
+}
+int tc_graph_stats(long long* captures, long long* replays) {
+    if (!captures || !replays) return tc::fail("captures and replays are required");
+    tc::graph_stats(captures, replays);
+    return 0;
+}
+
 int tc_set_persist_params(int levels, long long min_jobs) {
     return tc::set_persist_params(levels, min_jobs);
 }

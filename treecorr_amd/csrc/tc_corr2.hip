@@ -22,6 +22,7 @@
 #include "tc_engine.h"
 
 #include <algorithm>
+#include <cstring>
 #include <atomic>
 #include <cmath>
 #include <type_traits>
@@ -758,7 +759,7 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse(
 template <int D, int C, class IT, bool GM>
 __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
     const DNode* __restrict__ n1, const DNode* __restrict__ n2, const IT* __restrict__ jobs,
-    long long njobs, unsigned long long* __restrict__ job_next, int static_part,
+    long long njobs, unsigned long long* __restrict__ job_next, int static_part, int grab_div,
     const long long* __restrict__ bounds, IT* __restrict__ stacks, long long cap,
     unsigned int* __restrict__ flag, C2Params P, int R, double* __restrict__ slot_hi,
     double* __restrict__ slot_lo, unsigned long long* __restrict__ slot_np,
@@ -805,9 +806,9 @@ __global__ __launch_bounds__(TRAV_THREADS) void k_traverse_persist(
                     gn = need < s_send - s_snext ? need : s_send - s_snext;
                     s_snext += gn;
                 } else {
-                    // a batch of at most a sixteenth of a fair share, so the jobs left
+                    // a batch of at most 1/grab_div of a fair share, so the jobs left
                     // near the end stay spread over the workgroups
-                    long long want = njobs / (16LL * gridDim.x);
+                    long long want = njobs / ((long long)grab_div * gridDim.x);
                     want = want < 1 ? 1 : (want < need ? want : need);
                     const unsigned long long a = atomicAdd(job_next, (unsigned long long)want);
                     if (a < (unsigned long long)njobs) {
@@ -1014,6 +1015,12 @@ __global__ __launch_bounds__(256) void k_lpt_scatter(const float* __restrict__ y
 }
 
 
+// Clears n words (a kernel, not a memset, so that a captured graph replays it reliably).
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0ull;
+}
+
 // Fixed-order compensated reduction of the per-workgroup slots.
 // One wavefront per output entry (nh sums, nb npairs, NCNT counters, in that order):
 // lane l adds slots l, l+64, ... with compensation, then the 64 partial sums are
@@ -1187,6 +1194,48 @@ __global__ void k_concat_nodes(const DNode* __restrict__ src, int n, int offset,
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// The arguments of a persistent launch, as the key of its captured graph.
+struct GraphKey {
+    const void* fn;
+    const void* ptr[13];
+    long long num[7];
+    unsigned char params[sizeof(C2Params)];
+    bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
+};
+// A few captured launches, the least recently used replaced first.
+struct GraphCache {
+    static constexpr int N = 8;
+    GraphKey key[N];
+    hipGraphExec_t exec[N] = {};
+    long long used[N] = {};
+    long long clock = 0;
+    hipGraphExec_t find(const GraphKey& k) {
+        for (int i = 0; i < N; ++i)
+            if (exec[i] && key[i] == k) {
+                used[i] = ++clock;
+                return exec[i];
+            }
+        return nullptr;
+    }
+    void add(const GraphKey& k, hipGraphExec_t e) {
+        int v = 0;
+        for (int i = 0; i < N; ++i)
+            if (!exec[i] || used[i] < used[v]) { v = i; if (!exec[i]) break; }
+        if (exec[v]) (void)hipGraphExecDestroy(exec[v]);
+        std::memset(&key[v], 0, sizeof(GraphKey));
+        key[v] = k;
+        exec[v] = e;
+        used[v] = ++clock;
+    }
+    void clear() {
+        for (int i = 0; i < N; ++i) {
+            if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+            exec[i] = nullptr;
+        }
+    }
+    ~GraphCache() { clear(); }
+};
+
 // Traversal buffers, kept by the library for reuse (sized in bytes: a batched run
 // stores 16-byte items, a single correlation 8-byte ones).
 struct Corr2Scratch {
@@ -1222,6 +1271,7 @@ struct Corr2Scratch {
         ~Buf() { dev_free(p, bytes); }
     };
     Buf feat, pred, cost, prefix, bounds, keys, jobs2, jobsJ, jvis, logX, logV;
+    GraphCache graphs;      // persistent launches captured as hipGraphs (graph mode)
     unsigned long long* boff = nullptr; // two offset arrays, TRAV_SLOTS+1 each
     int nb = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1446,7 +1496,7 @@ static int traverse_stack(Corr2Scratch& S, TravFn<IT> fn, const DNode* n1, const
 
 template <class IT>
 using PersistFn = void (*)(const DNode*, const DNode*, const IT*, long long, unsigned long long*,
-                           int, const long long*, IT*, long long, unsigned int*, C2Params, int,
+                           int, int, const long long*, IT*, long long, unsigned int*, C2Params, int,
                            double*, double*, unsigned long long*, unsigned long long*, double*,
                            unsigned long long*, unsigned int*, unsigned long long*);
 
@@ -1479,6 +1529,19 @@ static PersistFn<ItemJ> pick_persist_logged(int D, int C) {
 static int g_trav_mode = TRAV_PERSIST_DYNAMIC;
 std::atomic<long long> g_persist_fallbacks{0};   // persistent runs redone chunked
 static int g_persist_levels = 0;    // stack levels per workgroup (0: from the tree depths)
+static int g_persist_grab_div = 16; // a shared-counter grab takes 1/div of a fair share
+static bool g_graph_mode = false;   // persistent launches captured in hipGraphs and replayed
+static long long g_graph_captures = 0, g_graph_replays = 0;
+int set_persist_grab(int div) {
+    if (div < 1) return fail("the grab divisor must be >= 1");
+    g_persist_grab_div = div;
+    return 0;
+}
+void set_graph_mode(bool on) { g_graph_mode = on; }
+void graph_stats(long long* captures, long long* replays) {
+    *captures = g_graph_captures;
+    *replays = g_graph_replays;
+}
 static long long g_persist_min_jobs = PERSIST_MIN_JOBS;
 int set_persist_params(int levels, long long min_jobs) {
     if (levels < 0 || min_jobs < 0) return fail("levels and min_jobs must be >= 0");
@@ -1555,16 +1618,54 @@ static int traverse_persist(Corr2Scratch& S, PersistFn<IT> fn, const DNode* n1, 
         S.pctl = static_cast<unsigned long long*>(dev_alloc(PCTL_WORDS * sizeof(unsigned long long)));
         if (!S.pctl) return fail("out of device memory for the traversal");
     }
-    TC_HIP(hipMemsetAsync(S.pctl, 0, PCTL_WORDS * sizeof(unsigned long long), st));
     KernelStat& ks = kernel_stat(KS_TRAVERSE2);
     KernelStat& kp = kernel_stat(KS_PERSIST);
     const bool timing = timing_enabled();
+    // the launch: the control words cleared, then the persistent kernel
+    auto enqueue = [&]() -> int {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, S.pctl, (long long)PCTL_WORDS);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(TRAV_THREADS), shbytes, st, n1, n2, jobs, njobs,
+                           S.pctl, static_part ? 1 : 0, g_persist_grab_div, bounds,
+                           static_cast<IT*>(S.pstack), cap,
+                           reinterpret_cast<unsigned int*>(S.pctl + 1), P, R, S.slot_hi,
+                           S.slot_lo, S.slot_np, S.slot_cnt, pair_hist, pair_np, job_visits,
+                           S.pctl + 2);
+        TC_HIP(hipGetLastError());
+        return 0;
+    };
     if (timing) TC_HIP(hipEventRecord(S.ev0, st));
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(TRAV_THREADS), shbytes, st, n1, n2, jobs, njobs,
-                       S.pctl, static_part ? 1 : 0, bounds, static_cast<IT*>(S.pstack), cap,
-                       reinterpret_cast<unsigned int*>(S.pctl + 1), P, R, S.slot_hi, S.slot_lo,
-                       S.slot_np, S.slot_cnt, pair_hist, pair_np, job_visits, S.pctl + 2);
-    TC_HIP(hipGetLastError());
+    if (!g_graph_mode) {
+        if (int rc = enqueue()) return rc;
+    } else {
+        // Captured once per launch signature (every argument the kernels take) and
+        // replayed: the arguments of a graph's kernel nodes are fixed at capture.
+        GraphKey key{};
+        key.fn = reinterpret_cast<const void*>(fn);
+        key.ptr[0] = n1; key.ptr[1] = n2; key.ptr[2] = jobs; key.ptr[3] = bounds;
+        key.ptr[4] = S.pstack; key.ptr[5] = S.pctl; key.ptr[6] = S.slot_hi;
+        key.ptr[7] = S.slot_lo; key.ptr[8] = S.slot_np; key.ptr[9] = S.slot_cnt;
+        key.ptr[10] = pair_hist; key.ptr[11] = pair_np; key.ptr[12] = job_visits;
+        key.num[0] = njobs; key.num[1] = cap; key.num[2] = grid; key.num[3] = (long long)shbytes;
+        key.num[4] = R; key.num[5] = static_part; key.num[6] = g_persist_grab_div;
+        std::memcpy(key.params, &P, sizeof(C2Params));
+        hipGraphExec_t exec = S.graphs.find(key);
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            TC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue();
+            const hipError_t e = hipStreamEndCapture(st, &graph);
+            if (rc) return rc;
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
+            S.graphs.add(key, exec);
+            ++g_graph_captures;
+        } else {
+            ++g_graph_replays;
+        }
+        TC_HIP(hipGraphLaunch(exec, st));
+    }
     if (timing) TC_HIP(hipEventRecord(S.ev1, st));
     unsigned long long ctl[PCTL_WORDS];
     TC_HIP(hipMemcpyAsync(ctl, S.pctl, sizeof ctl, hipMemcpyDeviceToHost, st));

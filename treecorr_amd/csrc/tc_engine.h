@@ -107,6 +107,9 @@ int traversal_mode();
 long long persist_fallbacks();
 void persist_counters(long long* c4);   // visits, accepted, leaf, refined of the last persistent launch
 int set_persist_params(int levels, long long min_jobs);   // 0, 0: the defaults
+int set_persist_grab(int div);          // a shared-counter grab: 1/div of a fair share
+void set_graph_mode(bool on);           // capture persistent launches as hipGraphs
+void graph_stats(long long* captures, long long* replays);
 struct Corr2Scratch;
 Corr2Scratch* new_corr2_scratch();
 void delete_corr2_scratch(Corr2Scratch*);
