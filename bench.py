@@ -562,11 +562,34 @@ def patch_step(tc, D, R):
     return nn, kk, cov
 
 
-def patch_cpu_baseline(tc, w, nn, kk, target_s=15.):
-    """The oracle on a sample of the jobs (patch-pair tree builds + traversals, all host
-    cores); the full-job time is extrapolated by the share of pairs sampled."""
-    from oracle import oracle as O
-    nthreads = int(os.environ.get('OMP_NUM_THREADS', '0')) or (os.cpu_count() or 1)
+def patch_cpu_baseline(tc, w, nn, kk, target_s=20.):
+    """The reference's own C++ engine (TreeCorr's _treecorr module compiled from its
+    sources into oracle/_ref; the oracle where it is not built) on a random sample of the
+    patch-pair jobs, each with its two tree builds, on all host cores; the full-job time
+    is extrapolated by the share of pairs sampled.  Every sampled pair's npairs is
+    compared with the GPU's."""
+    from oracle import ref as R
+    m = R.module()
+    if m is not None:
+        nthreads = int(m.GetOMPThreads())
+        kind = "reference"
+
+        def field(d, x, y, z, k, kw):
+            return R.make_field(d, 2, x, y, z, k=k, split_method=2, **kw)
+
+        def corr(d, f1, f2, bn, bin_size, b, a):
+            return R.corr2(d, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bin_size, b, a)
+    else:
+        from oracle import oracle as O
+        nthreads = int(os.environ.get('OMP_NUM_THREADS', '0')) or (os.cpu_count() or 1)
+        kind = "port"
+
+        def field(d, x, y, z, k, kw):
+            return O.Field(d, 2, x, y, z, k=k, **kw)
+
+        def corr(d, f1, f2, bn, bin_size, b, a):
+            return O.corr2(d, 2, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bin_size,
+                           b, a, exact=False, nthreads=nthreads)
     bn = PATCH_BIN
     bin_size = math.log(bn['max_sep'] / bn['min_sep']) / bn['nbins']
     jobs = [('nn', ij) for ij, r in nn.results.items() if r.nonzero] + \
@@ -574,39 +597,41 @@ def patch_cpu_baseline(tc, w, nn, kk, target_s=15.):
     rng = np.random.default_rng(7)
     rng.shuffle(jobs)
     tot = float(nn.npairs.sum() + kk.npairs.sum())
-    done, spent = 0., 0.
-    for kind, (i, j) in jobs:
-        a = 0.1 if kind == 'nn' else 1.0
+    done, spent, njob, equal = 0., 0., 0, True
+    for kd, (i, j) in jobs:
+        a = 0.1 if kd == 'nn' else 1.0
         b = bin_size * bn['bin_slop']
         b1 = min(a, b)
         kw = dict(min_size=bn['min_sep'] * b1 / (2 + 3 * b1), max_size=bn['max_sep'] * b1,
                   min_top=bn['min_top'])
         t0 = time.perf_counter()
         di = w['ld'] == i
-        if kind == 'nn':
+        if kd == 'nn':
             rj = w['lr'] == j
-            f1 = O.Field(O.NDATA, 2, w['x'][di], w['y'][di], w['z'][di], **kw)
-            f2 = O.Field(O.NDATA, 2, w['rx'][rj], w['ry'][rj], w['rz'][rj], **kw)
-            O.corr2(O.NDATA, 2, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bin_size,
-                    b, a, exact=False, nthreads=nthreads)
-            done += nn.results[(i, j)].npairs.sum()
+            f1 = field(0, w['x'][di], w['y'][di], w['z'][di], None, kw)
+            f2 = field(0, w['rx'][rj], w['ry'][rj], w['rz'][rj], None, kw)
+            r = corr(0, f1, f2, bn, bin_size, b, a)
+            mine = nn.results[(i, j)]
         else:
             dj = w['ld'] == j
-            f1 = O.Field(O.KDATA, 2, w['x'][di], w['y'][di], w['z'][di], k=w['k'][di], **kw)
-            f2 = None if i == j else O.Field(O.KDATA, 2, w['x'][dj], w['y'][dj], w['z'][dj],
-                                             k=w['k'][dj], **kw)
-            O.corr2(O.KDATA, 2, f1, f2, bn['nbins'], bn['min_sep'], bn['max_sep'], bin_size,
-                    b, a, exact=False, nthreads=nthreads)
-            done += kk.results[(i, j)].npairs.sum()
+            f1 = field(1, w['x'][di], w['y'][di], w['z'][di], w['k'][di], kw)
+            f2 = None if i == j else field(1, w['x'][dj], w['y'][dj], w['z'][dj], w['k'][dj], kw)
+            r = corr(1, f1, f2, bn, bin_size, b, a)
+            mine = kk.results[(i, j)]
         spent += time.perf_counter() - t0
+        done += mine.npairs.sum()
+        njob += 1
+        equal = equal and bool(np.array_equal(mine.npairs, r['npairs']))
         if spent > target_s:
             break
     t_full = spent * tot / max(done, 1.)
-    return {"value": tot / t_full, "unit": "pairs/s", "cores": nthreads, "kind": "port",
-            "sample": (f"oracle (OpenMP restatement of TreeCorr's engine) on {100 * done / tot:.2f}% "
-                       f"of the pairs: random patch-pair jobs, each with its two tree builds "
-                       f"({spent:.1f} s); full-job time extrapolated by pair count = "
-                       f"{t_full:.0f} s; jackknife not included")}
+    what = ("TreeCorr's C++ engine compiled from its sources (oracle/_ref)" if kind == "reference"
+            else "the oracle (an OpenMP restatement of TreeCorr's engine)")
+    return {"value": tot / t_full, "unit": "pairs/s", "cores": nthreads, "kind": kind,
+            "sample": (f"{what} on {njob} random patch-pair jobs, {100 * done / tot:.2f}% of the "
+                       f"pairs, each with its two tree builds ({spent:.1f} s); full-job time "
+                       f"extrapolated by pair count = {t_full:.0f} s; jackknife not included"),
+            "parity_vs_gpu": {"sampled_pairs_npairs_equal": equal}}
 
 
 def run_patch_workload(args):
@@ -621,6 +646,16 @@ def run_patch_workload(args):
         # patches from the centres (QuickAssign on the GPU), for both catalogs
         return (tc.Catalog(x=w['x'], y=w['y'], z=w['z'], k=w['k'], patch_centers=w['centers']),
                 tc.Catalog(x=w['rx'], y=w['ry'], z=w['rz'], patch_centers=w['centers']))
+    L.tc_set_graph_mode(1)
+    tuned = None
+    if args.autotune:
+        from treecorr_amd import tune
+        D, R = cats()
+        best, times = tune.autotune(lambda: patch_step(tc, D, R), min_jobs=(1 << 24, 1 << 26),
+                                    grab_div=(4, 16))
+        tuned = {"best": {"min_jobs": best[0], "grab_div": best[1]},
+                 "step_s": {f"{k[0]}/{k[1]}": v for k, v in times.items()}}
+        print(f"# autotuned: {tuned}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         patch_step(tc, *cats())
         print(f"# warmup {i} done", file=sys.stderr, flush=True)
@@ -656,6 +691,8 @@ def run_patch_workload(args):
         "jobs": {"nn_pairs_stored": len(nn.results), "kk_pairs_stored": len(kk.results)},
         "counters": {"nn": nn.counters, "kk": kk.counters},
         "cov_shape": list(cov.shape),
+        "graph_mode": True,
+        "autotune": tuned,
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = patch_cpu_baseline(tc, w, nn, kk)
@@ -907,6 +944,8 @@ def main():
     ap.add_argument('--patch-nd', type=int, default=4_000_000)
     ap.add_argument('--patch-nr', type=int, default=40_000_000)
     ap.add_argument('--npatch', type=int, default=100)
+    ap.add_argument('--autotune', action='store_true',
+                    help='patch workload: autotune job granularity and queue depth first')
     ap.add_argument('--mp-n', type=int, default=500_000,
                     help='points of the multipole (GGG, NNN) leg of the headline line')
     ap.add_argument('--no-mp', action='store_true', help='leave out the multipole leg')
