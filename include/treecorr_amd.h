@@ -138,6 +138,12 @@ int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
  * wins a tie, as in the reference.  patch receives int32 labels. */
 int tc_quick_assign(const double* centers, int npatch, const double* x, const double* y,
                     const double* z, long long n, int* patch);
+/* SelectPatch, src/KMeans.cpp:944-1011 (bound in src/module.cpp; called by
+ * treecorr/catalog.py:1576-1594 _get_patch_index for a catalog given patch_centers and a
+ * single patch): use[i] = 1 unless a centre other than `patch` is strictly nearer object
+ * i (a tie keeps it); centres npatch x 2 (z == NULL) or x 3.  On the GPU. */
+int tc_select_patch(int patch, const double* centers, int npatch, const double* x,
+                    const double* y, const double* z, long long n, int* use);
 /* Lloyd iterations of k-means on the GPU: KMeansRun (src/KMeans.cpp:773-833, bound
  * at src/Field.cpp:633) with alt = false, which treecorr/field.py:552-560
  * kmeans_refine_centers calls.  centers (npatch rows of dim = 2 or 3 values) is refined

@@ -203,6 +203,35 @@ def corr3_multipole(d, coords, f, nbins, min_sep, max_sep, bin_size, b, a, max_n
     return res
 
 
+def select_patch(patch, centers, x, y, z=None, chunk=1 << 16):
+    """SelectPatch (src/KMeans.cpp:944-1011): 1 for each object unless a centre other than
+    `patch` is strictly nearer.  Squared distances: dx^2 + dy^2 in 2-d; in 3-d the
+    compiled reference (its -ffast-math build) adds the selected centre's terms as
+    (dx^2 + dy^2) + dz^2 and the other centres' as dx^2 + (dy^2 + dz^2), which decides
+    objects equidistant from two equal centres (found by matching its outputs; pinned by
+    tests/test_oracle_patches.py).  numpy restatement, evaluated chunk by chunk."""
+    centers = np.asarray(centers, dtype=float)
+    dim = 2 if z is None else 3
+    centers = centers.reshape(-1, dim)
+    out = np.empty(len(x), dtype=np.int64)
+    others = np.arange(len(centers)) != patch
+    for s in range(0, len(x), chunk):
+        e = min(len(x), s + chunk)
+        dx = x[s:e, None] - centers[None, :, 0]
+        dy = y[s:e, None] - centers[None, :, 1]
+        if z is None:
+            rsq = dx * dx + dy * dy
+            psq = rsq[:, patch:patch + 1]
+        else:
+            dz = z[s:e, None] - centers[None, :, 2]
+            rsq = dx * dx + (dy * dy + dz * dz)
+            psq = (dx[:, patch:patch + 1] * dx[:, patch:patch + 1] +
+                   dy[:, patch:patch + 1] * dy[:, patch:patch + 1]) + \
+                dz[:, patch:patch + 1] * dz[:, patch:patch + 1]
+        out[s:e] = ~np.any(rsq[:, others] < psq, axis=1)
+    return out
+
+
 def quick_assign(centers, x, y, z=None, chunk=1 << 16):
     """QuickAssign (src/KMeans.cpp:890-942): for each object the index of the nearest
     patch centre, squared distance (dx^2 + dy^2) + dz^2 in that order, the first

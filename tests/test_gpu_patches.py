@@ -424,3 +424,61 @@ def test_gpu_catalog_npatch_uses_kmeans(gpu):
     d = ((pos[:, None, :] - cen[None, :, :]) ** 2).sum(axis=2)
     assert np.mean(np.argmin(d, axis=1) == p) > 0.999
     assert len(tcat.get_patches()) == 25
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dim', [2, 3])
+def test_gpu_select_patch_matches_oracle(gpu, oracle_lib, dim):
+    """SelectPatch on the GPU against the oracle's restatement (itself pinned to the
+    compiled reference), with two equal centres; and a single-patch Catalog built from
+    patch_centers keeps exactly those objects."""
+    import ctypes
+    from treecorr_amd import _lib
+    rng = np.random.default_rng(31 + dim)
+    n, npatch = 60_000, 29
+    x, y, z = rng.random(n), rng.random(n), rng.random(n)
+    cen = rng.random((npatch, dim))
+    cen[11] = cen[3]
+    for patch in (0, 3, 11, 28):
+        use = np.empty(n, dtype=np.int32)
+        _lib.check(_lib.lib().tc_select_patch(
+            patch, _lib.dptr(np.ascontiguousarray(cen.ravel())), npatch, _lib.dptr(x),
+            _lib.dptr(y), _lib.dptr(z) if dim == 3 else None, n,
+            use.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        ref = oracle_lib.select_patch(patch, cen, x, y, z if dim == 3 else None)
+        np.testing.assert_array_equal(use, ref)
+    kw = dict(x=x, y=y) if dim == 2 else dict(x=x, y=y, z=z)
+    cat = gpu.Catalog(patch=3, patch_centers=cen, **kw)
+    sel = oracle_lib.select_patch(3, cen, x, y, z if dim == 3 else None).astype(bool)
+    np.testing.assert_array_equal(cat.x, x[sel])
+    assert cat.patch == 3 and cat.npatch == npatch
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('cross', [False, True])
+def test_gpu_local_patch_method(gpu, oracle_lib, cross):
+    """patch_method='local' (corr2base.py:782-834, 918-926, 1016-1024): each patch of the
+    first catalog against the objects of the other near it.  With exact binning every
+    pair within max_sep of a patch's objects is among them, so results[(i, i)] equals the
+    oracle's cross correlation of patch i with the whole other catalog, and the totals
+    are the sums over the patches."""
+    O = oracle_lib
+    ng, min_top = 4, 3
+    c1, scale = C.make_catalog(FLAT, 5000, 351, 0.3)
+    c2 = C.make_catalog(FLAT, 4000, 352, 0.3)[0] if cross else c1
+    l1 = grid_labels(c1['x'], c1['y'], scale, ng)
+    l2 = grid_labels(c2['x'], c2['y'], scale, ng)
+    bn = C.binning(scale, 10, 0.01, 0.15, 0.0, 0.0)
+    kk = corr_obj(gpu, 1, bn, 0.0, 0.0, min_top)
+    a = front_catalog(gpu, c1, FLAT, l1)
+    if cross:
+        kk.process(a, front_catalog(gpu, c2, FLAT, l2), patch_method='local', finalize=False)
+    else:
+        kk.process(a, patch_method='local', finalize=False)
+    assert sorted(kk.results) == [(i, i) for i in range(ng * ng)]
+    tot = 0
+    for i in range(ng * ng):
+        ref = oracle_raw(O, 1, FLAT, sub(c1, l1 == i), c2, bn, min_top)
+        check_pair(kk.results[(i, i)], ref, 1)
+        tot += ref['npairs']
+    np.testing.assert_array_equal(kk.npairs, tot)

@@ -21,20 +21,24 @@ def _trees(O, R, coords, n, seed, max_top):
 
 @pytest.mark.parametrize('coords,npatch,max_top', [(3, 20, 4), (1, 7, 2), (2, 40, 5),
                                                    (3, 3, 4)])
-@pytest.mark.parametrize('init', ['tree', 'random'])
+@pytest.mark.parametrize('init', ['tree', 'random', 'kmeans++'])
 def test_kmeans_init_matches_compiled_reference(oracle_lib, coords, npatch, max_top, init):
     from oracle import ref as R
     from treecorr_amd import kmeans
     if R.module() is None:
         pytest.skip('oracle/_ref not built')
     d, rf = _trees(oracle_lib, R, coords, 5000, 90 + npatch, max_top)
-    T = kmeans.TreeView(d['pos'], d['n'], d['left'], d['right'], d['top'])
+    T = kmeans.TreeView(d['pos'], d['n'], d['left'], d['right'], d['top'], d['size'])
     dim = 2 if coords == 1 else 3
+    name = {'tree': 'KMeansInitTree', 'random': 'KMeansInitRand', 'kmeans++': 'KMeansInitKMPP'}
     for seed in (12345, 987654321):
         ref = np.zeros((npatch, dim))
-        getattr(rf, 'KMeansInitTree' if init == 'tree' else 'KMeansInitRand')(ref, npatch, seed)
-        fn = kmeans.init_tree if init == 'tree' else kmeans.init_random
-        got = fn(T, npatch, seed)[:, :dim]
+        getattr(rf, name[init])(ref, npatch, seed)
+        if init == 'kmeans++':
+            got = kmeans.init_kmpp(T, npatch, seed, coords == 2)[:, :dim]
+        else:
+            fn = kmeans.init_tree if init == 'tree' else kmeans.init_random
+            got = fn(T, npatch, seed)[:, :dim]
         np.testing.assert_array_equal(got, ref)
 
 

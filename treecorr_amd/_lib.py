@@ -46,6 +46,7 @@ SIGNATURES = {
     'tc_corr2_process_pairs': (_i, [_vp, ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp), _i, _ip,
                                     _ip, _i, _i, _i, _dp]),
     'tc_quick_assign': (_i, [_dp, _i, _dp, _dp, _dp, _ll, _ip]),
+    'tc_select_patch': (_i, [_i, _dp, _i, _dp, _dp, _dp, _ll, _ip]),
     'tc_set_tree_speculation': (_i, [_i]),
     'tc_set_traversal_mode': (_i, [_i]),
     'tc_traversal_fallbacks': (_ll, []),

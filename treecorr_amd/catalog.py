@@ -47,6 +47,10 @@ class Catalog:
         self._v1, self._v2 = arr(v1), arr(v2)
         self._t1, self._t2 = arr(t1), arr(t2)
         self._q1, self._q2 = arr(q1), arr(q2)
+        # catalog.py:862-875: an integer patch makes this the catalog of that one patch
+        single = None
+        if patch is not None and np.ndim(patch) == 0:
+            single, patch = int(patch), None
         self._patch = None if patch is None else np.asarray(patch, dtype=int).ravel()
         if patch_centers is not None and patch is not None:
             raise ValueError("Cannot provide both patch and patch_centers")
@@ -112,7 +116,9 @@ class Catalog:
 
         self._fields = {}
         npatch = cfg.get('npatch')
-        if self._centers is not None:
+        if single is not None:
+            self._select_single_patch(single)
+        elif self._centers is not None:
             self._assign_patches()
         elif self._patch is None and npatch is not None and int(npatch) != 1:
             # catalog.py:1473-1484: patches from k-means on an N field whose top level
@@ -130,6 +136,37 @@ class Catalog:
             self._fields = {}
         else:
             self._npatch = 1 if self._patch is None else int(self._patch.max()) + 1
+
+    def _select_single_patch(self, p):
+        # catalog.py:1576-1594, 1624-1630: keep the objects of patch p -- those for which
+        # no other centre is strictly nearer (SelectPatch, on the GPU) when centres are
+        # given, else all of them
+        if self._centers is not None:
+            c = self._centers
+            dim = 2 if self.coords == 'flat' else 3
+            if c.ndim != 2 or c.shape[1] != dim:
+                raise ValueError("Centers array has wrong shape.")
+            self._generate_xyz()
+            use = np.empty(self.ntot, dtype=np.int32)
+            z = None if dim == 2 else np.ascontiguousarray(self._z, dtype=float)
+            _lib.check(_lib.lib().tc_select_patch(
+                p, _lib.dptr(c.ravel()), len(c),
+                _lib.dptr(np.ascontiguousarray(self._x, dtype=float)),
+                _lib.dptr(np.ascontiguousarray(self._y, dtype=float)), _lib.dptr(z), self.ntot,
+                use.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+            self._select(np.where(use)[0])
+            self._npatch = len(c)
+        elif self.config.get('npatch') is not None:       # catalog.py:883-886
+            self._npatch = int(self.config['npatch'])
+            if self._npatch < 1:
+                raise ValueError("npatch must be >= 1")
+        elif p:
+            raise ValueError("Either npatch or patch_centers is required when providing "
+                             "an integer patch value.")
+        else:
+            self._npatch = 1
+        self._patch = None
+        self._single_patch = p
 
     def _assign_patches(self):
         # catalog.py:1466-1472, 1556-1566: the nearest of the given centres (QuickAssign
@@ -191,7 +228,9 @@ class Catalog:
     t2 = property(lambda self: self._t2)
     q1 = property(lambda self: self._q1)
     q2 = property(lambda self: self._q2)
-    patch = property(lambda self: self._patch)
+    # catalog.py:1199-1204: a single-patch catalog reports its patch number
+    patch = property(lambda self: self._single_patch if self._single_patch is not None
+                     else self._patch)
     npatch = property(lambda self: self._npatch)
     nontrivial_w = property(lambda self: self._nontrivial_w)
 

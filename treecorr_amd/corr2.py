@@ -318,8 +318,6 @@ class Corr2:
             raise NotImplementedError("MPI is not part of the single-GPU engine")
         if patch_method not in ('local', 'global'):
             raise ValueError("Invalid patch_method %s" % patch_method)
-        if patch_method == 'local':
-            raise NotImplementedError("patch_method='local' is not implemented")
         if cat2 is None and self._d2 is not None:
             raise TypeError("%s.process needs cat2" % type(self).__name__)
         if initialize:
@@ -331,7 +329,9 @@ class Corr2:
             cat1 = cat1.get_patches()
         if cat2 is not None and not isinstance(cat2, list):
             cat2 = cat2.get_patches()
-        if cat2 is None:
+        if patch_method == 'local' and (len(cat1) > 1 or cat1[0].npatch != 1):
+            self._process_all_local(cat1, cat2, metric, num_threads, corr_only)
+        elif cat2 is None:
             self._process_all_auto(cat1, metric, num_threads, corr_only)
         else:
             self._process_all_cross(cat1, cat2, metric, num_threads, corr_only)
@@ -496,6 +496,95 @@ class Corr2:
             i, j = idx1[ii], idx2[jj]
             self._record_pair((i, j), rows.get((ii, jj)), cat1[ii], cat2[jj],
                               i == j or n1 == 1 or n2 == 1)
+
+    def _make_expanded_patch(self, cat1, cat2):
+        # corr2base.py:782-834: the objects of the catalogs cat2 within max_sep + s1 of
+        # cat1's centre (distances on the unit sphere for 3-d positions), as one catalog
+        # whose sumw is all of cat2's (for NN's tot); None when there are none
+        from .catalog import Catalog, _COLUMNS
+        x1, y1, z1, s1 = cat1._get_center_size()
+        threed = self.coords == '3d'
+        if threed:
+            r = (x1 ** 2 + y1 ** 2 + z1 ** 2) ** 0.5
+            if r != 0:
+                x1, y1, z1 = x1 / r, y1 / r, z1 / r
+        cats, masks, sumw = [], [], 0.
+        for p in cat2:
+            p._generate_xyz()
+            sumw += p.sumw
+            x2, y2, z2, s2 = p._get_center_size()
+            if threed:
+                r = (x2 ** 2 + y2 ** 2 + z2 ** 2) ** 0.5
+                if r != 0:
+                    x2, y2, z2 = x2 / r, y2 / r, z2 / r
+            if ((x2 - x1) ** 2 + (y2 - y1) ** 2 + (z2 - z1) ** 2) ** 0.5 > \
+                    self._max_sep + s1 + s2:
+                continue
+            px, py, pz = p.x, p.y, (0 if p.z is None else p.z)
+            if threed:
+                pr = np.sqrt(px ** 2 + py ** 2 + pz ** 2)
+                pr[pr == 0] = 1
+                px, py, pz = px / pr, py / pr, pz / pr
+            d = np.sqrt((px - x1) ** 2 + (py - y1) ** 2 + (pz - z1) ** 2)
+            mask = np.where(d < s1 + self._max_sep)[0]
+            if len(mask) == 0:
+                continue
+            cats.append(p)
+            masks.append(mask)
+        if not cats:
+            return None
+        # Catalog.combine (catalog.py:1490-1560) of the masked rows
+        e = Catalog.__new__(Catalog)
+        e.__dict__.update(cats[0].__dict__)
+        for nm in _COLUMNS:
+            if getattr(cats[0], nm) is not None:
+                setattr(e, nm, np.concatenate([getattr(c, nm)[m] for c, m in zip(cats, masks)]))
+        e._patch = None
+        e._single_patch = None
+        e._npatch = 1
+        e._fields = {}
+        e._cen_s = None
+        e._patches = None
+        e._split_parent = None
+        e._cache = {'sumw': float(sumw)}
+        e.name = cat1.name + " (expanded)" if cat1.name else ''
+        return e
+
+    def _process_all_local(self, cat1, cat2, metric, num_threads, corr_only):
+        # corr2base.py:918-926 (auto) and 1016-1024 (cross): each patch of cat1 against the
+        # objects of cat2 (cat1 for an auto correlation) near it, stored as the pair (i, i)
+        # and added to the totals; all patches in one batched traversal
+        auto = cat2 is None
+        if self.npatch1 == 1:
+            self.npatch1 = cat1[0].npatch if cat1[0].npatch != 1 else len(cat1)
+        if auto:
+            self.npatch2 = self.npatch1
+        elif self.npatch2 == 1:
+            self.npatch2 = cat2[0].npatch if cat2[0].npatch != 1 else len(cat2)
+        self._set_metric(metric, cat1[0].coords, None if auto else cat2[0].coords)
+        if num_threads is not None:
+            set_omp_threads(num_threads)
+        self._corr_only = corr_only
+        others = cat1 if auto else cat2
+        idx1 = [c._single_patch if c._single_patch is not None else ii
+                for ii, c in enumerate(cat1)]
+        exp = [self._make_expanded_patch(c1, others) for c1 in cat1]
+        run = [e is not None and not self._trivially_zero(c1, e) for c1, e in zip(cat1, exp)]
+        jobs = [(ii, ii) for ii in range(len(cat1)) if run[ii]]
+        rows = {}
+        if jobs:
+            min_size, max_size = self._get_minmax_size()
+            kw = dict(min_size=min_size, max_size=max_size, split_method=self.split_method,
+                      min_top=self.min_top, max_top=self.max_top, coords=self.coords)
+            f1 = self._patch_fields(cat1, self.brute is True or self.brute == 1)
+            f2 = [self._field(e, 0 if auto else 1,
+                              brute=bool(self.brute is True or self.brute == 2), **kw)
+                  if e is not None else f1[ii] for ii, e in enumerate(exp)]
+            out = self._run_pairs(f1, f2, jobs, corr_only)
+            rows = {a: out[k] for k, (a, _) in enumerate(jobs)}
+        for ii, c1 in enumerate(cat1):
+            i = idx1[ii]
+            self._record_pair((i, i), rows.get(ii), c1, exp[ii], True)
 
     def _finalize(self):
         # corr2base.py:1274-1294

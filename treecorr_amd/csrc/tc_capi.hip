@@ -749,6 +749,17 @@ int tc_quick_assign(const double* centers, int npatch, const double* x, const do
     return tc::quick_assign(centers, npatch, x, y, z, n, patch);
 }
 
+int tc_select_patch(int patch, const double* centers, int npatch, const double* x,
+                    const double* y, const double* z, long long n, int* use) {
+    if (!centers || npatch <= 0 || !x || !y || n < 0 || (n > 0 && !use))
+        return tc::fail("NULL or invalid argument");
+    if (patch < 0 || patch >= npatch) return tc::fail("patch must be in [0, npatch)");
+    if (n == 0) return 0;
+    if (int rc = check_gpu()) return rc;
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::select_patch(patch, centers, npatch, x, y, z, n, use);
+}
+
 int tc_kmeans_run(double* centers, int npatch, int dim, const double* x, const double* y,
                   const double* z, const double* w, long long n, int max_iter, double tol,
                   double size, int sphere, int* niter) {

@@ -70,6 +70,10 @@ int kmeans_run_noalloc(double* centers, int npatch, int dim, const double* x, co
 // each; z == nullptr means 2), host arrays in and out.
 int quick_assign(const double* centers, int npatch, const double* x, const double* y,
                  const double* z, long long n, int* patch);
+// SelectPatch on the device: use[i] = 1 unless a centre other than `patch` is strictly
+// nearer (host arrays in and out).
+int select_patch(int patch, const double* centers, int npatch, const double* x, const double* y,
+                 const double* z, long long n, int* use);
 
 // The learned cost model of the traversal's jobs (tc_cost.hip): a perceptron of
 // CM_NF inputs, two hidden layers of CM_H ReLU units and one output, FP32.

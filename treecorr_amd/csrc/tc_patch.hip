@@ -51,8 +51,71 @@ __global__ __launch_bounds__(QA_THREADS) void k_quick_assign(const double* __res
     if (live) out[i] = kmin;
 }
 
+// SelectPatch (src/KMeans.cpp:944-1011): use[i] = 1 unless another centre is strictly
+// nearer object i than centre `patch` (a tie keeps the object).  The squared distances
+// are added as the compiled reference adds them: dx^2 + dy^2 in 2-d; in 3-d
+// (dx^2 + dy^2) + dz^2 for the selected centre and dx^2 + (dy^2 + dz^2) for the others
+// (its -ffast-math build; this decides objects equidistant from two equal centres).
+template <int DIM>
+__global__ __launch_bounds__(QA_THREADS) void k_select_patch(const double* __restrict__ centers,
+                                                             int npatch, int patch,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ y,
+                                                             const double* __restrict__ z,
+                                                             long long n, int* __restrict__ use) {
+    __shared__ double sc[QA_CHUNK * DIM];
+    const long long i = (long long)blockIdx.x * QA_THREADS + threadIdx.x;
+    const bool live = i < n;
+    const double px = live ? x[i] : 0., py = live ? y[i] : 0.;
+    const double pz = (DIM == 3 && live) ? z[i] : 0.;
+    double p_dsq;
+    {
+        const double dx = px - centers[(size_t)patch * DIM], dy = py - centers[(size_t)patch * DIM + 1];
+        p_dsq = dx * dx + dy * dy;
+        if (DIM == 3) {
+            const double dz = pz - centers[(size_t)patch * DIM + 2];
+            p_dsq = p_dsq + dz * dz;
+        }
+    }
+    int keep = 1;
+    for (int c0 = 0; c0 < npatch; c0 += QA_CHUNK) {
+        const int nc = npatch - c0 < QA_CHUNK ? npatch - c0 : QA_CHUNK;
+        __syncthreads();
+        for (int t = threadIdx.x; t < nc * DIM; t += QA_THREADS) sc[t] = centers[(size_t)c0 * DIM + t];
+        __syncthreads();
+        for (int k = 0; k < nc; ++k) {
+            if (c0 + k == patch) continue;
+            const double dx = px - sc[k * DIM], dy = py - sc[k * DIM + 1];
+            double q_dsq;
+            if (DIM == 3) {
+                const double dz = pz - sc[k * DIM + 2];
+                q_dsq = dx * dx + (dy * dy + dz * dz);
+            } else {
+                q_dsq = dx * dx + dy * dy;
+            }
+            if (q_dsq < p_dsq) keep = 0;
+        }
+    }
+    if (live) use[i] = keep;
+}
+
+static int assign_or_select(const double* centers, int npatch, int select, const double* x,
+                            const double* y, const double* z, long long n, int* out);
+
 int quick_assign(const double* centers, int npatch, const double* x, const double* y,
                  const double* z, long long n, int* patch) {
+    return assign_or_select(centers, npatch, -1, x, y, z, n, patch);
+}
+
+int select_patch(int patch, const double* centers, int npatch, const double* x, const double* y,
+                 const double* z, long long n, int* use) {
+    if (patch < 0 || patch >= npatch) return fail("patch must be in [0, npatch)");
+    return assign_or_select(centers, npatch, patch, x, y, z, n, use);
+}
+
+// QuickAssign (select < 0) or SelectPatch (select = the patch), host arrays in and out.
+static int assign_or_select(const double* centers, int npatch, int select, const double* x,
+                            const double* y, const double* z, long long n, int* patch) {
     hipStream_t st = stream();
     const int dim = z ? 3 : 2;
     const size_t cb = sizeof(double) * (size_t)npatch * dim, pb = sizeof(double) * (size_t)n;
@@ -78,8 +141,14 @@ int quick_assign(const double* centers, int npatch, const double* x, const doubl
     if (e == hipSuccess && z) e = hipMemcpyAsync(dz, z, pb, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
         const dim3 grid((unsigned)((n + QA_THREADS - 1) / QA_THREADS));
-        if (z) hipLaunchKernelGGL(k_quick_assign<3>, grid, dim3(QA_THREADS), 0, st, dc, npatch, dx, dy, dz, n, dp);
-        else hipLaunchKernelGGL(k_quick_assign<2>, grid, dim3(QA_THREADS), 0, st, dc, npatch, dx, dy, dz, n, dp);
+        if (select >= 0) {
+            if (z) hipLaunchKernelGGL(k_select_patch<3>, grid, dim3(QA_THREADS), 0, st, dc, npatch, select, dx, dy, dz, n, dp);
+            else hipLaunchKernelGGL(k_select_patch<2>, grid, dim3(QA_THREADS), 0, st, dc, npatch, select, dx, dy, dz, n, dp);
+        } else if (z) {
+            hipLaunchKernelGGL(k_quick_assign<3>, grid, dim3(QA_THREADS), 0, st, dc, npatch, dx, dy, dz, n, dp);
+        } else {
+            hipLaunchKernelGGL(k_quick_assign<2>, grid, dim3(QA_THREADS), 0, st, dc, npatch, dx, dy, dz, n, dp);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(patch, dp, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, st);

@@ -72,8 +72,9 @@ class TreeView:
     """A tree as arrays: pos (nnode x 3), n, left, right (-1 at leaves), top (the
     top-level cells in TreeCorr's order)."""
 
-    def __init__(self, pos, n, left, right, top):
+    def __init__(self, pos, n, left, right, top, size=None):
         self.pos, self.n, self.left, self.right, self.top = pos, n, left, right, top
+        self.size = size          # the cells' sizes, float32 as the cells hold them
 
     @classmethod
     def from_device_nodes(cls, nodes, top):
@@ -82,7 +83,7 @@ class TreeView:
         left = np.asarray(nodes['child'], dtype=np.int64)
         right = np.where(left >= 0, left + 1, -1)
         return cls(pos, np.asarray(nodes['n'], dtype=np.int64), left, right,
-                   np.asarray(top, dtype=np.int64))
+                   np.asarray(top, dtype=np.int64), np.asarray(nodes['size'], dtype=np.float32))
 
 
 def _init_tree_rec(T, centers, cell, first, ncenters):
@@ -155,6 +156,89 @@ def init_random(T, npatch, seed):
         for i1 in range(i):
             if np.array_equal(centers[i1], centers[i]):
                 centers[i] = centers[i] * (1. + urand() * 1.e-8)
+    return centers
+
+
+class _LeafAlreadyUsed(Exception):
+    pass
+
+
+def _nsq(a, b):
+    d = a - b
+    return (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+
+
+def _kmpp_descend(T, cell, centers, ncen):
+    # InitializeCentersKMPP (recursive), src/KMeans.cpp:159-218: from a chosen top cell,
+    # walk down picking the daughter with probability proportional to its squared distance
+    # to the nearest centre so far, to a leaf not yet used as a centre
+    if T.size[cell] == 0:
+        p = T.pos[cell]
+        for j in range(ncen):
+            if np.array_equal(p, centers[j]):
+                raise _LeafAlreadyUsed()
+        return p
+    left, right = T.left[cell], T.right[cell]
+    dsq1 = _nsq(T.pos[left], centers[0])
+    dsq2 = _nsq(T.pos[right], centers[0])
+    for j in range(1, ncen):
+        d1 = _nsq(T.pos[left], centers[j])
+        d2 = _nsq(T.pos[right], centers[j])
+        if d1 < dsq1:
+            dsq1 = d1
+        if d2 < dsq2:
+            dsq2 = d2
+    if dsq1 + dsq2 == 0.:
+        raise _LeafAlreadyUsed()
+    u = urand() * (dsq1 + dsq2)
+    first, second = (left, right) if u < dsq1 else (right, left)
+    try:
+        return _kmpp_descend(T, first, centers, ncen)
+    except _LeafAlreadyUsed:
+        return _kmpp_descend(T, second, centers, ncen)
+
+
+def init_kmpp(T, npatch, seed, threed):
+    """Initial centres by k-means++ over the tree (InitializeCentersKMPP,
+    src/KMeans.cpp:220-313): the first at a random object; each next one from a top cell
+    drawn with probability (n - centres already there) (d^2 + c s^2) (d the distance to the
+    nearest centre, s the cell's size, c = 3/5 in 3-d, 1/2 otherwise), then by the descent
+    above.  Returns an (npatch, 3) array."""
+    centers = np.zeros((npatch, 3))
+    top = [int(c) for c in T.top]
+    ncells = len(top)
+    urand(seed)
+    ntot = int(sum(T.n[c] for c in top))
+    per_cell = [0] * ncells
+    m = int(urand() * ntot)
+    for k, c in enumerate(top):
+        if m < T.n[c]:
+            centers[0] = T.pos[_leaf_number(T, c, m)]
+            per_cell[k] += 1
+            break
+        m -= T.n[c]
+    coef = 0.6 if threed else 0.5
+    for i in range(1, npatch):
+        p = [0.] * ncells
+        sump = 0.
+        for k, c in enumerate(top):
+            dsq1 = _nsq(centers[0], T.pos[c])
+            for i1 in range(1, i):
+                dsq2 = _nsq(centers[i1], T.pos[c])
+                if dsq2 < dsq1:
+                    dsq1 = dsq2
+            s = float(T.size[c])                  # the cell's float size, as a double
+            ssq = s * s
+            p[k] = float(T.n[c] - per_cell[k]) * (dsq1 + coef * ssq)
+            sump += p[k]
+        u = urand()
+        for k, c in enumerate(top):
+            p[k] /= sump
+            if u < p[k]:
+                centers[i] = _kmpp_descend(T, c, centers, i)
+                per_cell[k] += 1
+                break
+            u -= p[k]
     return centers
 
 
