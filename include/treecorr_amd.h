@@ -309,6 +309,20 @@ int tc_corr3_create(int d, int bin_type, double minsep, double maxsep, int nbins
                     double* meanv, double* weight, double* weight_im, double* ntri,
                     tc_corr3** out);
 int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int metric);
+/* Multipole cross-correlations: replace BaseCorr3.processCross12 / processCross21 /
+ * processCross (bound at src/Corr3.cpp:4199-4201; ProcessCross12c :3731-3768,
+ * ProcessCross21c :3831-3870, ProcessCrossc :3938-3985).  `ordered` is the code
+ * treecorr/corr3base.py passes (:1968, 2027, 2086): cross12 0 or 1 (one point from f1,
+ * two from f2), cross21 0 or 3 (two from f1, one from f2), cross 0 to 4.  Each runs the
+ * reference's multipole passes (multipole(f1, f2) :1177-1236, multipole(f1, f2, f3,
+ * ordered) :1238-1305) and ADDS to the arrays, as processAuto does.  Every field must
+ * have the correlation's type and one coordinate system. */
+int tc_corr3_process_cross12(tc_corr3* c, tc_field* f1, tc_field* f2, int ordered, int dots,
+                             int quick, int metric);
+int tc_corr3_process_cross21(tc_corr3* c, tc_field* f1, tc_field* f2, int ordered, int dots,
+                             int quick, int metric);
+int tc_corr3_process_cross(tc_corr3* c, tc_field* f1, tc_field* f2, tc_field* f3, int ordered,
+                           int dots, int quick, int metric);
 /* Work counters of the last process call: Gn calls, zeta calls, zeta rows,
  * multipoleFinish stages (no reference counterpart; roofline accounting). */
 int tc_corr3_counters(const tc_corr3* c, long long* counters4);

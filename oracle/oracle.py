@@ -62,6 +62,10 @@ def lib():
             L.tco_corr3_multipole.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int] + [ctypes.c_double] * 5 + [
                 ctypes.c_int] * 3 + [_dp] * 17 + [ctypes.POINTER(ctypes.c_longlong)]
+        if hasattr(L, 'tco_corr3_multipole_x'):
+            L.tco_corr3_multipole_x.argtypes = [vp, vp, vp] + [ctypes.c_int] * 5 + [
+                ctypes.c_double] * 5 + [ctypes.c_int] + [_dp] * 15 + [
+                ctypes.POINTER(ctypes.c_longlong)]
         _LIB = L
     return _LIB
 
@@ -197,6 +201,46 @@ def corr3_multipole(d, coords, f, nbins, min_sep, max_sep, bin_size, b, a, max_n
                                *[_ptr(out[k]) for k in Z3_KEYS], None, None, cnt)
     if rc != 0:
         raise RuntimeError('oracle multipole failed')
+    shape = (nbins, nbins, 2 * max_n + 1)
+    res = {k: v.reshape(shape) for k, v in out.items()}
+    res['counters'] = dict(gn_calls=cnt[0], zeta_calls=cnt[1], zeta_rows=cnt[2], stages=cnt[3])
+    return res
+
+
+# The multipole passes of the cross-correlation drivers (src/Corr3.cpp: ProcessCross12c
+# :3731-3768, ProcessCross21c :3831-3870, ProcessCrossc :3938-3985): (c1 field, list
+# fields, ordered) per pass, fields named 1, 2, 3; one list field = the one-list form.
+MULTIPOLE_PASSES = {
+    ('cross12', 0): [((2, 1, 2), 1), ((1, 2), None)],
+    ('cross12', 1): [((1, 2), None)],
+    ('cross21', 0): [((2, 1, 1), 1), ((1, 2, 1), 1), ((1, 1, 2), 1)],
+    ('cross21', 3): [((1, 1, 2), 4)],
+    ('cross', 0): [((2, 1, 3), 1), ((3, 1, 2), 1), ((1, 2, 3), 1)],
+    ('cross', 1): [((1, 2, 3), 1)],
+    ('cross', 2): [((1, 2, 3), 4), ((3, 2, 1), 4)],
+    ('cross', 3): [((1, 2, 3), 4), ((2, 1, 3), 4)],
+    ('cross', 4): [((1, 2, 3), 4)],
+}
+
+
+def corr3_multipole_cross(d, coords, kind, ordered, fields, nbins, min_sep, max_sep, bin_size,
+                          b, a, max_n, nthreads=0):
+    """The oracle's multipole cross-correlation: kind 'cross12' (fields f1, f2; ordered 0
+    or 1), 'cross21' (f1, f2; 0 or 3) or 'cross' (f1, f2, f3; 0 to 4), run as the
+    reference's drivers run it, pass by pass.  Raw sums as corr3_multipole returns them."""
+    L = lib()
+    ntot = nbins * nbins * (2 * max_n + 1)
+    out = {k: np.zeros(ntot) for k in Z3_KEYS}
+    cnt = (ctypes.c_longlong * 4)()
+    for names, order in MULTIPOLE_PASSES[(kind, ordered)]:
+        fs = [fields[i - 1].h for i in names]
+        f3 = fs[2] if len(fs) == 3 else None
+        rc = L.tco_corr3_multipole_x(fs[0], fs[1], f3, 0 if order is None else order, d, coords,
+                                     nbins, int(max_n), float(min_sep), float(max_sep),
+                                     float(bin_size), float(b), float(a), int(nthreads),
+                                     *[_ptr(out[k]) for k in Z3_KEYS], cnt)
+        if rc != 0:
+            raise RuntimeError('oracle multipole failed')
     shape = (nbins, nbins, 2 * max_n + 1)
     res = {k: v.reshape(shape) for k, v in out.items()}
     res['counters'] = dict(gn_calls=cnt[0], zeta_calls=cnt[1], zeta_rows=cnt[2], stages=cnt[3])

@@ -154,11 +154,7 @@ def corr2(d, f1, f2, nbins, min_sep, max_sep, bin_size, b, a, quick=False, nthre
 _CORR3 = {0: 'NNNCorr', 4: 'GGGCorr'}
 
 
-def corr3_multipole(d, f, nbins, min_sep, max_sep, bin_size, b, a, max_n, nthreads=None):
-    """Run the reference LogMultipole auto-correlation (src/Corr3.cpp:1124-1175).
-
-    Constructor arguments follow treecorr/corr3base.py:1116-1131 with the
-    LogMultipole settings of :691-705 (nubins = max_n, u/v ranges zero)."""
+def _corr3_mp(d, nbins, min_sep, max_sep, bin_size, b, a, max_n, nthreads):
     m = module()
     if nthreads is not None:
         m.SetOMPThreads(int(nthreads))
@@ -176,6 +172,28 @@ def corr3_multipole(d, f, nbins, min_sep, max_sep, bin_size, b, a, max_n, nthrea
                                  out['meanlogd1'], out['meand2'], out['meanlogd2'], out['meand3'],
                                  out['meanlogd3'], out['meanu'], out['meanv'], out['weight'],
                                  out['weight_im'], out['ntri'])
+    return m, corr, out
+
+
+def corr3_multipole(d, f, nbins, min_sep, max_sep, bin_size, b, a, max_n, nthreads=None):
+    """Run the reference LogMultipole auto-correlation (src/Corr3.cpp:1124-1175).
+
+    Constructor arguments follow treecorr/corr3base.py:1116-1131 with the
+    LogMultipole settings of :691-705 (nubins = max_n, u/v ranges zero)."""
+    m, corr, out = _corr3_mp(d, nbins, min_sep, max_sep, bin_size, b, a, max_n, nthreads)
     corr.processAuto(f, False, False, m.Euclidean)
+    shape = (nbins, nbins, 2 * max_n + 1)
+    return {k: v.reshape(shape) for k, v in out.items()}
+
+
+def corr3_multipole_cross(d, kind, ordered, fields, nbins, min_sep, max_sep, bin_size, b, a,
+                          max_n, nthreads=None):
+    """The reference's LogMultipole cross-correlations through its own drivers:
+    processCross12 / processCross21 (fields f1, f2) and processCross (f1, f2, f3) with the
+    ordered code the Python layer passes (corr3base.py:1968, 2027, 2086)."""
+    m, corr, out = _corr3_mp(d, nbins, min_sep, max_sep, bin_size, b, a, max_n, nthreads)
+    fn = {'cross12': corr.processCross12, 'cross21': corr.processCross21,
+          'cross': corr.processCross}[kind]
+    fn(*fields, int(ordered), False, False, m.Euclidean)
     shape = (nbins, nbins, 2 * max_n + 1)
     return {k: v.reshape(shape) for k, v in out.items()}

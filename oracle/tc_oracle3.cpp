@@ -102,13 +102,19 @@ struct Out3 {
     }
 };
 
+// c1 cells come from T1; the cells of a list from T (and, for the second list of the
+// three-field form, T3), with their node sums S (S3).
 template <int C>
 struct MP {
     const P3& P;
+    const Tree& T1;
     const Tree& T;
     const std::vector<NodeSums>& S;
     Out3& O;
-    MP(const P3& p, const Tree& t, const std::vector<NodeSums>& s, Out3& o) : P(p), T(t), S(s), O(o) {}
+    const Tree* T3 = nullptr;
+    const std::vector<NodeSums>* S3 = nullptr;
+    MP(const P3& p, const Tree& t1, const Tree& t, const std::vector<NodeSums>& s, Out3& o)
+        : P(p), T1(t1), T(t), S(s), O(o) {}
 
     static double dist_sq(const Pt& a, const Pt& b) {
         Pt r{a.x - b.x, a.y - b.y, a.z - b.z};
@@ -159,8 +165,9 @@ struct MP {
     }
 
     // Corr3::calculateGn + MultipoleScratch::calculateGn
-    void calculate_gn(const Node& c1, int i2, double r, double logr, int k, Scratch& mp) const {
-        const Node& c2 = T.nodes[i2];
+    void calculate_gn(const Node& c1, const Tree& TL, const std::vector<NodeSums>& S, int i2,
+                      double r, double logr, int k, Scratch& mp) const {
+        const Node& c2 = TL.nodes[i2];
         O.gn_calls++;
         double n = double(c2.n), w = c2.w;
         mp.npairs[k] += n;
@@ -334,10 +341,11 @@ struct MP {
     }
 
     // splitC2Cells, src/Corr3.cpp:1323-1372
-    void split_c2(const Node& c1, const std::vector<int>& in, std::vector<int>& out) const {
+    void split_c2(const Node& c1, const Tree& TL, const std::vector<int>& in,
+                  std::vector<int>& out) const {
         double s1 = c1.size;
         for (int i2 : in) {
-            const Node& c2 = T.nodes[i2];
+            const Node& c2 = TL.nodes[i2];
             double s2 = c2.size;
             double rsq = dist_sq(c1.p, c2.p);
             double s1ps2 = s1 + s2;
@@ -350,13 +358,14 @@ struct MP {
     }
 
     // splitC2CellsOrCalculateGn, src/Corr3.cpp:1452-1546
-    double split_or_gn(const Node& c1, const std::vector<int>& in, std::vector<int>& out,
+    double split_or_gn(const Node& c1, const Tree& TL, const std::vector<NodeSums>& SL,
+                       const std::vector<int>& in, std::vector<int>& out,
                        bool& anysplit1, Scratch& mp, double prev_max) const {
         double s1 = c1.size;
         double max_remaining_r = 0.;
         double thresh1 = (1. - P.binsize) * prev_max;   // oneBinLessThan, BinType.h:153-154
         for (int i2 : in) {
-            const Node& c2 = T.nodes[i2];
+            const Node& c2 = TL.nodes[i2];
             double s2 = c2.size;
             double rsq = dist_sq(c1.p, c2.p);
             double s1ps2 = s1 + s2;
@@ -370,7 +379,7 @@ struct MP {
                         logr = std::log(r);
                         k = int((logr - P.logminsep) / P.binsize);
                     }
-                    calculate_gn(c1, i2, r, logr, k, mp);
+                    calculate_gn(c1, TL, SL, i2, r, logr, k, mp);
                 }
                 continue;
             }
@@ -389,10 +398,10 @@ struct MP {
     // multipoleFinish, src/Corr3.cpp:1548-1613
     void finish(int i1, const std::vector<int>& c2list, Scratch& mp, int mink_zeta, double maxr) const {
         O.stages++;
-        const Node& c1 = T.nodes[i1];
+        const Node& c1 = T1.nodes[i1];
         bool anysplit1 = false;
         std::vector<int> newlist;
-        maxr = split_or_gn(c1, c2list, newlist, anysplit1, mp, maxr);
+        maxr = split_or_gn(c1, T, S, c2list, newlist, anysplit1, mp, maxr);
         if (!newlist.empty()) {
             if (maxr < P.maxsep) {
                 int k = maxr < P.minsep ? 0
@@ -416,10 +425,10 @@ struct MP {
 
     // multipoleSplit1, src/Corr3.cpp:1374-1408
     void split1(int i1, const std::vector<int>& c2list, Scratch& mp) const {
-        const Node& c1 = T.nodes[i1];
+        const Node& c1 = T1.nodes[i1];
         double s1 = c1.size;
         std::vector<int> newlist;
-        split_c2(c1, c2list, newlist);
+        split_c2(c1, T, c2list, newlist);
         double maxbsq_eff = P.maxsepsq * std::min(P.bsq, P.asq);
         if (s1 * s1 > maxbsq_eff) {
             split1(c1.left, newlist, mp);
@@ -429,29 +438,127 @@ struct MP {
             finish(i1, newlist, mp, P.nbins, 0.);
         }
     }
+
+    // ---- three fields: c1 from T1, the first list from T, the second from T3 ----
+
+    // Corr3::calculateZeta with mp2, mp3 (src/Corr3.cpp:1976-2085) and
+    // MultipoleHelper<5>::CalculateZeta(wg1, ordered, mp2, mp3, ...) (:2851-3003).
+    // ordered 4: vertex 2 from the first list, 3 from the second; ordered 1: both ways.
+    void calculate_zeta3(const Node& c1, int ordered, Scratch& mp2, Scratch& mp3, int kstart,
+                         int mink_zeta) const {
+        O.zeta_calls++;
+        O.zeta_rows += std::max(0, P.nbins - kstart);
+        const int nb = P.nbins, maxn = P.maxn, nn = 2 * maxn + 1;
+        const double w1 = c1.w, n1 = double(c1.n);
+        const bool swap23 = ordered == 1;
+        for (int k2 = kstart; k2 < nb; ++k2) {
+            const int k3end = k2 < mink_zeta ? nb : mink_zeta;
+            for (int k3 = kstart; k3 < k3end; ++k3) {
+                const int i = (k2 * nb + k3) * nn + maxn;
+                if (!swap23) {
+                    // mp2 holds the d3 sums (cells across from p2), mp3 the d2 sums
+                    O.ntri[i] += n1 * mp3.npairs[k2] * mp2.npairs[k3];
+                    const double ww12 = w1 * mp3.sumw[k2], ww13 = w1 * mp2.sumw[k3];
+                    O.meand2[i] += ww13 * mp3.sumwr[k2];
+                    O.meanlogd2[i] += ww13 * mp3.sumwlogr[k2];
+                    O.meand3[i] += ww12 * mp2.sumwr[k3];
+                    O.meanlogd3[i] += ww12 * mp2.sumwlogr[k3];
+                } else {
+                    O.ntri[i] += n1 * (mp2.npairs[k2] * mp3.npairs[k3] +
+                                       mp3.npairs[k2] * mp2.npairs[k3]);
+                    O.meand2[i] += w1 * (mp2.sumw[k3] * mp3.sumwr[k2] + mp3.sumw[k3] * mp2.sumwr[k2]);
+                    O.meanlogd2[i] += w1 * (mp2.sumw[k3] * mp3.sumwlogr[k2] +
+                                            mp3.sumw[k3] * mp2.sumwlogr[k2]);
+                    O.meand3[i] += w1 * (mp2.sumw[k2] * mp3.sumwr[k3] + mp3.sumw[k2] * mp2.sumwr[k3]);
+                    // the second term pairs sumw and sumwlogr of bin k3, as the reference does
+                    O.meanlogd3[i] += w1 * (mp2.sumw[k2] * mp3.sumwlogr[k3] +
+                                            mp3.sumw[k3] * mp2.sumwlogr[k3]);
+                }
+                const int iw2 = mp3.Windex(k2), iw3 = mp2.Windex(k3);
+                for (int m = 0; m <= maxn; ++m) {
+                    cd www = cmul(cmul(cd(w1, 0.), mp3.Wn[iw2 + m]), std::conj(mp2.Wn[iw3 + m]));
+                    if (swap23)
+                        www += cmul(cmul(cd(w1, 0.), mp2.Wn[iw2 + m]), std::conj(mp3.Wn[iw3 + m]));
+                    O.weight[i + m] += www.real();
+                    O.weight_im[i + m] += www.imag();
+                    if (m > 0) {
+                        O.weight[i - m] += www.real();
+                        O.weight_im[i - m] -= www.imag();
+                    }
+                }
+                if (P.D != GData) continue;
+                // gam_mu(n) from G_a at bin k2 and G_b at bin k3 (the general form of
+                // :2883-2897): a = mp3, b = mp2, and with swap23 also a = mp2, b = mp3
+                const cd wg1(c1.gr, c1.gi), wg1c = std::conj(wg1);
+                auto gam = [&](const Scratch& A, const Scratch& B, int n, cd* g) {
+                    const int ia = A.Gindex(k2), ib = B.Gindex(k3);
+                    g[0] += cmul(cmul(wg1, A.Gn[ia + n - 1]), B.Gn[ib - n - 1]);
+                    g[1] += cmul(cmul(wg1c, A.Gn[ia + n + 1]), B.Gn[ib - n + 1]);
+                    g[2] += cmul(cmul(wg1, A.Gn[ia + n - 1]), std::conj(B.Gn[ib + n + 1]));
+                    g[3] += cmul(cmul(wg1, std::conj(A.Gn[ia - n + 1])), B.Gn[ib - n - 1]);
+                };
+                for (int n = -maxn; n <= maxn; ++n) {
+                    cd g[4] = {cd(0), cd(0), cd(0), cd(0)};
+                    gam(mp3, mp2, n, g);
+                    if (swap23) gam(mp2, mp3, n, g);
+                    add8(i + n, g[0], g[1], g[2], g[3]);
+                }
+            }
+        }
+    }
+
+    // multipoleFinish with two lists, src/Corr3.cpp:1615-1679
+    void finish3(int i1, const std::vector<int>& l2, const std::vector<int>& l3, int ordered,
+                 Scratch& mp2, Scratch& mp3, int mink_zeta, double maxr2, double maxr3) const {
+        O.stages++;
+        const Node& c1 = T1.nodes[i1];
+        bool anysplit1 = false;
+        std::vector<int> n2, n3;
+        maxr2 = split_or_gn(c1, T, S, l2, n2, anysplit1, mp2, maxr2);
+        maxr3 = split_or_gn(c1, *T3, *S3, l3, n3, anysplit1, mp3, maxr3);
+        const double maxr = std::max(maxr2, maxr3);
+        if (!n2.empty() || !n3.empty()) {
+            if (maxr < P.maxsep) {
+                int k = maxr < P.minsep ? 0
+                        : int((std::log(maxr) - P.logminsep) / P.binsize) + 1;
+                if (k < mink_zeta) {
+                    calculate_zeta3(c1, ordered, mp2, mp3, k, mink_zeta);
+                    mink_zeta = k;
+                }
+            }
+            if (anysplit1) {
+                Scratch c2 = mp2, c3 = mp3;
+                finish3(c1.left, n2, n3, ordered, mp2, mp3, mink_zeta, maxr2, maxr3);
+                finish3(c1.right, n2, n3, ordered, c2, c3, mink_zeta, maxr2, maxr3);
+            } else {
+                finish3(i1, n2, n3, ordered, mp2, mp3, mink_zeta, maxr2, maxr3);
+            }
+        } else {
+            calculate_zeta3(c1, ordered, mp2, mp3, 0, mink_zeta);
+        }
+    }
+
+    // multipoleSplit1 with two lists, src/Corr3.cpp:1410-1450
+    void split13(int i1, const std::vector<int>& l2, const std::vector<int>& l3, int ordered,
+                 Scratch& mp2, Scratch& mp3) const {
+        const Node& c1 = T1.nodes[i1];
+        const double s1 = c1.size;
+        std::vector<int> n2, n3;
+        split_c2(c1, T, l2, n2);
+        split_c2(c1, *T3, l3, n3);
+        const double maxbsq_eff = P.maxsepsq * std::min(P.bsq, P.asq);
+        if (s1 * s1 > maxbsq_eff) {
+            split13(c1.left, n2, n3, ordered, mp2, mp3);
+            split13(c1.right, n2, n3, ordered, mp2, mp3);
+        } else {
+            mp2.clear();
+            mp3.clear();
+            finish3(i1, n2, n3, ordered, mp2, mp3, P.nbins, 0., 0.);
+        }
+    }
 };
 
-template <int C>
-static void run_multipole(const P3& P, const Tree& t, int nthreads, Out3& total) {
-    std::vector<NodeSums> S;
-    node_sums(t, S);
-    int nt = 1;
-#ifdef _OPENMP
-    nt = nthreads > 0 ? nthreads : omp_get_max_threads();
-#endif
-    std::vector<std::unique_ptr<Out3>> outs;
-    for (int i = 0; i < nt; ++i) outs.emplace_back(new Out3(total.ntot));
-    const long n1 = long(t.top.size());
-#pragma omp parallel for schedule(dynamic) num_threads(nt)
-    for (long i = 0; i < n1; ++i) {
-        int tid = 0;
-#ifdef _OPENMP
-        tid = omp_get_thread_num();
-#endif
-        MP<C> mp(P, t, S, *outs[tid]);
-        Scratch sc(P.nbins, P.maxn, P.D);
-        mp.split1(t.top[i], t.top, sc);
-    }
+static void merge_outs(std::vector<std::unique_ptr<Out3>>& outs, Out3& total) {
     for (auto& o : outs) {
         for (int q = 0; q < 8; ++q) for (int j = 0; j < total.ntot; ++j) total.z[q][j] += o->z[q][j];
         for (int j = 0; j < total.ntot; ++j) {
@@ -465,11 +572,106 @@ static void run_multipole(const P3& P, const Tree& t, int nthreads, Out3& total)
     }
 }
 
+static int thread_count(int nthreads) {
+    int nt = 1;
+#ifdef _OPENMP
+    nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+    (void)nthreads;
+    return nt;
+}
+
+// BaseCorr3::multipole(field1, field2) (src/Corr3.cpp:1124-1175 for one field,
+// :1177-1236 for two): c1 runs over the top cells of t1, the c2 list starts as the top
+// cells of t2.
+template <int C>
+static void run_multipole(const P3& P, const Tree& t1, const Tree& t2, int nthreads, Out3& total) {
+    std::vector<NodeSums> S;
+    node_sums(t2, S);
+    const int nt = thread_count(nthreads);
+    std::vector<std::unique_ptr<Out3>> outs;
+    for (int i = 0; i < nt; ++i) outs.emplace_back(new Out3(total.ntot));
+    const long n1 = long(t1.top.size());
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+    for (long i = 0; i < n1; ++i) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        MP<C> mp(P, t1, t2, S, *outs[tid]);
+        Scratch sc(P.nbins, P.maxn, P.D);
+        mp.split1(t1.top[i], t2.top, sc);
+    }
+    merge_outs(outs, total);
+}
+
+// BaseCorr3::multipole(field1, field2, field3, ordered) (src/Corr3.cpp:1238-1305)
+template <int C>
+static void run_multipole3(const P3& P, const Tree& t1, const Tree& t2, const Tree& t3,
+                           int ordered, int nthreads, Out3& total) {
+    std::vector<NodeSums> S2, S3;
+    node_sums(t2, S2);
+    node_sums(t3, S3);
+    const int nt = thread_count(nthreads);
+    std::vector<std::unique_ptr<Out3>> outs;
+    for (int i = 0; i < nt; ++i) outs.emplace_back(new Out3(total.ntot));
+    const long n1 = long(t1.top.size());
+#pragma omp parallel for schedule(dynamic) num_threads(nt)
+    for (long i = 0; i < n1; ++i) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        MP<C> mp(P, t1, t2, S2, *outs[tid]);
+        mp.T3 = &t3;
+        mp.S3 = &S3;
+        Scratch sc2(P.nbins, P.maxn, P.D), sc3(P.nbins, P.maxn, P.D);
+        mp.split13(t1.top[i], t2.top, t3.top, ordered, sc2, sc3);
+    }
+    merge_outs(outs, total);
+}
+
 }  // namespace tco
 
 using namespace tco;
 
 extern "C" {
+
+static int multipole_common(void* f1, void* f2, void* f3, int ordered, int d, int coords,
+                            int nbins, int maxn, double minsep, double maxsep, double binsize,
+                            double b, double a, int nthreads, double* const* zs,
+                            double* const* ps, long long* counters) {
+    if (d != NData && d != GData) return 1;
+    P3 P;
+    P.nbins = nbins; P.maxn = maxn; P.D = d;
+    P.minsep = minsep; P.maxsep = maxsep; P.binsize = binsize; P.b = b; P.a = a;
+    P.logminsep = std::log(minsep); P.minsepsq = minsep * minsep; P.maxsepsq = maxsep * maxsep;
+    P.bsq = b * b; P.asq = a * a;
+    const int ntot = nbins * nbins * (2 * maxn + 1);
+    Out3 O(ntot);
+    const Tree& t1 = *static_cast<Tree*>(f1);
+    const Tree& t2 = *static_cast<Tree*>(f2);
+    if (f3) {
+        if (ordered != 1 && ordered != 4) return 1;
+        const Tree& t3 = *static_cast<Tree*>(f3);
+        if (coords == Flat) run_multipole3<Flat>(P, t1, t2, t3, ordered, nthreads, O);
+        else if (coords == ThreeD) run_multipole3<ThreeD>(P, t1, t2, t3, ordered, nthreads, O);
+        else run_multipole3<Sphere>(P, t1, t2, t3, ordered, nthreads, O);
+    } else {
+        if (coords == Flat) run_multipole<Flat>(P, t1, t2, nthreads, O);
+        else if (coords == ThreeD) run_multipole<ThreeD>(P, t1, t2, nthreads, O);
+        else run_multipole<Sphere>(P, t1, t2, nthreads, O);
+    }
+    for (int q = 0; q < 8; ++q) if (zs[q]) for (int j = 0; j < ntot; ++j) zs[q][j] += O.z[q][j];
+    std::vector<double>* vs[7] = {&O.meand2, &O.meanlogd2, &O.meand3, &O.meanlogd3, &O.weight,
+                                  &O.weight_im, &O.ntri};
+    for (int q = 0; q < 7; ++q) if (ps[q]) for (int j = 0; j < ntot; ++j) ps[q][j] += (*vs[q])[j];
+    if (counters) {
+        counters[0] += O.gn_calls; counters[1] += O.zeta_calls;
+        counters[2] += O.zeta_rows; counters[3] += O.stages;
+    }
+    return 0;
+}
 
 // Multipole auto-correlation of one oracle tree.  D in {0 (NNN), 4 (GGG)}.
 // Arrays (ntot = nbins*nbins*(2*maxn+1) each, NULL allowed): zeta0..7, meand2,
@@ -484,29 +686,27 @@ int tco_corr3_multipole(void* f, int d, int coords, int nbins, int maxn, double 
                         double* weight_im, double* ntri, double* unused1, double* unused2,
                         long long* counters) {
     (void)quick; (void)exact; (void)unused1; (void)unused2;
-    if (d != NData && d != GData) return 1;
-    P3 P;
-    P.nbins = nbins; P.maxn = maxn; P.D = d;
-    P.minsep = minsep; P.maxsep = maxsep; P.binsize = binsize; P.b = b; P.a = a;
-    P.logminsep = std::log(minsep); P.minsepsq = minsep * minsep; P.maxsepsq = maxsep * maxsep;
-    P.bsq = b * b; P.asq = a * a;
-    const int ntot = nbins * nbins * (2 * maxn + 1);
-    Out3 O(ntot);
-    Tree* t = static_cast<Tree*>(f);
-    if (coords == Flat) run_multipole<Flat>(P, *t, nthreads, O);
-    else if (coords == ThreeD) run_multipole<ThreeD>(P, *t, nthreads, O);
-    else run_multipole<Sphere>(P, *t, nthreads, O);
     double* zs[8] = {z0, z1, z2, z3, z4, z5, z6, z7};
-    for (int q = 0; q < 8; ++q) if (zs[q]) for (int j = 0; j < ntot; ++j) zs[q][j] += O.z[q][j];
     double* ps[7] = {meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri};
-    std::vector<double>* vs[7] = {&O.meand2, &O.meanlogd2, &O.meand3, &O.meanlogd3, &O.weight,
-                                  &O.weight_im, &O.ntri};
-    for (int q = 0; q < 7; ++q) if (ps[q]) for (int j = 0; j < ntot; ++j) ps[q][j] += (*vs[q])[j];
-    if (counters) {
-        counters[0] += O.gn_calls; counters[1] += O.zeta_calls;
-        counters[2] += O.zeta_rows; counters[3] += O.stages;
-    }
-    return 0;
+    return multipole_common(f, f, nullptr, 0, d, coords, nbins, maxn, minsep, maxsep, binsize, b,
+                            a, nthreads, zs, ps, counters);
+}
+
+// One multipole pass over several trees, as BaseCorr3::multipole is called by the
+// cross-correlation drivers (src/Corr3.cpp:3731-3768, 3831-3870, 3938-3985):
+// f3 NULL: c1 from f1, the c2 list from f2 (the one-list form, :1177-1236);
+// otherwise c1 from f1, lists from f2 and f3, ordered 1 or 4 (:1238-1305).
+// The same arrays as tco_corr3_multipole; values are ADDED.
+int tco_corr3_multipole_x(void* f1, void* f2, void* f3, int ordered, int d, int coords,
+                          int nbins, int maxn, double minsep, double maxsep, double binsize,
+                          double b, double a, int nthreads, double* z0, double* z1, double* z2,
+                          double* z3, double* z4, double* z5, double* z6, double* z7,
+                          double* meand2, double* meanlogd2, double* meand3, double* meanlogd3,
+                          double* weight, double* weight_im, double* ntri, long long* counters) {
+    double* zs[8] = {z0, z1, z2, z3, z4, z5, z6, z7};
+    double* ps[7] = {meand2, meanlogd2, meand3, meanlogd3, weight, weight_im, ntri};
+    return multipole_common(f1, f2, f3, ordered, d, coords, nbins, maxn, minsep, maxsep, binsize,
+                            b, a, nthreads, zs, ps, counters);
 }
 
 }  // extern "C"

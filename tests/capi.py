@@ -100,6 +100,30 @@ def corr3(d, f, nbins, min_sep, max_sep, bin_size, b, a, max_n, quick=False):
     return res
 
 
+def corr3_cross(d, kind, ordered, fields, nbins, min_sep, max_sep, bin_size, b, a, max_n,
+                quick=False):
+    """tc_corr3_process_cross12 / _cross21 (fields f1, f2) or _cross (f1, f2, f3)."""
+    ntot = nbins * nbins * (2 * max_n + 1)
+    out = {k: np.zeros(ntot) for k in KEYS3}
+    ptrs = [_lib.dptr(out[k]) if (d == 4 or not k.startswith('zeta')) else None for k in KEYS3]
+    h = ctypes.c_void_p()
+    L = _lib.lib()
+    _lib.check(L.tc_corr3_create(d, 5, float(min_sep), float(max_sep), int(nbins),
+                                 float(bin_size), float(b), float(a), int(max_n), *ptrs,
+                                 ctypes.byref(h)))
+    try:
+        fn = getattr(L, 'tc_corr3_process_' + kind)
+        _lib.check(fn(h, *[f.h for f in fields], int(ordered), 0, int(quick), 1))
+        c = (ctypes.c_longlong * 4)()
+        _lib.check(L.tc_corr3_counters(h, c))
+    finally:
+        L.tc_corr3_destroy(h)
+    shape = (nbins, nbins, 2 * max_n + 1)
+    res = {k: v.reshape(shape) for k, v in out.items()}
+    res['counters'] = dict(gn_calls=c[0], zeta_calls=c[1], zeta_rows=c[2], stages=c[3])
+    return res
+
+
 def multipole_binning(scale, nbins, lo, hi, max_n, bin_slop=None):
     """LogMultipole binning and field sizes (corr3base.py:714-745, 2955-2960)."""
     min_sep, max_sep = scale * lo, scale * hi

@@ -131,6 +131,25 @@ CORR3_CASES = [
 ]
 
 
+# Multipole cross-correlations: the same binning rules as CORR3_CASES, with one catalog
+# per field (seeds seed, seed+1, seed+2) over the same area.
+CORR3_CROSS_CASES = [
+    ('nnn_flat_x', N_, FLAT, 1200, 41, 6, 0.02, 0.3, 6, None, 3, 0.2),
+    ('ggg_flat_x', G_, FLAT, 1200, 42, 6, 0.02, 0.3, 7, None, 3, 0.2),
+    ('ggg_sphere_x', G_, SPHERE, 1200, 43, 5, 0.02, 0.3, 6, None, 3, 0.2),
+    ('ggg_3d_x', G_, THREED, 1000, 44, 5, 0.02, 0.4, 5, 0.5, 3, 0.2),
+]
+
+
+def corr3_cross_case(case, nfields):
+    cats = []
+    for i in range(nfields):
+        c = (case[0], case[1], case[2], case[3], case[4] + i) + tuple(case[5:])
+        cat, bn, extra = corr3_case(c)
+        cats.append(cat)
+    return cats, bn, extra
+
+
 def corr3_case(case):
     (name, d, coords, n, seed, nbins, lo, hi, max_n, bs, min_top, clumps) = case
     cat, scale = make_catalog(coords, n, seed, clumps)

@@ -10,6 +10,12 @@
 // output: 15*ntot f64 sums, 4 u64 counters, int64 nnodes, int64 ntop, then the nodes
 //         (nnodes DNode records of 48 bytes) and the top list (ntop int32).
 // Exit status 0 on success, 3 on an engine error (message on stderr).
+//
+// Cross mode (first int64 of the input negative): int64 header [-1, D, C, nfields, nbins,
+// maxn, mintop, maxtop, npass], the same double header, npass passes of 4 int64 (c1 field,
+// list field, second list field or -1, ordered), then per field int64 n and its six
+// columns.  Every field's tree is built by the device builder; each pass runs
+// corr3_run_trees.  Output: the 15*ntot sums added over the passes, 4 u64 counters.
 #include "tc_mp.h"
 
 #include <cstdio>
@@ -41,10 +47,75 @@ void dev_free(void* p, size_t) {
 }
 }  // namespace tc
 
+static int cross_main(std::ifstream& in, const char* outpath) {
+    using namespace tc;
+    long long h[8];
+    double p[7];
+    in.read((char*)h, sizeof h);
+    in.read((char*)p, sizeof p);
+    const int D = (int)h[0], C = (int)h[1], nf = (int)h[2];
+    std::vector<long long> passes(4 * h[7]);
+    in.read((char*)passes.data(), 8 * passes.size());
+    std::vector<DeviceTree> T(nf);
+    for (int f = 0; f < nf; ++f) {
+        long long n = 0;
+        in.read((char*)&n, 8);
+        std::vector<double> col[6];
+        for (auto& c : col) {
+            c.resize(n);
+            in.read((char*)c.data(), 8 * n);
+        }
+        if (!in) return 2;
+        const double* z = C == Flat ? nullptr : col[2].data();
+        const double* g1 = D == GData ? col[3].data() : nullptr;
+        const double* g2 = D == GData ? col[4].data() : nullptr;
+        if (build_tree(D, C, col[0].data(), col[1].data(), z, g1, g2, col[5].data(), nullptr, n,
+                       p[5], p[6], Mean, 0, (int)h[5], (int)h[6], T[f])) {
+            std::fprintf(stderr, "emu_lib3: tree build: %s\n", g_err.c_str());
+            return 3;
+        }
+    }
+    C3Params P{};
+    P.nbins = (int)h[3]; P.maxn = (int)h[4]; P.D = D; P.quick = 0;
+    P.minsep = p[0]; P.maxsep = p[1]; P.binsize = p[2]; P.b = p[3]; P.a = p[4];
+    P.logminsep = std::log(P.minsep);
+    P.minsepsq = P.minsep * P.minsep; P.maxsepsq = P.maxsep * P.maxsep;
+    P.bsq = P.b * P.b; P.asq = P.a * P.a;
+    P.maxbsq_eff = P.maxsepsq * std::min(P.bsq, P.asq);
+    const long long ntot = (long long)P.nbins * P.nbins * (2 * P.maxn + 1);
+    std::vector<double> out(15 * ntot), acc(15 * ntot, 0.);
+    unsigned long long tot[5] = {0, 0, 0, 0, 0};
+    for (size_t q = 0; q < passes.size(); q += 4) {
+        unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+        const long long a = passes[q], b = passes[q + 1], c = passes[q + 2];
+        if (corr3_run_trees(D, C, T[a], T[b], c >= 0 ? &T[c] : nullptr, (int)passes[q + 3], P,
+                            out.data(), cnt)) {
+            std::fprintf(stderr, "emu_lib3: multipole pass: %s\n", g_err.c_str());
+            return 3;
+        }
+        for (size_t j = 0; j < acc.size(); ++j) acc[j] += out[j];
+        for (int k = 0; k < 5; ++k) tot[k] += cnt[k];
+    }
+    std::ofstream o(outpath, std::ios::binary);
+    o.write((const char*)acc.data(), 8 * acc.size());
+    o.write((const char*)tot, 4 * sizeof(unsigned long long));
+    for (auto& t : T) {
+        dev_free(t.nodes, t.nodes_bytes);
+        dev_free(t.dtop, t.dtop_bytes);
+    }
+    return o ? 0 : 2;
+}
+
 int main(int argc, char** argv) {
     using namespace tc;
     if (argc != 3) return 2;
     std::ifstream in(argv[1], std::ios::binary);
+    {
+        long long mode = 0;
+        in.read((char*)&mode, 8);
+        if (mode < 0) return cross_main(in, argv[2]);
+        in.seekg(0);
+    }
     long long h[8];
     double p[7];
     in.read((char*)h, sizeof h);

@@ -113,7 +113,8 @@ def test_patch_split_is_a_stable_split_by_patch():
         sel = np.where(lab == p._single_patch)[0]
         for nm in ('x', 'y', 'z', 'w', 'k'):
             np.testing.assert_array_equal(getattr(p, nm), getattr(cat, nm)[sel])
-        assert p.npatch == 41 and p.patch is None
+        # Catalog.patch is the single patch number (catalog.py:1200-1205)
+        assert p.npatch == 41 and p.patch == p._single_patch
         assert math.isclose(p.sumw, float(np.sum(w[sel])), rel_tol=1e-12)
     bad = np.array([0, 1, 5], dtype=np.int32)
     col = np.zeros(3)

@@ -208,3 +208,70 @@ def test_emulated_library_multipole_on_device_tree(emu_lib_exe, oracle_lib, case
     ref = O.corr3_multipole(d, coords, f, bn['nbins'], bn['min_sep'], bn['max_sep'],
                             bn['bin_size'], bn['b'], bn['a'], bn['max_n'])
     assert out['counters'] == ref['counters']
+
+
+def run_emu_cross(exe, d, coords, cats, bn, min_top, passes, env):
+    tmp = tempfile.mkdtemp(prefix='tc_emu_x_')
+    try:
+        fin, fout = os.path.join(tmp, 'in.bin'), os.path.join(tmp, 'out.bin')
+        with open(fin, 'wb') as fh:
+            np.array([-1, d, coords, len(cats), bn['nbins'], bn['max_n'], min_top, 10,
+                      len(passes)], np.int64).tofile(fh)
+            np.array([bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'],
+                      bn['min_size'], bn['max_size']], np.float64).tofile(fh)
+            for names, order in passes:
+                third = names[2] - 1 if len(names) == 3 else -1
+                np.array([names[0] - 1, names[1] - 1, third, 0 if order is None else order],
+                         np.int64).tofile(fh)
+            for cat in cats:
+                n = len(cat['x'])
+                np.array([n], np.int64).tofile(fh)
+                for k in ('x', 'y', 'z', 'g1', 'g2', 'w'):
+                    v = cat.get(k)
+                    (np.zeros(n) if v is None else np.ascontiguousarray(v, np.float64)).tofile(fh)
+        r = subprocess.run([exe, fin, fout], env=dict(os.environ, ASAN_OPTIONS='detect_leaks=0',
+                                                      **env), capture_output=True, timeout=900)
+        assert r.returncode == 0, r.stderr.decode()[-3000:]
+        ntot = bn['nbins'] ** 2 * (2 * bn['max_n'] + 1)
+        raw = open(fout, 'rb').read()
+        sums = np.frombuffer(raw, np.float64, count=15 * ntot)
+        cnt = np.frombuffer(raw, np.uint64, count=4, offset=15 * ntot * 8)
+        shape = (bn['nbins'], bn['nbins'], 2 * bn['max_n'] + 1)
+        out = {k: sums[i * ntot:(i + 1) * ntot].reshape(shape) for i, k in enumerate(KEYS)}
+        out['counters'] = dict(zip(('gn_calls', 'zeta_calls', 'zeta_rows', 'stages'),
+                                   (int(c) for c in cnt)))
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+X_RUNS = [(C.CORR3_CROSS_CASES[0], 'cross12', 0), (C.CORR3_CROSS_CASES[2], 'cross', 2),
+          (C.CORR3_CROSS_CASES[3], 'cross21', 0)]
+
+
+@pytest.mark.parametrize('case,kind,ordered', X_RUNS, ids=['%s-%s%d' % (c[0], k, o)
+                                                          for c, k, o in X_RUNS])
+def test_emulated_library_multipole_cross(emu_lib_exe, oracle_lib, case, kind, ordered):
+    """The cross-correlation passes (one list over another field's tree, and two lists
+    with ordered 1 and 4) under the emulation with AddressSanitizer and shuffled thread
+    order, on the device builder's trees: the same sums and counters as the oracle."""
+    from oracle.oracle import MULTIPOLE_PASSES
+    O = oracle_lib
+    d, coords = case[1], case[2]
+    cats, bn, extra = C.corr3_cross_case(case, 3)
+    cats = [{k: (v[:500] if v is not None else None) for k, v in c.items()} for c in cats]
+    nf = 3 if kind == 'cross' else 2
+    passes = MULTIPOLE_PASSES[(kind, ordered)]
+    out = run_emu_cross(emu_lib_exe, d, coords, cats[:nf], bn, extra['min_top'], passes,
+                        {'EMU_WORKERS': '4', 'EMU_SHUFFLE': '29'})
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=extra['min_top'])
+    of = [O.Field(d, coords, c['x'], c['y'], c['z'], g1=c.get('g1'), g2=c.get('g2'), w=c['w'],
+                  **kw) for c in cats[:nf]]
+    ref = O.corr3_multipole_cross(d, coords, kind, ordered, of, bn['nbins'], bn['min_sep'],
+                                  bn['max_sep'], bn['bin_size'], bn['b'], bn['a'], bn['max_n'])
+    assert np.sum(ref['ntri']) > 0
+    np.testing.assert_array_equal(out['ntri'], ref['ntri'])
+    for k in KEYS:
+        if k != 'ntri' and (d == 4 or not k.startswith('zeta')):
+            assert normwise(out[k], ref[k]) < RTOL, k
+    assert out['counters'] == ref['counters']

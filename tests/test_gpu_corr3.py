@@ -129,3 +129,67 @@ def test_gpu_multipole_zeta_on_matrix_cores(gpu, mode, d, coords, n, max_n):
                 continue
             err = np.max(np.abs(zg[:, :, i] - zr[:, :, i])) / den
             assert err < 1e-6, (kr, i - max_n, err)
+
+
+CROSS_KINDS = [('cross12', 0), ('cross12', 1), ('cross21', 0), ('cross21', 3), ('cross', 0),
+               ('cross', 1), ('cross', 2), ('cross', 3), ('cross', 4)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('case', C.CORR3_CROSS_CASES, ids=[c[0] for c in C.CORR3_CROSS_CASES])
+def test_gpu_multipole_cross_matches_oracle(gpu, oracle_lib, case):
+    """Multipole cross-correlations through the C-ABI (processCross12 0/1, processCross21
+    0/3, processCross 0-4) against the oracle's passes on the same three catalogs: ntri
+    exact, the other sums within 1e-12 normwise, the work counters equal."""
+    import capi
+    O = oracle_lib
+    d, coords = case[1], case[2]
+    cats, bn, extra = C.corr3_cross_case(case, 3)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=extra['min_top'])
+    args = (bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'],
+            bn['max_n'])
+    gf = [capi.Field(d, coords, c, **kw) for c in cats]
+    of = [O.Field(d, coords, c['x'], c['y'], c['z'], g1=c['g1'], g2=c['g2'], w=c['w'], **kw)
+          for c in cats]
+    keys = ['weight', 'weight_im', 'meand2', 'meanlogd2', 'meand3', 'meanlogd3']
+    if d == 4:
+        keys += ['zeta%d' % i for i in range(8)]
+    for kind, ordered in CROSS_KINDS:
+        nf = 3 if kind == 'cross' else 2
+        print('multipole cross case', case[0], kind, ordered, flush=True)
+        out = capi.corr3_cross(d, kind, ordered, gf[:nf], *args)
+        ref = O.corr3_multipole_cross(d, coords, kind, ordered, of[:nf], *args)
+        assert np.sum(ref['ntri']) > 0
+        np.testing.assert_array_equal(out['ntri'], ref['ntri'], err_msg=f'{kind} {ordered}')
+        for k in keys:
+            assert normwise(out[k], ref[k]) < RTOL, (kind, ordered, k)
+        for k in ('gn_calls', 'zeta_calls', 'zeta_rows', 'stages'):
+            assert out['counters'][k] == ref['counters'][k], (kind, ordered, k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d,coords,n', [(4, 3, 20000), (0, 1, 25000)])
+def test_gpu_multipole_cross_larger(gpu, oracle_lib, d, coords, n):
+    """Cross-correlations of catalogs deep enough for many finish levels and scratch
+    copies (cross12 ordered 0 and the full cross ordered 0, every pass of each) against
+    the oracle."""
+    import capi
+    O = oracle_lib
+    cats = [C.make_catalog(coords, n, 200 + 7 * i + d + coords, 0.3)[0] for i in range(3)]
+    scale = C.make_catalog(coords, 10, 200, 0.)[1]
+    bn = capi.multipole_binning(scale, 10, 0.01, 0.15, 12)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=4)
+    args = (bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'],
+            bn['max_n'])
+    gf = [capi.Field(d, coords, c, **kw) for c in cats]
+    of = [O.Field(d, coords, c['x'], c['y'], c['z'], g1=c['g1'], g2=c['g2'], w=c['w'], **kw)
+          for c in cats]
+    for kind, ordered, nf in (('cross12', 0, 2), ('cross', 0, 3)):
+        print('multipole cross larger d', d, 'coords', coords, 'n', n, kind, ordered, flush=True)
+        out = capi.corr3_cross(d, kind, ordered, gf[:nf], *args)
+        ref = O.corr3_multipole_cross(d, coords, kind, ordered, of[:nf], *args)
+        np.testing.assert_array_equal(out['ntri'], ref['ntri'])
+        for k in ('weight', 'weight_im', 'meand2', 'meand3') + (('zeta0', 'zeta3', 'zeta6')
+                                                                if d == 4 else ()):
+            assert normwise(out[k], ref[k]) < RTOL, (kind, k)
+        assert out['counters'] == ref['counters']

@@ -67,3 +67,64 @@ def test_oracle_nnn_multipole_vs_triple_loop(oracle_lib):
                 ntri[k[j], k[kk]] += 1
     np.testing.assert_array_equal(out['ntri'][:, :, max_n], ntri)
     assert normwise(out['weight'], weight) < 1e-10
+
+
+CROSS_KINDS = [('cross12', 0), ('cross12', 1), ('cross21', 0), ('cross21', 3), ('cross', 0),
+               ('cross', 1), ('cross', 2), ('cross', 3), ('cross', 4)]
+
+
+@pytest.mark.parametrize('case', C.CORR3_CROSS_CASES, ids=[c[0] for c in C.CORR3_CROSS_CASES])
+def test_oracle_multipole_cross_against_literal_reference(oracle_lib, case):
+    """Multipole cross-correlations, every driver and ordered code (processCross12 0/1,
+    processCross21 0/3, processCross 0-4; src/Corr3.cpp:3731-3985): the oracle's passes
+    over two or three trees against the compiled reference's own drivers on the same
+    inputs.  ntri exact, every other sum within 1e-12 normwise; the three fields differ,
+    so a pass with the fields in the wrong roles or a dropped swap term fails."""
+    from oracle import ref as R
+    if R.module() is None:
+        pytest.skip('oracle/_ref not built')
+    O = oracle_lib
+    d, coords = case[1], case[2]
+    cats, bn, extra = C.corr3_cross_case(case, 3)
+    kw = dict(min_size=bn['min_size'], max_size=bn['max_size'], min_top=extra['min_top'])
+    args = (bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'],
+            bn['max_n'])
+
+    def fld(mod, cat):
+        return mod.Field(d, coords, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
+                         w=cat['w'], **kw) if mod is O else \
+            R.make_field(d, coords, cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
+                         w=cat['w'], **kw)
+    of = [fld(O, c) for c in cats]
+    rf = [fld(R, c) for c in cats]
+    seen = []
+    for kind, ordered in CROSS_KINDS:
+        nf = 3 if kind == 'cross' else 2
+        o = O.corr3_multipole_cross(d, coords, kind, ordered, of[:nf], *args)
+        r = R.corr3_multipole_cross(d, kind, ordered, rf[:nf], *args)
+        assert np.sum(r['ntri']) > 0
+        np.testing.assert_array_equal(o['ntri'], r['ntri'], err_msg=f'{kind} {ordered}')
+        for k, v in r.items():
+            if k in o and k != 'ntri':
+                assert normwise(o[k], v) < 1e-12, (kind, ordered, k)
+        seen.append(r['weight'])
+    # the orderings give different sums
+    assert normwise(seen[0], seen[1]) > 1e-3 and normwise(seen[4], seen[8]) > 1e-3
+
+
+def test_oracle_cross12_of_one_field_is_auto(oracle_lib):
+    """processCross12 with ordered=1 on (f, f) is the auto-correlation's pass: the same
+    sums as the one-field multipole."""
+    O = oracle_lib
+    case = C.CORR3_CROSS_CASES[1]
+    cats, bn, extra = C.corr3_cross_case(case, 1)
+    cat = cats[0]
+    f = O.Field(case[1], case[2], cat['x'], cat['y'], cat['z'], g1=cat['g1'], g2=cat['g2'],
+                w=cat['w'], min_size=bn['min_size'], max_size=bn['max_size'],
+                min_top=extra['min_top'])
+    args = (bn['nbins'], bn['min_sep'], bn['max_sep'], bn['bin_size'], bn['b'], bn['a'],
+            bn['max_n'])
+    a = O.corr3_multipole(case[1], case[2], f, *args, nthreads=1)
+    x = O.corr3_multipole_cross(case[1], case[2], 'cross12', 1, [f, f], *args, nthreads=1)
+    for k in O.Z3_KEYS:
+        np.testing.assert_array_equal(a[k], x[k])

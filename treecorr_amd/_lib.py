@@ -93,6 +93,9 @@ SIGNATURES = {
     'tc_corr3_create': (_i, [_i, _i, _d, _d, _i, _d, _d, _d, _i] + [_dp] * 19 +
                         [ctypes.POINTER(_vp)]),
     'tc_corr3_process_auto': (_i, [_vp, _vp, _i, _i, _i]),
+    'tc_corr3_process_cross12': (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
+    'tc_corr3_process_cross21': (_i, [_vp, _vp, _vp, _i, _i, _i, _i]),
+    'tc_corr3_process_cross': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i]),
     'tc_corr3_counters': (_i, [_vp, ctypes.POINTER(_ll)]),
     'tc_corr3_zeta_cells': (_i, [_vp, ctypes.POINTER(_ll)]),
     'tc_corr3_destroy': (_i, [_vp]),

@@ -944,34 +944,110 @@ int tc_corr3_create(int d, int bin_type, double minsep, double maxsep, int nbins
     return 0;
 }
 
-// BaseCorr3::multipole(field), src/Corr3.cpp:1124-1175 (via ProcessAuto :3654-3720).
-int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int metric) {
-    (void)dots;
-    if (!c || !f) return tc::fail("NULL argument");
+}  // extern "C"
+
+namespace {
+
+// One multipole pass: c1 from fields[a], lists from fields[b] (and fields[c] when c >= 0,
+// with the ordered code `o`).
+struct Pass3 { int a, b, c, o; };
+
+// Runs the passes and adds their sums into the correlation's arrays.
+int corr3_passes(tc_corr3* c, tc_field* const* fs, int nfs, const Pass3* passes, int npass,
+                 int quick, int metric) {
+    if (!c) return tc::fail("NULL argument");
+    for (int i = 0; i < nfs; ++i)
+        if (!fs[i]) return tc::fail("NULL argument");
     if (metric != 1) {
         tc::set_error("only the Euclidean metric is implemented");
         return TC_ERR_UNSUPPORTED;
     }
-    if (f->d != c->d) return tc::fail("field type does not match the correlation");
+    for (int i = 0; i < nfs; ++i) {
+        if (fs[i]->d != c->d) return tc::fail("field type does not match the correlation");
+        if (fs[i]->coords != fs[0]->coords) return tc::fail("the fields' coordinate systems differ");
+    }
     if (int rc = check_gpu()) return rc;
     tc::C3Params P = c->P;
     P.quick = quick ? 1 : 0;
     const int ntot = P.nbins * P.nbins * (2 * P.maxn + 1);
-    std::vector<double> out((size_t)15 * ntot);
-    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+    std::vector<double> out((size_t)15 * ntot), acc((size_t)15 * ntot, 0.);
+    unsigned long long tot[5] = {0, 0, 0, 0, 0};
     std::lock_guard<std::mutex> run_lock(g_run_m);
-    int rc = tc::corr3_run(c->d, f->coords, f->T.nodes, f->T.nnodes, f->T.dtop,
-                           (int)f->T.top.size(), P, out.data(), cnt);
-    if (rc) return rc;
+    for (int p = 0; p < npass; ++p) {
+        const Pass3& ps = passes[p];
+        unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+        int rc = tc::corr3_run_trees(c->d, fs[0]->coords, fs[ps.a]->T, fs[ps.b]->T,
+                                     ps.c >= 0 ? &fs[ps.c]->T : nullptr, ps.o, P, out.data(), cnt);
+        if (rc) return rc;
+        for (size_t j = 0; j < acc.size(); ++j) acc[j] += out[j];
+        for (int k = 0; k < 5; ++k) tot[k] += cnt[k];
+    }
     if (c->d == GData)
         for (int q = 0; q < 8; ++q)
-            for (int j = 0; j < ntot; ++j) c->z[q][j] += out[(size_t)q * ntot + j];
+            for (int j = 0; j < ntot; ++j) c->z[q][j] += acc[(size_t)q * ntot + j];
     double* dst[7] = {c->meand2, c->meanlogd2, c->meand3, c->meanlogd3, c->weight, c->weight_im,
                       c->ntri};
     for (int q = 0; q < 7; ++q)
-        for (int j = 0; j < ntot; ++j) dst[q][j] += out[(size_t)(8 + q) * ntot + j];
-    for (int k = 0; k < 5; ++k) c->counters[k] = (long long)cnt[k];
+        for (int j = 0; j < ntot; ++j) dst[q][j] += acc[(size_t)(8 + q) * ntot + j];
+    for (int k = 0; k < 5; ++k) c->counters[k] = (long long)tot[k];
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// BaseCorr3::multipole(field), src/Corr3.cpp:1124-1175 (via ProcessAuto :3654-3720).
+int tc_corr3_process_auto(tc_corr3* c, tc_field* f, int dots, int quick, int metric) {
+    (void)dots;
+    tc_field* fs[1] = {f};
+    const Pass3 p{0, 0, -1, 0};
+    return corr3_passes(c, fs, 1, &p, 1, quick, metric);
+}
+
+// ProcessCross12c, src/Corr3.cpp:3731-3768: ordered 1 is multipole(f1, f2); ordered 0
+// first adds multipole(f2, f1, f2, ordered 1).
+int tc_corr3_process_cross12(tc_corr3* c, tc_field* f1, tc_field* f2, int ordered, int dots,
+                             int quick, int metric) {
+    (void)dots;
+    if (ordered != 0 && ordered != 1) return tc::fail("processCross12: ordered must be 0 or 1");
+    tc_field* fs[2] = {f1, f2};
+    const Pass3 p0[2] = {{1, 0, 1, 1}, {0, 1, -1, 0}};
+    return ordered == 0 ? corr3_passes(c, fs, 2, p0, 2, quick, metric)
+                        : corr3_passes(c, fs, 2, p0 + 1, 1, quick, metric);
+}
+
+// ProcessCross21c, src/Corr3.cpp:3831-3870: ordered 3 is multipole(f1, f1, f2, 4);
+// ordered 0 the three placements of f2 with ordered 1.
+int tc_corr3_process_cross21(tc_corr3* c, tc_field* f1, tc_field* f2, int ordered, int dots,
+                             int quick, int metric) {
+    (void)dots;
+    if (ordered != 0 && ordered != 3) return tc::fail("processCross21: ordered must be 0 or 3");
+    tc_field* fs[2] = {f1, f2};
+    const Pass3 p0[3] = {{1, 0, 0, 1}, {0, 1, 0, 1}, {0, 0, 1, 1}};
+    const Pass3 p3[1] = {{0, 0, 1, 4}};
+    return ordered == 0 ? corr3_passes(c, fs, 2, p0, 3, quick, metric)
+                        : corr3_passes(c, fs, 2, p3, 1, quick, metric);
+}
+
+// ProcessCrossc, src/Corr3.cpp:3938-3985: the passes of each ordered code 0-4.
+int tc_corr3_process_cross(tc_corr3* c, tc_field* f1, tc_field* f2, tc_field* f3, int ordered,
+                           int dots, int quick, int metric) {
+    (void)dots;
+    tc_field* fs[3] = {f1, f2, f3};
+    static const Pass3 o0[3] = {{1, 0, 2, 1}, {2, 0, 1, 1}, {0, 1, 2, 1}};
+    static const Pass3 o1[1] = {{0, 1, 2, 1}};
+    static const Pass3 o2[2] = {{0, 1, 2, 4}, {2, 1, 0, 4}};
+    static const Pass3 o3[2] = {{0, 1, 2, 4}, {1, 0, 2, 4}};
+    static const Pass3 o4[1] = {{0, 1, 2, 4}};
+    switch (ordered) {
+        case 0: return corr3_passes(c, fs, 3, o0, 3, quick, metric);
+        case 1: return corr3_passes(c, fs, 3, o1, 1, quick, metric);
+        case 2: return corr3_passes(c, fs, 3, o2, 2, quick, metric);
+        case 3: return corr3_passes(c, fs, 3, o3, 2, quick, metric);
+        case 4: return corr3_passes(c, fs, 3, o4, 1, quick, metric);
+        default: return tc::fail("processCross: ordered must be 0 to 4");
+    }
 }
 
 int tc_corr3_counters(const tc_corr3* c, long long* counters4) {

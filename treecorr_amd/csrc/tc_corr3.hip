@@ -244,10 +244,13 @@ __device__ __forceinline__ void add_gn(const ScratchLayout& Lo, double* sc, cons
 struct SplitTask { int c1, off, len, pad; };
 // multipoleFinish(c1, list, mp, mink_zeta, maxr): mp lives in slot `slot`; `spare` is a
 // free slot for the right child of a split; fresh: the slot's scratch starts at zero.
+// With two lists, maxr is the first list's largest remaining separation and maxr3 the
+// second's (multipoleFinish's maxr2, maxr3, src/Corr3.cpp:1615-1679).
 struct MPTask {
     int c1, off, len, mink;
     double maxr;
     int slot, spare, fresh, pad;
+    double maxr3;
 };
 __device__ __forceinline__ bool has_children(const DNode& c, int nnodes) {
     return c.child >= 0 && c.child < nnodes - 1;
@@ -354,12 +357,13 @@ __device__ __forceinline__ int finish_class(const C3Params& P, const DNode& c1, 
 // ---------------------------------------------------------------------------
 // Phase 1: one level of multipoleSplit1
 // ---------------------------------------------------------------------------
-__global__ void k_split_init(const int* __restrict__ top, int ntop, SplitTask* __restrict__ tasks,
-                             int* __restrict__ pool) {
+// c1 runs over c1top, every task's list is ltop (BaseCorr3::multipole, src/Corr3.cpp:
+// 1124-1305: the list of all top cells of the other field, or fields)
+__global__ void k_split_init(const int* __restrict__ c1top, int nc1, const int* __restrict__ ltop,
+                             int nl, SplitTask* __restrict__ tasks, int* __restrict__ pool) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntop) return;
-    pool[i] = top[i];
-    tasks[i] = SplitTask{top[i], 0, ntop, 0};
+    if (i < nl) pool[i] = ltop[i];
+    if (i < nc1) tasks[i] = SplitTask{c1top[i], 0, nl, 0};
 }
 
 template <int C>
@@ -534,7 +538,7 @@ __global__ void k_units_to_tasks(const SplitTask* __restrict__ units, int n, int
     if (i >= n) return;
     const SplitTask u = units[i];
     MPTask t;
-    t.c1 = u.c1; t.off = u.off; t.len = u.len; t.mink = nbins; t.maxr = 0.;
+    t.c1 = u.c1; t.off = u.off; t.len = u.len; t.mink = nbins; t.maxr = 0.; t.maxr3 = 0.;
     t.slot = -1; t.spare = -1; t.fresh = 1; t.pad = 0;
     tasks[i] = t;
 }
@@ -551,7 +555,11 @@ __global__ void k_mp_prep(MPTask* __restrict__ tasks, int n, const int* __restri
     tasks[i] = t;
 }
 
-template <int D, int C>
+// X: the two-list form (src/Corr3.cpp:1615-1679).  A slot then holds two scratches, the
+// first list's at offset 0 and the second's at Lo.total; a list entry with node index
+// >= P.base3 is the second list's and adds into the second scratch, and each list keeps
+// its own largest remaining separation (maxr, maxr3) for its split threshold.
+template <int D, int C, bool X>
 __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
     const DNode* __restrict__ nodes, int nnodes, const NodeSum* __restrict__ nsum,
     const MPTask* __restrict__ tasks, const int* __restrict__ pool_in,
@@ -567,10 +575,11 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
     __shared__ int s_bad;
     ScratchLayout Lo;
     Lo.init(P.nbins, P.maxn, D);
+    const int stot = X ? 2 * Lo.total : Lo.total;      // doubles per slot
     const int t = blockIdx.x;
     const MPTask tk = tasks[t];
     MPTask none;
-    none.c1 = -1; none.off = 0; none.len = 0; none.mink = 0; none.maxr = 0.;
+    none.c1 = -1; none.off = 0; none.len = 0; none.mink = 0; none.maxr = 0.; none.maxr3 = 0.;
     none.slot = -1; none.spare = -1; none.fresh = 0; none.pad = 0;
     const MPJob nojob{-1, -1, 0, 0};
     const bool ok = (unsigned)tk.c1 < (unsigned)nnodes && tk.off >= 0 && tk.len >= 0 &&
@@ -585,22 +594,24 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
         }
         return;
     }
-    double* gs = slot_at(slots, tk.slot, Lo.total);
+    double* gs = slot_at(slots, tk.slot, stot);
     if (tk.fresh) {
-        for (int i = threadIdx.x; i < Lo.total; i += blockDim.x) sc[i] = 0.;
+        for (int i = threadIdx.x; i < stot; i += blockDim.x) sc[i] = 0.;
     } else {
-        for (int i = threadIdx.x; i < Lo.total; i += blockDim.x) sc[i] = gs[i];
+        for (int i = threadIdx.x; i < stot; i += blockDim.x) sc[i] = gs[i];
     }
     if (threadIdx.x == 0) s_bad = 0;
     __syncthreads();
     const DNode c1 = nodes[tk.c1];
     const double s1 = c1.size;
-    const double thresh1 = (1. - P.binsize) * tk.maxr;   // BinTypeHelper<Log>::oneBinLessThan
+    // BinTypeHelper<Log>::oneBinLessThan of each list's previous largest separation
+    const double thresh1 = (1. - P.binsize) * tk.maxr;
+    const double thresh13 = (1. - P.binsize) * tk.maxr3;
     const int* list = pool_in + tk.off;
     // pass 1: accepted pairs into the scratch, the size of the next list, the largest
     // remaining separation and whether any c2 needs c1 split
     int nlen = 0, lany = 0;
-    double lmax = 0.;
+    double lmax = 0., lmax3 = 0.;
     unsigned long long c_gn = 0;
     for (int base = 0; base < tk.len; base += blockDim.x) {
         const int i = base + threadIdx.x;
@@ -610,6 +621,7 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
             if ((unsigned)i2 >= (unsigned)nnodes) {
                 s_bad = 1;
             } else {
+                const bool second = X && i2 >= P.base3;
                 const DNode c2 = nodes[i2];
                 int k = -1;
                 double r = 0., logr = 0., rtot = 0.;
@@ -618,13 +630,15 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
                 if (cl == 1) {
                     if (k >= 0 && k < P.nbins) {
                         ++c_gn;
-                        add_gn<D, C>(Lo, sc, c1, c2, nsum[i2], r, logr, k, P.maxn, P.plain);
+                        add_gn<D, C>(Lo, second ? sc + Lo.total : sc, c1, c2, nsum[i2], r, logr,
+                                     k, P.maxn, P.plain);
                     } else {
                         s_bad = 1;     // r at the very edge of the range: no bin
                     }
                 } else if (cl == 2) {
-                    lmax = fmax(lmax, rtot);
-                    if (sp1 && rtot > thresh1) lany = 1;
+                    if (second) lmax3 = fmax(lmax3, rtot);
+                    else lmax = fmax(lmax, rtot);
+                    if (sp1 && rtot > (second ? thresh13 : thresh1)) lany = 1;
                     if (sp2 && !has_children(c2, nnodes)) s_bad = 1;
                     no = sp2 ? 2 : 1;
                 }
@@ -634,7 +648,9 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
         block_excl_scan(no, sh, tot);
         nlen += tot;
     }
-    const double nmaxr = block_max(lmax, shd);
+    const double nmaxr2 = block_max(lmax, shd);
+    const double nmaxr3 = X ? block_max(lmax3, shd) : 0.;
+    const double nmaxr = fmax(nmaxr2, nmaxr3);
     int nany;
     block_excl_scan(lany, sh, nany);
     const unsigned long long gn_tot = block_sum_u64(c_gn, shl);
@@ -676,7 +692,8 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
             wpos += tot;
         }
     }
-    // multipoleFinish's decisions after splitC2CellsOrCalculateGn (src/Corr3.cpp:1574-1611)
+    // multipoleFinish's decisions after splitC2CellsOrCalculateGn (src/Corr3.cpp:1574-1611,
+    // 1637-1677)
     MPJob job = nojob;
     int nmink = tk.mink;
     if (nlen > 0) {
@@ -691,8 +708,8 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
     }
     // the scratch after this stage: this level's zeta reads it from the slot; the right
     // child of a split continues from a copy (mp.duplicate())
-    double* gsp = split ? slot_at(slots, tk.spare, Lo.total) : nullptr;
-    for (int i = threadIdx.x; i < Lo.total; i += blockDim.x) {
+    double* gsp = split ? slot_at(slots, tk.spare, stot) : nullptr;
+    for (int i = threadIdx.x; i < stot; i += blockDim.x) {
         const double v = sc[i];
         gs[i] = v;
         if (split) gsp[i] = v;
@@ -702,7 +719,8 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
         MPTask a = none, b = none;
         int r0 = -1, r1 = -1;
         if (nlen > 0) {
-            a.off = (int)obase; a.len = nlen; a.mink = nmink; a.maxr = nmaxr;
+            a.off = (int)obase; a.len = nlen; a.mink = nmink;
+            a.maxr = X ? nmaxr2 : nmaxr; a.maxr3 = nmaxr3;
             if (split) {
                 b = a;
                 a.c1 = c1.child; a.slot = tk.slot;
@@ -720,12 +738,19 @@ __global__ __launch_bounds__(MP_THREADS) void k_mp_stage(
         atomicAdd(&counters[C3_GN], gn_tot);
         atomicAdd(&counters[C3_STAGES], 1ull);
         if (job.c1 >= 0) {
-            atomicAdd(&counters[C3_ZETA], 1ull);
-            atomicAdd(&counters[C3_ROWS], (unsigned long long)(job.mk - job.ks));
-            // cells (k2, k3 >= k2) of the rows [ks, mk)
             const int nb = P.nbins;
-            const long long a = nb - job.ks, b = nb - job.mk;
-            atomicAdd(&counters[C3_CELLS], (unsigned long long)((a * (a + 1) - b * (b + 1)) / 2));
+            atomicAdd(&counters[C3_ZETA], 1ull);
+            if (X) {
+                // rows [ks, nb); cells: {k2, k3 >= ks} without {k2, k3 >= mk}
+                const long long a2 = nb - job.ks, b2 = nb - job.mk;
+                atomicAdd(&counters[C3_ROWS], (unsigned long long)a2);
+                atomicAdd(&counters[C3_CELLS], (unsigned long long)(a2 * a2 - b2 * b2));
+            } else {
+                atomicAdd(&counters[C3_ROWS], (unsigned long long)(job.mk - job.ks));
+                // cells (k2, k3 >= k2) of the rows [ks, mk)
+                const long long a2 = nb - job.ks, b2 = nb - job.mk;
+                atomicAdd(&counters[C3_CELLS], (unsigned long long)((a2 * (a2 + 1) - b2 * (b2 + 1)) / 2));
+            }
         }
     }
 }
@@ -966,17 +991,200 @@ __global__ __launch_bounds__(ZT_THREADS) void k_mp_zeta(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Phase 3, two lists: zeta of the jobs of one level (calculateZeta with mp2, mp3,
+// src/Corr3.cpp:1976-2085; MultipoleHelper<5>, :2851-3003)
+// ---------------------------------------------------------------------------
+// There is no symmetry between the lists, so the outputs are kept per (k2, k3, m >= 0)
+// for every k3, NV doubles: the weight at n = +m (n = -m is its conjugate), for GGG the
+// four gam at n = +m [2,9] and at n = -m [10,17], then ntri, meand2, meanlogd2, meand3,
+// meanlogd3 at n = 0 (m = 0 only).  s2, s3: the first and second lists' scratches, which
+// hold the sums of the cells across from vertex 2 and vertex 3 respectively.
+template <int D>
+__device__ __forceinline__ void zeta_cell_x(const ScratchLayout& Lo, const double* __restrict__ s2,
+                                            const double* __restrict__ s3, const DNode& c1, int k2,
+                                            int k3, int m, const C3Params& P,
+                                            double* __restrict__ v) {
+    constexpr int NV = zeta_nv(D);
+    const double w1 = c1.w, n1 = double(c1.n);
+    const bool swap23 = P.order3 == 1;
+    double* two = v + NV - 5;
+    if (m == 0 && !P.quick) {
+        auto S2 = [&](int q, int k) { return s2[Lo.S(q, k)]; };
+        auto S3 = [&](int q, int k) { return s3[Lo.S(q, k)]; };
+        if (!swap23) {
+            two[0] += n1 * S3(S_NPAIRS, k2) * S2(S_NPAIRS, k3);
+            const double ww12 = w1 * S3(S_SUMW, k2), ww13 = w1 * S2(S_SUMW, k3);
+            two[1] += ww13 * S3(S_SUMWR, k2);
+            two[2] += ww13 * S3(S_SUMWLOGR, k2);
+            two[3] += ww12 * S2(S_SUMWR, k3);
+            two[4] += ww12 * S2(S_SUMWLOGR, k3);
+        } else {
+            two[0] += n1 * (S2(S_NPAIRS, k2) * S3(S_NPAIRS, k3) + S3(S_NPAIRS, k2) * S2(S_NPAIRS, k3));
+            two[1] += w1 * (S2(S_SUMW, k3) * S3(S_SUMWR, k2) + S3(S_SUMW, k3) * S2(S_SUMWR, k2));
+            two[2] += w1 * (S2(S_SUMW, k3) * S3(S_SUMWLOGR, k2) + S3(S_SUMW, k3) * S2(S_SUMWLOGR, k2));
+            two[3] += w1 * (S2(S_SUMW, k2) * S3(S_SUMWR, k3) + S3(S_SUMW, k2) * S2(S_SUMWR, k3));
+            // the second term pairs sumw and sumwlogr of bin k3, as the reference does
+            two[4] += w1 * (S2(S_SUMW, k2) * S3(S_SUMWLOGR, k3) + S3(S_SUMW, k3) * S2(S_SUMWLOGR, k3));
+        }
+    }
+    // weight: w1 W3(k2, m) conj(W2(k3, m)) (+ w1 W2(k2, m) conj(W3(k3, m)))
+    {
+        double xr, xi;
+        cmul3(w1 * s3[Lo.W(k2, m)], w1 * s3[Lo.W(k2, m) + 1], s2[Lo.W(k3, m)], -s2[Lo.W(k3, m) + 1],
+              xr, xi);
+        v[0] += xr;
+        v[1] += xi;
+        if (swap23) {
+            cmul3(w1 * s2[Lo.W(k2, m)], w1 * s2[Lo.W(k2, m) + 1], s3[Lo.W(k3, m)],
+                  -s3[Lo.W(k3, m) + 1], xr, xi);
+            v[0] += xr;
+            v[1] += xi;
+        }
+    }
+    if (D != GData) return;
+    const double g1r = c1.a, g1i = c1.b;
+    // gam_mu(n) from A's G at bin k2 and B's G at bin k3 (the general form of :2883-2897)
+    auto gam = [&](const double* A, const double* B, int n, double* g) {
+        double ar, ai, tr, ti;
+        const int a_nm = Lo.G(k2, n - 1), a_np = Lo.G(k2, n + 1), a_mnp = Lo.G(k2, -n + 1);
+        const int b_mnm = Lo.G(k3, -n - 1), b_mnp = Lo.G(k3, -n + 1), b_np = Lo.G(k3, n + 1);
+        cmul3(g1r, g1i, A[a_nm], A[a_nm + 1], ar, ai);
+        cmul3(ar, ai, B[b_mnm], B[b_mnm + 1], tr, ti); g[0] += tr; g[1] += ti;
+        cmul3(ar, ai, B[b_np], -B[b_np + 1], tr, ti); g[4] += tr; g[5] += ti;
+        cmul3(g1r, -g1i, A[a_np], A[a_np + 1], ar, ai);
+        cmul3(ar, ai, B[b_mnp], B[b_mnp + 1], tr, ti); g[2] += tr; g[3] += ti;
+        cmul3(g1r, g1i, A[a_mnp], -A[a_mnp + 1], ar, ai);
+        cmul3(ar, ai, B[b_mnm], B[b_mnm + 1], tr, ti); g[6] += tr; g[7] += ti;
+    };
+    gam(s3, s2, m, v + 2);
+    if (swap23) gam(s2, s3, m, v + 2);
+    if (m > 0) {
+        gam(s3, s2, -m, v + 10);
+        if (swap23) gam(s2, s3, -m, v + 10);
+    }
+}
+
+// One workgroup per (job chunk, range of mr multipoles, group of kg rows k2).  The
+// chunk's rows are k2 >= kmin, the smallest ks of its jobs; each thread owns up to
+// ZT_IPT cells (k2, k3, m) of its group's rows (every k3), sums in registers.  For each
+// job, the entries of both scratches that these multipoles read are copied into LDS
+// (the first list's at the slot's offsets, the second's after them); a cell takes the
+// job's terms when k2, k3 >= ks and not both >= mk.  Every cell of the group's rows
+// k2 >= kmin is written to the chunk's partial array (zero where no job contributes).
+template <int D>
+__global__ __launch_bounds__(ZT_THREADS) void k_mp_zeta_x(
+    const MPJob* __restrict__ jobs, int njobs, const DNode* __restrict__ nodes,
+    double* const* __restrict__ slots, C3Params P, int mr, int kg, int jpc,
+    double* __restrict__ partial, RowRange* __restrict__ chunk_rows, int nq, int nsub) {
+    HIP_DYNAMIC_SHARED(double, sc)
+    __shared__ int s_kmin[ZT_THREADS / 64];
+    constexpr int NV = zeta_nv(D);
+    int q, sub;
+    if (!xcd_chunk(nq, nsub, q, sub)) return;
+    const int nb = P.nbins, M = P.maxn;
+    const int nmr = (M + 1 + mr - 1) / mr;
+    const int m0 = (sub % nmr) * mr;
+    const int g0 = (sub / nmr) * kg;
+    ScratchLayout Lo;
+    Lo.init(nb, M, D);
+    const int j0 = q * jpc;
+    const int j1 = njobs < j0 + jpc ? njobs : j0 + jpc;
+    int kmin = nb;
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
+        const MPJob jb = jobs[j];
+        if (jb.c1 >= 0 && jb.ks < jb.mk) kmin = jb.ks < kmin ? jb.ks : kmin;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_xor(kmin, o, 64);
+        kmin = a < kmin ? a : kmin;
+    }
+    if ((threadIdx.x & 63) == 0) s_kmin[threadIdx.x >> 6] = kmin;
+    __syncthreads();
+    kmin = nb;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) kmin = s_kmin[w] < kmin ? s_kmin[w] : kmin;
+    if (threadIdx.x == 0 && sub == 0) chunk_rows[q] = RowRange{kmin, nb};
+    if (kmin >= nb) return;              // no jobs in this chunk (uniform)
+    int ck2[ZT_IPT], ck3[ZT_IPT], cm[ZT_IPT];
+#pragma unroll
+    for (int u = 0; u < ZT_IPT; ++u) {
+        const int c = (int)threadIdx.x + u * ZT_THREADS;
+        const int ml = c % mr;
+        const int pr = c / mr;
+        const int k2 = g0 + pr / nb;
+        ck2[u] = k2;
+        ck3[u] = pr % nb;
+        cm[u] = (k2 < nb && k2 < g0 + kg && k2 >= kmin && m0 + ml <= M) ? m0 + ml : -1;
+    }
+    double a0[NV], a1[NV];
+    static_assert(ZT_IPT == 2, "two cells per thread");
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { a0[v] = 0.; a1[v] = 0.; }
+    const int mhi = (m0 + mr - 1 < M ? m0 + mr - 1 : M);
+    const int nw = 2 * (mhi - m0 + 1);
+    const int gl = m0 - 1 < 0 ? 0 : m0 - 1;
+    const int ng = D == GData ? 2 * (mhi + 1 - gl + 1) : 0;
+    const int per_k = nw + 2 * ng;
+    const int nsum = Lo.total - Lo.sums;
+    const int ncopy = nb * per_k + nsum;
+    for (int j = j0; j < j1; ++j) {
+        const MPJob jb = jobs[j];
+        if (jb.c1 < 0 || jb.ks >= jb.mk) continue;
+        const double* gsl = slot_at(slots, jb.slot, 2 * Lo.total);
+        __syncthreads();                 // the previous job's entries are no longer read
+        for (int e = threadIdx.x; e < 2 * ncopy; e += blockDim.x) {
+            const int half = e >= ncopy ? 1 : 0;
+            const int ee = e - half * ncopy;
+            int off;
+            if (ee >= nb * per_k) {
+                off = Lo.sums + (ee - nb * per_k);
+            } else {
+                const int k = ee / per_k;
+                const int r = ee - k * per_k;
+                if (r < nw) off = Lo.W(k, m0) + r;
+                else if (r < nw + ng) off = Lo.G(k, gl) + (r - nw);
+                else off = Lo.G(k, -(mhi + 1)) + (r - nw - ng);
+            }
+            off += half * Lo.total;
+            sc[off] = gsl[off];
+        }
+        __syncthreads();
+        const DNode c1 = nodes[jb.c1];
+        auto takes = [&](int k2, int k3) {
+            return k2 >= jb.ks && k3 >= jb.ks && (k2 < jb.mk || k3 < jb.mk);
+        };
+        if (cm[0] >= 0 && takes(ck2[0], ck3[0]))
+            zeta_cell_x<D>(Lo, sc, sc + Lo.total, c1, ck2[0], ck3[0], cm[0], P, a0);
+        if (cm[1] >= 0 && takes(ck2[1], ck3[1]))
+            zeta_cell_x<D>(Lo, sc, sc + Lo.total, c1, ck2[1], ck3[1], cm[1], P, a1);
+    }
+    const size_t ncomp_chunk = (size_t)nb * nb * (M + 1) * NV;
+    if (cm[0] >= 0) {
+        double* dst = partial + (size_t)q * ncomp_chunk +
+                      (((size_t)ck2[0] * nb + ck3[0]) * (M + 1) + cm[0]) * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) dst[v] = a0[v];
+    }
+    if (cm[1] >= 0) {
+        double* dst = partial + (size_t)q * ncomp_chunk +
+                      (((size_t)ck2[1] * nb + ck3[1]) * (M + 1) + cm[1]) * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) dst[v] = a1[v];
+    }
+}
+
 // total += sum over the chunks (in chunk order) of the partial sums of the chunks whose
 // rows include this entry's row k2, compensated
+// (full: the two-list layout, every k3)
 __global__ void k_mp_zeta_reduce(const double* __restrict__ partial,
                                  const RowRange* __restrict__ chunk_rows, int nchunks, long long n,
                                  long long row_stride, int nbins, double* __restrict__ hi,
-                                 double* __restrict__ lo) {
+                                 double* __restrict__ lo, int full) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const int k2 = (int)(e / row_stride);
     const int k3 = (int)((e % row_stride) / (row_stride / nbins));
-    if (k3 < k2) return;                 // not a cell of the compressed layout
+    if (!full && k3 < k2) return;        // not a cell of the compressed layout
     double s = hi[e], c = lo[e];
     for (int q = 0; q < nchunks; ++q) {
         const RowRange rows = chunk_rows[q];
@@ -1051,6 +1259,30 @@ __global__ void k_mp_expand(const double* __restrict__ hi, const double* __restr
     for (int i = 0; i < 8; ++i) out[(O_Z0 + i) * ntot + idx] = z[i];
 }
 
+// The two-list sums -> TreeCorr's arrays (index (k2*nb + k3)*(2M+1) + M + n)
+template <int D>
+__global__ void k_mp_expand_x(const double* __restrict__ hi, const double* __restrict__ lo,
+                              C3Params P, double* __restrict__ out) {
+    constexpr int NV = zeta_nv(D);
+    const int nb = P.nbins, M = P.maxn, nn = 2 * M + 1;
+    const long long ntot = (long long)nb * nb * nn;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntot) return;
+    const int k2 = (int)(idx / ((long long)nb * nn));
+    const int k3 = (int)((idx / nn) % nb);
+    const int n = (int)(idx % nn) - M;
+    const int m = n >= 0 ? n : -n;
+    const long long cb = (((long long)k2 * nb + k3) * (M + 1) + m) * NV;
+    auto V = [&](int i) { return hi[cb + i] + lo[cb + i]; };
+    out[O_WEIGHT * ntot + idx] = V(0);
+    out[O_WEIGHT_IM * ntot + idx] = n >= 0 ? V(1) : -V(1);
+    const int oi[5] = {O_NTRI, O_MEAND2, O_MEANLOGD2, O_MEAND3, O_MEANLOGD3};
+    for (int i = 0; i < 5; ++i) out[oi[i] * ntot + idx] = n == 0 ? V(NV - 5 + i) : 0.;
+    if (D != GData) return;
+    const int base = n >= 0 ? 2 : 10;
+    for (int i = 0; i < 8; ++i) out[(O_Z0 + i) * ntot + idx] = V(base + i);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -1113,12 +1345,18 @@ static SplitFn pick_split(int C) {
     if (C == ThreeD) return &k_mp_split_level<ThreeD>;
     return &k_mp_split_level<Sphere>;
 }
-static StageFn pick_stage(int D, int C) {
-#define TC_SF(DD, CC) return &k_mp_stage<DD, CC>
+template <bool X>
+static StageFn pick_stage_t(int D, int C) {
+#define TC_SF(DD, CC) return &k_mp_stage<DD, CC, X>
     if (D == NData) { if (C == Flat) TC_SF(NData, Flat); if (C == ThreeD) TC_SF(NData, ThreeD); TC_SF(NData, Sphere); }
     if (C == Flat) TC_SF(GData, Flat); if (C == ThreeD) TC_SF(GData, ThreeD); TC_SF(GData, Sphere);
 #undef TC_SF
 }
+static StageFn pick_stage(int D, int C, bool X) {
+    return X ? pick_stage_t<true>(D, C) : pick_stage_t<false>(D, C);
+}
+using ZetaXFn = void (*)(const MPJob*, int, const DNode*, double* const*, C3Params, int, int, int,
+                         double*, RowRange*, int, int);
 
 static int set_err_bits(const char* phase, int bits) {
     std::string m = std::string(phase) + " stopped:";
@@ -1180,17 +1418,29 @@ int set_multipole_plain(int on) {
 // rows, stages, zeta cells).
 int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int ntop,
               const C3Params& P_in, double* out, unsigned long long* cnt) {
+    C3Params P = P_in;
+    P.order3 = 0;
+    return corr3_run_x(D, C, nodes, nnodes, dtop, ntop, dtop, ntop, P, out, cnt);
+}
+
+int corr3_run_x(int D, int C, const DNode* nodes, int nnodes, const int* c1top, int nc1,
+                const int* ltop, int nl, const C3Params& P_in, double* out,
+                unsigned long long* cnt) {
     hipStream_t st = stream();
     C3Params P = P_in;
     P.plain = g_mp_plain.load();
+    const bool X = P.order3 != 0;
+    if (X && P.order3 != 1 && P.order3 != 4) return fail("two-list multipole: ordered must be 1 or 4");
     const int nb = P.nbins, M = P.maxn;
     const long long ntot = (long long)nb * nb * (2 * M + 1);
     ScratchLayout Lo;
     Lo.init(nb, M, D);
-    const size_t shbytes = sizeof(double) * Lo.total;
+    const int stot = X ? 2 * Lo.total : Lo.total;        // doubles per scratch slot
+    const size_t shbytes = sizeof(double) * stot;
     if (shbytes > 150 * 1024) return fail("nbins and max_n too large for the LDS scratch");
-    if (ntop <= 0 || nnodes <= 0) return fail("empty field");
-    StageFn sfn = pick_stage(D, C);
+    if (nc1 <= 0 || nl <= 0 || nnodes <= 0) return fail("empty field");
+    const int ntop = nc1;
+    StageFn sfn = pick_stage(D, C, X);
     TC_HIP(hipFuncSetAttribute((const void*)sfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)shbytes));
     const int NV = zeta_nv(D);
@@ -1203,8 +1453,17 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
     const size_t zt_bytes = shbytes;    // the slot's layout, partly filled per job
     TC_HIP(hipFuncSetAttribute((const void*)zfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)zt_bytes));
-    // the off-diagonal zeta cells on the matrix cores (at most 32 bins)
-    const int zmode = nb <= 32 ? g_zeta_mode.load() : 0;
+    // two lists: cells (k2, k3, m) of every k3, groups of kg rows and xmr multipoles
+    ZetaXFn zxfn = D == GData ? &k_mp_zeta_x<GData> : &k_mp_zeta_x<NData>;
+    const int xcells = ZT_IPT * ZT_THREADS;
+    const int xmr = std::max(1, std::min(ZT_MR, xcells / (nb * nb)));
+    const int xkg = std::min(nb, xcells / (nb * xmr));
+    if (X && xkg < 1) return fail("nbins too large for the two-list zeta kernel (at most 1024 bins)");
+    const int xsub = X ? ((M + 1 + xmr - 1) / xmr) * ((nb + xkg - 1) / xkg) : 0;
+    if (X) TC_HIP(hipFuncSetAttribute((const void*)zxfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)shbytes));
+    // the off-diagonal zeta cells on the matrix cores (at most 32 bins; one list only)
+    const int zmode = (nb <= 32 && !X) ? g_zeta_mode.load() : 0;
     int mf_threads = 0;
     size_t mf_bytes = 0;
     ZetaMfmaFn mfn = pick_zeta_mfma(D, zmode, nb, mf_threads, mf_bytes);
@@ -1245,13 +1504,13 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
     // ---- phase 1: multipoleSplit1, one launch per level ----
     DBuf<int> spool;
     DBuf<SplitTask> stA, stB, schild, sunit_out, units;
-    if (spool.alloc((size_t)ntop + 2ull * ntop * ntop) || stA.alloc(ntop) || units.alloc(1024))
+    if (spool.alloc((size_t)nl + 2ull * nc1 * nl) || stA.alloc(nc1) || units.alloc(1024))
         return 2;
-    hipLaunchKernelGGL(k_split_init, dim3((ntop + 255) / 256), dim3(256), 0, st, dtop, ntop, stA.p,
-                       spool.p);
+    hipLaunchKernelGGL(k_split_init, dim3((std::max(nc1, nl) + 255) / 256), dim3(256), 0, st, c1top,
+                       nc1, ltop, nl, stA.p, spool.p);
     TC_HIP(hipGetLastError());
-    unsigned long long spool_top = (unsigned long long)ntop;
-    unsigned long long sumlen = (unsigned long long)ntop * ntop;
+    unsigned long long spool_top = (unsigned long long)nl;
+    unsigned long long sumlen = (unsigned long long)nc1 * nl;
     long long ntask = ntop, nunits = 0;
     unsigned long long usum = 0;      // the units' list lengths (bounds the first next lists)
     Compactor<SplitTask> cps;
@@ -1303,7 +1562,7 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
             (void)hipStreamSynchronize(stream());
             for (double* p : *c) dev_free(p, bytes);
         }
-    } cguard{&chunks, sizeof(double) * (size_t)Lo.total << SLOT_SHIFT};
+    } cguard{&chunks, sizeof(double) * (size_t)stot << SLOT_SHIFT};
     const long long ncomp = (long long)nb * ncell * NV;
     if (tq.alloc(std::max<long long>(nunits, 1)) || zhi.alloc(ncomp) || zlo.alloc(ncomp) ||
         red.alloc((size_t)NOUT3 * ntot) || chunk_tab.alloc(4096) || free_ids.alloc(1 << SLOT_SHIFT))
@@ -1369,7 +1628,7 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
         if (timing) TC_HIP(hipEventRecord(e1, st));
         // this level's zeta jobs, in task order
         // enough chunks for about 4096 workgroups, at least 8 jobs each
-        const long long jpc = std::max<long long>(8, std::min<long long>(2048, (ntask * nmr + 4095) / 4096));
+        const long long jpc = std::max<long long>(8, std::min<long long>(2048, (ntask * (X ? xsub : nmr) + 4095) / 4096));
         const int nchunks = (int)((ntask + jpc - 1) / jpc);
         if (chunk_rows.reserve(nchunks, 0, st)) return 2;
         if ((size_t)nchunks * ncomp > partial.n) {
@@ -1377,7 +1636,13 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
             if (partial.reserve((size_t)nchunks * ncomp, 0, st)) return 2;
             TC_HIP(hipMemsetAsync(partial.p, 0, sizeof(double) * partial.n, st));
         }
-        {
+        if (X) {
+            const int qpad = (nchunks + 7) / 8 * 8;
+            hipLaunchKernelGGL(zxfn, dim3((unsigned)(qpad * xsub)), dim3(ZT_THREADS), shbytes, st,
+                               jobs.p, (int)ntask, nodes, chunk_tab.p, P, xmr, xkg, (int)jpc,
+                               partial.p, chunk_rows.p, nchunks, xsub);
+            TC_HIP(hipGetLastError());
+        } else {
             // FP64: every cell; with the matrix cores, only the n = 0 sums, for m = 0
             const int zr_mr = mfn ? 1 : zmr, zr_sub = mfn ? 1 : nmr;
             const int qpad = (nchunks + 7) / 8 * 8;
@@ -1393,7 +1658,7 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
         }
         hipLaunchKernelGGL(k_mp_zeta_reduce, dim3((unsigned)((ncomp + 255) / 256)), dim3(256), 0, st,
                            partial.p, chunk_rows.p, nchunks, ncomp, (long long)nb * (M + 1) * NV,
-                           nb, zhi.p, zlo.p);
+                           nb, zhi.p, zlo.p, X ? 1 : 0);
         TC_HIP(hipGetLastError());
         if (timing) TC_HIP(hipEventRecord(e2, st));
         // next tasks, and the slots that became free, in task order
@@ -1425,7 +1690,8 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
         pin_is_split = false;
         if (level == 0) { spool.release(); units.release(); }   // the units' lists are consumed
     }
-    ExpandFn efn = D == GData ? &k_mp_expand<GData> : &k_mp_expand<NData>;
+    ExpandFn efn = X ? (D == GData ? &k_mp_expand_x<GData> : &k_mp_expand_x<NData>)
+                     : (D == GData ? &k_mp_expand<GData> : &k_mp_expand<NData>);
     hipLaunchKernelGGL(efn, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, zhi.p, zlo.p, P,
                        red.p);
     TC_HIP(hipGetLastError());
@@ -1433,6 +1699,74 @@ int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int
     TC_HIP(hipMemcpyAsync(cnt, counters.p, sizeof(unsigned long long) * NC3, hipMemcpyDeviceToHost, st));
     TC_HIP(hipStreamSynchronize(st));
     return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// Multipole passes over two or three fields (the cross-correlation drivers)
+// ---------------------------------------------------------------------------
+// dst = src with the child links moved by `off` (the trees laid end to end in one array)
+__global__ void k_nodes_shifted(const DNode* __restrict__ src, int n, int off,
+                                DNode* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DNode d = src[i];
+    if (d.child >= 0) d.child += off;
+    dst[i] = d;
+}
+__global__ void k_ints_shifted(const int* __restrict__ src, int n, int off, int* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + off;
+}
+
+int corr3_run_trees(int D, int C, const DeviceTree& t1, const DeviceTree& t2,
+                    const DeviceTree* t3, int ordered, const C3Params& P_in, double* out,
+                    unsigned long long* cnt) {
+    C3Params P = P_in;
+    if (!t3 && &t1 == &t2) {
+        P.order3 = 0;
+        return corr3_run_x(D, C, t1.nodes, t1.nnodes, t1.dtop, (int)t1.top.size(), t1.dtop,
+                           (int)t1.top.size(), P, out, cnt);
+    }
+    const DeviceTree* ts[3] = {&t1, &t2, t3};
+    const int nt = t3 ? 3 : 2;
+    long long total = 0;
+    int offs[3] = {0, 0, 0};
+    for (int i = 0; i < nt; ++i) {
+        offs[i] = (int)total;
+        total += ts[i]->nnodes;
+    }
+    if (total > 0x7ffffff0ll) return fail("multipole: the fields have more than 2^31 nodes in all");
+    const int nc1 = (int)t1.top.size();
+    int nl = 0;
+    for (int i = 1; i < nt; ++i) nl += (int)ts[i]->top.size();
+    DBuf<DNode> nodes;
+    DBuf<int> c1top, ltop;
+    if (nodes.alloc((size_t)total) || c1top.alloc(nc1) || ltop.alloc(nl)) return 2;
+    hipStream_t st = stream();
+    int lpos = 0;
+    for (int i = 0; i < nt; ++i) {
+        const int n = ts[i]->nnodes, nto = (int)ts[i]->top.size();
+        hipLaunchKernelGGL(k_nodes_shifted, dim3((n + 255) / 256), dim3(256), 0, st, ts[i]->nodes,
+                           n, offs[i], nodes.p + offs[i]);
+        TC_HIP(hipGetLastError());
+        if (i == 0) {
+            hipLaunchKernelGGL(k_ints_shifted, dim3((nto + 255) / 256), dim3(256), 0, st,
+                               ts[i]->dtop, nto, 0, c1top.p);
+        } else {
+            hipLaunchKernelGGL(k_ints_shifted, dim3((nto + 255) / 256), dim3(256), 0, st,
+                               ts[i]->dtop, nto, offs[i], ltop.p + lpos);
+            lpos += nto;
+        }
+        TC_HIP(hipGetLastError());
+    }
+    if (t3) {
+        P.order3 = ordered;
+        P.base3 = offs[2];
+    } else {
+        P.order3 = 0;
+    }
+    return corr3_run_x(D, C, nodes.p, (int)total, c1top.p, nc1, ltop.p, nl, P, out, cnt);
 }
 
 }  // namespace tc

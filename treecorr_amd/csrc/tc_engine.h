@@ -132,7 +132,10 @@ int corr2_run_pairs(Corr2Scratch& S, int D, int C, const std::vector<const Devic
 struct C3Params {
     int nbins, maxn, D, quick;
     int plain;      // Gn accumulation with every lane in multipole order (the untuned form)
-    int pad;
+    // Two lists (multipole(f1, f2, f3, ordered), src/Corr3.cpp:1238-1305): order3 is the
+    // ordered code, 1 or 4; list entries with node index >= base3 belong to the second
+    // list.  order3 = 0: one list (the auto-correlation and multipole(f1, f2)).
+    int order3, base3, pad;
     double minsep, maxsep, binsize, b, a;
     double logminsep, minsepsq, maxsepsq, bsq, asq, maxbsq_eff;
 };
@@ -142,6 +145,18 @@ struct C3Params {
 // calls, zeta calls, zeta rows, stages, zeta cells).
 int corr3_run(int D, int C, const DNode* nodes, int nnodes, const int* dtop, int ntop,
               const C3Params& P, double* out, unsigned long long* cnt);
+// One multipole pass over several trees laid out in one node array: c1 runs over the nc1
+// nodes c1top (device), the list starts as the nl nodes ltop (device); P.order3 and
+// P.base3 select the two-list form.  Writes and counts as corr3_run does.
+int corr3_run_x(int D, int C, const DNode* nodes, int nnodes, const int* c1top, int nc1,
+                const int* ltop, int nl, const C3Params& P, double* out,
+                unsigned long long* cnt);
+// One multipole pass of BaseCorr3::multipole (src/Corr3.cpp:1124-1305): c1 from t1, the
+// list from t2 (t3 NULL), or from t2 and t3 with ordered 1 or 4.  The trees are copied
+// into one node array unless t3 is NULL and t1, t2 are the same tree.
+int corr3_run_trees(int D, int C, const DeviceTree& t1, const DeviceTree& t2,
+                    const DeviceTree* t3, int ordered, const C3Params& P, double* out,
+                    unsigned long long* cnt);
 int set_multipole_plain(int on);   // < 0: returns the current setting
 // zeta contraction: 0 FP64 (default), 1 FP32 MFMA, 2 split-BF16 MFMA; < 0 returns it
 int set_zeta_mode(int mode);
