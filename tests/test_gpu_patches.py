@@ -482,3 +482,38 @@ def test_gpu_local_patch_method(gpu, oracle_lib, cross):
         check_pair(kk.results[(i, i)], ref, 1)
         tot += ref['npairs']
     np.testing.assert_array_equal(kk.npairs, tot)
+
+
+@pytest.mark.gpu
+def test_gpu_write_patch_results_round_trip(gpu, tmp_path):
+    """write(write_patch_results=True) then from_file: the object read back gives the
+    same jackknife and bootstrap covariances as the one processed (corr2base.py:
+    1959-2050), for KK and for NN with randoms (its rr written alongside)."""
+    rng = np.random.default_rng(44)
+    n, nr = 20000, 40000
+    x, y = rng.uniform(0, 100, n), rng.uniform(0, 100, n)
+    k = rng.normal(0, 1, n)
+    cat = gpu.Catalog(x=x, y=y, k=k, npatch=8, rng=np.random.default_rng(1))
+    rcat = gpu.Catalog(x=rng.uniform(0, 100, nr), y=rng.uniform(0, 100, nr),
+                       patch_centers=cat.patch_centers)
+    print('write_patch_results round trip n', n, flush=True)
+    kk = gpu.KKCorrelation(min_sep=1., max_sep=20., nbins=8, var_method='jackknife')
+    kk.process(cat)
+    f = str(tmp_path / 'kk.out')
+    kk.write(f, write_patch_results=True)
+    kk2 = gpu.KKCorrelation.from_file(f)
+    np.testing.assert_allclose(kk2.estimate_cov('jackknife'), kk.estimate_cov('jackknife'),
+                               rtol=1e-13, atol=0)
+    dd = gpu.NNCorrelation(min_sep=1., max_sep=20., nbins=8)
+    rr = gpu.NNCorrelation(min_sep=1., max_sep=20., nbins=8)
+    dd.process(cat)
+    rr.process(rcat)
+    dd.calculateXi(rr=rr)
+    f = str(tmp_path / 'nn.out')
+    dd.write(f, rr=rr, write_patch_results=True, precision=16)
+    dd2 = gpu.NNCorrelation.from_file(f)
+    assert dd2._rr is not None and dd2.tot == dd.tot
+    xi2, _ = dd2.calculateXi(rr=dd2._rr)
+    np.testing.assert_allclose(xi2, dd.xi, rtol=1e-10)
+    np.testing.assert_allclose(dd2.estimate_cov('jackknife'), dd.estimate_cov('jackknife'),
+                               rtol=1e-12)

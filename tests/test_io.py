@@ -1,0 +1,96 @@
+"""Output files: TreeCorr's ASCII layout, and write/read round trips of the correlation
+classes with their patch results (no GPU: the arrays are filled by hand)."""
+import os
+
+import numpy as np
+import pytest
+
+
+def _filled(cls, seed, nbins=6, npatch=3, **kw):
+    from treecorr_amd.corr2 import _PairSums
+    rng = np.random.default_rng(seed)
+    c = cls(min_sep=1., max_sep=50., nbins=nbins, **kw)
+    c.coords, c.metric = 'flat', 'Euclidean'
+    for a in [c.meanr, c.meanlogr, c.weight, c.npairs] + [x for x in c._xi if x is not None]:
+        a[:] = rng.normal(size=nbins)
+    c.weight[:] = np.abs(c.weight)
+    c._var_num = 0.3
+    c.npatch1 = c.npatch2 = npatch
+    for i in range(npatch):
+        for j in range(i, npatch):
+            ps = _PairSums(nbins, c._nxi, rng.normal(size=(8, nbins)))
+            if (i, j) == (0, 2):
+                ps.weight[:] = 0.       # a pair without weight
+            ps.tot = float(rng.uniform(1, 2))
+            c.results[(i, j)] = ps
+    if hasattr(c, 'tot'):
+        c.tot = 12.5
+    return c
+
+
+def test_ascii_layout_matches_treecorr_writer(tmp_path):
+    """writer.py:51-77: '## ext', '## {params}', names centred in precision + 8 columns
+    (the first one narrower by the '#'), rows in %{p+8}.{p}e."""
+    from treecorr_amd.io import make_writer
+    f = str(tmp_path / 'x.dat')
+    with make_writer(f, 3) as w:
+        w.write(['r_nom', 'xi'], [np.array([1.5, 20.]), np.array([-0.25, 3e-7])],
+                params={'corr': 'KK', 'tot': np.float64(2.)}, ext='main')
+    lines = open(f).read().splitlines()
+    assert lines[0] == '## main'
+    assert lines[1] == "## {'corr': 'KK', 'tot': 2.0}"
+    assert lines[2] == '#' + '{:^10}'.format('r_nom') + ' ' + '{:^11}'.format('xi')
+    assert lines[3] == '%11.3e %11.3e' % (1.5, -0.25)
+    assert lines[4] == '%11.3e %11.3e' % (20., 3e-7)
+
+
+@pytest.mark.parametrize('name', ['KK', 'GG', 'NG', 'NK', 'KQ', 'NN'])
+def test_write_read_round_trip_with_patch_results(tmp_path, name):
+    import treecorr_amd as tca
+    cls = getattr(tca, name + 'Correlation')
+    c = _filled(cls, 3)
+    f = str(tmp_path / ('c_%s.out' % name))
+    c.write(f, write_patch_results=True, precision=16)
+    d = cls.from_file(f)
+    assert d.nbins == c.nbins and d.npatch1 == 3 and d.coords == 'flat'
+    for nm in ('meanr', 'meanlogr', 'weight', 'npairs'):
+        np.testing.assert_array_equal(getattr(d, nm), getattr(c, nm))
+    for a, b in zip(c._xi, d._xi):
+        if a is not None:
+            np.testing.assert_array_equal(a, b)
+    assert sorted(d.results) == sorted(c.results)
+    for k, ps in c.results.items():
+        q = d.results[k]
+        if name == 'NN' and not ps._nonzero:
+            assert q.tot == ps.tot and not np.any(q.weight)      # stored as a zero pair
+            continue
+        for nm in ('meanr', 'meanlogr', 'weight', 'npairs'):
+            np.testing.assert_array_equal(getattr(q, nm), getattr(ps, nm))
+        for a, b in zip(ps._xi, q._xi):
+            if a is not None:
+                np.testing.assert_array_equal(a, b)
+    if name == 'NN':
+        assert d.tot == c.tot
+    # read() into an object of the same configuration
+    e = cls(min_sep=1., max_sep=50., nbins=6)
+    e.read(f)
+    np.testing.assert_array_equal(e.weight, c.weight)
+
+
+def test_default_precision_is_four_digits(tmp_path):
+    import treecorr_amd as tca
+    c = _filled(tca.KKCorrelation, 4)
+    f = str(tmp_path / 'kk.out')
+    c.write(f)
+    d = tca.KKCorrelation.from_file(f)
+    np.testing.assert_allclose(d.xi, c.xi, rtol=1e-4)
+    assert not np.array_equal(d.xi, c.xi)
+    assert d.results == {}
+
+
+def test_fits_and_hdf_are_refused(tmp_path):
+    import treecorr_amd as tca
+    c = _filled(tca.KKCorrelation, 5)
+    for ext in ('fits', 'hdf5'):
+        with pytest.raises(NotImplementedError):
+            c.write(str(tmp_path / ('kk.' + ext)))

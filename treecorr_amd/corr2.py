@@ -18,6 +18,7 @@ from .catalog import (Catalog, calculateVarG, calculateVarK, calculateVarQ, calc
                       calculateVarV, calculateVarZ)
 from .config import parse_unit
 from .field import build_patch_fields, coord_enum
+from .io import literal as _io_literal, make_minimal_config, make_reader, make_writer
 from .util import set_omp_threads
 
 LOG = 0
@@ -25,6 +26,18 @@ EUCLIDEAN, RPERP, RLENS, ARC, OLDRPERP, PERIODIC = 1, 2, 3, 4, 5, 6    # include
 # util.py:355-372; 'Rperp' is TreeCorr's default alias, 'FisherRperp'
 _METRIC_ENUM = {'Euclidean': EUCLIDEAN, 'Rperp': RPERP, 'FisherRperp': RPERP,
                 'OldRperp': OLDRPERP, 'Rlens': RLENS, 'Arc': ARC, 'Periodic': PERIODIC}
+
+
+# Corr2._valid_params (corr2base.py:271-345): the construction keys and their defaults,
+# which make_minimal_config drops from the params line of an output file
+_VALID_PARAMS = {
+    'nbins': None, 'bin_size': None, 'min_sep': None, 'max_sep': None, 'sep_units': None,
+    'bin_slop': None, 'angle_slop': None, 'brute': False, 'verbose': 1, 'log_file': None,
+    'output_dots': None, 'split_method': 'mean', 'min_top': None, 'max_top': 10,
+    'precision': 4, 'm2_uform': 'Crittenden', 'metric': 'Euclidean', 'bin_type': 'Log',
+    'min_rpar': None, 'max_rpar': None, 'period': None, 'xperiod': None, 'yperiod': None,
+    'zperiod': None, 'var_method': 'shot', 'num_bootstrap': 500, 'cross_patch_weight': None,
+    'num_threads': None}
 
 
 class _PairSums:
@@ -841,6 +854,184 @@ class Corr2:
         return self._cov if self._cov.ndim == 1 else self._cov.diagonal()
 
 
+    # ---- output files (corr2base.py:1949-2050; the per-class columns of kkcorrelation.py,
+    # zzcorrelation.py, nkcorrelation.py, nzcorrelation.py, kzcorrelation.py,
+    # nncorrelation.py) ----
+    # _write_xi: the xi columns of the class as (name, attribute, slot): the finalized
+    # object's column is the attribute; a patch pair's raw sums give _xi[slot].  slot None
+    # marks a sigma column, the square root of the attribute (a variance), zero for raw
+    # sums.
+    _write_xi = ()
+
+    def _xi_columns(self, src, raw):
+        cols = []
+        for _, attr, slot in self._write_xi:
+            if slot is None:
+                cols.append(np.zeros(self.nbins) if raw else np.sqrt(getattr(src, attr)))
+            else:
+                cols.append(src._xi[slot] if raw else getattr(src, attr))
+        return cols
+
+    def _set_xi_columns(self, data):
+        var = []
+        for name, _, slot in self._write_xi:
+            col = np.array(data[name], dtype=float)
+            if slot is None:
+                var.append(col ** 2)
+            else:
+                self._xi[slot] = col
+        self._varxi = var if var else None
+
+    @property
+    def _write_col_names(self):
+        return ['r_nom', 'meanr', 'meanlogr'] + [c[0] for c in self._write_xi] + ['weight',
+                                                                                 'npairs']
+
+    def _write_data_of(self, src, raw):
+        return [np.asarray(c, dtype=float).ravel() for c in
+                [self.rnom, src.meanr, src.meanlogr] + self._xi_columns(src, raw) +
+                [src.weight, src.npairs]]
+
+    @property
+    def _write_params(self):
+        params = make_minimal_config(self.config, _VALID_PARAMS)
+        params['coords'] = self.coords
+        params['metric'] = self.metric
+        params['corr'] = self._letters
+        return params
+
+    def _pair_params(self, ps):
+        return self._write_params
+
+    def write(self, file_name, *, file_type=None, precision=None, write_patch_results=False,
+              write_cov=False):
+        """Write the correlation to a file (ASCII; kkcorrelation.py:120-157).  With
+        write_patch_results every patch pair's raw sums follow at 16 digits, so that a
+        covariance estimate of the object read back equals this one's."""
+        precision = self.config.get('precision', 4) if precision is None else precision
+        with make_writer(file_name, precision, file_type) as writer:
+            self._write(writer, None, write_patch_results, write_cov=write_cov)
+
+    def _write(self, writer, name, write_patch_results, write_cov=False, zero_tot=False):
+        # corr2base.py:1959-2003
+        if name is None and (write_patch_results or write_cov):
+            name = 'main'
+        col_names = self._write_col_names
+        data = self._write_data_of(self, raw=False)
+        params = self._write_params
+        if write_patch_results:
+            params['npatch1'] = self.npatch1
+            params['npatch2'] = self.npatch2
+            params['num_rows'] = len(self.rnom.ravel())
+            num_patch_pairs = len(self.results)
+            if zero_tot:
+                i = 0
+                for key, ps in self.results.items():
+                    if not ps._nonzero:
+                        params[name + '_zp_%d' % i] = repr((key, float(ps.tot)))
+                        num_patch_pairs -= 1
+                        i += 1
+                params['num_zero_patch'] = i
+            params['num_patch_pairs'] = num_patch_pairs
+        if write_cov:
+            params['cov_shape'] = self.cov.shape
+        writer.write(col_names, data, params=params, ext=name)
+        if write_patch_results:
+            writer.set_precision(16)
+            i = 0
+            for key, ps in self.results.items():
+                if zero_tot and not ps._nonzero:
+                    continue
+                pparams = self._pair_params(ps)
+                pparams['key'] = repr(key)
+                writer.write(self._pair_col_names, self._write_data_of(ps, raw=True),
+                             params=pparams, ext=name + '_pp_%d' % i)
+                i += 1
+            assert i == num_patch_pairs
+        if write_cov:
+            writer.write_array(self.cov, ext='cov')
+
+    @property
+    def _pair_col_names(self):
+        return self._write_col_names
+
+    def read(self, file_name, *, file_type=None):
+        """Read values written by write() into this object, which must have the same
+        configuration (corr2base.py:2052-2070)."""
+        with make_reader(file_name, file_type) as reader:
+            self._do_read(reader)
+
+    @classmethod
+    def from_file(cls, file_name, *, file_type=None, logger=None, rng=None):
+        """A new object from a file written by write() (corr2base.py:1722-1760)."""
+        with make_reader(file_name, file_type) as reader:
+            params = reader.read_params(ext=None)
+            if params.get('corr', None) != cls._letters:
+                raise OSError("Trying to read a %sCorrelation output file with %s"
+                              % (params.get('corr'), cls.__name__))
+            kwargs = make_minimal_config(params, _VALID_PARAMS)
+            corr = cls(**kwargs, rng=rng)
+            corr._do_read(reader, name=None, params=params)
+        return corr
+
+    def _do_read(self, reader, name=None, params=None):
+        self._read(reader, name, params)
+
+    def _read(self, reader, name=None, params=None):
+        # corr2base.py:2005-2036
+        if params is None:
+            params = reader.read_params(ext=name)
+        num_rows = params.get('num_rows', None)
+        num_patch_pairs = params.get('num_patch_pairs', 0)
+        num_zero_patch = params.get('num_zero_patch', 0)
+        cov_shape = params.get('cov_shape', None)
+        name = 'main' if num_patch_pairs and name is None else name
+        data = reader.read_data(max_rows=num_rows, ext=name)
+        self._read_from_data(data, params)
+        self.results = {}
+        for i in range(num_zero_patch):
+            key, tot = _io_literal(params[name + '_zp_%d' % i])
+            self.results[key] = _PairSums(self.nbins, self._nxi, None, tot=tot)
+        for i in range(num_patch_pairs):
+            pname = name + '_pp_%d' % i
+            pparams = reader.read_params(ext=pname)
+            pdata = reader.read_data(max_rows=num_rows, ext=pname)
+            key = _io_literal(pparams['key'])
+            self.results[key] = self._pair_from_data(pdata, pparams)
+        if cov_shape is not None:
+            if isinstance(cov_shape, str):
+                cov_shape = _io_literal(cov_shape)
+            self._cov = reader.read_array(tuple(cov_shape), ext='cov')
+
+    def _read_from_data(self, data, params):
+        self.meanr = np.array(data['meanr'], dtype=float)
+        self.meanlogr = np.array(data['meanlogr'], dtype=float)
+        if 'weight' in data.dtype.names:
+            self.weight = np.array(data['weight'], dtype=float)
+        self.npairs = np.array(data['npairs'], dtype=float)
+        self._set_xi_columns(data)
+        self.coords = params['coords'].strip() if params.get('coords') else params.get('coords')
+        self.metric = params['metric'].strip() if params.get('metric') else params.get('metric')
+        self.npatch1 = params.get('npatch1', 1)
+        self.npatch2 = params.get('npatch2', 1)
+        # the engine object holds the old arrays: drop it (a new one is made on use)
+        h = getattr(self, '_corr', None)
+        if h is not None and h.value:
+            _lib.lib().tc_corr2_destroy(h)
+        self._corr = None
+
+    def _pair_from_data(self, data, params):
+        ps = _PairSums(self.nbins, self._nxi, None)
+        ps.meanr = np.array(data['meanr'], dtype=float)
+        ps.meanlogr = np.array(data['meanlogr'], dtype=float)
+        ps.weight = np.array(data['weight'], dtype=float)
+        ps.npairs = np.array(data['npairs'], dtype=float)
+        for nm, _, slot in self._write_xi:
+            if slot is not None:
+                ps._xi[slot] = np.array(data[nm], dtype=float)
+        return ps
+
+
 class NNCorrelation(Corr2):
     """Count-count correlation (treecorr/nncorrelation.py)."""
     _letters = 'NN'
@@ -1035,11 +1226,133 @@ class NNCorrelation(Corr2):
         self._rr_weight = denom
 
 
+
+    # ---- output files (nncorrelation.py:560-700) ----
+    _write_rr = _write_dr = _write_rd = None
+    _write_patch_results = False
+
+    @property
+    def _write_col_names(self):
+        cols = ['r_nom', 'meanr', 'meanlogr']
+        rr, dr, rd = self._write_rr, self._write_dr, self._write_rd
+        if rr is None:
+            if self.xi is not None:
+                cols += ['xi', 'sigma_xi']
+            cols += ['DD', 'npairs']
+        else:
+            cols += ['xi', 'sigma_xi', 'DD', 'RR']
+            if dr is not None and rd is not None:
+                cols += ['DR', 'RD']
+            elif dr is not None or rd is not None:
+                cols += ['DR']
+            cols += ['npairs']
+        return cols
+
+    @property
+    def _pair_col_names(self):
+        return ['r_nom', 'meanr', 'meanlogr', 'DD', 'npairs']
+
+    def _write_data_of(self, src, raw):
+        if raw:
+            data = [self.rnom, src.meanr, src.meanlogr, src.weight, src.npairs]
+            return [np.asarray(c, dtype=float).ravel() for c in data]
+        data = [self.rnom, self.meanr, self.meanlogr]
+        rr, dr, rd = self._write_rr, self._write_dr, self._write_rd
+        if rr is None:
+            if dr is not None:
+                raise TypeError("rr must be provided if dr is not None")
+            if rd is not None:
+                raise TypeError("rr must be provided if rd is not None")
+            if self.xi is not None:
+                data += [self.xi, np.sqrt(self.varxi)]
+            data += [self.weight, self.npairs]
+        else:
+            xi, varxi = self.calculateXi(rr=rr, dr=dr, rd=rd)
+            data += [xi, np.sqrt(varxi), self.weight, rr.weight * (self.tot / rr.tot)]
+            if dr is not None and rd is not None:
+                data += [dr.weight * (self.tot / dr.tot), rd.weight * (self.tot / rd.tot)]
+            elif dr is not None or rd is not None:
+                d = dr if dr is not None else rd
+                data += [d.weight * (self.tot / d.tot)]
+            data += [self.npairs]
+        return [np.asarray(c, dtype=float).ravel() for c in data]
+
+    @property
+    def _write_params(self):
+        params = super()._write_params
+        params['tot'] = float(self.tot)
+        if self._write_patch_results:
+            params['_rr'] = bool(self._rr)
+            params['_dr'] = bool(self._dr)
+            params['_rd'] = bool(self._rd)
+        return params
+
+    def _pair_params(self, ps):
+        params = Corr2._write_params.fget(self)
+        params['tot'] = float(ps.tot)
+        return params
+
+    def write(self, file_name, *, rr=None, dr=None, rd=None, file_type=None, precision=None,
+              write_patch_results=False, write_cov=False):
+        """Write DD (and xi from rr, dr, rd when given) to a file (nncorrelation.py:560-640);
+        with write_patch_results the randoms' correlations used by calculateXi follow, so
+        that covariances can be computed from the object read back."""
+        precision = self.config.get('precision', 4) if precision is None else precision
+        self._write_rr, self._write_dr, self._write_rd = rr, dr, rd
+        self._write_patch_results = write_patch_results
+        try:
+            with make_writer(file_name, precision, file_type) as writer:
+                self._write(writer, None, write_patch_results, write_cov=write_cov,
+                            zero_tot=True)
+                if write_patch_results:
+                    rr = rr or self._rr
+                    dr = dr or self._dr
+                    rd = rd or self._rd
+                    for nm, c in (('_rr', rr), ('_dr', dr), ('_rd', rd)):
+                        if c:
+                            c._write(writer, nm, write_patch_results, zero_tot=True)
+        finally:
+            self._write_rr = self._write_dr = self._write_rd = None
+            self._write_patch_results = False
+
+    def _do_read(self, reader, name=None, params=None):
+        self._read(reader, name, params)
+        # _read leaves flags (whether rr, dr, rd follow in the file) in _rr, _dr, _rd
+        flags = [(nm, getattr(self, nm)) for nm in ('_rr', '_dr', '_rd')]
+        for nm, _ in flags:
+            setattr(self, nm, None)
+        for nm, flag in flags:
+            if flag:
+                c = self.copy()
+                c._read(reader, name=nm)
+                setattr(self, nm, c)
+
+    def _read_from_data(self, data, params):
+        super()._read_from_data(data, params)
+        self.weight = np.array(data['DD'], dtype=float)
+        self.tot = params['tot']
+        if 'xi' in data.dtype.names:
+            self.xi = np.array(data['xi'], dtype=float)
+            self.varxi = np.array(data['sigma_xi'], dtype=float) ** 2
+        self._rr = params.get('_rr', None) or None
+        self._dr = params.get('_dr', None) or None
+        self._rd = params.get('_rd', None) or None
+
+    def _pair_from_data(self, data, params):
+        ps = _PairSums(self.nbins, 0, None, tot=params['tot'])
+        ps.meanr = np.array(data['meanr'], dtype=float)
+        ps.meanlogr = np.array(data['meanlogr'], dtype=float)
+        ps.weight = np.array(data['DD'], dtype=float)
+        ps.npairs = np.array(data['npairs'], dtype=float)
+        return ps
+
+
 class KKCorrelation(Corr2):
     """Scalar-scalar correlation (treecorr/kkcorrelation.py)."""
     _letters = 'KK'
     _d = 1
     _nxi = 1
+    _write_xi = (('xi', 'xi', 0), ('sigma_xi', 'varxi', None))
     _default_angle_slop = 1
 
     @property
@@ -1066,6 +1379,9 @@ class GGCorrelation(Corr2):
     _letters = 'GG'
     _d = 4
     _nxi = 4
+    _write_xi = (('xip', 'xip', 0), ('xim', 'xim', 2), ('xip_im', 'xip_im', 1),
+                 ('xim_im', 'xim_im', 3), ('sigma_xip', 'varxip', None),
+                 ('sigma_xim', 'varxim', None))
 
     xip = property(lambda self: self._xi[0])
     xip_im = property(lambda self: self._xi[1])
@@ -1100,6 +1416,14 @@ class GGCorrelation(Corr2):
         return self._varxi[1]
 
 
+# The xi column names of the mixed classes (nkcorrelation.py _write_col_names,
+# nzcorrelation.py _zreal/_zimag, kzcorrelation.py _xireal/_xiimag of each class)
+_MIXED_COLUMNS = {'NK': ('kappa',), 'NG': ('gamT', 'gamX'), 'KG': ('kgamT', 'kgamX'),
+                  'NZ': ('z_real', 'z_imag'), 'KZ': ('xi', 'xi_im'), 'NV': ('vR', 'vT'),
+                  'KV': ('xi', 'xi_im'), 'NT': ('tR', 'tR_im'), 'KT': ('xi', 'xi_im'),
+                  'NQ': ('qR', 'qR_im'), 'KQ': ('xi', 'xi_im')}
+
+
 class _MixedCorrelation(Corr2):
     """Count or scalar around points: NK, NG, KG (cross only).  xi (and xi_im for a
     shear second field); NK and NG can subtract the same correlation around random
@@ -1116,6 +1440,20 @@ class _MixedCorrelation(Corr2):
 
     raw_xi = property(lambda self: self._xi[0])
     raw_xi_im = property(lambda self: self._xi[1])
+
+    @property
+    def _write_xi(self):
+        names = _MIXED_COLUMNS[self._letters]
+        cols = [(names[0], 'xi', 0)]
+        if len(names) > 1:
+            cols.append((names[1], 'xi_im', 1))
+        return tuple(cols) + (('sigma', 'varxi', None),)
+
+    def _read_from_data(self, data, params):
+        super()._read_from_data(data, params)
+        self._use_raw()
+        self._rz = None
+        self._comp_varxi = None
 
     def _use_raw(self):
         self.xi = self.raw_xi
