@@ -156,6 +156,15 @@ int tc_select_patch(int patch, const double* centers, int npatch, const double* 
 int tc_kmeans_run(double* centers, int npatch, int dim, const double* x, const double* y,
                   const double* z, const double* w, long long n, int max_iter, double tol,
                   double size, int sphere, int* niter);
+/* KMeansRun with alt=True (src/KMeans.cpp:774-833, bound at :1066-1067; called from
+ * treecorr/field.py:454-560): every iteration first estimates each patch's inertia on
+ * the field's tree (CalculateInertia over the cells where the candidate pruning of
+ * FindCellsInPatches stops, :388-447, 544-643), then moves the centres with the
+ * assignment penalised by that inertia.  The objects (x, y, z, w: host arrays, positions
+ * as the tree holds them) and the field must be the same catalog's. */
+int tc_kmeans_run_alt(const tc_field* f, double* centers, int npatch, const double* x,
+                      const double* y, const double* z, const double* w, long long n,
+                      int max_iter, double tol, double size, int* niter);
 /* tc_kmeans_run without allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same arguments and results as

@@ -75,6 +75,8 @@ SIGNATURES = {
     'tc_set_zeta_mode': (_i, [_i]),
     'tc_kmeans_run': (_i, [_dp, _i, _i, _dp, _dp, _dp, _dp, _ll, _i, _d, _d, _i,
                            ctypes.POINTER(ctypes.c_int)]),
+    'tc_kmeans_run_alt': (_i, [_vp, _dp, _i, _dp, _dp, _dp, _dp, _ll, _i, _d, _d,
+                               ctypes.POINTER(ctypes.c_int)]),
     'tc_kmeans_run_noalloc': (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _ll, _i, _d, _d, _i,
                                    ctypes.POINTER(ctypes.c_int)]),
     'tc_patch_split': (_i, [_ip, _ll, _i, _i, ctypes.POINTER(_dp), ctypes.POINTER(_dp),

@@ -772,6 +772,25 @@ int tc_kmeans_run(double* centers, int npatch, int dim, const double* x, const d
                           niter);
 }
 
+// KMeansRun with alt=True (src/KMeans.cpp:774-833): each iteration first estimates every
+// patch's inertia on the field's tree, then moves the centres with the inertia-penalised
+// assignment of the objects.
+int tc_kmeans_run_alt(const tc_field* f, double* centers, int npatch, const double* x,
+                      const double* y, const double* z, const double* w, long long n,
+                      int max_iter, double tol, double size, int* niter) {
+    if (!f) return tc::fail("NULL field");
+    const int dim = f->coords == Flat ? 2 : 3;
+    if (!centers || npatch <= 0 || !x || !y || (dim == 3 && !z) || !w || n <= 0 ||
+        max_iter < 0 || !(tol >= 0.) || !(size >= 0.))
+        return tc::fail("NULL or invalid argument");
+    if (f->T.top.empty()) return tc::fail("the field has no tree");
+    if (int rc = check_gpu()) return rc;
+    std::lock_guard<std::mutex> run_lock(g_run_m);
+    return tc::kmeans_run_tree(centers, npatch, dim, x, y, z, w, n, max_iter, tol, size,
+                               f->coords == Sphere ? 1 : 0, f->T.nodes, f->T.nnodes, f->T.dtop,
+                               (int)f->T.top.size(), niter);
+}
+
 int tc_kmeans_run_noalloc(double* centers, int npatch, int dim, const double* x,
                           const double* y, const double* z, const double* w, long long n,
                           int max_iter, double tol, double size, int sphere, int* niter) {

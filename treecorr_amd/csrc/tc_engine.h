@@ -61,6 +61,12 @@ struct C2Params {
 int kmeans_run(double* centers, int npatch, int dim, const double* x, const double* y,
                const double* z, const double* w, long long n, int max_iter, double tol,
                double size, int sphere, int* niter);
+// kmeans_run with the alternate algorithm's inertia penalty (KMeansRun1 with alt,
+// src/KMeans.cpp:791-814) computed on the field's tree (nodes, top cells on the device)
+int kmeans_run_tree(double* centers, int npatch, int dim, const double* x, const double* y,
+                    const double* z, const double* w, long long n, int max_iter, double tol,
+                    double size, int sphere, const DNode* nodes, int nnodes, const int* dtop,
+                    int ntop, int* niter);
 // kmeans_run on device-resident columns and centres, with a fixed static scratch array
 // (no allocation): at most 8192 centres.
 int kmeans_run_noalloc(double* centers, int npatch, int dim, const double* x, const double* y,

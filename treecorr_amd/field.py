@@ -157,11 +157,10 @@ class Field:
     def kmeans_refine_centers(self, centers, *, max_iter=200, tol=1.e-5, alt=False):
         """Lloyd iterations on the GPU (field.py:454-560); centers is updated in place."""
         from . import kmeans
-        if alt:
-            # the alternate algorithm penalises by an inertia the reference estimates per
-            # tree cell (0.75 w s^2, src/KMeans.cpp:396-407); not restated here
-            raise NotImplementedError("alt=True k-means is not implemented by treecorr_amd")
         x, y, z, w = self._kmeans_objects()
+        if alt:
+            return kmeans.refine_alt(self.data, centers, x, y, z, w, self.size,
+                                     max_iter=max_iter, tol=tol)
         return kmeans.refine(centers, x, y, z, w, self.size, self.coords, max_iter=max_iter,
                              tol=tol)
 

@@ -257,6 +257,21 @@ def refine(centers, x, y, z, w, size, coords, max_iter=200, tol=1.e-5):
     return niter.value
 
 
+def refine_alt(field_handle, centers, x, y, z, w, size, max_iter=200, tol=1.e-5):
+    """The alternate k-means (KMeansRun with alt=True, src/KMeans.cpp:774-833): each
+    iteration estimates the patches' inertia on the field's tree, then assigns every
+    object to its least |p - c|^2 + I (the inertia over the mean patch weight / 3)."""
+    dim = centers.shape[1]
+    c = np.ascontiguousarray(centers, dtype=float)
+    niter = ctypes.c_int()
+    _lib.check(_lib.lib().tc_kmeans_run_alt(
+        field_handle, _lib.dptr(c), c.shape[0], _lib.dptr(x), _lib.dptr(y),
+        _lib.dptr(z) if dim == 3 else None, _lib.dptr(w), len(x), int(max_iter), float(tol),
+        float(size), ctypes.byref(niter)))
+    centers[:] = c
+    return niter.value
+
+
 def assign(centers, x, y, z):
     """Each object's nearest centre, the lowest index on ties."""
     n = len(x)
