@@ -280,9 +280,16 @@ int tc_graph_stats(long long* captures, long long* replays);
  * side, and any difference makes the build start again without speculation, so the
  * trees are always the reference's.  mode 0: off; 1 (default): levels whose segments
  * hold 16384 objects or more; 3: every level of at most 64 segments; 2: as 3 with the
- * check forced to fail (2 and 3 are for tests).  Returns 0, or with mode < 0 changes
- * nothing and returns the number of exact rebuilds so far.  No reference counterpart:
- * TreeCorr builds its trees on the host (src/Cell.cpp:556-615). */
+ * check forced to fail (2 and 3 are for tests).  Only levels above min_top are
+ * speculated on.  When a single tree's speculative decisions pass a validation by
+ * rounding bounds (no object within the bound of a split value, no cell near size 0),
+ * the build returns while the exact pass still runs: it writes only the records above
+ * the top cells and the field's centre and size, which the traversals do not read, and
+ * every call that does read them (tc_field_size, tc_field_center, tc_field_copy_nodes,
+ * cross-correlation early exits, k-means, the multipole) waits for it.  Returns 0, or
+ * with mode -2 the number of such builds so far, with another mode < 0 the number of
+ * exact rebuilds so far (nothing changes).  No reference counterpart: TreeCorr builds
+ * its trees on the host (src/Cell.cpp:556-615). */
 int tc_set_tree_speculation(int mode);
 /* The multipole's Gn accumulation in its plain form: every lane of a wavefront adds the
  * multipoles m = 0, 1, ... in order, so lanes with pairs in the same bin contend for one

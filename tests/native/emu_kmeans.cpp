@@ -61,7 +61,7 @@ int main(int argc, char** argv) {
     const double* z = C == Flat ? nullptr : col[2].data();
     if (build_tree(NData, C, col[0].data(), col[1].data(), z, nullptr, nullptr, col[3].data(),
                    nullptr, n, 0., std::numeric_limits<double>::infinity(), Mean, 0, 0,
-                   (int)h[4], T)) {
+                   (int)h[4], T) || settle_tree(T)) {
         std::fprintf(stderr, "emu_kmeans: tree build: %s\n", g_err.c_str());
         return 3;
     }

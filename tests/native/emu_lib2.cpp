@@ -63,6 +63,7 @@ int main(int argc, char** argv) {
         const double* g2 = D == GData ? col[s][4].data() : nullptr;
         int rc = build_tree(D, C, col[s][0].data(), col[s][1].data(), z, g1, g2, col[s][5].data(),
                             nullptr, n[s], p[5], p[6], Mean, 0, (int)h[8], 10, T[s]);
+        if (rc == 0) rc = settle_tree(T[s]);
         if (rc) {
             std::fprintf(stderr, "emu_lib2: tree build: %s\n", g_err.c_str());
             return 3;

@@ -19,6 +19,7 @@
 #include "tc_mp.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -70,7 +71,7 @@ static int cross_main(std::ifstream& in, const char* outpath) {
         const double* g1 = D == GData ? col[3].data() : nullptr;
         const double* g2 = D == GData ? col[4].data() : nullptr;
         if (build_tree(D, C, col[0].data(), col[1].data(), z, g1, g2, col[5].data(), nullptr, n,
-                       p[5], p[6], Mean, 0, (int)h[5], (int)h[6], T[f])) {
+                       p[5], p[6], Mean, 0, (int)h[5], (int)h[6], T[f]) || settle_tree(T[f])) {
             std::fprintf(stderr, "emu_lib3: tree build: %s\n", g_err.c_str());
             return 3;
         }
@@ -109,6 +110,8 @@ static int cross_main(std::ifstream& in, const char* outpath) {
 int main(int argc, char** argv) {
     using namespace tc;
     if (argc != 3) return 2;
+    // EMU_TREE_SPEC: the tree builder's speculation mode (tc_set_tree_speculation)
+    if (const char* m = std::getenv("EMU_TREE_SPEC")) set_tree_speculation(std::atoi(m));
     std::ifstream in(argv[1], std::ios::binary);
     {
         long long mode = 0;
@@ -134,6 +137,7 @@ int main(int argc, char** argv) {
     const double* g2 = D == GData ? col[4].data() : nullptr;
     int rc = build_tree(D, C, col[0].data(), col[1].data(), z, g1, g2, col[5].data(), nullptr, n,
                         p[5], p[6], Mean, 0, (int)h[6], (int)h[7], T);
+    if (rc == 0) rc = settle_tree(T);
     if (rc) {
         std::fprintf(stderr, "emu_lib3: tree build: %s\n", g_err.c_str());
         return 3;

@@ -172,9 +172,14 @@ def run_emu_lib(exe, d, coords, cat, bn, min_top, env):
 
 
 LIB_RUNS = [(c, {'EMU_WORKERS': '4', 'EMU_SHUFFLE': str(11 + i)}) for i, c in enumerate(C.CORR3_CASES)]
+# the tree's top levels partitioned speculatively (every level above min_top), validated
+# by rounding bounds, their exact pass finished before the tree is read
+LIB_RUNS += [(C.CORR3_CASES[1], {'EMU_WORKERS': '4', 'EMU_SHUFFLE': '41', 'EMU_TREE_SPEC': '3'}),
+             (C.CORR3_CASES[2], {'EMU_WORKERS': '4', 'EMU_SHUFFLE': '43', 'EMU_TREE_SPEC': '3'})]
 
 
-@pytest.mark.parametrize('case,env', LIB_RUNS, ids=[c[0] for c, _ in LIB_RUNS])
+@pytest.mark.parametrize('case,env', LIB_RUNS,
+                         ids=[c[0] + ('-spec' if 'EMU_TREE_SPEC' in e else '') for c, e in LIB_RUNS])
 def test_emulated_library_multipole_on_device_tree(emu_lib_exe, oracle_lib, case, env):
     """Device tree build + multipole, all under the emulation: the device tree equals the
     oracle's node for node, ntri equals the reference's golden vector, the sums agree to

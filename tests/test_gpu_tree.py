@@ -250,21 +250,27 @@ def test_speculative_multi_root_build(gpu, oracle_lib):
 
 @pytest.mark.gpu
 def test_speculative_build_degenerate_split(gpu, oracle_lib):
-    """A mean split one ulp from a cluster of objects: whichever side the approximate
-    centroid puts them, the tree is the reference's (checked, rebuilt if it differs)."""
+    """The root's mean split within an ulp of a cluster of objects (20 objects at x = 1
+    and the next double, the others balanced at 0.5 and 1.5): the validation by rounding
+    bounds cannot vouch for the decision, so the build waits for the exact pass (and
+    rebuilds if a decision differs); whichever side the approximate centroid puts them,
+    the tree is the reference's."""
     O = oracle_lib
     rng = np.random.default_rng(5)
     n = 300
-    x = np.repeat(rng.uniform(0, 1, 30), 10)
-    y = np.repeat(rng.uniform(0, 1e-3, 30), 10)
-    x[:10] = 1.0
-    x[10:20] = np.nextafter(1.0, 2.0)
-    y[:20] = 0.5
+    x = np.concatenate([np.full(10, 1.0), np.full(10, np.nextafter(1.0, 2.0)),
+                        np.full(140, 0.5), np.full(140, 1.5)])
+    y = rng.uniform(0, 1e-3, n)
     cat = dict(x=x, y=y, z=None, w=np.ones(n), k=None, g1=rng.normal(0, .2, n),
                g2=rng.normal(0, .2, n))
     kw = dict(min_size=0., max_size=np.inf, split_method=2, brute=False, min_top=2, max_top=10)
+    from treecorr_amd import _lib
+    d0 = _lib.lib().tc_set_tree_speculation(-2)
     with _Speculation(3):
         f, nd, ntop = device_tree(gpu, 4, 1, cat, kw)
+    # objects within the rounding bound of the split value: the build waits for the
+    # exact pass (and rebuilds if a decision differs)
+    assert _lib.lib().tc_set_tree_speculation(-2) == d0
     ref = O.Field(4, 1, x, y, None, g1=cat['g1'], g2=cat['g2'], w=cat['w'], **kw).dump()
     assert_same_tree(nd, ntop, ref)
 
@@ -277,11 +283,17 @@ def test_speculative_build_full_size_equals_exact_build(gpu):
     cat, scale = C.make_catalog(C.SPHERE, 2000000, 2026, 0.3)
     kw = dict(min_size=scale * 1e-4, max_size=scale * 0.01, split_method=2, brute=False,
               min_top=4, max_top=10)
+    from treecorr_amd import _lib
+    L = _lib.lib()
     trees = []
     for mode in (1, 0):
+        d0 = L.tc_set_tree_speculation(-2)
         with _Speculation(mode) as s:
             f, nd, ntop = device_tree(gpu, 4, 3, cat, kw)
         assert s.redo == 0
+        # mode 1: the decisions pass the rounding-bound validation, so the build returned
+        # before its exact pass (the node dump above waited for it)
+        assert L.tc_set_tree_speculation(-2) - d0 == (1 if mode == 1 else 0)
         trees.append((nd, ntop))
         del f
     (a, ta), (b, tb) = trees

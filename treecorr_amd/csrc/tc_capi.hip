@@ -125,6 +125,7 @@ struct tc_field {
     DeviceTree T;
     std::shared_ptr<SharedNodes> shared;   // set when T.nodes belongs to a multi-root build
     ~tc_field() {
+        (void)tc::settle_tree(T);    // the exact pass of the top levels writes T.nodes
         if (!shared) tc::dev_free(T.nodes, T.nodes_bytes);
         tc::dev_free(T.dtop, T.dtop_bytes);
     }
@@ -319,6 +320,14 @@ int tc_field_create_multi(int d, int coords, const double* x, const double* y, c
     return rc;
 }
 
+// The field's centre and size and its records above the top cells are complete once
+// the exact pass of the build's speculative levels has finished.
+static int settled(const tc_field* f) {
+    static std::mutex m;
+    std::lock_guard<std::mutex> g(m);
+    return tc::settle_tree(const_cast<tc_field*>(f)->T);
+}
+
 int tc_field_nobj(const tc_field* f, long long* nobj) {
     if (!f || !nobj) return tc::fail("NULL argument");
     *nobj = f->nobj;
@@ -326,6 +335,7 @@ int tc_field_nobj(const tc_field* f, long long* nobj) {
 }
 int tc_field_size(const tc_field* f, double* size) {
     if (!f || !size) return tc::fail("NULL argument");
+    if (int rc = settled(f)) return rc;
     *size = std::sqrt(f->T.info.sizesq);
     return 0;
 }
@@ -336,6 +346,7 @@ int tc_field_ntop(const tc_field* f, long long* ntop) {
 }
 int tc_field_center(const tc_field* f, double* c) {
     if (!f || !c) return tc::fail("NULL argument");
+    if (int rc = settled(f)) return rc;
     c[0] = f->T.info.cx; c[1] = f->T.info.cy; c[2] = f->T.info.cz;
     return 0;
 }
@@ -346,6 +357,7 @@ int tc_field_nnodes(const tc_field* f, long long* nn) {
 }
 int tc_field_copy_nodes(const tc_field* f, void* nodes_out, int* top_out) {
     if (!f || !nodes_out || !top_out) return tc::fail("NULL argument");
+    if (int rc = settled(f)) return rc;
     TC_HIP(hipMemcpy(nodes_out, f->T.nodes, sizeof(DNode) * f->T.nnodes, hipMemcpyDeviceToHost));
     std::memcpy(top_out, f->T.top.data(), sizeof(int) * f->T.top.size());
     return 0;
@@ -443,6 +455,7 @@ static int set_metric(const tc_corr2* c, int metric, int coords, C2Params& P) {
 
 // BaseCorr2::process(f1,f2) early exit, src/Corr2.cpp:215-231: the two fields are
 // entirely closer than minsep or farther than maxsep.
+// (the fields' centres and sizes: both fields settled)
 static bool fields_out_of_range(const C2Params& P, int coords, const tc_field* f1,
                                 const tc_field* f2) {
     const FieldInfo& a = f1->T.info;
@@ -475,7 +488,11 @@ static int corr2_common(tc_corr2* c, tc_field* f1, tc_field* f2, int quick, int 
     P.quick = quick ? 1 : 0;
     const int nb = P.nbins;
     for (int k = 0; k < 4; ++k) c->counters[k] = 0;
-    if (f2 && fields_out_of_range(P, f1->coords, f1, f2)) return 0;
+    if (f2) {
+        if (int rc = settled(f1)) return rc;
+        if (int rc = settled(f2)) return rc;
+        if (fields_out_of_range(P, f1->coords, f1, f2)) return 0;
+    }
     std::vector<double> out((size_t)7 * nb);
     std::vector<unsigned long long> np(nb), cnt(4);
     std::lock_guard<std::mutex> run_lock(g_run_m);
@@ -529,6 +546,10 @@ int tc_corr2_process_pairs(tc_corr2* c, tc_field* const* fields1, int nfield1,
     };
     if (int rc = check_side(fields1, nfield1, c->d)) return rc;
     if (!same) if (int rc = check_side(fields2, nfield2, c->d2)) return rc;
+    for (int k = 0; k < nfield1; ++k)
+        if (int rc = settled(fields1[k])) return rc;
+    for (int k = 0; !same && k < nfield2; ++k)
+        if (int rc = settled(fields2[k])) return rc;
     const int n2 = same ? nfield1 : nfield2;
     for (int p = 0; p < npair; ++p)
         if (p1[p] < 0 || p1[p] >= nfield1 || p2[p] < 0 || p2[p] >= n2)
@@ -785,6 +806,7 @@ int tc_kmeans_run_alt(const tc_field* f, double* centers, int npatch, const doub
         return tc::fail("NULL or invalid argument");
     if (f->T.top.empty()) return tc::fail("the field has no tree");
     if (int rc = check_gpu()) return rc;
+    if (int rc = settled(f)) return rc;
     std::lock_guard<std::mutex> run_lock(g_run_m);
     return tc::kmeans_run_tree(centers, npatch, dim, x, y, z, w, n, max_iter, tol, size,
                                f->coords == Sphere ? 1 : 0, f->T.nodes, f->T.nnodes, f->T.dtop,
@@ -986,6 +1008,8 @@ int corr3_passes(tc_corr3* c, tc_field* const* fs, int nfs, const Pass3* passes,
         if (fs[i]->coords != fs[0]->coords) return tc::fail("the fields' coordinate systems differ");
     }
     if (int rc = check_gpu()) return rc;
+    for (int i = 0; i < nfs; ++i)
+        if (int rc = settled(fs[i])) return rc;
     tc::C3Params P = c->P;
     P.quick = quick ? 1 : 0;
     const int ntot = P.nbins * P.nbins * (2 * P.maxn + 1);

@@ -2,6 +2,7 @@
 // translation units (tree build, 2-point traversal, 3-point multipole, C ABI).
 #pragma once
 #include "tc_common.h"
+#include <memory>
 #include <vector>
 
 namespace tc {
@@ -9,7 +10,13 @@ namespace tc {
 // A tree resident in GPU memory.
 // Device arrays come from the library's caching allocator (dev_alloc/dev_free), so
 // they carry their sizes.
+// The exact pass of a build's speculative top levels, still running on its own streams:
+// it writes the node records of those levels (above the top cells, which the
+// traversals never read) and the field's centre and size.  settle_tree waits for it.
+struct TreeCheck;
+
 struct DeviceTree {
+    std::shared_ptr<TreeCheck> pending;   // set while the exact pass runs
     DNode* nodes = nullptr;
     size_t nodes_bytes = 0;
     int nnodes = 0;
@@ -28,6 +35,7 @@ int build_tree(int D, int C, const double* x, const double* y, const double* z, 
 // Trees of several root ranges [bounds[r], bounds[r+1]) built together, sharing one
 // node array (node ids, child links and top lists are indices into it).
 struct MultiTree {
+    std::shared_ptr<TreeCheck> pending;   // single-root builds only
     DNode* nodes = nullptr;     // owned by the caller after the build (dev_free)
     size_t nodes_bytes = 0;
     int nnodes = 0;
@@ -37,9 +45,16 @@ struct MultiTree {
 };
 // Speculative centroids for the top tree levels: 0 off, 1 on for levels of large
 // segments (default), 2 on every top level with the check forced to fail, so the exact
-// rebuild runs, 3 on every top level (2 and 3 for tests).  A negative mode changes
-// nothing and returns the number of exact rebuilds so far.
+// rebuild runs, 3 on every top level (2 and 3 for tests).  Only levels above min_top
+// are speculated on.  A negative mode changes nothing: -2 returns the number of builds
+// that returned before their exact pass ended (validated decisions), any other the
+// number of exact rebuilds so far.
 int set_tree_speculation(int mode);
+// Waits for T's exact pass, if one is running, and fills T.info; an error if that pass
+// found a decision different from the validated speculative one (which the validation
+// rules out).  Everything that reads node records above the top cells or T.info calls
+// it first; the tree's owner calls it before freeing the nodes.
+int settle_tree(DeviceTree& T);
 int build_trees(int D, int C, const double* x, const double* y, const double* z, const double* d1,
                 const double* d2, const double* w, const double* wpos, long long n,
                 const std::vector<long long>& bounds, double minsize, double maxsize, int sm,

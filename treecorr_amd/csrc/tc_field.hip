@@ -52,6 +52,7 @@ constexpr int SPEC_BLOCKS = 1024;       // slices per speculative level, at most
 constexpr int SPEC_LEVELS_MAX = 8;      // speculative levels per build, at most
 constexpr long long SPEC_MIN_N = 1 << 14;   // objects per segment worth speculating on
 constexpr size_t SPEC_SNAP_MAX = size_t(16) << 30;   // bytes of snapshots per build, at most
+constexpr int SPEC_P1 = 9;              // per-slice sums: 5 centroid sums, 4 sums of |terms|
 
 struct SpecArgs {
     const double* c1;   // SPEC_FAST: approximate centroids, 4 per segment
@@ -598,40 +599,89 @@ __global__ __launch_bounds__(SPEC_NT) void k_spec_sums(const Seg* segs, Elems E,
     const long long s = blockIdx.y;
     long long a, b;
     spec_slice(segs[s], k, K, a, b);
-    double part[5] = {0., 0., 0., 0., 0.};
+    // the centroid sums and, for the bound on their rounding, the sums of |terms|
+    double part[SPEC_P1] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
     for (long long i = a + threadIdx.x; i < b; i += SPEC_NT) {
         const double wp = E.wp[i];
-        part[0] += E.x[i] * wp; part[1] += E.y[i] * wp;
-        part[2] += C == Flat ? 0. : E.z[i] * wp;
+        const double tx = E.x[i] * wp, ty = E.y[i] * wp, tz = C == Flat ? 0. : E.z[i] * wp;
+        part[0] += tx; part[1] += ty; part[2] += tz;
         part[3] += wp; part[4] += double(E.w[i]);
+        part[5] += fabs(tx); part[6] += fabs(ty); part[7] += fabs(tz); part[8] += fabs(wp);
     }
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < SPEC_P1; ++c) {
         const double v = block_sum<SPEC_NT>(part[c], red);
-        if (threadIdx.x == 0) p1[((size_t)s * K + k) * 5 + c] = v;
+        if (threadIdx.x == 0) p1[((size_t)s * K + k) * SPEC_P1 + c] = v;
     }
 }
 
+// hb: for each segment and coordinate, a bound on |centroid in the reference's order -
+// this centroid|.  Any order of summing n terms is within gamma_n sum |t| of the exact sum
+// (gamma_n = n u / (1 - n u), u = 2^-53), so the two orders' sums are within twice that;
+// the bound follows the division by the weight sum (and on the sphere the normalisation),
+// doubled, plus a few units of rounding.
 template <int C>
 __global__ __launch_bounds__(SPEC_NT) void k_spec_centroid(const Seg* segs, Elems E, int K,
-                                                           const double* p1, double* c1) {
+                                                           const double* p1, double* c1,
+                                                           double* hb) {
     __shared__ double red[SPEC_NT / 64];
     const long long s = blockIdx.x;
-    double part[5] = {0., 0., 0., 0., 0.};
+    double part[SPEC_P1] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
     for (int k = threadIdx.x; k < K; k += SPEC_NT) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) part[c] += p1[((size_t)s * K + k) * 5 + c];
+        for (int c = 0; c < SPEC_P1; ++c) part[c] += p1[((size_t)s * K + k) * SPEC_P1 + c];
     }
-    double sums[5];
+    double sums[SPEC_P1];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sums[c] = block_sum<SPEC_NT>(part[c], red);
+    for (int c = 0; c < SPEC_P1; ++c) sums[c] = block_sum<SPEC_NT>(part[c], red);
     if (threadIdx.x == 0) {
-        const long long i = segs[s].start;
+        const Seg sg = segs[s];
+        const long long i = sg.start;
         double cx = E.x[i], cy = E.y[i], cz = E.z[i];
         float wf;
         centroid_from_sums<C>(sums, cx, cy, cz, wf);
         c1[4 * s] = cx; c1[4 * s + 1] = cy; c1[4 * s + 2] = cz; c1[4 * s + 3] = double(wf);
+        const double u = 0x1p-53;
+        const double nu = double(sg.end - sg.start) * u;
+        const double g2 = nu < 0.5 ? 2. * nu / (1. - nu) : INFINITY;
+        const double ew = g2 * sums[8];
+        const double aw = fabs(sums[3]) - ew;
+        double e[3], c[3] = {cx, cy, cz};
+        double esum = 0.;
+        for (int q = 0; q < 3; ++q) {
+            const double v = sums[q] / sums[3];
+            e[q] = aw > 0. ? (g2 * sums[5 + q] + fabs(v) * ew) / aw + 4. * u * fabs(v)
+                           : INFINITY;
+            esum += e[q];
+        }
+        for (int q = 0; q < 3; ++q) {
+            double h = e[q];
+            if (C == Sphere) {
+                // c = v / |v|: |dc| <= (|dv_q| + |c_q| sum |dv|) / (|v| - sum |dv|)
+                const double vn = sqrt((sums[0] * sums[0] + sums[1] * sums[1] +
+                                        sums[2] * sums[2])) / fabs(sums[3]);
+                h = vn - esum > 0. ? (e[q] + fabs(c[q]) * esum) / (vn - esum) + 8. * u
+                                   : INFINITY;
+            }
+            hb[3 * s + q] = (C == Flat && q == 2) ? 0. : 2. * h + 1e-300;
+        }
     }
+}
+
+// Validation of a speculative level's decisions without the reference-order sums: a
+// segment's partition (a Mean split) is the reference's if no object lies within hb of
+// the speculative split value; its top-cell decision (size 0 or not) is if the
+// speculative size exceeds the centroid's bound with room to spare.  Any doubt sets
+// *flag and the build waits for the exact pass.
+__global__ void k_spec_validate_segs(const Seg* segs, long long nseg, int K, const double* p2,
+                                     const double* hb, int* flag) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    if (segs[s].end - segs[s].start <= 1) return;
+    double ssq = 0.;
+    for (int k = 0; k < K; ++k) ssq = fmax(ssq, p2[((size_t)s * K + k) * 7]);
+    const double h = hb[3 * s] + hb[3 * s + 1] + hb[3 * s + 2];
+    if (!(sqrt(ssq) > 2. * h + 1e-290)) atomicOr(flag, 2);
 }
 
 template <int C>
@@ -919,6 +969,18 @@ __global__ void k_part_flags(Elems E, const SegOut* sout, long long n, int* flag
     flags[i] = f;
 }
 
+__global__ void k_spec_validate_objs(Elems E, const SegOut* sout, long long n,
+                                     const double* hb, int* flag) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = E.seg[i];
+    if (s < 0) return;
+    const SegOut& o = sout[s];
+    if (!o.split) return;
+    const double v = elem_coord(E, i, o.dim);
+    if (fabs(v - o.value) <= hb[3 * s + o.dim]) atomicOr(flag, 1);
+}
+
 __global__ void k_part_mid(const Seg* segs, SegOut* sout, int nseg, const int* pre) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg) return;
@@ -1051,10 +1113,41 @@ static int dalloc(DevBuf& b, size_t bytes) {
     return 0;
 }
 
+// The exact pass of a build whose speculative decisions were validated: it still runs
+// on the check streams when the build returns; the buffers it reads and writes (other
+// than the node array) are kept here until it has finished.
+struct TreeCheck {
+    hipEvent_t done[2] = {nullptr, nullptr};
+    std::vector<std::pair<void*, size_t>> bufs;
+    FieldInfo* finfo = nullptr;   // device, the root's entry (in bufs)
+    int* bad = nullptr;           // device flag of the exact pass (in bufs)
+    ~TreeCheck() {
+        for (hipEvent_t& e : done) {
+            if (e) { (void)hipEventSynchronize(e); (void)hipEventDestroy(e); }
+        }
+        for (auto& b : bufs) dev_free(b.first, b.second);
+    }
+};
+
+int settle_tree(DeviceTree& T) {
+    if (!T.pending) return 0;
+    std::shared_ptr<TreeCheck> pc = std::move(T.pending);
+    for (hipEvent_t e : pc->done)
+        if (e) TC_HIP(hipEventSynchronize(e));
+    int bad = 0;
+    TC_HIP(hipMemcpy(&bad, pc->bad, sizeof(int), hipMemcpyDeviceToHost));
+    TC_HIP(hipMemcpy(&T.info, pc->finfo, sizeof(FieldInfo), hipMemcpyDeviceToHost));
+    if (bad) return fail("tree build: the exact pass disagrees with a validated speculative "
+                         "decision");
+    return 0;
+}
+
 static std::atomic<int> g_tree_spec{1};         // tc_set_tree_speculation
 static std::atomic<long long> g_tree_spec_redo{0};   // exact rebuilds after a failed check
+static std::atomic<long long> g_tree_spec_defer{0};  // builds that did not wait for the check
 
 int set_tree_speculation(int mode) {
+    if (mode == -2) return (int)std::min<long long>(g_tree_spec_defer.load(), 1 << 30);
     if (mode < 0) return (int)std::min<long long>(g_tree_spec_redo.load(), 1 << 30);
     if (mode > 3) return fail("tree speculation mode must be 0 to 3");
     g_tree_spec.store(mode);
@@ -1111,7 +1204,13 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
     const int spec_mode = g_tree_spec.load();
     const bool spec_on = spec_mode != 0 && (sm == Mean || sm == Middle);
     DevBuf bspec, bsnap[SPEC_LEVELS_MAX];
-    if (spec_on && dalloc(bspec, sizeof(double) * (4 * SEG_TOP_MAX + 12 * SPEC_BLOCKS))) return 2;
+    if (spec_on && dalloc(bspec, sizeof(double) * (7 * SEG_TOP_MAX + (SPEC_P1 + 7) * SPEC_BLOCKS)))
+        return 2;
+    // A single tree whose speculative decisions all pass the validation does not wait
+    // for their exact pass: that pass only writes records above the top cells (the
+    // levels speculated on are above min_top) and the field's centre and size.
+    const bool defer_ok = spec_on && nroot == 1 && spec_mode != 2;
+    bool deferred = false;
     struct SpecGuard {   // declared after the buffers: the exact passes end before they go
         hipStream_t cs[2] = {nullptr, nullptr};
         hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1180,8 +1279,10 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
             Seg* segs = bsegs[cur].as<Seg>();
             SegOut* sout = bsout.as<SegOut>();
             const int l0 = level == 0 ? 1 : 0;
+            // only levels above the top cells (their segments all have mintop > 0, so the
+            // top-cell decision there depends on the centroid only through size == 0)
             bool sp = spec && !done && nspec < SPEC_LEVELS_MAX && nseg <= SEG_TOP_MAX &&
-                      (spec_mode >= 2 || n / nseg >= SPEC_MIN_N);
+                      level < mintop && (spec_mode >= 2 || n / nseg >= SPEC_MIN_N);
             const size_t segb = al(sizeof(Seg) * nseg), soutb = al(sizeof(SegOut) * nseg);
             const size_t db = al(sizeof(double) * n), fb = al(sizeof(float) * n);
             if (sp && (size_t)(nspec + 1) * (4 * db + 3 * fb) > SPEC_SNAP_MAX) sp = false;
@@ -1205,11 +1306,12 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
                 if (K < 1) K = 1;
                 double* c1 = bspec.as<double>();
                 double* p1 = c1 + 4 * SEG_TOP_MAX;
-                double* p2 = p1 + 5 * SPEC_BLOCKS;
+                double* p2 = p1 + SPEC_P1 * SPEC_BLOCKS;
+                double* hb = p2 + 7 * SPEC_BLOCKS;
                 hipLaunchKernelGGL((k_spec_sums<C>), dim3(K, (unsigned)nseg), dim3(SPEC_NT), 0,
                                    st, segs, E[cur], K, p1);
                 hipLaunchKernelGGL((k_spec_centroid<C>), dim3((unsigned)nseg), dim3(SPEC_NT), 0,
-                                   st, segs, E[cur], K, p1, c1);
+                                   st, segs, E[cur], K, p1, c1, hb);
                 hipLaunchKernelGGL((k_spec_bounds<C>), dim3(K, (unsigned)nseg), dim3(SPEC_NT), 0,
                                    st, segs, E[cur], K, c1, p2);
                 const SpecArgs sa{c1, p2, K, nullptr};
@@ -1217,6 +1319,12 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
                                    dim3(512), 0, st, segs, 0LL, sout, E[cur], M.nodes,
                                    bcnt.as<int>(), btops.as<TopEntry>(), P,
                                    bfinfo.as<FieldInfo>(), l0, 0LL, sa);
+                // the decisions' validation (Middle splits take an exact value)
+                hipLaunchKernelGGL(k_spec_validate_segs, grid(nseg), dim3(TPB), 0, st, segs, nseg,
+                                   K, p2, hb, bcnt.as<int>() + 3);
+                if (sm == Mean)
+                    hipLaunchKernelGGL(k_spec_validate_objs, grid(n), dim3(TPB), 0, st, E[cur],
+                                       sout, n, hb, bcnt.as<int>() + 3);
                 char* q = static_cast<char*>(bsnap[nspec].p);
                 SpecLevel& L = SB.lv[SB.nlev++];
                 Seg* ssegs = reinterpret_cast<Seg*>(q); q += segb;
@@ -1298,12 +1406,31 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
         }
         nlevels = level;
         if (SB.nlev > 0 && launch_check()) return 2;
-        for (int k = 0; k < nlaunch; ++k) TC_HIP(hipStreamSynchronize(guard.cs[k]));
+        deferred = false;
+        if (spec && defer_ok && nlaunch > 0) {
+            int vflag = 1;
+            TC_HIP(hipMemcpyAsync(&vflag, bcnt.as<int>() + 3, sizeof(int), hipMemcpyDeviceToHost,
+                                  st));
+            TC_HIP(hipStreamSynchronize(st));
+            deferred = vflag == 0;
+        }
+        if (!deferred)
+            for (int k = 0; k < nlaunch; ++k) TC_HIP(hipStreamSynchronize(guard.cs[k]));
+        else {
+            // hand the exact pass's streams to the tree: events mark their end
+            auto pc = std::make_shared<TreeCheck>();
+            for (int k = 0; k < nlaunch; ++k) {
+                TC_HIP(hipEventCreateWithFlags(&pc->done[k], hipEventDisableTiming));
+                TC_HIP(hipEventRecord(pc->done[k], guard.cs[k]));
+            }
+            M.pending = pc;
+            g_tree_spec_defer.fetch_add(1);
+        }
         return 0;
     };
 
     int rc = attempt(spec_on);
-    if (rc == 0 && spec_on) {
+    if (rc == 0 && !deferred && spec_on) {
         int flag = 0;
         TC_HIP(hipMemcpyAsync(&flag, bcnt.as<int>() + 2, sizeof(int), hipMemcpyDeviceToHost, st));
         TC_HIP(hipStreamSynchronize(st));
@@ -1314,14 +1441,28 @@ static int build_trees_t(const double* x, const double* y, const double* z, cons
         }
     }
     int cnts[2];
-    M.info.resize(nroot);
+    M.info.assign(nroot, FieldInfo{});
     if (rc == 0) {
         TC_HIP(hipMemcpyAsync(cnts, bcnt.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-        TC_HIP(hipMemcpyAsync(M.info.data(), bfinfo.p, sizeof(FieldInfo) * nroot,
-                              hipMemcpyDeviceToHost, st));
+        if (!deferred)   // otherwise the exact pass writes it: settle_tree reads it
+            TC_HIP(hipMemcpyAsync(M.info.data(), bfinfo.p, sizeof(FieldInfo) * nroot,
+                                  hipMemcpyDeviceToHost, st));
         TC_HIP(hipStreamSynchronize(st));
     }
     if (rc) return rc;
+    if (rc == 0 && deferred) {
+        // the buffers the exact pass still reads or writes go with the tree
+        TreeCheck& pc = *M.pending;
+        for (int k = 0; k < SPEC_LEVELS_MAX; ++k)
+            if (bsnap[k].p) { pc.bufs.push_back({bsnap[k].p, bsnap[k].bytes}); bsnap[k].p = nullptr; }
+        pc.finfo = bfinfo.as<FieldInfo>();
+        pc.bufs.push_back({bfinfo.p, bfinfo.bytes});
+        bfinfo.p = nullptr;
+        pc.bad = bcnt.as<int>() + 2;
+        pc.bufs.push_back({bcnt.p, bcnt.bytes});
+        bcnt.p = nullptr;
+        guard.cs[0] = guard.cs[1] = nullptr;     // the tree waits for them, not the build
+    }
     M.nnodes = cnts[0];
     M.depth = nlevels;
     std::vector<TopEntry> tops(cnts[1]);
@@ -1364,6 +1505,7 @@ int build_tree(int D, int C, const double* x, const double* y, const double* z, 
     T.nodes = M.nodes;      // the tree owns the node array from here on
     T.nodes_bytes = M.nodes_bytes;
     M.nodes = nullptr;
+    T.pending = std::move(M.pending);
     if (rc) return rc;
     T.nnodes = M.nnodes;
     T.depth = M.depth;
