@@ -463,6 +463,18 @@ inline hipError_t hipEventRecord(hipEvent_t, hipStream_t = nullptr) { return hip
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+// HIP graphs are not emulated: a capture fails, so callers keep to eager launches
+typedef struct emu_graph* hipGraph_t;
+typedef struct emu_graph_exec* hipGraphExec_t;
+enum { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+       hipStreamCaptureModeRelaxed = 2 };
+typedef int hipStreamCaptureMode;
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorInvalidValue; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) { *g = nullptr; return hipErrorInvalidValue; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t, void*, void*, size_t) { *e = nullptr; return hipErrorInvalidValue; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorInvalidValue; }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
 // an emulated device of 8 compute units holding 2 workgroups each (EMU_CUS overrides 8)
 enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 };
