@@ -342,6 +342,15 @@ int tc_corr3_process_cross(tc_corr3* c, tc_field* f1, tc_field* f2, tc_field* f3
 /* Work counters of the last process call: Gn calls, zeta calls, zeta rows,
  * multipoleFinish stages (no reference counterpart; roofline accounting). */
 int tc_corr3_counters(const tc_corr3* c, long long* counters4);
+/* Replaces TriviallyZeroCorr3 (src/Corr3.cpp:4069-4130, BaseCorr3::triviallyZero
+ * :275-299; Euclidean metric): *result = 1 when no triangle with a vertex in each of
+ * three cells (centre x, y, z and size s; z ignored for Flat) can fall in a bin.
+ * ordered is the reference's ordered code (0 = any vertex order); p13 != 0 when
+ * cell 2 is cell 1 or cell 3 (only the 1-3 distance is tested). */
+int tc_corr3_trivially_zero(tc_corr3* c, int coords, double x1, double y1, double z1,
+                            double s1, double x2, double y2, double z2, double s2, double x3,
+                            double y3, double z3, double s3, int ordered, int p13,
+                            int* result);
 /* Zeta cells of the last process call: the (k2, k3 >= k2) entries that its zeta
  * contractions added to, summed over the contractions (the unit of the zeta work,
  * times 2*max_n+1 multipoles).  No reference counterpart. */

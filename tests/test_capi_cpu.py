@@ -134,3 +134,61 @@ def test_inverted_rpar_range_is_rejected():
     import treecorr_amd as tca
     with pytest.raises(ValueError, match='min_rpar'):
         tca.NNCorrelation(nbins=5, min_sep=1., max_sep=10., min_rpar=5., max_rpar=-5.)
+
+
+def test_corr3_trivially_zero_is_conservative():
+    """TriviallyZero for 3-point patches (src/Corr3.cpp:275-299): whenever it says no
+    triangle can land in a bin, points drawn anywhere inside the three cells give none
+    (with the vertex order fixed for ordered=1, in any order for ordered=0); it is not
+    vacuous; and with cell 2 the same as cell 3 only the 1-3 distance decides."""
+    import ctypes as ct
+    import treecorr_amd as tc
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    corr = tc.NNNCorrelation(min_sep=1., max_sep=4., nbins=4, max_n=5,
+                             bin_type='LogMultipole')
+    h = corr.corr
+    rng = np.random.default_rng(5)
+
+    def tz(cells, ordered, p13):
+        r = ct.c_int()
+        args = []
+        for (x, y, s) in cells:
+            args += [x, y, 0., s]
+        assert L.tc_corr3_trivially_zero(h, 1, *args, ordered, p13, ct.byref(r)) == 0
+        return r.value
+
+    def draw(x, y, s, n=300):
+        rr = s * np.sqrt(rng.uniform(0, 1, n))
+        t = rng.uniform(0, 2 * np.pi, n)
+        return np.stack([x + rr * np.cos(t), y + rr * np.sin(t)], axis=1)
+
+    def ok(d):
+        return (d >= 1.) & (d < 4.)
+
+    hits = {0: 0, 1: 0}
+    for trial in range(400):
+        cells = [(rng.uniform(0, 8), rng.uniform(0, 8), rng.choice([0., rng.uniform(0, 1.5)]))
+                 for _ in range(3)]
+        for ordered in (0, 1):
+            if not tz(cells, ordered, 0):
+                continue
+            hits[ordered] += 1
+            p1, p2, p3 = [draw(*c) for c in cells]
+            d1 = np.linalg.norm(p2 - p3, axis=1)
+            d2 = np.linalg.norm(p1 - p3, axis=1)
+            d3 = np.linalg.norm(p1 - p2, axis=1)
+            if ordered:
+                assert not np.any(ok(d2) & ok(d3)), cells
+            else:
+                # some vertex as the first with the two sides from it in range
+                for a, b in ((d2, d3), (d1, d3), (d1, d2)):
+                    assert not np.any(ok(a) & ok(b)), cells
+    assert hits[0] > 20 and hits[1] > hits[0]
+    # cells whose every distance is in range are never skipped
+    assert tz([(0., 0., 0.1), (2., 0., 0.1), (0., 2., 0.1)], 1, 0) == 0
+    assert tz([(0., 0., 0.1), (2., 0., 0.1), (0., 2., 0.1)], 0, 0) == 0
+    # p13: only |c1 - c3| against max_sep + s1 + s3
+    far = [(0., 0., 0.5), (0., 0., 0.5), (5.1, 0., 0.5)]
+    assert tz(far, 1, 1) == 1
+    assert tz([(0., 0., 0.5), (0., 0., 0.5), (4.9, 0., 0.5)], 1, 1) == 0

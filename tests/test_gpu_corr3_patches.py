@@ -1,0 +1,167 @@
+"""3-point correlations with patches: the multipole algorithm's local rule.
+
+Each patch i contributes the triangles whose first vertex is in the patch and whose other
+two vertices are among the objects of the (other) catalogs near it, stored as
+results[(i, i, i)] (corr3base.py:1287-1351, 1388-1421, 1533-1563, 1760-1818).  With
+brute=True every object is its own leaf, so the nearby objects give exactly the triangles
+of the whole catalog: each results[(i, i, i)] must equal the oracle's processCross12
+(ordered=1) of patch i against the whole catalog, and the totals the whole catalog's
+auto-correlation (ntri exactly, the FP64 sums to 1e-12 normwise).  Unordered cross
+correlations, which the local rule runs as several ordered passes, must give the
+unpatched result.  Each test prints its case before the device calls."""
+import numpy as np
+import pytest
+
+import cases as C
+
+RTOL = 1e-12
+FLAT, SPHERE = 1, 3
+
+
+def normwise(a, b):
+    den = np.max(np.abs(b))
+    return 0. if den == 0 else np.max(np.abs(a - b)) / den
+
+
+def labels_for(cat, coords, scale, ng):
+    if coords == SPHERE:
+        ra = np.arctan2(cat['y'], cat['x'])
+        dec = np.arcsin(cat['z'])
+        u, v = ra / scale, (dec + scale / 2) / scale
+    else:
+        u, v = cat['x'] / scale, cat['y'] / scale
+    iu = np.clip((u * ng).astype(int), 0, ng - 1)
+    iv = np.clip((v * ng).astype(int), 0, ng - 1)
+    return iu * ng + iv
+
+
+def public_catalog(tc, cat, coords, labels=None, d=0):
+    kw = dict(w=cat['w'])
+    if coords == SPHERE:
+        ra = np.arctan2(cat['y'], cat['x'])
+        dec = np.arcsin(cat['z'])
+        kw.update(ra=ra, dec=dec, ra_units='rad', dec_units='rad')
+    else:
+        kw.update(x=cat['x'], y=cat['y'])
+    if d == 4:
+        kw.update(g1=cat['g1'], g2=cat['g2'])
+    if labels is not None:
+        kw['patch'] = labels
+    return tc.Catalog(**kw)
+
+
+def multipole(tc, d, scale, **kw):
+    cls = tc.GGGCorrelation if d == 4 else tc.NNNCorrelation
+    return cls(min_sep=scale * 0.02, max_sep=scale * 0.2, nbins=6, max_n=8,
+               bin_type='LogMultipole', brute=True, **kw)
+
+
+def raw(corr):
+    out = dict(ntri=corr.ntri, weight=corr.weightr, weight_im=corr.weighti,
+               meand2=corr.meand2, meand3=corr.meand3)
+    if corr._d == 4:
+        for i in range(8):
+            out['zeta%d' % i] = corr._z[i]
+    return out
+
+
+def sub(cat, sel):
+    return {k: (None if v is None else np.ascontiguousarray(v[sel])) for k, v in cat.items()}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('d,coords', [(0, FLAT), (4, FLAT), (4, SPHERE)])
+def test_gpu_local_patches_auto(gpu, oracle_lib, d, coords):
+    O = oracle_lib
+    ng = 2
+    cat, scale = C.make_catalog(coords, 1500, 71 + d + coords, 0.2)
+    lab = labels_for(cat, coords, scale, ng)
+    print('3pt local patches auto d', d, 'coords', coords, flush=True)
+    corr = multipole(gpu, d, scale)
+    pc = public_catalog(gpu, cat, coords, lab, d)
+    if coords == SPHERE:
+        # the oracle sees the positions the catalog made from ra, dec
+        pc._generate_xyz()
+        cat = dict(cat, x=pc._x, y=pc._y, z=pc._z)
+    corr.process(pc, finalize=False)
+    assert sorted(corr.results) == [(i, i, i) for i in range(ng * ng)]
+    assert corr.npatch1 == corr.npatch2 == corr.npatch3 == ng * ng
+    # brute fields have min_size 0 (field.py:569): every object its own leaf
+    kw = dict(min_size=0., max_size=corr._get_minmax_size()[1], brute=True)
+
+    def fld(c):
+        return O.Field(d, coords, c['x'], c['y'], c['z'], g1=c['g1'], g2=c['g2'], w=c['w'],
+                       **kw)
+    whole = fld(cat)
+    args = (corr.nbins, corr._min_sep, corr._max_sep, corr._bin_size, corr.b,
+            corr.angle_slop, corr.max_n)
+    keys = ['weight', 'weight_im', 'meand2', 'meand3'] + \
+        (['zeta%d' % i for i in range(8)] if d == 4 else [])
+    for i in range(ng * ng):
+        ref = O.corr3_multipole_cross(d, coords, 'cross12', 1, [fld(sub(cat, lab == i)), whole],
+                                      *args)
+        got = raw(corr.results[(i, i, i)])
+        assert np.sum(ref['ntri']) > 0
+        np.testing.assert_array_equal(got['ntri'], ref['ntri'], err_msg='patch %d' % i)
+        for k in keys:
+            assert normwise(got[k], ref[k]) < RTOL, (i, k)
+    # the totals are the sums of the patches, and the whole catalog's auto-correlation
+    tot = sum(corr.results[(i, i, i)].ntri for i in range(ng * ng))
+    np.testing.assert_array_equal(corr.ntri, tot)
+    auto = O.corr3_multipole(d, coords, whole, *args)
+    np.testing.assert_array_equal(corr.ntri, auto['ntri'])
+    for k in keys:
+        assert normwise(raw(corr)[k], auto[k]) < RTOL, k
+    if d == 0:
+        # each patch adds 1/2 w_i W^2 (its expanded patch carries the whole weight W)
+        W = np.sum(cat['w'])
+        assert corr.tot == pytest.approx(0.5 * W ** 3, rel=1e-12)
+
+
+@pytest.mark.gpu
+def test_gpu_local_patches_sas_results(gpu):
+    """LogSAS through toSAS keeps each patch's result as raw sums (corr3base.py:
+    2626-2640, 2665-2675): their weights add up to the total weight, their zetas to the
+    total zeta times the weight, and the finalized object equals the unpatched one."""
+    cat, scale = C.make_catalog(FLAT, 1200, 91, 0.2)
+    lab = labels_for(cat, FLAT, scale, 2)
+    kw = dict(min_sep=scale * 0.02, max_sep=scale * 0.2, nbins=5, nphi_bins=8, max_n=8,
+              brute=True)
+    print('3pt local patches sas', flush=True)
+    gp = gpu.GGGCorrelation(**kw)
+    gp.process(public_catalog(gpu, cat, FLAT, lab, 4))
+    g1 = gpu.GGGCorrelation(**kw)
+    g1.process(public_catalog(gpu, cat, FLAT, None, 4))
+    assert len(gp.results) == 4
+    np.testing.assert_allclose(sum(r.weightr for r in gp.results.values()), gp.weightr,
+                               rtol=1e-11, atol=1e-11 * np.max(np.abs(gp.weightr)))
+    for name in ('gam0', 'gam1', 'gam2', 'gam3'):
+        tot = sum(getattr(r, name) for r in gp.results.values())
+        want = getattr(gp, name) * gp.weightr
+        assert normwise(tot, want) < 1e-11, name
+        assert normwise(getattr(gp, name), getattr(g1, name)) < 1e-11, name
+    np.testing.assert_allclose(gp.ntri, g1.ntri, rtol=1e-11, atol=1e-9)
+    assert gp._var_num == pytest.approx(g1._var_num, rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('kind,ordered', [('cross12', False), ('cross12', True),
+                                          ('cross', False), ('cross', 3), ('cross', True)])
+def test_gpu_local_patches_cross(gpu, kind, ordered):
+    """Patched cross-correlations (the unordered ones as extra ordered passes with another
+    catalog first) against the same correlation without patches."""
+    cats = [C.make_catalog(FLAT, 900, 120 + i, 0.2)[0] for i in range(3)]
+    scale = 100.
+    nf = 2 if kind == 'cross12' else 3
+    print('3pt local patches', kind, ordered, flush=True)
+    patched = [public_catalog(gpu, c, FLAT, labels_for(c, FLAT, scale, 2)) for c in cats[:nf]]
+    plain = [public_catalog(gpu, c, FLAT) for c in cats[:nf]]
+    cp = multipole(gpu, 0, scale)
+    cp.process(*patched, ordered=ordered, finalize=False)
+    c1 = multipole(gpu, 0, scale)
+    c1.process(*plain, ordered=ordered, finalize=False)
+    assert np.sum(c1.ntri) > 0
+    np.testing.assert_array_equal(cp.ntri, c1.ntri)
+    for k in ('weightr', 'weighti', 'meand2', 'meand3'):
+        assert normwise(getattr(cp, k), getattr(c1, k)) < RTOL, k
+    assert sorted(cp.results) == [(i, i, i) for i in range(4)]

@@ -100,6 +100,7 @@ SIGNATURES = {
     'tc_corr3_process_cross': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i]),
     'tc_corr3_counters': (_i, [_vp, ctypes.POINTER(_ll)]),
     'tc_corr3_zeta_cells': (_i, [_vp, ctypes.POINTER(_ll)]),
+    'tc_corr3_trivially_zero': (_i, [_vp, _i] + [_d] * 12 + [_i, _i, ctypes.POINTER(_i)]),
     'tc_corr3_destroy': (_i, [_vp]),
 }
 

@@ -19,6 +19,8 @@ import numpy as np
 from . import _lib
 from .catalog import calculateVarG
 from .config import parse_unit
+from .corr2 import Corr2
+from .field import coord_enum
 from .util import set_omp_threads
 
 LOGMULTIPOLE = 5
@@ -238,7 +240,8 @@ class Corr3:
         b1 = min(self.angle_slop, self.b)
         return 2 * self._min_sep * b1 / (2. + 3 * b1), self._max_sep * b1
 
-    def clear(self):
+    def _clear(self):
+        # corr3base.py:2398-2417: the accumulated arrays only
         for a in self._z:
             if a is not None:
                 a[...] = 0
@@ -246,7 +249,13 @@ class Corr3:
             a = getattr(self, nm)
             if a.size:
                 a[...] = 0
+
+    def clear(self):
+        """Clear the sums, the per-patch results and the patch counts
+        (corr3base.py:1254-1260)."""
+        self._clear()
         self.results = {}
+        self.npatch1 = self.npatch2 = self.npatch3 = 1
 
     def _field(self, cat, letter):
         min_size, max_size = self._get_minmax_size()
@@ -362,35 +371,220 @@ class Corr3:
             raise ValueError("LogMultipole binning cannot use patch_method='global'")
         if patch_method not in (None, 'local', 'global'):
             raise ValueError("Invalid patch_method %s" % patch_method)
-        for c in (cat1, cat2, cat3):
-            if c is not None and (isinstance(c, list) or c.npatch != 1):
-                raise NotImplementedError("patch-based 3-point processing is not implemented")
+        if not isinstance(cat1, list):
+            cat1 = cat1.get_patches()
+        if cat2 is not None and not isinstance(cat2, list):
+            cat2 = cat2.get_patches()
+        if cat3 is not None and not isinstance(cat3, list):
+            cat3 = cat3.get_patches()
+        single = all(c is None or (len(c) == 1 and c[0].npatch == 1) for c in (cat1, cat2, cat3))
         if initialize:
             self.clear()
             self._corr_only = corr_only
+        kw = dict(metric=metric, num_threads=num_threads, corr_only=corr_only)
         if cat2 is None:
             if cat3 is not None:
                 raise ValueError("For two catalog case, use cat1,cat2, not cat1,cat3")
             if not (ordered is True or ordered is False):
                 raise ValueError("The integer options for ordered are only valid with 3 catalogs")
-            self.process_auto(cat1, metric=metric, num_threads=num_threads, corr_only=corr_only)
+            if single:
+                self.process_auto(cat1[0], **kw)
+            else:
+                self._local_auto(cat1, kw)
         elif cat3 is None:
             if not (ordered is True or ordered is False):
                 raise ValueError("The integer options for ordered are only valid with 3 catalogs")
             if self._letters[1] == self._letters[2]:
-                self.process_cross12(cat1, cat2, metric=metric, ordered=ordered,
-                                     num_threads=num_threads, corr_only=corr_only)
+                if single:
+                    self.process_cross12(cat1[0], cat2[0], ordered=ordered, **kw)
+                else:
+                    self._local_cross12(cat1, cat2, ordered, kw)
+            elif single:
+                self.process_cross21(cat1[0], cat2[0], ordered=ordered, **kw)
             else:
-                self.process_cross21(cat1, cat2, metric=metric, ordered=ordered,
-                                     num_threads=num_threads, corr_only=corr_only)
+                self._local_cross21(cat1, cat2, ordered, kw)
+        elif single:
+            self.process_cross(cat1[0], cat2[0], cat3[0], ordered=ordered, **kw)
         else:
-            self.process_cross(cat1, cat2, cat3, metric=metric, ordered=ordered,
-                               num_threads=num_threads, corr_only=corr_only)
+            self._local_cross(cat1, cat2, cat3, ordered, kw)
         if finalize:
             self._do_finalize(cat1, cat2, cat3, corr_only)
 
+    # ---- patches: the local rule of the multipole algorithm ----
+    # Each patch's triangles with their first vertex in the patch and the other two among
+    # the nearby objects of the other catalogs (the expanded patch), stored as results[(i,
+    # i, i)] and added to the totals (corr3base.py:1287-1351, 1388-1421, 1533-1563,
+    # 1660-1680, 1760-1818).
+
+    _make_expanded_patch = Corr2._make_expanded_patch
+
+    def copy(self):
+        """A copy with its own arrays (and its own engine object, made on use);
+        corr3base.py:1219-1233."""
+        ret = self.__class__.__new__(self.__class__)
+        for key, item in self.__dict__.items():
+            ret.__dict__[key] = item.copy() if isinstance(item, np.ndarray) else item
+        ret._z = [None if a is None else a.copy() for a in self._z]
+        ret.results = dict(self.results)
+        ret._corr = None
+        return ret
+
+    @property
+    def nonzero(self):
+        """Whether any weight is accumulated (corr3base.py:1263-1266)."""
+        return bool(np.any(self.weight))
+
+    def _sums(self):
+        """The accumulated arrays (the zetas of the class and the _ARRAYS in use)."""
+        return [a for a in self._z if a is not None] + \
+            [getattr(self, nm) for nm in _ARRAYS if getattr(self, nm).size]
+
+    def _same_binning(self, other):
+        return (self.bin_type == other.bin_type and self.nbins == other.nbins and
+                self.min_sep == other.min_sep and self.max_sep == other.max_sep and
+                self.data_shape == other.data_shape and self.sep_units == other.sep_units)
+
+    def __iadd__(self, other):
+        # corr3base.py:2419-2449
+        if not isinstance(other, self.__class__):
+            raise TypeError("Can only add another %s object" % type(self).__name__)
+        if not self._same_binning(other):
+            raise ValueError("%s to be added is not compatible with this one."
+                             % type(self).__name__)
+        if other.coords is not None:
+            if self.coords is not None and self.coords != other.coords:
+                raise ValueError("Cannot correlate catalogs with different coordinate systems.")
+            self.coords, self.metric = other.coords, other.metric
+        if not other.nonzero:
+            return self
+        for a, b in zip(self._sums(), other._sums()):
+            a += b
+        return self
+
+    def _trivially_zero(self, c1, c2, c3, ordered):
+        # corr3base.py:1277-1283
+        x1, y1, z1, s1 = c1._get_center_size()
+        x2, y2, z2, s2 = c2._get_center_size()
+        x3, y3, z3, s3 = c3._get_center_size()
+        r = ctypes.c_int()
+        _lib.check(_lib.lib().tc_corr3_trivially_zero(
+            self.corr, coord_enum(self.coords), x1, y1, z1, s1, x2, y2, z2, s2, x3, y3, z3, s3,
+            int(ordered), int(c2 is c3 or c1 is c2), ctypes.byref(r)))
+        return bool(r.value)
+
+    def _single_process(self, c1, c2, c3, ijk, ordered, temp, kw):
+        # corr3base.py:1287-1351 with force_write, as every local call has it: one vertex
+        # from c1 and two from c2 (c2 is c3) or one each from c2 and c3
+        temp._clear()
+        if c2 is not None and c3 is not None and not self._trivially_zero(c1, c2, c3, ordered):
+            if c2 is c3:
+                temp.process_cross12(c1, c2, ordered=ordered, **kw)
+            else:
+                temp.process_cross(c1, c2, c3, ordered=ordered, **kw)
+        if ijk in self.results and self.results[ijk].nonzero:
+            self.results[ijk] += temp
+        else:
+            self.results[ijk] = temp.copy()
+        self += temp
+
+    @staticmethod
+    def _npatch_of(cats):
+        return cats[0].npatch if cats[0].npatch != 1 else len(cats)
+
+    @staticmethod
+    def _patch_index(c, ii):
+        return c._single_patch if c._single_patch is not None else ii
+
+    def _local_temp(self, coords, metric):
+        self._set_metric(metric, *coords)
+        temp = self.copy()
+        temp.results = {}
+        return temp
+
+    def _local_auto(self, cat1, kw):
+        # corr3base.py:1392-1421
+        if self.npatch1 == 1:
+            self.npatch1 = self.npatch2 = self.npatch3 = self._npatch_of(cat1)
+        temp = self._local_temp([cat1[0].coords], kw['metric'])
+        for ii, c1 in enumerate(cat1):
+            i = self._patch_index(c1, ii)
+            c1e = self._make_expanded_patch(c1, cat1)
+            self._single_process(c1, c1e, c1e, (i, i, i), 1, temp, kw)
+
+    def _local_cross12(self, cat1, cat2, ordered, kw):
+        # corr3base.py:1507-1563
+        if self.npatch1 == 1:
+            self.npatch1 = self._npatch_of(cat1)
+        if self.npatch2 == 1:
+            self.npatch2 = self.npatch3 = self._npatch_of(cat2)
+        if self.npatch1 != self.npatch2 and self.npatch1 != 1 and self.npatch2 != 1:
+            raise RuntimeError("Cross correlation requires both catalogs use the same patches.")
+        temp = self._local_temp([cat1[0].coords, cat2[0].coords], kw['metric'])
+        for ii, c1 in enumerate(cat1):
+            i = self._patch_index(c1, ii)
+            c2e = self._make_expanded_patch(c1, cat2)
+            self._single_process(c1, c2e, c2e, (i, i, i), 1, temp, kw)
+        if not ordered and self._letters[0] == self._letters[1]:
+            # the local rule runs with ordered=1 only: the triangles with the first vertex
+            # from cat2 are a second pass
+            for ii, c2 in enumerate(cat2):
+                i = self._patch_index(c2, ii)
+                c1e = self._make_expanded_patch(c2, cat1)
+                c2e = self._make_expanded_patch(c2, cat2)
+                self._single_process(c2, c1e, c2e, (i, i, i), 1, temp, kw)
+
+    def _local_cross21(self, cat1, cat2, ordered, kw):
+        # corr3base.py:1634-1680
+        if self.npatch1 == 1:
+            self.npatch1 = self.npatch2 = self._npatch_of(cat1)
+        if self.npatch3 == 1:
+            self.npatch3 = self._npatch_of(cat2)
+        if self.npatch1 != self.npatch3 and self.npatch1 != 1 and self.npatch3 != 1:
+            raise RuntimeError("Cross correlation requires both catalogs use the same patches.")
+        temp = self._local_temp([cat1[0].coords, cat2[0].coords], kw['metric'])
+        for ii, c1 in enumerate(cat1):
+            i = self._patch_index(c1, ii)
+            c2e = self._make_expanded_patch(c1, cat1)
+            c3e = self._make_expanded_patch(c1, cat2)
+            self._single_process(c1, c2e, c3e, (i, i, i), 1 if not ordered else 4, temp, kw)
+
+    def _local_cross(self, cat1, cat2, cat3, ordered, kw):
+        # corr3base.py:1738-1818
+        for k, cats in enumerate((cat1, cat2, cat3)):
+            nm = 'npatch%d' % (k + 1)
+            if getattr(self, nm) == 1:
+                setattr(self, nm, self._npatch_of(cats))
+        n1, n2, n3 = self.npatch1, self.npatch2, self.npatch3
+        if (n1 != n2 and n1 != 1 and n2 != 1) or (n1 != n3 and n1 != 1 and n3 != 1):
+            raise RuntimeError("Cross correlation requires all catalogs use the same patches.")
+        temp = self._local_temp([cat1[0].coords, cat2[0].coords, cat3[0].coords], kw['metric'])
+        ordered = 0 if ordered is False else 4 if ordered is True else ordered
+        for ii, c1 in enumerate(cat1):
+            i = self._patch_index(c1, ii)
+            c2e = self._make_expanded_patch(c1, cat2)
+            c3e = self._make_expanded_patch(c1, cat3)
+            self._single_process(c1, c2e, c3e, (i, i, i), 1 if ordered in (0, 1) else 4, temp,
+                                 kw)
+        # the local rule runs with ordered 1 or 4 only: the other first vertices are passes
+        # of their own
+        o = 1 if ordered == 0 else 4
+        if ordered in (0, 3) and self._letters[0] == self._letters[1]:
+            for ii, c2 in enumerate(cat2):
+                i = self._patch_index(c2, ii)
+                c1e = self._make_expanded_patch(c2, cat1)
+                c3e = self._make_expanded_patch(c2, cat3)
+                self._single_process(c2, c1e, c3e, (i, i, i), o, temp, kw)
+        if ordered in (0, 2) and self._letters[0] == self._letters[2]:
+            for ii, c3 in enumerate(cat3):
+                i = self._patch_index(c3, ii)
+                c1e = self._make_expanded_patch(c3, cat1)
+                c2e = self._make_expanded_patch(c3, cat2)
+                self._single_process(c3, c2e, c1e, (i, i, i), o, temp, kw)
+
     def _meanw(self, cat):
-        return cat.sumw / cat.nobj
+        # catalog.py calculateMeanW: the weight per object over all patches
+        cats = cat if isinstance(cat, list) else [cat]
+        return np.sum([c.sumw for c in cats]) / np.sum([c.nobj for c in cats])
 
     def _do_finalize(self, cat1, cat2, cat3, corr_only):
         # corr3base.py:2252-2310
@@ -525,6 +719,24 @@ class Corr3:
         denom[denom == 0] = 1
         ratio = self.ntri[:, :, 0] / denom * phi_frac
         sas.ntri[:] = sas.weight * ratio[:, :, None]
+        # per-patch results: raw sums (the mean distances times the result's weight), the
+        # weights of all results in one product
+        keys = list(self.results)
+        sas.results = {}
+        if keys:
+            Wk = np.concatenate([np.concatenate([self.results[k].weightr.reshape(nb * nb, nn),
+                                                 self.results[k].weighti.reshape(nb * nb, nn)],
+                                                axis=1) for k in keys], axis=0)
+            Rw = _gemm(Wk, np.concatenate([er, -ei], axis=0)) * scale
+            for j, k in enumerate(keys):
+                temp = sas.copy()
+                temp.results = {}
+                temp.weightr[:] = Rw[j * nb * nb:(j + 1) * nb * nb].reshape(nb, nb, nphi)
+                temp.ntri[:] = temp.weight * ratio[:, :, None]
+                for nm in ('meand1', 'meand2', 'meand3', 'meanlogd1', 'meanlogd2', 'meanlogd3',
+                           'meanu'):
+                    getattr(temp, nm)[:] *= temp.weightr
+                sas.results[k] = temp
         zs = self._zetas
         if not zs:
             return self._to_sas_extra(sas)
@@ -545,6 +757,23 @@ class Corr3:
             sas._z[2 * i][:] = np.real(z)
             if sas._z[2 * i + 1] is not None:
                 sas._z[2 * i + 1][:] = np.imag(z)
+        if keys:
+            # the results keep raw sums: rotated, not divided by the weight
+            Zk = np.concatenate([np.concatenate([z.real.reshape(nb * nb, nn),
+                                                 z.imag.reshape(nb * nb, nn)], axis=1)
+                                 for k in keys for z in self.results[k]._zetas], axis=0)
+            Rk = _gemm(Zk, E) * scale
+            nz = len(zs)
+            for j, k in enumerate(keys):
+                temp = sas.results[k]
+                for i in range(nz):
+                    blk = Rk[(j * nz + i) * nb * nb:(j * nz + i + 1) * nb * nb]
+                    z = (blk[:, :nphi] + 1j * blk[:, nphi:]).reshape(nb, nb, nphi)
+                    if phases[i] is not None:
+                        z = z * phases[i]
+                    temp._z[2 * i][:] = np.real(z)
+                    if temp._z[2 * i + 1] is not None:
+                        temp._z[2 * i + 1][:] = np.imag(z)
         return self._to_sas_extra(sas)
 
     def _to_sas_extra(self, sas):
@@ -560,8 +789,8 @@ class NNNCorrelation(Corr3):
         super().__init__(config, logger=logger, **kwargs)
         self.tot = 0.
 
-    def clear(self):
-        super().clear()
+    def _clear(self):
+        super()._clear()
         self.tot = 0.
 
     def process_auto(self, cat, **kw):
@@ -581,7 +810,15 @@ class NNNCorrelation(Corr3):
 
     def _to_sas_extra(self, sas):
         sas.tot = self.tot
+        for k, v in self.results.items():
+            sas.results[k].tot = v.tot
         return sas
+
+    def __iadd__(self, other):
+        # nnncorrelation.py:184-193
+        super().__iadd__(other)
+        self.tot += other.tot
+        return self
 
 
 class GGGCorrelation(Corr3):

@@ -1099,6 +1099,62 @@ int tc_corr3_counters(const tc_corr3* c, long long* counters4) {
     return 0;
 }
 
+// TriviallyZero3 → BaseCorr3::triviallyZero, src/Corr3.cpp:275-299, 4069-4130, for the
+// Euclidean metric: whether no triangle with one vertex in each of three cells (centre,
+// size) can land in a LogMultipole bin.  p13: cell 2 is cell 1 or cell 3, so only the
+// distance from cell 1 to cell 3 counts (BinTypeHelper::tooLargeDist; the Euclidean
+// metric's own test is always true).  Otherwise trivial_stop, include/BinType.h (the
+// LogMultipole helper): ordered, any of d2, d3 out of range stops; unordered, two of
+// d1, d2, d3 below min_sep or two above max_sep.
+int tc_corr3_trivially_zero(tc_corr3* c, int coords, double x1, double y1, double z1,
+                            double s1, double x2, double y2, double z2, double s2, double x3,
+                            double y3, double z3, double s3, int ordered, int p13,
+                            int* result) {
+    if (!c || !result) return tc::fail("NULL argument");
+    if (coords != Flat && coords != ThreeD && coords != Sphere)
+        return tc::fail("coords must be 1 (Flat), 2 (ThreeD) or 3 (Sphere)");
+    const C3Params& P = c->P;
+    if (coords == Flat) { z1 = z2 = z3 = 0.; }
+    if (coords == Sphere) {   // Position<Sphere> puts its point on the unit sphere
+        auto norm = [](double& x, double& y, double& z) {
+            double n = std::sqrt(x * x + y * y + z * z);
+            if (n == 0.) { x = 1; y = 0; z = 0; return; }
+            double r = 1. / n; x *= r; y *= r; z *= r;
+        };
+        norm(x1, y1, z1); norm(x2, y2, z2); norm(x3, y3, z3);
+    }
+    auto dsq = [](double ax, double ay, double az, double bx, double by, double bz) {
+        double dx = ax - bx, dy = ay - by, dz = az - bz;
+        return dx * dx + dy * dy + dz * dz;
+    };
+    auto sq = [](double v) { return v * v; };
+    if (p13) {
+        double d = dsq(x1, y1, z1, x3, y3, z3), s = s1 + s3;
+        *result = (d >= P.maxsepsq && d >= sq(P.maxsep + s)) ? 1 : 0;
+        return 0;
+    }
+    const double d1sq = dsq(x2, y2, z2, x3, y3, z3);
+    const double d2sq = dsq(x1, y1, z1, x3, y3, z3);
+    const double d3sq = dsq(x1, y1, z1, x2, y2, z2);
+    const double mn = P.minsep, mnsq = P.minsepsq, mx = P.maxsep, mxsq = P.maxsepsq;
+    auto small = [&](double d, double s) {
+        return d < mnsq && s < mn && (s == 0. || d < sq(mn - s));
+    };
+    auto large = [&](double d, double s) { return d >= mxsq && (s == 0. || d >= sq(mx + s)); };
+    *result = 1;
+    int nsmall = 0, nlarge = 0;
+    if (small(d2sq, s1 + s3)) { if (ordered) return 0; ++nsmall; }
+    if (small(d3sq, s1 + s2)) { if (ordered) return 0; ++nsmall; }
+    if (!ordered && small(d1sq, s2 + s3)) ++nsmall;
+    if (!ordered && nsmall >= 2) return 0;
+    if (large(d2sq, s1 + s3)) { if (ordered) return 0; ++nlarge; }
+    if (large(d3sq, s1 + s2)) { if (ordered) return 0; ++nlarge; }
+    if (!ordered && large(d1sq, s2 + s3)) ++nlarge;
+    if (!ordered && nlarge >= 2) return 0;
+    *result = 0;
+    return 0;
+}
+
 int tc_corr3_zeta_cells(const tc_corr3* c, long long* cells) {
     if (!c || !cells) return tc::fail("NULL argument");
     *cells = c->counters[4];
