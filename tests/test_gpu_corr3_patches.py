@@ -165,3 +165,105 @@ def test_gpu_local_patches_cross(gpu, kind, ordered):
     for k in ('weightr', 'weighti', 'meand2', 'meand3'):
         assert normwise(getattr(cp, k), getattr(c1, k)) < RTOL, k
     assert sorted(cp.results) == [(i, i, i) for i in range(4)]
+
+
+def _ggg_stat(r):
+    return np.concatenate([(r['zeta%d' % (2 * i)] + 1j * r['zeta%d' % (2 * i + 1)]).ravel()
+                           for i in range(4)])
+
+
+def _gram(v, w=None):
+    vc = v - np.mean(v, axis=0)
+    return (vc.conj().T * (1. if w is None else w)).dot(vc)
+
+
+@pytest.mark.gpu
+def test_gpu_3pt_covariances_from_oracle_rows(gpu, oracle_lib):
+    """Covariances of a LogMultipole GGG with patches (corr3base.py:2678-2790, the triple
+    lists :2997-3330).  With brute=True, jackknife row m ('simple') holds the triangles
+    whose first vertex is outside patch m: the oracle's processCross12 (ordered=1) of the
+    catalog without patch m against the whole catalog; a 'sample' row is patch m's own
+    (the oracle's cross12 of the patch against the catalog).  The estimates must be the
+    textbook sums over those independently computed rows (corr2base.py:2423-2538); the
+    marked and plain bootstrap rows of local results are the same draws' sums, so equal
+    for one seed; shot is var_num / |weight|."""
+    O = oracle_lib
+    ng, n = 2, 4
+    cat, scale = C.make_catalog(FLAT, 1400, 301, 0.2)
+    lab = labels_for(cat, FLAT, scale, ng)
+    print('3pt covariances: jackknife, sample, bootstrap', flush=True)
+    g = gpu.GGGCorrelation(min_sep=scale * 0.02, max_sep=scale * 0.2, nbins=4, max_n=4,
+                           bin_type='LogMultipole', brute=True, rng=np.random.default_rng(8))
+    g.process(public_catalog(gpu, cat, FLAT, lab, 4))
+    kw = dict(min_size=0., max_size=g._get_minmax_size()[1], brute=True)
+
+    def fld(c):
+        return O.Field(4, FLAT, c['x'], c['y'], c['z'], g1=c['g1'], g2=c['g2'], w=c['w'], **kw)
+    whole = fld(cat)
+    args = (g.nbins, g._min_sep, g._max_sep, g._bin_size, g.b, g.angle_slop, g.max_n)
+    jk, own, wown = [], [], []
+    for m in range(n):
+        r = O.corr3_multipole_cross(4, FLAT, 'cross12', 1, [fld(sub(cat, lab != m)), whole],
+                                    *args)
+        jk.append(_ggg_stat(r))
+        r = O.corr3_multipole_cross(4, FLAT, 'cross12', 1, [fld(sub(cat, lab == m)), whole],
+                                    *args)
+        own.append(_ggg_stat(r))
+        wown.append(4 * np.sum(np.abs(r['weight'] + 1j * r['weight_im'])))
+    jk, own, wown = np.array(jk), np.array(own), np.array(wown)
+    want = (1 - 1 / n) * _gram(jk)
+    got = g.estimate_cov('jackknife', cross_patch_weight='simple')
+    assert normwise(got, want) < 1e-9
+    w = wown / np.sum(wown)
+    want = _gram(own, w) / (n - 1)
+    got = g.estimate_cov('sample')
+    assert normwise(got, want) < 1e-9
+    # the diagonal is real and the matrix Hermitian
+    np.testing.assert_allclose(got, got.conj().T, rtol=0, atol=1e-12 * np.max(np.abs(got)))
+    # bootstrap draws: rng.choice(n, n) per row; local results make both kinds equal
+    g._rng = np.random.default_rng(21)
+    cb = g.estimate_cov('bootstrap', num_bootstrap=40)
+    draws = np.random.default_rng(21)
+    rows = []
+    for _ in range(40):
+        idx = draws.choice(np.arange(n), size=n, replace=True)
+        rows.append(np.sum([own[i] for i in idx], axis=0))
+    want = _gram(np.array(rows)) / 39
+    assert normwise(cb, want) < 1e-9
+    g._rng = np.random.default_rng(21)
+    cm = g.estimate_cov('marked_bootstrap', num_bootstrap=40)
+    assert normwise(cm, cb) < 1e-12
+    shot = g.estimate_cov('shot')
+    wt = g.getWeight()
+    np.testing.assert_allclose(shot[wt != 0], g._var_num / wt[wt != 0], rtol=1e-14)
+    np.testing.assert_allclose(g.vargam0.ravel(), shot[:wt.size // 4].real, rtol=1e-14)
+
+
+@pytest.mark.gpu
+def test_gpu_3pt_logsas_jackknife_rows(gpu):
+    """LogSAS GGG with patches: jackknife rows built on the GPU (R = W S, then the Gram
+    product) against numpy sums of the stored patch results, normalised by their summed
+    weight (corr3base.py:2312-2320, 2451-2481); with cross_patch_weight 'mean' a row keeps
+    every triple with weight (number of its patches other than m) / 3."""
+    cat, scale = C.make_catalog(FLAT, 1200, 302, 0.2)
+    lab = labels_for(cat, FLAT, scale, 2)
+    print('3pt logsas jackknife', flush=True)
+    g = gpu.GGGCorrelation(min_sep=scale * 0.02, max_sep=scale * 0.2, nbins=4, nphi_bins=6,
+                           max_n=6, brute=True)
+    g.process(public_catalog(gpu, cat, FLAT, lab, 4))
+    res = [g.results[(i, i, i)] for i in range(4)]
+    for cpw in ('simple', 'mean'):
+        rows = []
+        for m in range(4):
+            keep = [r for i, r in enumerate(res) if i != m]
+            W = np.sum([r.weightr for r in keep], axis=0)
+            mask = W != 0
+            stat = []
+            for name in ('gam0', 'gam1', 'gam2', 'gam3'):
+                z = np.sum([getattr(r, name) for r in keep], axis=0)
+                z[mask] /= W[mask]
+                stat.append(z.ravel())
+            rows.append(np.concatenate(stat))
+        want = 0.75 * _gram(np.array(rows))
+        got = g.estimate_cov('jackknife', cross_patch_weight=cpw)
+        assert normwise(got, want) < 1e-9, cpw

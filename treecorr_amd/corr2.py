@@ -68,6 +68,14 @@ class _PairSums:
 
     _nonzero = nonzero
 
+    def _row_arrays(self, corr_only):
+        """The arrays Corr2._sum adds, in a fixed order (the columns of a design matrix's
+        pair sums)."""
+        arrs = [a for a in self._xi if a is not None] + [self.weight]
+        if not corr_only:
+            arrs += [self.meanr, self.meanlogr, self.npairs]
+        return arrs
+
     def __iadd__(self, other):
         for a, b in zip(self._xi, other._xi):
             if a is not None:
@@ -682,6 +690,8 @@ class Corr2:
             np.sum([x(c.npairs, w) for c, w in others], axis=0, out=self.npairs)
         self._varxi = None
         self._cov = None
+
+    _row_arrays = _PairSums._row_arrays
 
     def _get_npatch(self):
         return max(self.npatch1, self.npatch2)

@@ -17,6 +17,7 @@ import math
 import numpy as np
 
 from . import _lib
+from . import cov as _cov
 from .catalog import calculateVarG
 from .config import parse_unit
 from .corr2 import Corr2
@@ -123,6 +124,13 @@ class Corr3:
         self.bv = 0.
         self.brute = bool(cfg.get('brute', False))
         self.var_method = cfg.get('var_method', 'shot')
+        self.num_bootstrap = int(cfg.get('num_bootstrap', 500))
+        self._rng = rng
+        self.cross_patch_weight = cfg.get('cross_patch_weight')
+        if self.cross_patch_weight not in (None, 'simple', 'mean', 'match', 'geom'):
+            raise ValueError("Invalid cross_patch_weight %s" % self.cross_patch_weight)
+        self._cov = None
+        self._varzeta = None
         logr1d = np.linspace(0, nbins * bin_size, nbins, endpoint=False, dtype=float)
         logr1d += math.log(min_sep) + 0.5 * bin_size
         self.logr1d = logr1d
@@ -242,6 +250,8 @@ class Corr3:
 
     def _clear(self):
         # corr3base.py:2398-2417: the accumulated arrays only
+        self._cov = None
+        self._varzeta = None
         for a in self._z:
             if a is not None:
                 a[...] = 0
@@ -604,37 +614,75 @@ class Corr3:
         self.finalize()
 
     def _apply_units(self, mask):
+        # corr3base.py:2896-2929
+        sas = self.bin_type == 'LogSAS'
         if self.coords == 'spherical' and self.metric == 'Euclidean':
+            if sas:
+                # cos of the great-circle angle from the chord triangle's
+                cosphi = np.cos(self.meanu[mask])
+                cosphi -= 0.25 * self.meand2[mask] * self.meand3[mask]
+                cosphi /= np.sqrt((1 - self.meand2[mask] ** 2 / 4) *
+                                  (1 - self.meand3[mask] ** 2 / 4))
+                self.meanu[mask] = np.arccos(np.clip(cosphi, -1, 1))
+                self.meand1[mask] = 2. * np.arcsin(self.meand1[mask] / 2.)
+                self.meanlogd1[mask] = np.log(2. * np.arcsin(np.exp(self.meanlogd1[mask]) / 2.))
             self.meand2[mask] = 2. * np.arcsin(self.meand2[mask] / 2.)
             self.meanlogd2[mask] = np.log(2. * np.arcsin(np.exp(self.meanlogd2[mask]) / 2.))
             self.meand3[mask] = 2. * np.arcsin(self.meand3[mask] / 2.)
             self.meanlogd3[mask] = np.log(2. * np.arcsin(np.exp(self.meanlogd3[mask]) / 2.))
+        if sas:
+            self.meanu[mask] /= self._phi_units
+            self.meand1[mask] /= self._sep_units
+            self.meanlogd1[mask] -= self._log_sep_units
         self.meand2[mask] /= self._sep_units
         self.meanlogd2[mask] -= self._log_sep_units
         self.meand3[mask] /= self._sep_units
         self.meanlogd3[mask] -= self._log_sep_units
 
     def _finalize(self):
-        # corr3base.py:2312-2396 (LogMultipole branches)
+        # corr3base.py:2312-2396: LogMultipole keeps its zetas unnormalised and broadcasts
+        # the distances from n = 0; LogSAS (the sums of patch results) divides everything
+        # by the weight
         mask1 = self.weightr != 0
+        sas = self.bin_type == 'LogSAS'
+        if sas:
+            for a in self._z:
+                if a is not None:
+                    a[mask1] /= self.weightr[mask1]
         if self._corr_only:
             self.meand2[:] = self.d2nom
             self.meanlogd2[:] = self.logd2
             self.meand3[:] = self.d3nom
             self.meanlogd3[:] = self.logd3
-            self.meanu[:] = 0
-            self.meand1[:] = 0
-            self.meanlogd1[:] = 0
+            if sas:
+                self.meanu[:] = self.phi
+                self.meand1[:] = np.sqrt(self.d2nom ** 2 + self.d3nom ** 2
+                                         - 2 * self.d2nom * self.d3nom * np.cos(self.phi))
+                self.meanlogd1[:] = np.log(self.meand1)
+            else:
+                self.meanu[:] = 0
+                self.meand1[:] = 0
+                self.meanlogd1[:] = 0
             self.ntri[:] = self.weightr / self._meanwww
+            return
+        names = ('meand2', 'meanlogd2', 'meand3', 'meanlogd3')
+        if sas:
+            names += ('meand1', 'meanlogd1', 'meanu')
+        for nm in names:
+            getattr(self, nm)[mask1] /= self.weightr[mask1]
+        self._apply_units(mask1)
+        mask2 = self.weightr == 0
+        self.meand2[mask2] = self.d2nom[mask2]
+        self.meanlogd2[mask2] = self.logd2[mask2]
+        self.meand3[mask2] = self.d3nom[mask2]
+        self.meanlogd3[mask2] = self.logd3[mask2]
+        if sas:
+            self.meanu[mask2] = self.phi[mask2]
+            self.meand1[mask2] = np.sqrt(self.d2nom[mask2] ** 2 + self.d3nom[mask2] ** 2
+                                         - 2 * self.d2nom[mask2] * self.d3nom[mask2] *
+                                         np.cos(self.phi[mask2]))
+            self.meanlogd1[mask2] = np.log(self.meand1[mask2])
         else:
-            for nm in ('meand2', 'meanlogd2', 'meand3', 'meanlogd3'):
-                getattr(self, nm)[mask1] /= self.weightr[mask1]
-            self._apply_units(mask1)
-            mask2 = self.weightr == 0
-            self.meand2[mask2] = self.d2nom[mask2]
-            self.meanlogd2[mask2] = self.logd2[mask2]
-            self.meand3[mask2] = self.d3nom[mask2]
-            self.meanlogd3[mask2] = self.logd3[mask2]
             self.meanu[mask2] = 0
             self.meand1[mask2] = 0
             self.meanlogd1[mask2] = 0
@@ -642,6 +690,205 @@ class Corr3:
             for nm in ('ntri', 'meand2', 'meanlogd2', 'meand3', 'meanlogd3'):
                 a = getattr(self, nm)
                 a[:, :, :] = a[:, :, M][:, :, None]
+
+    # ---- covariances (corr3base.py:1085-1110, 2451-2487, 2678-2850, 2966-3350) ----
+    # With the multipole algorithm every patch result is a local one, (i, i, i), and all
+    # catalogs carry the same patches, so the triple lists below are the reference's for
+    # npatch1 == npatch2 == npatch3.
+
+    @property
+    def rng(self):
+        if self._rng is None:
+            self._rng = np.random.default_rng()
+        return self._rng
+
+    def _row_arrays(self, corr_only):
+        """The arrays _sum adds, in a fixed order (corr3base.py:2451-2481)."""
+        arrs = [a for a in self._z if a is not None] + [self.weightr]
+        if self.weighti.size:
+            arrs.append(self.weighti)
+        if not corr_only:
+            arrs += [self.meand1, self.meanlogd1, self.meand2, self.meanlogd2, self.meand3,
+                     self.meanlogd3, self.meanu, self.ntri]
+        return arrs
+
+    def _sum(self, others, corr_only):
+        # corr3base.py:2451-2481: self = sum of w * other
+        if _cov._SUM_HOOK is not None:      # a design matrix's rows, formed on the GPU
+            _cov._SUM_HOOK(self, others, corr_only)
+        else:
+            cols = [c._row_arrays(corr_only) for c, _ in others]
+            for q, a in enumerate(self._row_arrays(corr_only)):
+                np.sum([b[q] if w == 1 else b[q] * w for b, (_, w) in zip(cols, others)],
+                       axis=0, out=a)
+        self._cov = None
+        self._varzeta = None
+
+    def _get_npatch(self):
+        return max(self.npatch1, self.npatch2, self.npatch3)
+
+    def _calculate_xi_from_pairs(self, pairs, corr_only):
+        # corr3base.py:2969-2975 ("pairs" are triples (i, j, k, w))
+        self._sum([(self.results[(i, j, k)], w) for i, j, k, w in pairs], corr_only)
+        self._finalize()
+
+    @property
+    def _ok(self):
+        ok = np.zeros((self.npatch1, self.npatch2, self.npatch3), dtype=bool)
+        for (i, j, k) in self.results:
+            ok[i, j, k] = True
+        return ok
+
+    def _keep_ok(self, pairs):
+        ok = self._ok
+        return [(i, j, k, w) for i, j, k, w in pairs if ok[i, j, k] and w != 0]
+
+    def _check_cpw(self, method, cross_patch_weight):
+        # corr3base.py:3332-3349 (the default's warning is not repeated here)
+        if cross_patch_weight is None:
+            cross_patch_weight = self.cross_patch_weight
+        return 'simple' if cross_patch_weight is None else cross_patch_weight
+
+    def _patch_keys(self):
+        n = self._get_npatch()
+        if not (self.npatch1 == self.npatch2 == self.npatch3 == n):
+            raise NotImplementedError("3-point covariances need the same patches in every "
+                                      "catalog")
+        return sorted(self.results)
+
+    def _jackknife_pairs(self, cross_patch_weight):
+        # corr3base.py:2997-3097 (all three patch indices in play)
+        cpw = cross_patch_weight
+        if cpw not in ('simple', 'mean', 'match'):
+            raise ValueError(f"cross_patch_weight = {cpw} is invalid for jackknife")
+        keys = self._patch_keys()
+        n = self._get_npatch()
+        w1 = 1 - n / (2 + 2 ** 0.5 * (n - 1))
+        w2 = 1 - 2 ** 0.5 * n / (3 * (n - 1 + 2 ** 0.5))
+        rows = []
+        for m in range(n):
+            row = []
+            for key in keys:
+                out = sum(p != m for p in key)
+                if cpw == 'simple':
+                    if out == 3:
+                        row.append((*key, 1))
+                elif out > 0:
+                    if cpw == 'mean':
+                        row.append((*key, out / 3))
+                    else:
+                        i, j, k = key
+                        row.append((*key, 1 if out == 3 else
+                                    w2 if (i != j and i != k and j != k) else w1))
+            rows.append(row)
+        return rows
+
+    def _sample_pairs(self, cross_patch_weight):
+        # corr3base.py:3099-3155
+        cpw = cross_patch_weight
+        if cpw not in ('simple', 'mean'):
+            raise ValueError(f"cross_patch_weight = {cpw} is invalid for sample")
+        keys = self._patch_keys()
+        rows = []
+        for m in range(self._get_npatch()):
+            if cpw == 'simple':
+                rows.append([(*key, 1) for key in keys if key[0] == m])
+            else:
+                rows.append([(*key, sum(p == m for p in key) / 3) for key in keys
+                             if m in key])
+        return rows
+
+    def _marked_pairs(self, index, cross_patch_weight):
+        # corr3base.py:3162-3233: the drawn patches as first vertices, each as often as
+        # drawn
+        cpw = cross_patch_weight
+        if cpw not in ('simple', 'mean'):
+            raise ValueError(f"cross_patch_weight = {cpw} is invalid for marked_bootstrap")
+        keys = self._patch_keys()
+        index, counts = np.unique(index, return_counts=True)
+        out = []
+        for i, w in zip(index, counts):
+            for key in keys:
+                if key[0] == i:
+                    out.append((*key, w if cpw == 'simple' else sum(p == i for p in key) / 3 * w))
+        return out
+
+    def _bootstrap_pairs(self, index, cross_patch_weight):
+        # corr3base.py:3235-3330: each (i, i, i) once per draw of i, then the triples with
+        # patches that differ, weighted by the draws of their patches
+        cpw = cross_patch_weight
+        if cpw not in ('simple', 'mean', 'geom'):
+            raise ValueError(f"cross_patch_weight = {cpw} is invalid for bootstrap")
+        keys = self._patch_keys()
+        ok = set(keys)
+        index, counts = np.unique(index, return_counts=True)
+        cnt = dict(zip(index, counts))
+        out = [(i, i, i, w) for i, w in zip(index, counts) if (i, i, i) in ok]
+        for (i, j, k) in keys:
+            if i == j == k:
+                continue
+            if cpw == 'mean':
+                out.append((i, j, k, (cnt.get(i, 0) + cnt.get(j, 0) + cnt.get(k, 0)) / 3))
+            elif i in cnt and j in cnt and k in cnt:
+                wi, wj, wk = cnt[i], cnt[j], cnt[k]
+                if cpw == 'simple':
+                    w = wi * wj if (i == k or j == k) else wi * wk if i == j else wi * wj * wk
+                else:
+                    w = (wi * wj * wk) ** (1. / 3.)
+                out.append((i, j, k, w))
+        return out
+
+    def estimate_cov(self, method, *, func=None, comm=None, num_bootstrap=None,
+                     cross_patch_weight=None):
+        """Covariance estimate: 'shot', 'jackknife', 'sample', 'bootstrap' or
+        'marked_bootstrap' (corr3base.py:2678-2790)."""
+        if num_bootstrap is None:
+            num_bootstrap = self.num_bootstrap
+        all_func = None if func is None else (lambda corrs: func(corrs[0]))
+        return _cov.estimate_multi_cov([self], method, func=all_func, comm=comm,
+                                       num_bootstrap=num_bootstrap,
+                                       cross_patch_weight=cross_patch_weight)
+
+    def build_cov_design_matrix(self, method, *, func=None, comm=None, num_bootstrap=None,
+                                cross_patch_weight=None):
+        """The design matrix behind a covariance estimate (corr3base.py:2792-2850)."""
+        if num_bootstrap is None:
+            num_bootstrap = self.num_bootstrap
+        all_func = None if func is None else (lambda corrs: func(corrs[0]))
+        return _cov.build_multi_cov_design_matrix([self], method, func=all_func, comm=comm,
+                                                  num_bootstrap=num_bootstrap,
+                                                  cross_patch_weight=cross_patch_weight)
+
+    @property
+    def cov(self):
+        # corr3base.py:1090-1099
+        if self._cov is None:
+            self._cov = self.estimate_cov(self.var_method)
+        return np.diag(self._cov) if self._cov.ndim == 1 else self._cov
+
+    @property
+    def cov_diag(self):
+        # corr3base.py:1101-1112
+        if self._cov is None:
+            self._cov = self.estimate_cov(self.var_method)
+        return self._cov if self._cov.ndim == 1 else self._cov.diagonal()
+
+    def getStat(self):
+        """The statistic as a 1-d array (corr3base.py:1849-1858)."""
+        return self.zeta.ravel()
+
+    def getWeight(self):
+        """The weights of getStat, |weight| for the complex multipole weight
+        (corr3base.py:1860-1868)."""
+        return np.abs(self.weight.ravel())
+
+    def _calculate_varzeta(self, n):
+        # corr3base.py:2483-2487
+        size = int(np.prod(self.data_shape))
+        self._varzeta = [np.zeros(self.data_shape) for _ in range(n)]
+        if self._var_num != 0:
+            for i in range(n):
+                self._varzeta[i].ravel()[:] = self.cov_diag[i * size:(i + 1) * size].real
 
     # ---- LogSAS through the Fourier sums ----
 
@@ -820,6 +1067,20 @@ class NNNCorrelation(Corr3):
         self.tot += other.tot
         return self
 
+    def _sum(self, others, corr_only):
+        # nnncorrelation.py:155-168: tot from every triple, sums from the nonzero ones
+        tot = np.sum([c.tot * w for c, w in others])
+        others = [(c, w) for c, w in others if c.nonzero]
+        if len(others) == 0:
+            self._clear()
+        else:
+            super()._sum(others, corr_only)
+        self.tot = tot
+
+    def getStat(self):
+        # nnncorrelation.py:200-207
+        raise RuntimeError("You need to call calculateZeta before calling estimate_cov.")
+
 
 class GGGCorrelation(Corr3):
     """Shear-shear-shear correlation (treecorr/gggcorrelation.py)."""
@@ -852,3 +1113,26 @@ class GGGCorrelation(Corr3):
         self._var_num = 4 * varg1 * varg2 * varg3
         if self.bin_type in ('LogSAS', 'LogMultipole'):
             self._var_num *= 2
+        self._cov = None
+        self._varzeta = None
+
+    vargam0 = property(lambda self: self._vargam(0))
+    vargam1 = property(lambda self: self._vargam(1))
+    vargam2 = property(lambda self: self._vargam(2))
+    vargam3 = property(lambda self: self._vargam(3))
+
+    def _vargam(self, i):
+        # gggcorrelation.py:180-201
+        if self._varzeta is None:
+            self._calculate_varzeta(4)
+        return self._varzeta[i]
+
+    def getStat(self):
+        """gam0, gam1, gam2, gam3 raveled and concatenated (gggcorrelation.py:203-215);
+        complex."""
+        return np.concatenate([self.gam0.ravel(), self.gam1.ravel(), self.gam2.ravel(),
+                               self.gam3.ravel()])
+
+    def getWeight(self):
+        # gggcorrelation.py:217-222
+        return np.concatenate([np.abs(self.weight.ravel())] * 4)

@@ -94,10 +94,13 @@ def build_multi_cov_design_matrix(corrs, method, *, func=None, comm=None, num_bo
 
 
 def _gram(v, w):
-    """Sum_r w_r (v_r - mean)^H (v_r - mean) (w None: unit weights), on the GPU for real v."""
+    """Sum_r w_r (v_r - mean)^H (v_r - mean) (w None: unit weights), on the GPU."""
     if np.iscomplexobj(v):
-        vc = v - np.mean(v, axis=0)
-        return (vc.conj().T * (1. if w is None else w)).dot(vc)
+        # (a - ib)^T (a + ib) = a^T a + b^T b + i (a^T b - b^T a): one real product of
+        # the columns [Re v, Im v]
+        n = v.shape[1]
+        G = _gram(np.concatenate([v.real, v.imag], axis=1), w)
+        return (G[:n, :n] + G[n:, n:]) + 1j * (G[:n, n:] - G[n:, :n])
     v = np.ascontiguousarray(v, dtype=float)
     nrow, ncol = v.shape
     C = np.zeros((ncol, ncol))
@@ -123,24 +126,23 @@ class _Rows:
             raise RuntimeError("design matrix rows replayed out of order")
         row = self.results[self.next]
         self.next += 1
-        nb = corr.nbins
-        for q in range(4):
-            if corr._xi[q] is not None:
-                corr._xi[q][:] = row[q * nb:(q + 1) * nb]
-        corr.weight[:] = row[6 * nb:7 * nb]
-        if not corr_only:
-            corr.meanr[:] = row[4 * nb:5 * nb]
-            corr.meanlogr[:] = row[5 * nb:6 * nb]
-            corr.npairs[:] = row[7 * nb:8 * nb]
+        off = 0
+        for a in corr._row_arrays(corr_only):
+            a[...] = row[off:off + a.size].reshape(a.shape)
+            off += a.size
 
     def solve(self):
-        """Every recorded call's weighted sum, R = W S, on the GPU (one product per
-        bin count, as the calls of one design matrix may mix correlations)."""
+        """Every recorded call's weighted sum, R = W S, on the GPU: one product per kind
+        of call (class, corr_only and array sizes), as the calls of one design matrix may
+        mix correlations.  A row of S is one pair sum's arrays, in the order of the
+        class's _row_arrays."""
         self.results = [None] * len(self.calls)
         groups = {}
-        for k, (corr, _, _) in enumerate(self.calls):
-            groups.setdefault(corr.nbins, []).append(k)
-        for nb, ks in groups.items():
+        for k, (corr, _, corr_only) in enumerate(self.calls):
+            sizes = tuple(a.size for a in corr._row_arrays(corr_only))
+            groups.setdefault((type(corr), bool(corr_only), sizes), []).append(k)
+        for (_, corr_only, sizes), ks in groups.items():
+            ncol = sum(sizes)
             pid = {}
             stack = []
             for k in ks:
@@ -148,21 +150,18 @@ class _Rows:
                     if id(ps) not in pid:
                         pid[id(ps)] = len(stack)
                         stack.append(ps)
-            S = np.zeros((max(1, len(stack)), 8 * nb))
+            S = np.zeros((max(1, len(stack)), ncol))
             for r, ps in enumerate(stack):
-                for q in range(4):
-                    if ps._xi[q] is not None:
-                        S[r, q * nb:(q + 1) * nb] = ps._xi[q]
-                S[r, 4 * nb:5 * nb] = ps.meanr
-                S[r, 5 * nb:6 * nb] = ps.meanlogr
-                S[r, 6 * nb:7 * nb] = ps.weight
-                S[r, 7 * nb:8 * nb] = ps.npairs
+                arrs = ps._row_arrays(corr_only)
+                if tuple(a.size for a in arrs) != sizes:
+                    raise ValueError("a patch result does not match its correlation's layout")
+                S[r] = np.concatenate([a.ravel() for a in arrs])
             W = np.zeros((len(ks), S.shape[0]))
             for row, k in enumerate(ks):
                 for ps, w in self.calls[k][1]:
                     W[row, pid[id(ps)]] += w
-            R = np.zeros((len(ks), 8 * nb))
-            _lib.check(_lib.lib().tc_cov_rowsums(len(ks), S.shape[0], 8 * nb, _lib.dptr(W),
+            R = np.zeros((len(ks), ncol))
+            _lib.check(_lib.lib().tc_cov_rowsums(len(ks), S.shape[0], ncol, _lib.dptr(W),
                                                  _lib.dptr(S), _lib.dptr(R)))
             for row, k in enumerate(ks):
                 self.results[k] = R[row]
