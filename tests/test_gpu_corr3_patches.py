@@ -267,3 +267,47 @@ def test_gpu_3pt_logsas_jackknife_rows(gpu):
         want = 0.75 * _gram(np.array(rows))
         got = g.estimate_cov('jackknife', cross_patch_weight=cpw)
         assert normwise(got, want) < 1e-9, cpw
+
+
+@pytest.mark.gpu
+def test_gpu_nnn_calculate_zeta(gpu):
+    """NNN's estimator (nnncorrelation.py:228-460).  Randoms that are the data themselves
+    give zeta = 0 in every bin with triangles, for (DDD-RRR)/RRR and for
+    (DDD-RDD+DRR-RRR)/RRR: the unordered DRR and RDD hold each triangle of DDD three
+    times, and the ratios of the totals (W^3/6 for an auto-correlation, W^3/2 for a
+    cross12) must cancel that.
+    With patches the estimate equals the one without, and the jackknife rows (the
+    randoms summed over the same triples) give a positive variance."""
+    cat, scale = C.make_catalog(FLAT, 900, 401, 0.3)
+    rnd = C.make_catalog(FLAT, 1500, 402, 0.0)[0]
+    kw = dict(min_sep=scale * 0.03, max_sep=scale * 0.25, nbins=4, nphi_bins=6, max_n=6,
+              brute=True)
+    print('nnn calculateZeta', flush=True)
+
+    def nnn(*cats):
+        # DRR and RDD unordered, as the estimator expects (a random in any vertex)
+        c = gpu.NNNCorrelation(**kw)
+        c.process(*cats, ordered=len(cats) == 1)
+        return c
+    d = public_catalog(gpu, cat, FLAT)
+    ddd = nnn(d)
+    same = nnn(d)
+    drr, rdd = nnn(d, d), nnn(d, d)
+    z, vz = ddd.calculateZeta(rrr=same)
+    mask = same.weight != 0
+    assert np.sum(mask) > 20
+    assert np.max(np.abs(z[mask])) < 1e-12
+    z, vz = ddd.calculateZeta(rrr=same, drr=drr, rdd=rdd)
+    assert np.max(np.abs(z[mask])) < 1e-12
+    # with randoms, patched against plain
+    r = public_catalog(gpu, rnd, FLAT)
+    ld, lr = labels_for(cat, FLAT, scale, 2), labels_for(rnd, FLAT, scale, 2)
+    dp, rp = public_catalog(gpu, cat, FLAT, ld), public_catalog(gpu, rnd, FLAT, lr)
+    z0, v0 = nnn(d).calculateZeta(rrr=nnn(r), drr=nnn(d, r), rdd=nnn(r, d))
+    dddp = nnn(dp)
+    z1, v1 = dddp.calculateZeta(rrr=nnn(rp), drr=nnn(dp, rp), rdd=nnn(rp, dp))
+    assert normwise(z1, z0) < 1e-10
+    np.testing.assert_allclose(v1, v0, rtol=1e-10)
+    cov = dddp.estimate_cov('jackknife')
+    assert np.all(np.isfinite(cov)) and np.all(np.diag(cov)[dddp.getWeight() != 0] >= 0)
+    assert np.sum(np.diag(cov) > 0) > 20

@@ -1035,6 +1035,123 @@ class NNNCorrelation(Corr3):
     def __init__(self, config=None, *, logger=None, **kwargs):
         super().__init__(config, logger=logger, **kwargs)
         self.tot = 0.
+        self.zeta = None
+        self.varzeta = None
+        self._rrr_weight = None
+        self._rrr = self._drr = self._rdd = None
+
+    def copy(self):
+        # nnncorrelation.py:88-96: the randoms are copied with it
+        ret = super().copy()
+        for nm in ('_rrr', '_drr', '_rdd'):
+            if getattr(self, nm) is not None:
+                setattr(ret, nm, getattr(self, nm).copy())
+        return ret
+
+    def _zero_copy(self, tot):
+        # nnncorrelation.py:98-128: an empty result that only carries tot
+        ret = Corr3.copy(self)
+        ret._clear()
+        ret.results = {}
+        ret.tot = tot
+        ret._rrr = ret._drr = ret._rdd = None
+        return ret
+
+    def _mean_weight(self):
+        mean_ntri = np.mean(self.ntri)
+        return 1 if mean_ntri == 0 else np.mean(self.weight) / mean_ntri
+
+    def calculateZeta(self, *, rrr, drr=None, rdd=None):
+        """zeta = (DDD - RRR) / RRR, or (DDD - RDD + DRR - RRR) / RRR with the data-random
+        cross correlations, each scaled by the ratio of the totals of possible triangles
+        (nnncorrelation.py:228-396).  Returns (zeta, varzeta)."""
+        if rrr.tot == 0:
+            raise ValueError("rrr has tot=0.")
+        if (rdd is not None) != (drr is not None):
+            raise TypeError("Must provide both rdd and drr (or neither).")
+        if self.bin_type == 'LogMultipole':
+            raise TypeError("calculateZeta is not valid for LogMultipole binning.")
+        rrrf = self.tot / rrr.tot
+        if drr is not None:
+            if drr.tot == 0:
+                raise ValueError("drr has tot=0.")
+            drrf = self.tot / drr.tot
+            if rdd.tot == 0:
+                raise ValueError("rdd has tot=0.")
+            rddf = self.tot / rdd.tot
+        denom = rrr.weight * rrrf
+        if rdd is None:
+            zeta = self.weight - denom
+        else:
+            zeta = self.weight - rdd.weight * rddf + drr.weight * drrf - denom
+        denom[rrr.weight == 0] = 1.
+        self.zeta = zeta / denom
+        dddw = self._mean_weight()
+        factor = 1 + rrrf * rrr._mean_weight() / dddw
+        if rdd is not None:
+            factor += drrf * drr._mean_weight() / dddw + rddf * rdd._mean_weight() / dddw
+        self._var_num = dddw * factor ** 2
+        self._rrr_weight = np.abs(rrr.weight) * rrrf
+        self._rrr, self._drr, self._rdd = rrr, drr, rdd
+        if len(self.results) > 0:
+            if rrr.npatch1 != 1 and rrr.npatch1 != self.npatch1:
+                raise RuntimeError("If using patches, RRR must be run with the same patches "
+                                   "as DDD")
+            if drr is not None and (len(drr.results) == 0 or drr.npatch1 != self.npatch1 or
+                                    drr.npatch2 not in (self.npatch2, 1)):
+                raise RuntimeError("DRR must be run with the same patches as DDD")
+            if rdd is not None and (len(rdd.results) == 0 or rdd.npatch2 != self.npatch2 or
+                                    rdd.npatch1 not in (self.npatch1, 1)):
+                raise RuntimeError("RDD must be run with the same patches as DDD")
+            # triples the randoms have and the data lack enter as empty results
+            add = set()
+            if rrr.npatch1 != 1:
+                add |= set(rrr.results) - set(self.results)
+            if drr is not None and drr.npatch2 != 1:
+                add |= set(drr.results) - set(self.results)
+            if rdd is not None and rdd.npatch1 != 1:
+                add |= set(rdd.results) - set(self.results)
+            for ijk in add:
+                self.results[ijk] = self._zero_copy(0)
+        self._cov = self.estimate_cov(self.var_method)
+        self.varzeta = self.cov_diag.reshape(self.zeta.shape)
+        return self.zeta, self.varzeta
+
+    def _calculate_xi_from_pairs(self, pairs, corr_only):
+        # nnncorrelation.py:398-460: DDD from the triples, the randoms from the same
+        # triples when they have patches, else scaled by the area fraction the triples
+        # cover
+        super()._calculate_xi_from_pairs(pairs, corr_only)
+        if self._rrr is None:
+            return
+        ddd = self.weight
+        if len(self._rrr.results) > 0:
+            self._rrr._calculate_xi_from_pairs(self._rrr._keep_ok(pairs), corr_only=True)
+            ddd_tot = self.tot
+        else:
+            area_frac = np.sum([(self.results[(i, j, k)].tot * w) ** (1. / 3.)
+                                for i, j, k, w in pairs if i == j == k])
+            area_frac /= np.sum([c.tot ** (1. / 3.) for ijk, c in self.results.items()
+                                 if ijk[0] == ijk[1] == ijk[2]])
+            ddd_tot = np.sum([c.tot for c in self.results.values()]) * area_frac
+        rrrf = ddd_tot / self._rrr.tot
+        denom = self._rrr.weight * rrrf
+        if self._drr is None:
+            zeta = ddd - denom
+        else:
+            pairs2 = pairs
+            if self._drr.npatch2 == 1:
+                pairs2 = [(i, 0, 0, w) for i, j, k, w in pairs2 if i == j == k]
+            self._drr._calculate_xi_from_pairs(self._drr._keep_ok(pairs2), corr_only=True)
+            pairs3 = pairs
+            if self._rdd.npatch1 == 1 and not all(p[0] == 0 for p in pairs):
+                pairs3 = [(0, j, k, w) for i, j, k, w in pairs3 if i == j or i == k]
+            self._rdd._calculate_xi_from_pairs(self._rdd._keep_ok(pairs3), corr_only=True)
+            zeta = (ddd - self._rdd.weight * (ddd_tot / self._rdd.tot) +
+                    self._drr.weight * (ddd_tot / self._drr.tot) - denom)
+        denom[denom == 0] = 1
+        self.zeta = zeta / denom
+        self._rrr_weight = denom
 
     def _clear(self):
         super()._clear()
@@ -1079,7 +1196,15 @@ class NNNCorrelation(Corr3):
 
     def getStat(self):
         # nnncorrelation.py:200-207
-        raise RuntimeError("You need to call calculateZeta before calling estimate_cov.")
+        if self.zeta is None:
+            raise RuntimeError("You need to call calculateZeta before calling estimate_cov.")
+        return self.zeta.ravel()
+
+    def getWeight(self):
+        # nnncorrelation.py:209-218: the scaled RRR of calculateZeta
+        if self._rrr_weight is not None:
+            return self._rrr_weight.ravel()
+        return self.tot
 
 
 class GGGCorrelation(Corr3):
