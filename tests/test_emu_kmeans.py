@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import cases as C
+import emu_build
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -23,24 +24,7 @@ ROOT = os.path.dirname(HERE)
 def emu_kmeans_exe():
     if shutil.which('g++') is None:
         pytest.skip('g++ is not available')
-    out = tempfile.mkdtemp(prefix='tc_emu_km_')
-    inc = ['-I', os.path.join(HERE, 'native', 'hip_emu'), '-I', os.path.join(ROOT, 'treecorr_amd', 'csrc')]
-    flags = ['g++', '-std=c++17', '-O1', '-ffp-contract=off', '-fsanitize=address',
-             '-fno-omit-frame-pointer']
-    objs, procs = [], []
-    for src in ('tc_field.hip', 'tc_patch.hip'):
-        o = os.path.join(out, src + '.o')
-        procs.append(subprocess.Popen(flags + ['-x', 'c++', *inc, '-c',
-                                               os.path.join(ROOT, 'treecorr_amd', 'csrc', src),
-                                               '-o', o], stderr=subprocess.PIPE))
-        objs.append(o)
-    for p in procs:
-        _, e = p.communicate(timeout=900)
-        assert p.returncode == 0, e.decode()[-3000:]
-    exe = os.path.join(out, 'emu_kmeans')
-    r = subprocess.run(flags + [*inc, os.path.join(HERE, 'native', 'emu_kmeans.cpp'), *objs, '-o',
-                                exe, '-pthread'], capture_output=True, timeout=300)
-    assert r.returncode == 0, r.stderr.decode()[-3000:]
+    exe, out = emu_build.link('emu_kmeans', ('tc_field.hip', 'tc_patch.hip'))
     yield exe
     shutil.rmtree(out, ignore_errors=True)
 
