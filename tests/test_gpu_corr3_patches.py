@@ -311,3 +311,20 @@ def test_gpu_nnn_calculate_zeta(gpu):
     cov = dddp.estimate_cov('jackknife')
     assert np.all(np.isfinite(cov)) and np.all(np.diag(cov)[dddp.getWeight() != 0] >= 0)
     assert np.sum(np.diag(cov) > 0) > 20
+
+
+@pytest.mark.gpu
+def test_gpu_3pt_write_patch_results_round_trip(gpu, tmp_path):
+    """write(write_patch_results=True) then Corr3.from_file: the object read back gives the
+    same jackknife covariance as the one processed (corr3base.py:3403-3570)."""
+    cat, scale = C.make_catalog(FLAT, 1000, 303, 0.2)
+    lab = labels_for(cat, FLAT, scale, 2)
+    print('3pt write_patch_results round trip', flush=True)
+    g = gpu.GGGCorrelation(min_sep=scale * 0.02, max_sep=scale * 0.2, nbins=4, nphi_bins=6,
+                           max_n=6, brute=True)
+    g.process(public_catalog(gpu, cat, FLAT, lab, 4))
+    f = str(tmp_path / 'ggg.out')
+    g.write(f, write_patch_results=True)
+    g2 = gpu.Corr3.from_file(f)
+    c1 = g.estimate_cov('jackknife')
+    assert normwise(g2.estimate_cov('jackknife'), c1) < 1e-10

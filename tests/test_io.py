@@ -94,3 +94,54 @@ def test_fits_and_hdf_are_refused(tmp_path):
     for ext in ('fits', 'hdf5'):
         with pytest.raises(NotImplementedError):
             c.write(str(tmp_path / ('kk.' + ext)))
+
+
+@pytest.mark.parametrize('kind,bin_type', [('GGG', 'LogSAS'), ('GGG', 'LogMultipole'),
+                                           ('NNN', 'LogSAS'), ('NNN', 'LogMultipole')])
+def test_corr3_write_read_round_trip(tmp_path, kind, bin_type):
+    """3-point files (corr3base.py:3352-3662, gggcorrelation.py:224-275,
+    nnncorrelation.py:459-770): the column layout of the reference, and every array,
+    patch result, NNN tot and the variances back from Corr3.from_file."""
+    import treecorr_amd as tc
+    rng = np.random.default_rng(7)
+    kw = dict(min_sep=1., max_sep=10., nbins=3, bin_type=bin_type, sep_units='arcmin')
+    kw.update(max_n=4) if bin_type == 'LogMultipole' else kw.update(nphi_bins=5)
+    c = getattr(tc, kind + 'Correlation')(**kw)
+    c.coords, c.metric = 'flat', 'Euclidean'
+
+    def fill(o):
+        for a in o._sums():
+            # meanv belongs to LogRUV binning, meanu (phi) to LogSAS
+            if a is not o.meanv and (a is not o.meanu or bin_type == 'LogSAS'):
+                a[...] = rng.uniform(0.5, 2., a.shape)
+        if kind == 'NNN':
+            o.tot = float(rng.uniform(1, 9))
+    fill(c)
+    c._var_num = 0.25
+    c.npatch1 = c.npatch2 = c.npatch3 = 3
+    for i in range(3):
+        r = c.copy()
+        r.results = {}
+        fill(r)
+        c.results[(i, i, i)] = r
+    if kind == 'NNN':
+        c.results[(1, 1, 1)]._clear()       # an empty result keeps only its tot
+    f = str(tmp_path / ('%s.out' % kind))
+    c.write(f, write_patch_results=True)
+    header = open(f).readlines()[2].split()
+    first = ['#', 'd2_nom', 'd3_nom', 'n' if bin_type == 'LogMultipole' else 'phi_nom']
+    assert header[:4] == first
+    c2 = tc.Corr3.from_file(f)
+    assert type(c2) is type(c) and c2.bin_type == bin_type and c2.npatch3 == 3
+    for a, b in zip(c2._sums(), c._sums()):
+        np.testing.assert_allclose(a, b, rtol=1e-4)     # the main table keeps 4 digits
+    assert sorted(c2.results) == sorted(c.results)
+    for key, r in c.results.items():
+        for a, b in zip(c2.results[key]._sums(), r._sums()):
+            np.testing.assert_allclose(a, b, rtol=1e-15)
+        if kind == 'NNN':
+            assert c2.results[key].tot == r.tot
+    if kind == 'GGG':
+        np.testing.assert_allclose(c2.vargam2, c.vargam2, rtol=1e-4)
+    else:
+        assert c2.tot == c.tot

@@ -22,6 +22,7 @@ from .catalog import calculateVarG
 from .config import parse_unit
 from .corr2 import Corr2
 from .field import coord_enum
+from .io import literal as _io_literal, make_minimal_config, make_reader, make_writer
 from .util import set_omp_threads
 
 LOGMULTIPOLE = 5
@@ -41,6 +42,19 @@ def _gemm(A, B):
     C = np.zeros((m, n))
     _lib.check(_lib.lib().tc_cov_rowsums(m, k, n, _lib.dptr(A), _lib.dptr(B), _lib.dptr(C)))
     return C
+
+
+# corr3base.py:480-520: the constructor parameters (and their defaults) kept in the
+# params line of an output file
+_VALID3 = {
+    'nbins': None, 'bin_size': None, 'min_sep': None, 'max_sep': None, 'sep_units': None,
+    'bin_slop': None, 'angle_slop': None, 'brute': False, 'verbose': 1, 'log_file': None,
+    'output_dots': None, 'split_method': 'mean', 'min_top': None, 'max_top': 10,
+    'precision': 4, 'metric': 'Euclidean', 'bin_type': 'LogSAS', 'min_phi': None,
+    'max_phi': None, 'nphi_bins': None, 'phi_bin_size': None, 'phi_units': None,
+    'max_n': None, 'var_method': 'shot', 'num_bootstrap': 500, 'cross_patch_weight': None,
+    'num_threads': None,
+}
 
 
 def _spin(letter):
@@ -882,6 +896,163 @@ class Corr3:
         (corr3base.py:1860-1868)."""
         return np.abs(self.weight.ravel())
 
+    # ---- output files (corr3base.py:3352-3662) ----
+
+    @property
+    def _write_params(self):
+        params = make_minimal_config(self.config, _VALID3)
+        params['coords'] = self.coords
+        params['metric'] = self.metric
+        params['corr'] = self._letters
+        return params
+
+    @property
+    def _write_col_names(self):
+        sas = self.bin_type == 'LogSAS'
+        names = ['d2_nom', 'd3_nom', 'phi_nom' if sas else 'n', 'meand1', 'meanlogd1',
+                 'meand2', 'meanlogd2', 'meand3', 'meanlogd3'] + (['meanphi'] if sas else [])
+        return names + list(self._write_class_col_names) + self._write_weight_names()
+
+    @property
+    def _write_data(self):
+        sas = self.bin_type == 'LogSAS'
+        data = [self.d2nom, self.d3nom, self.phi if sas else self.n, self.meand1,
+                self.meanlogd1, self.meand2, self.meanlogd2, self.meand3, self.meanlogd3]
+        if sas:
+            data.append(self.meanu)
+        data += list(self._write_class_data) + self._write_weight_data()
+        return [np.asarray(c, dtype=float).ravel() for c in data]
+
+    _write_class_col_names = ()
+    _write_class_data = ()
+
+    def _write_weight_names(self):
+        if self.bin_type == 'LogMultipole':
+            return ['weightr', 'weighti', 'ntri']
+        return ['weight', 'ntri']
+
+    def _write_weight_data(self):
+        if self.bin_type == 'LogMultipole':
+            return [self.weightr, self.weighti, self.ntri]
+        return [self.weightr, self.ntri]
+
+    def write(self, file_name, *, file_type=None, precision=None, write_patch_results=False,
+              write_cov=False):
+        """Write the correlation to a file (ASCII; corr3base.py:3449-3530).  With
+        write_patch_results every patch result follows at 16 digits, so that the
+        covariances of the object read back equal this one's."""
+        precision = self.config.get('precision', 4) if precision is None else precision
+        with make_writer(file_name, precision, file_type) as writer:
+            self._write(writer, None, write_patch_results, write_cov=write_cov)
+
+    def _write(self, writer, name, write_patch_results, write_cov=False, zero_tot=False):
+        # corr3base.py:3403-3447
+        if name is None and (write_patch_results or write_cov):
+            name = 'main'
+        params = self._write_params
+        if write_patch_results:
+            params['npatch1'] = self.npatch1
+            params['npatch2'] = self.npatch2
+            params['npatch3'] = self.npatch3
+            params['num_rows'] = int(np.prod(self.data_shape))
+            num_patch_tri = len(self.results)
+            if zero_tot:
+                i = 0
+                for key, corr in self.results.items():
+                    if not corr.nonzero:
+                        params[name + '_zp_%d' % i] = repr((key, float(corr.tot)))
+                        num_patch_tri -= 1
+                        i += 1
+                params['num_zero_patch'] = i
+            params['num_patch_tri'] = num_patch_tri
+        if write_cov:
+            params['cov_shape'] = self.cov.shape
+        writer.write(self._write_col_names, self._write_data, params=params, ext=name)
+        if write_patch_results:
+            writer.set_precision(16)
+            i = 0
+            for key, corr in self.results.items():
+                if zero_tot and not corr.nonzero:
+                    continue
+                pparams = corr._write_params
+                pparams['key'] = repr(tuple(int(k) for k in key))
+                writer.write(corr._write_col_names, corr._write_data, params=pparams,
+                             ext=name + '_pp_%d' % i)
+                i += 1
+        if write_cov:
+            writer.write_array(self.cov, ext='cov')
+
+    def read(self, file_name, *, file_type=None):
+        """Read values written by write() into this object, which must have the same
+        configuration (corr3base.py:3600-3622)."""
+        with make_reader(file_name, file_type) as reader:
+            self._read(reader)
+
+    @classmethod
+    def from_file(cls, file_name, *, file_type=None, logger=None, rng=None):
+        """A new object from a file written by write() (corr3base.py:3624-3662); from the
+        base class, the class the file names."""
+        with make_reader(file_name, file_type) as reader:
+            params = reader.read_params(ext=None)
+            letters = params.get('corr', None)
+            if cls is Corr3:
+                kinds = {'NNN': NNNCorrelation, 'GGG': GGGCorrelation}
+                if letters not in kinds:
+                    raise OSError("%s does not seem to be a valid treecorr output file."
+                                  % file_name)
+                cls = kinds[letters]
+            elif letters != cls._letters:
+                raise OSError("Trying to read a %sCorrelation output file with %s"
+                              % (letters, cls.__name__))
+            corr = cls(make_minimal_config(params, _VALID3), rng=rng)
+            corr._read(reader, name=None, params=params)
+        return corr
+
+    def _read(self, reader, name=None, params=None):
+        # corr3base.py:3532-3570
+        if params is None:
+            params = reader.read_params(ext=name)
+        num_rows = params.get('num_rows', None)
+        num_patch_tri = params.get('num_patch_tri', 0)
+        num_zero_patch = params.get('num_zero_patch', 0)
+        cov_shape = params.get('cov_shape', None)
+        name = 'main' if num_patch_tri and name is None else name
+        self._read_from_data(reader.read_data(max_rows=num_rows, ext=name), params)
+        self.results = {}
+        for i in range(num_zero_patch):
+            key, tot = _io_literal(params[name + '_zp_%d' % i])
+            self.results[key] = self._zero_copy(tot)
+        for i in range(num_patch_tri):
+            pname = name + '_pp_%d' % i
+            corr = self.copy()
+            corr.results = {}
+            pparams = reader.read_params(ext=pname)
+            corr._read_from_data(reader.read_data(max_rows=num_rows, ext=pname), pparams)
+            self.results[_io_literal(pparams['key'])] = corr
+        if cov_shape is not None:
+            if isinstance(cov_shape, str):
+                cov_shape = _io_literal(cov_shape)
+            self._cov = reader.read_array(tuple(cov_shape), ext='cov')
+
+    def _read_from_data(self, data, params):
+        # corr3base.py:3572-3598; in place, so the engine object keeps valid pointers
+        s = self.data_shape
+        for nm in ('meand1', 'meanlogd1', 'meand2', 'meanlogd2', 'meand3', 'meanlogd3',
+                   'ntri'):
+            getattr(self, nm)[...] = data[nm].reshape(s)
+        if self.bin_type == 'LogSAS':
+            self.meanu[...] = data['meanphi'].reshape(s)
+        if self.bin_type == 'LogMultipole':
+            self.weightr[...] = data['weightr'].reshape(s)
+            self.weighti[...] = data['weighti'].reshape(s)
+        elif 'weight' in data.dtype.names:
+            self.weightr[...] = data['weight'].reshape(s)
+        self.coords = params['coords'].strip() if params.get('coords') else None
+        self.metric = params['metric'].strip() if params.get('metric') else None
+        self.npatch1 = params.get('npatch1', 1)
+        self.npatch2 = params.get('npatch2', 1)
+        self.npatch3 = params.get('npatch3', 1)
+
     def _calculate_varzeta(self, n):
         # corr3base.py:2483-2487
         size = int(np.prod(self.data_shape))
@@ -1194,6 +1365,53 @@ class NNNCorrelation(Corr3):
             super()._sum(others, corr_only)
         self.tot = tot
 
+    # output files without the randoms' columns (nnncorrelation.py:459-770, rrr=None)
+    def write(self, file_name, *, rrr=None, drr=None, rdd=None, file_type=None,
+              precision=None, write_patch_results=False, write_cov=False):
+        """Write DDD (and zeta after calculateZeta) to a file (nnncorrelation.py:459-600).
+        The RRR, DRR, RDD columns of the reference's rrr= form are not written here."""
+        if rrr is not None or drr is not None or rdd is not None:
+            raise NotImplementedError("NNN files with RRR/DRR/RDD columns are not implemented")
+        super().write(file_name, file_type=file_type, precision=precision,
+                      write_patch_results=write_patch_results, write_cov=write_cov)
+
+    def _write(self, writer, name, write_patch_results, write_cov=False, zero_tot=True):
+        super()._write(writer, name, write_patch_results, write_cov=write_cov, zero_tot=True)
+
+    @property
+    def _write_params(self):
+        params = super()._write_params
+        params['tot'] = float(self.tot)
+        return params
+
+    @property
+    def _write_class_col_names(self):
+        return ['zeta', 'sigma_zeta'] if self.zeta is not None else []
+
+    @property
+    def _write_class_data(self):
+        return [self.zeta, np.sqrt(self.varzeta)] if self.zeta is not None else []
+
+    def _write_weight_names(self):
+        multi = ['weightr', 'weighti'] if self.weighti.size else []
+        return multi + ['DDD', 'ntri']
+
+    def _write_weight_data(self):
+        if self.weighti.size:
+            return [self.weightr, self.weighti, np.abs(self.weight), self.ntri]
+        return [self.weightr, self.ntri]
+
+    def _read_from_data(self, data, params):
+        # nnncorrelation.py:754-770
+        super()._read_from_data(data, params)
+        s = self.data_shape
+        if not self.weighti.size:
+            self.weightr[...] = data['DDD'].reshape(s)
+        if 'zeta' in data.dtype.names:
+            self.zeta = data['zeta'].reshape(s)
+            self.varzeta = data['sigma_zeta'].reshape(s) ** 2
+        self.tot = params['tot']
+
     def getStat(self):
         # nnncorrelation.py:200-207
         if self.zeta is None:
@@ -1251,6 +1469,23 @@ class GGGCorrelation(Corr3):
         if self._varzeta is None:
             self._calculate_varzeta(4)
         return self._varzeta[i]
+
+    # gggcorrelation.py:247-275
+    _write_class_col_names = ('gam0r', 'gam0i', 'gam1r', 'gam1i', 'gam2r', 'gam2i', 'gam3r',
+                              'gam3i', 'sigma_gam0', 'sigma_gam1', 'sigma_gam2', 'sigma_gam3')
+
+    @property
+    def _write_class_data(self):
+        return list(self._z) + [np.sqrt(self.vargam0), np.sqrt(self.vargam1),
+                                np.sqrt(self.vargam2), np.sqrt(self.vargam3)]
+
+    def _read_from_data(self, data, params):
+        super()._read_from_data(data, params)
+        s = self.data_shape
+        for i, nm in enumerate(('gam0r', 'gam0i', 'gam1r', 'gam1i', 'gam2r', 'gam2i', 'gam3r',
+                                'gam3i')):
+            self._z[i][...] = data[nm].reshape(s)
+        self._varzeta = [data['sigma_gam%d' % i].reshape(s) ** 2 for i in range(4)]
 
     def getStat(self):
         """gam0, gam1, gam2, gam3 raveled and concatenated (gggcorrelation.py:203-215);
