@@ -192,3 +192,47 @@ def test_corr3_trivially_zero_is_conservative():
     far = [(0., 0., 0.5), (0., 0., 0.5), (5.1, 0., 0.5)]
     assert tz(far, 1, 1) == 1
     assert tz([(0., 0., 0.5), (0., 0., 0.5), (4.9, 0., 0.5)], 1, 1) == 0
+
+
+def test_corr3_trivially_zero_matches_compiled_reference():
+    """tc_corr3_trivially_zero against the compiled reference's own triviallyZero
+    (src/Corr3.cpp:4069-4130) on random cell triples: the same decision every time,
+    flat, 3-d and spherical positions, ordered 0/1, with and without p13."""
+    import ctypes as ct
+    import math as _m
+    import treecorr_amd as tc
+    from oracle import ref as R
+    if R.module() is None:
+        pytest.skip('oracle/_ref not built')
+    from treecorr_amd import _lib
+    L = _lib.lib()
+    min_sep, max_sep, nbins, max_n = 1., 4., 4, 5
+    bin_size = _m.log(max_sep / min_sep) / nbins
+    corr = tc.NNNCorrelation(min_sep=min_sep, max_sep=max_sep, nbins=nbins, max_n=max_n,
+                             bin_type='LogMultipole')
+    m, rc, keep = R._corr3_mp(0, nbins, min_sep, max_sep, bin_size, corr.b, corr.angle_slop,
+                              max_n, None)
+    rng = np.random.default_rng(11)
+    coords = {1: m.Flat, 2: m.ThreeD, 3: m.Sphere}
+    seen = set()
+    for trial in range(3000):
+        c = int(rng.integers(1, 4))
+        cells = []
+        for _ in range(3):
+            if c == 3:
+                # near (1, 0, 0), off the unit sphere by up to 10% (the reference
+                # normalises spherical positions)
+                x, y, z = 1 + rng.normal(0, 0.25), rng.normal(0, 0.25), rng.normal(0, 0.25)
+                s = rng.choice([0., rng.uniform(0, 0.3)])
+            else:
+                x, y = rng.uniform(0, 8, 2)
+                z = rng.uniform(0, 8) if c == 2 else 0.
+                s = rng.choice([0., rng.uniform(0, 1.5)])
+            cells += [float(x), float(y), float(z), float(s)]
+        ordered, p13 = int(rng.integers(0, 2)), int(rng.integers(0, 2))
+        r = ct.c_int()
+        assert L.tc_corr3_trivially_zero(corr.corr, c, *cells, ordered, p13, ct.byref(r)) == 0
+        want = rc.triviallyZero(m.Euclidean, coords[c], *cells, ordered, bool(p13))
+        assert bool(r.value) == bool(want), (c, cells, ordered, p13)
+        seen.add((c, bool(want)))
+    assert len(seen) == 6        # both answers in every coordinate system
