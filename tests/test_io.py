@@ -145,3 +145,43 @@ def test_corr3_write_read_round_trip(tmp_path, kind, bin_type):
         np.testing.assert_allclose(c2.vargam2, c.vargam2, rtol=1e-4)
     else:
         assert c2.tot == c.tot
+
+
+def test_nnn_write_with_randoms_round_trip(tmp_path):
+    """NNN files with the randoms (nnncorrelation.py:459-770): zeta from calculateZeta,
+    the RRR, DRR, RDD columns scaled to DDD's total, and with write_patch_results the
+    randoms' groups, read back into _rrr, _drr, _rdd with their patch results."""
+    import treecorr_amd as tc
+    rng = np.random.default_rng(9)
+    kw = dict(min_sep=1., max_sep=10., nbins=3, nphi_bins=4)
+
+    def make(tot):
+        c = tc.NNNCorrelation(**kw)
+        c.coords, c.metric = 'flat', 'Euclidean'
+        c.npatch1 = c.npatch2 = c.npatch3 = 2
+        c.weightr[...] = rng.uniform(1., 2., c.weightr.shape)
+        c.ntri[...] = c.weightr * 3
+        c.tot = tot
+        for i in range(2):
+            r = tc.Corr3.copy(c)
+            r.results = {}
+            r.weightr[...] = c.weightr / 2
+            r.tot = tot / 2
+            c.results[(i, i, i)] = r
+        return c
+    ddd, rrr, drr, rdd = make(10.), make(40.), make(20.), make(30.)
+    f = str(tmp_path / 'nnn.out')
+    ddd.write(f, rrr=rrr, drr=drr, rdd=rdd, write_patch_results=True)
+    header = open(f).readlines()[2].split()
+    assert header[-7:] == ['zeta', 'sigma_zeta', 'DDD', 'RRR', 'DRR', 'RDD', 'ntri']
+    want = (ddd.weightr - rdd.weightr / 3 + drr.weightr / 2 - rrr.weightr / 4) / \
+        (rrr.weightr / 4)
+    np.testing.assert_allclose(ddd.zeta, want, rtol=1e-14)
+    d2 = tc.Corr3.from_file(f)
+    np.testing.assert_allclose(d2.zeta, ddd.zeta, rtol=1e-4)
+    assert d2.tot == ddd.tot
+    for nm, o in (('_rrr', rrr), ('_drr', drr), ('_rdd', rdd)):
+        r2 = getattr(d2, nm)
+        assert r2.tot == o.tot and sorted(r2.results) == sorted(o.results)
+        np.testing.assert_allclose(r2.results[(1, 1, 1)].weightr, o.results[(1, 1, 1)].weightr,
+                                   rtol=1e-15)

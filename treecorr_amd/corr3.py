@@ -949,6 +949,7 @@ class Corr3:
         # corr3base.py:3403-3447
         if name is None and (write_patch_results or write_cov):
             name = 'main'
+        col_names, data = self._write_col_names, self._write_data
         params = self._write_params
         if write_patch_results:
             params['npatch1'] = self.npatch1
@@ -967,7 +968,7 @@ class Corr3:
             params['num_patch_tri'] = num_patch_tri
         if write_cov:
             params['cov_shape'] = self.cov.shape
-        writer.write(self._write_col_names, self._write_data, params=params, ext=name)
+        writer.write(col_names, data, params=params, ext=name)
         if write_patch_results:
             writer.set_precision(16)
             i = 0
@@ -986,7 +987,10 @@ class Corr3:
         """Read values written by write() into this object, which must have the same
         configuration (corr3base.py:3600-3622)."""
         with make_reader(file_name, file_type) as reader:
-            self._read(reader)
+            self._do_read(reader)
+
+    def _do_read(self, reader, name=None, params=None):
+        self._read(reader, name, params)
 
     @classmethod
     def from_file(cls, file_name, *, file_type=None, logger=None, rng=None):
@@ -1005,7 +1009,7 @@ class Corr3:
                 raise OSError("Trying to read a %sCorrelation output file with %s"
                               % (letters, cls.__name__))
             corr = cls(make_minimal_config(params, _VALID3), rng=rng)
-            corr._read(reader, name=None, params=params)
+            corr._do_read(reader, name=None, params=params)
         return corr
 
     def _read(self, reader, name=None, params=None):
@@ -1215,8 +1219,9 @@ class NNNCorrelation(Corr3):
         # nnncorrelation.py:88-96: the randoms are copied with it
         ret = super().copy()
         for nm in ('_rrr', '_drr', '_rdd'):
-            if getattr(self, nm) is not None:
-                setattr(ret, nm, getattr(self, nm).copy())
+            o = getattr(self, nm)
+            if o is not None and o is not True:
+                setattr(ret, nm, o.copy())
         return ret
 
     def _zero_copy(self, tot):
@@ -1365,41 +1370,86 @@ class NNNCorrelation(Corr3):
             super()._sum(others, corr_only)
         self.tot = tot
 
-    # output files without the randoms' columns (nnncorrelation.py:459-770, rrr=None)
+    # output files (nnncorrelation.py:459-770): DDD, and with the randoms zeta, RRR, DRR,
+    # RDD scaled to DDD's total; with write_patch_results the randoms' own groups follow
+    _write_rrr = _write_drr = _write_rdd = None
+    _write_patch_results = False
+
     def write(self, file_name, *, rrr=None, drr=None, rdd=None, file_type=None,
               precision=None, write_patch_results=False, write_cov=False):
-        """Write DDD (and zeta after calculateZeta) to a file (nnncorrelation.py:459-600).
-        The RRR, DRR, RDD columns of the reference's rrr= form are not written here."""
-        if rrr is not None or drr is not None or rdd is not None:
-            raise NotImplementedError("NNN files with RRR/DRR/RDD columns are not implemented")
-        super().write(file_name, file_type=file_type, precision=precision,
-                      write_patch_results=write_patch_results, write_cov=write_cov)
+        """Write the correlation to a file (nnncorrelation.py:459-597).  Given rrr (and
+        drr, rdd), zeta is computed by calculateZeta and the randoms' columns are added."""
+        precision = self.config.get('precision', 4) if precision is None else precision
+        self._write_rrr, self._write_drr, self._write_rdd = rrr, drr, rdd
+        self._write_patch_results = write_patch_results
+        try:
+            with make_writer(file_name, precision, file_type) as writer:
+                self._write(writer, None, write_patch_results, write_cov=write_cov,
+                            zero_tot=True)
+                if write_patch_results:
+                    for nm, o in (('_rrr', rrr or self._rrr), ('_drr', drr or self._drr),
+                                  ('_rdd', rdd or self._rdd)):
+                        if o is not None:
+                            o._write(writer, nm, True, zero_tot=True)
+        finally:
+            self._write_rrr = self._write_drr = self._write_rdd = None
+            self._write_patch_results = False
 
     def _write(self, writer, name, write_patch_results, write_cov=False, zero_tot=True):
         super()._write(writer, name, write_patch_results, write_cov=write_cov, zero_tot=True)
 
     @property
     def _write_params(self):
+        # nnncorrelation.py:673-681
         params = super()._write_params
         params['tot'] = float(self.tot)
+        if self._write_patch_results:
+            params['_rrr'] = self._rrr is not None
+            params['_drr'] = self._drr is not None
+            params['_rdd'] = self._rdd is not None
         return params
 
     @property
     def _write_class_col_names(self):
-        return ['zeta', 'sigma_zeta'] if self.zeta is not None else []
+        return ['zeta', 'sigma_zeta'] if (self.zeta is not None or self._write_rrr) else []
 
     @property
     def _write_class_data(self):
+        if self._write_rrr is not None:
+            zeta, varzeta = self.calculateZeta(rrr=self._write_rrr, drr=self._write_drr,
+                                               rdd=self._write_rdd)
+            return [zeta, np.sqrt(varzeta)]
+        if self._write_drr is not None or self._write_rdd is not None:
+            raise TypeError("rrr must be provided if other combinations are not None")
         return [self.zeta, np.sqrt(self.varzeta)] if self.zeta is not None else []
 
     def _write_weight_names(self):
+        if self._write_rrr is not None:
+            return ['DDD', 'RRR'] + (['DRR', 'RDD'] if self._write_drr is not None else []) + \
+                ['ntri']
         multi = ['weightr', 'weighti'] if self.weighti.size else []
         return multi + ['DDD', 'ntri']
 
     def _write_weight_data(self):
+        rrr, drr, rdd = self._write_rrr, self._write_drr, self._write_rdd
+        if rrr is not None:
+            out = [self.weight, rrr.weight * (self.tot / rrr.tot)]
+            if drr is not None:
+                out += [drr.weight * (self.tot / drr.tot), rdd.weight * (self.tot / rdd.tot)]
+            return out + [self.ntri]
         if self.weighti.size:
             return [self.weightr, self.weighti, np.abs(self.weight), self.ntri]
         return [self.weightr, self.ntri]
+
+    def _do_read(self, reader, name=None, params=None):
+        # nnncorrelation.py:722-736: the randoms' groups follow when they were written
+        self._read(reader, name, params)
+        for nm in ('_rrr', '_drr', '_rdd'):
+            if getattr(self, nm):
+                o = Corr3.copy(self)
+                o._rrr = o._drr = o._rdd = None
+                o._read(reader, name=nm)
+                setattr(self, nm, o)
 
     def _read_from_data(self, data, params):
         # nnncorrelation.py:754-770
@@ -1411,6 +1461,9 @@ class NNNCorrelation(Corr3):
             self.zeta = data['zeta'].reshape(s)
             self.varzeta = data['sigma_zeta'].reshape(s) ** 2
         self.tot = params['tot']
+        self._rrr = params.get('_rrr', None) or None
+        self._drr = params.get('_drr', None) or None
+        self._rdd = params.get('_rdd', None) or None
 
     def getStat(self):
         # nnncorrelation.py:200-207
